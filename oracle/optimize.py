@@ -1,0 +1,154 @@
+"""Direct minimisation of the energy over phi = sqrt(rho), restating
+``optimization/optimization.py:281-539`` for one spin channel, with the
+truncated-Newton direction (``:91-147``) and the CG directions of
+``math_utils.py:395-423``.
+
+``evaluator(rho, need_e)`` returns (energy, potential).
+"""
+import numpy as np
+
+from .dcsrch import line_search
+
+
+def get_direction_TN(evaluator, res0, phi, mu, maxfun=50):
+    """``optimization.py:91-147``: CG on the Hessian with a forward-difference
+    Hessian-vector product, eps = 1e-9."""
+    direction = np.zeros_like(res0)
+    epsi = 1.0e-9
+    res = -res0.copy()
+    p = res.copy()
+    r0Norm = np.einsum("ijk,ijk->", res, res)
+    r1Norm = r0Norm
+    rConv = r0Norm * 0.1
+    best = direction
+    rLists = [1e10]
+    it = 0
+    for it in range(maxfun):
+        phi1 = phi + epsi * p
+        rho1 = phi1 * phi1
+        _, v = evaluator(rho1, False)
+        Ap = ((v - mu) * phi1 - res0) / epsi
+        pAp = np.einsum("ijk,ijk->", p, Ap)
+        if pAp < 0.0:
+            if it == 0:
+                direction = r0Norm / pAp * p
+            break
+        alpha = r0Norm / pAp
+        direction = direction + alpha * p
+        res = res - alpha * Ap
+        r1Norm = np.einsum("ijk,ijk->", res, res)
+        if r1Norm < min(rLists):
+            best = direction.copy()
+        rLists.append(r1Norm)
+        if r1Norm < rConv:
+            break
+        elif r1Norm > 1000 * min(rLists[:-1]):
+            direction = best
+            break
+        elif it > 10 and abs(r0Norm - r1Norm) < 0.1 * r0Norm:
+            direction = best
+            break
+        beta = r1Norm / r0Norm
+        r0Norm = r1Norm
+        p = res + beta * p
+    return direction, it + 1
+
+
+def get_direction_CG(resA, dirA, method="CG-HS"):
+    """``math_utils.py:395-423`` (HS, PR, FR)."""
+    if len(resA) == 1 or len(dirA) == 0:
+        return -resA[-1]
+    r1, r0 = resA[-1], resA[-2]
+    if method in ("CG-HS", "CG"):
+        beta = np.sum(r1 * (r1 - r0)) / np.sum(dirA[-1] * (r1 - r0))
+    elif method == "CG-PR":
+        beta = max(np.sum(r1 * (r1 - r0)) / np.sum(r0 ** 2), 0.0)
+    else:
+        beta = np.sum(r1 ** 2) / np.sum(r0 ** 2)
+    return -r1 + beta * dirA[-1]
+
+
+def optimize(evaluator, rho, dV, method="TN", econv=1e-5, maxiter=200, maxfun=50, maxls=30,
+             c1=1e-4, c2=0.2, xtol=1e-12, ncheck=2, log=None, callback=None):
+    """``optimization.py:281-520``.  Returns dict with rho, energy, mu,
+    converged flag (0 ok, 1 line-search failure, 2 maxiter), history and the
+    per-iteration (Nd, Nls) counts."""
+    integral = lambda f: f.sum() * dV
+    phi = rho.copy()
+    m = rho > 0
+    phi[m] = np.sqrt(rho[m])
+    phi[~m] = 1e-300
+    phi *= np.sqrt(rho.sum() / (phi * phi).sum())
+    rho = phi * phi
+    energy, pot = evaluator(rho, True)
+    N = integral(rho)
+    mu = integral(pot * rho) / N
+    residual = (pot - mu) * phi
+    resA = [residual]
+    dirA = []
+    history = [energy]
+    counts = []
+    theta = 0.1
+    converged = 2
+    for it in range(1, maxiter):
+        if method == "TN":
+            p, nd = get_direction_TN(evaluator, resA[-1], phi, mu, maxfun)
+        else:
+            p, nd = get_direction_CG(resA, dirA, method), 1
+        # OrthogonalNormalization (optimization.py:204-225)
+        Ne = integral(phi * phi)
+        p = p - (integral(p * phi) / Ne) * phi
+        pNorm = integral(p * p)
+        theta0 = np.sqrt(pNorm / Ne)
+        p = p * np.sqrt(Ne / pNorm)
+        d0 = np.einsum("ijk,ijk,ijk->", pot, phi, p) * 2.0
+        if d0 > 0:
+            p = -resA[-1]
+            Ne = integral(phi * phi)
+            p = p - (integral(p * phi) / Ne) * phi
+            pNorm = integral(p * p)
+            theta0 = np.sqrt(pNorm / Ne)
+            p = p * np.sqrt(Ne / pNorm)
+        theta = min(theta0, theta)
+
+        def fvd(th, cur=None):
+            # ValueAndDerivative (optimization.py:227-279), EMM
+            newphi = phi * np.cos(th) + p * np.sin(th)
+            newrho = newphi * newphi
+            if cur is None:
+                e, v = evaluator(newrho, True)
+            else:
+                e, v = cur
+            p2 = p * np.cos(th) - phi * np.sin(th)
+            grad = 2.0 * np.einsum("ijk,ijk,ijk->", v, newphi, p2) * dV
+            return [e, grad, newphi, (e, v)]
+
+        func0 = fvd(0.0, (energy, pot))
+        theta, task, nls, vd = line_search(fvd, theta, func0, c1=c1, c2=c2, amax=np.pi, amin=0.0,
+                                           xtol=xtol, maxiter=maxls)
+        if theta is None:
+            converged = 1
+            break
+        phi = vd[2]
+        energy, pot = vd[3]
+        rho = phi * phi
+        mu = integral(pot * rho) / integral(rho)
+        residual = (pot - mu) * phi
+        resA.append(residual)
+        history.append(energy)
+        counts.append((nd, nls))
+        if log is not None:
+            log(it, energy, history[-1] - history[-2], integral(residual * residual), nd, nls)
+        if callback is not None:
+            callback(it, energy)
+        # check_converge (optimization.py:522-535)
+        if len(history) - 1 >= ncheck and all(abs(history[-2 - i] - energy) <= econv for i in range(ncheck)):
+            converged = 0
+            break
+        dirA.append(p)
+        if len(resA) > 2:
+            resA.pop(0)
+        if len(dirA) > 2:
+            dirA.pop(0)
+    return dict(rho=rho, phi=phi, energy=energy, mu=mu, converged=converged, history=history,
+                counts=counts, potential=pot)

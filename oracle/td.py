@@ -1,0 +1,63 @@
+"""TD-OFDFT Crank-Nicolson propagation, complex128, restating
+``td/hamiltonian.py:70-114``, ``field.py:343-350`` (smoothed Laplacian),
+``td/propagator/crank_nicholson.py:44-56,147-188``, ``linear_solver.py:8-47``
+(CG without conjugation), ``utils/utils.py:40-41`` (calc_rho) and
+``td/utils.py:10-16`` (delta kick)."""
+import numpy as np
+
+
+def hamiltonian_apply(psi, v, grid, sigma=0.025):
+    """H psi = -1/2 lap_sigma(psi) + v psi, lap_sigma = IFFT(-G^2 e^{-G^2 s^2/2} FFT)."""
+    gg = grid.ggF
+    lap = np.fft.ifftn(-gg * np.fft.fftn(psi) * np.exp(-gg * sigma ** 2 / 2.0))
+    return -0.5 * lap + v * psi
+
+
+def _norm(x):
+    return np.sqrt(np.real(np.sum(np.conj(x) * x)))
+
+
+def cg(A, b, x0, tol=1e-8, maxiter=100, atol=None):
+    """``linear_solver.py:24-47``: complex-symmetric CG, sums without conj."""
+    bn = _norm(b)
+    atol = (tol * bn if bn != 0 else tol) if atol is None else max(atol, tol * bn)
+    res = b - A(x0)
+    if _norm(res) < atol:
+        return x0, 0, 0
+    p = res
+    x = x0
+    k = 0
+    while k < maxiter:
+        v = A(p)
+        alpha = np.sum(res * res) / np.sum(p * v)
+        x = x + alpha * p
+        old = res
+        res = old - alpha * v
+        if _norm(res) < atol:
+            return x, 0, k + 1
+        beta = np.sum(res * res) / np.sum(old * old)
+        p = res + beta * p
+        k += 1
+    return x, 1, k
+
+
+def cn_step(psi0, v, grid, dt, tol=1e-8, maxiter=100, sigma=0.025):
+    """``crank_nicholson.py:147-188``: (1 + i H dt/2) psi1 = (1 - i H dt/2) psi0."""
+    H = lambda x: hamiltonian_apply(x, v, grid, sigma)
+    A = lambda x: x + 1j * H(x) * dt / 2.0
+    b = psi0 - 1j * H(psi0) * dt / 2.0
+    return cg(A, b, psi0, tol, maxiter)
+
+
+def initial_kick(k, direction, psi, grid):
+    return psi * np.exp(1j * k * grid.r[direction])
+
+
+def energy(psi, v, grid, sigma=0.025):
+    return np.real(np.conj(psi) * hamiltonian_apply(psi, v, grid, sigma)).sum() * grid.dV
+
+
+def dipole(rho, rho0, grid):
+    """delta mu = int (rho - rho0) r dV (``real_time_runner.py:251-253``)."""
+    d = rho - rho0
+    return np.array([(d * grid.r[i]).sum() * grid.dV for i in range(3)])
