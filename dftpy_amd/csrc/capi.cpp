@@ -1,0 +1,257 @@
+// C ABI of libofdft_hip.so: plan lifetime, FFTs with DFTpy's normalisation,
+// functional settings and evaluation, reductions, local PP / Ewald setup.
+// Every entry point catches exceptions and reports them through
+// ofdft_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "internal.h"
+
+namespace ofdft {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
+              const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+              const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, double *vreal,
+              hipStream_t s);
+void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges, int pme,
+           int order, double precision, double *out, hipStream_t s);
+
+template <typename F> static int guard(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("unknown error");
+        return 1;
+    }
+}
+
+static void inv3(const double *a, double *r) {
+    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                       a[2] * (a[3] * a[7] - a[4] * a[6]);
+    OFDFT_REQUIRE(std::fabs(det) > 0, "singular lattice");
+    r[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    r[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    r[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+static hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+void plan_aux_setup(Plan &P);
+void plan_aux_free(Plan &P);
+
+}  // namespace ofdft
+
+using namespace ofdft;
+
+struct ofdft_plan : public Plan {};
+
+extern "C" {
+
+int ofdft_version(void) { return 1; }
+
+const char *ofdft_last_error(void) { return g_err.c_str(); }
+
+int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, int full, int max_batch,
+                      ofdft_plan **out) {
+    return guard([&] {
+        OFDFT_REQUIRE(out != nullptr, "plan_create: null output pointer");
+        OFDFT_REQUIRE(precision == OFDFT_FP64 || precision == OFDFT_FP32, "precision must be FP64 or FP32");
+        for (int d = 0; d < 3; ++d) OFDFT_REQUIRE(nr[d] >= 1 && nr[d] <= 4096, "grid size out of range");
+        ofdft_plan *P = new ofdft_plan();
+        try {
+            std::memset(P->tw, 0, sizeof(P->tw));
+            P->spec = P->work = P->aux = nullptr;
+            P->red = P->scal = nullptr;
+            for (int d = 0; d < 3; ++d) P->nr[d] = nr[d];
+            P->nzc = nr[2] / 2 + 1;
+            P->N = (long long)nr[0] * nr[1] * nr[2];
+            P->Nc = (long long)nr[0] * nr[1] * P->nzc;
+            P->Nfull = P->N;
+            std::memcpy(P->lat, lattice, sizeof(P->lat));
+            double inv[9];
+            inv3(lattice, inv);
+            // B = 2 pi inv(A)^T: row i of B is column i of inv(A)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) P->B[3 * i + j] = 2.0 * M_PI * inv[3 * j + i];
+            const double *a = lattice;
+            const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                               a[2] * (a[3] * a[7] - a[4] * a[6]);
+            P->volume = std::fabs(det);
+            P->dV = P->volume / (double)P->N;
+            P->prec = precision;
+            P->full = full ? 1 : 0;
+            P->max_batch = max_batch < 1 ? 1 : max_batch;
+            plan_fft_setup(*P);
+            const size_t rs = precision == OFDFT_FP64 ? 8 : 4;
+            const int nslot = P->max_batch > 4 ? P->max_batch : 4;
+            // spectra: nslot half grids, or nslot/2 full complex grids
+            const size_t spec_bytes = std::max((size_t)(nslot * P->Nc), (size_t)(2 * P->N)) * 2 * rs;
+            OFDFT_HIP(hipMalloc(&P->spec, spec_bytes));
+            // work: 4 real fields, at least one FP64 field for setup arrays
+            const size_t work_bytes = std::max((size_t)(4 * P->N * rs), (size_t)(P->N * 8));
+            OFDFT_HIP(hipMalloc(&P->work, work_bytes));
+            P->red_blocks = 4096;
+            OFDFT_HIP(hipMalloc(&P->red, sizeof(double) * 4096 * 16));
+            OFDFT_HIP(hipMalloc(&P->scal, sizeof(double) * 256));
+            OFDFT_HIP(hipMemset(P->scal, 0, sizeof(double) * 256));
+            P->ws_bytes = spec_bytes + work_bytes + sizeof(double) * (4096 * 16 + 256);
+            plan_aux_setup(*P);
+            P->ws_bytes += P->aux_bytes;
+            OFDFT_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            ofdft_plan_destroy(P);
+            throw;
+        }
+        *out = P;
+    });
+}
+
+int ofdft_plan_destroy(ofdft_plan *P) {
+    if (!P) return 0;
+    return guard([&] {
+        plan_fft_free(*P);
+        plan_aux_free(*P);
+        if (P->spec) (void)hipFree(P->spec);
+        if (P->work) (void)hipFree(P->work);
+        if (P->red) (void)hipFree(P->red);
+        if (P->scal) (void)hipFree(P->scal);
+        delete P;
+    });
+}
+
+size_t ofdft_plan_workspace_bytes(const ofdft_plan *P) { return P ? P->ws_bytes : 0; }
+
+int ofdft_fft_r2c(ofdft_plan *P, const void *in, void *out, int batch, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && in && out && batch >= 1, "fft_r2c: bad arguments");
+        if (P->prec == OFDFT_FP64)
+            fft_r2c<double>(*P, (const double *)in, (double2 *)out, batch, P->N, P->Nc, P->dV, nullptr, nullptr,
+                            S(stream));
+        else
+            fft_r2c<float>(*P, (const float *)in, (float2 *)out, batch, P->N, P->Nc, P->dV, nullptr, nullptr,
+                           S(stream));
+    });
+}
+
+int ofdft_fft_c2r(ofdft_plan *P, const void *in, void *out, int batch, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && in && out && batch >= 1, "fft_c2r: bad arguments");
+        const int nslot = P->max_batch > 4 ? P->max_batch : 4;
+        OFDFT_REQUIRE(batch <= nslot, "fft_c2r: batch exceeds the plan's max_batch");
+        const double sc = 1.0 / ((double)P->N * P->dV);
+        if (P->prec == OFDFT_FP64)
+            fft_c2r<double>(*P, (const double2 *)in, (double2 *)P->spec, (double *)out, batch, P->Nc, P->Nc, P->N,
+                            sc, S(stream));
+        else
+            fft_c2r<float>(*P, (const float2 *)in, (float2 *)P->spec, (float *)out, batch, P->Nc, P->Nc, P->N, sc,
+                           S(stream));
+    });
+}
+
+int ofdft_fft_c2c(ofdft_plan *P, const void *in, void *out, int inverse, int batch, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && in && out && batch >= 1, "fft_c2c: bad arguments");
+        const double sc = inverse ? 1.0 / ((double)P->N * P->dV) : P->dV;
+        if (P->prec == OFDFT_FP64)
+            fft_c2c<double>(*P, (const double2 *)in, (double2 *)out, inverse != 0, batch, P->N, sc, S(stream));
+        else
+            fft_c2c<float>(*P, (const float2 *)in, (float2 *)out, inverse != 0, batch, P->N, sc, S(stream));
+    });
+}
+
+int ofdft_set_kedf(ofdft_plan *P, double x, double y, double alpha, double beta, double rho0, double sigma) {
+    return guard([&] {
+        OFDFT_REQUIRE(P, "null plan");
+        OFDFT_REQUIRE(std::fabs(alpha - beta) < 1e-9, "WT with alpha != beta is not supported");
+        P->x = x;
+        P->y = y;
+        P->alpha = alpha;
+        P->beta = beta;
+        P->rho0 = rho0;
+        P->sigma = sigma;
+    });
+}
+
+int ofdft_set_terms(ofdft_plan *P, unsigned terms) {
+    return guard([&] {
+        OFDFT_REQUIRE(P, "null plan");
+        OFDFT_REQUIRE((terms & ~63u) == 0, "unknown term bits");
+        P->terms = terms;
+    });
+}
+
+int ofdft_set_vloc(ofdft_plan *P, const void *vreal) {
+    return guard([&] {
+        OFDFT_REQUIRE(P, "null plan");
+        P->vloc = vreal;
+    });
+}
+
+int ofdft_set_eion(ofdft_plan *P, double eion) {
+    return guard([&] {
+        OFDFT_REQUIRE(P, "null plan");
+        P->eion = eion;
+    });
+}
+
+int ofdft_eval(ofdft_plan *P, const void *rho, unsigned calc, void *pot, double *energies, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho, "eval: bad arguments");
+        OFDFT_REQUIRE(!((P->terms & OFDFT_TERM_PP) && P->vloc == nullptr), "eval: PP term set but no vloc");
+        if (P->prec == OFDFT_FP64)
+            eval<double>(*P, (const double *)rho, calc, (double *)pot, energies, S(stream));
+        else
+            eval<float>(*P, (const float *)rho, calc, (float *)pot, energies, S(stream));
+    });
+}
+
+int ofdft_dot(ofdft_plan *P, const void *a, const void *b, double *out, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && a && out, "dot: bad arguments");
+        if (P->prec == OFDFT_FP64)
+            dot<double>(*P, (const double *)a, (const double *)b, nullptr, P->N, out, 1.0, S(stream));
+        else
+            dot<float>(*P, (const float *)a, (const float *)b, nullptr, P->N, out, 1.0, S(stream));
+    });
+}
+
+int ofdft_local_pp(ofdft_plan *P, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                   const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+                   const double *pos_cart, const double *pos_frac, const int *species, int pme, int order,
+                   void *vreal, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && vreal && nat >= 1, "local_pp: bad arguments");
+        for (int a = 0; a < nat; ++a)
+            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "local_pp: species index out of range");
+        local_pp(*P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species, pme, order,
+                 (double *)vreal, S(stream));
+    });
+}
+
+int ofdft_ewald(ofdft_plan *P, int nat, const double *pos_cart, const double *pos_frac, const double *charges,
+                int pme, int order, double precision, double *out, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && nat >= 1 && out, "ewald: bad arguments");
+        ewald(*P, nat, pos_cart, pos_frac, charges, pme, order, precision, out, S(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,345 @@
+// One TotalFunctional evaluation (total_functional.py:115-146) for the terms
+// TF + vW + WT + PZ-LDA + Hartree + local PP.
+//
+// Pipeline (3 batched spectra for vW, WT and Hartree):
+//   1. z pass R2C with the point-wise prologue applied on load
+//      (sqrt(rhom) for vW, max(rho,1e-30)^beta for WT, rho for Hartree),
+//      also summing rho for rho0 = mean(rho);
+//   2. y and x passes;
+//   3. G-space multiply: vW 1/2 G^2 (vw.py:45-49), WT Lindhard kernel
+//      evaluated from |G| (kernel.py:13-136,325-334), Hartree 4 pi / G^2
+//      (hartree.py:34-36), times 1/N so the unscaled forward+inverse pair
+//      equals DFTpy's (rfftn * dV, irfftn / dV);
+//   4. inverse x, y passes and z pass C2R;
+//   5. post kernel: the total potential and the FP64 energy sums of every
+//      term (tf.py:13-80, vw.py:29-94, wt.py:12-28,124-125,
+//      semilocal_xc.py:252-290, hartree.py:38, pseudo/__init__.py:228).
+#include <cmath>
+
+#include "internal.h"
+#include "reduce.h"
+
+namespace ofdft {
+
+// ------------------------------------------------------------------ Lindhard
+// Inverse Lindhard function with the reference's branch priority
+// normal < zero (eta < tol) < one (|eta-1| < tol) < large (eta > 3.65).
+__device__ __forceinline__ double lindhard(double eta, double lbda, double mu) {
+    const double tol = 1e-10;
+    if (eta > 3.65) {
+        const double inv = 1.0 / (eta * eta);
+        double s = -3.00624946457977689e-3;
+        s = -3.38745061493813488e-3 + inv * s;
+        s = -3.84763737842981233e-3 + inv * s;
+        s = -4.41060829979912465e-3 + inv * s;
+        s = -5.10997527675418131e-3 + inv * s;
+        s = -5.9945117538835746e-3 + inv * s;
+        s = -7.1372762502456763e-3 + inv * s;
+        s = -8.65192783339199453e-3 + inv * s;
+        s = -1.07236045520990083e-2 + inv * s;
+        s = -1.36715733124818332e-2 + inv * s;
+        s = -1.80879839616166146e-2 + inv * s;
+        s = -2.51824061652633074e-2 + inv * s;
+        s = -3.77825602968460128e-2 + inv * s;
+        s = -6.39999999999999875e-2 + inv * s;
+        s = -0.13714285714285712 + inv * s;
+        return 3.0 * (1.0 - mu) * eta * eta - lbda - 0.6 + inv * s;
+    }
+    if (fabs(eta - 1.0) < tol) return 2.0 - lbda - 3.0 * mu + 20.0 * (eta - 1.0);
+    if (eta < tol) return 1.0 - lbda + eta * eta * (1.0 / 3.0 - 3.0 * mu);
+    return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) -
+           3.0 * mu * eta * eta - lbda;
+}
+
+struct GParams {
+    double B[9];
+    int nx, ny, nz, nzc;
+    int full;
+    double scale;
+    int s_vw, s_wt, s_h;
+    double sigma;
+    double lbda, alpha, beta;
+    double rho0;             // used when rho0_dev == nullptr
+    const double *rho0_dev;  // cached rho0 on device
+    long long bs;
+};
+
+__device__ __forceinline__ int ffreq(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
+__device__ __forceinline__ int wrapf(int f, int n) { return ((n & 1) == 0 && f == n / 2) ? -n / 2 : f; }
+
+__device__ __forceinline__ double gsq(const double *B, int fx, int fy, int fz) {
+    const double gx = fx * B[0] + fy * B[3] + fz * B[6];
+    const double gy = fx * B[1] + fy * B[4] + fz * B[7];
+    const double gz = fx * B[2] + fy * B[5] + fz * B[8];
+    return gx * gx + gy * gy + gz * gz;
+}
+
+struct KVals {
+    double vw, wt, h;
+};
+
+__device__ __forceinline__ KVals kernel_vals(const GParams &p, double gg, bool g0, double tkf, double wtfac) {
+    KVals k;
+    k.vw = 0.5 * gg;
+    if (p.sigma > 0) k.vw *= exp(-gg * p.sigma * p.sigma / 4.0);
+    k.wt = lindhard(sqrt(gg) / tkf, p.lbda, 1.0) * wtfac;
+    k.h = g0 ? 0.0 : 4.0 * M_PI / gg;
+    return k;
+}
+
+template <typename T>
+__global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
+    const double rho0 = p.rho0_dev ? *p.rho0_dev : p.rho0;
+    const double tkf = 2.0 * cbrt(3.0 * rho0 * M_PI * M_PI);
+    const double ctf = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+    const double wtfac = 5.0 / (9.0 * p.alpha * p.beta * pow(rho0, p.alpha + p.beta - 5.0 / 3.0)) * ctf;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % p.nzc);
+        const long long rest = i / p.nzc;
+        const int iy = (int)(rest % p.ny);
+        const int ix = (int)(rest / p.ny);
+        int fx = ffreq(ix, p.nx), fy = ffreq(iy, p.ny), fz = iz;
+        const bool g0 = (ix == 0 && iy == 0 && iz == 0);
+        KVals k;
+        if (p.full) {
+            // full-grid convention: z Nyquist is negative; Nyquist aliases average
+            fz = wrapf(fz, p.nz);
+            const bool nyq = ((p.nx & 1) == 0 && ix == p.nx / 2) || ((p.ny & 1) == 0 && iy == p.ny / 2) ||
+                             ((p.nz & 1) == 0 && iz == p.nz / 2);
+            k = kernel_vals(p, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
+            if (nyq) {
+                const KVals k2 = kernel_vals(
+                    p, gsq(p.B, wrapf(-fx, p.nx), wrapf(-fy, p.ny), wrapf(-fz, p.nz)), g0, tkf, wtfac);
+                k.vw = 0.5 * (k.vw + k2.vw);
+                k.wt = 0.5 * (k.wt + k2.wt);
+                k.h = 0.5 * (k.h + k2.h);
+            }
+        } else {
+            k = kernel_vals(p, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
+        }
+        if (p.s_vw >= 0) {
+            cpx<T> &v = spec[p.s_vw * p.bs + i];
+            const double f = k.vw * p.scale;
+            v.x = (T)(v.x * f);
+            v.y = (T)(v.y * f);
+        }
+        if (p.s_wt >= 0) {
+            cpx<T> &v = spec[p.s_wt * p.bs + i];
+            const double f = k.wt * p.scale;
+            v.x = (T)(v.x * f);
+            v.y = (T)(v.y * f);
+        }
+        if (p.s_h >= 0) {
+            cpx<T> &v = spec[p.s_h * p.bs + i];
+            const double f = k.h * p.scale;
+            v.x = (T)(v.x * f);
+            v.y = (T)(v.y * f);
+        }
+    }
+}
+
+// rho0 = sum / N with the kernel-reuse threshold of wt.py:64
+__global__ void k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) {
+        const double r = acc * inv_n;
+        if (fabs(*rho0_cache - r) > 1e-6) *rho0_cache = r;
+    }
+}
+
+struct PostParams {
+    int s_vw, s_wt, s_h;
+    unsigned terms;
+    double x, y, alpha, beta;
+    long long n;
+    long long bs;  // work batch stride (reals)
+    int want_v, want_e;
+};
+
+// PZ LDA constants (semilocal_xc.py:257-263)
+constexpr double PZ_A = 0.0311, PZ_B = -0.048, PZ_C = 0.0020, PZ_D = -0.0116;
+constexpr double PZ_G = -0.1423, PZ_B1 = 1.0529, PZ_B2 = 0.3334;
+
+template <typename T, typename CT>
+__global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, const T *__restrict__ vloc,
+                       T *__restrict__ pot, double *partials, PostParams p) {
+    __shared__ double sm[16];
+    const CT ctf = (CT)(0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0));
+    const CT cbrt34pi = (CT)cbrt(3.0 / (4.0 * M_PI));
+    const CT cx_e = (CT)(-3.0 / 4.0 * cbrt(3.0 / M_PI));
+    const CT cx_v = (CT)cbrt(-3.0 / M_PI);
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const CT r = (CT)rho[i];
+        CT v = (CT)0;
+        if (p.terms & OFDFT_TERM_TF) {
+            const CT r2 = r * r;
+            if (p.want_e) acc[0] += (double)(ctf * cbrt(r2 * r2 * r));
+            v += (CT)p.x * ((CT)(5.0 / 3.0) * ctf * cbrt(r2));
+        }
+        if (p.s_vw >= 0) {
+            const CT a = (CT)work[p.s_vw * p.bs + i];
+            CT sq = sqrt(r > (CT)0 ? r : (CT)1e-30);
+            if (fabs(sq) < (CT)1e-30) sq = (CT)1e-30;
+            const CT vw = (CT)p.y * (a / sq);
+            if (p.want_e && r > (CT)0) acc[1] += (double)(r * vw);
+            v += vw;
+        }
+        if (p.s_wt >= 0) {
+            const CT c = (CT)work[p.s_wt * p.bs + i];
+            const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
+            const CT rb = pow(rc, (CT)p.beta);
+            const CT vnl = (CT)(2.0 * p.alpha) * (rb / rc) * c;
+            if (p.want_e) acc[2] += (double)(vnl * r) / (2.0 * p.alpha);
+            v += vnl;
+        }
+        if (p.terms & OFDFT_TERM_XC) {
+            CT rc3 = cbrt(r);
+            if (rc3 < (CT)1e-30) rc3 = (CT)1e-30;
+            const CT rs = cbrt34pi / rc3;
+            CT ec, vc;
+            if (rs < (CT)1) {
+                const CT lr = log(rs);
+                ec = (CT)PZ_A * lr + (CT)PZ_B + (CT)PZ_C * rs * lr + (CT)PZ_D * rs;
+                vc = lr * ((CT)PZ_A + (CT)(2.0 / 3.0 * PZ_C) * rs) + (CT)PZ_B - (CT)(1.0 / 3.0 * PZ_A) +
+                     (CT)(1.0 / 3.0 * (2 * PZ_D - PZ_C)) * rs;
+            } else {
+                const CT sr = sqrt(rs);
+                const CT den = (CT)1 + (CT)PZ_B1 * sr + (CT)PZ_B2 * rs;
+                ec = (CT)PZ_G / den;
+                vc = ((CT)PZ_G + (CT)(7.0 / 6.0 * PZ_G * PZ_B1) * sr + (CT)(4.0 / 3.0 * PZ_G * PZ_B2) * rs) /
+                     (den * den);
+            }
+            if (p.want_e) acc[3] += (double)((cx_e * rc3 + ec) * r);
+            v += cx_v * rc3 + vc;
+        }
+        if (p.s_h >= 0) {
+            const CT vh = (CT)work[p.s_h * p.bs + i];
+            if (p.want_e) acc[4] += 0.5 * (double)vh * (double)r;
+            v += vh;
+        }
+        if (vloc) {
+            const CT vl = (CT)vloc[i];
+            if (p.want_e) acc[5] += (double)vl * (double)r;
+            v += vl;
+        }
+        acc[6] += (double)r;
+        if (p.want_v) pot[i] = (T)v;
+    }
+    if (p.want_e) {
+        for (int t = 0; t < 7; ++t) {
+            const double s = block_sum(acc[t], sm);
+            if (threadIdx.x == 0) partials[(long long)blockIdx.x * 7 + t] = s;
+        }
+    }
+}
+
+// energies[0..5] = term sums * dV (TF also * x), [6] = their sum + E_ion,
+// [7] = electron count
+__global__ void k_energy_final(const double *__restrict__ partials, int nb, double *out, double dV, double x,
+                               double eion) {
+    __shared__ double sm[16];
+    double tot = 0.0;
+    for (int t = 0; t < 7; ++t) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[(long long)b * 7 + t];
+        acc = block_sum(acc, sm);
+        if (threadIdx.x == 0) {
+            double e = acc * dV;
+            if (t == 0) e *= x;
+            if (t < 6) {
+                out[t] = e;
+                tot += e;
+            } else {
+                out[7] = e;
+            }
+        }
+    }
+    if (threadIdx.x == 0) out[6] = tot + eion;
+}
+
+static int grid_blocks(long long n, int per = 256 * 4) {
+    long long b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return (int)b;
+}
+
+template <typename T>
+void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s) {
+    const unsigned terms = P.terms;
+    int ns = 0;
+    const int s_vw = (terms & OFDFT_TERM_VW) ? ns++ : -1;
+    const int s_wt = (terms & OFDFT_TERM_WT) ? ns++ : -1;
+    const int s_h = (terms & OFDFT_TERM_HARTREE) ? ns++ : -1;
+    cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
+    T *work = reinterpret_cast<T *>(P.work);
+    double *rho0_cache = P.scal + 0;
+    if (ns > 0) {
+        Prologue pro{};
+        if (s_vw >= 0) pro.code[s_vw] = OP_SQRT_RHOM;
+        if (s_wt >= 0) pro.code[s_wt] = OP_POW_BETA;
+        if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
+        pro.beta = P.beta;
+        const bool auto_rho0 = (s_wt >= 0) && !(P.rho0 > 0);
+        fft_r2c<T>(P, rho, spec, ns, 0, P.Nc, 1.0, &pro, auto_rho0 ? P.red : nullptr, s);
+        if (auto_rho0) {
+            const long long nrows = (long long)P.nr[0] * P.nr[1];
+            const int nb = (int)((nrows + P.rows_r2c.tile - 1) / P.rows_r2c.tile);
+            hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(256), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
+            OFDFT_HIP(hipGetLastError());
+        }
+        GParams gp;
+        for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
+        gp.nx = P.nr[0];
+        gp.ny = P.nr[1];
+        gp.nz = P.nr[2];
+        gp.nzc = P.nzc;
+        gp.full = P.full;
+        gp.scale = 1.0 / (double)P.N;
+        gp.s_vw = s_vw;
+        gp.s_wt = s_wt;
+        gp.s_h = s_h;
+        gp.sigma = P.sigma;
+        gp.lbda = P.x;  // WT kernel lambda = x, mu = 1 (wt.py:66, kernel.py:331-334)
+        gp.alpha = P.alpha;
+        gp.beta = P.beta;
+        gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
+        gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
+        gp.bs = P.Nc;
+        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(P.Nc)), dim3(256), 0, s, spec, P.Nc, gp);
+        OFDFT_HIP(hipGetLastError());
+        fft_c2r<T>(P, spec, spec, work, ns, P.Nc, P.Nc, P.N, 1.0, s);
+    }
+    PostParams pp;
+    pp.s_vw = s_vw;
+    pp.s_wt = s_wt;
+    pp.s_h = s_h;
+    pp.terms = terms;
+    pp.x = P.x;
+    pp.y = P.y;
+    pp.alpha = P.alpha;
+    pp.beta = P.beta;
+    pp.n = P.N;
+    pp.bs = P.N;
+    pp.want_v = (calc & OFDFT_CALC_V) && pot != nullptr;
+    pp.want_e = (calc & OFDFT_CALC_E) && energies != nullptr;
+    const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
+    const int nb = std::min(reduce_blocks(P.N), 1024);
+    hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, P.red, pp);
+    OFDFT_HIP(hipGetLastError());
+    if (pp.want_e) {
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
+        OFDFT_HIP(hipGetLastError());
+    }
+}
+
+template void eval<float>(Plan &, const float *, unsigned, float *, double *, hipStream_t);
+template void eval<double>(Plan &, const double *, unsigned, double *, double *, hipStream_t);
+
+}  // namespace ofdft

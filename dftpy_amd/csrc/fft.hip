@@ -1,0 +1,661 @@
+// Mixed-radix (2/3/4/5/8, and 7/11/13) 3D FFT for gfx950.
+//
+// A 3D transform is three axis passes.  The z pass works on contiguous rows
+// (R2C/C2R pack two real rows into one complex FFT); the y and x passes work
+// on strided pencils, staging a tile of `tile` adjacent columns x n points in
+// LDS so every global access is a contiguous run of tile*sizeof(complex)
+// bytes.  Inside LDS each pencil is transformed by a Stockham autosort
+// sequence of radix-R stages (one butterfly per thread per stage), twiddles
+// read from an LDS copy of exp(-2 pi i m / n).
+//
+// Normalisation is left to the caller through `scale`, applied on the final
+// store of the last pass.  The DFTpy conventions (forward * dV, inverse
+// / (N dV), field.py:373-390,739-764) are applied by the C ABI layer.
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace ofdft {
+
+// ------------------------------------------------------------------ complex
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename C> __device__ __forceinline__ C mul_mi(C a) {
+    return INV ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+// ------------------------------------------------------------------ small DFTs
+template <bool INV, typename C> __device__ __forceinline__ void dft2(C &a, C &b) {
+    C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV, typename C> __device__ __forceinline__ void dft4(C &v0, C &v1, C &v2, C &v3) {
+    C t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), t3 = mul_mi<INV>(csub(v1, v3));
+    v0 = cadd(t0, t2);
+    v2 = csub(t0, t2);
+    v1 = cadd(t1, t3);
+    v3 = csub(t1, t3);
+}
+
+template <int R, bool INV, typename C> struct DFT;
+
+template <bool INV, typename C> struct DFT<2, INV, C> {
+    __device__ __forceinline__ static void run(C *v) { dft2<INV>(v[0], v[1]); }
+};
+template <bool INV, typename C> struct DFT<4, INV, C> {
+    __device__ __forceinline__ static void run(C *v) { dft4<INV>(v[0], v[1], v[2], v[3]); }
+};
+template <bool INV, typename C> struct DFT<3, INV, C> {
+    __device__ __forceinline__ static void run(C *v) {
+        using T = decltype(v[0].x);
+        const T s = (T)0.86602540378443864676;  // sin(2 pi / 3)
+        C t = cadd(v[1], v[2]);
+        C m = C{v[0].x - (T)0.5 * t.x, v[0].y - (T)0.5 * t.y};
+        C d = csub(v[1], v[2]);
+        C sd = mul_mi<INV>(C{s * d.x, s * d.y});  // -i s d (forward)
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, sd);
+        v[2] = csub(m, sd);
+    }
+};
+template <bool INV, typename C> struct DFT<5, INV, C> {
+    __device__ __forceinline__ static void run(C *v) {
+        using T = decltype(v[0].x);
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+        C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        C t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        C a1 = C{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+        C a2 = C{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+        C b1 = mul_mi<INV>(C{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+        C b2 = mul_mi<INV>(C{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+        v[0] = C{v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y};
+        v[1] = cadd(a1, b1);
+        v[4] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[3] = csub(a2, b2);
+    }
+};
+template <bool INV, typename C> struct DFT<8, INV, C> {
+    __device__ __forceinline__ static void run(C *v) {
+        using T = decltype(v[0].x);
+        const T h = (T)0.70710678118654752440;
+        C a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+        C b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+        dft4<INV>(a0, a1, a2, a3);
+        dft4<INV>(b0, b1, b2, b3);
+        // b_k *= W8^k (forward W8 = e^{-i pi/4}), conjugated for the inverse
+        if (!INV) {
+            b1 = C{h * (b1.x + b1.y), h * (b1.y - b1.x)};
+            b2 = C{b2.y, -b2.x};
+            b3 = C{h * (b3.y - b3.x), -h * (b3.x + b3.y)};
+        } else {
+            b1 = C{h * (b1.x - b1.y), h * (b1.x + b1.y)};
+            b2 = C{-b2.y, b2.x};
+            b3 = C{-h * (b3.x + b3.y), h * (b3.x - b3.y)};
+        }
+        v[0] = cadd(a0, b0);
+        v[4] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[5] = csub(a1, b1);
+        v[2] = cadd(a2, b2);
+        v[6] = csub(a2, b2);
+        v[3] = cadd(a3, b3);
+        v[7] = csub(a3, b3);
+    }
+};
+
+// Odd-prime radix by the symmetric direct DFT: for k = 1..(R-1)/2
+// a_k = x0 + sum_j cos(2 pi jk/R)(x_j + x_{R-j}), b_k = sum_j sin(2 pi jk/R)(x_j - x_{R-j}),
+// y_k = a_k - i b_k, y_{R-k} = a_k + i b_k (forward; conjugate for inverse).
+template <int R> struct OddTab;
+template <> struct OddTab<7> {
+    static constexpr double c[3] = {6.23489801858733594386e-01, -2.22520933956314337365e-01, -9.00968867902419034976e-01};
+    static constexpr double s[3] = {7.81831482468029803634e-01, 9.74927912181823619342e-01, 4.33883739117558231424e-01};
+};
+template <> struct OddTab<11> {
+    static constexpr double c[5] = {8.41253532831181205509e-01, 4.15415013001886435084e-01, -1.42314838273285004799e-01, -6.54860733945284989588e-01, -9.59492973614497368651e-01};
+    static constexpr double s[5] = {5.40640817455597555430e-01, 9.09631995354518330110e-01, 9.89821441880932795243e-01, 7.55749574354258268905e-01, 2.81732556841429671035e-01};
+};
+template <> struct OddTab<13> {
+    static constexpr double c[6] = {8.85456025653209910509e-01, 5.68064746731155922888e-01, 1.20536680255323006006e-01, -3.54604887042535454889e-01, -7.48510748171101192305e-01, -9.70941817426052011797e-01};
+    static constexpr double s[6] = {4.64723172043768506523e-01, 8.22983865893656352242e-01, 9.92708874098053972723e-01, 9.35016242685414833424e-01, 6.63122658240795193052e-01, 2.39315664287557683387e-01};
+};
+
+template <int R, bool INV, typename C> struct DFTOdd {
+    __device__ __forceinline__ static void run(C *v) {
+        using T = decltype(v[0].x);
+        constexpr int H = (R - 1) / 2;
+        C sp[H], sm[H];
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            sp[j - 1] = cadd(v[j], v[R - j]);
+            sm[j - 1] = csub(v[j], v[R - j]);
+        }
+        C y0 = v[0];
+#pragma unroll
+        for (int j = 0; j < H; ++j) y0 = cadd(y0, sp[j]);
+        C out[R];
+        out[0] = y0;
+#pragma unroll
+        for (int k = 1; k <= H; ++k) {
+            C a = v[0], b = C{(T)0, (T)0};
+#pragma unroll
+            for (int j = 1; j <= H; ++j) {
+                const int m = (j * k) % R;  // cos/sin of 2 pi m / R with symmetry
+                const int mm = m <= H ? m : R - m;
+                const T cj = (T)OddTab<R>::c[mm - 1];
+                const T sj = (T)(m <= H ? OddTab<R>::s[mm - 1] : -OddTab<R>::s[mm - 1]);
+                a = C{a.x + cj * sp[j - 1].x, a.y + cj * sp[j - 1].y};
+                b = C{b.x + sj * sm[j - 1].x, b.y + sj * sm[j - 1].y};
+            }
+            const C ib = mul_mi<INV>(b);  // -i b (forward)
+            out[k] = cadd(a, ib);
+            out[R - k] = csub(a, ib);
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = out[k];
+    }
+};
+template <bool INV, typename C> struct DFT<7, INV, C> : DFTOdd<7, INV, C> {};
+template <bool INV, typename C> struct DFT<11, INV, C> : DFTOdd<11, INV, C> {};
+template <bool INV, typename C> struct DFT<13, INV, C> : DFTOdd<13, INV, C> {};
+
+// One Stockham stage on `npen` pencils of length n held in LDS.  Element e of
+// pencil p lives at buf[p * pstride + e * estride].
+template <int R, bool INV, bool PEN_FAST, typename C>
+__device__ __forceinline__ void stage(C *buf, const C *tw, int n, int Ns, int npen, int pstride, int estride) {
+    const int nb = n / R;
+    const int total = nb * npen;
+    const int b = threadIdx.x;
+    const bool act = b < total;
+    C v[R];
+    int pen = 0, j = 0;
+    if (act) {
+        if (PEN_FAST) {
+            pen = b % npen;
+            j = b / npen;
+        } else {
+            pen = b / nb;
+            j = b - pen * nb;
+        }
+        const C *base = buf + pen * pstride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = base[(j + r * nb) * estride];
+        if (Ns > 1) {
+            const int k = j % Ns;
+            const int step = n / (Ns * R);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C w = tw[r * k * step];
+                if (INV) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+    }
+    __syncthreads();
+    if (act) {
+        DFT<R, INV, C>::run(v);
+        const int k = j % Ns;
+        const int d = (j / Ns) * Ns * R + k;
+        C *base = buf + pen * pstride;
+#pragma unroll
+        for (int r = 0; r < R; ++r) base[(d + r * Ns) * estride] = v[r];
+    }
+    __syncthreads();
+}
+
+template <bool INV, bool PEN_FAST, typename C>
+__device__ void lds_fft(C *buf, const C *tw, const AxisPlan &ap, int npen, int pstride, int estride) {
+    int Ns = 1;
+    for (int s = 0; s < ap.nst; ++s) {
+        const int R = ap.rad[s];
+        switch (R) {
+            case 2: stage<2, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            case 3: stage<3, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            case 4: stage<4, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            case 5: stage<5, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            case 7: stage<7, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            case 11: stage<11, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            case 13: stage<13, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+            default: stage<8, INV, PEN_FAST>(buf, tw, ap.n, Ns, npen, pstride, estride); break;
+        }
+        Ns *= R;
+    }
+}
+
+// ------------------------------------------------------------------ prologue
+template <typename T>
+__device__ __forceinline__ T prologue(int code, T x, double beta, const T *aux, long long idx) {
+    switch (code) {
+        case OP_SQRT_RHOM: return sqrt(x > (T)0 ? x : (T)1e-30);
+        case OP_POW_BETA: return pow(x < (T)1e-30 ? (T)1e-30 : x, (T)beta);
+        case OP_DSQRT: {
+            T r = aux[idx];
+            T s = sqrt(r > (T)0 ? r : (T)1e-30);
+            return x / ((T)2 * s);
+        }
+        case OP_DPOW_BETA: {
+            T r = aux[idx];
+            T rc = r < (T)1e-30 ? (T)1e-30 : r;
+            return (T)beta * pow(rc, (T)beta - (T)1) * x;
+        }
+        default: return x;
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+// Columns pass: data viewed as [outer][n][M]; transform along n for a tile of
+// `tile` adjacent columns.  in may equal out.
+template <typename T, bool INV>
+__global__ void k_cols(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, long long bs, int n,
+                       long long M, int tiles, AxisPlan ap, const cpx<T> *__restrict__ twg, int tile, T scale) {
+    using C = cpx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *buf = tw + n;
+    const int t = blockIdx.x % tiles;
+    const long long o = blockIdx.x / tiles;
+    const long long off = (long long)blockIdx.y * bs + o * n * M + (long long)t * tile;
+    const C *src = in + off;
+    C *dst = out + off;
+    const int mv = (int)min((long long)tile, M - (long long)t * tile);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    const int tot = n * tile;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int i = e / tile, c = e - i * tile;
+        buf[e] = (c < mv) ? src[(long long)i * M + c] : C{(T)0, (T)0};
+    }
+    __syncthreads();
+    lds_fft<INV, true>(buf, tw, ap, tile, 1, tile);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int i = e / tile, c = e - i * tile;
+        if (c < mv) {
+            C v = buf[e];
+            dst[(long long)i * M + c] = C{v.x * scale, v.y * scale};
+        }
+    }
+}
+
+// Rows pass, real -> half complex.  Rows 2p and 2p+1 are packed into one
+// complex FFT z = a + i b and separated afterwards:
+// A_k = (Z_k + conj Z_{-k}) / 2, B_k = (Z_k - conj Z_{-k}) / (2i).
+// Optionally accumulates the raw row sum of batch slot `sum_slot` in FP64
+// partials[blockIdx.x] (used for rho0 = mean(rho)).
+template <typename T>
+__global__ void k_rows_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out, long long in_bs, long long out_bs,
+                           long long nrows, int n, int nzc, AxisPlan ap, const cpx<T> *__restrict__ twg, int rows,
+                           T scale, Prologue pro, double *partials, int sum_slot) {
+    using C = cpx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *buf = tw + n;
+    const int slot = blockIdx.y;
+    const long long row0 = (long long)blockIdx.x * rows;
+    const T *src = in + (long long)slot * in_bs + row0 * n;
+    C *dst = out + (long long)slot * out_bs + row0 * nzc;
+    const int nv = (int)min((long long)rows, nrows - row0);
+    const int code = pro.code[slot & 3];
+    const T *aux = reinterpret_cast<const T *>(pro.aux);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    const int tot = rows * n;
+    double acc = 0.0;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / n;
+        T x = (T)0;
+        if (r < nv) {
+            x = src[e];
+            acc += (double)x;
+            x = prologue<T>(code, x, pro.beta, aux, row0 * n + e);
+        }
+        T *pb = reinterpret_cast<T *>(&buf[(r >> 1) * n + (e - r * n)]);
+        pb[r & 1] = x;
+    }
+    if (partials != nullptr && slot == sum_slot) {
+        // block sum of the raw input (deterministic tree order)
+        __shared__ double red[16];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        if (lane == 0) red[wid] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[w];
+            partials[blockIdx.x] = s;
+        }
+    }
+    __syncthreads();
+    lds_fft<false, false>(buf, tw, ap, rows >> 1, n, 1);
+    const int toto = rows * nzc;
+    for (int e = threadIdx.x; e < toto; e += blockDim.x) {
+        const int r = e / nzc, k = e - r * nzc;
+        if (r < nv) {
+            const int p = r >> 1;
+            C z = buf[p * n + k];
+            C zc = buf[p * n + (k == 0 ? 0 : n - k)];
+            zc.y = -zc.y;
+            C o;
+            if ((r & 1) == 0) {
+                o = C{(z.x + zc.x) * (T)0.5, (z.y + zc.y) * (T)0.5};
+            } else {
+                o = C{(z.y - zc.y) * (T)0.5, -(z.x - zc.x) * (T)0.5};
+            }
+            dst[e] = C{o.x * scale, o.y * scale};
+        }
+    }
+}
+
+// Rows pass, half complex -> real (irfft semantics: the imaginary parts of
+// the k = 0 and Nyquist entries are ignored).
+template <typename T>
+__global__ void k_rows_c2r(const cpx<T> *__restrict__ in, T *__restrict__ out, long long in_bs, long long out_bs,
+                           long long nrows, int n, int nzc, AxisPlan ap, const cpx<T> *__restrict__ twg, int rows,
+                           T scale) {
+    using C = cpx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *stg = tw + n;
+    C *buf = stg + rows * nzc;
+    const long long row0 = (long long)blockIdx.x * rows;
+    const C *src = in + (long long)blockIdx.y * in_bs + row0 * nzc;
+    T *dst = out + (long long)blockIdx.y * out_bs + row0 * n;
+    const int nv = (int)min((long long)rows, nrows - row0);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    const int tot = rows * nzc;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int r = e / nzc, k = e - r * nzc;
+        C v = (r < nv) ? src[e] : C{(T)0, (T)0};
+        if (k == 0 || 2 * k == n) v.y = (T)0;
+        stg[e] = v;
+    }
+    __syncthreads();
+    const int npen = rows >> 1;
+    const int totb = npen * n;
+    for (int e = threadIdx.x; e < totb; e += blockDim.x) {
+        const int p = e / n, k = e - p * n;
+        C A, B;
+        if (k < nzc) {
+            A = stg[(2 * p) * nzc + k];
+            B = stg[(2 * p + 1) * nzc + k];
+        } else {
+            A = stg[(2 * p) * nzc + n - k];
+            B = stg[(2 * p + 1) * nzc + n - k];
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        buf[e] = C{A.x - B.y, A.y + B.x};
+    }
+    __syncthreads();
+    lds_fft<true, false>(buf, tw, ap, npen, n, 1);
+    const int toto = rows * n;
+    for (int e = threadIdx.x; e < toto; e += blockDim.x) {
+        const int r = e / n;
+        if (r < nv) {
+            const C z = buf[(r >> 1) * n + (e - r * n)];
+            dst[e] = ((r & 1) ? z.y : z.x) * scale;
+        }
+    }
+}
+
+// Rows pass, complex -> complex on contiguous rows of length n.
+template <typename T, bool INV>
+__global__ void k_rows_c2c(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, long long bs, long long nrows,
+                           int n, AxisPlan ap, const cpx<T> *__restrict__ twg, int rows, T scale) {
+    using C = cpx<T>;
+    extern __shared__ unsigned char smem_raw[];
+    C *tw = reinterpret_cast<C *>(smem_raw);
+    C *buf = tw + n;
+    const long long row0 = (long long)blockIdx.x * rows;
+    const C *src = in + (long long)blockIdx.y * bs + row0 * n;
+    C *dst = out + (long long)blockIdx.y * bs + row0 * n;
+    const int nv = (int)min((long long)rows, nrows - row0);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
+    const int tot = rows * n;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) buf[e] = (e / n < nv) ? src[e] : C{(T)0, (T)0};
+    __syncthreads();
+    lds_fft<INV, false>(buf, tw, ap, rows, n, 1);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x)
+        if (e / n < nv) {
+            C v = buf[e];
+            dst[e] = C{v.x * scale, v.y * scale};
+        }
+}
+
+// ------------------------------------------------------------------ host side
+static AxisPlan make_axis(int n) {
+    AxisPlan a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = n;
+    int m = n, e2 = 0;
+    std::vector<int> r;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++e2;
+    }
+    if (e2 % 3 == 0) {
+        for (int i = 0; i < e2 / 3; ++i) r.push_back(8);
+    } else if (e2 % 3 == 1) {
+        if (e2 >= 4) {
+            for (int i = 0; i < e2 / 3 - 1; ++i) r.push_back(8);
+            r.push_back(4);
+            r.push_back(4);
+        } else {
+            r.push_back(2);
+        }
+    } else {
+        for (int i = 0; i < e2 / 3; ++i) r.push_back(8);
+        r.push_back(4);
+    }
+    while (m % 5 == 0) {
+        m /= 5;
+        r.push_back(5);
+    }
+    while (m % 3 == 0) {
+        m /= 3;
+        r.push_back(3);
+    }
+    for (int pr : {7, 11, 13})
+        while (m % pr == 0) {
+            m /= pr;
+            r.push_back(pr);
+        }
+    OFDFT_REQUIRE(m == 1, "FFT length " + std::to_string(n) + " has a prime factor above 13");
+    OFDFT_REQUIRE((int)r.size() <= kMaxStages, "FFT length too large");
+    a.nst = (int)r.size();
+    a.rmin = 64;
+    for (size_t i = 0; i < r.size(); ++i) {
+        a.rad[i] = r[i];
+        a.rmin = std::min(a.rmin, r[i]);
+    }
+    if (a.nst == 0) a.rmin = 1;
+    return a;
+}
+
+static int round64(int t) { return ((t + 63) / 64) * 64; }
+
+static PassShape cols_shape(const AxisPlan &a, long long M, size_t cs) {
+    const int tpp = a.n / a.rmin;
+    int tile = cs == 8 ? 32 : 16;  // 256-byte contiguous runs
+    if ((long long)tile > M) tile = (int)M;
+    const size_t lds_cap = 64 * 1024;
+    while (tile > 1 && (tile * tpp > 1024 || (size_t)(tile + 1) * a.n * cs > lds_cap)) tile = (tile + 1) / 2;
+    OFDFT_REQUIRE(tile * tpp <= 1024 && (size_t)(tile + 1) * a.n * cs <= lds_cap, "FFT axis too long");
+    PassShape p;
+    p.tile = tile;
+    p.threads = std::min(1024, std::max(64, round64(tile * tpp)));
+    p.lds = (int)((size_t)(tile + 1) * a.n * cs);
+    return p;
+}
+
+static PassShape rows_shape(const AxisPlan &a, int nzc, size_t cs, int mode /*0 r2c,1 c2r,2 c2c*/) {
+    const int tpp = a.n / a.rmin;
+    int npen = 16;
+    const size_t lds_cap = 64 * 1024;
+    auto lds_of = [&](int np) -> size_t {
+        if (mode == 0) return (size_t)(np * a.n + a.n) * cs;
+        if (mode == 1) return (size_t)(2 * np * nzc + np * a.n + a.n) * cs;
+        return (size_t)(np * a.n + a.n) * cs;
+    };
+    while (npen > 1 && (npen * tpp > 1024 || lds_of(npen) > lds_cap)) --npen;
+    OFDFT_REQUIRE(npen * tpp <= 1024 && lds_of(npen) <= lds_cap, "FFT row too long");
+    PassShape p;
+    p.tile = mode == 2 ? npen : 2 * npen;
+    p.threads = std::min(1024, std::max(64, round64(npen * tpp)));
+    p.lds = (int)lds_of(npen);
+    return p;
+}
+
+void plan_fft_setup(Plan &P) {
+    const size_t cs = P.prec == OFDFT_FP64 ? 16 : 8;
+    for (int d = 0; d < 3; ++d) {
+        P.ax[d] = make_axis(P.nr[d]);
+        const int n = P.nr[d];
+        std::vector<double> tw(2 * n);
+        for (int m = 0; m < n; ++m) {
+            // exp(-2 pi i m / n) with the angle reduced to the first octant
+            long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+            tw[2 * m] = (double)cosl(ang);
+            tw[2 * m + 1] = (double)-sinl(ang);
+        }
+        OFDFT_HIP(hipMalloc(&P.tw[d], n * cs));
+        if (P.prec == OFDFT_FP64) {
+            OFDFT_HIP(hipMemcpy(P.tw[d], tw.data(), n * cs, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> twf(2 * n);
+            for (int i = 0; i < 2 * n; ++i) twf[i] = (float)tw[i];
+            OFDFT_HIP(hipMemcpy(P.tw[d], twf.data(), n * cs, hipMemcpyHostToDevice));
+        }
+    }
+    P.rows_r2c = rows_shape(P.ax[2], P.nzc, cs, 0);
+    P.rows_c2r = rows_shape(P.ax[2], P.nzc, cs, 1);
+    P.rows_c2c = rows_shape(P.ax[2], P.nzc, cs, 2);
+    P.cols_y = cols_shape(P.ax[1], P.nzc, cs);
+    P.cols_x = cols_shape(P.ax[0], (long long)P.nr[1] * P.nzc, cs);
+    P.cols_yc = cols_shape(P.ax[1], P.nr[2], cs);
+    P.cols_xc = cols_shape(P.ax[0], (long long)P.nr[1] * P.nr[2], cs);
+}
+
+void plan_fft_free(Plan &P) {
+    for (int d = 0; d < 3; ++d)
+        if (P.tw[d]) {
+            (void)hipFree(P.tw[d]);
+            P.tw[d] = nullptr;
+        }
+}
+
+template <typename T>
+void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in, cpx<T> *out, int batch,
+              long long bs, double scale, hipStream_t s) {
+    const int n = P.nr[axis];
+    if (n == 1 && in == out && scale == 1.0) return;
+    const long long zlen = full_grid ? P.nr[2] : P.nzc;
+    long long M, outer;
+    PassShape sh;
+    if (axis == 1) {
+        M = zlen;
+        outer = P.nr[0];
+        sh = full_grid ? P.cols_yc : P.cols_y;
+    } else {
+        M = (long long)P.nr[1] * zlen;
+        outer = 1;
+        sh = full_grid ? P.cols_xc : P.cols_x;
+    }
+    const int tiles = (int)((M + sh.tile - 1) / sh.tile);
+    dim3 grid((unsigned)(outer * tiles), (unsigned)batch);
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[axis]);
+    if (inverse)
+        hipLaunchKernelGGL((k_cols<T, true>), grid, dim3(sh.threads), sh.lds, s, in, out, bs, n, M, tiles,
+                           P.ax[axis], tw, sh.tile, (T)scale);
+    else
+        hipLaunchKernelGGL((k_cols<T, false>), grid, dim3(sh.threads), sh.lds, s, in, out, bs, n, M, tiles,
+                           P.ax[axis], tw, sh.tile, (T)scale);
+    OFDFT_HIP(hipGetLastError());
+}
+
+template <typename T>
+void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, double scale,
+             const Prologue *pro, double *sum_partials, hipStream_t s) {
+    Prologue pr;
+    std::memset(&pr, 0, sizeof(pr));
+    int sum_slot = -1;
+    if (pro) pr = *pro;
+    if (sum_partials) {
+        sum_slot = 0;
+        for (int b = 0; b < batch; ++b)
+            if (pr.code[b & 3] == OP_IDENTITY) sum_slot = b;
+    }
+    const long long nrows = (long long)P.nr[0] * P.nr[1];
+    const PassShape sh = P.rows_r2c;
+    dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
+    const bool last_z = (P.nr[0] == 1 && P.nr[1] == 1);
+    hipLaunchKernelGGL((k_rows_r2c<T>), grid, dim3(sh.threads), sh.lds, s, in, out, in_bs, out_bs, nrows, P.nr[2],
+                       P.nzc, P.ax[2], tw, sh.tile, (T)(last_z ? scale : 1.0), pr, sum_partials, sum_slot);
+    OFDFT_HIP(hipGetLastError());
+    if (last_z) return;
+    fft_cols<T>(P, 1, false, false, out, out, batch, out_bs, 1.0, s);
+    fft_cols<T>(P, 0, false, false, out, out, batch, out_bs, scale, s);
+}
+
+template <typename T>
+void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long long in_bs, long long tmp_bs,
+             long long out_bs, double scale, hipStream_t s) {
+    const cpx<T> *src = in;
+    long long src_bs = in_bs;
+    if (!(P.nr[0] == 1 && P.nr[1] == 1)) {
+        OFDFT_REQUIRE(in_bs == tmp_bs, "c2r: batch strides of input and scratch differ");
+        fft_cols<T>(P, 0, false, true, in, tmp, batch, tmp_bs, 1.0, s);
+        fft_cols<T>(P, 1, false, true, tmp, tmp, batch, tmp_bs, 1.0, s);
+        src = tmp;
+        src_bs = tmp_bs;
+    }
+    const long long nrows = (long long)P.nr[0] * P.nr[1];
+    const PassShape sh = P.rows_c2r;
+    dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
+    hipLaunchKernelGGL((k_rows_c2r<T>), grid, dim3(sh.threads), sh.lds, s, src, out, src_bs, out_bs, nrows,
+                       P.nr[2], P.nzc, P.ax[2], tw, sh.tile, (T)scale);
+    OFDFT_HIP(hipGetLastError());
+}
+
+template <typename T>
+void fft_c2c(Plan &P, const cpx<T> *in, cpx<T> *out, bool inverse, int batch, long long bs, double scale,
+             hipStream_t s) {
+    const long long nrows = (long long)P.nr[0] * P.nr[1];
+    const PassShape sh = P.rows_c2c;
+    dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
+    const bool last_z = (P.nr[0] == 1 && P.nr[1] == 1);
+    const T sz = (T)(last_z ? scale : 1.0);
+    if (inverse)
+        hipLaunchKernelGGL((k_rows_c2c<T, true>), grid, dim3(sh.threads), sh.lds, s, in, out, bs, nrows, P.nr[2],
+                           P.ax[2], tw, sh.tile, sz);
+    else
+        hipLaunchKernelGGL((k_rows_c2c<T, false>), grid, dim3(sh.threads), sh.lds, s, in, out, bs, nrows, P.nr[2],
+                           P.ax[2], tw, sh.tile, sz);
+    OFDFT_HIP(hipGetLastError());
+    if (last_z) return;
+    fft_cols<T>(P, 1, true, inverse, out, out, batch, bs, 1.0, s);
+    fft_cols<T>(P, 0, true, inverse, out, out, batch, bs, scale, s);
+}
+
+#define INST(T)                                                                                               \
+    template void fft_cols<T>(Plan &, int, bool, bool, const cpx<T> *, cpx<T> *, int, long long, double,       \
+                              hipStream_t);                                                                   \
+    template void fft_r2c<T>(Plan &, const T *, cpx<T> *, int, long long, long long, double, const Prologue *, \
+                             double *, hipStream_t);                                                          \
+    template void fft_c2r<T>(Plan &, const cpx<T> *, cpx<T> *, T *, int, long long, long long, long long,      \
+                             double, hipStream_t);                                                            \
+    template void fft_c2c<T>(Plan &, const cpx<T> *, cpx<T> *, bool, int, long long, double, hipStream_t);
+INST(float)
+INST(double)
+#undef INST
+
+}  // namespace ofdft

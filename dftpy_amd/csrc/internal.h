@@ -1,0 +1,127 @@
+// Internal declarations shared by the libofdft_hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ofdft_hip.h"
+
+namespace ofdft {
+
+#define OFDFT_HIP(x)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess)                                                             \
+            throw std::runtime_error(std::string(#x) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define OFDFT_REQUIRE(cond, msg)                                  \
+    do {                                                          \
+        if (!(cond)) throw std::invalid_argument(std::string(msg)); \
+    } while (0)
+
+void set_error(const std::string &msg);
+
+template <typename T> struct cpx_t;
+template <> struct cpx_t<float> { using type = float2; };
+template <> struct cpx_t<double> { using type = double2; };
+template <typename T> using cpx = typename cpx_t<T>::type;
+
+constexpr int kMaxStages = 12;
+
+// Factorisation of one FFT length into Stockham stages.
+struct AxisPlan {
+    int n;
+    int nst;
+    int rad[kMaxStages];
+    int rmin;
+};
+
+// Launch shape of one axis pass (tunable).
+struct PassShape {
+    int tile;     // pencils (columns pass) or rows (rows pass) per workgroup
+    int threads;  // workgroup size
+    int lds;      // dynamic LDS bytes
+};
+
+// Point-wise prologue applied to real inputs of the R2C rows pass.
+enum PrologueOp {
+    OP_IDENTITY = 0,
+    OP_SQRT_RHOM = 1,    // sqrt(rho > 0 ? rho : 1e-30)                vw.py:34-41
+    OP_POW_BETA = 2,     // max(rho, 1e-30)^beta                         wt.py:15-19
+    OP_DSQRT = 3,        // drho / (2 sqrt(rhom))      (linear response of sqrt)
+    OP_DPOW_BETA = 4,    // beta * rhoc^(beta-1) * drho (linear response of rho^beta)
+};
+
+struct Prologue {
+    int code[4];
+    double beta;
+    const void *aux;  // second input for the linear-response ops (rho)
+};
+
+// One plan: grid geometry, FFT plans, workspace and functional settings.
+struct Plan {
+    int nr[3];
+    int nzc;
+    long long N, Nc, Nfull;
+    double lat[9];
+    double B[9];  // reciprocal lattice rows b_i (grid.py:405-414)
+    double volume, dV;
+    int prec;   // OFDFT_FP64 / OFDFT_FP32
+    int full;   // full-grid (fftfreq on all axes) semantics
+    int max_batch;
+
+    AxisPlan ax[3];
+    PassShape rows_r2c, rows_c2r, rows_c2c, cols_y, cols_x, cols_yc, cols_xc;
+    void *tw[3];  // device twiddle tables, exp(-2 pi i m / n), plan precision
+
+    // workspace
+    void *spec;   // max(4, max_batch) * Nc complex   (also full-grid c2c scratch)
+    void *work;   // 4 * N real
+    double *red;  // reduction partials
+    int red_blocks;
+    double *scal;  // device scalars
+    void *aux;     // optimizer / propagator fields (allocated lazily at first use? no: at create)
+    size_t aux_bytes;
+    size_t ws_bytes;
+
+    // functional settings
+    double x = 1.0, y = 1.0, alpha = 5.0 / 6.0, beta = 5.0 / 6.0, rho0 = -1.0, sigma = -1.0;
+    unsigned terms = 63;
+    const void *vloc = nullptr;
+    double eion = 0.0;
+    bool rho0_cached_valid = false;
+};
+
+// fft.hip
+void plan_fft_setup(Plan &P);
+void plan_fft_free(Plan &P);
+template <typename T>
+void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
+             double scale, const Prologue *pro, double *sum_partials, hipStream_t s);
+template <typename T>
+void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long long in_bs,
+             long long tmp_bs, long long out_bs, double scale, hipStream_t s);
+template <typename T>
+void fft_c2c(Plan &P, const cpx<T> *in, cpx<T> *out, bool inverse, int batch, long long bs, double scale,
+             hipStream_t s);
+// the x/y passes alone (in place allowed), for fused pipelines
+template <typename T>
+void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in, cpx<T> *out, int batch,
+              long long bs, double scale, hipStream_t s);
+
+// reductions (vec.hip)
+int reduce_blocks(long long n);
+template <typename T>
+void dot(Plan &P, const T *a, const T *b, const T *c, long long n, double *out, double scale, hipStream_t s);
+void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
+                  hipStream_t s);
+
+// eval.hip
+template <typename T>
+void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s);
+
+}  // namespace ofdft

@@ -1,0 +1,525 @@
+// Density optimizer (Optimization.optimize_rho, optimization.py:281-539) run
+// natively: direct minimisation over phi = sqrt(rho) with norm-conserving
+// rotations phi cos(theta) + p sin(theta), a truncated-Newton (or CG)
+// direction and the Moré-Thuente line search on theta in [0, pi].
+//
+// Field updates are fused HIP kernels with FP64 partial sums; the scalar
+// control flow (TN inner CG, line search, convergence test) runs on the host
+// on FP64 scalars fetched once per step.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dcsrch.h"
+#include "internal.h"
+#include "reduce.h"
+
+namespace ofdft {
+
+enum AuxField {
+    F_PHI, F_RES, F_P, F_DIR, F_BEST, F_TRES, F_TP, F_PHI1, F_RHO1, F_POT1, F_AP, F_RES_PREV, F_DIR_PREV,
+    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_COUNT
+};
+
+void plan_aux_setup(Plan &P) {
+    const size_t rs = P.prec == OFDFT_FP64 ? 8 : 4;
+    P.aux_bytes = (size_t)F_COUNT * P.N * rs;
+    OFDFT_HIP(hipMalloc(&P.aux, P.aux_bytes));
+}
+
+void plan_aux_free(Plan &P) {
+    if (P.aux) (void)hipFree(P.aux);
+    P.aux = nullptr;
+}
+
+template <typename T> static T *fld(Plan &P, int f) { return reinterpret_cast<T *>(P.aux) + (size_t)f * P.N; }
+
+template <typename T> __global__ void k_fill(T *y, T v, long long n);
+
+// ------------------------------------------------------------------ kernels
+#define GRID_LOOP(i, n) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+template <int K> __device__ __forceinline__ void write_partials(double *acc, double *partials) {
+    __shared__ double sm[16];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const double s = block_sum(acc[t], sm);
+        if (threadIdx.x == 0) partials[(long long)blockIdx.x * K + t] = s;
+    }
+}
+
+// phi = rho > 0 ? sqrt(rho) : 1e-300 ; sums rho and phi^2 (optimization.py:305-315)
+template <typename T> __global__ void k_init_phi(const T *rho, T *phi, long long n, double *partials) {
+    double acc[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        const T r = rho[i];
+        const T f = r > (T)0 ? sqrt(r) : (T)1e-300;
+        phi[i] = f;
+        acc[0] += (double)r;
+        acc[1] += (double)f * (double)f;
+    }
+    write_partials<2>(acc, partials);
+}
+
+// phi *= f ; rho = phi^2
+template <typename T> __global__ void k_scale_square(T *phi, T *rho, double f, long long n) {
+    GRID_LOOP(i, n) {
+        const T v = (T)((double)phi[i] * f);
+        phi[i] = v;
+        rho[i] = v * v;
+    }
+}
+
+// sums  V*rho  and  rho  (mu = int V rho / N, optimization.py:537-539)
+template <typename T> __global__ void k_mu_sums(const T *pot, const T *rho, long long n, double *partials) {
+    double acc[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        acc[0] += (double)pot[i] * (double)rho[i];
+        acc[1] += (double)rho[i];
+    }
+    write_partials<2>(acc, partials);
+}
+
+// res = (V - mu) phi ; sum res^2
+template <typename T>
+__global__ void k_residual(const T *pot, const T *phi, T *res, const double *mu, long long n, double *partials) {
+    const double m = *mu;
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        const T r = (T)(((double)pot[i] - m) * (double)phi[i]);
+        res[i] = r;
+        acc[0] += (double)r * (double)r;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// TN start: tres = -res, tp = tres, dir = 0 ; sum tres^2
+template <typename T>
+__global__ void k_tn_start(const T *res, T *tres, T *tp, T *dir, long long n, double *partials) {
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        const T r = -res[i];
+        tres[i] = r;
+        tp[i] = r;
+        dir[i] = (T)0;
+        acc[0] += (double)r * (double)r;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// phi1 = phi + eps p ; rho1 = phi1^2
+template <typename T>
+__global__ void k_tn_shift(const T *phi, const T *p, double eps, T *phi1, T *rho1, long long n) {
+    GRID_LOOP(i, n) {
+        const T v = (T)((double)phi[i] + eps * (double)p[i]);
+        phi1[i] = v;
+        rho1[i] = v * v;
+    }
+}
+
+// Ap = ((V1 - mu) phi1 - res0) / eps ; sum p.Ap   (optimization.py:105-114)
+template <typename T>
+__global__ void k_tn_ap(const T *pot1, const T *phi1, const T *res0, const T *p, const double *mu, double eps, T *Ap,
+                        long long n, double *partials) {
+    const double m = *mu;
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        const double a = (((double)pot1[i] - m) * (double)phi1[i] - (double)res0[i]) / eps;
+        Ap[i] = (T)a;
+        acc[0] += (double)p[i] * (double)(T)a;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// dir += alpha p ; tres -= alpha Ap ; sum tres^2
+template <typename T>
+__global__ void k_tn_update(T *dir, T *tres, const T *p, const T *Ap, double alpha, long long n, double *partials) {
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        dir[i] = (T)((double)dir[i] + alpha * (double)p[i]);
+        const T r = (T)((double)tres[i] - alpha * (double)Ap[i]);
+        tres[i] = r;
+        acc[0] += (double)r * (double)r;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// y = a x + b y
+template <typename T> __global__ void k_axpby(double a, const T *x, double b, T *y, long long n) {
+    GRID_LOOP(i, n) y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+template <typename T> __global__ void k_copy(const T *x, T *y, long long n) {
+    GRID_LOOP(i, n) y[i] = x[i];
+}
+
+template <typename T> __global__ void k_fill(T *y, T v, long long n) {
+    GRID_LOOP(i, n) y[i] = v;
+}
+
+// sums p*phi and phi*phi
+template <typename T> __global__ void k_pphi(const T *p, const T *phi, long long n, double *partials) {
+    double acc[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        acc[0] += (double)p[i] * (double)phi[i];
+        acc[1] += (double)phi[i] * (double)phi[i];
+    }
+    write_partials<2>(acc, partials);
+}
+
+// p -= f phi ; sum p^2
+template <typename T> __global__ void k_orth(T *p, const T *phi, const double *f, long long n, double *partials) {
+    const double ff = *f;
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        const T v = (T)((double)p[i] - ff * (double)phi[i]);
+        p[i] = v;
+        acc[0] += (double)v * (double)v;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// sum V phi p
+template <typename T>
+__global__ void k_dot3(const T *a, const T *b, const T *c, long long n, double *partials) {
+    double acc[1] = {0};
+    GRID_LOOP(i, n) acc[0] += (double)a[i] * (double)b[i] * (double)c[i];
+    write_partials<1>(acc, partials);
+}
+
+// newphi = phi cos + p sin ; newrho = newphi^2   (optimization.py:236-238)
+template <typename T>
+__global__ void k_rotate(const T *phi, const T *p, double c, double s, T *nphi, T *nrho, long long n) {
+    GRID_LOOP(i, n) {
+        const T v = (T)((double)phi[i] * c + (double)p[i] * s);
+        nphi[i] = v;
+        nrho[i] = v * v;
+    }
+}
+
+// sum V * newphi * (p cos - phi sin)   (optimization.py:270-276)
+template <typename T>
+__global__ void k_theta_grad(const T *pot, const T *nphi, const T *p, const T *phi, double c, double s, long long n,
+                             double *partials) {
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        const T p2 = (T)((double)p[i] * c - (double)phi[i] * s);
+        acc[0] += (double)pot[i] * (double)nphi[i] * (double)p2;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// p = -res (steepest descent)
+template <typename T> __global__ void k_neg(const T *x, T *y, long long n) {
+    GRID_LOOP(i, n) y[i] = -x[i];
+}
+
+// CG directions (math_utils.py:395-423): sums r1(r1 - r0), d(r1 - r0), r1^2, r0^2
+template <typename T>
+__global__ void k_cg_sums(const T *r1, const T *r0, const T *d, long long n, double *partials) {
+    double acc[4] = {0, 0, 0, 0};
+    GRID_LOOP(i, n) {
+        const double a = r1[i], b = r0[i], dd = d[i];
+        acc[0] += a * (a - b);
+        acc[1] += dd * (a - b);
+        acc[2] += a * a;
+        acc[3] += b * b;
+    }
+    write_partials<4>(acc, partials);
+}
+
+#undef GRID_LOOP
+
+// ------------------------------------------------------------------ host helpers
+struct Ctx {
+    Plan &P;
+    hipStream_t s;
+    int nb;
+    double *host;  // pinned
+    Ctx(Plan &p, hipStream_t st) : P(p), s(st) {
+        nb = std::min(reduce_blocks(P.N), 1024);
+        OFDFT_HIP(hipHostMalloc((void **)&host, 64 * sizeof(double), hipHostMallocDefault));
+    }
+    ~Ctx() { (void)hipHostFree(host); }
+    dim3 grid() const { return dim3(nb); }
+    // reduce K partial columns into scal[slot..slot+K) and copy them to the host
+    const double *fetch(int K, int slot = 16) {
+        reduce_final_cols(K, slot);
+        OFDFT_HIP(hipMemcpyAsync(host, P.scal + slot, K * sizeof(double), hipMemcpyDeviceToHost, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+        return host;
+    }
+    void reduce_final_cols(int K, int slot) {
+        // partials laid out [block][K]
+        std::vector<double> ones(K, 1.0);
+        reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s);
+    }
+    void set_scalar(int slot, double v) {
+        host[32] = v;
+        OFDFT_HIP(hipMemcpyAsync(P.scal + slot, host + 32, sizeof(double), hipMemcpyHostToDevice, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+    }
+};
+
+template <typename T> struct Opt {
+    Plan &P;
+    Ctx C;
+    const long long n;
+    int n_ev = 0, n_v = 0;
+    double energies_h[OFDFT_E_COUNT];
+    double *energies_d;
+    int map[F_COUNT];
+    Opt(Plan &p, hipStream_t s) : P(p), C(p, s), n(p.N) {
+        energies_d = P.scal + 32;
+        for (int i = 0; i < F_COUNT; ++i) map[i] = i;
+    }
+
+    // fields are addressed through a permutation so accepting a trial point
+    // swaps pointers instead of copying
+    T *F(int f) { return fld<T>(P, map[f]); }
+
+    double eval_ev(const T *rho, T *pot) {
+        eval<T>(P, rho, OFDFT_CALC_E | OFDFT_CALC_V, pot, energies_d, C.s);
+        OFDFT_HIP(hipMemcpyAsync(energies_h, energies_d, sizeof(energies_h), hipMemcpyDeviceToHost, C.s));
+        OFDFT_HIP(hipStreamSynchronize(C.s));
+        ++n_ev;
+        return energies_h[OFDFT_E_SUM];
+    }
+    void eval_v(const T *rho, T *pot) {
+        eval<T>(P, rho, OFDFT_CALC_V, pot, nullptr, C.s);
+        ++n_v;
+    }
+    // mu = sum(V rho) / sum(rho), stored on device at scal[2]
+    double chemical_potential(const T *pot, const T *rho) {
+        hipLaunchKernelGGL((k_mu_sums<T>), C.grid(), dim3(256), 0, C.s, pot, rho, n, P.red);
+        const double *h = C.fetch(2);
+        const double mu = h[0] / h[1];
+        C.set_scalar(2, mu);
+        return mu;
+    }
+    double residual(const T *pot, const T *phi, T *res) {
+        hipLaunchKernelGGL((k_residual<T>), C.grid(), dim3(256), 0, C.s, pot, phi, res, P.scal + 2, n, P.red);
+        return C.fetch(1)[0] * P.dV;
+    }
+
+    // get_direction_TN (optimization.py:91-147); returns number of V evaluations
+    int direction_tn(int maxfun, double eps) {
+        T *dir = F(F_DIR), *best = F(F_BEST), *tres = F(F_TRES), *tp = F(F_TP);
+        T *phi1 = F(F_PHI1), *rho1 = F(F_RHO1), *pot1 = F(F_POT1), *Ap = F(F_AP);
+        hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
+        double r0 = C.fetch(1)[0];
+        const double rconv = r0 * 0.1;
+        bool best_zero = true;
+        std::vector<double> rl{1e10};
+        int it = 0;
+        for (it = 0; it < maxfun; ++it) {
+            hipLaunchKernelGGL((k_tn_shift<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, eps, phi1, rho1, n);
+            eval_v(rho1, pot1);
+            hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps, Ap,
+                               n, P.red);
+            const double pAp = C.fetch(1)[0];
+            if (pAp < 0.0) {
+                if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, r0 / pAp, tp, 0.0, dir, n);
+                break;
+            }
+            const double alpha = r0 / pAp;
+            hipLaunchKernelGGL((k_tn_update<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, alpha, n, P.red);
+            const double r1 = C.fetch(1)[0];
+            double rmin = rl[0];
+            for (double v : rl) rmin = std::min(rmin, v);
+            if (r1 < rmin) {
+                hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, dir, best, n);
+                best_zero = false;
+            }
+            rl.push_back(r1);
+            if (r1 < rconv) break;
+            double rmin_prev = rl[0];
+            for (size_t i = 0; i + 1 < rl.size(); ++i) rmin_prev = std::min(rmin_prev, rl[i]);
+            if (r1 > 1000.0 * rmin_prev || (it > 10 && std::fabs(r0 - r1) < 0.1 * r0)) {
+                // fall back to the best direction seen
+                if (best_zero)
+                    hipLaunchKernelGGL((k_fill<T>), C.grid(), dim3(256), 0, C.s, dir, (T)0, n);
+                else
+                    hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, best, dir, n);
+                break;
+            }
+            const double beta = r1 / r0;
+            r0 = r1;
+            hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, 1.0, tres, beta, tp, n);
+        }
+        return std::min(it + 1, maxfun);
+    }
+
+    // OrthogonalNormalization (optimization.py:204-225) of F_P against phi;
+    // returns theta0.
+    double orthonormalize() {
+        T *p = F(F_P), *phi = F(F_PHI);
+        hipLaunchKernelGGL((k_pphi<T>), C.grid(), dim3(256), 0, C.s, p, phi, n, P.red);
+        const double *h = C.fetch(2);
+        const double Ne = h[1] * P.dV;
+        const double f = h[0] * P.dV / Ne;
+        C.set_scalar(3, f);
+        hipLaunchKernelGGL((k_orth<T>), C.grid(), dim3(256), 0, C.s, p, phi, P.scal + 3, n, P.red);
+        const double pnorm = C.fetch(1)[0] * P.dV;
+        const double theta0 = std::sqrt(pnorm / Ne);
+        hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, 0.0, p, std::sqrt(Ne / pnorm), p, n);
+        return theta0;
+    }
+
+    // value and derivative at theta (ValueAndDerivative, optimization.py:227-279)
+    void value_deriv(double th, double &e, double &g) {
+        const double c = std::cos(th), s = std::sin(th);
+        hipLaunchKernelGGL((k_rotate<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_P), c, s, F(F_NPHI), F(F_NRHO), n);
+        e = eval_ev(F(F_NRHO), F(F_NPOT));
+        hipLaunchKernelGGL((k_theta_grad<T>), C.grid(), dim3(256), 0, C.s, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s,
+                           n, P.red);
+        g = 2.0 * C.fetch(1)[0] * P.dV;
+    }
+
+    // b takes a's contents (a becomes scratch)
+    void swap_fields(int a, int b) { std::swap(map[a], map[b]); }
+
+    int run(T *rho_io, T *pot_out, int method, int hvp, const double *opts, double *stats, double *history,
+            int hcap) {
+        const double econv = opts[0];
+        const int maxiter = (int)opts[1], maxfun = (int)opts[2], maxls = (int)opts[3];
+        const double c1 = opts[4], c2 = opts[5], xtol = opts[6];
+        const int ncheck = (int)opts[7];
+        (void)hvp;
+        const auto t0 = std::chrono::steady_clock::now();
+        // initial phi (optimization.py:305-316)
+        hipLaunchKernelGGL((k_init_phi<T>), C.grid(), dim3(256), 0, C.s, rho_io, F(F_PHI), n, P.red);
+        const double *h = C.fetch(2);
+        const double factor = std::sqrt(h[0] / h[1]);
+        hipLaunchKernelGGL((k_scale_square<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_RHO), factor, n);
+        double energy = eval_ev(F(F_RHO), F(F_POT));
+        double mu = chemical_potential(F(F_POT), F(F_RHO));
+        residual(F(F_POT), F(F_PHI), F(F_RES));
+        std::vector<double> hist{energy};
+        double theta = 0.1;
+        int converged = 2, it = 0, nres = 0;
+        for (it = 1; it < maxiter; ++it) {
+            int nd = 1;
+            if (method == 0) {
+                nd = direction_tn(maxfun, 1.0e-9);
+                hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR), F(F_P), n);
+            } else {
+                // CG (math_utils.py:395-423); first step and missing history: -res
+                if (nres < 1) {
+                    hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
+                } else {
+                    hipLaunchKernelGGL((k_cg_sums<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_RES_PREV),
+                                       F(F_DIR_PREV), n, P.red);
+                    const double *q = C.fetch(4);
+                    double beta;
+                    if (method == 1)
+                        beta = q[0] / q[1];
+                    else if (method == 2)
+                        beta = std::max(q[0] / q[3], 0.0);
+                    else
+                        beta = q[2] / q[3];
+                    hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR_PREV), F(F_P), n);
+                    hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, -1.0, F(F_RES), beta, F(F_P), n);
+                }
+            }
+            double theta0 = orthonormalize();
+            hipLaunchKernelGGL((k_dot3<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), F(F_PHI), F(F_P), n, P.red);
+            const double d0 = C.fetch(1)[0] * 2.0;
+            if (d0 > 0) {
+                hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
+                theta0 = orthonormalize();
+            }
+            theta = std::min(theta0, theta);
+            // line search (math_utils.py:76-106) with func0 at theta = 0
+            hipLaunchKernelGGL((k_theta_grad<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), F(F_PHI), F(F_P), F(F_PHI), 1.0,
+                               0.0, n, P.red);
+            double f = energy, g = 2.0 * C.fetch(1)[0] * P.dV;
+            double st[16];
+            LsTask task = LS_START;
+            double stp = theta;
+            bool ok = false, evaluated = false;
+            for (int i = 0; i < maxls; ++i) {
+                task = dcsrch(stp, f, g, c1, c2, xtol, 0.0, M_PI, task, st);
+                if (task == LS_FG) {
+                    value_deriv(stp, f, g);
+                    evaluated = true;
+                } else {
+                    ok = (task == LS_CONVERGENCE);
+                    break;
+                }
+            }
+            if (!ok || !evaluated) {
+                converged = 1;
+                break;
+            }
+            theta = stp;
+            // accept: phi <- newphi, rho <- newrho, pot <- newpot
+            swap_fields(F_NPHI, F_PHI);
+            swap_fields(F_NRHO, F_RHO);
+            swap_fields(F_NPOT, F_POT);
+            energy = f;
+            if (method != 0) {
+                swap_fields(F_RES, F_RES_PREV);
+                swap_fields(F_P, F_DIR_PREV);
+                ++nres;
+            }
+            mu = chemical_potential(F(F_POT), F(F_RHO));
+            residual(F(F_POT), F(F_PHI), F(F_RES));
+            hist.push_back(energy);
+            (void)nd;
+            if ((int)hist.size() - 1 >= ncheck) {
+                bool conv = true;
+                for (int i = 0; i < ncheck; ++i)
+                    if (std::fabs(hist[hist.size() - 2 - i] - energy) > econv) conv = false;
+                if (conv) {
+                    converged = 0;
+                    break;
+                }
+            }
+        }
+        hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_RHO), rho_io, n);
+        if (pot_out) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), pot_out, n);
+        OFDFT_HIP(hipStreamSynchronize(C.s));
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) {
+            for (int i = 0; i < 16; ++i) stats[i] = 0.0;
+            stats[0] = converged;
+            stats[1] = (double)(hist.size() - 1);
+            stats[2] = energy;
+            stats[3] = mu;
+            stats[4] = n_v;
+            stats[5] = n_ev;
+            stats[6] = secs;
+        }
+        if (history)
+            for (int i = 0; i < hcap && i < (int)hist.size(); ++i) history[i] = hist[i];
+        return converged;
+    }
+};
+
+}  // namespace ofdft
+
+using namespace ofdft;
+
+extern "C" int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method, int hvp, const double *opts,
+                              double *stats, double *history, int hcap, void *stream) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    try {
+        OFDFT_REQUIRE(P && rho && opts, "optimize: bad arguments");
+        OFDFT_REQUIRE(method >= 0 && method <= 3, "optimize: unknown method");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (P->prec == OFDFT_FP64) {
+            Opt<double> o(*P, s);
+            o.run((double *)rho, (double *)pot, method, hvp, opts, stats, history, hcap);
+        } else {
+            Opt<float> o(*P, s);
+            o.run((float *)rho, (float *)pot, method, hvp, opts, stats, history, hcap);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}

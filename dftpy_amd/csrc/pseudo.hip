@@ -1,0 +1,526 @@
+// Local pseudopotential and Ewald ion-ion energy on the grid (setup, FP64).
+//
+//  * vloc(|G|) from a per-species piecewise-cubic table (the interpolating
+//    cubic spline of pseudo/__init__.py:641-658) with the quartic patch
+//    below the first table step and zero beyond the table
+//    (pseudo/__init__.py:350-375).
+//  * exact structure factor exp(-i G.R) (ions.py:121-124,
+//    pseudo/__init__.py:377-387) or order-n cardinal B-spline PME
+//    (ewald.py:11-194, pseudo/__init__.py:389-404).
+//  * Ewald: real-space erfc sum over periodic images (ewald.py:280-320),
+//    reciprocal sum on the G grid with the half-space mask
+//    (grid.py:657-695; ewald.py:448-465, 786-808), correction
+//    (ewald.py:467-482).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "reduce.h"
+
+namespace ofdft {
+
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxOrder = 16;
+
+struct VlocTable {
+    const double *coef[kMaxSpecies];
+    double q0[kMaxSpecies], dq[kMaxSpecies], qmax[kMaxSpecies];
+    int ncoef[kMaxSpecies];
+    double v0[kMaxSpecies], v1[kMaxSpecies], v2[kMaxSpecies], dqs[kMaxSpecies];
+};
+
+__device__ __forceinline__ double vloc_eval(const VlocTable &t, int s, double q) {
+    if (q < t.dqs[s]) {
+        // quartic through (v2, v1, v0, v1, v2) (math_utils.py:457-467), odd terms vanish
+        const double f0 = t.v2[s], f1 = t.v1[s], f2 = t.v0[s];
+        const double t2 = -f0 + 16.0 * f1 - 30.0 * f2 + 16.0 * f1 - f0;
+        const double t4 = f0 - 4.0 * f1 + 6.0 * f2 - 4.0 * f1 + f0;
+        const double dx = q / t.dqs[s];
+        return f2 + dx * (dx * (t2 + dx * (dx * t4))) / 24.0;
+    }
+    if (!(q < t.qmax[s])) return 0.0;
+    int i = (int)floor((q - t.q0[s]) / t.dq[s]);
+    if (i < 0) i = 0;
+    if (i >= t.ncoef[s]) i = t.ncoef[s] - 1;
+    const double x = q - (t.q0[s] + i * t.dq[s]);
+    const double *c = t.coef[s] + 4 * (long long)i;
+    return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+}
+
+__device__ __forceinline__ int ffreq_(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
+__device__ __forceinline__ int wrapf_(int f, int n) { return ((n & 1) == 0 && f == n / 2) ? -n / 2 : f; }
+
+struct GGeom {
+    double B[9];
+    int nx, ny, nz, nzc, full;
+};
+
+__device__ __forceinline__ void gvec(const GGeom &g, int fx, int fy, int fz, double *G) {
+    G[0] = fx * g.B[0] + fy * g.B[3] + fz * g.B[6];
+    G[1] = fx * g.B[1] + fy * g.B[4] + fz * g.B[7];
+    G[2] = fx * g.B[2] + fy * g.B[5] + fz * g.B[8];
+}
+
+// integer frequencies of half-grid point (ix,iy,iz) in the grid's convention
+// and, on a full grid at a Nyquist point, those of the alias partner.
+__device__ __forceinline__ bool freqs_of(const GGeom &g, int ix, int iy, int iz, int *f, int *fp) {
+    f[0] = ffreq_(ix, g.nx);
+    f[1] = ffreq_(iy, g.ny);
+    f[2] = iz;
+    if (!g.full) return false;
+    f[2] = wrapf_(iz, g.nz);
+    const bool nyq = ((g.nx & 1) == 0 && ix == g.nx / 2) || ((g.ny & 1) == 0 && iy == g.ny / 2) ||
+                     ((g.nz & 1) == 0 && iz == g.nz / 2);
+    if (!nyq) return false;
+    fp[0] = wrapf_(-f[0], g.nx);
+    fp[1] = wrapf_(-f[1], g.ny);
+    fp[2] = wrapf_(-f[2], g.nz);
+    return true;
+}
+
+// v(G) = sum_a vloc_s(a)(|G|) exp(-i G.R_a) on the half grid, scaled.
+__global__ void k_vloc_exact(double2 *__restrict__ out, long long nc, GGeom g, VlocTable t, int nat,
+                             const double *__restrict__ pos, const int *__restrict__ spec, double scale) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % g.nzc);
+        const long long rest = i / g.nzc;
+        const int iy = (int)(rest % g.ny), ix = (int)(rest / g.ny);
+        int f[3], fp[3];
+        const bool two = freqs_of(g, ix, iy, iz, f, fp);
+        double G1[3], G2[3];
+        gvec(g, f[0], f[1], f[2], G1);
+        double vs1[kMaxSpecies], vs2[kMaxSpecies];
+        const double q1 = sqrt(G1[0] * G1[0] + G1[1] * G1[1] + G1[2] * G1[2]);
+        for (int s = 0; s < kMaxSpecies; ++s) vs1[s] = t.coef[s] ? vloc_eval(t, s, q1) : 0.0;
+        if (two) {
+            // partner enters as f(-G_partner)
+            gvec(g, fp[0], fp[1], fp[2], G2);
+            G2[0] = -G2[0];
+            G2[1] = -G2[1];
+            G2[2] = -G2[2];
+            const double q2 = sqrt(G2[0] * G2[0] + G2[1] * G2[1] + G2[2] * G2[2]);
+            for (int s = 0; s < kMaxSpecies; ++s) vs2[s] = t.coef[s] ? vloc_eval(t, s, q2) : 0.0;
+        }
+        double re = 0.0, im = 0.0;
+        for (int a = 0; a < nat; ++a) {
+            const double *R = pos + 3 * a;
+            double sn, cs;
+            sincos(G1[0] * R[0] + G1[1] * R[1] + G1[2] * R[2], &sn, &cs);
+            double w = two ? 0.5 : 1.0;
+            re += w * vs1[spec[a]] * cs;
+            im -= w * vs1[spec[a]] * sn;
+            if (two) {
+                sincos(G2[0] * R[0] + G2[1] * R[1] + G2[2] * R[2], &sn, &cs);
+                re += 0.5 * vs2[spec[a]] * cs;
+                im -= 0.5 * vs2[spec[a]] * sn;
+            }
+        }
+        out[i] = double2{re * scale, im * scale};
+    }
+}
+
+// Cardinal B-spline values M_n(x + k), k = 0..order-1 -> m[k] (ewald.py:66-86)
+__device__ __forceinline__ void calc_mn(double x, int order, double *m /* order+1 */) {
+    for (int i = 0; i <= order; ++i) m[i] = 0.0;
+    m[1] = x;
+    m[2] = 1.0 - x;
+    for (int i = 3; i <= order; ++i)
+        for (int j = 0; j < i; ++j) {
+            const int n = i - j;
+            m[n] = ((x + n - 1) * m[n] + (j + 1 - x) * m[n - 1]) / (i - 1);
+        }
+}
+
+// Q[floor(u)+1-i] += w M(frac(u)+i) per axis, one workgroup per atom.
+__global__ void k_pme_spread(double *__restrict__ Q, int nx, int ny, int nz, int order, int nat,
+                             const double *__restrict__ frac, const double *__restrict__ w, const int *__restrict__ spec,
+                             int want_species) {
+    const int a = blockIdx.x;
+    if (a >= nat) return;
+    if (want_species >= 0 && spec[a] != want_species) return;
+    __shared__ double mx[kMaxOrder + 1], my[kMaxOrder + 1], mz[kMaxOrder + 1];
+    __shared__ int fl[3];
+    const int nr[3] = {nx, ny, nz};
+    if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        const double u = frac[3 * a + d] * nr[d];
+        const double f = floor(u);
+        fl[d] = (int)f;
+        double m[kMaxOrder + 1];
+        calc_mn(u - f, order, m);
+        for (int k = 0; k <= order; ++k) (d == 0 ? mx : (d == 1 ? my : mz))[k] = m[k];
+    }
+    __syncthreads();
+    const double wa = w ? w[a] : 1.0;
+    const int o3 = order * order * order;
+    for (int e = threadIdx.x; e < o3; e += blockDim.x) {
+        const int i = e / (order * order), j = (e / order) % order, k = e % order;
+        int lx = (1 + fl[0] - i) % nx;
+        if (lx < 0) lx += nx;
+        int ly = (1 + fl[1] - j) % ny;
+        if (ly < 0) ly += ny;
+        int lz = (1 + fl[2] - k) % nz;
+        if (lz < 0) lz += nz;
+        atomicAdd(&Q[((long long)lx * ny + ly) * nz + lz], wa * mx[i + 1] * my[j + 1] * mz[k + 1]);
+    }
+}
+
+struct BTab {
+    const double2 *bx, *by, *bz;
+};
+
+// acc(G) += vloc_s(|G|)_eff * S_s(G) * B(G)  (S_s = rfftn of the species Q array)
+__global__ void k_vloc_pme_acc(double2 *__restrict__ acc, const double2 *__restrict__ sq, long long nc, GGeom g,
+                               VlocTable t, int s, BTab b, int first) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % g.nzc);
+        const long long rest = i / g.nzc;
+        const int iy = (int)(rest % g.ny), ix = (int)(rest / g.ny);
+        int f[3], fp[3];
+        const bool two = freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        double v = vloc_eval(t, s, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]));
+        if (two) {
+            gvec(g, fp[0], fp[1], fp[2], G);
+            v = 0.5 * (v + vloc_eval(t, s, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2])));
+        }
+        const double2 bx = b.bx[ix], by = b.by[iy], bz = b.bz[iz];
+        double2 bb{bx.x * by.x - bx.y * by.y, bx.x * by.y + bx.y * by.x};
+        bb = double2{bb.x * bz.x - bb.y * bz.y, bb.x * bz.y + bb.y * bz.x};
+        const double2 q = sq[i];
+        const double2 val{v * (q.x * bb.x - q.y * bb.y), v * (q.x * bb.y + q.y * bb.x)};
+        if (first)
+            acc[i] = val;
+        else
+            acc[i] = double2{acc[i].x + val.x, acc[i].y + val.y};
+    }
+}
+
+// half-space deduplication mask of grid.py:657-695 (serial form)
+__device__ __forceinline__ bool gmask(int ix, int iy, int iz, int nx, int ny, int nz, int full) {
+    const int D0 = nx / 2, D1 = ny / 2, D2 = nz / 2;
+    const bool e0 = (nx & 1) == 0, e1 = (ny & 1) == 0, e2 = (nz & 1) == 0;
+    if (full && iz > D2) return false;
+    if (ix == 0 && iy > D1 && iz == 0) return false;
+    if (ix > D0 && iz == 0) return false;
+    if (e2) {
+        if (ix == 0 && iy == 0 && iz == D2) return false;
+        if (ix == 0 && iy > D1 && iz == D2) return false;
+        if (ix > D0 && iz == D2) return false;
+        if (e1 && ix == 0 && iy == D1 && iz == D2) return false;
+        if (e0 && ix == D0 && iy == 0 && iz == D2) return false;
+        if (e0 && ix == D0 && iy > D1 && iz == D2) return false;
+    }
+    if (e0) {
+        if (ix == D0 && iy > D1 && iz == 0) return false;
+        if (e1 && ix == D0 && iy == D1 && iz == 0) return false;
+    }
+    if (e1 && ix == 0 && iy == D1 && iz == 0) return false;
+    if (e0 && e1 && e2 && ix == D0 && iy == D1 && iz == D2) return false;
+    return true;
+}
+
+// sum over masked G != 0 of |S(G)|^2 exp(-G^2/(4 eta)) / G^2 with S either
+// given (PME: rfftn(Q) * B) or computed from the charges (exact).
+__global__ void k_ewald_rec(const double2 *__restrict__ sq, long long nc, GGeom g, double eta, BTab b, int nat,
+                            const double *__restrict__ pos, const double *__restrict__ q, double *partials) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % g.nzc);
+        const long long rest = i / g.nzc;
+        const int iy = (int)(rest % g.ny), ix = (int)(rest / g.ny);
+        if (!gmask(ix, iy, iz, g.nx, g.ny, g.nz, g.full)) continue;
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        int f[3], fp[3];
+        freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        const double gg = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        double s2;
+        if (sq) {
+            const double2 bx = b.bx[ix], by = b.by[iy], bz = b.bz[iz];
+            double2 bb{bx.x * by.x - bx.y * by.y, bx.x * by.y + bx.y * by.x};
+            bb = double2{bb.x * bz.x - bb.y * bz.y, bb.x * bz.y + bb.y * bz.x};
+            const double2 v = sq[i];
+            const double re = v.x * bb.x - v.y * bb.y, im = v.x * bb.y + v.y * bb.x;
+            s2 = re * re + im * im;
+        } else {
+            double re = 0.0, im = 0.0;
+            for (int a = 0; a < nat; ++a) {
+                double sn, cs;
+                sincos(G[0] * pos[3 * a] + G[1] * pos[3 * a + 1] + G[2] * pos[3 * a + 2], &sn, &cs);
+                re += q[a] * cs;
+                im -= q[a] * sn;
+            }
+            s2 = re * re + im * im;
+        }
+        acc += s2 * exp(-gg / (4.0 * eta)) / gg;
+    }
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// real-space sum over atom pairs and images: q_i q_j erfc(sqrt(eta) d) / d
+__global__ void k_ewald_real(int nat, const double *__restrict__ pos, const double *__restrict__ q, double L00,
+                             double L01, double L02, double L10, double L11, double L12, double L20, double L21,
+                             double L22, int N0, int N1, int N2, double rmax, double seta, double *partials) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    const long long npair = (long long)nat * nat;
+    for (long long pidx = (long long)blockIdx.x * blockDim.x + threadIdx.x; pidx < npair;
+         pidx += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(pidx / nat), j = (int)(pidx % nat);
+        const double dx0 = pos[3 * i] - pos[3 * j], dy0 = pos[3 * i + 1] - pos[3 * j + 1],
+                     dz0 = pos[3 * i + 2] - pos[3 * j + 2];
+        double s = 0.0;
+        for (int a = -N0; a <= N0; ++a)
+            for (int bb = -N1; bb <= N1; ++bb)
+                for (int c = -N2; c <= N2; ++c) {
+                    const double rx = a * L00 + bb * L10 + c * L20;
+                    const double ry = a * L01 + bb * L11 + c * L21;
+                    const double rz = a * L02 + bb * L12 + c * L22;
+                    const double dx = dx0 + rx, dy = dy0 + ry, dz = dz0 + rz;
+                    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (d < rmax && d > 0.001) s += erfc(seta * d) / d;
+                }
+        acc += q[i] * q[j] * s;
+    }
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// ------------------------------------------------------------------ host
+static GGeom geom_of(const Plan &P) {
+    GGeom g;
+    for (int i = 0; i < 9; ++i) g.B[i] = P.B[i];
+    g.nx = P.nr[0];
+    g.ny = P.nr[1];
+    g.nz = P.nr[2];
+    g.nzc = P.nzc;
+    g.full = P.full;
+    return g;
+}
+
+static void host_calc_mn(double x, int order, double *m) {
+    for (int i = 0; i <= order; ++i) m[i] = 0.0;
+    m[1] = x;
+    m[2] = 1.0 - x;
+    for (int i = 3; i <= order; ++i)
+        for (int j = 0; j < i; ++j) {
+            const int n = i - j;
+            m[n] = ((x + n - 1) * m[n] + (j + 1 - x) * m[n - 1]) / (i - 1);
+        }
+}
+
+// Euler exponential spline factors b(m) (ewald.py:88-104), per axis
+static std::vector<double> bspline_bm(int n, int len, int order) {
+    double M1[kMaxOrder + 1];
+    host_calc_mn(1.0, order, M1);
+    std::vector<double> out(2 * len);
+    for (int m = 0; m < len; ++m) {
+        const double q = 2.0 * M_PI * m / n;
+        double fr = 0.0, fi = 0.0;
+        for (int k = 1; k < order; ++k) {
+            fr += M1[k] * std::cos(k * q);
+            fi -= M1[k] * std::sin(k * q);
+        }
+        const double tr = std::cos((order - 1.0) * q), ti = -std::sin((order - 1.0) * q);
+        const double den = fr * fr + fi * fi;
+        out[2 * m] = (tr * fr + ti * fi) / den;
+        out[2 * m + 1] = (ti * fr - tr * fi) / den;
+    }
+    return out;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    DevBuf(size_t bytes) { OFDFT_HIP(hipMalloc(&p, bytes ? bytes : 8)); }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename X> X *as() { return reinterpret_cast<X *>(p); }
+};
+
+struct BTabHost {
+    DevBuf bx, by, bz;
+    BTab tab;
+    BTabHost(const Plan &P, int order)
+        : bx(sizeof(double2) * P.nr[0]), by(sizeof(double2) * P.nr[1]), bz(sizeof(double2) * P.nzc) {
+        auto x = bspline_bm(P.nr[0], P.nr[0], order);
+        auto y = bspline_bm(P.nr[1], P.nr[1], order);
+        auto z = bspline_bm(P.nr[2], P.nzc, order);
+        OFDFT_HIP(hipMemcpy(bx.p, x.data(), x.size() * 8, hipMemcpyHostToDevice));
+        OFDFT_HIP(hipMemcpy(by.p, y.data(), y.size() * 8, hipMemcpyHostToDevice));
+        OFDFT_HIP(hipMemcpy(bz.p, z.data(), z.size() * 8, hipMemcpyHostToDevice));
+        tab.bx = bx.as<double2>();
+        tab.by = by.as<double2>();
+        tab.bz = bz.as<double2>();
+    }
+};
+
+static int nblk(long long n) {
+    long long b = (n + 255) / 256;
+    return (int)std::max(1LL, std::min(b, 4096LL));
+}
+
+void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
+              const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+              const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, double *vreal,
+              hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "local_pp needs an FP64 plan");
+    OFDFT_REQUIRE(nspecies >= 1 && nspecies <= kMaxSpecies, "1..8 species supported");
+    OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
+    VlocTable t;
+    std::memset(&t, 0, sizeof(t));
+    std::vector<DevBuf *> coefbufs;
+    for (int sp = 0; sp < nspecies; ++sp) {
+        DevBuf *b = new DevBuf(sizeof(double) * 4 * ncoef[sp]);
+        coefbufs.push_back(b);
+        OFDFT_HIP(hipMemcpy(b->p, coef[sp], sizeof(double) * 4 * ncoef[sp], hipMemcpyHostToDevice));
+        t.coef[sp] = b->as<double>();
+        t.q0[sp] = q0[sp];
+        t.dq[sp] = dq[sp];
+        t.qmax[sp] = qmax[sp];
+        t.ncoef[sp] = ncoef[sp];
+        t.v0[sp] = vsmall[3 * sp];
+        t.v1[sp] = vsmall[3 * sp + 1];
+        t.v2[sp] = vsmall[3 * sp + 2];
+        t.dqs[sp] = dqs[sp];
+    }
+    DevBuf dpos(sizeof(double) * 3 * nat), dfrac(sizeof(double) * 3 * nat), dspec(sizeof(int) * nat);
+    OFDFT_HIP(hipMemcpy(dpos.p, pos_cart, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
+    OFDFT_HIP(hipMemcpy(dfrac.p, pos_frac, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
+    OFDFT_HIP(hipMemcpy(dspec.p, species, sizeof(int) * nat, hipMemcpyHostToDevice));
+    const GGeom g = geom_of(P);
+    double2 *spec = reinterpret_cast<double2 *>(P.spec);  // slot 0: accumulated v(G), slot 1: species S(G)
+    double *Q = reinterpret_cast<double *>(P.work);
+    try {
+        if (!pme) {
+            // vreal = irfftn(v) / dV
+            hipLaunchKernelGGL(k_vloc_exact, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, nat,
+                               dpos.as<double>(), dspec.as<int>(), 1.0);
+            OFDFT_HIP(hipGetLastError());
+        } else {
+            BTabHost bt(P, order);
+            for (int sp = 0; sp < nspecies; ++sp) {
+                OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
+                hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order,
+                                   nat, dfrac.as<double>(), nullptr, dspec.as<int>(), sp);
+                OFDFT_HIP(hipGetLastError());
+                fft_r2c<double>(P, Q, spec + P.Nc, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
+                // v = sum_s vloc_s * rfftn(Q_s) * dV * B * N / V = sum_s vloc_s rfftn(Q_s) B
+                hipLaunchKernelGGL(k_vloc_pme_acc, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc, g,
+                                   t, sp, bt.tab, sp == 0 ? 1 : 0);
+                OFDFT_HIP(hipGetLastError());
+            }
+            OFDFT_HIP(hipStreamSynchronize(s));
+        }
+        fft_c2r<double>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
+        OFDFT_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        for (auto *b : coefbufs) delete b;
+        throw;
+    }
+    for (auto *b : coefbufs) delete b;
+}
+
+static double erfcinv_host(double y) {
+    // Newton iterations on erfc(x) = y from a log-based start (y small)
+    double x = std::sqrt(-std::log(y / 2.0));
+    for (int it = 0; it < 100; ++it) {
+        const double f = std::erfc(x) - y;
+        const double df = -2.0 / std::sqrt(M_PI) * std::exp(-x * x);
+        const double dx = f / df;
+        x -= dx;
+        if (std::fabs(dx) < 1e-15 * std::fabs(x)) break;
+    }
+    return x;
+}
+
+void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges, int pme,
+           int order, double precision, double *out, hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "ewald needs an FP64 plan");
+    // gmax = sqrt(2 ecut), ecut from the grid spacing (grid.py:456-459)
+    double lengths[3], sp_max = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        lengths[d] = std::sqrt(P.lat[3 * d] * P.lat[3 * d] + P.lat[3 * d + 1] * P.lat[3 * d + 1] +
+                               P.lat[3 * d + 2] * P.lat[3 * d + 2]);
+        const double sp1 = lengths[d] / P.nr[d], sp2 = lengths[d] / (P.nr[d] - 1);
+        sp_max = std::max(sp_max, 0.5 * (sp1 + sp2));
+    }
+    const double ecut = M_PI * M_PI / (2.0 * sp_max * sp_max);
+    const double gmax = std::sqrt(2.0 * ecut);
+    double q2 = 0.0, qsum = 0.0;
+    for (int a = 0; a < nat; ++a) {
+        q2 += charges[a] * charges[a];
+        qsum += charges[a];
+    }
+    double eta = 1.6;
+    while (true) {
+        const double ub = 4.0 * M_PI * nat * q2 * std::sqrt(eta / M_PI) * std::erfc(gmax / 2.0 * std::sqrt(1.0 / eta));
+        if (ub < precision) break;
+        eta -= 0.01;
+        OFDFT_REQUIRE(eta > 0.0, "ewald: no eta satisfies the precision");
+    }
+    const double e_corr = -std::sqrt(eta / M_PI) * q2 - 4.0 * M_PI * (1.0 / (4.0 * eta * P.volume) / 2.0) * qsum * qsum;
+    // real space
+    const double rmax = erfcinv_host(precision / 3.0) / std::sqrt(eta);
+    int Nimg[3];
+    for (int d = 0; d < 3; ++d) Nimg[d] = (int)std::ceil(rmax / lengths[d]);
+    DevBuf dpos(sizeof(double) * 3 * nat), dfrac(sizeof(double) * 3 * nat), dq(sizeof(double) * nat);
+    OFDFT_HIP(hipMemcpy(dpos.p, pos_cart, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
+    OFDFT_HIP(hipMemcpy(dfrac.p, pos_frac, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
+    OFDFT_HIP(hipMemcpy(dq.p, charges, sizeof(double) * nat, hipMemcpyHostToDevice));
+    double *res = P.scal + 8;
+    const long long npair = (long long)nat * nat;
+    const int nbr = nblk(npair);
+    const double *L = P.lat;
+    hipLaunchKernelGGL(k_ewald_real, dim3(nbr), dim3(256), 0, s, nat, dpos.as<double>(), dq.as<double>(), L[0], L[1],
+                       L[2], L[3], L[4], L[5], L[6], L[7], L[8], Nimg[0], Nimg[1], Nimg[2], rmax, std::sqrt(eta),
+                       P.red);
+    OFDFT_HIP(hipGetLastError());
+    reduce_final(P.red, nbr, 1, res, nullptr, s);
+    // reciprocal space
+    const GGeom g = geom_of(P);
+    const int nbg = std::min(nblk(P.Nc), kRedBlocks);
+    double2 *spec = reinterpret_cast<double2 *>(P.spec);
+    if (pme) {
+        BTabHost bt(P, order);
+        double *Q = reinterpret_cast<double *>(P.work);
+        OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
+        hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
+                           dfrac.as<double>(), dq.as<double>(), nullptr, -1);
+        OFDFT_HIP(hipGetLastError());
+        fft_r2c<double>(P, Q, spec, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
+        hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, spec, P.Nc, g, eta, bt.tab, nat, nullptr,
+                           nullptr, P.red);
+        OFDFT_HIP(hipGetLastError());
+        const double c = 4.0 * M_PI / P.volume;
+        reduce_final(P.red, nbg, 1, res + 1, &c, s);
+        OFDFT_HIP(hipStreamSynchronize(s));
+    } else {
+        BTab none{nullptr, nullptr, nullptr};
+        hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, nullptr, P.Nc, g, eta, none, nat,
+                           dpos.as<double>(), dq.as<double>(), P.red);
+        OFDFT_HIP(hipGetLastError());
+        const double c = 4.0 * M_PI / P.volume;
+        reduce_final(P.red, nbg, 1, res + 1, &c, s);
+    }
+    double h[2];
+    OFDFT_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+    OFDFT_HIP(hipStreamSynchronize(s));
+    const double e_real = h[0] / 2.0, e_rec = h[1];
+    out[0] = e_corr + e_real + e_rec;
+    out[1] = e_corr;
+    out[2] = e_real;
+    out[3] = e_rec;
+    out[4] = eta;
+}
+
+}  // namespace ofdft

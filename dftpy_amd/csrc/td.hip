@@ -1,0 +1,228 @@
+// TD-OFDFT Crank-Nicolson step (td/propagator/crank_nicholson.py:44-56,
+// 147-188): solve (1 + i H dt/2) psi1 = (1 - i H dt/2) psi0 with the
+// complex-symmetric CG of linear_solver.py:24-47 (sums without conjugation,
+// norms with it), H psi = -1/2 lap_sigma psi + v psi (td/hamiltonian.py:70-114)
+// where lap_sigma = IFFT(-G^2 exp(-G^2 sigma^2 / 2) FFT) on the full complex
+// grid (field.py:343-350).  Inner products accumulate in FP64.
+#include <cmath>
+
+#include "internal.h"
+#include "reduce.h"
+
+namespace ofdft {
+
+#define GRID_LOOP(i, n) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+template <int K> __device__ __forceinline__ void td_partials(double *acc, double *partials) {
+    __shared__ double sm[16];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const double s = block_sum(acc[t], sm);
+        if (threadIdx.x == 0) partials[(long long)blockIdx.x * K + t] = s;
+    }
+}
+
+__device__ __forceinline__ int ffreq_td(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
+
+// spec *= 1/2 G^2 exp(-G^2 sigma^2 / 2) / N   (full grid, fftfreq on all axes)
+template <typename T>
+__global__ void k_td_gmul(cpx<T> *spec, long long n, int nx, int ny, int nz, double B0, double B1, double B2,
+                          double B3, double B4, double B5, double B6, double B7, double B8, double sigma,
+                          double scale) {
+    GRID_LOOP(i, n) {
+        const int iz = (int)(i % nz);
+        const long long r = i / nz;
+        const int iy = (int)(r % ny), ix = (int)(r / ny);
+        const int fx = ffreq_td(ix, nx), fy = ffreq_td(iy, ny), fz = ffreq_td(iz, nz);
+        const double gx = fx * B0 + fy * B3 + fz * B6;
+        const double gy = fx * B1 + fy * B4 + fz * B7;
+        const double gz = fx * B2 + fy * B5 + fz * B8;
+        const double gg = gx * gx + gy * gy + gz * gz;
+        const double f = 0.5 * gg * exp(-gg * sigma * sigma / 2.0) * scale;
+        cpx<T> v = spec[i];
+        spec[i] = cpx<T>{(T)(v.x * f), (T)(v.y * f)};
+    }
+}
+
+// out = x + sgn * i (dt/2) (kin + v x)   with kin = -1/2 lap x already in `kin`
+// sums: (mode 1) p.out complex (no conj)   (mode 2) |out|^2
+template <typename T, int MODE>
+__global__ void k_td_apply(const cpx<T> *x, const cpx<T> *kin, const T *v, double hdt, cpx<T> *out, long long n,
+                           double *partials) {
+    double acc[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        const cpx<T> a = x[i], k = kin[i];
+        const double vv = (double)v[i];
+        const double hr = (double)k.x + vv * (double)a.x, hi = (double)k.y + vv * (double)a.y;
+        // i * hdt * (hr + i hi) = -hdt hi + i hdt hr
+        const double orr = (double)a.x - hdt * hi, oi = (double)a.y + hdt * hr;
+        out[i] = cpx<T>{(T)orr, (T)oi};
+        if (MODE == 1) {
+            acc[0] += (double)a.x * orr - (double)a.y * oi;
+            acc[1] += (double)a.x * oi + (double)a.y * orr;
+        } else if (MODE == 2) {
+            acc[0] += orr * orr + oi * oi;
+        }
+    }
+    if (MODE == 1) td_partials<2>(acc, partials);
+    if (MODE == 2) td_partials<1>(acc, partials);
+}
+
+// res = b - res (res holds A x0) ; p = res ; sums res.res (complex), |res|^2
+template <typename T>
+__global__ void k_td_init(const cpx<T> *b, cpx<T> *res, cpx<T> *p, long long n, double *partials) {
+    double acc[3] = {0, 0, 0};
+    GRID_LOOP(i, n) {
+        const cpx<T> bb = b[i], ax = res[i];
+        const cpx<T> r{bb.x - ax.x, bb.y - ax.y};
+        res[i] = r;
+        p[i] = r;
+        acc[0] += (double)r.x * r.x - (double)r.y * r.y;
+        acc[1] += 2.0 * (double)r.x * r.y;
+        acc[2] += (double)r.x * r.x + (double)r.y * r.y;
+    }
+    td_partials<3>(acc, partials);
+}
+
+// x += a p ; res -= a v ; sums res.res (complex), |res|^2
+template <typename T>
+__global__ void k_td_update(cpx<T> *x, cpx<T> *res, const cpx<T> *p, const cpx<T> *v, double ar, double ai, long long n,
+                            double *partials) {
+    double acc[3] = {0, 0, 0};
+    GRID_LOOP(i, n) {
+        const cpx<T> pp = p[i], vv = v[i], xx = x[i], rr = res[i];
+        x[i] = cpx<T>{(T)(xx.x + ar * pp.x - ai * pp.y), (T)(xx.y + ar * pp.y + ai * pp.x)};
+        const double r0 = rr.x - (ar * vv.x - ai * vv.y), r1 = rr.y - (ar * vv.y + ai * vv.x);
+        res[i] = cpx<T>{(T)r0, (T)r1};
+        acc[0] += r0 * r0 - r1 * r1;
+        acc[1] += 2.0 * r0 * r1;
+        acc[2] += r0 * r0 + r1 * r1;
+    }
+    td_partials<3>(acc, partials);
+}
+
+// p = res + beta p
+template <typename T> __global__ void k_td_pupd(const cpx<T> *res, cpx<T> *p, double br, double bi, long long n) {
+    GRID_LOOP(i, n) {
+        const cpx<T> pp = p[i], rr = res[i];
+        p[i] = cpx<T>{(T)(rr.x + br * pp.x - bi * pp.y), (T)(rr.y + br * pp.y + bi * pp.x)};
+    }
+}
+
+template <typename T> __global__ void k_td_copy(const cpx<T> *a, cpx<T> *b, long long n) {
+    GRID_LOOP(i, n) b[i] = a[i];
+}
+
+#undef GRID_LOOP
+
+template <typename T> struct CN {
+    Plan &P;
+    hipStream_t s;
+    long long n;
+    int nb;
+    double *host;
+    double sigma;
+    cpx<T> *kin, *tmp;
+    CN(Plan &p, hipStream_t st, double sg) : P(p), s(st), n(p.N), sigma(sg) {
+        nb = std::min(reduce_blocks(n), 1024);
+        OFDFT_HIP(hipHostMalloc((void **)&host, 16 * sizeof(double), hipHostMallocDefault));
+        kin = reinterpret_cast<cpx<T> *>(P.spec);
+    }
+    ~CN() { (void)hipHostFree(host); }
+    const double *fetch(int K) {
+        std::vector<double> ones(K, 1.0);
+        reduce_final(P.red, nb, K, P.scal + 64, ones.data(), s);
+        OFDFT_HIP(hipMemcpyAsync(host, P.scal + 64, K * sizeof(double), hipMemcpyDeviceToHost, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+        return host;
+    }
+    // kin = -1/2 lap_sigma(x)
+    void kinetic(const cpx<T> *x) {
+        fft_c2c<T>(P, x, kin, false, 1, n, 1.0, s);
+        hipLaunchKernelGGL((k_td_gmul<T>), dim3(std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s, kin, n,
+                           P.nr[0], P.nr[1], P.nr[2], P.B[0], P.B[1], P.B[2], P.B[3], P.B[4], P.B[5], P.B[6], P.B[7],
+                           P.B[8], sigma, 1.0 / (double)n);
+        OFDFT_HIP(hipGetLastError());
+        fft_c2c<T>(P, kin, kin, true, 1, n, 1.0, s);
+    }
+    template <int MODE> void apply(const cpx<T> *x, const T *v, double hdt, cpx<T> *out) {
+        kinetic(x);
+        hipLaunchKernelGGL((k_td_apply<T, MODE>), dim3(nb), dim3(256), 0, s, x, kin, v, hdt, out, n, P.red);
+        OFDFT_HIP(hipGetLastError());
+    }
+
+    void step(const cpx<T> *psi0, const T *v, double dt, double tol, int maxiter, cpx<T> *x, double *info) {
+        cpx<T> *base = reinterpret_cast<cpx<T> *>(P.aux);
+        cpx<T> *b = base, *res = base + n, *p = base + 2 * n, *Ap = base + 3 * n;
+        // b = psi0 - i dt/2 H psi0, ||b||
+        apply<2>(psi0, v, -0.5 * dt, b);
+        const double bn = std::sqrt(fetch(1)[0]);
+        const double atol = bn == 0.0 ? tol : tol * bn;
+        // x = psi0 ; res = b - A x
+        if (x != psi0) hipLaunchKernelGGL((k_td_copy<T>), dim3(nb), dim3(256), 0, s, psi0, x, n);
+        apply<0>(x, v, 0.5 * dt, res);
+        hipLaunchKernelGGL((k_td_init<T>), dim3(nb), dim3(256), 0, s, b, res, p, n, P.red);
+        const double *h = fetch(3);
+        double rr_r = h[0], rr_i = h[1];
+        double rnorm = std::sqrt(h[2]);
+        int status = 1, k = 0;
+        if (rnorm < atol) {
+            status = 0;
+        } else {
+            while (k < maxiter) {
+                apply<1>(p, v, 0.5 * dt, Ap);
+                h = fetch(2);
+                const double pv_r = h[0], pv_i = h[1];
+                const double den = pv_r * pv_r + pv_i * pv_i;
+                const double ar = (rr_r * pv_r + rr_i * pv_i) / den, ai = (rr_i * pv_r - rr_r * pv_i) / den;
+                hipLaunchKernelGGL((k_td_update<T>), dim3(nb), dim3(256), 0, s, x, res, p, Ap, ar, ai, n, P.red);
+                h = fetch(3);
+                const double nr_r = h[0], nr_i = h[1];
+                rnorm = std::sqrt(h[2]);
+                ++k;
+                if (rnorm < atol) {
+                    status = 0;
+                    break;
+                }
+                const double d2 = rr_r * rr_r + rr_i * rr_i;
+                const double br = (nr_r * rr_r + nr_i * rr_i) / d2, bi = (nr_i * rr_r - nr_r * rr_i) / d2;
+                hipLaunchKernelGGL((k_td_pupd<T>), dim3(nb), dim3(256), 0, s, res, p, br, bi, n);
+                rr_r = nr_r;
+                rr_i = nr_i;
+            }
+        }
+        OFDFT_HIP(hipStreamSynchronize(s));
+        if (info) {
+            info[0] = status;
+            info[1] = k;
+            info[2] = rnorm;
+            info[3] = atol;
+        }
+    }
+};
+
+}  // namespace ofdft
+
+using namespace ofdft;
+
+extern "C" int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, double sigma, double tol,
+                             int maxiter, void *psi1, double *info, void *stream) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    try {
+        OFDFT_REQUIRE(P && psi0 && v && psi1, "cn_step: bad arguments");
+        OFDFT_REQUIRE(P->aux_bytes >= (size_t)4 * P->N * (P->prec == OFDFT_FP64 ? 16 : 8), "cn_step: workspace");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (P->prec == OFDFT_FP64) {
+            CN<double> c(*P, s, sigma);
+            c.step((const double2 *)psi0, (const double *)v, dt, tol, maxiter, (double2 *)psi1, info);
+        } else {
+            CN<float> c(*P, s, sigma);
+            c.step((const float2 *)psi0, (const float *)v, dt, tol, maxiter, (float2 *)psi1, info);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,89 @@
+// FP64-accumulated reductions over grid fields (MP.einsum / MP.asum,
+// mpi/mpi.py:199-246, serial semantics) and small vector kernels used by the
+// optimizer and propagator.
+#include "internal.h"
+#include "reduce.h"
+
+namespace ofdft {
+
+int reduce_blocks(long long n) {
+    long long b = (n + 256 * 8 - 1) / (256 * 8);
+    if (b < 1) b = 1;
+    if (b > kRedBlocks) b = kRedBlocks;
+    return (int)b;
+}
+
+// partials[blockIdx.x] = sum a*b*c over a grid-stride share (b, c optional)
+template <typename T>
+__global__ void k_dot(const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c, long long n,
+                      double *partials) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        double v = (double)a[i];
+        if (b) v *= (double)b[i];
+        if (c) v *= (double)c[i];
+        acc += v;
+    }
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+struct Coefs {
+    double c[kMaxTerms];
+    double add;  // added to out[nterms] (sum of the scaled terms) when sum_out
+    int sum_out;
+};
+
+// out[t] = coef[t] * sum_b partials[b * nterms + t]; optionally out[nterms] =
+// sum_t out[t] + add.
+__global__ void k_reduce_final(const double *__restrict__ partials, int nblocks, int nterms, double *out,
+                               Coefs co) {
+    __shared__ double sm[16];
+    double tot = 0.0;
+    for (int t = 0; t < nterms; ++t) {
+        double acc = 0.0;
+        for (int bk = threadIdx.x; bk < nblocks; bk += blockDim.x) acc += partials[(long long)bk * nterms + t];
+        acc = block_sum(acc, sm);
+        if (threadIdx.x == 0) {
+            out[t] = acc * co.c[t];
+            tot += acc * co.c[t];
+        }
+    }
+    if (threadIdx.x == 0 && co.sum_out) out[nterms] = tot + co.add;
+}
+
+void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
+                  hipStream_t s) {
+    Coefs co{};
+    for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
+    co.sum_out = 0;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partials, nblocks, nterms, out, co);
+    OFDFT_HIP(hipGetLastError());
+}
+
+void reduce_final_sum(const double *partials, int nblocks, int nterms, double *out, const double *coef,
+                      double add, hipStream_t s) {
+    Coefs co{};
+    for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
+    co.sum_out = 1;
+    co.add = add;
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partials, nblocks, nterms, out, co);
+    OFDFT_HIP(hipGetLastError());
+}
+
+template <typename T>
+void dot(Plan &P, const T *a, const T *b, const T *c, long long n, double *out, double scale, hipStream_t s) {
+    const int nb = reduce_blocks(n);
+    hipLaunchKernelGGL((k_dot<T>), dim3(nb), dim3(256), 0, s, a, b, c, n, P.red);
+    OFDFT_HIP(hipGetLastError());
+    reduce_final(P.red, nb, 1, out, &scale, s);
+}
+
+template void dot<float>(Plan &, const float *, const float *, const float *, long long, double *, double,
+                         hipStream_t);
+template void dot<double>(Plan &, const double *, const double *, const double *, long long, double *, double,
+                          hipStream_t);
+
+}  // namespace ofdft

@@ -1,0 +1,257 @@
+"""Functionals with the reference's interface (functional/__init__.py:1-50,
+abstract_functional.py:12-28, functional_output.py:8-131).
+
+Every functional is a callable ``f(rho, calcType={'E','V'}) ->
+FunctionalOutput``.  All of them evaluate through one fused HIP pipeline
+(``ofdft_eval``) restricted to their own terms; a TotalFunctional runs the
+union of its members' terms in a single call.
+"""
+import copy
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..field import DirectField
+
+__all__ = ["FunctionalOutput", "AbstractFunctional", "KEDF", "XC", "Hartree", "LocalPseudo", "TotalFunctional",
+           "Functional", "ZeroFunctional"]
+
+
+class FunctionalOutput:
+    """Energy (float, Ha) and potential (DirectField) of one evaluation
+    (functional_output.py:8-108)."""
+
+    def __init__(self, name=None, energy=None, potential=None, energydensity=None, **kwargs):
+        if name is None:
+            raise AttributeError("Functional name must be specified")
+        self.name = name
+        if energy is not None:
+            self.energy = energy
+        if potential is not None:
+            self.potential = potential
+        if energydensity is not None:
+            self.energydensity = energydensity
+        self.parts = kwargs.get("parts")
+
+    def _items(self):
+        for k in ("energy", "potential", "energydensity"):
+            if hasattr(self, k):
+                yield k, getattr(self, k)
+
+    def __add__(self, other):
+        if other is None:
+            return self.copy()
+        res = FunctionalOutput(name=self.name if self.name == other.name else self.name + other.name)
+        for k, v in self._items():
+            setattr(res, k, v + getattr(other, k) if hasattr(other, k) else v)
+        for k, v in other._items():
+            if not hasattr(res, k):
+                setattr(res, k, v)
+        return res
+
+    def __sub__(self, other):
+        res = self.copy()
+        for k, v in self._items():
+            setattr(res, k, v - getattr(other, k) if hasattr(other, k) else v)
+        for k, v in other._items():
+            if not hasattr(self, k):
+                setattr(res, k, -v)
+        return res
+
+    def __mul__(self, x):
+        res = self.copy()
+        for k, v in self._items():
+            setattr(res, k, v * x)
+        return res
+
+    def __truediv__(self, x):
+        if x == 0:
+            raise ValueError("Dividing zero")
+        res = self.copy()
+        for k, v in self._items():
+            setattr(res, k, v / x)
+        return res
+
+    def copy(self):
+        res = FunctionalOutput(name=self.name)
+        for k, v in self._items():
+            setattr(res, k, v.copy() if hasattr(v, "copy") else copy.copy(v))
+        return res
+
+
+class AbstractFunctional:
+    type = ""
+    name = ""
+    terms = 0
+
+    def __call__(self, rho, *args, **kwargs):
+        return self.compute(rho, *args, **kwargs)
+
+    def kedf_options(self):
+        return None
+
+    def compute(self, rho, calcType={"E", "V"}, **kwargs):
+        return evaluate([self], rho, calcType, name=self.name)
+
+    def __repr__(self):
+        return self.name
+
+
+def _energy_of(terms, e):
+    s = 0.0
+    for bit, idx in ((_lib.TERM_TF, _lib.E_TF), (_lib.TERM_VW, _lib.E_VW), (_lib.TERM_WT, _lib.E_NL),
+                     (_lib.TERM_XC, _lib.E_XC), (_lib.TERM_HARTREE, _lib.E_H), (_lib.TERM_PP, _lib.E_PP)):
+        if terms & bit:
+            s += e[idx]
+    return s
+
+
+def configure(plan, funcs, rho_grid, eion=0.0):
+    """Push the members' settings into the plan; returns the term mask and
+    the keep-alive vloc tensor."""
+    terms = 0
+    kedf = dict(x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0=None, sigma=None)
+    vloc = None
+    for f in funcs:
+        terms |= f.terms
+        o = f.kedf_options()
+        if o:
+            kedf.update({k: v for k, v in o.items() if k in kedf})
+        if f.terms & _lib.TERM_PP:
+            vloc = f.vreal_for(rho_grid)
+    rho0 = kedf["rho0"]
+    _lib.check(_lib.lib.ofdft_set_kedf(plan.handle, float(kedf["x"]), float(kedf["y"]), float(kedf["alpha"]),
+                                       float(kedf["beta"]), float(rho0) if rho0 else -1.0,
+                                       float(kedf["sigma"]) if kedf["sigma"] else -1.0))
+    _lib.check(_lib.lib.ofdft_set_terms(plan.handle, terms))
+    _lib.check(_lib.lib.ofdft_set_vloc(plan.handle, vloc.data.data_ptr() if vloc is not None else None))
+    _lib.check(_lib.lib.ofdft_set_eion(plan.handle, float(eion)))
+    return terms, vloc
+
+
+def evaluate(funcs, rho, calcType, name="TOTAL", eion=0.0):
+    """One fused evaluation of the union of ``funcs`` on ``rho``."""
+    if not isinstance(rho, DirectField):
+        raise TypeError("rho must be a DirectField")
+    if rho.rank > 1:
+        raise NotImplementedError("spin-polarised densities are not supported")
+    calc = set(calcType)
+    if calc - {"E", "V"}:
+        raise NotImplementedError("calcType %s not supported (only 'E' and 'V')" % sorted(calc - {"E", "V"}))
+    grid = rho.grid
+    plan = grid.plan(grid.dtype)
+    terms, vloc = configure(plan, funcs, grid, eion)
+    flags = (_lib.CALC_E if "E" in calc else 0) | (_lib.CALC_V if "V" in calc else 0)
+    src = rho.data.contiguous()
+    pot = torch.empty_like(src) if "V" in calc else None
+    en = torch.empty(_lib.E_COUNT, dtype=torch.float64, device=src.device) if "E" in calc else None
+    _lib.check(_lib.lib.ofdft_eval(plan.handle, src.data_ptr(), flags, pot.data_ptr() if pot is not None else None,
+                                   en.data_ptr() if en is not None else None, _lib.stream()))
+    out = FunctionalOutput(name=name)
+    if en is not None:
+        e = en.cpu().numpy()
+        out.energy = _energy_of(terms, e) + eion
+        out.parts = dict(TF=e[_lib.E_TF], vW=e[_lib.E_VW], NL=e[_lib.E_NL], XC=e[_lib.E_XC], HARTREE=e[_lib.E_H],
+                         PSEUDO=e[_lib.E_PP], II=eion, N=e[_lib.E_NELEC])
+    if pot is not None:
+        out.potential = DirectField(grid, data=pot)
+    del vloc
+    return out
+
+
+class ZeroFunctional(AbstractFunctional):
+    name = "ZERO"
+
+    def __init__(self, *args, **kwargs):
+        pass
+
+    def compute(self, rho, calcType={"E", "V"}, **kwargs):
+        out = FunctionalOutput(name="ZERO")
+        if "E" in calcType:
+            out.energy = 0.0
+        if "V" in calcType:
+            out.potential = DirectField(rho.grid)
+        return out
+
+
+# KEDF name -> term bits (functional/kedf/__init__.py:27-69)
+KEDF_TERMS = {
+    "TF": _lib.TERM_TF, "VW": _lib.TERM_VW, "WT-NL": _lib.TERM_WT,
+    "TFVW": _lib.TERM_TF | _lib.TERM_VW, "X_TF_Y_VW": _lib.TERM_TF | _lib.TERM_VW, "XTFYVW": _lib.TERM_TF | _lib.TERM_VW,
+    "WT": _lib.TERM_TF | _lib.TERM_VW | _lib.TERM_WT, "NONE": 0,
+}
+
+
+class KEDF(AbstractFunctional):
+    """``KEDF(name='WT', **options)`` (functional/kedf/__init__.py:90-184).
+    Options: x (TF scale), y (vW scale), alpha, beta, rho0, sigma."""
+
+    def __init__(self, name="WT", kedf=None, **kwargs):
+        self.type = "KEDF"
+        self.name = (kedf or name).upper()
+        if self.name not in KEDF_TERMS:
+            raise NotImplementedError("KEDF %s is not implemented on the GPU path" % self.name)
+        self.terms = KEDF_TERMS[self.name]
+        self.options = dict(kwargs)
+        self.options["kedf"] = self.name
+        a, b = self.options.get("alpha", 5.0 / 6.0), self.options.get("beta", 5.0 / 6.0)
+        if abs(a - b) > 1e-9 and self.terms & _lib.TERM_WT:
+            raise NotImplementedError("WT with alpha != beta is not implemented")
+
+    def kedf_options(self):
+        return self.options
+
+    def compute(self, rho, calcType={"E", "V"}, split=False, **kwargs):
+        out = evaluate([self], rho, calcType, name="KEDF")
+        if split:
+            raise NotImplementedError("split KEDF output is not supported")
+        return out
+
+
+class XC(AbstractFunctional):
+    """``XC(xc='LDA')``: the reference's built-in Perdew-Zunger LDA
+    (functional/xc/__init__.py:7-74, semilocal_xc.py:252-290)."""
+
+    def __init__(self, xc=None, name=None, libxc=None, core_density=None, **kwargs):
+        xc = xc or name
+        if libxc and not xc:
+            names = [libxc] if isinstance(libxc, str) else list(libxc)
+            if sorted(n.lower() for n in names) == ["lda_c_pz", "lda_x"]:
+                xc = "LDA"
+        if xc is None or xc.upper() != "LDA":
+            raise NotImplementedError("only the PZ LDA is implemented (got %r)" % (xc or libxc))
+        if core_density is not None:
+            raise NotImplementedError("non-linear core correction is not implemented")
+        self.type = "XC"
+        self.name = "LDA"
+        self.terms = _lib.TERM_XC
+        self.options = kwargs
+
+
+class Hartree(AbstractFunctional):
+    """Hartree functional (functional/hartree.py:11-56)."""
+
+    def __init__(self, **kwargs):
+        self.type = "HARTREE"
+        self.name = "HARTREE"
+        self.terms = _lib.TERM_HARTREE
+        self.options = kwargs
+
+
+from .pseudo import LocalPseudo  # noqa: E402
+from .total_functional import TotalFunctional  # noqa: E402
+
+FunctionalTypeDict = {"KEDF": KEDF, "PSEUDO": LocalPseudo, "HARTREE": Hartree, "XC": XC}
+
+
+def Functional(type=None, optional_kwargs=None, **kwargs):
+    """Factory ``Functional(type='KEDF', name='WT')`` (functional/__init__.py:22-50)."""
+    type = type.upper()
+    if optional_kwargs is not None:
+        kwargs.update(optional_kwargs)
+    try:
+        cls = FunctionalTypeDict[type]
+    except KeyError:
+        raise AttributeError("Unknown functional type: %s" % type)
+    return cls(**kwargs)

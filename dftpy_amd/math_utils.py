@@ -1,0 +1,79 @@
+"""Grid-size helpers (math_utils.py:243-254,337-365 and
+mpi/mp_serial.py:17-90 of the reference) and the interpolating cubic spline
+used for radial tables."""
+import numpy as np
+
+
+def best_fft_size(N, max_prime=13, scale=0.99, even=True):
+    """Smallest 2^a 3^b 5^c 7^d (times at most one of 11..97 up to max_prime)
+    size >= ceil(scale*N) - 1 (FFTW-friendly sizes, ``mp_serial.py:17-90``)."""
+    one = [11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97]
+    a = int(np.ceil(np.log2(N))) + 1
+    b = int(np.ceil(np.log(N) / np.log(3))) + 1
+    c = int(np.ceil(np.log(N) / np.log(5))) + 1
+    d = int(np.ceil(np.log(N) / np.log(7))) + 1
+    lo = 1 if even else 0
+    if max_prime == 2:
+        cand = 2 ** np.arange(lo, a)
+    elif max_prime == 3:
+        e = np.mgrid[lo:a, :b].reshape(2, -1)
+        cand = 2 ** e[0] * 3 ** e[1]
+    elif max_prime == 5:
+        e = np.mgrid[lo:a, :b, :c].reshape(3, -1)
+        cand = 2 ** e[0] * 3 ** e[1] * 5 ** e[2]
+    else:
+        e = np.mgrid[lo:a, :b, :c, :d].reshape(4, -1)
+        cand = 2 ** e[0] * 3 ** e[1] * 5 ** e[2] * 7 ** e[3]
+    hi = 2 * N if N < 100 else 1.2 * N
+    base = cand[(cand > N / (max_prime + 1)) & (cand < hi)]
+    allc = [base] + [base * p for p in one if p <= max_prime]
+    allc = np.concatenate(allc)
+    return int(allc[allc > np.ceil(scale * N) - 1].min())
+
+
+def smooth_fft_size(N, even=True):
+    """Smallest 2/3/5-smooth size >= N (the sizes the HIP FFT supports)."""
+    n = int(N)
+    while True:
+        m = n
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1 and (not even or n % 2 == 0):
+            return n
+        n += 1
+
+
+def spacing2ecut(spacing):
+    return np.pi ** 2 / (2 * spacing ** 2)
+
+
+def ecut2spacing(ecut):
+    return np.sqrt(np.pi ** 2 / ecut * 0.5)
+
+
+def ecut2nr(ecut=None, lattice=None, optfft=True, spacing=None, max_prime=13, **kwargs):
+    """``math_utils.py:349-365``; the HIP FFT has radices 2,3,4,5,7,8,11,13
+    so every size best_fft_size returns for max_prime <= 13 is supported."""
+    lattice = np.asarray(lattice, dtype=np.float64).reshape(3, 3)
+    if spacing is None:
+        spacing = ecut2spacing(ecut)
+    metric = lattice @ lattice.T
+    nr = np.zeros(3, dtype=np.int64)
+    for i in range(3):
+        n = int(np.sqrt(metric[i, i]) / spacing)
+        n += n % 2
+        nr[i] = best_fft_size(n, max_prime=max_prime, **kwargs) if optfft else n
+    return nr
+
+
+def cubic_table(x, y):
+    """Interpolating cubic spline with not-a-knot ends through (x, y) on a
+    uniform grid, as per-interval polynomial coefficients
+    c[i] = (c0, c1, c2, c3) of c0 + c1 t + c2 t^2 + c3 t^3, t = q - x[i].
+    Same interpolant as FITPACK splrep(k=3, s=0), which places the interior
+    knots at x[2:-2] (the reference's pseudo/__init__.py:641-658)."""
+    from scipy.interpolate import CubicSpline
+    cs = CubicSpline(x, y, bc_type="not-a-knot")
+    c = cs.c  # (4, n-1), highest power first
+    return np.ascontiguousarray(np.stack([c[3], c[2], c[1], c[0]], axis=1))

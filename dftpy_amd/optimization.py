@@ -1,0 +1,77 @@
+"""Optimization (optimization/optimization.py:31-542 of the reference):
+direct minimisation over phi = sqrt(rho), run natively in libofdft_hip
+(``ofdft_optimize``) on the fused evaluation of a TotalFunctional."""
+import numpy as np
+import torch
+
+from . import _lib
+from .field import DirectField
+from .functional import AbstractFunctional, configure
+from .functional.total_functional import TotalFunctional
+
+METHODS = {"TN": 0, "CG-HS": 1, "CG": 1, "CG-PR": 2, "CG-FR": 3}
+
+
+class Optimization:
+    def __init__(self, optimization_method="CG-HS", optimization_options=None, EnergyEvaluator=None, guess_rho=None):
+        self.rho = guess_rho
+        self.optimization_options = dict(optimization_options or {})
+        defaults = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200, "maxls": 30,
+                    "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
+                    "ncheck": 2, "hvp": "auto"}
+        for k, v in defaults.items():
+            self.optimization_options.setdefault(k, v)
+        if EnergyEvaluator is None:
+            raise AttributeError("Must provide an energy evaluator")
+        if not isinstance(EnergyEvaluator, AbstractFunctional):
+            raise TypeError("EnergyEvaluator must be a dftpy_amd functional")
+        if optimization_method not in METHODS:
+            raise NotImplementedError("optimization method %s is not implemented" % optimization_method)
+        o = self.optimization_options
+        if o["algorithm"] != "EMM" or o["vector"] != "Orthogonalization":
+            raise NotImplementedError("only the EMM algorithm with orthogonalised rotations is implemented")
+        self.EnergyEvaluator = EnergyEvaluator
+        self.optimization_method = optimization_method
+        self.stats = None
+        self.history = None
+
+    def optimize_rho(self, guess_rho=None, guess_phi=None, lphi=False):
+        if guess_rho is None and self.rho is None:
+            raise AttributeError("Must provide a guess density")
+        rho_in = guess_rho if guess_rho is not None else self.rho
+        if guess_phi is not None:
+            rho_in = guess_phi * guess_phi
+        grid = rho_in.grid
+        plan = grid.plan(grid.dtype)
+        ev = self.EnergyEvaluator
+        funcs = list(ev.funcDict.values()) if isinstance(ev, TotalFunctional) else [ev]
+        eion = ev.eion() if isinstance(ev, TotalFunctional) else 0.0
+        _, vloc = configure(plan, funcs, grid, eion)
+        o = self.optimization_options
+        hvp = o["hvp"]
+        hvp = 0 if hvp == "auto" or hvp == "fd" else 1
+        opts = np.array([o["econv"], o["maxiter"], o["maxfun"], o["maxls"], o["c1"], o["c2"], o["xtol"], o["ncheck"]],
+                        dtype=np.float64)
+        stats = np.zeros(16)
+        hist = np.zeros(int(o["maxiter"]) + 2)
+        rho = rho_in.data.clone().contiguous()
+        pot = torch.empty_like(rho)
+        _lib.check(_lib.lib.ofdft_optimize(plan.handle, rho.data_ptr(), pot.data_ptr(),
+                                           METHODS[self.optimization_method], hvp, _lib.dptr(opts), _lib.dptr(stats),
+                                           _lib.dptr(hist), len(hist), _lib.stream()))
+        del vloc
+        n = int(stats[1]) + 1
+        self.history = hist[:n].copy()
+        self.stats = dict(converged=int(stats[0]), niter=int(stats[1]), energy=stats[2], mu=stats[3],
+                          n_eval_v=int(stats[4]), n_eval_ev=int(stats[5]), seconds=stats[6])
+        self.converged = int(stats[0])
+        self.mu = stats[3]
+        self.rho = DirectField(grid, data=rho)
+        self.phi = DirectField(grid, data=torch.sqrt(torch.clamp(rho, min=0)))
+        self.functional = type("Out", (), {})()
+        self.functional.energy = stats[2]
+        self.functional.potential = DirectField(grid, data=pot)
+        return self.rho
+
+    def __call__(self, guess_rho=None, calcType={"E", "V"}, guess_phi=None, lphi=False):
+        return self.optimize_rho(guess_rho=guess_rho, guess_phi=guess_phi, lphi=lphi)

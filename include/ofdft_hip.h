@@ -1,0 +1,161 @@
+/*
+ * ofdft_hip.h -- C ABI of libofdft_hip.so, the MI355X (gfx950) implementation of
+ * the orbital-free DFT energy/potential evaluation, density optimizer and
+ * Crank-Nicolson propagator of DFTpy (shaoxc/dftpy).
+ *
+ * Conventions shared by every entry point
+ *  - Device buffers are owned by the caller; pointers are plain device
+ *    addresses, C-contiguous, real fields (nx,ny,nz) and R2C spectra
+ *    (nx,ny,nz/2+1) with the last axis contiguous, full complex fields
+ *    (nx,ny,nz).  Real type is double (precision OFDFT_FP64) or float
+ *    (OFDFT_FP32); complex is the matching interleaved (re,im) pair.
+ *  - Scalar results are FP64 in both precisions and are written to device
+ *    memory (double*), so a call never blocks the host.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls on one plan
+ *    must be ordered on one stream; different plans are independent.
+ *  - Return value: 0 on success, nonzero on error; ofdft_last_error() gives
+ *    the message (thread-local).
+ *  - The plan owns its workspace (allocated at create, freed at destroy);
+ *    no entry point allocates device memory after plan creation.
+ *
+ * Each declaration names the reference interface it replaces, as
+ * path:line relative to the reference's src/dftpy.
+ */
+#ifndef OFDFT_HIP_H
+#define OFDFT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ofdft_plan ofdft_plan;
+
+enum { OFDFT_FP64 = 0, OFDFT_FP32 = 1 };
+
+/* calcType flags of AbstractFunctional.__call__ (functional/abstract_functional.py:22-28) */
+enum { OFDFT_CALC_E = 1, OFDFT_CALC_V = 2 };
+
+/* functional terms of a TotalFunctional (functional/total_functional.py:49-59) */
+enum {
+    OFDFT_TERM_TF = 1,      /* functional/kedf/tf.py:65-80 */
+    OFDFT_TERM_VW = 2,      /* functional/kedf/vw.py:114-131 */
+    OFDFT_TERM_WT = 4,      /* functional/kedf/wt.py:108-132 */
+    OFDFT_TERM_XC = 8,      /* functional/xc/semilocal_xc.py:252-290 (PZ LDA) */
+    OFDFT_TERM_HARTREE = 16,/* functional/hartree.py:22-45 */
+    OFDFT_TERM_PP = 32      /* functional/pseudo/__init__.py:219-234 */
+};
+
+/* energies[] layout written by ofdft_eval (Ha) */
+enum { OFDFT_E_TF = 0, OFDFT_E_VW, OFDFT_E_NL, OFDFT_E_XC, OFDFT_E_H, OFDFT_E_PP, OFDFT_E_SUM, OFDFT_E_NELEC, OFDFT_E_COUNT };
+
+/* Library version and last error message (thread-local). */
+int ofdft_version(void);
+const char *ofdft_last_error(void);
+
+/* Plan = DirectGrid + ReciprocalGrid + FFT plans + workspace.
+ * Replaces DirectGrid(lattice, nr, full) (grid.py:302-318) and
+ * get_reciprocal() (grid.py:384-415).  lattice: rows are the cell vectors
+ * in bohr.  full != 0 reproduces the reference's full-complex-grid results
+ * (fftfreq on all axes) exactly on the R2C half grid.  max_batch bounds
+ * the batch argument of the FFT calls (workspace sizing). */
+int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, int full,
+                      int max_batch, ofdft_plan **plan);
+int ofdft_plan_destroy(ofdft_plan *plan);
+/* Bytes of device workspace the plan holds. */
+size_t ofdft_plan_workspace_bytes(const ofdft_plan *plan);
+
+/* FFT with DFTpy's normalisation.
+ * r2c: DirectField.fft = rfftn * dV (field.py:373-390), batch fields.
+ * c2r: ReciprocalField.ifft = irfftn(s=nr) / dV (field.py:739-764).
+ * c2c: full complex grid, forward fftn * dV / inverse ifftn / dV
+ *      (field.py:159-172 cplx branch).  In-place allowed for c2c only. */
+int ofdft_fft_r2c(ofdft_plan *plan, const void *in, void *out, int batch, void *stream);
+int ofdft_fft_c2r(ofdft_plan *plan, const void *in, void *out, int batch, void *stream);
+int ofdft_fft_c2c(ofdft_plan *plan, const void *in, void *out, int inverse, int batch, void *stream);
+
+/* Functional configuration.
+ * KEDF options (functional/kedf/__init__.py:90-184): x scales TF, y scales
+ * vW, alpha/beta the WT exponents, rho0 <= 0 means rho0 = mean(rho)
+ * (wt.py:112-113), sigma > 0 applies vW's Gaussian filter (vw.py:45-49).
+ * The WT kernel (kernel.py:325-334) is evaluated in-kernel from |G|. */
+int ofdft_set_kedf(ofdft_plan *plan, double x, double y, double alpha, double beta,
+                   double rho0, double sigma);
+/* Active terms (OFDFT_TERM_* mask) of the TotalFunctional. */
+int ofdft_set_terms(ofdft_plan *plan, unsigned terms);
+/* Local pseudopotential in real space (LocalPseudo._vreal,
+ * pseudo/__init__.py:236-246): device array of N reals in the plan
+ * precision, kept by pointer (caller keeps it alive) or NULL. */
+int ofdft_set_vloc(ofdft_plan *plan, const void *vreal);
+/* Ion-ion (Ewald) energy added to OFDFT_E_SUM (total_functional.py:134-137). */
+int ofdft_set_eion(ofdft_plan *plan, double eion);
+
+/* One TotalFunctional evaluation (total_functional.py:115-146):
+ * energies (device, OFDFT_E_N doubles) and/or the total potential pot
+ * (device, N reals).  rho is not modified (unlike wt.py:15-26). */
+int ofdft_eval(ofdft_plan *plan, const void *rho, unsigned calc, void *pot, double *energies,
+               void *stream);
+
+/* Sum_r a(r) b(r) in FP64 (MP.einsum, mpi/mpi.py:199-206); b == NULL gives
+ * Sum_r a(r).  Not multiplied by dV. */
+int ofdft_dot(ofdft_plan *plan, const void *a, const void *b, double *out, void *stream);
+
+/* Local PP setup on the grid (LocalPseudo.local_PP, pseudo/__init__.py:236-246,
+ * _PP_Reciprocal 377-387, _PP_Reciprocal_PME 389-404).
+ * Species s (0..nspecies-1) has a radial table as a piecewise cubic on a
+ * uniform grid: q0, dq, ncoef intervals, coef[4*ncoef] (value = c0 + c1 t +
+ * c2 t^2 + c3 t^3, t = q - (q0 + i dq)), qmax (zero beyond), and the quartic
+ * patch below qsmall: vsmall[3] = (v0, v1, v2), dqs.  Host arrays.
+ * Atoms: pos_cart[3*nat] (bohr), pos_frac[3*nat], species[nat].
+ * pme != 0 uses order-`order` cardinal B-splines; else the exact
+ * structure factor.  Writes vreal (device, N reals, plan precision). */
+int ofdft_local_pp(ofdft_plan *plan, int nspecies, const double *q0, const double *dq,
+                   const int *ncoef, const double *const *coef, const double *qmax,
+                   const double *vsmall, const double *dqs, int nat, const double *pos_cart,
+                   const double *pos_frac, const int *species, int pme, int order, void *vreal,
+                   void *stream);
+
+/* Ewald energy (ewald.py:197-506): real-space erfc sum over images
+ * (Energy_real, 280-320), reciprocal sum over the plan's G grid, exact
+ * (Energy_rec, 448-465) or PME (Energy_rec_PME, 786-808), and the
+ * correction (Energy_corr, 467-482).  eta as in Get_Best_eta (256-279).
+ * out (host): [e_total, e_corr, e_real, e_rec, eta].  Synchronises stream. */
+int ofdft_ewald(ofdft_plan *plan, int nat, const double *pos_cart, const double *pos_frac,
+                const double *charges, int pme, int order, double precision, double *out,
+                void *stream);
+
+/* Density optimizer (Optimization.optimize_rho, optimization/optimization.py:281-520).
+ * method: 0 = TN (get_direction_TN :91-147), 1 = CG-HS, 2 = CG-PR, 3 = CG-FR
+ * (math_utils.py:395-423).  Line search: Moré-Thuente dcsrch on theta in
+ * [0, pi] (math_utils.py:76-106).  hvp: 0 = forward difference with
+ * eps = 1e-9 as the reference; 1 = analytic linearised potential (exact
+ * Hessian-vector product, used in FP32).  rho: device, N reals, overwritten
+ * with the optimised density; pot (device, N) receives the final potential.
+ * opts (host, 8 doubles): econv, maxiter, maxfun, maxls, c1, c2, xtol, ncheck.
+ * stats (host, 16 doubles): [converged, niter, energy, mu, n_eval_v, n_eval_ev,
+ * seconds, ...].  history (host, capacity hcap) receives energies per step. */
+int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method, int hvp, const double *opts,
+                   double *stats, double *history, int hcap, void *stream);
+
+/* Moré-Thuente line search step (MINPACK-2 dcsrch, reached through
+ * scipy in math_utils.py:6-9,76-106): reverse-communication state machine
+ * on the host.  task: 0 START, 1 FG, 2 CONVERGENCE, 3 WARNING, 4 ERROR.
+ * state: 32 doubles owned by the caller. */
+int ofdft_dcsrch(double *stp, double f, double g, double ftol, double gtol, double xtol,
+                 double stpmin, double stpmax, int *task, double *state);
+
+/* TD-OFDFT Crank-Nicolson step (td/propagator/crank_nicholson.py:147-188):
+ * solves (1 + i H dt/2) psi1 = (1 - i H dt/2) psi0 with the complex-
+ * symmetric CG of linear_solver.py:24-47, H psi = -1/2 lap_sigma(psi) + v psi
+ * (td/hamiltonian.py:70-114, field.py:343-350).  psi0/psi1: device full
+ * complex fields (nx,ny,nz); v: device real potential; inner products FP64.
+ * info (host, 4 doubles): [status, iterations, final residual, atol]. */
+int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, double sigma,
+                  double tol, int maxiter, void *psi1, double *info, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OFDFT_HIP_H */

@@ -180,7 +180,7 @@ class Grid:
         (``force_real``, ``field.py:739-764``)."""
         if self.full:
             return np.real(np.fft.ifftn(F)) / self.dV
-        return np.fft.irfftn(F, s=tuple(self.nr)) / self.dV
+        return np.fft.irfftn(F, s=tuple(self.nr), axes=(0, 1, 2)) / self.dV
 
     def fft_c(self, f):
         return np.fft.fftn(f) * self.dV

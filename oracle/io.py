@@ -177,7 +177,7 @@ def write_recpot(path, q, v, header="DFTpy"):
         for line in vv[: nl * 3].reshape(-1, 3):
             fw.write("     {0[0]:22.15E}     {0[1]:22.15E}     {0[2]:22.15E}\n".format(line))
         for x in vv[nl * 3:]:
-            fw.write("{0:22.15E}".format(x))
+            fw.write("     {0:22.15E}".format(x))
         if len(vv[nl * 3:]) > 0:
             fw.write("\n")
         fw.write("1000\n")

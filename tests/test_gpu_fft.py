@@ -1,0 +1,87 @@
+"""HIP FFTs against numpy's pocketfft with DFTpy's normalisation
+(field.py:373-390,739-764): forward * dV, inverse / dV."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(8, 8, 8), (6, 10, 15), (20, 30, 22), (36, 36, 72), (7, 11, 13), (1, 6, 10), (5, 1, 9), (32, 32, 32),
+          (27, 25, 49), (96, 96, 96), (128, 64, 56)]
+
+
+def _grid(shape, dtype, full=False):
+    from dftpy_amd import DirectGrid
+    lat = np.array([[7.1, 0.2, 0.0], [0.3, 8.0, 0.1], [0.0, 0.5, 9.3]])
+    return DirectGrid(lat, shape, full=full, dtype=dtype)
+
+
+def _tol(dtype):
+    return 1e-12 if dtype == torch.float64 else 2e-6
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_r2c_c2r(shape, dtype):
+    from dftpy_amd import DirectField
+    g = _grid(shape, dtype)
+    rng = np.random.default_rng(hash(shape) % 2 ** 32)
+    f = rng.standard_normal(shape)
+    F = DirectField(g, data=f).fft()
+    ref = np.fft.rfftn(f) * g.dV
+    got = F.data.cpu().numpy()
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) <= _tol(dtype) * np.max(np.abs(ref)) * 4
+    back = F.ifft().data.cpu().numpy()
+    assert np.max(np.abs(back - f)) <= _tol(dtype) * np.max(np.abs(f)) * 8
+    # irfftn semantics on a spectrum that is not Hermitian at k=0 / Nyquist
+    spec = np.fft.rfftn(f) * g.dV + 0.3j
+    from dftpy_amd import ReciprocalField
+    got = ReciprocalField(g.get_reciprocal(), data=torch.as_tensor(spec)).ifft().data.cpu().numpy()
+    ref = np.fft.irfftn(spec, s=shape, axes=(0, 1, 2)) / g.dV
+    assert np.max(np.abs(got - ref)) <= _tol(dtype) * np.max(np.abs(ref)) * 8
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 10, 15), (16, 26, 18), (7, 11, 13), (32, 32, 32), (64, 48, 40)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_c2c(shape, dtype):
+    from dftpy_amd import DirectField
+    g = _grid(shape, dtype, full=True)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    fld = DirectField(g, data=torch.as_tensor(f), cplx=True)
+    F = fld.fft()
+    ref = np.fft.fftn(f) * g.dV
+    got = F.data.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= _tol(dtype) * np.max(np.abs(ref)) * 4
+    back = F.ifft().data.cpu().numpy()
+    assert np.max(np.abs(back - f)) <= _tol(dtype) * np.max(np.abs(f)) * 8
+
+
+def test_gaussian_pair():
+    # examples/test/test_field.py:247-285 (a 2/3/5-smooth grid instead of 51^3)
+    from dftpy_amd import DirectField, DirectGrid
+    nr = (50, 50, 50)
+    g = DirectGrid(np.eye(3) * 10.0, nr, full=False)
+    r = g.r.cpu().numpy()
+    c = r[:, 25, 25, 25]
+    s = 0.5
+    a = r - c[:, None, None, None]
+    df = (s * np.sqrt(2 * np.pi)) ** -3 * np.exp(-0.5 * np.einsum("lijk,lijk->ijk", a, a) / s ** 2)
+    F = DirectField(g, data=df).fft()
+    gv = g.get_reciprocal().g.cpu().numpy()
+    gg = (gv ** 2).sum(0)
+    ref = np.exp(-s ** 2 * gg / 2.0) * np.exp(-1j * np.einsum("lijk,l->ijk", gv, c))
+    assert np.allclose(F.data.cpu().numpy(), ref, atol=1e-8)
+    assert np.isclose(DirectField(g, data=df).integral(), 1.0)
+
+
+def test_160_cube_fp32_fp64():
+    from dftpy_amd import DirectField
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 3e-6)):
+        g = _grid((160, 160, 160), dt)
+        rng = np.random.default_rng(11)
+        f = rng.random((160, 160, 160))
+        F = DirectField(g, data=f).fft()
+        ref = np.fft.rfftn(f) * g.dV
+        assert np.max(np.abs(F.data.cpu().numpy() - ref)) <= tol * np.max(np.abs(ref))

@@ -1,0 +1,149 @@
+"""Fused energy/potential evaluation, local PP and Ewald on the GPU against
+the oracle and the reference's goldens."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import data
+from oracle import functionals as OF
+from oracle import grid as OG
+from oracle import io as OIO
+from oracle import pseudo as OP
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def al_pp():
+    return OIO.read_qepp(data("Al_fde_rho.pp"))
+
+
+def _dg(lat, nr, full, dtype=torch.float64):
+    from dftpy_amd import DirectGrid
+    return DirectGrid(lat, nr, full=full, dtype=dtype)
+
+
+@pytest.mark.parametrize("full", [True, False])
+def test_wt_golden_and_potential(al_pp, full):
+    # examples/test/test_functional.py:23-27 (the reference reads the pp file onto a full grid)
+    from dftpy_amd import DirectField
+    from dftpy_amd.functional import Functional
+    atoms, rho, lat = al_pp
+    g = _dg(lat, rho.shape, full)
+    out = Functional(type="KEDF", name="WT")(DirectField(g, data=rho))
+    og = OG.Grid(lat, rho.shape, full=full)
+    e_ref = OF.tf(rho, og)[0] + OF.vw(rho, og)[0] + OF.wt(rho, og)[0]
+    v_ref = OF.tf(rho, og)[1] + OF.vw(rho, og)[1] + OF.wt(rho, og)[1]
+    assert np.isclose(out.energy, e_ref, rtol=1e-12, atol=0)
+    if full:
+        assert np.isclose(out.energy, 2.916818700014412, rtol=1e-11, atol=0)
+    v = out.potential.numpy()
+    assert np.max(np.abs(v - v_ref)) < 1e-10 * max(1.0, np.max(np.abs(v_ref)))
+
+
+def _c1_density(nr, lat, seed=0):
+    rng = np.random.default_rng(seed)
+    og = OG.Grid(lat, nr)
+    r = og.r
+    rho = 0.027 * (1 + 0.4 * np.cos(2 * np.pi * r[0] / lat[0, 0]) * np.cos(2 * np.pi * r[1] / lat[1, 1]))
+    rho += 0.002 * rng.random(nr)
+    return rho
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_total_terms_vs_oracle(dtype):
+    from dftpy_amd import DirectField, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    ions, nr = systems.c1_system()
+    lat = ions.cell
+    rho = _c1_density(nr, lat)
+    g = _dg(lat, nr, False, dtype)
+    pp = Functional(type="PSEUDO", grid=g, ions=ions, PP_list={"Al": systems.empty_core_recpot()})
+    tot = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                          HARTREE=Functional(type="HARTREE"), PSEUDO=pp)
+    out = tot(DirectField(g, data=rho))
+    # oracle
+    og = OG.Grid(lat, nr)
+    oat = OIO.Atoms(ions.symbols, ions.positions, ions.cell)
+    gp, vp, z = OIO.read_recpot(systems.empty_core_recpot())
+    oat.set_charges({"Al": z})
+    vreal = OP.local_pp(og, oat, {"Al": (gp, vp)}, pme=True)
+    eII = OP.ewald_energy(og, oat, pme=True)
+    e_ref, v_ref, parts = OF.total(rho, og, vreal, OF.wt_kernel(og.q, rho.mean()), ewald=eII)
+    nat = ions.nat
+    if dtype == torch.float64:
+        for k, kk in (("TF", "TF"), ("vW", "vW"), ("NL", "NL"), ("XC", "XC"), ("HARTREE", "H"), ("PSEUDO", "PP"),
+                      ("II", "II")):
+            assert np.isclose(out.parts[k], parts[kk], rtol=1e-11, atol=1e-11), k
+        assert abs(out.energy - e_ref) < 1e-9
+        assert np.max(np.abs(out.potential.numpy() - v_ref)) < 1e-9
+    else:
+        assert abs(out.energy - e_ref) / nat < 2e-5
+        assert np.max(np.abs(out.potential.numpy() - v_ref)) < 2e-4
+
+
+@pytest.mark.parametrize("pme", [True, False])
+@pytest.mark.parametrize("full", [True, False])
+def test_local_pp_golden_and_vreal(al_pp, pme, full):
+    # examples/test/test_read_pp.py:12,31
+    from dftpy_amd import DirectField, Ions
+    from dftpy_amd.functional.pseudo import LocalPseudo
+    atoms, rho, lat = al_pp
+    g = _dg(lat, rho.shape, full)
+    ions = Ions(symbols=atoms.symbols, positions=atoms.positions, cell=lat)
+    pp = LocalPseudo(grid=g, ions=ions, PP_list={"Al": data("al.lda.recpot")}, PME=pme)
+    e = pp(DirectField(g, data=rho)).energy
+    assert np.isclose(e, 1.3497046, atol=1e-4)
+    og = OG.Grid(lat, rho.shape, full=full)
+    gp, vp, _ = OIO.read_recpot(data("al.lda.recpot"))
+    v_ref = OP.local_pp(og, atoms, {"Al": (gp, vp)}, pme=pme)
+    assert np.max(np.abs(pp.vreal.numpy() - v_ref)) < 1e-10 * np.max(np.abs(v_ref))
+
+
+@pytest.mark.parametrize("pme", [True, False])
+def test_ewald_golden(al_pp, pme):
+    # examples/test/test_ewald.py:33-35
+    from dftpy_amd import Ions
+    from dftpy_amd.functional.pseudo import Ewald
+    atoms, rho, lat = al_pp
+    g = _dg(lat, rho.shape, True)
+    ions = Ions(symbols=atoms.symbols, positions=atoms.positions, cell=lat)
+    ions.set_charges({"Al": 3.0})
+    ew = Ewald(ions, g, PME=pme)
+    assert np.isclose(ew.Energy_corr(), -13.003839613564997, atol=1e-5)
+    assert np.isclose(ew.Energy_real(), 0.0, atol=1e-5)
+    assert np.isclose(ew.Energy_rec(), 7.606943756474895, atol=1e-5)
+
+
+def test_ewald_500_atoms_vs_oracle():
+    from dftpy_amd import systems
+    from dftpy_amd.functional.pseudo import Ewald
+    ions, nr = systems.c2_system()
+    ions.set_charges({"Al": 3.0})
+    lat = ions.cell
+    nr = (64, 64, 64)
+    g = _dg(lat, nr, False)
+    og = OG.Grid(lat, nr)
+    oat = OIO.Atoms(ions.symbols, ions.positions, lat)
+    oat.set_charges({"Al": 3.0})
+    eta = OP.ewald_eta(og, oat)
+    ew = Ewald(ions, g, PME=False)
+    assert np.isclose(ew.eta, eta)
+    assert np.isclose(ew.Energy_real(), OP.ewald_real(og, oat, eta), rtol=1e-10, atol=1e-9)
+    assert np.isclose(ew.Energy_rec(), OP.ewald_rec(og, oat, eta), rtol=1e-9)
+
+
+def test_rho0_explicit_and_fp32_wt(al_pp):
+    from dftpy_amd import DirectField
+    from dftpy_amd.functional import Functional
+    atoms, rho, lat = al_pp
+    og = OG.Grid(lat, rho.shape, full=False)
+    r0 = 0.03
+    e_ref = OF.wt(rho, og, rho0=r0)[0]
+    g = _dg(lat, rho.shape, False)
+    out = Functional(type="KEDF", name="WT-NL", rho0=r0)(DirectField(g, data=rho))
+    assert np.isclose(out.energy, e_ref, rtol=1e-11)
+    g32 = _dg(lat, rho.shape, False, torch.float32)
+    out32 = Functional(type="KEDF", name="WT")(DirectField(g32, data=rho))
+    e64 = OF.tf(rho, og)[0] + OF.vw(rho, og)[0] + OF.wt(rho, og)[0]
+    assert abs(out32.energy - e64) < 2e-5

@@ -1,0 +1,69 @@
+"""Crank-Nicolson TD-OFDFT on the GPU against the reference's golden
+dipole and the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import data
+from oracle import functionals as OF
+from oracle import grid as OG
+from oracle import io as OIO
+from oracle import pseudo as OP
+from oracle import td as OTD
+from oracle import units as OU
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cn_gaas_dipole():
+    # examples/test/test_propagator.py:28-81, CN branch
+    from dftpy_amd import DirectField, DirectGrid, formats
+    from dftpy_amd.constants import Units
+    from dftpy_amd.functional import Functional, TotalFunctional
+    from dftpy_amd.functional.pseudo import LocalPseudo
+    from dftpy_amd.td import Hamiltonian, Propagator, calc_rho, dipole, initial_kick
+    ions, rho_np, lat = formats.read_xsf(data("GaAs_random.xsf"))
+    rho_np = rho_np * (1.0 / Units.Bohr) ** 3
+    grid = DirectGrid(lat, rho_np.shape, full=True)
+    rho0 = DirectField(grid, data=rho_np)
+    PP = LocalPseudo(grid=grid, ions=ions, PP_list={"Ga": data("Ga_lda.oe04.recpot"),
+                                                     "As": data("As_lda.oe04.recpot")})
+    ev = TotalFunctional(KE=Functional(type="KEDF", name="TF"), XC=Functional(type="XC", name="LDA"),
+                         H=Functional(type="HARTREE"), PSEUDO=PP)
+    ham = Hamiltonian()
+    cn = Propagator(ham, 1e-3, name="crank-nicholson")
+    psi = initial_kick(1e-3, 0, DirectField(grid, data=torch.sqrt(rho0.data)))
+    ham.v = ev.compute(rho0, calcType=["V"]).potential
+    E0 = ham.energy(psi)
+    norm0 = (psi.data.abs() ** 2).sum().item()
+    for _ in range(10):
+        psi, info = cn(psi)
+        assert info == 0
+        rho = calc_rho(psi)
+        ham.v = ev.compute(rho, calcType=["V"]).potential
+    assert np.isclose(ham.energy(psi), E0, rtol=1e-3)
+    d = dipole(rho, rho0)
+    assert np.isclose(d[0], 0.011423, rtol=1e-3)
+    assert np.isclose((psi.data.abs() ** 2).sum().item(), norm0, rtol=1e-9)
+    # oracle, same steps
+    oat, orho0, olat = OIO.read_xsf(data("GaAs_random.xsf"))
+    orho0 = orho0 * (1.0 / OU.Bohr) ** 3
+    og = OG.Grid(olat, orho0.shape, full=True)
+    pp, z = {}, {}
+    for k, f in (("Ga", "Ga_lda.oe04.recpot"), ("As", "As_lda.oe04.recpot")):
+        gp, vp, zz = OIO.read_recpot(data(f))
+        pp[k], z[k] = (gp, vp), zz
+    oat.set_charges(z)
+    vreal = OP.local_pp(og, oat, pp, pme=True)
+
+    def pot(r):
+        return OF.tf(r, og)[1] + OF.lda(r, og)[1] + OF.hartree(r, og)[1] + vreal
+
+    ps = OTD.initial_kick(1e-3, 0, np.sqrt(orho0), og)
+    v = pot(orho0)
+    for _ in range(10):
+        ps, st, _ = OTD.cn_step(ps, v, og, 1e-3)
+        orho = np.real(ps * np.conj(ps))
+        v = pot(orho)
+    assert np.max(np.abs(psi.numpy() - ps)) < 1e-9
+    assert np.allclose(d, OTD.dipole(orho, orho0, og), rtol=1e-7, atol=1e-12)

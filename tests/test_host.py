@@ -1,0 +1,143 @@
+"""Host-side logic of dftpy_amd and the C ABI surface (CPU only: nothing
+here launches a GPU kernel)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.interpolate import splev, splrep
+
+from conftest import ROOT, data
+from oracle import dcsrch as D
+from oracle import grid as OG
+from oracle import io as OIO
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "ofdft_hip.h")).read()
+    return sorted(set(re.findall(r"\b(ofdft_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from dftpy_amd import _lib
+    syms = header_symbols()
+    assert len(syms) >= 19
+    for s in syms:
+        assert hasattr(_lib.lib, s), s
+        assert s in _lib.SIGNATURES, s
+    assert _lib.lib.ofdft_version() == 1
+
+
+def _cpp_line_search(fun, step0, c1=1e-4, c2=0.2, xtol=1e-12, maxiter=50):
+    from dftpy_amd import _lib
+    st = np.zeros(32)
+    stp = ctypes.c_double(step0)
+    task = ctypes.c_int(0)
+    f, g = fun(0.0)
+    trials = []
+    for _ in range(maxiter):
+        _lib.check(_lib.lib.ofdft_dcsrch(ctypes.byref(stp), f, g, c1, c2, xtol, 0.0, np.pi, ctypes.byref(task),
+                                         _lib.dptr(st)))
+        if task.value == 1:
+            trials.append(stp.value)
+            f, g = fun(stp.value)
+        else:
+            break
+    return (stp.value if task.value == 2 else None), trials
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_cpp_dcsrch_matches_oracle(case):
+    funs = [lambda a: (np.cos(a + 0.3) ** 2 - a * 0.1, -2 * np.cos(a + 0.3) * np.sin(a + 0.3) - 0.1),
+            lambda a: ((a - 1.3) ** 2, 2 * (a - 1.3)),
+            lambda a: (-np.sin(3 * a) + a ** 4, -3 * np.cos(3 * a) + 4 * a ** 3),
+            lambda a: (np.exp(-a) + 0.05 * a ** 3, -np.exp(-a) + 0.15 * a ** 2)]
+    for s0 in (0.01, 0.1, 1.0, 2.5):
+        a1, t1 = _cpp_line_search(funs[case], s0)
+        tr = []
+
+        def f(a):
+            tr.append(a)
+            return list(funs[case](a))
+
+        a2, task, _, _ = D.line_search(f, s0, list(funs[case](0.0)), c1=1e-4, c2=0.2, amax=np.pi, amin=0.0,
+                                       xtol=1e-12, maxiter=50)
+        assert (a1 is None) == (a2 is None)
+        assert np.allclose(t1, tr, rtol=1e-15, atol=0)
+
+
+def test_cpp_dcsrch_rejects_uphill_start():
+    from dftpy_amd import _lib
+    st = np.zeros(32)
+    stp = ctypes.c_double(0.1)
+    task = ctypes.c_int(0)
+    _lib.lib.ofdft_dcsrch(ctypes.byref(stp), 1.0, +0.5, 1e-4, 0.2, 1e-12, 0.0, np.pi, ctypes.byref(task),
+                          _lib.dptr(st))
+    assert task.value == 4
+
+
+def test_cubic_table_matches_fitpack():
+    from dftpy_amd.math_utils import cubic_table
+    gp, vp, _ = OIO.read_recpot(data("al.lda.recpot"))
+    c = cubic_table(gp[1:], vp[1:])
+    tck = splrep(gp[1:], vp[1:], k=3)
+    q = np.linspace(gp[1], gp[-1] * 0.999, 5001)
+    dq = (gp[-1] - gp[1]) / (len(gp) - 2)
+    i = np.clip(np.floor((q - gp[1]) / dq).astype(int), 0, c.shape[0] - 1)
+    t = q - (gp[1] + i * dq)
+    v = c[i, 0] + t * (c[i, 1] + t * (c[i, 2] + t * c[i, 3]))
+    ref = splev(q, tck)
+    assert np.max(np.abs(v - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_recpot_roundtrip_and_empty_core(tmp_path):
+    from dftpy_amd import formats, systems
+    q, v = systems.empty_core_table()
+    assert len(q) == 4001 and np.isclose(q[1], 0.01)
+    p = str(tmp_path / "ec.recpot")
+    formats.write_recpot(p, q, v)
+    q2, v2, z = formats.read_recpot(p)
+    assert z == 3
+    assert np.allclose(q2, q, rtol=1e-14) and np.allclose(v2, v, rtol=1e-14)
+    q3, v3, z3 = OIO.read_recpot(p)
+    assert np.array_equal(v3, v2) and z3 == 3
+
+
+def test_readers_match_oracle():
+    from dftpy_amd import formats
+    i1, r1, l1 = formats.read_qepp(data("Al_fde_rho.pp"))
+    i2, r2, l2 = OIO.read_qepp(data("Al_fde_rho.pp"))
+    assert np.array_equal(r1, r2) and np.array_equal(l1, l2) and np.allclose(i1.positions, i2.positions)
+    p1 = formats.read_poscar(data("GaAs_random.vasp"))
+    p2 = OIO.read_poscar(data("GaAs_random.vasp"))
+    assert np.allclose(p1.positions, p2.positions, rtol=1e-15) and p1.symbols == p2.symbols
+    x1 = formats.read_xsf(data("GaAs_random.xsf"))
+    x2 = OIO.read_xsf(data("GaAs_random.xsf"))
+    assert np.array_equal(x1[1], x2[1]) and np.allclose(x1[2], x2[2])
+
+
+@pytest.mark.parametrize("n", [7, 20, 33, 64, 77, 100, 157, 160, 250, 301])
+def test_best_fft_size_matches_oracle(n):
+    from dftpy_amd.math_utils import best_fft_size
+    assert best_fft_size(n) == OG.best_fft_size(n)
+
+
+def test_fft_sizes_are_supported_radices():
+    from dftpy_amd.math_utils import best_fft_size
+    for n in range(4, 400, 2):
+        m = best_fft_size(n)
+        for p in (2, 3, 5, 7, 11, 13):
+            while m % p == 0:
+                m //= p
+        assert m == 1
+
+
+def test_fcc_builder():
+    from dftpy_amd import systems
+    ions = systems.fcc_al((5, 5, 5), sigma=0.1, seed=7)
+    assert ions.nat == 500
+    assert np.allclose(np.diag(ions.cell), 38.0)
+    ref = systems.fcc_al((5, 5, 5))
+    d = ions.positions - ref.positions
+    assert 0.08 < d.std() < 0.12

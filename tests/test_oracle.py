@@ -1,0 +1,221 @@
+"""The CPU oracle against the reference's own golden values and against
+closed forms (CPU only)."""
+import numpy as np
+import pytest
+from scipy.optimize._dcsrch import DCSRCH
+
+from conftest import data
+from oracle import dcsrch as D
+from oracle import functionals as F
+from oracle import grid as G
+from oracle import io, optimize, pseudo, td, units
+
+
+@pytest.fixture(scope="module")
+def al_pp():
+    atoms, rho, lat = io.read_qepp(data("Al_fde_rho.pp"))
+    return atoms, rho, lat
+
+
+def test_units_codata2014():
+    # ase derives Bohr and Hartree from the CODATA-2014 fundamental constants;
+    # the results agree with the tabulated CODATA-2014 values within their
+    # standard uncertainties (0.52917721067(12) A, 27.21138602(17) eV)
+    assert abs(units.Bohr - 0.52917721067) < 1.2e-10
+    assert abs(units.Ha - 27.21138602) < 1.7e-7
+
+
+def test_wt_golden_full_grid(al_pp):
+    # examples/test/test_functional.py:23-27: the pp reader builds a full grid
+    atoms, rho, lat = al_pp
+    g = G.Grid(lat, rho.shape, full=True)
+    e = F.tf(rho, g)[0] + F.vw(rho, g)[0] + F.wt(rho, g)[0]
+    assert np.isclose(e, 2.916818700014412, rtol=1e-12, atol=0)
+
+
+def test_wt_half_grid_differs_only_at_nyquist(al_pp):
+    # the R2C convention changes the Nyquist planes of this non-orthogonal cell
+    atoms, rho, lat = al_pp
+    g = G.Grid(lat, rho.shape, full=False)
+    e = F.tf(rho, g)[0] + F.vw(rho, g)[0] + F.wt(rho, g)[0]
+    assert abs(e - 2.916818700014412) < 2e-3 and abs(e - 2.916818700014412) > 1e-6
+
+
+@pytest.mark.parametrize("pme", [True, False])
+def test_local_pp_golden(al_pp, pme):
+    # examples/test/test_read_pp.py:12,31
+    atoms, rho, lat = al_pp
+    gp, vp, z = io.read_recpot(data("al.lda.recpot"))
+    assert z == 3
+    g = G.Grid(lat, rho.shape, full=True)
+    v = pseudo.local_pp(g, atoms, {"Al": (gp, vp)}, pme=pme)
+    assert np.isclose((v * rho).sum() * g.dV, 1.3497046, atol=1e-4)
+
+
+def test_ewald_goldens(al_pp):
+    # examples/test/test_ewald.py:33-35
+    atoms, rho, lat = al_pp
+    atoms.set_charges({"Al": 3})
+    g = G.Grid(lat, rho.shape, full=True)
+    eta = pseudo.ewald_eta(g, atoms)
+    assert np.isclose(pseudo.ewald_corr(g, atoms, eta), -13.003839613564997, atol=1e-5)
+    assert np.isclose(pseudo.ewald_real(g, atoms, eta), 0.0, atol=1e-5)
+    assert np.isclose(pseudo.ewald_rec(g, atoms, eta), 7.606943756474895, atol=1e-5)
+    assert np.isclose(pseudo.ewald_rec_pme(g, atoms, eta), 7.606943756474895, atol=1e-5)
+
+
+def test_fcc_madelung():
+    # point charges Z on an fcc lattice in a neutralising background:
+    # E/atom = -alpha_WS Z^2 / (2 r_ws) ... checked through the known
+    # fcc Madelung energy per ion, -0.5 * 1.79175 * Z^2 / r_s (jellium), r_s^3 = 3 V/(4 pi N)
+    a = 7.6
+    cell = np.eye(3) * a
+    frac = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]])
+    at = io.Atoms(["Al"] * 4, frac @ cell, cell)
+    at.set_charges({"Al": 3.0})
+    g = G.Grid(cell, (32, 32, 32))
+    e = pseudo.ewald_energy(g, at, pme=False)
+    rs = (3 * a ** 3 / 4 / (4 * np.pi)) ** (1 / 3)
+    assert np.isclose(e / 4, -0.5 * 1.79174723 * 9 / rs, rtol=1e-5)
+
+
+def test_fft_conventions_and_brute_force():
+    rng = np.random.default_rng(0)
+    lat = np.array([[5.0, 0.3, 0.0], [0.2, 6.0, 0.1], [0.0, 0.4, 7.0]])
+    g = G.Grid(lat, (3, 5, 4))
+    f = rng.random((3, 5, 4))
+    F_ = g.fft(f)
+    # brute-force DFT of one coefficient
+    k = (1, 2, 1)
+    idx = np.indices(f.shape).reshape(3, -1).T
+    ph = np.exp(-2j * np.pi * (idx[:, 0] * k[0] / 3 + idx[:, 1] * k[1] / 5 + idx[:, 2] * k[2] / 4))
+    assert np.isclose(F_[k], (f.reshape(-1) * ph).sum() * g.dV)
+    assert np.allclose(g.ifft(F_), f)
+
+
+def test_lindhard_limits():
+    assert np.isclose(F.lindhard(np.array([0.0]), 1.0, 1.0)[0], 0.0)
+    assert np.isclose(F.lindhard(np.array([1.0]), 1.0, 1.0)[0], 2.0 - 1.0 - 3.0)
+    e = np.array([3.65 - 1e-9, 3.65 + 1e-9])
+    v = F.lindhard(e, 1.0, 1.0)
+    assert abs(v[0] - v[1]) < 1e-6 * abs(v[0])
+
+
+def test_lda_potential_is_derivative():
+    lat = np.eye(3) * 5
+    g = G.Grid(lat, (4, 4, 4))
+    rng = np.random.default_rng(1)
+    rho = 0.01 + 0.5 * rng.random((4, 4, 4))  # spans rs < 1 and rs >= 1
+    e0, v = F.lda(rho, g)
+    d = np.zeros_like(rho)
+    d[1, 2, 3] = 1e-6
+    ep = F.lda(rho + d, g)[0]
+    em = F.lda(rho - d, g)[0]
+    assert np.isclose((ep - em) / (2e-6 * g.dV), v[1, 2, 3], rtol=1e-6)
+
+
+def test_hartree_parseval():
+    lat = np.diag([6.0, 7.0, 8.0])
+    g = G.Grid(lat, (12, 14, 16))
+    rng = np.random.default_rng(2)
+    rho = rng.random((12, 14, 16))
+    e, v = F.hartree(rho, g)
+    rg = np.fft.fftn(rho) * g.dV
+    gg = g.ggF
+    gg[0, 0, 0] = 1
+    inv = 1 / gg
+    inv[0, 0, 0] = 0
+    e2 = 2 * np.pi / g.volume * np.sum(np.abs(rg) ** 2 * inv)
+    assert np.isclose(e, e2, rtol=1e-12)
+
+
+def test_vw_uniform_is_zero():
+    g = G.Grid(np.eye(3) * 5, (8, 8, 8))
+    e, v = F.vw(np.full((8, 8, 8), 0.03), g)
+    assert abs(e) < 1e-14 and np.abs(v).max() < 1e-12
+
+
+def _ls_sequence(fun, step0, ours=True, c1=1e-4, c2=0.2, xtol=1e-12):
+    trials = []
+
+    def f(a):
+        trials.append(a)
+        return list(fun(a))
+
+    if ours:
+        a, task, n, _ = D.line_search(f, step0, list(fun(0.0)), c1=c1, c2=c2, amax=np.pi, amin=0.0, xtol=xtol,
+                                      maxiter=50)
+    else:
+        ph = lambda a: f(a)[0]
+        dph = lambda a: fun(a)[1]
+        ls = DCSRCH(ph, dph, c1, c2, xtol, 0.0, np.pi)
+        a, _, _, task = ls(step0, phi0=fun(0.0)[0], derphi0=fun(0.0)[1], maxiter=50)
+    return a, trials
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_dcsrch_matches_scipy(case):
+    funs = [lambda a: (np.cos(a + 0.3) ** 2 - a * 0.1, -2 * np.cos(a + 0.3) * np.sin(a + 0.3) - 0.1),
+            lambda a: ((a - 1.3) ** 2, 2 * (a - 1.3)),
+            lambda a: (-np.sin(3 * a) + a ** 4, -3 * np.cos(3 * a) + 4 * a ** 3),
+            lambda a: (np.exp(-a) + 0.05 * a ** 3, -np.exp(-a) + 0.15 * a ** 2)]
+    for s0 in (0.01, 0.1, 1.0, 2.5):
+        a1, t1 = _ls_sequence(funs[case], s0, True)
+        a2, t2 = _ls_sequence(funs[case], s0, False)
+        assert (a1 is None) == (a2 is None)
+        assert np.allclose(t1, t2, rtol=1e-14, atol=0)
+
+
+def test_tn_gaas_golden():
+    # examples/test/test_density_opt.py:16-46
+    atoms = io.read_poscar(data("GaAs_random.vasp"))
+    nr = G.ecut2nr(atoms.cell, 0.4)
+    assert list(nr) == [20, 30, 22]
+    g = G.Grid(atoms.cell, nr)
+    pp, z = {}, {}
+    for k, f in (("Ga", "Ga_lda.oe04.recpot"), ("As", "As_lda.oe04.recpot")):
+        gp, vp, zz = io.read_recpot(data(f))
+        pp[k], z[k] = (gp, vp), zz
+    atoms.set_charges(z)
+    vreal = pseudo.local_pp(g, atoms, pp, pme=True)
+    eII = pseudo.ewald_energy(g, atoms, pme=True)
+    rho = np.full(tuple(nr), atoms.ncharge() / g.volume)
+    cache = {}
+
+    def ev(r, need_e):
+        r0 = r.mean()
+        if "k" not in cache or abs(cache["r0"] - r0) > 1e-6:
+            cache["k"], cache["r0"] = F.wt_kernel(g.q, r0), r0
+        e, v, _ = F.total(r, g, vreal, cache["k"], ewald=eII)
+        return e, v
+
+    res = optimize.optimize(ev, rho, g.dV, method="TN", econv=1e-6 * atoms.nat, maxiter=100, maxfun=50, maxls=30)
+    assert res["converged"] == 0
+    assert np.isclose(ev(res["rho"], True)[0], -43.3046, rtol=1e-5)
+
+
+def test_td_dipole_golden():
+    # examples/test/test_propagator.py:28-81 (CN part)
+    atoms, rho0, lat = io.read_xsf(data("GaAs_random.xsf"))
+    rho0 = rho0 * (1.0 / units.Bohr) ** 3
+    g = G.Grid(lat, rho0.shape, full=True)
+    pp, z = {}, {}
+    for k, f in (("Ga", "Ga_lda.oe04.recpot"), ("As", "As_lda.oe04.recpot")):
+        gp, vp, zz = io.read_recpot(data(f))
+        pp[k], z[k] = (gp, vp), zz
+    atoms.set_charges(z)
+    vreal = pseudo.local_pp(g, atoms, pp, pme=True)
+
+    def pot(r):
+        return F.tf(r, g)[1] + F.lda(r, g)[1] + F.hartree(r, g)[1] + vreal
+
+    psi = td.initial_kick(1e-3, 0, np.sqrt(rho0), g)
+    v = pot(rho0)
+    E0 = td.energy(psi, v, g)
+    for _ in range(10):
+        psi, st, _ = td.cn_step(psi, v, g, 1e-3)
+        assert st == 0
+        rho = np.real(psi * np.conj(psi))
+        v = pot(rho)
+    assert np.isclose(td.energy(psi, v, g), E0, rtol=1e-3)
+    assert np.isclose(td.dipole(rho, rho0, g)[0], 0.011423, rtol=1e-3)
