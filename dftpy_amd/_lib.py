@@ -34,6 +34,7 @@ SIGNATURES = {
     "ofdft_plan_create": (_I, [_IP, _DP, _I, _I, _I, ctypes.POINTER(_P)]),
     "ofdft_plan_destroy": (_I, [_P]),
     "ofdft_plan_workspace_bytes": (ctypes.c_size_t, [_P]),
+    "ofdft_plan_set_fft_kernels": (_I, [_P, _I]),
     "ofdft_fft_r2c": (_I, [_P, _P, _P, _I, _P]),
     "ofdft_fft_c2r": (_I, [_P, _P, _P, _I, _P]),
     "ofdft_fft_c2c": (_I, [_P, _P, _P, _I, _I, _P]),
@@ -43,6 +44,7 @@ SIGNATURES = {
     "ofdft_set_eion": (_I, [_P, _D]),
     "ofdft_eval": (_I, [_P, _P, ctypes.c_uint, _P, _P, _P]),
     "ofdft_dot": (_I, [_P, _P, _P, _P, _P]),
+    "ofdft_bench_pass": (_I, [_P, _I, _I, _I, _P, _DP, _P]),
     "ofdft_local_pp": (_I, [_P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP, _I, _I,
                             _P, _P]),
     "ofdft_ewald": (_I, [_P, _I, _DP, _DP, _DP, _I, _I, _D, _DP, _P]),

@@ -1,6 +1,6 @@
 """Build libofdft_hip.so in-tree with hipcc for gfx950.
 
-    python -m dftpy_amd.build        # compiles dftpy_amd/lib/libofdft_hip.so
+    python dftpy_amd/build.py        # compiles dftpy_amd/lib/libofdft_hip.so
 
 Every translation unit is compiled to an object (in parallel), then linked
 into one shared library with the C ABI of include/ofdft_hip.h.
@@ -17,7 +17,7 @@ SRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libofdft_hip.so")
 
-SOURCES = ["fft.hip", "vec.hip", "eval.hip", "pseudo.hip", "opt.hip", "td.hip", "capi.cpp", "dcsrch.cpp"]
+SOURCES = ["fft.hip", "fft_fixed.hip", "vec.hip", "eval.hip", "pseudo.hip", "opt.hip", "td.hip", "capi.cpp", "dcsrch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OFDFT_ARCH", "gfx950")

@@ -177,6 +177,13 @@ int ofdft_fft_c2c(ofdft_plan *P, const void *in, void *out, int inverse, int bat
     });
 }
 
+int ofdft_plan_set_fft_kernels(ofdft_plan *P, int fixed) {
+    return guard([&] {
+        OFDFT_REQUIRE(P, "null plan");
+        P->use_fixed = fixed != 0;
+    });
+}
+
 int ofdft_set_kedf(ofdft_plan *P, double x, double y, double alpha, double beta, double rho0, double sigma) {
     return guard([&] {
         OFDFT_REQUIRE(P, "null plan");
@@ -230,6 +237,62 @@ int ofdft_dot(ofdft_plan *P, const void *a, const void *b, double *out, void *st
             dot<double>(*P, (const double *)a, (const double *)b, nullptr, P->N, out, 1.0, S(stream));
         else
             dot<float>(*P, (const float *)a, (const float *)b, nullptr, P->N, out, 1.0, S(stream));
+    });
+}
+
+int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void *rho, double *avg_ms,
+                     void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && avg_ms && iters >= 1 && batch >= 1 && batch <= 3, "bench_pass: bad arguments");
+        OFDFT_REQUIRE(which >= 0 && which <= 4, "bench_pass: unknown pass");
+        OFDFT_REQUIRE(which != 0 || rho != nullptr, "bench_pass: the z pass needs rho");
+        hipStream_t s = S(stream);
+        hipEvent_t e0, e1;
+        OFDFT_HIP(hipEventCreate(&e0));
+        OFDFT_HIP(hipEventCreate(&e1));
+        auto run = [&]() {
+            if (P->prec == OFDFT_FP64) {
+                double2 *spec = (double2 *)P->spec;
+                if (which == 0) {
+                    Prologue pro{};
+                    pro.code[0] = OP_SQRT_RHOM;
+                    pro.code[1] = OP_POW_BETA;
+                    pro.beta = P->beta;
+                    bench_rows_r2c<double>(*P, (const double *)rho, spec, batch, &pro, s);
+                } else if (which == 1 || which == 2) {
+                    fft_cols<double>(*P, which == 1 ? 1 : 0, false, false, spec, spec, batch, P->Nc, 1.0, s);
+                } else if (which == 3) {
+                    bench_rows_c2r<double>(*P, spec, (double *)P->work, batch, s);
+                } else {
+                    fft_c2c<double>(*P, spec, spec, false, 1, P->N, 1.0, s);
+                }
+            } else {
+                float2 *spec = (float2 *)P->spec;
+                if (which == 0) {
+                    Prologue pro{};
+                    pro.code[0] = OP_SQRT_RHOM;
+                    pro.code[1] = OP_POW_BETA;
+                    pro.beta = P->beta;
+                    bench_rows_r2c<float>(*P, (const float *)rho, spec, batch, &pro, s);
+                } else if (which == 1 || which == 2) {
+                    fft_cols<float>(*P, which == 1 ? 1 : 0, false, false, spec, spec, batch, P->Nc, 1.0, s);
+                } else if (which == 3) {
+                    bench_rows_c2r<float>(*P, spec, (float *)P->work, batch, s);
+                } else {
+                    fft_c2c<float>(*P, spec, spec, false, 1, P->N, 1.0, s);
+                }
+            }
+        };
+        run();
+        OFDFT_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < iters; ++i) run();
+        OFDFT_HIP(hipEventRecord(e1, s));
+        OFDFT_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        OFDFT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = (double)ms / iters;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     });
 }
 

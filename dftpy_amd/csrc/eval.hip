@@ -289,8 +289,7 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         const bool auto_rho0 = (s_wt >= 0) && !(P.rho0 > 0);
         fft_r2c<T>(P, rho, spec, ns, 0, P.Nc, 1.0, &pro, auto_rho0 ? P.red : nullptr, s);
         if (auto_rho0) {
-            const long long nrows = (long long)P.nr[0] * P.nr[1];
-            const int nb = (int)((nrows + P.rows_r2c.tile - 1) / P.rows_r2c.tile);
+            const int nb = r2c_num_partials(P);
             hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(256), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
             OFDFT_HIP(hipGetLastError());
         }
@@ -341,5 +340,142 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
 
 template void eval<float>(Plan &, const float *, unsigned, float *, double *, hipStream_t);
 template void eval<double>(Plan &, const double *, unsigned, double *, double *, hipStream_t);
+
+// ------------------------------------------------------------------ linear response
+// dV[drho] of the same functional at rho: the FFT terms are linear in their
+// transformed inputs, so the spectra of d sqrt(rho) = drho / (2 sqrt(rho)),
+// d rho^beta = beta rho^(beta-1) drho and drho go through the identical
+// G-space multiply; the local terms use dV/drho point-wise.
+template <typename T>
+void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
+    const unsigned terms = P.terms;
+    int ns = 0;
+    const int s_vw = (terms & OFDFT_TERM_VW) ? ns++ : -1;
+    const int s_wt = (terms & OFDFT_TERM_WT) ? ns++ : -1;
+    const int s_h = (terms & OFDFT_TERM_HARTREE) ? ns++ : -1;
+    if (ns == 0) return;
+    cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
+    T *work = reinterpret_cast<T *>(P.work);
+    Prologue pro{};
+    if (s_vw >= 0) pro.code[s_vw] = OP_DSQRT;
+    if (s_wt >= 0) pro.code[s_wt] = OP_DPOW_BETA;
+    if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
+    pro.beta = P.beta;
+    pro.aux = rho;
+    fft_r2c<T>(P, drho, spec, ns, 0, P.Nc, 1.0, &pro, nullptr, s);
+    GParams gp;
+    for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
+    gp.nx = P.nr[0];
+    gp.ny = P.nr[1];
+    gp.nz = P.nr[2];
+    gp.nzc = P.nzc;
+    gp.full = P.full;
+    gp.scale = 1.0 / (double)P.N;
+    gp.s_vw = s_vw;
+    gp.s_wt = s_wt;
+    gp.s_h = s_h;
+    gp.sigma = P.sigma;
+    gp.lbda = P.x;
+    gp.alpha = P.alpha;
+    gp.beta = P.beta;
+    gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
+    gp.rho0_dev = (s_wt >= 0 && !(P.rho0 > 0)) ? P.scal + 0 : nullptr;
+    gp.bs = P.Nc;
+    hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(P.Nc)), dim3(256), 0, s, spec, P.Nc, gp);
+    OFDFT_HIP(hipGetLastError());
+    fft_c2r<T>(P, spec, spec, work, ns, P.Nc, P.Nc, P.N, 1.0, s);
+}
+
+// Ap = (V - mu) p + phi dV[2 phi p]  and  sum p.Ap.
+// base_a = 1/2 L(sqrt(rho)) and base_c = C(rho^beta) of the evaluation at rho.
+template <typename T, typename CT>
+__global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ drho, const T *__restrict__ phi,
+                      const T *__restrict__ p, const T *__restrict__ pot, const T *__restrict__ work,
+                      const T *__restrict__ base_a, const T *__restrict__ base_c, const double *__restrict__ mu,
+                      T *__restrict__ Ap, double *partials, PostParams pp) {
+    __shared__ double sm[16];
+    const CT ctf = (CT)(0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0));
+    const CT cbrt34pi = (CT)cbrt(3.0 / (4.0 * M_PI));
+    const CT cx3 = (CT)(-cbrt(3.0 / M_PI) / 3.0);
+    const double m = *mu;
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < pp.n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const CT r = (CT)rho[i], dr = (CT)drho[i];
+        const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
+        CT dv = (CT)0;
+        if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf / cbrt(rp) * dr;  // tf.py:38-40
+        if (pp.s_vw >= 0) {
+            const CT sq = sqrt(r > (CT)0 ? r : (CT)1e-30);
+            const CT ds = dr / ((CT)2 * sq);
+            const CT vwb = (CT)pp.y * (CT)base_a[i] / sq;
+            dv += ((CT)pp.y * (CT)work[pp.s_vw * pp.bs + i] - vwb * ds) / sq;
+        }
+        if (pp.s_wt >= 0) {
+            const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
+            const CT rb1 = pow(rc, (CT)pp.beta - (CT)1);
+            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 / rc * dr * (CT)base_c[i] +
+                                           rb1 * (CT)work[pp.s_wt * pp.bs + i]);
+        }
+        if (pp.terms & OFDFT_TERM_XC) {
+            // semilocal_xc.py:291-303
+            const CT rc3 = cbrt(rp);
+            const CT rs = cbrt34pi / rc3;
+            CT fc;
+            if (rs < (CT)1) {
+                fc = -(CT)(PZ_A / 3.0) - ((CT)(PZ_C / 9.0) * (log(rs) * (CT)2 + (CT)1) + (CT)(PZ_D * 2.0 / 9.0)) * rs;
+            } else {
+                const CT ta = (CT)PZ_B1 * sqrt(rs), tb = (CT)PZ_B2 * rs;
+                const CT de = (CT)1 + ta + tb;
+                fc = (CT)(PZ_G / 36.0) *
+                     ((CT)5 * ta + (CT)7 * ta * ta + (CT)8 * tb + (CT)16 * tb * tb + (CT)21 * ta * tb) / (de * de * de);
+            }
+            dv += (cx3 * rc3 / rp + fc / rp) * dr;
+        }
+        if (pp.s_h >= 0) dv += (CT)work[pp.s_h * pp.bs + i];
+        const double a = ((double)pot[i] - m) * (double)p[i] + (double)phi[i] * (double)dv;
+        Ap[i] = (T)a;
+        acc += (double)p[i] * (double)(T)a;
+    }
+    const double sum = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+}
+
+template <typename T>
+void hvp(Plan &P, const T *rho, const T *drho, const T *phi, const T *p, const T *pot, const T *base_a,
+         const T *base_c, const double *mu, T *Ap, int nb, hipStream_t s) {
+    eval_linear<T>(P, rho, drho, s);
+    const unsigned terms = P.terms;
+    int ns = 0;
+    PostParams pp;
+    pp.s_vw = (terms & OFDFT_TERM_VW) ? ns++ : -1;
+    pp.s_wt = (terms & OFDFT_TERM_WT) ? ns++ : -1;
+    pp.s_h = (terms & OFDFT_TERM_HARTREE) ? ns++ : -1;
+    pp.terms = terms;
+    pp.x = P.x;
+    pp.y = P.y;
+    pp.alpha = P.alpha;
+    pp.beta = P.beta;
+    pp.n = P.N;
+    pp.bs = P.N;
+    pp.want_v = 1;
+    pp.want_e = 0;
+    hipLaunchKernelGGL((k_hvp<T, T>), dim3(nb), dim3(256), 0, s, rho, drho, phi, p, pot,
+                       reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
+    OFDFT_HIP(hipGetLastError());
+}
+
+template void hvp<float>(Plan &, const float *, const float *, const float *, const float *, const float *,
+                         const float *, const float *, const double *, float *, int, hipStream_t);
+template void hvp<double>(Plan &, const double *, const double *, const double *, const double *, const double *,
+                          const double *, const double *, const double *, double *, int, hipStream_t);
+
+// slots of the FFT terms in P.work after an evaluation: (vW, WT, Hartree), -1 if absent
+void work_slots(const Plan &P, int *slot) {
+    int ns = 0;
+    slot[0] = (P.terms & OFDFT_TERM_VW) ? ns++ : -1;
+    slot[1] = (P.terms & OFDFT_TERM_WT) ? ns++ : -1;
+    slot[2] = (P.terms & OFDFT_TERM_HARTREE) ? ns++ : -1;
+}
 
 }  // namespace ofdft

@@ -94,7 +94,10 @@ struct Plan {
     const void *vloc = nullptr;
     double eion = 0.0;
     bool rho0_cached_valid = false;
+    bool use_fixed = true;  // size-specialised FFT kernels where available
 };
+
+int r2c_num_partials(const Plan &P);
 
 // fft.hip
 void plan_fft_setup(Plan &P);
@@ -113,6 +116,12 @@ template <typename T>
 void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in, cpx<T> *out, int batch,
               long long bs, double scale, hipStream_t s);
 
+// single passes, for timing
+template <typename T>
+void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue *pro, hipStream_t s);
+template <typename T>
+void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int batch, hipStream_t s);
+
 // reductions (vec.hip)
 int reduce_blocks(long long n);
 template <typename T>
@@ -123,5 +132,10 @@ void reduce_final(const double *partials, int nblocks, int nterms, double *out, 
 // eval.hip
 template <typename T>
 void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s);
+// analytic Hessian-vector product of the residual (V - mu) phi along p
+template <typename T>
+void hvp(Plan &P, const T *rho, const T *drho, const T *phi, const T *p, const T *pot, const T *base_a,
+         const T *base_c, const double *mu, T *Ap, int nb, hipStream_t s);
+void work_slots(const Plan &P, int *slot);
 
 }  // namespace ofdft

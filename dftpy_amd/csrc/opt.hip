@@ -19,7 +19,7 @@ namespace ofdft {
 
 enum AuxField {
     F_PHI, F_RES, F_P, F_DIR, F_BEST, F_TRES, F_TP, F_PHI1, F_RHO1, F_POT1, F_AP, F_RES_PREV, F_DIR_PREV,
-    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_COUNT
+    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_BASE_A, F_BASE_C, F_DRHO, F_COUNT
 };
 
 void plan_aux_setup(Plan &P) {
@@ -117,6 +117,11 @@ __global__ void k_tn_shift(const T *phi, const T *p, double eps, T *phi1, T *rho
         phi1[i] = v;
         rho1[i] = v * v;
     }
+}
+
+// drho = 2 phi p
+template <typename T> __global__ void k_drho(const T *phi, const T *p, T *drho, long long n) {
+    GRID_LOOP(i, n) drho[i] = (T)(2.0 * (double)phi[i] * (double)p[i]);
 }
 
 // Ap = ((V1 - mu) phi1 - res0) / eps ; sum p.Ap   (optimization.py:105-114)
@@ -304,10 +309,19 @@ template <typename T> struct Opt {
         return C.fetch(1)[0] * P.dV;
     }
 
-    // get_direction_TN (optimization.py:91-147); returns number of V evaluations
-    int direction_tn(int maxfun, double eps) {
+    // get_direction_TN (optimization.py:91-147); returns number of V evaluations.
+    // hvp = 0: forward difference with eps (the reference); hvp = 1: exact
+    // linear response (needs the last E+V evaluation's work slots at phi).
+    int direction_tn(int maxfun, double eps, int hvp_mode) {
         T *dir = F(F_DIR), *best = F(F_BEST), *tres = F(F_TRES), *tp = F(F_TP);
         T *phi1 = F(F_PHI1), *rho1 = F(F_RHO1), *pot1 = F(F_POT1), *Ap = F(F_AP);
+        if (hvp_mode) {
+            int sl[3];
+            work_slots(P, sl);
+            const T *work = reinterpret_cast<const T *>(P.work);
+            if (sl[0] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[0] * n, F(F_BASE_A), n);
+            if (sl[1] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[1] * n, F(F_BASE_C), n);
+        }
         hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
         double r0 = C.fetch(1)[0];
         const double rconv = r0 * 0.1;
@@ -315,10 +329,17 @@ template <typename T> struct Opt {
         std::vector<double> rl{1e10};
         int it = 0;
         for (it = 0; it < maxfun; ++it) {
-            hipLaunchKernelGGL((k_tn_shift<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, eps, phi1, rho1, n);
-            eval_v(rho1, pot1);
-            hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps, Ap,
-                               n, P.red);
+            if (hvp_mode) {
+                hipLaunchKernelGGL((k_drho<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, F(F_DRHO), n);
+                hvp<T>(P, F(F_RHO), F(F_DRHO), F(F_PHI), tp, F(F_POT), F(F_BASE_A), F(F_BASE_C), P.scal + 2, Ap,
+                       C.nb, C.s);
+                ++n_v;
+            } else {
+                hipLaunchKernelGGL((k_tn_shift<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, eps, phi1, rho1, n);
+                eval_v(rho1, pot1);
+                hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps,
+                                   Ap, n, P.red);
+            }
             const double pAp = C.fetch(1)[0];
             if (pAp < 0.0) {
                 if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, r0 / pAp, tp, 0.0, dir, n);
@@ -387,7 +408,6 @@ template <typename T> struct Opt {
         const int maxiter = (int)opts[1], maxfun = (int)opts[2], maxls = (int)opts[3];
         const double c1 = opts[4], c2 = opts[5], xtol = opts[6];
         const int ncheck = (int)opts[7];
-        (void)hvp;
         const auto t0 = std::chrono::steady_clock::now();
         // initial phi (optimization.py:305-316)
         hipLaunchKernelGGL((k_init_phi<T>), C.grid(), dim3(256), 0, C.s, rho_io, F(F_PHI), n, P.red);
@@ -398,12 +418,13 @@ template <typename T> struct Opt {
         double mu = chemical_potential(F(F_POT), F(F_RHO));
         residual(F(F_POT), F(F_PHI), F(F_RES));
         std::vector<double> hist{energy};
+        std::vector<double> times{std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()};
         double theta = 0.1;
         int converged = 2, it = 0, nres = 0;
         for (it = 1; it < maxiter; ++it) {
             int nd = 1;
             if (method == 0) {
-                nd = direction_tn(maxfun, 1.0e-9);
+                nd = direction_tn(maxfun, 1.0e-9, hvp);
                 hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR), F(F_P), n);
             } else {
                 // CG (math_utils.py:395-423); first step and missing history: -res
@@ -451,7 +472,9 @@ template <typename T> struct Opt {
                 }
             }
             if (!ok || !evaluated) {
-                converged = 1;
+                // FP32: past the first steps a failed search means the energy
+                // differences fell below single-precision resolution
+                converged = (P.prec == OFDFT_FP32 && it > ncheck) ? 3 : 1;
                 break;
             }
             theta = stp;
@@ -468,6 +491,7 @@ template <typename T> struct Opt {
             mu = chemical_potential(F(F_POT), F(F_RHO));
             residual(F(F_POT), F(F_PHI), F(F_RES));
             hist.push_back(energy);
+            times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
             (void)nd;
             if ((int)hist.size() - 1 >= ncheck) {
                 bool conv = true;
@@ -494,7 +518,10 @@ template <typename T> struct Opt {
             stats[6] = secs;
         }
         if (history)
-            for (int i = 0; i < hcap && i < (int)hist.size(); ++i) history[i] = hist[i];
+            for (int i = 0; i < hcap && i < (int)hist.size(); ++i) {
+                history[i] = hist[i];
+                history[hcap + i] = times[i];
+            }
         return converged;
     }
 };

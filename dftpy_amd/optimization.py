@@ -48,20 +48,29 @@ class Optimization:
         eion = ev.eion() if isinstance(ev, TotalFunctional) else 0.0
         _, vloc = configure(plan, funcs, grid, eion)
         o = self.optimization_options
+        # Hessian-vector product of the TN inner solve: the reference's forward
+        # difference (eps = 1e-9) needs FP64; FP32 grids use the exact linear
+        # response of the potential
         hvp = o["hvp"]
-        hvp = 0 if hvp == "auto" or hvp == "fd" else 1
+        if hvp == "auto":
+            hvp = "fd" if grid.dtype == torch.float64 else "analytic"
+        if hvp not in ("fd", "analytic"):
+            raise ValueError("hvp must be 'auto', 'fd' or 'analytic'")
+        hvp = 0 if hvp == "fd" else 1
         opts = np.array([o["econv"], o["maxiter"], o["maxfun"], o["maxls"], o["c1"], o["c2"], o["xtol"], o["ncheck"]],
                         dtype=np.float64)
         stats = np.zeros(16)
-        hist = np.zeros(int(o["maxiter"]) + 2)
+        hcap = int(o["maxiter"]) + 2
+        hist = np.zeros(2 * hcap)
         rho = rho_in.data.clone().contiguous()
         pot = torch.empty_like(rho)
         _lib.check(_lib.lib.ofdft_optimize(plan.handle, rho.data_ptr(), pot.data_ptr(),
                                            METHODS[self.optimization_method], hvp, _lib.dptr(opts), _lib.dptr(stats),
-                                           _lib.dptr(hist), len(hist), _lib.stream()))
+                                           _lib.dptr(hist), hcap, _lib.stream()))
         del vloc
         n = int(stats[1]) + 1
         self.history = hist[:n].copy()
+        self.history_seconds = hist[hcap:hcap + n].copy()
         self.stats = dict(converged=int(stats[0]), niter=int(stats[1]), energy=stats[2], mu=stats[3],
                           n_eval_v=int(stats[4]), n_eval_ev=int(stats[5]), seconds=stats[6])
         self.converged = int(stats[0])

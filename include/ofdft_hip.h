@@ -16,7 +16,9 @@
  *  - Return value: 0 on success, nonzero on error; ofdft_last_error() gives
  *    the message (thread-local).
  *  - The plan owns its workspace (allocated at create, freed at destroy);
- *    no entry point allocates device memory after plan creation.
+ *    the FFT, evaluation, reduction, optimizer and propagator entry points
+ *    allocate no device memory; the one-time setup calls (local_pp, ewald)
+ *    allocate small per-call tables.
  *
  * Each declaration names the reference interface it replaces, as
  * path:line relative to the reference's src/dftpy.
@@ -63,6 +65,10 @@ const char *ofdft_last_error(void);
 int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, int full,
                       int max_batch, ofdft_plan **plan);
 int ofdft_plan_destroy(ofdft_plan *plan);
+/* FFT kernel selection (no counterpart in the reference): fixed != 0 (the
+ * default) uses the size-specialised kernels for the lengths that have them,
+ * 0 forces the generic mixed-radix kernels everywhere. */
+int ofdft_plan_set_fft_kernels(ofdft_plan *plan, int fixed);
 /* Bytes of device workspace the plan holds. */
 size_t ofdft_plan_workspace_bytes(const ofdft_plan *plan);
 
@@ -92,7 +98,7 @@ int ofdft_set_vloc(ofdft_plan *plan, const void *vreal);
 int ofdft_set_eion(ofdft_plan *plan, double eion);
 
 /* One TotalFunctional evaluation (total_functional.py:115-146):
- * energies (device, OFDFT_E_N doubles) and/or the total potential pot
+ * energies (device, OFDFT_E_COUNT doubles) and/or the total potential pot
  * (device, N reals).  rho is not modified (unlike wt.py:15-26). */
 int ofdft_eval(ofdft_plan *plan, const void *rho, unsigned calc, void *pot, double *energies,
                void *stream);
@@ -100,6 +106,15 @@ int ofdft_eval(ofdft_plan *plan, const void *rho, unsigned calc, void *pot, doub
 /* Sum_r a(r) b(r) in FP64 (MP.einsum, mpi/mpi.py:199-206); b == NULL gives
  * Sum_r a(r).  Not multiplied by dV. */
 int ofdft_dot(ofdft_plan *plan, const void *a, const void *b, double *out, void *stream);
+
+/* Measurement entry point (no counterpart in the reference): launches one
+ * pass of the evaluation pipeline `iters` times on `stream` between two HIP
+ * events and returns the mean duration in ms.  which: 0 z-pass R2C with the
+ * vW/WT prologue (reads rho), 1 y-pass, 2 x-pass (C2C on the half grid),
+ * 3 z-pass C2R, 4 full-grid C2C 3D transform; batch 1..3 spectra.  Uses the
+ * plan's workspace as data. */
+int ofdft_bench_pass(ofdft_plan *plan, int which, int batch, int iters, const void *rho, double *avg_ms,
+                     void *stream);
 
 /* Local PP setup on the grid (LocalPseudo.local_PP, pseudo/__init__.py:236-246,
  * _PP_Reciprocal 377-387, _PP_Reciprocal_PME 389-404).
@@ -134,7 +149,11 @@ int ofdft_ewald(ofdft_plan *plan, int nat, const double *pos_cart, const double 
  * with the optimised density; pot (device, N) receives the final potential.
  * opts (host, 8 doubles): econv, maxiter, maxfun, maxls, c1, c2, xtol, ncheck.
  * stats (host, 16 doubles): [converged, niter, energy, mu, n_eval_v, n_eval_ev,
- * seconds, ...].  history (host, capacity hcap) receives energies per step. */
+ * seconds, ...]; converged: 0 converged, 1 line search failed, 2 maxiter
+ * reached (optimization.py:440-444,509-511), 3 FP32 plan stopped at single-
+ * precision resolution (line search failure after ncheck steps).
+ * history (host, 2*hcap doubles): energies per step in [0, hcap), seconds since
+ * the call started at each step in [hcap, 2*hcap). */
 int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method, int hvp, const double *opts,
                    double *stats, double *history, int hcap, void *stream);
 

@@ -94,3 +94,44 @@ def test_c1_vs_oracle(method):
         r_tn = _c1_oracle(1e-8)
         # same algorithm, same trajectory length up to a step
         assert abs(len(opt.history) - len(r_tn["history"])) <= 1
+
+
+@pytest.mark.parametrize("dtype,hvp", [(torch.float64, "analytic"), (torch.float32, "auto")])
+def test_c1_analytic_hvp(dtype, hvp):
+    """TN with the exact linear-response Hessian-vector product (FP64 and the
+    FP32-bulk form) converges to the oracle's energy within 1e-6 Ha/atom."""
+    from dftpy_amd import DirectField, DirectGrid, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    from dftpy_amd.optimization import Optimization
+    ions, nr = systems.c1_system()
+    grid = DirectGrid(ions.cell, nr, full=False, dtype=dtype)
+    pp = Functional(type="PSEUDO", grid=grid, ions=ions, PP_list={"Al": systems.empty_core_recpot()})
+    ev = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                         H=Functional(type="HARTREE"), PSEUDO=pp)
+    rho = DirectField(grid)
+    rho[:] = ions.get_ncharges() / grid.volume
+    econv = 1e-7 if dtype == torch.float64 else 1e-6 * ions.nat
+    opt = Optimization(optimization_method="TN", EnergyEvaluator=ev,
+                       optimization_options={"econv": econv, "maxiter": 100, "hvp": hvp})
+    out = opt.optimize_rho(rho)
+    assert opt.converged in ((0,) if dtype == torch.float64 else (0, 3))
+    ref = _c1_oracle(1e-10)
+    # the energy of the optimised density, evaluated in FP64
+    g64 = DirectGrid(ions.cell, nr, full=False)
+    pp64 = Functional(type="PSEUDO", grid=g64, ions=ions, PP_list={"Al": systems.empty_core_recpot()})
+    ev64 = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                           H=Functional(type="HARTREE"), PSEUDO=pp64)
+    e64 = ev64(DirectField(g64, data=out.data.double()), calcType={"E"}).energy
+    assert abs(e64 - ref["energy"]) / ions.nat < 1e-6
+    assert abs(opt.functional.energy - ref["energy"]) / ions.nat < (1e-6 if dtype == torch.float64 else 1e-5)
+    assert np.max(np.abs(out.numpy() - ref["rho"])) < (1e-5 if dtype == torch.float64 else 2e-4)
+
+
+def test_gaas_fp32_analytic():
+    from dftpy_amd.optimization import Optimization
+    ions, grid, rho_ini, ev = _gaas_setup(torch.float32)
+    opt = Optimization(EnergyEvaluator=ev, optimization_options={"econv": 1e-6 * ions.nat, "maxiter": 100},
+                       optimization_method="TN")
+    new_rho = opt.optimize_rho(guess_rho=rho_ini)
+    assert opt.converged == 0
+    assert np.isclose(opt.functional.energy, -43.3046, rtol=1e-5)
