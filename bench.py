@@ -156,6 +156,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--passes-only", action="store_true",
+                    help="only launch the evaluation passes (and a reference copy kernel) for counter collection")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-cell path on one device; "
@@ -164,6 +166,17 @@ def main():
     import torch
     log("device:", torch.cuda.get_device_name(0))
     ions, nr, g64, g32, ev, rho64, rho32 = build_system(args.seed)
+    if args.passes_only:
+        ev(rho32, calcType={"E", "V"})
+        log("fp32 passes (ms):", pass_timings(g32, rho32, iters=args.steps))
+        log("fp64 passes (ms):", pass_timings(g64, rho64, iters=args.steps))
+        # a plain device copy of 256 MiB, to calibrate the HBM byte counters
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        return
     nat = ions.nat
     tol = THRESH * nat
     ref_path = os.path.join(ROOT, "tests", "golden", "c2_oracle.json")

@@ -82,6 +82,10 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             P->red = P->scal = nullptr;
             for (int d = 0; d < 3; ++d) P->nr[d] = nr[d];
             P->nzc = nr[2] / 2 + 1;
+            {
+                const int per = precision == OFDFT_FP64 ? 8 : 16;  // complex elements per 128 bytes
+                P->nzp = ((P->nzc + per - 1) / per) * per;
+            }
             P->N = (long long)nr[0] * nr[1] * nr[2];
             P->Nc = (long long)nr[0] * nr[1] * P->nzc;
             P->Nfull = P->N;
@@ -103,7 +107,8 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             const size_t rs = precision == OFDFT_FP64 ? 8 : 4;
             const int nslot = P->max_batch > 4 ? P->max_batch : 4;
             // spectra: nslot half grids, or nslot/2 full complex grids
-            const size_t spec_bytes = std::max((size_t)(nslot * P->Nc), (size_t)(2 * P->N)) * 2 * rs;
+            const size_t ncp = (size_t)nr[0] * nr[1] * P->nzp;
+            const size_t spec_bytes = std::max((size_t)nslot * ncp, (size_t)(2 * P->N)) * 2 * rs;
             OFDFT_HIP(hipMalloc(&P->spec, spec_bytes));
             // work: 4 real fields, at least one FP64 field for setup arrays
             const size_t work_bytes = std::max((size_t)(4 * P->N * rs), (size_t)(P->N * 8));
@@ -260,7 +265,9 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                     pro.beta = P->beta;
                     bench_rows_r2c<double>(*P, (const double *)rho, spec, batch, &pro, s);
                 } else if (which == 1 || which == 2) {
-                    fft_cols<double>(*P, which == 1 ? 1 : 0, false, false, spec, spec, batch, P->Nc, 1.0, s);
+                    const long long pt = pipe_pitch(*P);
+                    fft_cols<double>(*P, which == 1 ? 1 : 0, false, false, spec, spec, batch,
+                                     (long long)P->nr[0] * P->nr[1] * pt, 1.0, s, pt);
                 } else if (which == 3) {
                     bench_rows_c2r<double>(*P, spec, (double *)P->work, batch, s);
                 } else {
@@ -275,7 +282,9 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                     pro.beta = P->beta;
                     bench_rows_r2c<float>(*P, (const float *)rho, spec, batch, &pro, s);
                 } else if (which == 1 || which == 2) {
-                    fft_cols<float>(*P, which == 1 ? 1 : 0, false, false, spec, spec, batch, P->Nc, 1.0, s);
+                    const long long pt = pipe_pitch(*P);
+                    fft_cols<float>(*P, which == 1 ? 1 : 0, false, false, spec, spec, batch,
+                                    (long long)P->nr[0] * P->nr[1] * pt, 1.0, s, pt);
                 } else if (which == 3) {
                     bench_rows_c2r<float>(*P, spec, (float *)P->work, batch, s);
                 } else {

@@ -54,6 +54,7 @@ __device__ __forceinline__ double lindhard(double eta, double lbda, double mu) {
 struct GParams {
     double B[9];
     int nx, ny, nz, nzc;
+    int pitch;  // row pitch of the spectra (>= nzc)
     int full;
     double scale;
     int s_vw, s_wt, s_h;
@@ -89,14 +90,16 @@ __device__ __forceinline__ KVals kernel_vals(const GParams &p, double gg, bool g
 
 template <typename T>
 __global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
+    // nc counts the pitched layout; columns iz >= nzc are padding
     const double rho0 = p.rho0_dev ? *p.rho0_dev : p.rho0;
     const double tkf = 2.0 * cbrt(3.0 * rho0 * M_PI * M_PI);
     const double ctf = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
     const double wtfac = 5.0 / (9.0 * p.alpha * p.beta * pow(rho0, p.alpha + p.beta - 5.0 / 3.0)) * ctf;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
          i += (long long)gridDim.x * blockDim.x) {
-        const int iz = (int)(i % p.nzc);
-        const long long rest = i / p.nzc;
+        const int iz = (int)(i % p.pitch);
+        if (iz >= p.nzc) continue;
+        const long long rest = i / p.pitch;
         const int iy = (int)(rest % p.ny);
         const int ix = (int)(rest / p.ny);
         int fx = ffreq(ix, p.nx), fy = ffreq(iy, p.ny), fz = iz;
@@ -287,9 +290,11 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
         pro.beta = P.beta;
         const bool auto_rho0 = (s_wt >= 0) && !(P.rho0 > 0);
-        fft_r2c<T>(P, rho, spec, ns, 0, P.Nc, 1.0, &pro, auto_rho0 ? P.red : nullptr, s);
+        const long long pitch = pipe_pitch(P);
+        const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+        fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch);
         if (auto_rho0) {
-            const int nb = r2c_num_partials(P);
+            const int nb = r2c_num_partials(P, ns);
             hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(256), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
             OFDFT_HIP(hipGetLastError());
         }
@@ -310,10 +315,11 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         gp.beta = P.beta;
         gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
         gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
-        gp.bs = P.Nc;
-        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(P.Nc)), dim3(256), 0, s, spec, P.Nc, gp);
+        gp.bs = ncp;
+        gp.pitch = (int)pitch;
+        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);
         OFDFT_HIP(hipGetLastError());
-        fft_c2r<T>(P, spec, spec, work, ns, P.Nc, P.Nc, P.N, 1.0, s);
+        fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch);
     }
     PostParams pp;
     pp.s_vw = s_vw;
@@ -362,7 +368,9 @@ void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
     if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
     pro.beta = P.beta;
     pro.aux = rho;
-    fft_r2c<T>(P, drho, spec, ns, 0, P.Nc, 1.0, &pro, nullptr, s);
+    const long long pitch = pipe_pitch(P);
+    const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+    fft_r2c<T>(P, drho, spec, ns, 0, ncp, 1.0, &pro, nullptr, s, pitch);
     GParams gp;
     for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
     gp.nx = P.nr[0];
@@ -380,10 +388,11 @@ void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
     gp.beta = P.beta;
     gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
     gp.rho0_dev = (s_wt >= 0 && !(P.rho0 > 0)) ? P.scal + 0 : nullptr;
-    gp.bs = P.Nc;
-    hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(P.Nc)), dim3(256), 0, s, spec, P.Nc, gp);
+    gp.bs = ncp;
+    gp.pitch = (int)pitch;
+    hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);
     OFDFT_HIP(hipGetLastError());
-    fft_c2r<T>(P, spec, spec, work, ns, P.Nc, P.Nc, P.N, 1.0, s);
+    fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch);
 }
 
 // Ap = (V - mu) p + phi dV[2 phi p]  and  sum p.Ap.

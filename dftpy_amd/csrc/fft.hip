@@ -283,19 +283,25 @@ __global__ void k_rows_c2c(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ o
 bool fixed_size(int n);
 template <typename T>
 bool cols_fixed(int n, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,
-                long long Q, long long S, const void *tw, double scale, hipStream_t s);
+                long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s);
 template <typename T>
 bool r2c_fixed(int n, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               const void *tw, double scale, const Prologue &pro, double *partials, int sum_slot, hipStream_t s);
+               long long opitch, const void *tw, double scale, const Prologue &pro, double *partials, int sum_slot,
+               hipStream_t s);
 template <typename T>
 bool c2r_fixed(int n, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               const void *tw, double scale, hipStream_t s);
-int r2c_fixed_tiles(int n, int prec, long long nrows);
+               long long ipitch, const void *tw, double scale, hipStream_t s);
+int r2c_fixed_tiles(int n, int prec, int batch, long long nrows);
 
-int r2c_num_partials(const Plan &P) {
+int r2c_num_partials(const Plan &P, int batch) {
     const long long nrows = (long long)P.nr[0] * P.nr[1];
-    if (P.use_fixed && fixed_size(P.nr[2])) return r2c_fixed_tiles(P.nr[2], P.prec, nrows);
+    if (P.use_fixed && fixed_size(P.nr[2])) return r2c_fixed_tiles(P.nr[2], P.prec, batch, nrows);
     return (int)((nrows + P.rows_r2c.tile - 1) / P.rows_r2c.tile);
+}
+
+long long pipe_pitch(const Plan &P) {
+    if (P.use_fixed && fixed_size(P.nr[0]) && fixed_size(P.nr[1]) && fixed_size(P.nr[2])) return P.nzp;
+    return P.nzc;
 }
 
 static AxisPlan make_axis(int n) {
@@ -421,26 +427,30 @@ void plan_fft_free(Plan &P) {
 
 template <typename T>
 void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in, cpx<T> *out, int batch,
-              long long bs, double scale, hipStream_t s) {
+              long long bs, double scale, hipStream_t s, long long pitch) {
     const int n = P.nr[axis];
-    if (n == 1 && in == out && scale == 1.0) return;
     const long long zlen = full_grid ? P.nr[2] : P.nzc;
+    if (pitch <= 0) pitch = zlen;
+    if (n == 1 && in == out && scale == 1.0) return;
     if (P.use_fixed && fixed_size(n)) {
-        // flattened pencils: base(c) = (c / Pp) * Qq + c % Pp, stride Ss
+        // flattened pencils: base(c) = (c / Pp) * Qq + c % Pp, stride Ss; columns
+        // at or beyond zlen within a pitched row are padding and skipped
         long long ncols, Pp, Qq, Ss;
         if (axis == 1) {
-            ncols = (long long)P.nr[0] * zlen;
-            Pp = zlen;
-            Qq = (long long)P.nr[1] * zlen;
-            Ss = zlen;
+            ncols = (long long)P.nr[0] * pitch;
+            Pp = pitch;
+            Qq = (long long)P.nr[1] * pitch;
+            Ss = pitch;
         } else {
-            ncols = (long long)P.nr[1] * zlen;
-            Pp = Qq = Ss = ncols;
+            ncols = (long long)P.nr[1] * pitch;
+            Pp = Qq = pitch;
+            Ss = ncols;
         }
-        cols_fixed<T>(n, inverse, in, out, batch, bs, ncols, Pp, Qq, Ss, P.tw[axis], scale, s);
+        cols_fixed<T>(n, inverse, in, out, batch, bs, ncols, Pp, Qq, Ss, zlen, P.tw[axis], scale, s);
         OFDFT_HIP(hipGetLastError());
         return;
     }
+    OFDFT_REQUIRE(pitch == zlen, "generic column pass needs an unpadded layout");
     long long M, outer;
     PassShape sh;
     if (axis == 1) {
@@ -466,7 +476,8 @@ void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in,
 
 template <typename T>
 void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, double scale,
-             const Prologue *pro, double *sum_partials, hipStream_t s) {
+             const Prologue *pro, double *sum_partials, hipStream_t s, long long pitch) {
+    if (pitch <= 0) pitch = P.nzc;
     Prologue pr;
     std::memset(&pr, 0, sizeof(pr));
     int sum_slot = -1;
@@ -480,34 +491,37 @@ void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long
     const PassShape sh = P.rows_r2c;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
     const bool last_z = (P.nr[0] == 1 && P.nr[1] == 1);
-    if (!(P.use_fixed && r2c_fixed<T>(P.nr[2], in, out, batch, in_bs, out_bs, nrows, P.tw[2], last_z ? scale : 1.0, pr,
-                                      sum_partials, sum_slot, s))) {
+    if (!(P.use_fixed && r2c_fixed<T>(P.nr[2], in, out, batch, in_bs, out_bs, nrows, pitch, P.tw[2],
+                                      last_z ? scale : 1.0, pr, sum_partials, sum_slot, s))) {
+        OFDFT_REQUIRE(pitch == P.nzc, "generic R2C pass needs an unpadded layout");
         dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
         hipLaunchKernelGGL((k_rows_r2c<T>), grid, dim3(sh.threads), sh.lds, s, in, out, in_bs, out_bs, nrows, P.nr[2],
                            P.nzc, P.ax[2], tw, sh.tile, (T)(last_z ? scale : 1.0), pr, sum_partials, sum_slot);
     }
     OFDFT_HIP(hipGetLastError());
     if (last_z) return;
-    fft_cols<T>(P, 1, false, false, out, out, batch, out_bs, 1.0, s);
-    fft_cols<T>(P, 0, false, false, out, out, batch, out_bs, scale, s);
+    fft_cols<T>(P, 1, false, false, out, out, batch, out_bs, 1.0, s, pitch);
+    fft_cols<T>(P, 0, false, false, out, out, batch, out_bs, scale, s, pitch);
 }
 
 template <typename T>
 void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long long in_bs, long long tmp_bs,
-             long long out_bs, double scale, hipStream_t s) {
+             long long out_bs, double scale, hipStream_t s, long long pitch) {
+    if (pitch <= 0) pitch = P.nzc;
     const cpx<T> *src = in;
     long long src_bs = in_bs;
     if (!(P.nr[0] == 1 && P.nr[1] == 1)) {
         OFDFT_REQUIRE(in_bs == tmp_bs, "c2r: batch strides of input and scratch differ");
-        fft_cols<T>(P, 0, false, true, in, tmp, batch, tmp_bs, 1.0, s);
-        fft_cols<T>(P, 1, false, true, tmp, tmp, batch, tmp_bs, 1.0, s);
+        fft_cols<T>(P, 0, false, true, in, tmp, batch, tmp_bs, 1.0, s, pitch);
+        fft_cols<T>(P, 1, false, true, tmp, tmp, batch, tmp_bs, 1.0, s, pitch);
         src = tmp;
         src_bs = tmp_bs;
     }
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const PassShape sh = P.rows_c2r;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
-    if (!(P.use_fixed && c2r_fixed<T>(P.nr[2], src, out, batch, src_bs, out_bs, nrows, P.tw[2], scale, s))) {
+    if (!(P.use_fixed && c2r_fixed<T>(P.nr[2], src, out, batch, src_bs, out_bs, nrows, pitch, P.tw[2], scale, s))) {
+        OFDFT_REQUIRE(pitch == P.nzc, "generic C2R pass needs an unpadded layout");
         dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
         hipLaunchKernelGGL((k_rows_c2r<T>), grid, dim3(sh.threads), sh.lds, s, src, out, src_bs, out_bs, nrows,
                            P.nr[2], P.nzc, P.ax[2], tw, sh.tile, (T)scale);
@@ -532,14 +546,16 @@ void fft_c2c(Plan &P, const cpx<T> *in, cpx<T> *out, bool inverse, int batch, lo
                            P.ax[2], tw, sh.tile, sz);
     OFDFT_HIP(hipGetLastError());
     if (last_z) return;
-    fft_cols<T>(P, 1, true, inverse, out, out, batch, bs, 1.0, s);
-    fft_cols<T>(P, 0, true, inverse, out, out, batch, bs, scale, s);
+    fft_cols<T>(P, 1, true, inverse, out, out, batch, bs, 1.0, s, P.nr[2]);
+    fft_cols<T>(P, 0, true, inverse, out, out, batch, bs, scale, s, P.nr[2]);
 }
 
 template <typename T>
 void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue *pro, hipStream_t s) {
     const long long nrows = (long long)P.nr[0] * P.nr[1];
-    if (P.use_fixed && r2c_fixed<T>(P.nr[2], in, out, batch, 0LL, P.Nc, nrows, P.tw[2], 1.0, *pro, nullptr, -1, s)) {
+    const long long pitch = pipe_pitch(P);
+    if (P.use_fixed && r2c_fixed<T>(P.nr[2], in, out, batch, 0LL, (long long)P.nr[0] * P.nr[1] * pitch, nrows, pitch,
+                                    P.tw[2], 1.0, *pro, nullptr, -1, s)) {
         OFDFT_HIP(hipGetLastError());
         return;
     }
@@ -553,7 +569,9 @@ void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue
 
 template <typename T> void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int batch, hipStream_t s) {
     const long long nrows = (long long)P.nr[0] * P.nr[1];
-    if (P.use_fixed && c2r_fixed<T>(P.nr[2], in, out, batch, P.Nc, P.N, nrows, P.tw[2], 1.0, s)) {
+    const long long pitch = pipe_pitch(P);
+    if (P.use_fixed && c2r_fixed<T>(P.nr[2], in, out, batch, (long long)P.nr[0] * P.nr[1] * pitch, P.N, nrows, pitch,
+                                    P.tw[2], 1.0, s)) {
         OFDFT_HIP(hipGetLastError());
         return;
     }
@@ -571,11 +589,11 @@ template void bench_rows_c2r<double>(Plan &, const double2 *, double *, int, hip
 
 #define INST(T)                                                                                               \
     template void fft_cols<T>(Plan &, int, bool, bool, const cpx<T> *, cpx<T> *, int, long long, double,       \
-                              hipStream_t);                                                                   \
+                              hipStream_t, long long);                                                        \
     template void fft_r2c<T>(Plan &, const T *, cpx<T> *, int, long long, long long, double, const Prologue *, \
-                             double *, hipStream_t);                                                          \
+                             double *, hipStream_t, long long);                                               \
     template void fft_c2r<T>(Plan &, const cpx<T> *, cpx<T> *, T *, int, long long, long long, long long,      \
-                             double, hipStream_t);                                                            \
+                             double, hipStream_t, long long);                                                 \
     template void fft_c2c<T>(Plan &, const cpx<T> *, cpx<T> *, bool, int, long long, double, hipStream_t);
 INST(float)
 INST(double)

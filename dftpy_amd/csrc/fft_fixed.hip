@@ -129,7 +129,7 @@ __device__ __forceinline__ void cstages(C *buf, const C *tw, const C *src, C *ds
 template <typename T, int N, int TILE, bool INV>
 __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     k_cols_f(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, long long bs, long long ncols, long long P,
-             long long Q, long long S, const cpx<T> *__restrict__ twg, T scale) {
+             long long Q, long long S, long long vp, const cpx<T> *__restrict__ twg, T scale) {
     using C = cpx<T>;
     __shared__ C tw[N];
     __shared__ C buf[N * TILE];
@@ -137,8 +137,9 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     const int j = threadIdx.x / TILE;
     for (int i = threadIdx.x; i < N; i += TILE * SL<N>::tpp) tw[i] = twg[i];
     const long long c = (long long)blockIdx.x * TILE + pen;
-    const bool valid = c < ncols;
-    const long long base = valid ? (c / P) * Q + (c % P) : 0;
+    const long long cp = c % P;
+    const bool valid = c < ncols && cp < vp;  // pad columns of a pitched layout are skipped
+    const long long base = valid ? (c / P) * Q + cp : 0;
     const long long off = (long long)blockIdx.y * bs + base;
     cstages<C, N, TILE, INV, 0, 1>(buf, tw, in + off, out + off, S, valid, pen, j, scale);
 }
@@ -203,57 +204,62 @@ __device__ __forceinline__ T prologue_f(int code, T x, T beta, const T *aux, lon
     }
 }
 
-// Rows R2C, NPEN packed pairs of rows per workgroup.  blockIdx.x selects the
-// batch slot (fastest, so the slots reading the same input rows run close
-// together and share L2), blockIdx.y the row tile.
-template <typename T, int N, int NPEN>
-__global__ void __launch_bounds__(NPEN *SL<N>::tpp)
+// Rows R2C for NF spectra at once: NPEN packed pairs of rows per spectrum,
+// so the NF*NPEN pencils run through the same LDS stages.  When all spectra
+// transform the same input (in_bs == 0, each with its own prologue) the input
+// rows are read once.  Output rows have pitch opitch >= N/2+1.
+template <typename T, int N, int NPEN, int NF>
+__global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
     k_rows_r2c_f(const T *__restrict__ in, cpx<T> *__restrict__ out, long long in_bs, long long out_bs, long long nrows,
-                 const cpx<T> *__restrict__ twg, T scale, Prologue pro, double *partials, int sum_slot) {
+                 long long opitch, const cpx<T> *__restrict__ twg, T scale, Prologue pro, double *partials,
+                 int sum_slot) {
     using C = cpx<T>;
-    constexpr int NT = NPEN * SL<N>::tpp;
+    constexpr int NT = NF * NPEN * SL<N>::tpp;
     constexpr int ROWS = 2 * NPEN;
     constexpr int NZC = N / 2 + 1;
     constexpr int LD = N + 1;
     __shared__ C tw[N];
-    __shared__ C buf[NPEN * LD];
+    __shared__ C buf[NF * NPEN * LD];
     __shared__ double red[16];
-    const int slot = blockIdx.x;
+    const int slot0 = blockIdx.x * NF;
     const long long row0 = (long long)blockIdx.y * ROWS;
-    const T *src = in + (long long)slot * in_bs + row0 * N;
-    C *dst = out + (long long)slot * out_bs + row0 * NZC;
     const int nv = (int)min((long long)ROWS, nrows - row0);
-    const int code = pro.code[slot & 3];
     const T beta = (T)pro.beta;
     const T *aux = reinterpret_cast<const T *>(pro.aux);
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
     double acc = 0.0;
     for (int e = threadIdx.x; e < ROWS * N; e += NT) {
         const int r = e / N;
-        T x = (T)0;
-        if (r < nv) {
-            x = src[e];
-            acc += (double)x;
-            x = prologue_f<T>(code, x, beta, aux, row0 * N + e);
+        const int pos = e - r * N;
+        T x0 = (T)0;
+        if (r < nv) x0 = in[(long long)slot0 * in_bs + row0 * N + e];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            T x = x0;
+            if (f > 0 && in_bs != 0 && r < nv) x = in[(long long)(slot0 + f) * in_bs + row0 * N + e];
+            if (partials != nullptr && slot0 + f == sum_slot) acc += (double)x;
+            if (r < nv) x = prologue_f<T>(pro.code[(slot0 + f) & 3], x, beta, aux, row0 * N + e);
+            reinterpret_cast<T *>(&buf[(f * NPEN + (r >> 1)) * LD + pos])[r & 1] = x;
         }
-        reinterpret_cast<T *>(&buf[(r >> 1) * LD + (e - r * N)])[r & 1] = x;
     }
-    if (partials != nullptr && slot == sum_slot) {
+    const bool do_sum = partials != nullptr && sum_slot >= slot0 && sum_slot < slot0 + NF;
+    if (do_sum) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     }
     __syncthreads();
-    if (partials != nullptr && slot == sum_slot && threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < (NT + 63) / 64; ++w) s += red[w];
-        partials[blockIdx.y] = s;
+    if (do_sum && threadIdx.x == 0) {
+        double sm = 0.0;
+        for (int w = 0; w < (NT + 63) / 64; ++w) sm += red[w];
+        partials[blockIdx.y] = sm;
     }
-    rstages<C, N, LD, NPEN, false, 0, 1>(buf, tw);
-    for (int e = threadIdx.x; e < ROWS * NZC; e += NT) {
-        const int r = e / NZC, k = e - r * NZC;
+    rstages<C, N, LD, NF * NPEN, false, 0, 1>(buf, tw);
+    for (int e = threadIdx.x; e < NF * ROWS * NZC; e += NT) {
+        const int fr = e / NZC, k = e - fr * NZC;
+        const int f = fr / ROWS, r = fr - f * ROWS;
         if (r < nv) {
-            const int p = r >> 1;
+            const int p = f * NPEN + (r >> 1);
             const C z = buf[p * LD + k];
             C zc = buf[p * LD + (k == 0 ? 0 : N - k)];
             zc.y = -zc.y;
@@ -262,7 +268,7 @@ __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
                 o = C{(z.x + zc.x) * (T)0.5, (z.y + zc.y) * (T)0.5};
             else
                 o = C{(z.y - zc.y) * (T)0.5, -(z.x - zc.x) * (T)0.5};
-            dst[e] = C{o.x * scale, o.y * scale};
+            out[(long long)(slot0 + f) * out_bs + (row0 + r) * opitch + k] = C{o.x * scale, o.y * scale};
         }
     }
 }
@@ -271,7 +277,7 @@ __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
 template <typename T, int N, int NPEN>
 __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
     k_rows_c2r_f(const cpx<T> *__restrict__ in, T *__restrict__ out, long long in_bs, long long out_bs,
-                 long long nrows, const cpx<T> *__restrict__ twg, T scale) {
+                 long long nrows, long long ipitch, const cpx<T> *__restrict__ twg, T scale) {
     using C = cpx<T>;
     constexpr int NT = NPEN * SL<N>::tpp;
     constexpr int ROWS = 2 * NPEN;
@@ -281,13 +287,13 @@ __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
     __shared__ C stg[ROWS * NZC];
     __shared__ C buf[NPEN * LD];
     const long long row0 = (long long)blockIdx.y * ROWS;
-    const C *src = in + (long long)blockIdx.x * in_bs + row0 * NZC;
+    const C *src = in + (long long)blockIdx.x * in_bs + row0 * ipitch;
     T *dst = out + (long long)blockIdx.x * out_bs + row0 * N;
     const int nv = (int)min((long long)ROWS, nrows - row0);
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
     for (int e = threadIdx.x; e < ROWS * NZC; e += NT) {
         const int r = e / NZC, k = e - r * NZC;
-        C v = (r < nv) ? src[e] : C{(T)0, (T)0};
+        C v = (r < nv) ? src[r * ipitch + k] : C{(T)0, (T)0};
         if (k == 0 || 2 * k == N) v.y = (T)0;
         stg[e] = v;
     }
@@ -331,36 +337,58 @@ template <typename T, int N> constexpr int row_pens() {
         --p;
     return p;
 }
+template <typename T, int N, int NF> constexpr int r2c_pens() {
+    int p = 8;
+    while (p > 1 && (NF * p * SL<N>::tpp > 1024 || (NF * p * (N + 1) + N) * 2 * (int)sizeof(T) + 128 > 64 * 1024))
+        --p;
+    return p;
+}
 
 template <typename T, int N>
 static void launch_cols(bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,
-                        long long Q, long long S, const cpx<T> *tw, T scale, hipStream_t s) {
+                        long long Q, long long S, long long vp, const cpx<T> *tw, T scale, hipStream_t s) {
     constexpr int TILE = col_tile<T, N>();
     constexpr int NT = TILE * SL<N>::tpp;
     dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);
     if (inv)
-        hipLaunchKernelGGL((k_cols_f<T, N, TILE, true>), grid, dim3(NT), 0, s, in, out, bs, ncols, P, Q, S, tw, scale);
+        hipLaunchKernelGGL((k_cols_f<T, N, TILE, true>), grid, dim3(NT), 0, s, in, out, bs, ncols, P, Q, S, vp, tw,
+                           scale);
     else
-        hipLaunchKernelGGL((k_cols_f<T, N, TILE, false>), grid, dim3(NT), 0, s, in, out, bs, ncols, P, Q, S, tw, scale);
+        hipLaunchKernelGGL((k_cols_f<T, N, TILE, false>), grid, dim3(NT), 0, s, in, out, bs, ncols, P, Q, S, vp, tw,
+                           scale);
+}
+
+template <typename T, int N, int NF>
+static void launch_r2c_nf(const T *in, cpx<T> *out, int nblk, long long in_bs, long long out_bs, long long nrows,
+                          long long opitch, const cpx<T> *tw, T scale, const Prologue &pro, double *partials,
+                          int sum_slot, hipStream_t s) {
+    constexpr int NP = r2c_pens<T, N, NF>();
+    const long long tiles = (nrows + 2 * NP - 1) / (2 * NP);
+    OFDFT_REQUIRE(tiles <= 65535, "too many rows for the R2C pass");
+    dim3 grid((unsigned)nblk, (unsigned)tiles);
+    hipLaunchKernelGGL((k_rows_r2c_f<T, N, NP, NF>), grid, dim3(NF * NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs,
+                       nrows, opitch, tw, scale, pro, partials, sum_slot);
 }
 
 template <typename T, int N>
 static void launch_r2c(const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, long long nrows,
-                       const cpx<T> *tw, T scale, const Prologue &pro, double *partials, int sum_slot,
-                       hipStream_t s) {
-    constexpr int NP = row_pens<T, N>();
-    dim3 grid((unsigned)batch, (unsigned)((nrows + 2 * NP - 1) / (2 * NP)));
-    hipLaunchKernelGGL((k_rows_r2c_f<T, N, NP>), grid, dim3(NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs, nrows, tw,
-                       scale, pro, partials, sum_slot);
+                       long long opitch, const cpx<T> *tw, T scale, const Prologue &pro, double *partials,
+                       int sum_slot, hipStream_t s) {
+    if (batch == 3)
+        launch_r2c_nf<T, N, 3>(in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
+    else if (batch == 2)
+        launch_r2c_nf<T, N, 2>(in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
+    else
+        launch_r2c_nf<T, N, 1>(in, out, batch, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
 }
 
 template <typename T, int N>
 static void launch_c2r(const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs, long long nrows,
-                       const cpx<T> *tw, T scale, hipStream_t s) {
+                       long long ipitch, const cpx<T> *tw, T scale, hipStream_t s) {
     constexpr int NP = row_pens<T, N>();
     dim3 grid((unsigned)batch, (unsigned)((nrows + 2 * NP - 1) / (2 * NP)));
-    hipLaunchKernelGGL((k_rows_c2r_f<T, N, NP>), grid, dim3(NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs, nrows, tw,
-                       scale);
+    hipLaunchKernelGGL((k_rows_c2r_f<T, N, NP>), grid, dim3(NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs, nrows,
+                       ipitch, tw, scale);
 }
 
 #define OFDFT_FIXED_SIZES(X) X(32) X(48) X(64) X(72) X(80) X(96) X(100) X(120) X(128) X(144) X(160) X(192) X(200) X(240) X(256)
@@ -377,12 +405,12 @@ bool fixed_size(int n) {
 
 template <typename T>
 bool cols_fixed(int n, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,
-                long long Q, long long S, const void *tw, double scale, hipStream_t s) {
+                long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s) {
     switch (n) {
-#define CASE(N)                                                                                                 \
-    case N:                                                                                                     \
-        launch_cols<T, N>(inv, in, out, batch, bs, ncols, P, Q, S, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
-                          s);                                                                                   \
+#define CASE(N)                                                                                                     \
+    case N:                                                                                                         \
+        launch_cols<T, N>(inv, in, out, batch, bs, ncols, P, Q, S, vp, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
+                          s);                                                                                       \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -392,12 +420,13 @@ bool cols_fixed(int n, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long 
 
 template <typename T>
 bool r2c_fixed(int n, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               const void *tw, double scale, const Prologue &pro, double *partials, int sum_slot, hipStream_t s) {
+               long long opitch, const void *tw, double scale, const Prologue &pro, double *partials, int sum_slot,
+               hipStream_t s) {
     switch (n) {
-#define CASE(N)                                                                                                \
-    case N:                                                                                                    \
-        launch_r2c<T, N>(in, out, batch, in_bs, out_bs, nrows, reinterpret_cast<const cpx<T> *>(tw), (T)scale, pro, \
-                         partials, sum_slot, s);                                                               \
+#define CASE(N)                                                                                                      \
+    case N:                                                                                                          \
+        launch_r2c<T, N>(in, out, batch, in_bs, out_bs, nrows, opitch, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
+                         pro, partials, sum_slot, s);                                                                \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -407,11 +436,12 @@ bool r2c_fixed(int n, const T *in, cpx<T> *out, int batch, long long in_bs, long
 
 template <typename T>
 bool c2r_fixed(int n, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               const void *tw, double scale, hipStream_t s) {
+               long long ipitch, const void *tw, double scale, hipStream_t s) {
     switch (n) {
-#define CASE(N)                                                                                                   \
-    case N:                                                                                                       \
-        launch_c2r<T, N>(in, out, batch, in_bs, out_bs, nrows, reinterpret_cast<const cpx<T> *>(tw), (T)scale, s); \
+#define CASE(N)                                                                                                       \
+    case N:                                                                                                           \
+        launch_c2r<T, N>(in, out, batch, in_bs, out_bs, nrows, ipitch, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
+                         s);                                                                                          \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -419,12 +449,19 @@ bool c2r_fixed(int n, const cpx<T> *in, T *out, int batch, long long in_bs, long
     }
 }
 
-// number of row tiles of the fixed R2C kernel (its partial-sum count)
-int r2c_fixed_tiles(int n, int prec, long long nrows) {
+// number of row tiles of the fixed R2C kernel for `batch` spectra (its partial-sum count)
+int r2c_fixed_tiles(int n, int prec, int batch, long long nrows) {
     int np = 1;
     switch (n) {
-#define CASE(N) \
-    case N: np = prec == OFDFT_FP64 ? row_pens<double, N>() : row_pens<float, N>(); break;
+#define CASE(N)                                                                                      \
+    case N:                                                                                          \
+        if (batch == 3)                                                                              \
+            np = prec == OFDFT_FP64 ? r2c_pens<double, N, 3>() : r2c_pens<float, N, 3>();            \
+        else if (batch == 2)                                                                         \
+            np = prec == OFDFT_FP64 ? r2c_pens<double, N, 2>() : r2c_pens<float, N, 2>();            \
+        else                                                                                         \
+            np = prec == OFDFT_FP64 ? r2c_pens<double, N, 1>() : r2c_pens<float, N, 1>();            \
+        break;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
         default: return -1;
@@ -434,11 +471,11 @@ int r2c_fixed_tiles(int n, int prec, long long nrows) {
 
 #define INST(T)                                                                                                      \
     template bool cols_fixed<T>(int, bool, const cpx<T> *, cpx<T> *, int, long long, long long, long long, long long, \
-                                long long, const void *, double, hipStream_t);                                       \
-    template bool r2c_fixed<T>(int, const T *, cpx<T> *, int, long long, long long, long long, const void *, double,  \
-                               const Prologue &, double *, int, hipStream_t);                                        \
-    template bool c2r_fixed<T>(int, const cpx<T> *, T *, int, long long, long long, long long, const void *, double,  \
-                               hipStream_t);
+                                long long, long long, const void *, double, hipStream_t);                            \
+    template bool r2c_fixed<T>(int, const T *, cpx<T> *, int, long long, long long, long long, long long, const void *, \
+                               double, const Prologue &, double *, int, hipStream_t);                                \
+    template bool c2r_fixed<T>(int, const cpx<T> *, T *, int, long long, long long, long long, long long, const void *, \
+                               double, hipStream_t);
 INST(float)
 INST(double)
 #undef INST

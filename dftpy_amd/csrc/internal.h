@@ -66,6 +66,7 @@ struct Prologue {
 struct Plan {
     int nr[3];
     int nzc;
+    int nzp;  // padded row pitch of the pipeline spectra (128-byte multiple)
     long long N, Nc, Nfull;
     double lat[9];
     double B[9];  // reciprocal lattice rows b_i (grid.py:405-414)
@@ -97,24 +98,29 @@ struct Plan {
     bool use_fixed = true;  // size-specialised FFT kernels where available
 };
 
-int r2c_num_partials(const Plan &P);
+int r2c_num_partials(const Plan &P, int batch);
+// row pitch of the evaluation pipeline's spectra (128-byte aligned rows when
+// every axis has a size-specialised kernel, else nz/2+1)
+long long pipe_pitch(const Plan &P);
 
 // fft.hip
 void plan_fft_setup(Plan &P);
 void plan_fft_free(Plan &P);
+// `pitch` is the row pitch (complex elements) of the half spectra; <= 0 means
+// the unpadded nz/2+1 layout of the public API
 template <typename T>
 void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
-             double scale, const Prologue *pro, double *sum_partials, hipStream_t s);
+             double scale, const Prologue *pro, double *sum_partials, hipStream_t s, long long pitch = 0);
 template <typename T>
 void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long long in_bs,
-             long long tmp_bs, long long out_bs, double scale, hipStream_t s);
+             long long tmp_bs, long long out_bs, double scale, hipStream_t s, long long pitch = 0);
 template <typename T>
 void fft_c2c(Plan &P, const cpx<T> *in, cpx<T> *out, bool inverse, int batch, long long bs, double scale,
              hipStream_t s);
 // the x/y passes alone (in place allowed), for fused pipelines
 template <typename T>
 void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in, cpx<T> *out, int batch,
-              long long bs, double scale, hipStream_t s);
+              long long bs, double scale, hipStream_t s, long long pitch = 0);
 
 // single passes, for timing
 template <typename T>
