@@ -33,6 +33,7 @@ SIGNATURES = {
     "ofdft_last_error": (ctypes.c_char_p, []),
     "ofdft_plan_create": (_I, [_IP, _DP, _I, _I, _I, ctypes.POINTER(_P)]),
     "ofdft_plan_destroy": (_I, [_P]),
+    "ofdft_plan_set_lattice": (_I, [_P, _DP]),
     "ofdft_plan_workspace_bytes": (ctypes.c_size_t, [_P]),
     "ofdft_plan_set_fft_kernels": (_I, [_P, _I]),
     "ofdft_fft_r2c": (_I, [_P, _P, _P, _I, _P]),
@@ -51,6 +52,19 @@ SIGNATURES = {
     "ofdft_optimize": (_I, [_P, _P, _P, _I, _I, _DP, _DP, _DP, _I, _P]),
     "ofdft_dcsrch": (_I, [_DP, _D, _D, _D, _D, _D, _D, _D, _IP, _DP]),
     "ofdft_cn_step": (_I, [_P, _P, _P, _D, _D, _D, _I, _P, _DP, _P]),
+    "ofdft_cnn_conv_in": (_I, [_IP, _I, _P, _P, _P, _P, _P]),
+    "ofdft_cnn_conv_mid": (_I, [_IP, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "ofdft_cnn_conv_out": (_I, [_IP, _I, _P, _P, _P, _P, _P, _P]),
+    "ofdft_cnn_out_backward": (_I, [_IP, _I, _P, _P, _P, _P, _P, _P]),
+    "ofdft_cnn_wgrad_partials": (ctypes.c_size_t, [_I, _I]),
+    "ofdft_cnn_wgrad": (_I, [_IP, _I, _I, _P, _P, _P, _I, _P, _P]),
+    "ofdft_cnn_prep_weights": (_I, [_P, _P, _P, _P]),
+    "ofdft_cnn_loss": (_I, [_P, _P, _P, _P, _P, _P]),
+    "ofdft_cnn_loss_spectral": (_I, [_P, _P, _P, _P, _P, _P, _P]),
+    "ofdft_cnn_adam": (_I, [ctypes.c_longlong, _P, _P, _P, _P, _D, _D, _D, _D, _I, _P]),
+    "ofdft_cnn_features": (_I, [_P, _P, _P, _D, _D, _D, _P, _P]),
+    "ofdft_cnn_target": (_I, [_P, _P, _P, _D, _P, _P]),
+    "ofdft_cnn_apply": (_I, [_P, _P, _P, _D, _D, _P, _P]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

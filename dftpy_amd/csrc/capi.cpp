@@ -52,6 +52,21 @@ static void inv3(const double *a, double *r) {
     r[8] = (a[0] * a[4] - a[1] * a[3]) / det;
 }
 
+// lattice rows a_i (bohr) -> B = 2 pi inv(A)^T (grid.py:405-414), volume, dV
+static void set_lattice(Plan &P, const double *lattice) {
+    std::memcpy(P.lat, lattice, sizeof(P.lat));
+    double inv[9];
+    inv3(lattice, inv);
+    // row i of B is column i of inv(A)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P.B[3 * i + j] = 2.0 * M_PI * inv[3 * j + i];
+    const double *a = lattice;
+    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                       a[2] * (a[3] * a[7] - a[4] * a[6]);
+    P.volume = std::fabs(det);
+    P.dV = P.volume / (double)P.N;
+}
+
 static hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 void plan_aux_setup(Plan &P);
@@ -89,17 +104,7 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             P->N = (long long)nr[0] * nr[1] * nr[2];
             P->Nc = (long long)nr[0] * nr[1] * P->nzc;
             P->Nfull = P->N;
-            std::memcpy(P->lat, lattice, sizeof(P->lat));
-            double inv[9];
-            inv3(lattice, inv);
-            // B = 2 pi inv(A)^T: row i of B is column i of inv(A)
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) P->B[3 * i + j] = 2.0 * M_PI * inv[3 * j + i];
-            const double *a = lattice;
-            const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
-                               a[2] * (a[3] * a[7] - a[4] * a[6]);
-            P->volume = std::fabs(det);
-            P->dV = P->volume / (double)P->N;
+            set_lattice(*P, lattice);
             P->prec = precision;
             P->full = full ? 1 : 0;
             P->max_batch = max_batch < 1 ? 1 : max_batch;
@@ -126,6 +131,13 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             throw;
         }
         *out = P;
+    });
+}
+
+int ofdft_plan_set_lattice(ofdft_plan *plan, const double lattice[9]) {
+    return guard([&] {
+        OFDFT_REQUIRE(plan != nullptr && lattice != nullptr, "plan_set_lattice: null pointer");
+        set_lattice(*plan, lattice);
     });
 }
 

@@ -1,0 +1,953 @@
+// Learned warm start: a periodic residual 3D CNN on the density grid.
+//
+// No counterpart in the reference.  The network maps a cheap start density
+// (superposition features) to a log-density correction; the optimizer of
+// optimization.py:281-520 then runs from the corrected density.
+//
+// Layout: hidden fields are channels-last, (N points) x 32 channels, bfloat16
+// (64 bytes per point); the two input channels and the output are planar FP32
+// fields.  Tap t = (i*3 + j)*3 + k reads the input at p + d*(i-1, j-1, k-1)
+// with periodic wrap (d = dilation of the layer).
+//
+//   h1 = relu(conv(x0; W1) + b1)                          FP32 FMA (K = 54)
+//   h_l = h_{l-1} + relu(conv(h_{l-1}; W_l) + b_l), l=2..5  BF16 MFMA, FP32 acc
+//   out = conv(h5; W6) + b6                               FP32 FMA (K = 864)
+//
+// The 32 -> 32 layers are an implicit GEMM per tap: D[p][co] += X[p+off][ci]
+// W[co][ci] with v_mfma_f32_16x16x32_bf16 (16 points x 16 channels x 32 input
+// channels).  Backward-data is the same kernel with the weights transposed and
+// the taps mirrored; the weight gradient is a second MFMA kernel reducing over
+// points (K = points) through LDS-transposed tiles.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+#include "reduce.h"
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+namespace ofdft {
+
+constexpr int CH = 32;
+constexpr int NTAP = 27;
+constexpr int KMID = NTAP * CH * CH;  // weights of one 32 -> 32 layer
+constexpr int KIN = NTAP * 2 * CH;
+constexpr int KOUT = NTAP * CH;
+
+struct Geo {
+    int nx, ny, nz, d;
+    long long n;
+};
+
+__device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// index of p + d*(i-1, j-1, k-1) for the point (x, y, z)
+struct Nbr {
+    int bx[3], by[3], bz[3];
+    __device__ __forceinline__ void init(const Geo &g, int x, int y, int z) {
+        const int nyz = g.ny * g.nz;
+        bx[0] = wrapi(x - g.d, g.nx) * nyz;
+        bx[1] = x * nyz;
+        bx[2] = wrapi(x + g.d, g.nx) * nyz;
+        by[0] = wrapi(y - g.d, g.ny) * g.nz;
+        by[1] = y * g.nz;
+        by[2] = wrapi(y + g.d, g.ny) * g.nz;
+        bz[0] = wrapi(z - g.d, g.nz);
+        bz[1] = z;
+        bz[2] = wrapi(z + g.d, g.nz);
+    }
+    __device__ __forceinline__ int at(int t) const { return bx[t / 9] + by[(t / 3) % 3] + bz[t % 3]; }
+};
+
+__device__ __forceinline__ void decompose(const Geo &g, long long p, int &x, int &y, int &z) {
+    z = (int)(p % g.nz);
+    const long long r = p / g.nz;
+    y = (int)(r % g.ny);
+    x = (int)(r / g.ny);
+}
+
+// ------------------------------------------------------------ layer 1 (2 -> 32)
+// The weights are read with workgroup-uniform addresses (scalar loads).
+__global__ void __launch_bounds__(256) k_cnn_in(Geo g, const float *__restrict__ x0, const float *__restrict__ w,
+                                                const float *__restrict__ b, __bf16 *__restrict__ h) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n) return;
+    int x, y, z;
+    decompose(g, p, x, y, z);
+    Nbr nb;
+    nb.init(g, x, y, z);
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = b[c];
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+        const int q = nb.at(t);
+        const float v0 = x0[q], v1 = x0[g.n + q];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += w[t * 64 + c] * v0 + w[t * 64 + 32 + c] * v1;
+    }
+    bf16x8 *dst = reinterpret_cast<bf16x8 *>(h + p * CH);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) o[e] = (__bf16)fmaxf(acc[q * 8 + e], 0.0f);
+        dst[q] = o;
+    }
+}
+
+// ------------------------------------------------------- layers 2..5 (MFMA)
+// MODE 0 (forward):  r = relu(conv(x) + bias); out0 (bf16) = x + r; out1 (bf16) = r
+// MODE 1 (backward): v = conv(x) + resid (fp32); out0 (fp32) = v;
+//                    out1 (bf16) = mask > 0 ? v : 0
+// Workgroup: 8 waves x 64 points; each wave 4 M-tiles of 16 points x 2 N-tiles
+// of 16 output channels.  The weights of the layer (27 x 32 x 32 bf16,
+// 55,296 bytes) are staged in LDS.
+constexpr int MID_THREADS = 512;
+constexpr int MID_POINTS = MID_THREADS;  // 64 points per wave
+
+template <int MODE>
+__global__ void __launch_bounds__(MID_THREADS)
+    k_cnn_mid(Geo g, const __bf16 *__restrict__ xin, const __bf16 *__restrict__ w, const float *__restrict__ bias,
+              const float *__restrict__ resid, const __bf16 *__restrict__ mask, void *__restrict__ out0,
+              __bf16 *__restrict__ out1) {
+    __shared__ bf16x8 wl[NTAP * CH * 4];
+    const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(w);
+    for (int i = threadIdx.x; i < NTAP * CH * 4; i += MID_THREADS) wl[i] = wg[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const long long pbase = (long long)blockIdx.x * MID_POINTS + wave * 64;
+    Nbr nb[4];
+    bool valid[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const long long p = pbase + mt * 16 + m;
+        valid[mt] = p < g.n;
+        int x, y, z;
+        decompose(g, valid[mt] ? p : 0, x, y, z);
+        nb[mt].init(g, x, y, z);
+    }
+    __syncthreads();
+    floatx4 acc[4][2];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt][0] = acc[mt][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
+    const bf16x8 zero8 = {};
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+        const bf16x8 b0 = wl[(t * CH + m) * 4 + kq];
+        const bf16x8 b1 = wl[(t * CH + 16 + m) * 4 + kq];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const bf16x8 a = valid[mt] ? xv[(long long)nb[mt].at(t) * 4 + kq] : zero8;
+            acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[mt][0], 0, 0, 0);
+            acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[mt][1], 0, 0, 0);
+        }
+    }
+    // D[row][col]: lane holds rows kq*4 + i (points), column m (channel)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int co = nt * 16 + m;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const long long p = pbase + mt * 16 + kq * 4 + i;
+                if (p >= g.n) continue;
+                const long long e = p * CH + co;
+                float v = acc[mt][nt][i];
+                if (MODE == 0) {
+                    v += bias[co];
+                    const float r = fmaxf(v, 0.0f);
+                    reinterpret_cast<__bf16 *>(out0)[e] = (__bf16)((float)xin[e] + r);
+                    out1[e] = (__bf16)r;
+                } else {
+                    v += resid[e];
+                    reinterpret_cast<float *>(out0)[e] = v;
+                    out1[e] = (float)mask[e] > 0.0f ? (__bf16)v : (__bf16)0.0f;
+                }
+            }
+        }
+    }
+}
+
+// Row-tiled variant for nz % 16 == 0, nz <= 160 and ny % (4 d) == 0: a
+// workgroup owns 4 output rows y0 + k d (k = 0..3) of one x plane, all of z.
+// For each x offset the 6 input rows y0 + (k-1) d of plane x + dx are staged
+// once in LDS (6 nz points x 64 bytes) and the 9 (dy, dz) taps read their A
+// operands from there, so each input point is read from HBM/L2 about 4.5
+// times per layer instead of 27.  Rows in a residue class mod d keep the
+// dilated y taps inside the staged rows.  Weights (B operands) are read from
+// global memory (L1/L2 resident).  8 waves; wave w owns M-tiles w, w+8, ...
+constexpr int ROWS_THREADS = 512;
+constexpr int ROWS_MAX_MT = 5;  // M-tiles per wave for nz = 160 (40 tiles / 8 waves)
+
+template <int MODE, int MT>
+__global__ void __launch_bounds__(ROWS_THREADS)
+    k_cnn_mid_rows(Geo g, const __bf16 *__restrict__ xin, const __bf16 *__restrict__ w,
+                   const float *__restrict__ bias, const float *__restrict__ resid, const __bf16 *__restrict__ mask,
+                   void *__restrict__ out0, __bf16 *__restrict__ out1) {
+    extern __shared__ bf16x8 rows[];  // [6][nz][4]
+    const int nz = g.nz, d = g.d;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const int x = blockIdx.y;
+    const int gy = blockIdx.x;  // row group: residue gy % d, block gy / d
+    const int y0 = (gy / d) * 4 * d + gy % d;
+    const int tiles_per_row = nz / 16;
+    const long long nyz = (long long)g.ny * nz;
+    floatx4 acc[MT][2];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) acc[i][0] = acc[i][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
+    const bf16x8 *wv = reinterpret_cast<const bf16x8 *>(w);
+    const int nload = 6 * nz * 4;
+#pragma unroll 1
+    for (int ix = 0; ix < 3; ++ix) {
+        const int xp = wrapi(x + (ix - 1) * d, g.nx);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nload; e += ROWS_THREADS) {
+            const int r = e / (nz * 4), rem = e - r * nz * 4;
+            const int yr = wrapi(y0 + (r - 1) * d, g.ny);
+            rows[e] = xv[((long long)xp * nyz + (long long)yr * nz) * 4 + rem];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int iy = 0; iy < 3; ++iy) {
+#pragma unroll
+            for (int iz = 0; iz < 3; ++iz) {
+                const int t = (ix * 3 + iy) * 3 + iz;
+                const bf16x8 b0 = wv[(t * CH + m) * 4 + kq];
+                const bf16x8 b1 = wv[(t * CH + 16 + m) * 4 + kq];
+#pragma unroll
+                for (int i = 0; i < MT; ++i) {
+                    const int tile = wave + 8 * i;  // tile = k * tiles_per_row + zt
+                    const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
+                    int z = zt * 16 + m + (iz - 1) * d;
+                    z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+                    const bf16x8 a = rows[((k + iy) * nz + z) * 4 + kq];
+                    acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[i][0], 0, 0, 0);
+                    acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[i][1], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+        const int tile = wave + 8 * i;
+        const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
+        const long long rowbase = (long long)x * nyz + (long long)(y0 + k * d) * nz + zt * 16;
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int co = nt * 16 + m;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const long long e = (rowbase + kq * 4 + j) * CH + co;
+                float v = acc[i][nt][j];
+                if (MODE == 0) {
+                    v += bias[co];
+                    const float r = fmaxf(v, 0.0f);
+                    reinterpret_cast<__bf16 *>(out0)[e] = (__bf16)((float)xin[e] + r);
+                    out1[e] = (__bf16)r;
+                } else {
+                    v += resid[e];
+                    reinterpret_cast<float *>(out0)[e] = v;
+                    out1[e] = (float)mask[e] > 0.0f ? (__bf16)v : (__bf16)0.0f;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ layer 6 (32 -> 1)
+__global__ void __launch_bounds__(256) k_cnn_out(Geo g, const __bf16 *__restrict__ h, const float *__restrict__ w,
+                                                 const float *__restrict__ b, float *__restrict__ out) {
+    __shared__ float ws[KOUT];
+    for (int i = threadIdx.x; i < KOUT; i += blockDim.x) ws[i] = w[i];
+    __syncthreads();
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n) return;
+    int x, y, z;
+    decompose(g, p, x, y, z);
+    Nbr nb;
+    nb.init(g, x, y, z);
+    float acc = b[0];
+    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h);
+#pragma unroll 3
+    for (int t = 0; t < NTAP; ++t) {
+        const long long q = nb.at(t);
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+            const bf16x8 v = hv[q * 4 + c4];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc += (float)v[e] * ws[t * CH + c4 * 8 + e];
+        }
+    }
+    out[p] = acc;
+}
+
+// Output layer as projection + gather: u[t][q] = sum_c W6[t][c] h[q][c]
+// (planar, 27 fields), then out[p] = b + sum_t u[t][p + off_t].  Each hidden
+// point is read once and every u value once, instead of 27 reads of h.
+__global__ void __launch_bounds__(256) k_cnn_out_proj(Geo g, const __bf16 *__restrict__ h,
+                                                      const float *__restrict__ w, float *__restrict__ u) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= g.n) return;
+    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h) + q * 4;
+    float hc[CH];
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+        const bf16x8 v = hv[c4];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) hc[c4 * 8 + e] = (float)v[e];
+    }
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) s += w[t * CH + c] * hc[c];
+        u[(long long)t * g.n + q] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cnn_out_gather(Geo g, const float *__restrict__ u, const float *__restrict__ b,
+                                                        float *__restrict__ out) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n) return;
+    int x, y, z;
+    decompose(g, p, x, y, z);
+    Nbr nb;
+    nb.init(g, x, y, z);
+    float acc = b[0];
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) acc += u[(long long)t * g.n + nb.at(t)];
+    out[p] = acc;
+}
+
+// dh[q][c] = sum_t W6[t][c] dout[q - off_t];  g = mask > 0 ? dh : 0 (bf16)
+__global__ void __launch_bounds__(256) k_cnn_out_bwd(Geo g, const float *__restrict__ dout,
+                                                     const float *__restrict__ w, const __bf16 *__restrict__ mask,
+                                                     float *__restrict__ dh, __bf16 *__restrict__ gout) {
+    __shared__ float ws[KOUT];
+    for (int i = threadIdx.x; i < KOUT; i += blockDim.x) ws[i] = w[i];
+    __syncthreads();
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.n) return;
+    int x, y, z;
+    decompose(g, p, x, y, z);
+    Nbr nb;
+    nb.init(g, x, y, z);
+    float acc[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) acc[c] = 0.0f;
+#pragma unroll 3
+    for (int t = 0; t < NTAP; ++t) {
+        const float dv = dout[nb.at(NTAP - 1 - t)];  // q - off_t = q + off_{26-t}
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] += ws[t * CH + c] * dv;
+    }
+    float4 *dd = reinterpret_cast<float4 *>(dh + p * CH);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dd[q] = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+    const bf16x8 *mv = reinterpret_cast<const bf16x8 *>(mask + p * CH);
+    bf16x8 *gv = reinterpret_cast<bf16x8 *>(gout + p * CH);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bf16x8 mk = mv[q];
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) o[e] = (float)mk[e] > 0.0f ? (__bf16)acc[q * 8 + e] : (__bf16)0.0f;
+        gv[q] = o;
+    }
+}
+
+// --------------------------------------------------------- weight gradients
+constexpr int WG_CHUNK = 256;  // points per chunk
+
+// dW6[t][c] = sum_p dout[p] h[p + off_t][c]; db6 = sum_p dout[p].
+// 128 threads: thread j < 108 owns (t = j / 4, channels 8*(j%4)..+8); j == 108
+// the bias.  Grid-stride over chunks; partials[block][KOUT + 1].
+__global__ void __launch_bounds__(128) k_cnn_wgrad_out(Geo g, const float *__restrict__ dout,
+                                                       const __bf16 *__restrict__ h, float *__restrict__ partials) {
+    __shared__ float ds[WG_CHUNK];
+    const int j = threadIdx.x;
+    const int t = j / 4, cg = j % 4;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    float accb = 0.0f;
+    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h);
+    const long long nchunk = (g.n + WG_CHUNK - 1) / WG_CHUNK;
+    for (long long ck = blockIdx.x; ck < nchunk; ck += gridDim.x) {
+        const long long p0 = ck * WG_CHUNK;
+        __syncthreads();
+        for (int i = j; i < WG_CHUNK; i += 128) ds[i] = (p0 + i < g.n) ? dout[p0 + i] : 0.0f;
+        __syncthreads();
+        const int np = (int)min((long long)WG_CHUNK, g.n - p0);
+        if (j < 108) {
+            int x, y, z;
+            decompose(g, p0, x, y, z);
+            for (int i = 0; i < np; ++i) {
+                Nbr nb;
+                nb.init(g, x, y, z);
+                const bf16x8 v = hv[(long long)nb.at(t) * 4 + cg];
+                const float dv = ds[i];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[e] += dv * (float)v[e];
+                if (++z == g.nz) {
+                    z = 0;
+                    if (++y == g.ny) {
+                        y = 0;
+                        ++x;
+                    }
+                }
+            }
+        } else if (j == 108) {
+            for (int i = 0; i < np; ++i) accb += ds[i];
+        }
+    }
+    float *pp = partials + (long long)blockIdx.x * (KOUT + 1);
+    if (j < 108) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pp[t * CH + cg * 8 + e] = acc[e];
+    } else if (j == 108) {
+        pp[KOUT] = accb;
+    }
+}
+
+// dW1[t][c][co] = sum_p g[p][co] x0[c][p + off_t]; db1[co] = sum_p g[p][co].
+// 256 threads: j < 216 owns (t = j / 8, c = (j / 4) % 2, co group j % 4);
+// 216 <= j < 220 the bias of co group j - 216.  partials[block][KIN + 32].
+__global__ void __launch_bounds__(256) k_cnn_wgrad_in(Geo g, const __bf16 *__restrict__ gin,
+                                                      const float *__restrict__ x0, float *__restrict__ partials) {
+    __shared__ bf16x8 gs[WG_CHUNK * 4];
+    const int j = threadIdx.x;
+    const int t = j / 8, c = (j / 4) % 2, cg = j % 4;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bf16x8 *gv = reinterpret_cast<const bf16x8 *>(gin);
+    const long long nchunk = (g.n + WG_CHUNK - 1) / WG_CHUNK;
+    const bf16x8 zero8 = {};
+    for (long long ck = blockIdx.x; ck < nchunk; ck += gridDim.x) {
+        const long long p0 = ck * WG_CHUNK;
+        __syncthreads();
+        for (int i = j; i < WG_CHUNK * 4; i += 256) gs[i] = (p0 + i / 4 < g.n) ? gv[p0 * 4 + i] : zero8;
+        __syncthreads();
+        const int np = (int)min((long long)WG_CHUNK, g.n - p0);
+        if (j < 216) {
+            int x, y, z;
+            decompose(g, p0, x, y, z);
+            const float *xc = x0 + (long long)c * g.n;
+            for (int i = 0; i < np; ++i) {
+                Nbr nb;
+                nb.init(g, x, y, z);
+                const float xv = xc[nb.at(t)];
+                const bf16x8 gg = gs[i * 4 + cg];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[e] += xv * (float)gg[e];
+                if (++z == g.nz) {
+                    z = 0;
+                    if (++y == g.ny) {
+                        y = 0;
+                        ++x;
+                    }
+                }
+            }
+        } else if (j < 220) {
+            for (int i = 0; i < np; ++i) {
+                const bf16x8 gg = gs[i * 4 + (j - 216)];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[e] += (float)gg[e];
+            }
+        }
+    }
+    float *pp = partials + (long long)blockIdx.x * (KIN + CH);
+    if (j < 216) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pp[(t * 2 + c) * CH + cg * 8 + e] = acc[e];
+    } else if (j < 220) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pp[KIN + (j - 216) * 8 + e] = acc[e];
+    }
+}
+
+// Mid-layer weight gradient with MFMA: dW[t][co][ci] = sum_p g[p][co] x[p+off_t][ci].
+// 4 waves; wave w owns the 16x16 block (co block w>>1, ci block w&1) of all 27
+// taps.  Per chunk of 256 points: g is staged transposed in LDS (gT[co][p])
+// once, and for each tap the shifted x rows are staged transposed (xT[ci][p]);
+// the MFMA K dimension runs over points.  partials[block][KMID + 32].
+constexpr int WG_PITCH = WG_CHUNK + 8;  // bf16 elements per LDS row (16-byte aligned)
+
+__global__ void __launch_bounds__(256) k_cnn_wgrad_mid(Geo g, const __bf16 *__restrict__ gin,
+                                                       const __bf16 *__restrict__ xin, float *__restrict__ partials) {
+    __shared__ __attribute__((aligned(16))) __bf16 gT[CH * WG_PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 xT[CH * WG_PITCH];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kq = lane >> 4;
+    const int mt = wave >> 1, nt = wave & 1;
+    floatx4 acc[NTAP];
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float dbacc = 0.0f;
+    const bf16x8 *gv = reinterpret_cast<const bf16x8 *>(gin);
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
+    const long long nchunk = (g.n + WG_CHUNK - 1) / WG_CHUNK;
+    const bf16x8 zero8 = {};
+    for (long long ck = blockIdx.x; ck < nchunk; ck += gridDim.x) {
+        const long long p = ck * WG_CHUNK + tid;  // this thread's point
+        const bool valid = p < g.n;
+        int x, y, z;
+        decompose(g, valid ? p : 0, x, y, z);
+        Nbr nb;
+        nb.init(g, x, y, z);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bf16x8 v = valid ? gv[p * 4 + q] : zero8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) gT[(q * 8 + e) * WG_PITCH + tid] = v[e];
+        }
+        __syncthreads();
+        {  // bias gradient: thread (c = tid / 8, part = tid % 8) sums 32 points of channel c
+            const int c = tid >> 3, part = tid & 7;
+            float s = 0.0f;
+#pragma unroll 8
+            for (int i = 0; i < 32; ++i) s += (float)gT[c * WG_PITCH + part * 32 + i];
+            dbacc += s;
+        }
+        bf16x8 a[WG_CHUNK / 32];
+#pragma unroll
+        for (int ks = 0; ks < WG_CHUNK / 32; ++ks)
+            a[ks] = *reinterpret_cast<const bf16x8 *>(&gT[(mt * 16 + m) * WG_PITCH + ks * 32 + kq * 8]);
+        bf16x8 xr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xr[q] = valid ? xv[(long long)nb.at(0) * 4 + q] : zero8;
+#pragma unroll
+        for (int t = 0; t < NTAP; ++t) {
+            __syncthreads();  // previous tap's readers are done with xT
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) xT[(q * 8 + e) * WG_PITCH + tid] = xr[q][e];
+            __syncthreads();
+            if (t + 1 < NTAP) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) xr[q] = valid ? xv[(long long)nb.at(t + 1) * 4 + q] : zero8;
+            }
+#pragma unroll
+            for (int ks = 0; ks < WG_CHUNK / 32; ++ks) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&xT[(nt * 16 + m) * WG_PITCH + ks * 32 + kq * 8]);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    float *pp = partials + (long long)blockIdx.x * (KMID + CH);
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pp[t * CH * CH + (mt * 16 + kq * 4 + i) * CH + nt * 16 + m] = acc[t][i];
+    // bias: 8 partial sums per channel, reduced in LDS
+    __syncthreads();
+    float *red = reinterpret_cast<float *>(gT);
+    red[tid] = dbacc;
+    __syncthreads();
+    if (tid < CH) {
+        float s = 0.0f;
+        for (int part = 0; part < 8; ++part) s += red[tid * 8 + part];
+        pp[KMID + tid] = s;
+    }
+}
+
+// out[j] = sum_b partials[b * K + j]  (FP64 accumulation)
+__global__ void k_cnn_reduce(const float *__restrict__ partials, int nblk, int K, float *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= K) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += (double)partials[(long long)b * K + j];
+    out[j] = (float)s;
+}
+
+// wf[t][co][ci] = bf16(W[t][co][ci]);  wb[t][ci][co] = bf16(W[26-t][co][ci])
+__global__ void k_cnn_prep(const float *__restrict__ w, __bf16 *__restrict__ wf, __bf16 *__restrict__ wb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= KMID) return;
+    const int t = i / (CH * CH), co = (i / CH) % CH, ci = i % CH;
+    wf[i] = (__bf16)w[i];
+    wb[((NTAP - 1 - t) * CH + ci) * CH + co] = (__bf16)w[i];
+}
+
+// d = pred - target; dpred = 2 d / n; partial sums of d^2
+__global__ void k_cnn_loss(long long n, const float *__restrict__ pred, const float *__restrict__ target,
+                           float *__restrict__ dpred, double *partials) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    const float s = 2.0f / (float)n;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float d = pred[i] - target[i];
+        dpred[i] = s * d;
+        acc += (double)d * (double)d;
+    }
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// d = pred - target
+__global__ void k_cnn_diff(long long n, const float *__restrict__ a, const float *__restrict__ b, float *__restrict__ d) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        d[i] = a[i] - b[i];
+}
+
+// Spectral loss: D *= w (half spectrum, dense nzc rows), partial sums of
+// m(kz) w |D|^2 with m = 1 at kz = 0 and the Nyquist column, else 2
+__global__ void k_cnn_metric(long long nc, int nzc, int nz, float2 *__restrict__ D, const float *__restrict__ w,
+                             double *partials) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
+        const int kz = (int)(i % nzc);
+        const float wi = w[i];
+        const float2 v = D[i];
+        const double m = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
+        acc += m * (double)wi * ((double)v.x * v.x + (double)v.y * v.y);
+        D[i] = make_float2(v.x * wi, v.y * wi);
+    }
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// Adam (Kingma & Ba 2015) with bias correction, FP32 master weights
+__global__ void k_cnn_adam(long long n, float *__restrict__ p, const float *__restrict__ gr, float *__restrict__ m,
+                           float *__restrict__ v, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float g = gr[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * g;
+        const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+
+// Input features: x0[0] = c (rho_s - rho0) / rho0, x0[1] = sv * vloc
+template <typename T>
+__global__ void k_cnn_features(long long n, const T *__restrict__ rho_s, const T *__restrict__ vloc, double rho0,
+                               double c, double sv, float *__restrict__ x0) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        x0[i] = (float)(c * ((double)rho_s[i] - rho0) / rho0);
+        x0[n + i] = (float)(sv * (double)vloc[i]);
+    }
+}
+
+// Training target: log(rho / rho0) - x0[0]
+template <typename T>
+__global__ void k_cnn_target(long long n, const T *__restrict__ rho, const float *__restrict__ x0, double rho0,
+                             float *__restrict__ target) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        target[i] = (float)(log((double)rho[i] / rho0) - (double)x0[i]);
+}
+
+// rho = rho0 exp(x0[0] + delta), with the FP64 sum of rho in partials
+template <typename T>
+__global__ void k_cnn_apply(long long n, const float *__restrict__ x0, const float *__restrict__ delta, double rho0,
+                            T *__restrict__ rho, double *partials) {
+    __shared__ double sm[16];
+    double acc = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double r = rho0 * exp((double)x0[i] + (double)(delta ? delta[i] : 0.0f));
+        rho[i] = (T)r;
+        acc += r;
+    }
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// rho *= ne / (sum * dV)
+template <typename T> __global__ void k_cnn_normalise(long long n, T *rho, const double *sum, double ne, double dV) {
+    const double f = ne / (*sum * dV);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        rho[i] = (T)((double)rho[i] * f);
+}
+
+static Geo make_geo(const int *nr, int dil) {
+    OFDFT_REQUIRE(nr != nullptr && nr[0] > 0 && nr[1] > 0 && nr[2] > 0, "bad grid shape");
+    OFDFT_REQUIRE(dil >= 1 && dil < nr[0] && dil < nr[1] && dil < nr[2], "dilation must be in [1, min(nr))");
+    Geo g;
+    g.nx = nr[0];
+    g.ny = nr[1];
+    g.nz = nr[2];
+    g.d = dil;
+    g.n = (long long)nr[0] * nr[1] * nr[2];
+    OFDFT_REQUIRE(g.n < (1LL << 31), "grid too large for 32-bit point indices");
+    return g;
+}
+
+static unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+static int cnn_guard_impl(void (*f)(void *), void *ctx) {
+    try {
+        f(ctx);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    } catch (...) {
+        set_error("unknown error");
+        return 1;
+    }
+}
+
+template <typename F> static int cnn_guard(F &&f) {
+    return cnn_guard_impl([](void *c) { (*reinterpret_cast<F *>(c))(); }, &f);
+}
+
+static hipStream_t St(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace ofdft
+
+using namespace ofdft;
+
+struct ofdft_plan : public Plan {};
+
+extern "C" {
+
+int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w, const float *b, void *h,
+                      void *stream) {
+    return cnn_guard([&] {
+        const Geo g = make_geo(nr, dil);
+        OFDFT_REQUIRE(x0 && w && b && h, "null pointer");
+        hipLaunchKernelGGL(k_cnn_in, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g, x0, w, b,
+                           reinterpret_cast<__bf16 *>(h));
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const void *w, const float *bias,
+                       const float *resid, const void *mask, void *out0, void *out1, void *stream) {
+    return cnn_guard([&] {
+        const Geo g = make_geo(nr, dil);
+        OFDFT_REQUIRE(x && w && out0 && out1, "null pointer");
+        const __bf16 *xb = reinterpret_cast<const __bf16 *>(x);
+        const __bf16 *wb = reinterpret_cast<const __bf16 *>(w);
+        OFDFT_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (forward) or 1 (backward-data)");
+        OFDFT_REQUIRE(mode == 1 || bias, "forward mode needs the bias");
+        OFDFT_REQUIRE(mode == 0 || (resid && mask), "backward mode needs resid and mask");
+        const int tiles = 4 * g.nz / 16;
+        if (g.nz % 16 == 0 && g.nz <= 160 && g.ny % (4 * g.d) == 0 && tiles % 8 == 0 && tiles / 8 <= ROWS_MAX_MT) {
+            const dim3 grid((unsigned)(g.ny / 4), (unsigned)g.nx);
+            const size_t lds = (size_t)6 * g.nz * CH * sizeof(__bf16);
+            const int mt = tiles / 8;
+#define ROWS_LAUNCH(MTV)                                                                                            \
+    do {                                                                                                            \
+        if (mode == 0)                                                                                              \
+            hipLaunchKernelGGL((k_cnn_mid_rows<0, MTV>), grid, dim3(ROWS_THREADS), lds, St(stream), g, xb, wb, bias, \
+                               nullptr, nullptr, out0, reinterpret_cast<__bf16 *>(out1));                            \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_cnn_mid_rows<1, MTV>), grid, dim3(ROWS_THREADS), lds, St(stream), g, xb, wb,      \
+                               nullptr, resid, reinterpret_cast<const __bf16 *>(mask), out0,                         \
+                               reinterpret_cast<__bf16 *>(out1));                                                   \
+    } while (0)
+            switch (mt) {
+                case 1: ROWS_LAUNCH(1); break;
+                case 2: ROWS_LAUNCH(2); break;
+                case 3: ROWS_LAUNCH(3); break;
+                case 4: ROWS_LAUNCH(4); break;
+                default: ROWS_LAUNCH(5); break;
+            }
+#undef ROWS_LAUNCH
+            OFDFT_HIP(hipGetLastError());
+            return;
+        }
+        const dim3 grid(blocks_for(g.n, MID_POINTS));
+        if (mode == 0) {
+            OFDFT_REQUIRE(bias, "forward mode needs the bias");
+            hipLaunchKernelGGL(k_cnn_mid<0>, grid, dim3(MID_THREADS), 0, St(stream), g, xb, wb, bias, nullptr, nullptr,
+                               out0, reinterpret_cast<__bf16 *>(out1));
+        } else if (mode == 1) {
+            OFDFT_REQUIRE(resid && mask, "backward mode needs resid and mask");
+            hipLaunchKernelGGL(k_cnn_mid<1>, grid, dim3(MID_THREADS), 0, St(stream), g, xb, wb, nullptr, resid,
+                               reinterpret_cast<const __bf16 *>(mask), out0, reinterpret_cast<__bf16 *>(out1));
+        } else {
+            OFDFT_REQUIRE(false, "mode must be 0 (forward) or 1 (backward-data)");
+        }
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, const float *b, float *out,
+                       float *work, void *stream) {
+    return cnn_guard([&] {
+        const Geo g = make_geo(nr, dil);
+        OFDFT_REQUIRE(h && w && b && out, "null pointer");
+        if (work) {
+            hipLaunchKernelGGL(k_cnn_out_proj, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g,
+                               reinterpret_cast<const __bf16 *>(h), w, work);
+            OFDFT_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_cnn_out_gather, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g, work, b, out);
+        } else {
+            hipLaunchKernelGGL(k_cnn_out, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g,
+                               reinterpret_cast<const __bf16 *>(h), w, b, out);
+        }
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_out_backward(const int nr[3], int dil, const float *dout, const float *w, const void *mask, float *dh,
+                           void *gout, void *stream) {
+    return cnn_guard([&] {
+        const Geo g = make_geo(nr, dil);
+        OFDFT_REQUIRE(dout && w && mask && dh && gout, "null pointer");
+        hipLaunchKernelGGL(k_cnn_out_bwd, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g, dout, w,
+                           reinterpret_cast<const __bf16 *>(mask), dh, reinterpret_cast<__bf16 *>(gout));
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+size_t ofdft_cnn_wgrad_partials(int layer, int nblk) {
+    const int K = layer == 0 ? KIN + CH : (layer == 1 ? KMID + CH : KOUT + 1);
+    return (size_t)nblk * K;
+}
+
+int ofdft_cnn_wgrad(const int nr[3], int dil, int layer, const void *g_in, const void *x, float *partials, int nblk,
+                    float *dw, void *stream) {
+    return cnn_guard([&] {
+        const Geo g = make_geo(nr, dil);
+        OFDFT_REQUIRE(g_in && x && partials && dw, "null pointer");
+        OFDFT_REQUIRE(nblk >= 1 && nblk <= 65535, "nblk out of range");
+        int K = 0;
+        if (layer == 0) {
+            K = KIN + CH;
+            hipLaunchKernelGGL(k_cnn_wgrad_in, dim3(nblk), dim3(256), 0, St(stream), g,
+                               reinterpret_cast<const __bf16 *>(g_in), reinterpret_cast<const float *>(x), partials);
+        } else if (layer == 1) {
+            K = KMID + CH;
+            hipLaunchKernelGGL(k_cnn_wgrad_mid, dim3(nblk), dim3(256), 0, St(stream), g,
+                               reinterpret_cast<const __bf16 *>(g_in), reinterpret_cast<const __bf16 *>(x), partials);
+        } else if (layer == 2) {
+            K = KOUT + 1;
+            hipLaunchKernelGGL(k_cnn_wgrad_out, dim3(nblk), dim3(128), 0, St(stream), g,
+                               reinterpret_cast<const float *>(g_in), reinterpret_cast<const __bf16 *>(x), partials);
+        } else {
+            OFDFT_REQUIRE(false, "layer must be 0 (input), 1 (32->32) or 2 (output)");
+        }
+        OFDFT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_cnn_reduce, dim3(blocks_for(K, 256)), dim3(256), 0, St(stream), partials, nblk, K, dw);
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_prep_weights(const float *w, void *wf, void *wb, void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(w && wf && wb, "null pointer");
+        hipLaunchKernelGGL(k_cnn_prep, dim3(blocks_for(KMID, 256)), dim3(256), 0, St(stream), w,
+                           reinterpret_cast<__bf16 *>(wf), reinterpret_cast<__bf16 *>(wb));
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_loss(ofdft_plan *plan, const float *pred, const float *target, float *dpred, double *loss,
+                   void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(plan && pred && target && dpred && loss, "null pointer");
+        const int nb = reduce_blocks(plan->N);
+        hipLaunchKernelGGL(k_cnn_loss, dim3(nb), dim3(256), 0, St(stream), plan->N, pred, target, dpred, plan->red);
+        OFDFT_HIP(hipGetLastError());
+        const double s = 1.0 / (double)plan->N;
+        reduce_final(plan->red, nb, 1, loss, &s, St(stream));
+    });
+}
+
+int ofdft_cnn_loss_spectral(ofdft_plan *plan, const float *pred, const float *target, const float *w, float *dpred,
+                            double *loss, void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(plan && pred && target && w && dpred && loss, "null pointer");
+        OFDFT_REQUIRE(plan->prec == OFDFT_FP32, "spectral loss needs an FP32 plan");
+        Plan &P = *plan;
+        const hipStream_t s = St(stream);
+        float *d = reinterpret_cast<float *>(P.work);
+        float2 *D = reinterpret_cast<float2 *>(P.spec);
+        const int nb = reduce_blocks(P.N);
+        hipLaunchKernelGGL(k_cnn_diff, dim3(std::min<unsigned>(blocks_for(P.N, 256), 4096u)), dim3(256), 0, s, P.N,
+                           pred, target, d);
+        OFDFT_HIP(hipGetLastError());
+        fft_r2c<float>(P, d, D, 1, P.N, P.Nc, 1.0, nullptr, nullptr, s, 0);
+        const int nbc = reduce_blocks(P.Nc);
+        hipLaunchKernelGGL(k_cnn_metric, dim3(nbc), dim3(256), 0, s, P.Nc, P.nzc, P.nr[2], D, w, P.red);
+        OFDFT_HIP(hipGetLastError());
+        const double inv_n2 = 1.0 / ((double)P.N * (double)P.N);
+        reduce_final(P.red, nbc, 1, loss, &inv_n2, s);
+        // d loss / d pred = (2 / N^2) sum_q w D e^{iqr}: unnormalised C2R scaled by 2 / N^2
+        fft_c2r<float>(P, D, D, dpred, 1, P.Nc, P.Nc, P.N, 2.0 * inv_n2, s, 0);
+        (void)nb;
+    });
+}
+
+int ofdft_cnn_adam(long long n, float *p, const float *g, float *m, float *v, double lr, double b1, double b2,
+                   double eps, int step, void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(n > 0 && p && g && m && v && step >= 1, "bad Adam arguments");
+        const float bc1 = (float)(1.0 - std::pow(b1, step)), bc2 = (float)(1.0 - std::pow(b2, step));
+        hipLaunchKernelGGL(k_cnn_adam, dim3(std::min<unsigned>(blocks_for(n, 256), 1024u)), dim3(256), 0, St(stream),
+                           n, p, g, m, v, (float)lr, (float)b1, (float)b2, (float)eps, bc1, bc2);
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_features(ofdft_plan *plan, const void *rho_s, const void *vloc, double rho0, double c, double sv,
+                       float *x0, void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(plan && rho_s && vloc && x0, "null pointer");
+        const unsigned nb = std::min<unsigned>(blocks_for(plan->N, 256), 4096u);
+        if (plan->prec == OFDFT_FP64)
+            hipLaunchKernelGGL(k_cnn_features<double>, dim3(nb), dim3(256), 0, St(stream), plan->N,
+                               reinterpret_cast<const double *>(rho_s), reinterpret_cast<const double *>(vloc), rho0,
+                               c, sv, x0);
+        else
+            hipLaunchKernelGGL(k_cnn_features<float>, dim3(nb), dim3(256), 0, St(stream), plan->N,
+                               reinterpret_cast<const float *>(rho_s), reinterpret_cast<const float *>(vloc), rho0, c,
+                               sv, x0);
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_target(ofdft_plan *plan, const void *rho, const float *x0, double rho0, float *target, void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(plan && rho && x0 && target && rho0 > 0, "bad arguments");
+        const unsigned nb = std::min<unsigned>(blocks_for(plan->N, 256), 4096u);
+        if (plan->prec == OFDFT_FP64)
+            hipLaunchKernelGGL(k_cnn_target<double>, dim3(nb), dim3(256), 0, St(stream), plan->N,
+                               reinterpret_cast<const double *>(rho), x0, rho0, target);
+        else
+            hipLaunchKernelGGL(k_cnn_target<float>, dim3(nb), dim3(256), 0, St(stream), plan->N,
+                               reinterpret_cast<const float *>(rho), x0, rho0, target);
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+int ofdft_cnn_apply(ofdft_plan *plan, const float *x0, const float *delta, double rho0, double ne, void *rho,
+                    void *stream) {
+    return cnn_guard([&] {
+        OFDFT_REQUIRE(plan && x0 && rho, "null pointer");
+        const int nb = reduce_blocks(plan->N);
+        double *sum = plan->scal + 200;
+        const double one = 1.0;
+        if (plan->prec == OFDFT_FP64) {
+            double *r = reinterpret_cast<double *>(rho);
+            hipLaunchKernelGGL(k_cnn_apply<double>, dim3(nb), dim3(256), 0, St(stream), plan->N, x0, delta, rho0, r,
+                               plan->red);
+            OFDFT_HIP(hipGetLastError());
+            reduce_final(plan->red, nb, 1, sum, &one, St(stream));
+            hipLaunchKernelGGL(k_cnn_normalise<double>, dim3(nb), dim3(256), 0, St(stream), plan->N, r, sum, ne,
+                               plan->dV);
+        } else {
+            float *r = reinterpret_cast<float *>(rho);
+            hipLaunchKernelGGL(k_cnn_apply<float>, dim3(nb), dim3(256), 0, St(stream), plan->N, x0, delta, rho0, r,
+                               plan->red);
+            OFDFT_HIP(hipGetLastError());
+            reduce_final(plan->red, nb, 1, sum, &one, St(stream));
+            hipLaunchKernelGGL(k_cnn_normalise<float>, dim3(nb), dim3(256), 0, St(stream), plan->N, r, sum, ne,
+                               plan->dV);
+        }
+        OFDFT_HIP(hipGetLastError());
+    });
+}
+
+}  // extern "C"

@@ -32,15 +32,20 @@ class Plan:
     def workspace_bytes(self):
         return int(_lib.lib.ofdft_plan_workspace_bytes(self.handle))
 
+    def set_lattice(self, lattice):
+        lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+        _lib.check(_lib.lib.ofdft_plan_set_lattice(self.handle, _lib.dptr(lat)))
+
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value:
-            try:
-                torch.cuda.synchronize()
-            except Exception:
-                pass
+        if h is None or not h.value:
+            return
+        try:
+            torch.cuda.synchronize()
             _lib.lib.ofdft_plan_destroy(h)
-            self.handle = None
+        except Exception:  # interpreter shutdown: module globals may be gone
+            pass
+        self.handle = None
 
 
 def _dtype_precision(dtype):
@@ -149,6 +154,16 @@ class DirectGrid(BaseGrid):
         if key not in self._plans:
             self._plans[key] = Plan(self._nrR, self._lattice, prec, self._full, max_batch)
         return self._plans[key]
+
+    def set_lattice(self, lattice):
+        """Move this grid (and its plans) to a new cell of the same shape, e.g.
+        a strained copy; cached coordinate arrays are dropped."""
+        lattice = np.asarray(getattr(lattice, "array", lattice), dtype=np.float64).reshape(3, 3)
+        self._lattice = lattice
+        for p in self._plans.values():
+            p.set_lattice(lattice)
+        self._rgrid = None
+        self._r = None
 
     def with_dtype(self, dtype):
         """A grid of the same geometry with another precision (shares nothing)."""

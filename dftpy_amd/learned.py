@@ -1,0 +1,369 @@
+"""Learned warm start for the density optimizer.
+
+The reference starts ``Optimization.optimize_rho`` from a guess density:
+the uniform electron gas or a superposition of atomic densities
+(``density/density.py:199-207,258-317``).  This module adds a start density
+produced by a small periodic 3D CNN (the HIP kernels of ``csrc/cnn.hip``)
+from two cheap fields:
+
+* the superposition density rho_s: each atom carries the linear density
+  response of the homogeneous electron gas at the mean density rho0 to its
+  own local pseudopotential, rho_a(q) = -v_a(q) / K(q), where K(q) is the
+  second functional derivative of the total functional at rho0 (TF, vW, WT
+  and LDA from their definitions, plus Hartree 4 pi / q^2), with
+  rho_a(0) = Z.  It is placed on the grid with the same structure-factor /
+  PME machinery as the local pseudopotential (``ofdft_local_pp``).
+* the local pseudopotential field vloc.
+
+Network input x0 = [c (rho_s - rho0) / rho0, sv * vloc]; output delta; start
+density rho = rho0 exp(x0[0] + delta), normalised to the electron count.
+The optimizer then finishes from that density, so the plain method remains
+both the fallback and the judge of the result.
+"""
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+from .field import DirectField
+from .functional.pseudo import RadialTable
+
+C = 32
+NTAP = 27
+LAYER_SHAPES = [("W1", (NTAP, 2, C)), ("b1", (C,))]
+for _l in range(2, 6):
+    LAYER_SHAPES += [("W%d" % _l, (NTAP, C, C)), ("b%d" % _l, (C,))]
+LAYER_SHAPES += [("W6", (NTAP, C)), ("b6", (1,))]
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "warmstart_al_wt.npz")
+
+CTF = 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
+
+
+# ------------------------------------------------------------ host-side tables
+def _lindhard_inv(eta, lbda=1.0, mu=1.0):
+    """Inverse Lindhard function (kernel.py:13-136), for the host-side
+    response table only; the evaluation kernels compute it on the device."""
+    eta = np.asarray(eta, dtype=np.float64)
+    out = np.empty_like(eta)
+    small = eta < 1e-10
+    one = np.abs(eta - 1.0) < 1e-10
+    large = eta > 3.65
+    mid = ~(small | one | large)
+    e = eta[mid]
+    out[mid] = 1.0 / (0.5 + 0.25 * (1.0 - e * e) * np.log((1.0 + e) / np.abs(1.0 - e)) / e) - 3.0 * mu * e * e - lbda
+    e = eta[small]
+    out[small] = 1.0 - lbda + e * e * (1.0 / 3.0 - 3.0 * mu)
+    e = eta[one]
+    out[one] = 2.0 - lbda - 3.0 * mu + 20.0 * (e - 1.0)
+    e = eta[large]
+    # 1/eta^2 series of the inverse Lindhard function (mu = 1 here)
+    coefs = [-0.13714285714285712, -6.39999999999999875e-2, -3.77825602968460128e-2, -2.51824061652633074e-2,
+             -1.80879839616166146e-2, -1.36715733124818332e-2, -1.07236045520990083e-2, -8.65192783339199453e-3,
+             -7.1372762502456763e-3, -5.9945117538835746e-3, -5.10997527675418131e-3, -4.41060829979912465e-3,
+             -3.84763737842981233e-3, -3.38745061493813488e-3, -3.00624946457977689e-3]
+    inv = 1.0 / (e * e)
+    s = np.full_like(e, coefs[-1])
+    for c in coefs[-2::-1]:
+        s = c + inv * s
+    out[large] = 3.0 * (1.0 - mu) * e * e - lbda - 0.6 + inv * s
+    return out
+
+
+def _lda_dv(rho0):
+    """d v_xc / d rho of the PZ LDA at a uniform density (semilocal_xc.py:291-303)."""
+    a, c, d = 0.0311, 0.0020, -0.0116
+    g, b1, b2 = -0.1423, 1.0529, 0.3334
+    rc = np.cbrt(rho0)
+    rs = np.cbrt(3.0 / (4.0 * np.pi)) / rc
+    fx = -np.cbrt(3.0 / np.pi) / 3.0 * rc / rho0
+    if rs < 1:
+        fc = -a / 3.0 - (c / 9.0 * (np.log(rs) * 2.0 + 1.0) + d * 2.0 / 9.0) * rs
+    else:
+        ta, tb = b1 * np.sqrt(rs), b2 * rs
+        fc = g / 36.0 * (5.0 * ta + 7.0 * ta * ta + 8.0 * tb + 16.0 * tb * tb + 21.0 * ta * tb) / (1 + ta + tb) ** 3
+    return fx + fc / rho0
+
+
+def response_kernel(q, rho0, x=1.0, y=1.0, alpha=5.0 / 6.0):
+    """K(q) = d^2 E / d rho(q)^2 of TF + vW + WT + LDA + Hartree at the
+    uniform density rho0 (alpha = beta)."""
+    q = np.asarray(q, dtype=np.float64)
+    tkf = 2.0 * (3.0 * rho0 * np.pi ** 2) ** (1.0 / 3.0)
+    wtfac = 5.0 / (9.0 * alpha * alpha * rho0 ** (2 * alpha - 5.0 / 3.0)) * CTF
+    k_tf = x * (10.0 / 9.0) * CTF * rho0 ** (-1.0 / 3.0)
+    k_vw = y * q * q / (4.0 * rho0)
+    k_wt = 2.0 * alpha * alpha * rho0 ** (2 * alpha - 2) * _lindhard_inv(q / tkf, x, 1.0) * wtfac
+    with np.errstate(divide="ignore"):
+        k_h = np.where(q > 0, 4.0 * np.pi / np.maximum(q, 1e-300) ** 2, np.inf)
+    return k_tf + k_vw + k_wt + k_h + _lda_dv(rho0)
+
+
+def response_table(gp, vp, z, rho0):
+    """Radial table of one ion's screening density rho_a(q) = -v(q)/K(q), with
+    rho_a(0) = Z, on the recpot q grid."""
+    gp = np.asarray(gp, dtype=np.float64)
+    vp = np.asarray(vp, dtype=np.float64)
+    K = response_kernel(gp, rho0)
+    ra = np.empty_like(vp)
+    ra[1:] = -vp[1:] / K[1:]
+    ra[0] = float(z)
+    return RadialTable(gp, ra)
+
+
+def metric_weights(grid, rho0):
+    """w(q) = K(q) / K(q -> TF) on the half spectrum of ``grid`` (w(0) = 0):
+    the Hessian of the total energy at the uniform density, so that the
+    spectral training loss sum_q w |d(q)|^2 of a log-density error d weighs
+    each wavelength by the energy it costs."""
+    nr = [int(n) for n in grid.nrR]
+    B = 2.0 * np.pi * np.linalg.inv(np.asarray(grid.lattice)).T
+    kx = np.fft.fftfreq(nr[0], 1.0 / nr[0])
+    ky = np.fft.fftfreq(nr[1], 1.0 / nr[1])
+    kz = np.arange(nr[2] // 2 + 1, dtype=np.float64)
+    K = np.stack(np.meshgrid(kx, ky, kz, indexing="ij"), axis=-1)
+    q = np.linalg.norm(K @ B, axis=-1)
+    qq = np.where(q > 0, q, 1.0)
+    w = response_kernel(qq, rho0) / ((10.0 / 9.0) * CTF * rho0 ** (-1.0 / 3.0))
+    w[0, 0, 0] = 0.0
+    w = np.maximum(w, 0.0)
+    return torch.as_tensor(w.astype(np.float32), device=grid.device).contiguous()
+
+
+def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
+    """rho_s on ``grid`` (FP64 device tensor): sum over atoms of the screening
+    densities (the G = 0 term gives the electron count)."""
+    plan = grid.plan(torch.float64)
+    keys = sorted(readpp.tables)
+    tabs = [response_table(readpp.gp[k], readpp.vp[k], readpp.zval[k], rho0) for k in keys]
+    ns = len(keys)
+    q0 = np.array([t.q0 for t in tabs])
+    dq = np.array([t.dq for t in tabs])
+    nco = np.array([t.coef.shape[0] for t in tabs], dtype=np.int32)
+    qmax = np.array([t.qmax for t in tabs])
+    vsmall = np.array([v for t in tabs for v in t.vsmall])
+    dqs = np.array([t.dqs for t in tabs])
+    coefs = [np.ascontiguousarray(t.coef.reshape(-1)) for t in tabs]
+    cptr = (_lib._DP * ns)(*[_lib.dptr(c) for c in coefs])
+    pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
+    frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
+    spec = np.array([keys.index(s) for s in ions.symbols], dtype=np.int32)
+    out = torch.empty(tuple(int(n) for n in grid.nrR), dtype=torch.float64, device=grid.device)
+    _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, _lib.dptr(q0), _lib.dptr(dq), _lib.iptr(nco), cptr,
+                                       _lib.dptr(qmax), _lib.dptr(vsmall), _lib.dptr(dqs), ions.nat, _lib.dptr(pos),
+                                       _lib.dptr(frac), _lib.iptr(spec), int(bool(pme)), int(order), out.data_ptr(),
+                                       _lib.stream()))
+    return out
+
+
+# ------------------------------------------------------------------ network
+class WarmStartNet:
+    """The 6-layer periodic residual CNN (32 channels, 3x3x3 taps).
+
+    Parameters live in one flat FP32 device vector (Adam updates it in one
+    launch); the 32 -> 32 weights are also kept as bf16 copies for the MFMA
+    kernels (forward and transposed/mirrored for backward-data)."""
+
+    def __init__(self, dil=(1, 2, 4, 4, 2, 1), c=0.7, sv=1.0, seed=0, device="cuda"):
+        _lib.require_gpu()
+        self.dil = tuple(int(d) for d in dil)
+        self.c, self.sv = float(c), float(sv)
+        self.device = torch.device(device)
+        self.offsets = {}
+        off = 0
+        for name, shp in LAYER_SHAPES:
+            n = int(np.prod(shp))
+            self.offsets[name] = (off, shp)
+            off += n
+        self.nparam = off
+        self.flat = torch.zeros(off, dtype=torch.float32, device=self.device)
+        self.grad = torch.zeros_like(self.flat)
+        self.adam_m = torch.zeros_like(self.flat)
+        self.adam_v = torch.zeros_like(self.flat)
+        self.step = 0
+        self.wf = [torch.empty(NTAP * C * C, dtype=torch.bfloat16, device=self.device) for _ in range(4)]
+        self.wb = [torch.empty_like(w) for w in self.wf]
+        self._bufs = {}
+        self._init(seed)
+
+    # parameters ----------------------------------------------------------
+    def param(self, name, which=None):
+        off, shp = self.offsets[name]
+        src = self.flat if which is None else which
+        return src[off:off + int(np.prod(shp))].view(shp)
+
+    def _init(self, seed):
+        rng = np.random.default_rng(seed)
+        p = {}
+        p["W1"] = rng.normal(0, np.sqrt(2.0 / (NTAP * 2)), (NTAP, 2, C))
+        for l in range(2, 6):
+            p["W%d" % l] = rng.normal(0, 0.5 * np.sqrt(2.0 / (NTAP * C)), (NTAP, C, C))
+        p["W6"] = rng.normal(0, 0.1 * np.sqrt(1.0 / (NTAP * C)), (NTAP, C))
+        self.set_params(p)
+
+    def set_params(self, p):
+        for name, shp in LAYER_SHAPES:
+            v = p.get(name)
+            self.param(name).copy_(torch.zeros(shp) if v is None else torch.as_tensor(np.asarray(v, np.float32)))
+        self._prep()
+
+    def params(self):
+        return {name: self.param(name).cpu().numpy().astype(np.float64) for name, _ in LAYER_SHAPES}
+
+    def _prep(self):
+        s = _lib.stream()
+        for i, l in enumerate(range(2, 6)):
+            _lib.check(_lib.lib.ofdft_cnn_prep_weights(self.param("W%d" % l).data_ptr(), self.wf[i].data_ptr(),
+                                                       self.wb[i].data_ptr(), s))
+
+    def save(self, path, **meta):
+        np.savez(path, flat=self.flat.cpu().numpy(), dil=np.array(self.dil), c=self.c, sv=self.sv,
+                 **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path=DEFAULT_WEIGHTS, device="cuda"):
+        z = np.load(path, allow_pickle=False)
+        net = cls(dil=tuple(int(d) for d in z["dil"]), c=float(z["c"]), sv=float(z["sv"]), device=device)
+        flat = torch.as_tensor(z["flat"].astype(np.float32), device=net.device)
+        if flat.numel() != net.nparam:
+            raise ValueError("weight file has %d parameters, the network %d" % (flat.numel(), net.nparam))
+        net.flat.copy_(flat)
+        net._prep()
+        return net
+
+    # buffers ----------------------------------------------------------------
+    def _buffers(self, nr, train):
+        key = (tuple(int(n) for n in nr), bool(train))
+        if key in self._bufs:
+            return self._bufs[key]
+        n = int(np.prod(nr))
+        dev = self.device
+        bf = lambda: torch.empty((n, C), dtype=torch.bfloat16, device=dev)  # noqa: E731
+        b = {"out": torch.empty(n, dtype=torch.float32, device=dev),
+             "proj": torch.empty(NTAP * n, dtype=torch.float32, device=dev)}
+        if train:
+            b["h"] = [bf() for _ in range(5)]  # h1..h5
+            b["r"] = [bf() for _ in range(4)]  # r2..r5
+            b["dh"] = torch.empty((n, C), dtype=torch.float32, device=dev)
+            b["g"] = [bf(), bf()]
+            b["dout"] = torch.empty(n, dtype=torch.float32, device=dev)
+            b["nblk"] = 512
+            kmax = max(int(_lib.lib.ofdft_cnn_wgrad_partials(layer, 512)) for layer in range(3))
+            b["partials"] = torch.empty(kmax, dtype=torch.float32, device=dev)
+            b["loss"] = torch.zeros(1, dtype=torch.float64, device=dev)
+        else:
+            b["h"] = [bf(), bf()]
+            b["r"] = [bf()]
+        self._bufs = {k: v for k, v in self._bufs.items() if k[1] != train}  # one grid shape per mode
+        self._bufs[key] = b
+        return b
+
+    # forward / backward -------------------------------------------------------
+    def forward(self, x0, nr, train=False):
+        """x0: FP32 device tensor (2, N).  Returns the output field (N,) (a
+        buffer owned by the network, overwritten by the next call)."""
+        nr_c = np.ascontiguousarray(nr, dtype=np.int32)
+        nrp = _lib.iptr(nr_c)
+        s = _lib.stream()
+        b = self._buffers(nr, train)
+        L = _lib.lib
+        h = b["h"]
+        _lib.check(L.ofdft_cnn_conv_in(nrp, self.dil[0], x0.data_ptr(), self.param("W1").data_ptr(),
+                                       self.param("b1").data_ptr(), h[0].data_ptr(), s))
+        for i, l in enumerate(range(2, 6)):
+            if train:
+                hin, hout, r = h[i], h[i + 1], b["r"][i]
+            else:
+                hin, hout, r = h[i % 2], h[(i + 1) % 2], b["r"][0]
+            _lib.check(L.ofdft_cnn_conv_mid(nrp, self.dil[l - 1], 0, hin.data_ptr(), self.wf[i].data_ptr(),
+                                            self.param("b%d" % l).data_ptr(), None, None, hout.data_ptr(),
+                                            r.data_ptr(), s))
+        hlast = h[4] if train else h[0]
+        _lib.check(L.ofdft_cnn_conv_out(nrp, self.dil[5], hlast.data_ptr(), self.param("W6").data_ptr(),
+                                        self.param("b6").data_ptr(), b["out"].data_ptr(), b["proj"].data_ptr(), s))
+        return b["out"]
+
+    def backward(self, x0, nr, dout):
+        """Gradients of sum(dout * out) for the last training forward; fills
+        self.grad."""
+        nr_c = np.ascontiguousarray(nr, dtype=np.int32)
+        nrp = _lib.iptr(nr_c)
+        s = _lib.stream()
+        b = self._buffers(nr, True)
+        L = _lib.lib
+        h, r, g, dh, part, nblk = b["h"], b["r"], b["g"], b["dh"], b["partials"], b["nblk"]
+        gp = lambda name: self.param(name, self.grad).data_ptr()  # noqa: E731
+        _lib.check(L.ofdft_cnn_wgrad(nrp, self.dil[5], 2, dout.data_ptr(), h[4].data_ptr(), part.data_ptr(), nblk,
+                                     gp("W6"), s))
+        _lib.check(L.ofdft_cnn_out_backward(nrp, self.dil[5], dout.data_ptr(), self.param("W6").data_ptr(),
+                                            r[3].data_ptr(), dh.data_ptr(), g[0].data_ptr(), s))
+        cur = 0
+        for l in range(5, 1, -1):
+            i = l - 2
+            _lib.check(L.ofdft_cnn_wgrad(nrp, self.dil[l - 1], 1, g[cur].data_ptr(), h[i].data_ptr(), part.data_ptr(),
+                                         nblk, gp("W%d" % l), s))
+            mask = r[i - 1] if l > 2 else h[0]
+            _lib.check(L.ofdft_cnn_conv_mid(nrp, self.dil[l - 1], 1, g[cur].data_ptr(), self.wb[i].data_ptr(), None,
+                                            dh.data_ptr(), mask.data_ptr(), dh.data_ptr(), g[1 - cur].data_ptr(), s))
+            cur = 1 - cur
+        _lib.check(L.ofdft_cnn_wgrad(nrp, self.dil[0], 0, g[cur].data_ptr(), x0.data_ptr(), part.data_ptr(), nblk,
+                                     gp("W1"), s))
+
+    def loss(self, plan, out, target, dout, loss, w=None):
+        """Mean squared error (w None) or the spectral loss with weights w."""
+        s = _lib.stream()
+        if w is None:
+            _lib.check(_lib.lib.ofdft_cnn_loss(plan.handle, out.data_ptr(), target.data_ptr(), dout.data_ptr(),
+                                               loss.data_ptr(), s))
+        else:
+            _lib.check(_lib.lib.ofdft_cnn_loss_spectral(plan.handle, out.data_ptr(), target.data_ptr(), w.data_ptr(),
+                                                        dout.data_ptr(), loss.data_ptr(), s))
+
+    def train_step(self, plan, x0, nr, target, lr, b1=0.9, b2=0.999, eps=1e-8, w=None):
+        """One Adam step on the loss of the output vs target (mean squared
+        error, or the spectral loss with weights w); returns the loss as a
+        device tensor (no host sync)."""
+        b = self._buffers(nr, True)
+        out = self.forward(x0, nr, train=True)
+        s = _lib.stream()
+        self.loss(plan, out, target, b["dout"], b["loss"], w)
+        self.backward(x0, nr, b["dout"])
+        self.step += 1
+        _lib.check(_lib.lib.ofdft_cnn_adam(self.nparam, self.flat.data_ptr(), self.grad.data_ptr(),
+                                           self.adam_m.data_ptr(), self.adam_v.data_ptr(), float(lr), b1, b2, eps,
+                                           self.step, s))
+        self._prep()
+        return b["loss"]
+
+    # start density ---------------------------------------------------------
+    def features(self, plan, rho_s, vloc, rho0, out=None):
+        n = rho_s.numel()
+        x0 = out if out is not None else torch.empty((2, n), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib.ofdft_cnn_features(plan.handle, rho_s.data_ptr(), vloc.data_ptr(), float(rho0), self.c,
+                                               self.sv, x0.data_ptr(), _lib.stream()))
+        return x0
+
+    def start_density(self, grid, x0, rho0, ne, use_net=True):
+        """rho0 exp(x0[0] + delta) normalised to ne, as a DirectField on grid."""
+        plan = grid.plan(grid.dtype)
+        delta = self.forward(x0, grid.nrR) if use_net else None
+        rho = torch.empty(tuple(int(n) for n in grid.nrR), dtype=grid.dtype, device=grid.device)
+        _lib.check(_lib.lib.ofdft_cnn_apply(plan.handle, x0.data_ptr(), None if delta is None else delta.data_ptr(),
+                                            float(rho0), float(ne), rho.data_ptr(), _lib.stream()))
+        return DirectField(grid, data=rho)
+
+
+def learned_start(grid, ions, pseudo, net=None, use_net=True):
+    """Start density for ``Optimization.optimize_rho`` on ``grid`` from the
+    superposition features and the network (``use_net=False`` gives the
+    superposition start alone).  ``pseudo`` is the LocalPseudo of the run."""
+    net = net or WarmStartNet.load()
+    ne = float(ions.get_ncharges())
+    rho0 = ne / grid.volume
+    g64 = pseudo.grid if pseudo.grid.dtype == torch.float64 else pseudo.grid.with_dtype(torch.float64)
+    rho_s = superposition_density(g64, ions, pseudo.readpp, rho0)
+    vloc = pseudo.vreal_for(g64).data
+    x0 = net.features(g64.plan(torch.float64), rho_s, vloc, rho0)
+    return net.start_density(grid, x0, rho0, ne, use_net=use_net)

@@ -65,6 +65,10 @@ const char *ofdft_last_error(void);
 int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, int full,
                       int max_batch, ofdft_plan **plan);
 int ofdft_plan_destroy(ofdft_plan *plan);
+/* Replace the cell of a plan (same grid shape): a strained copy of the cell
+ * reuses the FFT plans and workspace (Ions/DirectGrid rebuilt on a new cell
+ * in the reference, grid.py:302-318). */
+int ofdft_plan_set_lattice(ofdft_plan *plan, const double lattice[9]);
 /* FFT kernel selection (no counterpart in the reference): fixed != 0 (the
  * default) uses the size-specialised kernels for the lengths that have them,
  * 0 forces the generic mixed-radix kernels everywhere. */
@@ -172,6 +176,63 @@ int ofdft_dcsrch(double *stp, double f, double g, double ftol, double gtol, doub
  * info (host, 4 doubles): [status, iterations, final residual, atol]. */
 int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, double sigma,
                   double tol, int maxiter, void *psi1, double *info, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no counterpart in the reference: it produces the start
+ * density handed to Optimization.optimize_rho, optimization.py:281-520, in
+ * place of the guess densities of density/density.py:199-207,258-317).
+ * A periodic residual 3D CNN on the density grid: 2 input channels (planar
+ * FP32 fields), 32 hidden channels (channels-last bf16, N x 32), 1 output
+ * channel (FP32).  Tap t = (i*3+j)*3+k reads p + dil*(i-1, j-1, k-1) with
+ * periodic wrap.  nr: grid shape; every pointer is device memory.
+ * ------------------------------------------------------------------------- */
+/* h = relu(conv(x0; w) + b): x0 [2][N] FP32, w [27][2][32] FP32, b [32], h [N][32] bf16 */
+int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w, const float *b, void *h,
+                      void *stream);
+/* 32 -> 32 layer on MFMA (bf16 in, FP32 accumulation), w [27][32 out][32 in] bf16.
+ * mode 0 (forward): r = relu(conv(x) + bias); out0 (bf16 [N][32]) = x + r; out1 (bf16) = r.
+ * mode 1 (backward-data, w from ofdft_cnn_prep_weights' wb): v = conv(x) + resid (FP32);
+ *        out0 (FP32 [N][32]) = v; out1 (bf16) = mask > 0 ? v : 0.  out0 must not alias x
+ *        (it may alias resid). */
+int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const void *w, const float *bias,
+                       const float *resid, const void *mask, void *out0, void *out1, void *stream);
+/* out = conv(h; w) + b: h [N][32] bf16, w [27][32] FP32, b [1], out [N] FP32.
+ * work (27 N floats) selects the projection + gather form; NULL the direct one. */
+int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, const float *b, float *out,
+                       float *work, void *stream);
+/* dh [N][32] FP32 = d(sum dout*out)/dh; gout (bf16) = mask > 0 ? dh : 0 */
+int ofdft_cnn_out_backward(const int nr[3], int dil, const float *dout, const float *w, const void *mask, float *dh,
+                           void *gout, void *stream);
+/* Floats of the partials workspace of ofdft_cnn_wgrad for `layer` and nblk blocks. */
+size_t ofdft_cnn_wgrad_partials(int layer, int nblk);
+/* Weight and bias gradients, reduced over all points into dw (FP32):
+ * layer 0: g [N][32] bf16, x = x0 [2][N] FP32 -> dw = [27][2][32] then db [32]
+ * layer 1: g [N][32] bf16, x = layer input [N][32] bf16 (MFMA) -> dw = [27][32][32] then db [32]
+ * layer 2: g = dout [N] FP32, x = h [N][32] bf16 -> dw = [27][32] then db [1] */
+int ofdft_cnn_wgrad(const int nr[3], int dil, int layer, const void *g, const void *x, float *partials, int nblk,
+                    float *dw, void *stream);
+/* wf = bf16(w) [27][32][32]; wb[t][ci][co] = bf16(w[26-t][co][ci]) for backward-data */
+int ofdft_cnn_prep_weights(const float *w, void *wf, void *wb, void *stream);
+/* loss (device double) = mean (pred - target)^2; dpred = 2 (pred - target) / N */
+int ofdft_cnn_loss(ofdft_plan *plan, const float *pred, const float *target, float *dpred, double *loss,
+                   void *stream);
+/* Spectral loss (FP32 plan): with d = pred - target and D = unnormalised R2C
+ * of d, loss = (1/N^2) sum over the full spectrum of w(q) |D(q)|^2 (w given
+ * on the half spectrum [nx][ny][nz/2+1]; w = 1 gives mean(d^2)); dpred =
+ * d loss / d pred.  Uses the plan's workspace. */
+int ofdft_cnn_loss_spectral(ofdft_plan *plan, const float *pred, const float *target, const float *w, float *dpred,
+                            double *loss, void *stream);
+/* One Adam step (bias-corrected, FP32 master weights), step >= 1 */
+int ofdft_cnn_adam(long long n, float *p, const float *g, float *m, float *v, double lr, double b1, double b2,
+                   double eps, int step, void *stream);
+/* Network inputs: x0[0] = c (rho_s - rho0) / rho0, x0[1] = sv * vloc (plan precision inputs) */
+int ofdft_cnn_features(ofdft_plan *plan, const void *rho_s, const void *vloc, double rho0, double c, double sv,
+                       float *x0, void *stream);
+/* Training target: target = log(rho / rho0) - x0[0] (rho: converged density, plan precision) */
+int ofdft_cnn_target(ofdft_plan *plan, const void *rho, const float *x0, double rho0, float *target, void *stream);
+/* rho = rho0 exp(x0[0] + delta) scaled to ne electrons (delta may be NULL) */
+int ofdft_cnn_apply(ofdft_plan *plan, const float *x0, const float *delta, double rho0, double ne, void *rho,
+                    void *stream);
 
 #ifdef __cplusplus
 }

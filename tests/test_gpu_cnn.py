@@ -1,0 +1,148 @@
+"""Warm-start network kernels (csrc/cnn.hip) against the FP64 oracle
+(oracle/cnn.py) with the same bfloat16 rounding points.  The grid shape is
+not a multiple of the kernels' point tiles, and dilations > 1 wrap around
+the periodic cell."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (12, 10, 15)
+DIL = (1, 2, 1, 3, 2, 1)
+# (12, 10, 15): every layer on the point-tiled kernel; (6, 8, 32) and
+# (4, 16, 64): layers with ny % 4d == 0 on the row-tiled kernel, the rest
+# on the point-tiled one
+SHAPES = [(12, 10, 15), (6, 8, 32), (4, 16, 64)]
+
+
+def _setup(seed=0, scale=1.5, shape=SHAPE):
+    from oracle import cnn
+    rng = np.random.default_rng(seed)
+    p = cnn.init_params(seed + 10, scale=scale)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = rng.normal(0, 0.2, p[k].shape)
+    p = {k: v.astype(np.float32).astype(np.float64) for k, v in p.items()}
+    x0 = rng.normal(0, 1, (2,) + shape).astype(np.float32).astype(np.float64)
+    target = rng.normal(0, 0.3, shape).astype(np.float32).astype(np.float64)
+    return p, x0, target
+
+
+def _net(p):
+    from dftpy_amd.learned import WarmStartNet
+    net = WarmStartNet(dil=DIL)
+    net.set_params(p)
+    return net
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("train", [False, True])
+def test_forward_matches_oracle(train, shape):
+    from oracle import cnn
+    p, x0, _ = _setup(shape=shape)
+    ref, _ = cnn.forward(p, x0, DIL, bf16=True)
+    net = _net(p)
+    x = torch.as_tensor(x0.reshape(2, -1), dtype=torch.float32, device="cuda")
+    out = net.forward(x, shape, train=train).cpu().numpy().astype(np.float64).reshape(shape)
+    print("forward rel L2 %.2e max %.2e (|ref| max %.2e)" % (_rel(out, ref), np.abs(out - ref).max(), np.abs(ref).max()))
+    assert _rel(out, ref) < 2e-3
+    assert np.abs(out - ref).max() < 2e-2 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_backward_matches_oracle(shape):
+    from oracle import cnn
+    from dftpy_amd import DirectGrid
+    p, x0, target = _setup(1, shape=shape)
+    ref_out, cache = cnn.forward(p, x0, DIL, bf16=True)
+    ref_loss, dout = cnn.loss(ref_out, target)
+    gref = cnn.backward(p, cache, dout)
+    net = _net(p)
+    grid = DirectGrid(np.eye(3) * 5.0, shape, full=False, dtype=torch.float32)
+    plan = grid.plan(torch.float32)
+    x = torch.as_tensor(x0.reshape(2, -1), dtype=torch.float32, device="cuda")
+    tgt = torch.as_tensor(target.reshape(-1), dtype=torch.float32, device="cuda")
+    loss = net.train_step(plan, x, shape, tgt, lr=0.0)  # lr 0: gradients only
+    assert abs(float(loss.item()) - ref_loss) < 1e-2 * ref_loss
+    for k, g in gref.items():
+        got = net.param(k, net.grad).cpu().numpy().astype(np.float64)
+        print("%s rel %.2e" % (k, _rel(got, g)))
+        assert _rel(got, g) < 1e-2, k
+
+
+def _oracle_adam(p, x0, tgt, lr, steps):
+    from oracle import cnn
+    q = {k: v.copy() for k, v in p.items()}
+    m = {k: 0 * v for k, v in q.items()}
+    v2 = {k: 0 * v for k, v in q.items()}
+    losses = []
+    for t in range(1, steps + 1):
+        out, c = cnn.forward(q, x0, DIL, bf16=True)
+        loss, d = cnn.loss(out, tgt)
+        g = cnn.backward(q, c, d)
+        losses.append(loss)
+        for k in q:  # Adam, Kingma & Ba (2015), with bias correction
+            m[k] = 0.9 * m[k] + 0.1 * g[k]
+            v2[k] = 0.999 * v2[k] + 0.001 * g[k] ** 2
+            q[k] -= lr * (m[k] / (1 - 0.9 ** t)) / (np.sqrt(v2[k] / (1 - 0.999 ** t)) + 1e-8)
+    return losses
+
+
+def test_training_trajectory_matches_oracle():
+    """Sixty Adam steps on a learnable target (a linear map of the inputs)
+    with the GPU forward, backward and Adam kernels follow the oracle's
+    training trajectory and lower the loss."""
+    from dftpy_amd import DirectGrid
+    p, x0, _ = _setup(2, scale=1.0)
+    tgt64 = 0.3 * x0[0] - 0.1 * x0[1]
+    ref = _oracle_adam(p, x0, tgt64, 5e-4, 60)
+    net = _net(p)
+    grid = DirectGrid(np.eye(3) * 5.0, SHAPE, full=False, dtype=torch.float32)
+    plan = grid.plan(torch.float32)
+    x = torch.as_tensor(x0.reshape(2, -1), dtype=torch.float32, device="cuda")
+    tgt = torch.as_tensor(tgt64.reshape(-1), dtype=torch.float32, device="cuda")
+    losses = [float(net.train_step(plan, x, SHAPE, tgt, lr=5e-4).item()) for _ in range(60)]
+    print(["%.4f" % v for v in losses[::6]])
+    print(["%.4f" % v for v in ref[::6]])
+    assert np.max(np.abs(np.array(losses) - ref) / np.array(ref)) < 0.02
+    assert min(losses[-10:]) < 0.5 * losses[0]
+
+
+def test_spectral_loss_matches_numpy():
+    """Spectral (Hessian-metric) loss and its gradient against numpy's rfftn;
+    w = 1 reduces to the mean squared error (Parseval)."""
+    from dftpy_amd import DirectGrid
+    from dftpy_amd import _lib
+    rng = np.random.default_rng(4)
+    grid = DirectGrid(np.eye(3) * 5.0, SHAPE, full=False, dtype=torch.float32)
+    plan = grid.plan(torch.float32)
+    pred = rng.normal(0, 1, SHAPE).astype(np.float32)
+    tgt = rng.normal(0, 1, SHAPE).astype(np.float32)
+    nzc = SHAPE[2] // 2 + 1
+    for w in (np.ones(SHAPE[:2] + (nzc,), np.float32), rng.uniform(0, 3, SHAPE[:2] + (nzc,)).astype(np.float32)):
+        d = pred.astype(np.float64) - tgt
+        D = np.fft.rfftn(d)
+        m = np.full(nzc, 2.0)
+        m[0] = 1.0
+        if SHAPE[2] % 2 == 0:
+            m[-1] = 1.0
+        n = d.size
+        ref = float((m * w * np.abs(D) ** 2).sum() / n ** 2)
+        gref = 2.0 / n * np.fft.irfftn(w * D, s=SHAPE)
+        tp = torch.as_tensor(pred.reshape(-1), device="cuda")
+        tt = torch.as_tensor(tgt.reshape(-1), device="cuda")
+        tw = torch.as_tensor(w, device="cuda").contiguous()
+        dp = torch.empty_like(tp)
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        _lib.check(_lib.lib.ofdft_cnn_loss_spectral(plan.handle, tp.data_ptr(), tt.data_ptr(), tw.data_ptr(),
+                                                    dp.data_ptr(), loss.data_ptr(), _lib.stream()))
+        assert abs(float(loss.item()) - ref) < 1e-5 * ref
+        g = dp.cpu().numpy().astype(np.float64).reshape(SHAPE)
+        assert np.abs(g - gref).max() < 1e-5 * np.abs(gref).max()
+        if np.all(w == 1):
+            assert abs(ref - float((d * d).mean())) < 1e-9 * ref

@@ -1,0 +1,33 @@
+"""What the trained warm-start network gains over the method without it,
+on held-out cells (108-atom fcc Al, 96^3, seeds outside the training range
+0..1999): the start density's energy error, and the evaluations the FP32 TN
+optimizer needs under the reference stop rule.  Both starts must reach the
+same minimum (within 1e-6 Ha/atom): the optimizer remains the judge."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("seed", [10001, 10002])
+def test_learned_start_gains_over_superposition(seed):
+    from dftpy_amd.learned import WarmStartNet
+    from dftpy_amd.warmstart_train import CellFactory, converge, energy64
+    net = WarmStartNet.load()
+    fac = CellFactory()
+    ions, ev, ne, rho0, x0 = fac.cell(seed, net)
+    nat = ions.nat
+    ref_rho, _, _ = converge(ev, net.start_density(fac.g32, x0, rho0, ne, use_net=True), 1e-9, nat, 100)
+    estar = energy64(ev, fac.g64, ref_rho)
+    res = {}
+    for name, use in (("superposition", False), ("learned", True)):
+        start = net.start_density(fac.g32, x0, rho0, ne, use_net=use)
+        e0 = energy64(ev, fac.g64, start)
+        rho, opt, secs = converge(ev, start, 1e-6, nat, 60)
+        e1 = energy64(ev, fac.g64, rho)
+        res[name] = dict(start_err=(e0 - estar) / nat, final_err=(e1 - estar) / nat, iters=opt.stats["niter"],
+                         evals=opt.stats["n_eval_v"] + opt.stats["n_eval_ev"], seconds=secs)
+    print(seed, res)
+    sup, lrn = res["superposition"], res["learned"]
+    assert 0 <= lrn["start_err"] < 0.05 * sup["start_err"]
+    assert lrn["evals"] < sup["evals"]
+    assert abs(lrn["final_err"]) <= 1e-6 and abs(sup["final_err"]) <= 1e-6

@@ -219,3 +219,50 @@ def test_td_dipole_golden():
         v = pot(rho)
     assert np.isclose(td.energy(psi, v, g), E0, rtol=1e-3)
     assert np.isclose(td.dipole(rho, rho0, g)[0], 0.011423, rtol=1e-3)
+
+
+# ---------------------------------------------------------------- warm-start network
+def test_cnn_bf16_rounding_matches_torch():
+    import torch
+    from oracle import cnn
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.normal(0, 3, 4000), rng.normal(0, 1e-3, 1000), [0.0, -0.0, 1.0, 3.0e38, -7.5]])
+    ref = torch.tensor(x, dtype=torch.float32).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(cnn.to_bf16(x), ref)
+
+
+def test_cnn_oracle_gradient_is_the_derivative():
+    """Central differences of the FP64 loss along random directions, one
+    parameter group at a time (dilations > 1 and uneven axes included),
+    against the analytic backward.  ReLU kinks can spoil a difference
+    quotient, so each group must agree at one of two small steps; a wrong,
+    dropped or misplaced gradient term fails at every step."""
+    from oracle import cnn
+    rng = np.random.default_rng(1)
+    shape = (5, 6, 7)
+    dil = (1, 2, 1, 2, 1, 1)
+    p = cnn.init_params(3, scale=2.0)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = rng.normal(0, 0.1, p[k].shape)
+    x0 = rng.normal(0, 1, (2,) + shape)
+    target = rng.normal(0, 0.1, shape)
+
+    def L(q):
+        out, _ = cnn.forward(q, x0, dil)
+        return cnn.loss(out, target)[0]
+
+    out, cache = cnn.forward(p, x0, dil)
+    _, dout = cnn.loss(out, target)
+    g = cnn.backward(p, cache, dout)
+    for k in p:
+        v = rng.normal(0, 1, p[k].shape)
+        an = float((g[k] * v).sum())
+        errs = []
+        for eps in (1e-7, 4e-8):
+            q1 = dict(p)
+            q1[k] = p[k] + eps * v
+            q2 = dict(p)
+            q2[k] = p[k] - eps * v
+            errs.append(abs((L(q1) - L(q2)) / (2 * eps) - an))
+        assert min(errs) <= 2e-8 * max(1.0, abs(an)) + 5e-8, (k, an, errs)
