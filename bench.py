@@ -147,7 +147,7 @@ def pass_timings(grid, rho, iters=50):
     from dftpy_amd import _lib
     plan = grid.plan(grid.dtype)
     out = {}
-    for which, name in ((0, "z_r2c_prologue"), (1, "y_c2c"), (2, "x_c2c"), (3, "z_c2r")):
+    for which, name in ((0, "z_r2c_prologue"), (1, "y_c2c"), (2, "x_c2c"), (3, "z_c2r"), (5, "evaluation_E_V")):
         ms = ctypes.c_double()
         _lib.check(_lib.lib.ofdft_bench_pass(plan.handle, which, 3, iters, rho.data.data_ptr(), ctypes.byref(ms),
                                              _lib.stream()))

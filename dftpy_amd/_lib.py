@@ -52,6 +52,8 @@ SIGNATURES = {
     "ofdft_optimize": (_I, [_P, _P, _P, _I, _I, _DP, _DP, _DP, _I, _P]),
     "ofdft_dcsrch": (_I, [_DP, _D, _D, _D, _D, _D, _D, _D, _IP, _DP]),
     "ofdft_cn_step": (_I, [_P, _P, _P, _D, _D, _D, _I, _P, _DP, _P]),
+    "ofdft_cn_batch_workspace_bytes": (ctypes.c_size_t, [_P, _I]),
+    "ofdft_cn_step_batch": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _P, _DP, _P, ctypes.c_size_t, _P]),
     "ofdft_cnn_conv_in": (_I, [_IP, _I, _P, _P, _P, _P, _P]),
     "ofdft_cnn_conv_mid": (_I, [_IP, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ofdft_cnn_conv_out": (_I, [_IP, _I, _P, _P, _P, _P, _P, _P]),

@@ -261,8 +261,8 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                      void *stream) {
     return guard([&] {
         OFDFT_REQUIRE(P && avg_ms && iters >= 1 && batch >= 1 && batch <= 3, "bench_pass: bad arguments");
-        OFDFT_REQUIRE(which >= 0 && which <= 4, "bench_pass: unknown pass");
-        OFDFT_REQUIRE(which != 0 || rho != nullptr, "bench_pass: the z pass needs rho");
+        OFDFT_REQUIRE(which >= 0 && which <= 5, "bench_pass: unknown pass");
+        OFDFT_REQUIRE((which != 0 && which != 5) || rho != nullptr, "bench_pass: the z pass needs rho");
         hipStream_t s = S(stream);
         hipEvent_t e0, e1;
         OFDFT_HIP(hipEventCreate(&e0));
@@ -275,6 +275,7 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                     pro.code[0] = OP_SQRT_RHOM;
                     pro.code[1] = OP_POW_BETA;
                     pro.beta = P->beta;
+                    pro.b56 = std::fabs(P->beta - 5.0 / 6.0) < 1e-14;
                     bench_rows_r2c<double>(*P, (const double *)rho, spec, batch, &pro, s);
                 } else if (which == 1 || which == 2) {
                     const long long pt = pipe_pitch(*P);
@@ -282,6 +283,9 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                                      (long long)P->nr[0] * P->nr[1] * pt, 1.0, s, pt);
                 } else if (which == 3) {
                     bench_rows_c2r<double>(*P, spec, (double *)P->work, batch, s);
+                } else if (which == 5) {
+                    eval<double>(*P, (const double *)rho, OFDFT_CALC_E | OFDFT_CALC_V, (double *)P->aux,
+                                 P->scal + 100, s);
                 } else {
                     fft_c2c<double>(*P, spec, spec, false, 1, P->N, 1.0, s);
                 }
@@ -292,6 +296,7 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                     pro.code[0] = OP_SQRT_RHOM;
                     pro.code[1] = OP_POW_BETA;
                     pro.beta = P->beta;
+                    pro.b56 = std::fabs(P->beta - 5.0 / 6.0) < 1e-14;
                     bench_rows_r2c<float>(*P, (const float *)rho, spec, batch, &pro, s);
                 } else if (which == 1 || which == 2) {
                     const long long pt = pipe_pitch(*P);
@@ -299,6 +304,9 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                                     (long long)P->nr[0] * P->nr[1] * pt, 1.0, s, pt);
                 } else if (which == 3) {
                     bench_rows_c2r<float>(*P, spec, (float *)P->work, batch, s);
+                } else if (which == 5) {
+                    eval<float>(*P, (const float *)rho, OFDFT_CALC_E | OFDFT_CALC_V, (float *)P->aux, P->scal + 100,
+                                s);
                 } else {
                     fft_c2c<float>(*P, spec, spec, false, 1, P->N, 1.0, s);
                 }

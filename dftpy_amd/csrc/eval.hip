@@ -16,77 +16,12 @@
 //      semilocal_xc.py:252-290, hartree.py:38, pseudo/__init__.py:228).
 #include <cmath>
 
+#include "gspace.h"
 #include "internal.h"
+#include "pointwise.h"
 #include "reduce.h"
 
 namespace ofdft {
-
-// ------------------------------------------------------------------ Lindhard
-// Inverse Lindhard function with the reference's branch priority
-// normal < zero (eta < tol) < one (|eta-1| < tol) < large (eta > 3.65).
-__device__ __forceinline__ double lindhard(double eta, double lbda, double mu) {
-    const double tol = 1e-10;
-    if (eta > 3.65) {
-        const double inv = 1.0 / (eta * eta);
-        double s = -3.00624946457977689e-3;
-        s = -3.38745061493813488e-3 + inv * s;
-        s = -3.84763737842981233e-3 + inv * s;
-        s = -4.41060829979912465e-3 + inv * s;
-        s = -5.10997527675418131e-3 + inv * s;
-        s = -5.9945117538835746e-3 + inv * s;
-        s = -7.1372762502456763e-3 + inv * s;
-        s = -8.65192783339199453e-3 + inv * s;
-        s = -1.07236045520990083e-2 + inv * s;
-        s = -1.36715733124818332e-2 + inv * s;
-        s = -1.80879839616166146e-2 + inv * s;
-        s = -2.51824061652633074e-2 + inv * s;
-        s = -3.77825602968460128e-2 + inv * s;
-        s = -6.39999999999999875e-2 + inv * s;
-        s = -0.13714285714285712 + inv * s;
-        return 3.0 * (1.0 - mu) * eta * eta - lbda - 0.6 + inv * s;
-    }
-    if (fabs(eta - 1.0) < tol) return 2.0 - lbda - 3.0 * mu + 20.0 * (eta - 1.0);
-    if (eta < tol) return 1.0 - lbda + eta * eta * (1.0 / 3.0 - 3.0 * mu);
-    return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) -
-           3.0 * mu * eta * eta - lbda;
-}
-
-struct GParams {
-    double B[9];
-    int nx, ny, nz, nzc;
-    int pitch;  // row pitch of the spectra (>= nzc)
-    int full;
-    double scale;
-    int s_vw, s_wt, s_h;
-    double sigma;
-    double lbda, alpha, beta;
-    double rho0;             // used when rho0_dev == nullptr
-    const double *rho0_dev;  // cached rho0 on device
-    long long bs;
-};
-
-__device__ __forceinline__ int ffreq(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
-__device__ __forceinline__ int wrapf(int f, int n) { return ((n & 1) == 0 && f == n / 2) ? -n / 2 : f; }
-
-__device__ __forceinline__ double gsq(const double *B, int fx, int fy, int fz) {
-    const double gx = fx * B[0] + fy * B[3] + fz * B[6];
-    const double gy = fx * B[1] + fy * B[4] + fz * B[7];
-    const double gz = fx * B[2] + fy * B[5] + fz * B[8];
-    return gx * gx + gy * gy + gz * gz;
-}
-
-struct KVals {
-    double vw, wt, h;
-};
-
-__device__ __forceinline__ KVals kernel_vals(const GParams &p, double gg, bool g0, double tkf, double wtfac) {
-    KVals k;
-    k.vw = 0.5 * gg;
-    if (p.sigma > 0) k.vw *= exp(-gg * p.sigma * p.sigma / 4.0);
-    k.wt = lindhard(sqrt(gg) / tkf, p.lbda, 1.0) * wtfac;
-    k.h = g0 ? 0.0 : 4.0 * M_PI / gg;
-    return k;
-}
 
 template <typename T>
 __global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
@@ -154,8 +89,43 @@ __global__ void k_rho0_update(const double *partials, int nb, double inv_n, doub
     }
 }
 
+static int grid_blocks(long long n, int per = 256 * 4) {
+    long long b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return (int)b;
+}
+
+template <typename T>
+bool xfused_fixed(int n, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch, long long vp,
+                  const void *tw, const GParams &gp, hipStream_t s);
+
+// the x pass can run fused with the G-space multiply (size-specialised x kernel)
+static bool x_fusable(const Plan &P) { return P.use_fixed && P.nr[0] > 1 && P.nr[1] * P.nr[2] > 1 && fixed_size(P.nr[0]); }
+
+// G-space multiply of the ns pipeline spectra: fused into the x pass when
+// `fused` (the caller skipped the x passes of its R2C / C2R), else the
+// stand-alone kernel between complete 3D transforms
+template <typename T>
+static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long pitch, GParams &gp, bool fused,
+                         hipStream_t s) {
+    for (int i = 0; i < 4; ++i) gp.kind[i] = -1;
+    if (gp.s_vw >= 0) gp.kind[gp.s_vw] = 0;
+    if (gp.s_wt >= 0) gp.kind[gp.s_wt] = 1;
+    if (gp.s_h >= 0) gp.kind[gp.s_h] = 2;
+    if (fused) {
+        const long long ncols = (long long)P.nr[1] * pitch;
+        OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], spec, ns, ncp, ncols, pitch, P.nzc, P.tw[0], gp, s),
+                      "fused x pass unavailable");
+    } else {
+        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);
+    }
+    OFDFT_HIP(hipGetLastError());
+}
+
 struct PostParams {
     int s_vw, s_wt, s_h;
+    int b56;  // beta == 5/6
     unsigned terms;
     double x, y, alpha, beta;
     long long n;
@@ -180,14 +150,19 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
          i += (long long)gridDim.x * blockDim.x) {
         const CT r = (CT)rho[i];
         CT v = (CT)0;
+        // one cube root and one square root serve every term (for beta = 5/6)
+        const CT cr = cbrt(r);
+        const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
+        const CT c_rc = r < (CT)1e-30 ? (CT)1e-10 : cr;  // cbrt(max(rho, 1e-30))
+        const CT s_rc = sqrt(rc);
         if (p.terms & OFDFT_TERM_TF) {
-            const CT r2 = r * r;
-            if (p.want_e) acc[0] += (double)(ctf * cbrt(r2 * r2 * r));
-            v += (CT)p.x * ((CT)(5.0 / 3.0) * ctf * cbrt(r2));
+            const CT c2 = cr * cr;  // rho^(2/3)
+            if (p.want_e) acc[0] += (double)(ctf * c2 * r);
+            v += (CT)p.x * ((CT)(5.0 / 3.0) * ctf * c2);
         }
         if (p.s_vw >= 0) {
             const CT a = (CT)work[p.s_vw * p.bs + i];
-            CT sq = sqrt(r > (CT)0 ? r : (CT)1e-30);
+            CT sq = (r > (CT)0 && r < (CT)1e-30) ? sqrt(r) : s_rc;  // sqrt(rho > 0 ? rho : 1e-30)
             if (fabs(sq) < (CT)1e-30) sq = (CT)1e-30;
             const CT vw = (CT)p.y * (a / sq);
             if (p.want_e && r > (CT)0) acc[1] += (double)(r * vw);
@@ -195,14 +170,13 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
         }
         if (p.s_wt >= 0) {
             const CT c = (CT)work[p.s_wt * p.bs + i];
-            const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
-            const CT rb = pow(rc, (CT)p.beta);
-            const CT vnl = (CT)(2.0 * p.alpha) * (rb / rc) * c;
+            const CT rb_rc = p.b56 ? c_rc / s_rc : pow(rc, (CT)p.beta) / rc;  // rho^beta / rho
+            const CT vnl = (CT)(2.0 * p.alpha) * rb_rc * c;
             if (p.want_e) acc[2] += (double)(vnl * r) / (2.0 * p.alpha);
             v += vnl;
         }
         if (p.terms & OFDFT_TERM_XC) {
-            CT rc3 = cbrt(r);
+            CT rc3 = cr;
             if (rc3 < (CT)1e-30) rc3 = (CT)1e-30;
             const CT rs = cbrt34pi / rc3;
             CT ec, vc;
@@ -266,12 +240,6 @@ __global__ void k_energy_final(const double *__restrict__ partials, int nb, doub
     if (threadIdx.x == 0) out[6] = tot + eion;
 }
 
-static int grid_blocks(long long n, int per = 256 * 4) {
-    long long b = (n + per - 1) / per;
-    if (b < 1) b = 1;
-    if (b > 4096) b = 4096;
-    return (int)b;
-}
 
 template <typename T>
 void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s) {
@@ -289,10 +257,12 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         if (s_wt >= 0) pro.code[s_wt] = OP_POW_BETA;
         if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
         pro.beta = P.beta;
+        pro.b56 = beta_is_5_6(P.beta);
         const bool auto_rho0 = (s_wt >= 0) && !(P.rho0 > 0);
         const long long pitch = pipe_pitch(P);
         const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
-        fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch);
+        const bool fused = x_fusable(P);
+        fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch, fused);
         if (auto_rho0) {
             const int nb = r2c_num_partials(P, ns);
             hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(256), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
@@ -317,9 +287,8 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
         gp.bs = ncp;
         gp.pitch = (int)pitch;
-        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);
-        OFDFT_HIP(hipGetLastError());
-        fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch);
+        gspace_apply<T>(P, spec, ns, ncp, pitch, gp, fused, s);
+        fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch, fused);
     }
     PostParams pp;
     pp.s_vw = s_vw;
@@ -330,6 +299,7 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
     pp.y = P.y;
     pp.alpha = P.alpha;
     pp.beta = P.beta;
+    pp.b56 = beta_is_5_6(P.beta);
     pp.n = P.N;
     pp.bs = P.N;
     pp.want_v = (calc & OFDFT_CALC_V) && pot != nullptr;
@@ -367,10 +337,12 @@ void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
     if (s_wt >= 0) pro.code[s_wt] = OP_DPOW_BETA;
     if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
     pro.beta = P.beta;
+    pro.b56 = beta_is_5_6(P.beta);
     pro.aux = rho;
     const long long pitch = pipe_pitch(P);
     const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
-    fft_r2c<T>(P, drho, spec, ns, 0, ncp, 1.0, &pro, nullptr, s, pitch);
+    const bool fused = x_fusable(P);
+    fft_r2c<T>(P, drho, spec, ns, 0, ncp, 1.0, &pro, nullptr, s, pitch, fused);
     GParams gp;
     for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
     gp.nx = P.nr[0];
@@ -390,9 +362,8 @@ void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
     gp.rho0_dev = (s_wt >= 0 && !(P.rho0 > 0)) ? P.scal + 0 : nullptr;
     gp.bs = ncp;
     gp.pitch = (int)pitch;
-    hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);
-    OFDFT_HIP(hipGetLastError());
-    fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch);
+    gspace_apply<T>(P, spec, ns, ncp, pitch, gp, fused, s);
+    fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch, fused);
 }
 
 // Ap = (V - mu) p + phi dV[2 phi p]  and  sum p.Ap.
@@ -412,23 +383,23 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ drho, con
          i += (long long)gridDim.x * blockDim.x) {
         const CT r = (CT)rho[i], dr = (CT)drho[i];
         const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
+        const CT c_rp = cbrt(rp), s_rp = sqrt(rp);
         CT dv = (CT)0;
-        if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf / cbrt(rp) * dr;  // tf.py:38-40
+        if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf / c_rp * dr;  // tf.py:38-40
         if (pp.s_vw >= 0) {
-            const CT sq = sqrt(r > (CT)0 ? r : (CT)1e-30);
+            const CT sq = (r > (CT)0 && r < (CT)1e-30) ? sqrt(r) : s_rp;  // sqrt(rho > 0 ? rho : 1e-30)
             const CT ds = dr / ((CT)2 * sq);
             const CT vwb = (CT)pp.y * (CT)base_a[i] / sq;
             dv += ((CT)pp.y * (CT)work[pp.s_vw * pp.bs + i] - vwb * ds) / sq;
         }
         if (pp.s_wt >= 0) {
-            const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
-            const CT rb1 = pow(rc, (CT)pp.beta - (CT)1);
-            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 / rc * dr * (CT)base_c[i] +
+            const CT rb1 = pp.b56 ? c_rp / s_rp : pow(rp, (CT)pp.beta - (CT)1);  // rho^(beta-1)
+            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 / rp * dr * (CT)base_c[i] +
                                            rb1 * (CT)work[pp.s_wt * pp.bs + i]);
         }
         if (pp.terms & OFDFT_TERM_XC) {
             // semilocal_xc.py:291-303
-            const CT rc3 = cbrt(rp);
+            const CT rc3 = c_rp;
             const CT rs = cbrt34pi / rc3;
             CT fc;
             if (rs < (CT)1) {
@@ -465,6 +436,7 @@ void hvp(Plan &P, const T *rho, const T *drho, const T *phi, const T *p, const T
     pp.y = P.y;
     pp.alpha = P.alpha;
     pp.beta = P.beta;
+    pp.b56 = beta_is_5_6(P.beta);
     pp.n = P.N;
     pp.bs = P.N;
     pp.want_v = 1;

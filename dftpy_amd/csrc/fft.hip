@@ -16,6 +16,7 @@
 
 #include "dft.h"
 #include "internal.h"
+#include "pointwise.h"
 
 namespace ofdft {
 
@@ -84,10 +85,10 @@ __device__ void lds_fft(C *buf, const C *tw, const AxisPlan &ap, int npen, int p
 
 // ------------------------------------------------------------------ prologue
 template <typename T>
-__device__ __forceinline__ T prologue(int code, T x, double beta, const T *aux, long long idx) {
+__device__ __forceinline__ T prologue(int code, T x, double beta, bool b56, const T *aux, long long idx) {
     switch (code) {
         case OP_SQRT_RHOM: return sqrt(x > (T)0 ? x : (T)1e-30);
-        case OP_POW_BETA: return pow(x < (T)1e-30 ? (T)1e-30 : x, (T)beta);
+        case OP_POW_BETA: return pow_beta<T>(x < (T)1e-30 ? (T)1e-30 : x, (T)beta, b56);
         case OP_DSQRT: {
             T r = aux[idx];
             T s = sqrt(r > (T)0 ? r : (T)1e-30);
@@ -96,7 +97,7 @@ __device__ __forceinline__ T prologue(int code, T x, double beta, const T *aux, 
         case OP_DPOW_BETA: {
             T r = aux[idx];
             T rc = r < (T)1e-30 ? (T)1e-30 : r;
-            return (T)beta * pow(rc, (T)beta - (T)1) * x;
+            return (T)beta * pow_beta_m1<T>(rc, (T)beta, b56) * x;
         }
         default: return x;
     }
@@ -164,7 +165,7 @@ __global__ void k_rows_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out, l
         if (r < nv) {
             x = src[e];
             acc += (double)x;
-            x = prologue<T>(code, x, pro.beta, aux, row0 * n + e);
+            x = prologue<T>(code, x, pro.beta, pro.b56, aux, row0 * n + e);
         }
         T *pb = reinterpret_cast<T *>(&buf[(r >> 1) * n + (e - r * n)]);
         pb[r & 1] = x;
@@ -476,8 +477,9 @@ void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in,
 
 template <typename T>
 void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, double scale,
-             const Prologue *pro, double *sum_partials, hipStream_t s, long long pitch) {
+             const Prologue *pro, double *sum_partials, hipStream_t s, long long pitch, bool skip_x) {
     if (pitch <= 0) pitch = P.nzc;
+    OFDFT_REQUIRE(!skip_x || scale == 1.0, "r2c without the x pass cannot scale");
     Prologue pr;
     std::memset(&pr, 0, sizeof(pr));
     int sum_slot = -1;
@@ -501,19 +503,19 @@ void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long
     OFDFT_HIP(hipGetLastError());
     if (last_z) return;
     fft_cols<T>(P, 1, false, false, out, out, batch, out_bs, 1.0, s, pitch);
-    fft_cols<T>(P, 0, false, false, out, out, batch, out_bs, scale, s, pitch);
+    if (!skip_x) fft_cols<T>(P, 0, false, false, out, out, batch, out_bs, scale, s, pitch);
 }
 
 template <typename T>
 void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long long in_bs, long long tmp_bs,
-             long long out_bs, double scale, hipStream_t s, long long pitch) {
+             long long out_bs, double scale, hipStream_t s, long long pitch, bool skip_x) {
     if (pitch <= 0) pitch = P.nzc;
     const cpx<T> *src = in;
     long long src_bs = in_bs;
     if (!(P.nr[0] == 1 && P.nr[1] == 1)) {
         OFDFT_REQUIRE(in_bs == tmp_bs, "c2r: batch strides of input and scratch differ");
-        fft_cols<T>(P, 0, false, true, in, tmp, batch, tmp_bs, 1.0, s, pitch);
-        fft_cols<T>(P, 1, false, true, tmp, tmp, batch, tmp_bs, 1.0, s, pitch);
+        if (!skip_x) fft_cols<T>(P, 0, false, true, in, tmp, batch, tmp_bs, 1.0, s, pitch);
+        fft_cols<T>(P, 1, false, true, skip_x ? in : tmp, tmp, batch, tmp_bs, 1.0, s, pitch);
         src = tmp;
         src_bs = tmp_bs;
     }
@@ -591,9 +593,9 @@ template void bench_rows_c2r<double>(Plan &, const double2 *, double *, int, hip
     template void fft_cols<T>(Plan &, int, bool, bool, const cpx<T> *, cpx<T> *, int, long long, double,       \
                               hipStream_t, long long);                                                        \
     template void fft_r2c<T>(Plan &, const T *, cpx<T> *, int, long long, long long, double, const Prologue *, \
-                             double *, hipStream_t, long long);                                               \
+                             double *, hipStream_t, long long, bool);                                               \
     template void fft_c2r<T>(Plan &, const cpx<T> *, cpx<T> *, T *, int, long long, long long, long long,      \
-                             double, hipStream_t, long long);                                                 \
+                             double, hipStream_t, long long, bool);                                                 \
     template void fft_c2c<T>(Plan &, const cpx<T> *, cpx<T> *, bool, int, long long, double, hipStream_t);
 INST(float)
 INST(double)

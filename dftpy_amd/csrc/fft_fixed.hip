@@ -10,7 +10,9 @@
 // covers the y pass (P = nzc, Q = ny*nzc, S = nzc) and the x pass
 // (P = Q = S = ny*nzc) without partial tiles at plane boundaries.
 #include "dft.h"
+#include "gspace.h"
 #include "internal.h"
+#include "pointwise.h"
 
 namespace ofdft {
 
@@ -144,6 +146,60 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     cstages<C, N, TILE, INV, 0, 1>(buf, tw, in + off, out + off, S, valid, pen, j, scale);
 }
 
+// Stages with a choice of source and destination: GIN reads the first stage
+// from HBM (else from LDS), GOUT writes the last stage to HBM (else to LDS in
+// natural order).
+template <typename C, int N, int TILE, bool INV, int I, int NS, bool GIN, bool GOUT>
+__device__ __forceinline__ void cstages_io(C *buf, const C *tw, const C *src, C *dst, long long S, bool valid,
+                                           int pen, int j, typename Real<C>::type scale) {
+    if constexpr (I < SL<N>::nst) {
+        constexpr int R = SL<N>::template rad<I>();
+        cstage<C, N, TILE, R, NS, INV, GIN && I == 0, GOUT && I == SL<N>::nst - 1>(buf, tw, src, dst, S, valid, pen,
+                                                                                    j, scale);
+        cstages_io<C, N, TILE, INV, I + 1, NS * R, GIN, GOUT>(buf, tw, src, dst, S, valid, pen, j, scale);
+    }
+}
+
+// x pass of the evaluation pipeline with the G-space multiply fused in: the
+// forward x transform of a column tile ends in LDS, each element is scaled by
+// its kernel value (gfactor: vW, WT or Hartree by spectrum slot), and the
+// inverse x transform starts from LDS and writes back in place.  Saves one
+// HBM round trip of every spectrum on each side of the multiply.
+template <typename T, int N, int TILE>
+__global__ void __launch_bounds__(TILE *SL<N>::tpp)
+    k_xfused_f(cpx<T> *__restrict__ data, long long bs, long long ncols, long long pitch, long long vp,
+               const cpx<T> *__restrict__ twg, GParams gp) {
+    using C = cpx<T>;
+    __shared__ C tw[N];
+    __shared__ C buf[N * TILE];
+    constexpr int NT = TILE * SL<N>::tpp;
+    const int pen = threadIdx.x % TILE;
+    const int j = threadIdx.x / TILE;
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
+    const long long c = (long long)blockIdx.x * TILE + pen;
+    const bool valid = c < ncols && c % pitch < vp;
+    C *col = data + (long long)blockIdx.y * bs + (valid ? c : 0);
+    const long long S = ncols;  // x stride of the (nx, ny, pitch) layout
+    double tkf = 1.0, wtfac = 1.0;
+    const int kind = gp.kind[blockIdx.y];
+    if (kind == 1) wt_consts(gp, tkf, wtfac);
+    cstages_io<C, N, TILE, false, 0, 1, true, false>(buf, tw, col, nullptr, S, valid, pen, j, (T)1);
+    for (int i = threadIdx.x; i < N * TILE; i += NT) {
+        const int e = i / TILE, pn = i - e * TILE;
+        const long long cc = (long long)blockIdx.x * TILE + pn;
+        if (cc >= ncols) continue;
+        const int iy = (int)(cc / pitch), iz = (int)(cc % pitch);
+        if (iz >= vp) continue;
+        const double f = gfactor(gp, kind, e, iy, iz, tkf, wtfac);
+        C v = buf[i];
+        v.x = (T)(v.x * f);
+        v.y = (T)(v.y * f);
+        buf[i] = v;
+    }
+    __syncthreads();
+    cstages_io<C, N, TILE, true, 0, 1, false, true>(buf, tw, nullptr, col, S, valid, pen, j, (T)1);
+}
+
 // ---------------------------------------------------------------- rows passes
 // LDS stage for contiguous rows: pencil p element e at buf[p*LD + e].
 template <typename C, int N, int LD, int NPEN, int R, int NS, bool INV>
@@ -187,10 +243,10 @@ __device__ __forceinline__ void rstages(C *buf, const C *tw) {
 }
 
 template <typename T>
-__device__ __forceinline__ T prologue_f(int code, T x, T beta, const T *aux, long long idx) {
+__device__ __forceinline__ T prologue_f(int code, T x, T beta, bool b56, const T *aux, long long idx) {
     switch (code) {
         case OP_SQRT_RHOM: return sqrt(x > (T)0 ? x : (T)1e-30);
-        case OP_POW_BETA: return pow(x < (T)1e-30 ? (T)1e-30 : x, beta);
+        case OP_POW_BETA: return pow_beta<T>(x < (T)1e-30 ? (T)1e-30 : x, beta, b56);
         case OP_DSQRT: {
             const T r = aux[idx];
             return x / ((T)2 * sqrt(r > (T)0 ? r : (T)1e-30));
@@ -198,7 +254,7 @@ __device__ __forceinline__ T prologue_f(int code, T x, T beta, const T *aux, lon
         case OP_DPOW_BETA: {
             const T r = aux[idx];
             const T rc = r < (T)1e-30 ? (T)1e-30 : r;
-            return beta * pow(rc, beta - (T)1) * x;
+            return beta * pow_beta_m1<T>(rc, beta, b56) * x;
         }
         default: return x;
     }
@@ -238,7 +294,7 @@ __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
             T x = x0;
             if (f > 0 && in_bs != 0 && r < nv) x = in[(long long)(slot0 + f) * in_bs + row0 * N + e];
             if (partials != nullptr && slot0 + f == sum_slot) acc += (double)x;
-            if (r < nv) x = prologue_f<T>(pro.code[(slot0 + f) & 3], x, beta, aux, row0 * N + e);
+            if (r < nv) x = prologue_f<T>(pro.code[(slot0 + f) & 3], x, beta, pro.b56 != 0, aux, row0 * N + e);
             reinterpret_cast<T *>(&buf[(f * NPEN + (r >> 1)) * LD + pos])[r & 1] = x;
         }
     }
@@ -402,6 +458,34 @@ bool fixed_size(int n) {
         default: return false;
     }
 }
+
+template <typename T, int N>
+static void launch_xfused(cpx<T> *data, int batch, long long bs, long long ncols, long long pitch, long long vp,
+                          const cpx<T> *tw, const GParams &gp, hipStream_t s) {
+    constexpr int TILE = col_tile<T, N>();
+    dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);
+    hipLaunchKernelGGL((k_xfused_f<T, N, TILE>), grid, dim3(TILE * SL<N>::tpp), 0, s, data, bs, ncols, pitch, vp, tw,
+                       gp);
+}
+
+template <typename T>
+bool xfused_fixed(int n, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch, long long vp,
+                  const void *tw, const GParams &gp, hipStream_t s) {
+    switch (n) {
+#define CASE(N)                                                                                                   \
+    case N:                                                                                                       \
+        launch_xfused<T, N>(data, batch, bs, ncols, pitch, vp, reinterpret_cast<const cpx<T> *>(tw), gp, s);      \
+        return true;
+        OFDFT_FIXED_SIZES(CASE)
+#undef CASE
+        default: return false;
+    }
+}
+
+template bool xfused_fixed<float>(int, cpx<float> *, int, long long, long long, long long, long long, const void *,
+                                  const GParams &, hipStream_t);
+template bool xfused_fixed<double>(int, cpx<double> *, int, long long, long long, long long, long long, const void *,
+                                   const GParams &, hipStream_t);
 
 template <typename T>
 bool cols_fixed(int n, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,

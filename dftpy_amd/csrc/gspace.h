@@ -1,0 +1,113 @@
+// G-space kernels of the evaluation pipeline, shared by the stand-alone
+// multiply (eval.hip) and the x-pass that fuses forward FFT, multiply and
+// inverse FFT (fft_fixed.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace ofdft {
+
+// ------------------------------------------------------------------ Lindhard
+// Inverse Lindhard function with the reference's branch priority
+// normal < zero (eta < tol) < one (|eta-1| < tol) < large (eta > 3.65).
+__device__ __forceinline__ double lindhard(double eta, double lbda, double mu) {
+    const double tol = 1e-10;
+    if (eta > 3.65) {
+        const double inv = 1.0 / (eta * eta);
+        double s = -3.00624946457977689e-3;
+        s = -3.38745061493813488e-3 + inv * s;
+        s = -3.84763737842981233e-3 + inv * s;
+        s = -4.41060829979912465e-3 + inv * s;
+        s = -5.10997527675418131e-3 + inv * s;
+        s = -5.9945117538835746e-3 + inv * s;
+        s = -7.1372762502456763e-3 + inv * s;
+        s = -8.65192783339199453e-3 + inv * s;
+        s = -1.07236045520990083e-2 + inv * s;
+        s = -1.36715733124818332e-2 + inv * s;
+        s = -1.80879839616166146e-2 + inv * s;
+        s = -2.51824061652633074e-2 + inv * s;
+        s = -3.77825602968460128e-2 + inv * s;
+        s = -6.39999999999999875e-2 + inv * s;
+        s = -0.13714285714285712 + inv * s;
+        return 3.0 * (1.0 - mu) * eta * eta - lbda - 0.6 + inv * s;
+    }
+    if (fabs(eta - 1.0) < tol) return 2.0 - lbda - 3.0 * mu + 20.0 * (eta - 1.0);
+    if (eta < tol) return 1.0 - lbda + eta * eta * (1.0 / 3.0 - 3.0 * mu);
+    return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) -
+           3.0 * mu * eta * eta - lbda;
+}
+
+struct GParams {
+    double B[9];
+    int kind[4];  // per spectrum slot: 0 vW, 1 WT, 2 Hartree, -1 none
+    int nx, ny, nz, nzc;
+    int pitch;  // row pitch of the spectra (>= nzc)
+    int full;
+    double scale;
+    int s_vw, s_wt, s_h;
+    double sigma;
+    double lbda, alpha, beta;
+    double rho0;             // used when rho0_dev == nullptr
+    const double *rho0_dev;  // cached rho0 on device
+    long long bs;
+};
+
+__device__ __forceinline__ int ffreq(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
+__device__ __forceinline__ int wrapf(int f, int n) { return ((n & 1) == 0 && f == n / 2) ? -n / 2 : f; }
+
+__device__ __forceinline__ double gsq(const double *B, int fx, int fy, int fz) {
+    const double gx = fx * B[0] + fy * B[3] + fz * B[6];
+    const double gy = fx * B[1] + fy * B[4] + fz * B[7];
+    const double gz = fx * B[2] + fy * B[5] + fz * B[8];
+    return gx * gx + gy * gy + gz * gz;
+}
+
+struct KVals {
+    double vw, wt, h;
+};
+
+__device__ __forceinline__ KVals kernel_vals(const GParams &p, double gg, bool g0, double tkf, double wtfac) {
+    KVals k;
+    k.vw = 0.5 * gg;
+    if (p.sigma > 0) k.vw *= exp(-gg * p.sigma * p.sigma / 4.0);
+    k.wt = lindhard(sqrt(gg) / tkf, p.lbda, 1.0) * wtfac;
+    k.h = g0 ? 0.0 : 4.0 * M_PI / gg;
+    return k;
+}
+
+
+// multiplier of spectrum kind `kind` at frequency indices (ix, iy, iz) of the
+// half grid, including the full-grid Nyquist averaging and the 1/N scale
+__device__ __forceinline__ double gfactor(const GParams &p, int kind, int ix, int iy, int iz, double tkf,
+                                          double wtfac) {
+    int fx = ffreq(ix, p.nx), fy = ffreq(iy, p.ny), fz = iz;
+    const bool g0 = (ix == 0 && iy == 0 && iz == 0);
+    KVals k;
+    if (p.full) {
+        fz = wrapf(fz, p.nz);
+        const bool nyq = ((p.nx & 1) == 0 && ix == p.nx / 2) || ((p.ny & 1) == 0 && iy == p.ny / 2) ||
+                         ((p.nz & 1) == 0 && iz == p.nz / 2);
+        k = kernel_vals(p, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
+        if (nyq) {
+            const KVals k2 =
+                kernel_vals(p, gsq(p.B, wrapf(-fx, p.nx), wrapf(-fy, p.ny), wrapf(-fz, p.nz)), g0, tkf, wtfac);
+            k.vw = 0.5 * (k.vw + k2.vw);
+            k.wt = 0.5 * (k.wt + k2.wt);
+            k.h = 0.5 * (k.h + k2.h);
+        }
+    } else {
+        k = kernel_vals(p, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
+    }
+    const double f = kind == 0 ? k.vw : (kind == 1 ? k.wt : k.h);
+    return f * p.scale;
+}
+
+__device__ __forceinline__ void wt_consts(const GParams &p, double &tkf, double &wtfac) {
+    const double rho0 = p.rho0_dev ? *p.rho0_dev : p.rho0;
+    tkf = 2.0 * cbrt(3.0 * rho0 * M_PI * M_PI);
+    const double ctf = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+    wtfac = 5.0 / (9.0 * p.alpha * p.beta * pow(rho0, p.alpha + p.beta - 5.0 / 3.0)) * ctf;
+}
+
+}  // namespace ofdft

@@ -59,6 +59,7 @@ enum PrologueOp {
 struct Prologue {
     int code[4];
     double beta;
+    int b56;          // beta == 5/6: powers by sqrt and cbrt
     const void *aux;  // second input for the linear-response ops (rho)
 };
 
@@ -108,12 +109,17 @@ void plan_fft_setup(Plan &P);
 void plan_fft_free(Plan &P);
 // `pitch` is the row pitch (complex elements) of the half spectra; <= 0 means
 // the unpadded nz/2+1 layout of the public API
+// skip_x leaves out the x-axis pass (forward: the last pass; inverse: the
+// first), for pipelines that run it fused with the G-space multiply
 template <typename T>
 void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
-             double scale, const Prologue *pro, double *sum_partials, hipStream_t s, long long pitch = 0);
+             double scale, const Prologue *pro, double *sum_partials, hipStream_t s, long long pitch = 0,
+             bool skip_x = false);
 template <typename T>
 void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long long in_bs,
-             long long tmp_bs, long long out_bs, double scale, hipStream_t s, long long pitch = 0);
+             long long tmp_bs, long long out_bs, double scale, hipStream_t s, long long pitch = 0,
+             bool skip_x = false);
+bool fixed_size(int n);
 template <typename T>
 void fft_c2c(Plan &P, const cpx<T> *in, cpx<T> *out, bool inverse, int batch, long long bs, double scale,
              hipStream_t s);

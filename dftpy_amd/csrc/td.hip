@@ -45,6 +45,27 @@ __global__ void k_td_gmul(cpx<T> *spec, long long n, int nx, int ny, int nz, dou
     }
 }
 
+// batched form: element i of the stacked runs has grid index i % n
+template <typename T>
+__global__ void k_td_gmul_b(cpx<T> *spec, long long tot, long long n, int nx, int ny, int nz, double B0, double B1,
+                            double B2, double B3, double B4, double B5, double B6, double B7, double B8, double sigma,
+                            double scale) {
+    GRID_LOOP(j, tot) {
+        const long long i = j % n;
+        const int iz = (int)(i % nz);
+        const long long r = i / nz;
+        const int iy = (int)(r % ny), ix = (int)(r / ny);
+        const int fx = ffreq_td(ix, nx), fy = ffreq_td(iy, ny), fz = ffreq_td(iz, nz);
+        const double gx = fx * B0 + fy * B3 + fz * B6;
+        const double gy = fx * B1 + fy * B4 + fz * B7;
+        const double gz = fx * B2 + fy * B5 + fz * B8;
+        const double gg = gx * gx + gy * gy + gz * gz;
+        const double f = 0.5 * gg * exp(-gg * sigma * sigma / 2.0) * scale;
+        cpx<T> v = spec[j];
+        spec[j] = cpx<T>{(T)(v.x * f), (T)(v.y * f)};
+    }
+}
+
 // out = x + sgn * i (dt/2) (kin + v x)   with kin = -1/2 lap x already in `kin`
 // sums: (mode 1) p.out complex (no conj)   (mode 2) |out|^2
 template <typename T, int MODE>
@@ -202,9 +223,266 @@ template <typename T> struct CN {
     }
 };
 
+// ---------------------------------------------------------------- batched
+// nb independent runs (each its own psi, v and CG scalars) advance in lock
+// step: every Hamiltonian application is one batched C2C transform pair and
+// one launch over all runs (blockIdx.y = run); per-run FP64 partial sums are
+// laid out partials[(run * nblk + block) * K + t].  Runs that have converged
+// stop updating; the loop ends when all have.
+constexpr int kMaxRuns = 16;
+
+struct RunScalars {
+    double ar[kMaxRuns], ai[kMaxRuns];
+    int active[kMaxRuns];
+};
+
+template <int K> __device__ __forceinline__ void run_partials(double *acc, double *partials) {
+    __shared__ double sm[16];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const double s = block_sum(acc[t], sm);
+        if (threadIdx.x == 0) partials[((long long)blockIdx.y * gridDim.x + blockIdx.x) * K + t] = s;
+    }
+}
+
+#define RUN_LOOP(i, n) \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
+
+template <typename T, int MODE>
+__global__ void k_tdb_apply(const cpx<T> *x, const cpx<T> *kin, const T *v, double hdt, cpx<T> *out, long long n,
+                            double *partials) {
+    const long long o = (long long)blockIdx.y * n;
+    x += o;
+    kin += o;
+    v += o;
+    out += o;
+    double acc[2] = {0, 0};
+    RUN_LOOP(i, n) {
+        const cpx<T> a = x[i], k = kin[i];
+        const double vv = (double)v[i];
+        const double hr = (double)k.x + vv * (double)a.x, hi = (double)k.y + vv * (double)a.y;
+        const double orr = (double)a.x - hdt * hi, oi = (double)a.y + hdt * hr;
+        out[i] = cpx<T>{(T)orr, (T)oi};
+        if (MODE == 1) {
+            acc[0] += (double)a.x * orr - (double)a.y * oi;
+            acc[1] += (double)a.x * oi + (double)a.y * orr;
+        } else if (MODE == 2) {
+            acc[0] += orr * orr + oi * oi;
+        }
+    }
+    if (MODE == 1) run_partials<2>(acc, partials);
+    if (MODE == 2) run_partials<1>(acc, partials);
+}
+
+template <typename T>
+__global__ void k_tdb_init(const cpx<T> *b, cpx<T> *res, cpx<T> *p, long long n, double *partials) {
+    const long long o = (long long)blockIdx.y * n;
+    double acc[3] = {0, 0, 0};
+    RUN_LOOP(i, n) {
+        const cpx<T> bb = b[o + i], ax = res[o + i];
+        const cpx<T> r{bb.x - ax.x, bb.y - ax.y};
+        res[o + i] = r;
+        p[o + i] = r;
+        acc[0] += (double)r.x * r.x - (double)r.y * r.y;
+        acc[1] += 2.0 * (double)r.x * r.y;
+        acc[2] += (double)r.x * r.x + (double)r.y * r.y;
+    }
+    run_partials<3>(acc, partials);
+}
+
+template <typename T>
+__global__ void k_tdb_update(cpx<T> *x, cpx<T> *res, const cpx<T> *p, const cpx<T> *v, RunScalars rs, long long n,
+                             double *partials) {
+    const int run = blockIdx.y;
+    const long long o = (long long)run * n;
+    double acc[3] = {0, 0, 0};
+    if (rs.active[run]) {
+        const double ar = rs.ar[run], ai = rs.ai[run];
+        RUN_LOOP(i, n) {
+            const cpx<T> pp = p[o + i], vv = v[o + i], xx = x[o + i], rr = res[o + i];
+            x[o + i] = cpx<T>{(T)(xx.x + ar * pp.x - ai * pp.y), (T)(xx.y + ar * pp.y + ai * pp.x)};
+            const double r0 = rr.x - (ar * vv.x - ai * vv.y), r1 = rr.y - (ar * vv.y + ai * vv.x);
+            res[o + i] = cpx<T>{(T)r0, (T)r1};
+            acc[0] += r0 * r0 - r1 * r1;
+            acc[1] += 2.0 * r0 * r1;
+            acc[2] += r0 * r0 + r1 * r1;
+        }
+    }
+    run_partials<3>(acc, partials);
+}
+
+template <typename T> __global__ void k_tdb_pupd(const cpx<T> *res, cpx<T> *p, RunScalars rs, long long n) {
+    const int run = blockIdx.y;
+    if (!rs.active[run]) return;
+    const long long o = (long long)run * n;
+    const double br = rs.ar[run], bi = rs.ai[run];
+    RUN_LOOP(i, n) {
+        const cpx<T> pp = p[o + i], rr = res[o + i];
+        p[o + i] = cpx<T>{(T)(rr.x + br * pp.x - bi * pp.y), (T)(rr.y + br * pp.y + bi * pp.x)};
+    }
+}
+
+// out[run * K + t] = sum_b partials[(run * nblk + b) * K + t]
+__global__ void k_tdb_reduce(const double *partials, int nblk, int K, double *out) {
+    __shared__ double sm[16];
+    const int run = blockIdx.x / K, t = blockIdx.x % K;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[((long long)run * nblk + b) * K + t];
+    acc = block_sum(acc, sm);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+#undef RUN_LOOP
+
+template <typename T> struct CNBatch {
+    Plan &P;
+    hipStream_t s;
+    long long n;
+    int nb, nblk;
+    double *host;
+    double sigma;
+    cpx<T> *kin, *b, *res, *p, *Ap;
+    CNBatch(Plan &pl, hipStream_t st, double sg, int nruns, void *work)
+        : P(pl), s(st), n(pl.N), nb(nruns), sigma(sg) {
+        nblk = std::max(1, std::min(reduce_blocks(n), 256));
+        OFDFT_REQUIRE((long long)nb * nblk * 3 <= 4096LL * 16, "cn_step_batch: too many runs for the partials");
+        OFDFT_HIP(hipHostMalloc((void **)&host, 3 * kMaxRuns * sizeof(double), hipHostMallocDefault));
+        cpx<T> *w = reinterpret_cast<cpx<T> *>(work);
+        kin = w;
+        b = w + (long long)nb * n;
+        res = w + 2LL * nb * n;
+        p = w + 3LL * nb * n;
+        Ap = w + 4LL * nb * n;
+    }
+    ~CNBatch() { (void)hipHostFree(host); }
+    const double *fetch(int K) {
+        hipLaunchKernelGGL(k_tdb_reduce, dim3(nb * K), dim3(256), 0, s, P.red, nblk, K, P.scal + 128);
+        OFDFT_HIP(hipGetLastError());
+        OFDFT_HIP(hipMemcpyAsync(host, P.scal + 128, (size_t)nb * K * sizeof(double), hipMemcpyDeviceToHost, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+        return host;
+    }
+    void kinetic(const cpx<T> *x) {
+        fft_c2c<T>(P, x, kin, false, nb, n, 1.0, s);
+        const long long tot = (long long)nb * n;
+        hipLaunchKernelGGL((k_td_gmul_b<T>), dim3(std::min<long long>((tot + 255) / 256, 8192)), dim3(256), 0, s, kin,
+                           tot, n, P.nr[0], P.nr[1], P.nr[2], P.B[0], P.B[1], P.B[2], P.B[3], P.B[4], P.B[5], P.B[6],
+                           P.B[7], P.B[8], sigma, 1.0 / (double)n);
+        OFDFT_HIP(hipGetLastError());
+        fft_c2c<T>(P, kin, kin, true, nb, n, 1.0, s);
+    }
+    template <int MODE> void apply(const cpx<T> *x, const T *v, double hdt, cpx<T> *out) {
+        kinetic(x);
+        hipLaunchKernelGGL((k_tdb_apply<T, MODE>), dim3(nblk, nb), dim3(256), 0, s, x, kin, v, hdt, out, n, P.red);
+        OFDFT_HIP(hipGetLastError());
+    }
+    void step(const cpx<T> *psi0, const T *v, double dt, double tol, int maxiter, cpx<T> *x, double *info) {
+        apply<2>(psi0, v, -0.5 * dt, b);
+        const double *h = fetch(1);
+        double atol[kMaxRuns], rr_r[kMaxRuns], rr_i[kMaxRuns], rnorm[kMaxRuns];
+        int status[kMaxRuns], iters[kMaxRuns];
+        for (int r = 0; r < nb; ++r) {
+            const double bn = std::sqrt(h[r]);
+            atol[r] = bn == 0.0 ? tol : tol * bn;
+        }
+        OFDFT_HIP(hipMemcpyAsync(x, psi0, (size_t)nb * n * sizeof(cpx<T>), hipMemcpyDeviceToDevice, s));
+        apply<0>(x, v, 0.5 * dt, res);
+        hipLaunchKernelGGL((k_tdb_init<T>), dim3(nblk, nb), dim3(256), 0, s, b, res, p, n, P.red);
+        OFDFT_HIP(hipGetLastError());
+        h = fetch(3);
+        RunScalars sc{};
+        int nactive = 0;
+        for (int r = 0; r < nb; ++r) {
+            rr_r[r] = h[3 * r];
+            rr_i[r] = h[3 * r + 1];
+            rnorm[r] = std::sqrt(h[3 * r + 2]);
+            iters[r] = 0;
+            status[r] = rnorm[r] < atol[r] ? 0 : 1;
+            sc.active[r] = status[r];
+            nactive += status[r];
+        }
+        for (int k = 0; k < maxiter && nactive > 0; ++k) {
+            apply<1>(p, v, 0.5 * dt, Ap);
+            h = fetch(2);
+            for (int r = 0; r < nb; ++r) {
+                if (!sc.active[r]) continue;
+                const double pv_r = h[2 * r], pv_i = h[2 * r + 1];
+                const double den = pv_r * pv_r + pv_i * pv_i;
+                sc.ar[r] = (rr_r[r] * pv_r + rr_i[r] * pv_i) / den;
+                sc.ai[r] = (rr_i[r] * pv_r - rr_r[r] * pv_i) / den;
+            }
+            hipLaunchKernelGGL((k_tdb_update<T>), dim3(nblk, nb), dim3(256), 0, s, x, res, p, Ap, sc, n, P.red);
+            OFDFT_HIP(hipGetLastError());
+            h = fetch(3);
+            RunScalars bs = sc;
+            for (int r = 0; r < nb; ++r) {
+                if (!sc.active[r]) continue;
+                const double nr_r = h[3 * r], nr_i = h[3 * r + 1];
+                rnorm[r] = std::sqrt(h[3 * r + 2]);
+                iters[r] = k + 1;
+                if (rnorm[r] < atol[r]) {
+                    status[r] = 0;
+                    bs.active[r] = 0;
+                    continue;
+                }
+                const double d2 = rr_r[r] * rr_r[r] + rr_i[r] * rr_i[r];
+                bs.ar[r] = (nr_r * rr_r[r] + nr_i * rr_i[r]) / d2;
+                bs.ai[r] = (nr_i * rr_r[r] - nr_r * rr_i[r]) / d2;
+                rr_r[r] = nr_r;
+                rr_i[r] = nr_i;
+            }
+            hipLaunchKernelGGL((k_tdb_pupd<T>), dim3(nblk, nb), dim3(256), 0, s, res, p, bs, n);
+            OFDFT_HIP(hipGetLastError());
+            nactive = 0;
+            for (int r = 0; r < nb; ++r) {
+                sc.active[r] = bs.active[r];
+                nactive += bs.active[r];
+            }
+        }
+        OFDFT_HIP(hipStreamSynchronize(s));
+        if (info)
+            for (int r = 0; r < nb; ++r) {
+                info[4 * r] = status[r];
+                info[4 * r + 1] = iters[r];
+                info[4 * r + 2] = rnorm[r];
+                info[4 * r + 3] = atol[r];
+            }
+    }
+};
+
 }  // namespace ofdft
 
 using namespace ofdft;
+
+extern "C" size_t ofdft_cn_batch_workspace_bytes(const ofdft_plan *plan, int nb) {
+    const Plan *P = reinterpret_cast<const Plan *>(plan);
+    if (!P || nb < 1) return 0;
+    return (size_t)5 * nb * P->N * (P->prec == OFDFT_FP64 ? 16 : 8);
+}
+
+extern "C" int ofdft_cn_step_batch(ofdft_plan *plan, int nb, const void *psi0, const void *v, double dt,
+                                   double sigma, double tol, int maxiter, void *psi1, double *info, void *work,
+                                   size_t work_bytes, void *stream) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    try {
+        OFDFT_REQUIRE(P && psi0 && v && psi1 && work, "cn_step_batch: bad arguments");
+        OFDFT_REQUIRE(nb >= 1 && nb <= kMaxRuns, "cn_step_batch: 1 to 16 runs");
+        OFDFT_REQUIRE(psi1 != psi0, "cn_step_batch: psi1 must not alias psi0");
+        OFDFT_REQUIRE(work_bytes >= ofdft_cn_batch_workspace_bytes(plan, nb), "cn_step_batch: workspace too small");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (P->prec == OFDFT_FP64) {
+            CNBatch<double> c(*P, s, sigma, nb, work);
+            c.step((const double2 *)psi0, (const double *)v, dt, tol, maxiter, (double2 *)psi1, info);
+        } else {
+            CNBatch<float> c(*P, s, sigma, nb, work);
+            c.step((const float2 *)psi0, (const float *)v, dt, tol, maxiter, (float2 *)psi1, info);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}
 
 extern "C" int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, double sigma, double tol,
                              int maxiter, void *psi1, double *info, void *stream) {

@@ -115,7 +115,8 @@ int ofdft_dot(ofdft_plan *plan, const void *a, const void *b, double *out, void 
  * pass of the evaluation pipeline `iters` times on `stream` between two HIP
  * events and returns the mean duration in ms.  which: 0 z-pass R2C with the
  * vW/WT prologue (reads rho), 1 y-pass, 2 x-pass (C2C on the half grid),
- * 3 z-pass C2R, 4 full-grid C2C 3D transform; batch 1..3 spectra.  Uses the
+ * 3 z-pass C2R, 4 full-grid C2C 3D transform, 5 one complete E+V evaluation
+ * of rho with the plan's functional settings; batch 1..3 spectra.  Uses the
  * plan's workspace as data. */
 int ofdft_bench_pass(ofdft_plan *plan, int which, int batch, int iters, const void *rho, double *avg_ms,
                      void *stream);
@@ -233,6 +234,18 @@ int ofdft_cnn_target(ofdft_plan *plan, const void *rho, const float *x0, double 
 /* rho = rho0 exp(x0[0] + delta) scaled to ne electrons (delta may be NULL) */
 int ofdft_cnn_apply(ofdft_plan *plan, const float *x0, const float *delta, double rho0, double ne, void *rho,
                     void *stream);
+
+/* Crank-Nicolson step for nb independent runs at once (1 <= nb <= 16; no
+ * counterpart in the reference, which propagates one run per process:
+ * td/propagator/crank_nicholson.py:147-188 per run).  psi0/psi1: nb stacked
+ * full complex fields, v: nb stacked real potentials, info (host, 4*nb):
+ * per run as ofdft_cn_step.  Each run keeps its own CG scalars and stops
+ * when it converges.  work: device buffer of at least
+ * ofdft_cn_batch_workspace_bytes(plan, nb) bytes. */
+size_t ofdft_cn_batch_workspace_bytes(const ofdft_plan *plan, int nb);
+int ofdft_cn_step_batch(ofdft_plan *plan, int nb, const void *psi0, const void *v, double dt, double sigma,
+                        double tol, int maxiter, void *psi1, double *info, void *work, size_t work_bytes,
+                        void *stream);
 
 #ifdef __cplusplus
 }
