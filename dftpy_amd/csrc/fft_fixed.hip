@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
         if (cc >= ncols) continue;
         const int iy = (int)(cc / pitch), iz = (int)(cc % pitch);
         if (iz >= vp) continue;
-        const double f = gfactor(gp, kind, e, iy, iz, tkf, wtfac);
+        const double f = gfactor<T>(gp, kind, e, iy, iz, tkf, wtfac);
         C v = buf[i];
         v.x = (T)(v.x * f);
         v.y = (T)(v.y * f);

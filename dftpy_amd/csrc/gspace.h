@@ -11,32 +11,35 @@ namespace ofdft {
 // ------------------------------------------------------------------ Lindhard
 // Inverse Lindhard function with the reference's branch priority
 // normal < zero (eta < tol) < one (|eta-1| < tol) < large (eta > 3.65).
-__device__ __forceinline__ double lindhard(double eta, double lbda, double mu) {
-    const double tol = 1e-10;
+// T = double for the FP64 pipeline, float for the FP32 one
+template <typename T> __device__ __forceinline__ T lindhard_t(T eta, T lbda, T mu) {
+    const T tol = (T)1e-10;
     if (eta > 3.65) {
-        const double inv = 1.0 / (eta * eta);
-        double s = -3.00624946457977689e-3;
-        s = -3.38745061493813488e-3 + inv * s;
-        s = -3.84763737842981233e-3 + inv * s;
-        s = -4.41060829979912465e-3 + inv * s;
-        s = -5.10997527675418131e-3 + inv * s;
-        s = -5.9945117538835746e-3 + inv * s;
-        s = -7.1372762502456763e-3 + inv * s;
-        s = -8.65192783339199453e-3 + inv * s;
-        s = -1.07236045520990083e-2 + inv * s;
-        s = -1.36715733124818332e-2 + inv * s;
-        s = -1.80879839616166146e-2 + inv * s;
-        s = -2.51824061652633074e-2 + inv * s;
-        s = -3.77825602968460128e-2 + inv * s;
-        s = -6.39999999999999875e-2 + inv * s;
-        s = -0.13714285714285712 + inv * s;
-        return 3.0 * (1.0 - mu) * eta * eta - lbda - 0.6 + inv * s;
+        const T inv = (T)1 / (eta * eta);
+        T s = (T)-3.00624946457977689e-3;
+        s = (T)-3.38745061493813488e-3 + inv * s;
+        s = (T)-3.84763737842981233e-3 + inv * s;
+        s = (T)-4.41060829979912465e-3 + inv * s;
+        s = (T)-5.10997527675418131e-3 + inv * s;
+        s = (T)-5.9945117538835746e-3 + inv * s;
+        s = (T)-7.1372762502456763e-3 + inv * s;
+        s = (T)-8.65192783339199453e-3 + inv * s;
+        s = (T)-1.07236045520990083e-2 + inv * s;
+        s = (T)-1.36715733124818332e-2 + inv * s;
+        s = (T)-1.80879839616166146e-2 + inv * s;
+        s = (T)-2.51824061652633074e-2 + inv * s;
+        s = (T)-3.77825602968460128e-2 + inv * s;
+        s = (T)-6.39999999999999875e-2 + inv * s;
+        s = (T)-0.13714285714285712 + inv * s;
+        return (T)3 * ((T)1 - mu) * eta * eta - lbda - (T)0.6 + inv * s;
     }
-    if (fabs(eta - 1.0) < tol) return 2.0 - lbda - 3.0 * mu + 20.0 * (eta - 1.0);
-    if (eta < tol) return 1.0 - lbda + eta * eta * (1.0 / 3.0 - 3.0 * mu);
-    return 1.0 / (0.5 + 0.25 * (1.0 - eta * eta) * log((1.0 + eta) / fabs(1.0 - eta)) / eta) -
-           3.0 * mu * eta * eta - lbda;
+    if (fabs(eta - (T)1) < tol) return (T)2 - lbda - (T)3 * mu + (T)20 * (eta - (T)1);
+    if (eta < tol) return (T)1 - lbda + eta * eta * ((T)(1.0 / 3.0) - (T)3 * mu);
+    return (T)1 / ((T)0.5 + (T)0.25 * ((T)1 - eta * eta) * log(((T)1 + eta) / fabs((T)1 - eta)) / eta) -
+           (T)3 * mu * eta * eta - lbda;
 }
+
+__device__ __forceinline__ double lindhard(double eta, double lbda, double mu) { return lindhard_t<double>(eta, lbda, mu); }
 
 struct GParams {
     double B[9];
@@ -77,29 +80,38 @@ __device__ __forceinline__ KVals kernel_vals(const GParams &p, double gg, bool g
 }
 
 
+// value of one kernel (kind 0 vW, 1 WT, 2 Hartree) at |G|^2 = gg; the WT
+// Lindhard function is evaluated in T
+template <typename T>
+__device__ __forceinline__ double kernel_val(const GParams &p, int kind, double gg, bool g0, double tkf, double wtfac) {
+    if (kind == 0) {
+        double v = 0.5 * gg;
+        if (p.sigma > 0) v *= exp(-gg * p.sigma * p.sigma / 4.0);
+        return v;
+    }
+    if (kind == 1) return (double)lindhard_t<T>((T)(sqrt(gg) / tkf), (T)p.lbda, (T)1) * wtfac;
+    return g0 ? 0.0 : 4.0 * M_PI / gg;
+}
+
 // multiplier of spectrum kind `kind` at frequency indices (ix, iy, iz) of the
 // half grid, including the full-grid Nyquist averaging and the 1/N scale
+template <typename T>
 __device__ __forceinline__ double gfactor(const GParams &p, int kind, int ix, int iy, int iz, double tkf,
                                           double wtfac) {
     int fx = ffreq(ix, p.nx), fy = ffreq(iy, p.ny), fz = iz;
     const bool g0 = (ix == 0 && iy == 0 && iz == 0);
-    KVals k;
+    double f;
     if (p.full) {
         fz = wrapf(fz, p.nz);
         const bool nyq = ((p.nx & 1) == 0 && ix == p.nx / 2) || ((p.ny & 1) == 0 && iy == p.ny / 2) ||
                          ((p.nz & 1) == 0 && iz == p.nz / 2);
-        k = kernel_vals(p, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
-        if (nyq) {
-            const KVals k2 =
-                kernel_vals(p, gsq(p.B, wrapf(-fx, p.nx), wrapf(-fy, p.ny), wrapf(-fz, p.nz)), g0, tkf, wtfac);
-            k.vw = 0.5 * (k.vw + k2.vw);
-            k.wt = 0.5 * (k.wt + k2.wt);
-            k.h = 0.5 * (k.h + k2.h);
-        }
+        f = kernel_val<T>(p, kind, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
+        if (nyq)
+            f = 0.5 * (f + kernel_val<T>(p, kind, gsq(p.B, wrapf(-fx, p.nx), wrapf(-fy, p.ny), wrapf(-fz, p.nz)), g0,
+                                         tkf, wtfac));
     } else {
-        k = kernel_vals(p, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
+        f = kernel_val<T>(p, kind, gsq(p.B, fx, fy, fz), g0, tkf, wtfac);
     }
-    const double f = kind == 0 ? k.vw : (kind == 1 ? k.wt : k.h);
     return f * p.scale;
 }
 

@@ -147,3 +147,44 @@ def test_rho0_explicit_and_fp32_wt(al_pp):
     out32 = Functional(type="KEDF", name="WT")(DirectField(g32, data=rho))
     e64 = OF.tf(rho, og)[0] + OF.vw(rho, og)[0] + OF.wt(rho, og)[0]
     assert abs(out32.energy - e64) < 2e-5
+
+
+@pytest.mark.parametrize("nr", [(64, 48, 32), (48, 64, 40)])
+@pytest.mark.parametrize("full", [True, False])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_fused_x_pass_matches_separate_passes(nr, full, dtype):
+    """The pipeline with the G-space multiply fused into the x pass (size-
+    specialised kernels) against the separate x passes + G-space kernel of
+    the generic kernels, and both against the FP64 oracle."""
+    from dftpy_amd import DirectField, _lib
+    from dftpy_amd.functional import Functional, TotalFunctional
+    lat = np.array([[9.1, 0.3, 0.0], [0.2, 8.4, 0.4], [0.0, 0.5, 7.7]])
+    g = _dg(lat, nr, full, dtype)
+    og = OG.Grid(lat, nr, full=full)
+    r = og.r
+    rho = 0.02 * (1.2 + 0.5 * np.cos(2 * np.pi * r[0] / 9.1) * np.sin(2 * np.pi * r[1] / 8.4 + 0.3)
+                  + 0.2 * np.cos(4 * np.pi * r[2] / 7.7))
+    tot = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                          H=Functional(type="HARTREE"))
+    plan = g.plan(dtype)
+    outs = []
+    for fixed in (1, 0):
+        _lib.check(_lib.lib.ofdft_plan_set_fft_kernels(plan.handle, fixed))
+        o = tot(DirectField(g, data=rho))
+        outs.append((o.energy, o.potential.numpy().astype(np.float64)))
+    _lib.check(_lib.lib.ofdft_plan_set_fft_kernels(plan.handle, 1))
+    kern = OF.wt_kernel(og.q, rho.mean())
+    e_ref = OF.tf(rho, og)[0] + OF.vw(rho, og)[0] + OF.wt(rho, og, kernel=kern)[0] + OF.lda(rho, og)[0] + \
+        OF.hartree(rho, og)[0]
+    v_ref = OF.tf(rho, og)[1] + OF.vw(rho, og)[1] + OF.wt(rho, og, kernel=kern)[1] + OF.lda(rho, og)[1] + \
+        OF.hartree(rho, og)[1]
+    # tolerances relative to the size of the individual terms (the total
+    # energy is a small difference of them)
+    e_scale = sum(abs(f(rho, og)[0]) for f in (OF.tf, OF.vw, OF.lda, OF.hartree)) + \
+        abs(OF.wt(rho, og, kernel=kern)[0])
+    etol, vtol = (1e-10, 1e-10) if dtype == torch.float64 else (2e-6, 1e-4)
+    for e, v in outs:
+        print("dE %.3e (scale %.3e) max dV %.3e" % (e - e_ref, e_scale, np.max(np.abs(v - v_ref))))
+        assert abs(e - e_ref) <= etol * e_scale
+        assert np.max(np.abs(v - v_ref)) <= vtol * np.max(np.abs(v_ref))
+    assert abs(outs[0][0] - outs[1][0]) <= etol * e_scale
