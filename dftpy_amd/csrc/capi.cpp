@@ -95,6 +95,7 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             std::memset(P->tw, 0, sizeof(P->tw));
             P->spec = P->work = P->aux = nullptr;
             P->red = P->scal = nullptr;
+            P->pinned = nullptr;
             for (int d = 0; d < 3; ++d) P->nr[d] = nr[d];
             P->nzc = nr[2] / 2 + 1;
             {
@@ -122,6 +123,7 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             OFDFT_HIP(hipMalloc(&P->red, sizeof(double) * 4096 * 16));
             OFDFT_HIP(hipMalloc(&P->scal, sizeof(double) * 256));
             OFDFT_HIP(hipMemset(P->scal, 0, sizeof(double) * 256));
+            OFDFT_HIP(hipHostMalloc((void **)&P->pinned, 128 * sizeof(double), hipHostMallocDefault));
             P->ws_bytes = spec_bytes + work_bytes + sizeof(double) * (4096 * 16 + 256);
             plan_aux_setup(*P);
             P->ws_bytes += P->aux_bytes;
@@ -150,6 +152,7 @@ int ofdft_plan_destroy(ofdft_plan *P) {
         if (P->work) (void)hipFree(P->work);
         if (P->red) (void)hipFree(P->red);
         if (P->scal) (void)hipFree(P->scal);
+        if (P->pinned) (void)hipHostFree(P->pinned);
         delete P;
     });
 }

@@ -323,7 +323,7 @@ template void eval<double>(Plan &, const double *, unsigned, double *, double *,
 // d rho^beta = beta rho^(beta-1) drho and drho go through the identical
 // G-space multiply; the local terms use dV/drho point-wise.
 template <typename T>
-void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
+void eval_linear(Plan &P, const T *rho, const T *phi, const T *p, hipStream_t s) {
     const unsigned terms = P.terms;
     int ns = 0;
     const int s_vw = (terms & OFDFT_TERM_VW) ? ns++ : -1;
@@ -339,10 +339,11 @@ void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
     pro.beta = P.beta;
     pro.b56 = beta_is_5_6(P.beta);
     pro.aux = rho;
+    pro.phi = phi;  // transforms of drho = 2 phi p
     const long long pitch = pipe_pitch(P);
     const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
     const bool fused = x_fusable(P);
-    fft_r2c<T>(P, drho, spec, ns, 0, ncp, 1.0, &pro, nullptr, s, pitch, fused);
+    fft_r2c<T>(P, p, spec, ns, 0, ncp, 1.0, &pro, nullptr, s, pitch, fused);
     GParams gp;
     for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
     gp.nx = P.nr[0];
@@ -369,7 +370,7 @@ void eval_linear(Plan &P, const T *rho, const T *drho, hipStream_t s) {
 // Ap = (V - mu) p + phi dV[2 phi p]  and  sum p.Ap.
 // base_a = 1/2 L(sqrt(rho)) and base_c = C(rho^beta) of the evaluation at rho.
 template <typename T, typename CT>
-__global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ drho, const T *__restrict__ phi,
+__global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
                       const T *__restrict__ p, const T *__restrict__ pot, const T *__restrict__ work,
                       const T *__restrict__ base_a, const T *__restrict__ base_c, const double *__restrict__ mu,
                       T *__restrict__ Ap, double *partials, PostParams pp) {
@@ -381,7 +382,8 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ drho, con
     double acc = 0.0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < pp.n;
          i += (long long)gridDim.x * blockDim.x) {
-        const CT r = (CT)rho[i], dr = (CT)drho[i];
+        const T phi_i = phi[i], p_i = p[i];
+        const CT r = (CT)rho[i], dr = (CT)(T)(2.0 * (double)phi_i * (double)p_i);  // drho = 2 phi p
         const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
         const CT c_rp = cbrt(rp), s_rp = sqrt(rp);
         CT dv = (CT)0;
@@ -413,18 +415,18 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ drho, con
             dv += (cx3 * rc3 / rp + fc / rp) * dr;
         }
         if (pp.s_h >= 0) dv += (CT)work[pp.s_h * pp.bs + i];
-        const double a = ((double)pot[i] - m) * (double)p[i] + (double)phi[i] * (double)dv;
+        const double a = ((double)pot[i] - m) * (double)p_i + (double)phi_i * (double)dv;
         Ap[i] = (T)a;
-        acc += (double)p[i] * (double)(T)a;
+        acc += (double)p_i * (double)(T)a;
     }
     const double sum = block_sum(acc, sm);
     if (threadIdx.x == 0) partials[blockIdx.x] = sum;
 }
 
 template <typename T>
-void hvp(Plan &P, const T *rho, const T *drho, const T *phi, const T *p, const T *pot, const T *base_a,
-         const T *base_c, const double *mu, T *Ap, int nb, hipStream_t s) {
-    eval_linear<T>(P, rho, drho, s);
+void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T *base_a, const T *base_c,
+         const double *mu, T *Ap, int nb, hipStream_t s) {
+    eval_linear<T>(P, rho, phi, p, s);
     const unsigned terms = P.terms;
     int ns = 0;
     PostParams pp;
@@ -441,15 +443,15 @@ void hvp(Plan &P, const T *rho, const T *drho, const T *phi, const T *p, const T
     pp.bs = P.N;
     pp.want_v = 1;
     pp.want_e = 0;
-    hipLaunchKernelGGL((k_hvp<T, T>), dim3(nb), dim3(256), 0, s, rho, drho, phi, p, pot,
+    hipLaunchKernelGGL((k_hvp<T, T>), dim3(nb), dim3(256), 0, s, rho, phi, p, pot,
                        reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
     OFDFT_HIP(hipGetLastError());
 }
 
 template void hvp<float>(Plan &, const float *, const float *, const float *, const float *, const float *,
-                         const float *, const float *, const double *, float *, int, hipStream_t);
+                         const float *, const double *, float *, int, hipStream_t);
 template void hvp<double>(Plan &, const double *, const double *, const double *, const double *, const double *,
-                          const double *, const double *, const double *, double *, int, hipStream_t);
+                          const double *, const double *, double *, int, hipStream_t);
 
 // slots of the FFT terms in P.work after an evaluation: (vW, WT, Hartree), -1 if absent
 void work_slots(const Plan &P, int *slot) {

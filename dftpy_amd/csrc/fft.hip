@@ -156,6 +156,7 @@ __global__ void k_rows_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out, l
     const int nv = (int)min((long long)rows, nrows - row0);
     const int code = pro.code[slot & 3];
     const T *aux = reinterpret_cast<const T *>(pro.aux);
+    const T *phi = reinterpret_cast<const T *>(pro.phi);
     for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = twg[i];
     const int tot = rows * n;
     double acc = 0.0;
@@ -164,6 +165,7 @@ __global__ void k_rows_r2c(const T *__restrict__ in, cpx<T> *__restrict__ out, l
         T x = (T)0;
         if (r < nv) {
             x = src[e];
+            if (phi) x = (T)(2.0 * (double)phi[row0 * n + e] * (double)x);
             acc += (double)x;
             x = prologue<T>(code, x, pro.beta, pro.b56, aux, row0 * n + e);
         }

@@ -282,13 +282,17 @@ __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
     const int nv = (int)min((long long)ROWS, nrows - row0);
     const T beta = (T)pro.beta;
     const T *aux = reinterpret_cast<const T *>(pro.aux);
+    const T *phi = reinterpret_cast<const T *>(pro.phi);
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
     double acc = 0.0;
     for (int e = threadIdx.x; e < ROWS * N; e += NT) {
         const int r = e / N;
         const int pos = e - r * N;
         T x0 = (T)0;
-        if (r < nv) x0 = in[(long long)slot0 * in_bs + row0 * N + e];
+        if (r < nv) {
+            x0 = in[(long long)slot0 * in_bs + row0 * N + e];
+            if (phi) x0 = (T)(2.0 * (double)phi[row0 * N + e] * (double)x0);
+        }
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             T x = x0;

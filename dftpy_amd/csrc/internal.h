@@ -61,6 +61,7 @@ struct Prologue {
     double beta;
     int b56;          // beta == 5/6: powers by sqrt and cbrt
     const void *aux;  // second input for the linear-response ops (rho)
+    const void *phi;  // if set, the input x is replaced by 2 phi x (drho = 2 phi p of a rotation p)
 };
 
 // One plan: grid geometry, FFT plans, workspace and functional settings.
@@ -89,6 +90,7 @@ struct Plan {
     void *aux;     // optimizer / propagator fields (allocated lazily at first use? no: at create)
     size_t aux_bytes;
     size_t ws_bytes;
+    double *pinned;  // 128 doubles of pinned host memory for scalar read-backs (optimizer, propagator)
 
     // functional settings
     double x = 1.0, y = 1.0, alpha = 5.0 / 6.0, beta = 5.0 / 6.0, rho0 = -1.0, sigma = -1.0;
@@ -145,9 +147,10 @@ void reduce_final(const double *partials, int nblocks, int nterms, double *out, 
 template <typename T>
 void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s);
 // analytic Hessian-vector product of the residual (V - mu) phi along p
+// (drho = 2 phi p is formed inside the kernels)
 template <typename T>
-void hvp(Plan &P, const T *rho, const T *drho, const T *phi, const T *p, const T *pot, const T *base_a,
-         const T *base_c, const double *mu, T *Ap, int nb, hipStream_t s);
+void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T *base_a, const T *base_c,
+         const double *mu, T *Ap, int nb, hipStream_t s);
 void work_slots(const Plan &P, int *slot);
 
 }  // namespace ofdft

@@ -19,7 +19,7 @@ namespace ofdft {
 
 enum AuxField {
     F_PHI, F_RES, F_P, F_DIR, F_BEST, F_TRES, F_TP, F_PHI1, F_RHO1, F_POT1, F_AP, F_RES_PREV, F_DIR_PREV,
-    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_BASE_A, F_BASE_C, F_DRHO, F_COUNT
+    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_BASE_A, F_BASE_C, F_COUNT
 };
 
 void plan_aux_setup(Plan &P) {
@@ -119,11 +119,6 @@ __global__ void k_tn_shift(const T *phi, const T *p, double eps, T *phi1, T *rho
     }
 }
 
-// drho = 2 phi p
-template <typename T> __global__ void k_drho(const T *phi, const T *p, T *drho, long long n) {
-    GRID_LOOP(i, n) drho[i] = (T)(2.0 * (double)phi[i] * (double)p[i]);
-}
-
 // Ap = ((V1 - mu) phi1 - res0) / eps ; sum p.Ap   (optimization.py:105-114)
 template <typename T>
 __global__ void k_tn_ap(const T *pot1, const T *phi1, const T *res0, const T *p, const double *mu, double eps, T *Ap,
@@ -138,15 +133,22 @@ __global__ void k_tn_ap(const T *pot1, const T *phi1, const T *res0, const T *p,
     write_partials<1>(acc, partials);
 }
 
-// dir += alpha p ; tres -= alpha Ap ; sum tres^2
+// dir += alpha p ; tres -= alpha Ap ; sum tres^2, with alpha = r0 / pAp taken
+// from the device (pAp reduced by the previous launch), so the inner CG needs one host round trip per iteration;
+// negative curvature (pAp < 0) leaves dir and tres untouched for the host.
 template <typename T>
-__global__ void k_tn_update(T *dir, T *tres, const T *p, const T *Ap, double alpha, long long n, double *partials) {
+__global__ void k_tn_update_dev(T *dir, T *tres, const T *p, const T *Ap, double r0, const double *pap, long long n,
+                                double *partials) {
+    const double pa = *pap;
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
-        dir[i] = (T)((double)dir[i] + alpha * (double)p[i]);
-        const T r = (T)((double)tres[i] - alpha * (double)Ap[i]);
-        tres[i] = r;
-        acc[0] += (double)r * (double)r;
+    if (!(pa < 0.0)) {
+        const double alpha = r0 / pa;
+        GRID_LOOP(i, n) {
+            dir[i] = (T)((double)dir[i] + alpha * (double)p[i]);
+            const T r = (T)((double)tres[i] - alpha * (double)Ap[i]);
+            tres[i] = r;
+            acc[0] += (double)r * (double)r;
+        }
     }
     write_partials<1>(acc, partials);
 }
@@ -245,9 +247,9 @@ struct Ctx {
     double *host;  // pinned
     Ctx(Plan &p, hipStream_t st) : P(p), s(st) {
         nb = std::min(reduce_blocks(P.N), 1024);
-        OFDFT_HIP(hipHostMalloc((void **)&host, 64 * sizeof(double), hipHostMallocDefault));
+        host = P.pinned;
     }
-    ~Ctx() { (void)hipHostFree(host); }
+
     dim3 grid() const { return dim3(nb); }
     // reduce K partial columns into scal[slot..slot+K) and copy them to the host
     const double *fetch(int K, int slot = 16) {
@@ -260,6 +262,12 @@ struct Ctx {
         // partials laid out [block][K]
         std::vector<double> ones(K, 1.0);
         reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s);
+    }
+    // copy scal[slot..slot+K) to the host
+    const double *to_host(int K, int slot) {
+        OFDFT_HIP(hipMemcpyAsync(host, P.scal + slot, K * sizeof(double), hipMemcpyDeviceToHost, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+        return host;
     }
     void set_scalar(int slot, double v) {
         host[32] = v;
@@ -330,9 +338,7 @@ template <typename T> struct Opt {
         int it = 0;
         for (it = 0; it < maxfun; ++it) {
             if (hvp_mode) {
-                hipLaunchKernelGGL((k_drho<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, F(F_DRHO), n);
-                hvp<T>(P, F(F_RHO), F(F_DRHO), F(F_PHI), tp, F(F_POT), F(F_BASE_A), F(F_BASE_C), P.scal + 2, Ap,
-                       C.nb, C.s);
+                hvp<T>(P, F(F_RHO), F(F_PHI), tp, F(F_POT), F(F_BASE_A), F(F_BASE_C), P.scal + 2, Ap, C.nb, C.s);
                 ++n_v;
             } else {
                 hipLaunchKernelGGL((k_tn_shift<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, eps, phi1, rho1, n);
@@ -340,14 +346,17 @@ template <typename T> struct Opt {
                 hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps,
                                    Ap, n, P.red);
             }
-            const double pAp = C.fetch(1)[0];
+            // pAp -> scal[16]; the update takes alpha from there; r1 -> scal[17]
+            C.reduce_final_cols(1, 16);
+            hipLaunchKernelGGL((k_tn_update_dev<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, r0, P.scal + 16,
+                               n, P.red);
+            C.reduce_final_cols(1, 17);
+            const double *hh = C.to_host(2, 16);
+            const double pAp = hh[0], r1 = hh[1];
             if (pAp < 0.0) {
                 if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, r0 / pAp, tp, 0.0, dir, n);
                 break;
             }
-            const double alpha = r0 / pAp;
-            hipLaunchKernelGGL((k_tn_update<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, alpha, n, P.red);
-            const double r1 = C.fetch(1)[0];
             double rmin = rl[0];
             for (double v : rl) rmin = std::min(rmin, v);
             if (r1 < rmin) {

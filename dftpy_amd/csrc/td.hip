@@ -147,10 +147,10 @@ template <typename T> struct CN {
     cpx<T> *kin, *tmp;
     CN(Plan &p, hipStream_t st, double sg) : P(p), s(st), n(p.N), sigma(sg) {
         nb = std::min(reduce_blocks(n), 1024);
-        OFDFT_HIP(hipHostMalloc((void **)&host, 16 * sizeof(double), hipHostMallocDefault));
+        host = P.pinned;
         kin = reinterpret_cast<cpx<T> *>(P.spec);
     }
-    ~CN() { (void)hipHostFree(host); }
+
     const double *fetch(int K) {
         std::vector<double> ones(K, 1.0);
         reduce_final(P.red, nb, K, P.scal + 64, ones.data(), s);
@@ -346,7 +346,7 @@ template <typename T> struct CNBatch {
         : P(pl), s(st), n(pl.N), nb(nruns), sigma(sg) {
         nblk = std::max(1, std::min(reduce_blocks(n), 256));
         OFDFT_REQUIRE((long long)nb * nblk * 3 <= 4096LL * 16, "cn_step_batch: too many runs for the partials");
-        OFDFT_HIP(hipHostMalloc((void **)&host, 3 * kMaxRuns * sizeof(double), hipHostMallocDefault));
+        host = P.pinned;
         cpx<T> *w = reinterpret_cast<cpx<T> *>(work);
         kin = w;
         b = w + (long long)nb * n;
@@ -354,7 +354,7 @@ template <typename T> struct CNBatch {
         p = w + 3LL * nb * n;
         Ap = w + 4LL * nb * n;
     }
-    ~CNBatch() { (void)hipHostFree(host); }
+
     const double *fetch(int K) {
         hipLaunchKernelGGL(k_tdb_reduce, dim3(nb * K), dim3(256), 0, s, P.red, nblk, K, P.scal + 128);
         OFDFT_HIP(hipGetLastError());

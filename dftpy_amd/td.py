@@ -62,6 +62,45 @@ class CrankNicholson:
         return DirectField(grid, data=out, cplx=True), int(info[0])
 
 
+class CrankNicholsonBatch:
+    """Crank-Nicolson steps of several independent runs at once
+    (``ofdft_cn_step_batch``): each run has its own orbital, potential and
+    CG scalars; every Hamiltonian application is one batched transform pair.
+    The reference propagates one run per process
+    (td/propagator/crank_nicholson.py:147-188)."""
+
+    def __init__(self, interval, tol=1e-8, maxiter=100, sigma=0.025):
+        self.interval = interval
+        self.tol = tol
+        self.maxiter = maxiter
+        self.sigma = sigma
+        self._work = {}
+        self.last_info = None
+
+    def __call__(self, grid, psi0, v):
+        """psi0: (nb, nx, ny, nz) complex tensor, v: (nb, nx, ny, nz) real.
+        Returns psi1 and the per-run status list."""
+        plan = grid.plan(grid.dtype)
+        nb = int(psi0.shape[0])
+        ctype = torch.complex128 if grid.dtype == torch.float64 else torch.complex64
+        src = psi0.to(ctype).contiguous()
+        vv = v.to(grid.dtype).contiguous()
+        out = torch.empty_like(src)
+        nbytes = int(_lib.lib.ofdft_cn_batch_workspace_bytes(plan.handle, nb))
+        key = (id(plan), nb)
+        if key not in self._work:
+            self._work = {key: torch.empty(nbytes, dtype=torch.uint8, device=src.device)}
+        work = self._work[key]
+        info = np.zeros(4 * nb)
+        _lib.check(_lib.lib.ofdft_cn_step_batch(plan.handle, nb, src.data_ptr(), vv.data_ptr(), float(self.interval),
+                                                float(self.sigma), float(self.tol), int(self.maxiter),
+                                                out.data_ptr(), _lib.dptr(info), work.data_ptr(), nbytes,
+                                                _lib.stream()))
+        self.last_info = [dict(status=int(info[4 * r]), iterations=int(info[4 * r + 1]), residual=info[4 * r + 2],
+                               atol=info[4 * r + 3]) for r in range(nb)]
+        return out, [d["status"] for d in self.last_info]
+
+
 class Propagator:
     """``Propagator(hamiltonian, interval, name='crank-nicholson')``."""
 

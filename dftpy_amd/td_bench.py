@@ -1,0 +1,138 @@
+"""Batched TD-OFDFT benchmark: six delta-kicked Crank-Nicolson runs in lock step.
+
+    python -m dftpy_amd.td_bench [--steps 2000] [--nr 128] [--reps 4] [--out FILE]
+
+Workload: fcc Al 4x4x4 (256 atoms, a = 7.60 bohr, sigma = 0.1 bohr
+displacements, seed 7), 128^3 full grid, ground state of TF + vW + WT + PZ-LDA
++ Hartree + the empty-core local PP.  The propagation follows the reference's
+TD runner for a real potential: psi0 = sqrt(rho0) exp(i k r_d)
+(td/utils.py:10-16), then per step psi <- CN(psi; v[|psi|^2]) with the
+potential of the current density (td/propagator/crank_nicholson.py:147-188,
+td/hamiltonian.py:70-114) and the dipole int (rho - rho0) r dV
+(real_time_runner.py:251-253).  The kinetic operator -1/2 lap carries the vW
+term, so v is the potential of TF + WT nonlocal + LDA + Hartree + local PP,
+and the ground state is stationary.  Kicks: k in {1e-3, 2e-3} along x, y and
+z (6 runs), dt = 0.1 a.u.
+
+The ground state comes from the FP32 TN optimizer from the learned start,
+finished in FP64.  The same six runs are propagated in FP32 (complex64 FFTs and
+matvecs, FP64 inner products) and in FP64 (complex128); the dipole-trace error
+of the FP32 runs is reported against the FP64 runs, and steps per second of the
+batch for both.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import DirectField, DirectGrid, systems
+from .functional import Functional, TotalFunctional
+from .learned import WarmStartNet, superposition_density
+from .optimization import Optimization
+from .td import CrankNicholsonBatch, calc_rho, initial_kick
+
+KICKS = [(1e-3, 0), (1e-3, 1), (1e-3, 2), (2e-3, 0), (2e-3, 1), (2e-3, 2)]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ground_state(ions, nr, pp_path):
+    g64 = DirectGrid(ions.cell, nr, full=True, dtype=torch.float64)
+    g32 = DirectGrid(ions.cell, nr, full=True, dtype=torch.float32)
+    pp = Functional(type="PSEUDO", grid=g64, ions=ions, PP_list={"Al": pp_path})
+    ev = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                         H=Functional(type="HARTREE"), PSEUDO=pp)
+    net = WarmStartNet.load()
+    ne = float(ions.get_ncharges())
+    rho0 = ne / g64.volume
+    rho_s = superposition_density(g64, ions, pp.readpp, rho0)
+    x0 = net.features(g64.plan(torch.float64), rho_s, pp.vreal_for(g64).data, rho0)
+    start = net.start_density(g32, x0, rho0, ne)
+    nat = ions.nat
+    r32 = Optimization("TN", optimization_options={"econv": 1e-8 * nat, "maxiter": 60},
+                       EnergyEvaluator=ev).optimize_rho(start)
+    o64 = Optimization("TN", optimization_options={"econv": 1e-10 * nat, "maxiter": 60, "hvp": "analytic"},
+                       EnergyEvaluator=ev)
+    r64 = o64.optimize_rho(DirectField(g64, data=r32.data.double()))
+    return pp, r64, o64.stats
+
+
+def propagate(ions, pp, rho0_64, dtype, nsteps, dt, tol):
+    nr = tuple(int(n) for n in rho0_64.grid.nrR)
+    g = DirectGrid(ions.cell, nr, full=True, dtype=dtype)
+    tdf = TotalFunctional(TF=Functional(type="KEDF", name="TF"), NL=Functional(type="KEDF", name="WT-NL"),
+                          XC=Functional(type="XC", name="LDA"), H=Functional(type="HARTREE"), PSEUDO=pp)
+    rho0 = DirectField(g, data=rho0_64.data.to(dtype))
+    sq = DirectField(g, data=torch.sqrt(rho0.data))
+    psi = torch.stack([initial_kick(k, d, sq).data for k, d in KICKS])
+    r = g.r.to(torch.float64)
+    cn = CrankNicholsonBatch(dt, tol=tol, maxiter=200)
+    nb = len(KICKS)
+    dip = np.zeros((nsteps, nb))
+    iters = np.zeros((nsteps, nb), dtype=np.int64)
+
+    def potentials(ps):
+        return torch.stack([tdf.compute(calc_rho(DirectField(g, data=ps[i], cplx=True)), calcType=["V"]).potential.data
+                            for i in range(nb)])
+
+    v = potentials(psi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(nsteps):
+        psi, st = cn(g, psi, v)
+        if any(st):
+            raise RuntimeError("CN solve did not converge at step %d: %s" % (s, cn.last_info))
+        iters[s] = [d["iterations"] for d in cn.last_info]
+        v = potentials(psi)
+        rho = (psi.real ** 2 + psi.imag ** 2).to(torch.float64)
+        d = ((rho - rho0.data.to(torch.float64)[None]) * torch.stack([r[dd] for _, dd in KICKS])).sum(dim=(1, 2, 3))
+        dip[s] = (d * g.dV).cpu().numpy()
+        if (s + 1) % 200 == 0:
+            log("%s step %d/%d (%.1f s)" % (str(dtype), s + 1, nsteps, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dip, iters, wall
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--nr", type=int, default=128)
+    ap.add_argument("--reps", type=int, default=4)
+    ap.add_argument("--dt", type=float, default=0.1)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    ions = systems.fcc_al((args.reps,) * 3, sigma=0.1, seed=7)
+    nr = (args.nr,) * 3
+    t0 = time.perf_counter()
+    pp, rho0, gs = ground_state(ions, nr, systems.empty_core_recpot())
+    log("ground state: E = %.10f, %d iters (%.2f s)" % (gs["energy"], gs["niter"], time.perf_counter() - t0))
+    res = {}
+    for dtype, tol in ((torch.float32, 1e-6), (torch.float64, 1e-10)):
+        dip, iters, wall = propagate(ions, pp, rho0, dtype, args.steps, args.dt, tol)
+        res[str(dtype).split(".")[-1]] = dict(dipole=dip, iters=iters, wall=wall)
+        log("%s: %d steps in %.2f s (%.1f steps/s), mean CG iterations %.2f" %
+            (dtype, args.steps, wall, args.steps / wall, iters.mean()))
+    d32, d64 = res["float32"]["dipole"], res["float64"]["dipole"]
+    scale = np.abs(d64).max(axis=0)
+    err = np.abs(d32 - d64).max(axis=0) / scale
+    line = dict(workload="fcc Al 4x4x4 (256 atoms), %d^3 full grid, 6 kicked runs batched, dt %.2f, %d steps" %
+                (args.nr, args.dt, args.steps),
+                steps_per_s_fp32=args.steps / res["float32"]["wall"], steps_per_s_fp64=args.steps / res["float64"]["wall"],
+                cg_iters_mean_fp32=float(res["float32"]["iters"].mean()),
+                cg_iters_mean_fp64=float(res["float64"]["iters"].mean()),
+                dipole_trace_rel_err_fp32_vs_fp64=err.tolist(), dipole_max_abs_fp64=scale.tolist(),
+                ground_state_energy=gs["energy"], kicks=KICKS)
+    print(json.dumps(line), flush=True)
+    if args.out:
+        np.savez(args.out, d32=d32, d64=d64, it32=res["float32"]["iters"], it64=res["float64"]["iters"])
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
