@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libofdft_hip.so")
 
 FP64, FP32 = 0, 1
 CALC_E, CALC_V = 1, 2
-TERM_TF, TERM_VW, TERM_WT, TERM_XC, TERM_HARTREE, TERM_PP = 1, 2, 4, 8, 16, 32
+TERM_TF, TERM_VW, TERM_WT, TERM_XC, TERM_HARTREE, TERM_PP, TERM_LMGP = 1, 2, 4, 8, 16, 32, 64
 TERMS_ALL = 63
 E_TF, E_VW, E_NL, E_XC, E_H, E_PP, E_SUM, E_NELEC, E_COUNT = range(9)
 
@@ -43,6 +43,9 @@ SIGNATURES = {
     "ofdft_set_terms": (_I, [_P, ctypes.c_uint]),
     "ofdft_set_vloc": (_I, [_P, _P]),
     "ofdft_set_eion": (_I, [_P, _D]),
+    "ofdft_mgp_kernel_table": (_I, [_I, _D, _I, _I, _DP, _DP, _DP]),
+    "ofdft_set_lmgp": (_I, [_P, _I, _D, _DP, _DP, _D]),
+    "ofdft_lmgp_reset": (_I, [_P]),
     "ofdft_eval": (_I, [_P, _P, ctypes.c_uint, _P, _P, _P]),
     "ofdft_dot": (_I, [_P, _P, _P, _P, _P]),
     "ofdft_bench_pass": (_I, [_P, _I, _I, _I, _P, _DP, _P]),

@@ -148,6 +148,7 @@ int ofdft_plan_destroy(ofdft_plan *P) {
     return guard([&] {
         plan_fft_free(*P);
         plan_aux_free(*P);
+        lmgp_free(*P);
         if (P->spec) (void)hipFree(P->spec);
         if (P->work) (void)hipFree(P->work);
         if (P->red) (void)hipFree(P->red);
@@ -220,7 +221,8 @@ int ofdft_set_kedf(ofdft_plan *P, double x, double y, double alpha, double beta,
 int ofdft_set_terms(ofdft_plan *P, unsigned terms) {
     return guard([&] {
         OFDFT_REQUIRE(P, "null plan");
-        OFDFT_REQUIRE((terms & ~63u) == 0, "unknown term bits");
+        OFDFT_REQUIRE((terms & ~127u) == 0, "unknown term bits");
+        OFDFT_REQUIRE(!((terms & OFDFT_TERM_WT) && (terms & OFDFT_TERM_LMGP)), "WT and LMGP are exclusive");
         P->terms = terms;
     });
 }

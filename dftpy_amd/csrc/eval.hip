@@ -312,6 +312,10 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
         OFDFT_HIP(hipGetLastError());
     }
+    if (terms & OFDFT_TERM_LMGP) {
+        OFDFT_REQUIRE(P.lmgp != nullptr, "LMGP term without tables (ofdft_set_lmgp)");
+        lmgp_eval<T>(P, rho, pp.want_v ? pot : nullptr, pp.want_e ? energies : nullptr, s);
+    }
 }
 
 template void eval<float>(Plan &, const float *, unsigned, float *, double *, hipStream_t);
@@ -426,6 +430,8 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
 template <typename T>
 void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T *base_a, const T *base_c,
          const double *mu, T *Ap, int nb, hipStream_t s) {
+    OFDFT_REQUIRE(!(P.terms & OFDFT_TERM_LMGP), "the analytic Hessian product does not cover LMGP; use the "
+                                                 "finite-difference one");
     eval_linear<T>(P, rho, phi, p, s);
     const unsigned terms = P.terms;
     int ns = 0;

@@ -64,6 +64,8 @@ struct Prologue {
     const void *phi;  // if set, the input x is replaced by 2 phi x (drho = 2 phi p of a rotation p)
 };
 
+struct LmgpState;  // LMGP nonlocal KEDF tables and workspace (lmgp.hip)
+
 // One plan: grid geometry, FFT plans, workspace and functional settings.
 struct Plan {
     int nr[3];
@@ -97,6 +99,7 @@ struct Plan {
     unsigned terms = 63;
     const void *vloc = nullptr;
     double eion = 0.0;
+    LmgpState *lmgp = nullptr;
     bool rho0_cached_valid = false;
     bool use_fixed = true;  // size-specialised FFT kernels where available
 };
@@ -152,5 +155,10 @@ template <typename T>
 void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T *base_a, const T *base_c,
          const double *mu, T *Ap, int nb, hipStream_t s);
 void work_slots(const Plan &P, int *slot);
+
+// lmgp.hip: adds the LMGP nonlocal energy (energies[E_NL], E_SUM) and
+// potential (into pot) of rho
+template <typename T> void lmgp_eval(Plan &P, const T *rho, T *pot, double *energies, hipStream_t s);
+void lmgp_free(Plan &P);
 
 }  // namespace ofdft

@@ -100,7 +100,7 @@ class AbstractFunctional:
 
 def _energy_of(terms, e):
     s = 0.0
-    for bit, idx in ((_lib.TERM_TF, _lib.E_TF), (_lib.TERM_VW, _lib.E_VW), (_lib.TERM_WT, _lib.E_NL),
+    for bit, idx in ((_lib.TERM_TF, _lib.E_TF), (_lib.TERM_VW, _lib.E_VW), (_lib.TERM_WT | _lib.TERM_LMGP, _lib.E_NL),
                      (_lib.TERM_XC, _lib.E_XC), (_lib.TERM_HARTREE, _lib.E_H), (_lib.TERM_PP, _lib.E_PP)):
         if terms & bit:
             s += e[idx]
@@ -124,6 +124,12 @@ def configure(plan, funcs, rho_grid, eion=0.0):
     _lib.check(_lib.lib.ofdft_set_kedf(plan.handle, float(kedf["x"]), float(kedf["y"]), float(kedf["alpha"]),
                                        float(kedf["beta"]), float(rho0) if rho0 else -1.0,
                                        float(kedf["sigma"]) if kedf["sigma"] else -1.0))
+    if terms & _lib.TERM_LMGP and not getattr(plan, "lmgp_ready", False):
+        from .lmgp import lmgp_tables
+        eta, kc, dc = lmgp_tables()
+        _lib.check(_lib.lib.ofdft_set_lmgp(plan.handle, int(eta.size), float(eta[-1]), _lib.dptr(kc), _lib.dptr(dc),
+                                           1.15))
+        plan.lmgp_ready = True
     _lib.check(_lib.lib.ofdft_set_terms(plan.handle, terms))
     _lib.check(_lib.lib.ofdft_set_vloc(plan.handle, vloc.data.data_ptr() if vloc is not None else None))
     _lib.check(_lib.lib.ofdft_set_eion(plan.handle, float(eion)))
@@ -180,6 +186,7 @@ KEDF_TERMS = {
     "TF": _lib.TERM_TF, "VW": _lib.TERM_VW, "WT-NL": _lib.TERM_WT,
     "TFVW": _lib.TERM_TF | _lib.TERM_VW, "X_TF_Y_VW": _lib.TERM_TF | _lib.TERM_VW, "XTFYVW": _lib.TERM_TF | _lib.TERM_VW,
     "WT": _lib.TERM_TF | _lib.TERM_VW | _lib.TERM_WT, "NONE": 0,
+    "LMGP-NL": _lib.TERM_LMGP, "LMGP": _lib.TERM_TF | _lib.TERM_VW | _lib.TERM_LMGP,
 }
 
 

@@ -28,6 +28,7 @@ import torch
 from . import _lib
 from .field import DirectField
 from .functional.pseudo import RadialTable
+from .math_utils import lindhard
 
 C = 32
 NTAP = 27
@@ -43,35 +44,6 @@ CTF = 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
 
 
 # ------------------------------------------------------------ host-side tables
-def _lindhard_inv(eta, lbda=1.0, mu=1.0):
-    """Inverse Lindhard function (kernel.py:13-136), for the host-side
-    response table only; the evaluation kernels compute it on the device."""
-    eta = np.asarray(eta, dtype=np.float64)
-    out = np.empty_like(eta)
-    small = eta < 1e-10
-    one = np.abs(eta - 1.0) < 1e-10
-    large = eta > 3.65
-    mid = ~(small | one | large)
-    e = eta[mid]
-    out[mid] = 1.0 / (0.5 + 0.25 * (1.0 - e * e) * np.log((1.0 + e) / np.abs(1.0 - e)) / e) - 3.0 * mu * e * e - lbda
-    e = eta[small]
-    out[small] = 1.0 - lbda + e * e * (1.0 / 3.0 - 3.0 * mu)
-    e = eta[one]
-    out[one] = 2.0 - lbda - 3.0 * mu + 20.0 * (e - 1.0)
-    e = eta[large]
-    # 1/eta^2 series of the inverse Lindhard function (mu = 1 here)
-    coefs = [-0.13714285714285712, -6.39999999999999875e-2, -3.77825602968460128e-2, -2.51824061652633074e-2,
-             -1.80879839616166146e-2, -1.36715733124818332e-2, -1.07236045520990083e-2, -8.65192783339199453e-3,
-             -7.1372762502456763e-3, -5.9945117538835746e-3, -5.10997527675418131e-3, -4.41060829979912465e-3,
-             -3.84763737842981233e-3, -3.38745061493813488e-3, -3.00624946457977689e-3]
-    inv = 1.0 / (e * e)
-    s = np.full_like(e, coefs[-1])
-    for c in coefs[-2::-1]:
-        s = c + inv * s
-    out[large] = 3.0 * (1.0 - mu) * e * e - lbda - 0.6 + inv * s
-    return out
-
-
 def _lda_dv(rho0):
     """d v_xc / d rho of the PZ LDA at a uniform density (semilocal_xc.py:291-303)."""
     a, c, d = 0.0311, 0.0020, -0.0116
@@ -95,7 +67,7 @@ def response_kernel(q, rho0, x=1.0, y=1.0, alpha=5.0 / 6.0):
     wtfac = 5.0 / (9.0 * alpha * alpha * rho0 ** (2 * alpha - 5.0 / 3.0)) * CTF
     k_tf = x * (10.0 / 9.0) * CTF * rho0 ** (-1.0 / 3.0)
     k_vw = y * q * q / (4.0 * rho0)
-    k_wt = 2.0 * alpha * alpha * rho0 ** (2 * alpha - 2) * _lindhard_inv(q / tkf, x, 1.0) * wtfac
+    k_wt = 2.0 * alpha * alpha * rho0 ** (2 * alpha - 2) * lindhard(q / tkf, x, 1.0) * wtfac
     with np.errstate(divide="ignore"):
         k_h = np.where(q > 0, 4.0 * np.pi / np.maximum(q, 1e-300) ** 2, np.inf)
     return k_tf + k_vw + k_wt + k_h + _lda_dv(rho0)

@@ -77,3 +77,35 @@ def cubic_table(x, y):
     cs = CubicSpline(x, y, bc_type="not-a-knot")
     c = cs.c  # (4, n-1), highest power first
     return np.ascontiguousarray(np.stack([c[3], c[2], c[1], c[0]], axis=1))
+
+
+_LARGE_ETA_SERIES = [-0.13714285714285712, -6.39999999999999875e-2, -3.77825602968460128e-2, -2.51824061652633074e-2,
+                     -1.80879839616166146e-2, -1.36715733124818332e-2, -1.07236045520990083e-2,
+                     -8.65192783339199453e-3, -7.1372762502456763e-3, -5.9945117538835746e-3, -5.10997527675418131e-3,
+                     -4.41060829979912465e-3, -3.84763737842981233e-3, -3.38745061493813488e-3,
+                     -3.00624946457977689e-3]
+
+
+def lindhard(eta, lbda=1.0, mu=1.0):
+    """Inverse Lindhard function with the reference's four branches
+    (functional/kedf/kernel.py:13-136), for host-side tables; the evaluation
+    kernels compute it on the device."""
+    eta = np.asarray(eta, dtype=np.float64)
+    out = np.empty_like(eta)
+    small = eta < 1e-10
+    one = np.abs(eta - 1.0) < 1e-10
+    large = eta > 3.65
+    mid = ~(small | one | large)
+    e = eta[mid]
+    out[mid] = 1.0 / (0.5 + 0.25 * (1.0 - e * e) * np.log((1.0 + e) / np.abs(1.0 - e)) / e) - 3.0 * mu * e * e - lbda
+    e = eta[small]
+    out[small] = 1.0 - lbda + e * e * (1.0 / 3.0 - 3.0 * mu)
+    e = eta[one]
+    out[one] = 2.0 - lbda - 3.0 * mu + 20.0 * (e - 1.0)
+    e = eta[large]
+    inv = 1.0 / (e * e)
+    acc = np.full_like(e, _LARGE_ETA_SERIES[-1])
+    for c in _LARGE_ETA_SERIES[-2::-1]:
+        acc = c + inv * acc
+    out[large] = 3.0 * (1.0 - mu) * e * e - lbda - 0.6 + inv * acc
+    return out

@@ -52,8 +52,12 @@ class Optimization:
         # difference (eps = 1e-9) needs FP64; FP32 grids use the exact linear
         # response of the potential
         hvp = o["hvp"]
+        lmgp = bool(ev.terms & _lib.TERM_LMGP)
         if hvp == "auto":
-            hvp = "fd" if grid.dtype == torch.float64 else "analytic"
+            hvp = "fd" if (grid.dtype == torch.float64 or lmgp) else "analytic"
+        if lmgp and hvp == "fd" and grid.dtype != torch.float64:
+            raise NotImplementedError("LMGP in FP32 needs the analytic Hessian product, which does not cover LMGP; "
+                                      "optimise LMGP on an FP64 grid")
         if hvp not in ("fd", "analytic"):
             raise ValueError("hvp must be 'auto', 'fd' or 'analytic'")
         hvp = 0 if hvp == "fd" else 1

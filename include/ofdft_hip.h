@@ -46,7 +46,8 @@ enum {
     OFDFT_TERM_WT = 4,      /* functional/kedf/wt.py:108-132 */
     OFDFT_TERM_XC = 8,      /* functional/xc/semilocal_xc.py:252-290 (PZ LDA) */
     OFDFT_TERM_HARTREE = 16,/* functional/hartree.py:22-45 */
-    OFDFT_TERM_PP = 32      /* functional/pseudo/__init__.py:219-234 */
+    OFDFT_TERM_PP = 32,     /* functional/pseudo/__init__.py:219-234 */
+    OFDFT_TERM_LMGP = 64    /* functional/kedf/lwt.py:501-575 (LMGP-NL), needs ofdft_set_lmgp */
 };
 
 /* energies[] layout written by ofdft_eval (Ha) */
@@ -98,6 +99,22 @@ int ofdft_set_terms(ofdft_plan *plan, unsigned terms);
  * pseudo/__init__.py:236-246): device array of N reals in the plan
  * precision, kept by pointer (caller keeps it alive) or NULL. */
 int ofdft_set_vloc(ofdft_plan *plan, const void *vreal);
+/* LMGP nonlocal kinetic functional (LWT with the MGP kernel, lwt.py:501-578;
+ * OFDFT_TERM_LMGP).  ofdft_mgp_kernel_table computes the MGP kernel table
+ * K(eta_k), eta_k = k etamax / (neta - 1) (kernel.py:266-322, t-integral with
+ * maxpoints terms) on the GPU into out (host); the inverse Lindhard function
+ * inside it is the cubic spline (4 (ng-1) coefficients gcoef) on the
+ * log-spaced grid gp[ng], clamped at its ends, as the reference evaluates it.  ofdft_set_lmgp takes the
+ * piecewise-cubic tables of K and of -eta dK/deta on that grid (4 (ntab-1)
+ * coefficients each, host) and the k_F ladder ratio (1.15 in the reference),
+ * and allocates the plan's LMGP workspace (setup call); ntab <= 0 disables.
+ * The saved k_F bounds (lwt.py:36-82) persist across evaluations until
+ * ofdft_lmgp_reset or the next ofdft_set_lmgp. */
+int ofdft_mgp_kernel_table(int neta, double etamax, int maxpoints, int ng, const double *gp, const double *gcoef,
+                           double *out);
+int ofdft_set_lmgp(ofdft_plan *plan, int ntab, double etamax, const double *kcoef, const double *dcoef,
+                   double ratio);
+int ofdft_lmgp_reset(ofdft_plan *plan);
 /* Ion-ion (Ewald) energy added to OFDFT_E_SUM (total_functional.py:134-137). */
 int ofdft_set_eion(ofdft_plan *plan, double eion);
 

@@ -266,3 +266,18 @@ def test_cnn_oracle_gradient_is_the_derivative():
             q2[k] = p[k] - eps * v
             errs.append(abs((L(q1) - L(q2)) / (2 * eps) - an))
         assert min(errs) <= 2e-8 * max(1.0, abs(an)) + 5e-8, (k, an, errs)
+
+
+def test_lmgp_golden_and_heg(al_pp):
+    """TF + vW + LMGP on Al_fde_rho.pp (examples/test/test_functional.py:29-34,
+    reference tolerance atol 1e-3; the restatement reproduces it to 1e-9), and
+    the homogeneous-gas limit (no nonlocal energy or potential)."""
+    from oracle import lmgp as OL
+    atoms, rho, lat = al_pp
+    g = G.Grid(lat, rho.shape, full=True)
+    tabs = OL.kernel_tables()
+    e, v = OL.lmgp_nl(rho, g, tabs)
+    tot = F.tf(rho, g)[0] + F.vw(rho, g)[0] + e
+    assert abs(tot - 2.9145872498710492) < 1e-9
+    e0, v0 = OL.lmgp_nl(np.full(rho.shape, rho.mean()), g, tabs)
+    assert e0 == 0.0 and not v0.any()
