@@ -36,6 +36,8 @@ SIGNATURES = {
     "ofdft_plan_set_lattice": (_I, [_P, _DP]),
     "ofdft_plan_workspace_bytes": (ctypes.c_size_t, [_P]),
     "ofdft_plan_set_fft_kernels": (_I, [_P, _I]),
+    "ofdft_plan_autotune": (_I, [_P, _IP, ctypes.POINTER(ctypes.c_float), _P]),
+    "ofdft_plan_set_variants": (_I, [_P, _IP]),
     "ofdft_fft_r2c": (_I, [_P, _P, _P, _I, _P]),
     "ofdft_fft_c2r": (_I, [_P, _P, _P, _I, _P]),
     "ofdft_fft_c2c": (_I, [_P, _P, _P, _I, _I, _P]),

@@ -136,6 +136,30 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
     });
 }
 
+int ofdft_plan_autotune(ofdft_plan *plan, int *variants, float *times, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(plan != nullptr, "plan_autotune: null plan");
+        plan_autotune(*plan, S(stream), times);
+        if (variants) {
+            variants[0] = plan->var_r2c;
+            variants[1] = plan->var_c2r;
+            variants[2] = plan->var_cols[0];
+            variants[3] = plan->var_cols[1];
+        }
+    });
+}
+
+int ofdft_plan_set_variants(ofdft_plan *plan, const int *variants) {
+    return guard([&] {
+        OFDFT_REQUIRE(plan != nullptr && variants != nullptr, "plan_set_variants: null pointer");
+        for (int i = 0; i < 4; ++i) OFDFT_REQUIRE(variants[i] >= 0 && variants[i] <= 2, "variant out of range");
+        plan->var_r2c = variants[0];
+        plan->var_c2r = variants[1];
+        plan->var_cols[0] = variants[2];
+        plan->var_cols[1] = variants[3];
+    });
+}
+
 int ofdft_plan_set_lattice(ofdft_plan *plan, const double lattice[9]) {
     return guard([&] {
         OFDFT_REQUIRE(plan != nullptr && lattice != nullptr, "plan_set_lattice: null pointer");

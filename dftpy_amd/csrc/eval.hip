@@ -97,8 +97,8 @@ static int grid_blocks(long long n, int per = 256 * 4) {
 }
 
 template <typename T>
-bool xfused_fixed(int n, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch, long long vp,
-                  const void *tw, const GParams &gp, hipStream_t s);
+bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
+                  long long vp, const void *tw, const GParams &gp, hipStream_t s);
 
 // the x pass can run fused with the G-space multiply (size-specialised x kernel)
 static bool x_fusable(const Plan &P) { return P.use_fixed && P.nr[0] > 1 && P.nr[1] * P.nr[2] > 1 && fixed_size(P.nr[0]); }
@@ -115,7 +115,7 @@ static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long
     if (gp.s_h >= 0) gp.kind[gp.s_h] = 2;
     if (fused) {
         const long long ncols = (long long)P.nr[1] * pitch;
-        OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], spec, ns, ncp, ncols, pitch, P.nzc, P.tw[0], gp, s),
+        OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], P.var_cols[0], spec, ns, ncp, ncols, pitch, P.nzc, P.tw[0], gp, s),
                       "fused x pass unavailable");
     } else {
         hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);

@@ -285,20 +285,20 @@ __global__ void k_rows_c2c(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ o
 // ------------------------------------------------------------------ host side
 bool fixed_size(int n);
 template <typename T>
-bool cols_fixed(int n, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,
-                long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s);
+bool cols_fixed(int n, int var, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols,
+                long long P, long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s);
 template <typename T>
-bool r2c_fixed(int n, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               long long opitch, const void *tw, double scale, const Prologue &pro, double *partials, int sum_slot,
-               hipStream_t s);
+bool r2c_fixed(int n, int var, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
+               long long nrows, long long opitch, const void *tw, double scale, const Prologue &pro, double *partials,
+               int sum_slot, hipStream_t s);
 template <typename T>
-bool c2r_fixed(int n, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               long long ipitch, const void *tw, double scale, hipStream_t s);
-int r2c_fixed_tiles(int n, int prec, int batch, long long nrows);
+bool c2r_fixed(int n, int var, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs,
+               long long nrows, long long ipitch, const void *tw, double scale, hipStream_t s);
+int r2c_fixed_tiles(int n, int var, int prec, int batch, long long nrows);
 
 int r2c_num_partials(const Plan &P, int batch) {
     const long long nrows = (long long)P.nr[0] * P.nr[1];
-    if (P.use_fixed && fixed_size(P.nr[2])) return r2c_fixed_tiles(P.nr[2], P.prec, batch, nrows);
+    if (P.use_fixed && fixed_size(P.nr[2])) return r2c_fixed_tiles(P.nr[2], P.var_r2c, P.prec, batch, nrows);
     return (int)((nrows + P.rows_r2c.tile - 1) / P.rows_r2c.tile);
 }
 
@@ -449,7 +449,7 @@ void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in,
             Pp = Qq = pitch;
             Ss = ncols;
         }
-        cols_fixed<T>(n, inverse, in, out, batch, bs, ncols, Pp, Qq, Ss, zlen, P.tw[axis], scale, s);
+        cols_fixed<T>(n, P.var_cols[axis], inverse, in, out, batch, bs, ncols, Pp, Qq, Ss, zlen, P.tw[axis], scale, s);
         OFDFT_HIP(hipGetLastError());
         return;
     }
@@ -495,7 +495,7 @@ void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long
     const PassShape sh = P.rows_r2c;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
     const bool last_z = (P.nr[0] == 1 && P.nr[1] == 1);
-    if (!(P.use_fixed && r2c_fixed<T>(P.nr[2], in, out, batch, in_bs, out_bs, nrows, pitch, P.tw[2],
+    if (!(P.use_fixed && r2c_fixed<T>(P.nr[2], P.var_r2c, in, out, batch, in_bs, out_bs, nrows, pitch, P.tw[2],
                                       last_z ? scale : 1.0, pr, sum_partials, sum_slot, s))) {
         OFDFT_REQUIRE(pitch == P.nzc, "generic R2C pass needs an unpadded layout");
         dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
@@ -524,7 +524,7 @@ void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long lon
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const PassShape sh = P.rows_c2r;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
-    if (!(P.use_fixed && c2r_fixed<T>(P.nr[2], src, out, batch, src_bs, out_bs, nrows, pitch, P.tw[2], scale, s))) {
+    if (!(P.use_fixed && c2r_fixed<T>(P.nr[2], P.var_c2r, src, out, batch, src_bs, out_bs, nrows, pitch, P.tw[2], scale, s))) {
         OFDFT_REQUIRE(pitch == P.nzc, "generic C2R pass needs an unpadded layout");
         dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
         hipLaunchKernelGGL((k_rows_c2r<T>), grid, dim3(sh.threads), sh.lds, s, src, out, src_bs, out_bs, nrows,
@@ -558,7 +558,7 @@ template <typename T>
 void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue *pro, hipStream_t s) {
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const long long pitch = pipe_pitch(P);
-    if (P.use_fixed && r2c_fixed<T>(P.nr[2], in, out, batch, 0LL, (long long)P.nr[0] * P.nr[1] * pitch, nrows, pitch,
+    if (P.use_fixed && r2c_fixed<T>(P.nr[2], P.var_r2c, in, out, batch, 0LL, (long long)P.nr[0] * P.nr[1] * pitch, nrows, pitch,
                                     P.tw[2], 1.0, *pro, nullptr, -1, s)) {
         OFDFT_HIP(hipGetLastError());
         return;
@@ -574,7 +574,7 @@ void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue
 template <typename T> void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int batch, hipStream_t s) {
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const long long pitch = pipe_pitch(P);
-    if (P.use_fixed && c2r_fixed<T>(P.nr[2], in, out, batch, (long long)P.nr[0] * P.nr[1] * pitch, P.N, nrows, pitch,
+    if (P.use_fixed && c2r_fixed<T>(P.nr[2], P.var_c2r, in, out, batch, (long long)P.nr[0] * P.nr[1] * pitch, P.N, nrows, pitch,
                                     P.tw[2], 1.0, s)) {
         OFDFT_HIP(hipGetLastError());
         return;
@@ -584,6 +584,68 @@ template <typename T> void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int
     hipLaunchKernelGGL((k_rows_c2r<T>), grid, dim3(sh.threads), sh.lds, s, in, out, P.Nc, P.N, nrows, P.nr[2],
                        P.nzc, P.ax[2], reinterpret_cast<const cpx<T> *>(P.tw[2]), sh.tile, (T)1);
     OFDFT_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times every tile variant of the size-specialised passes on the plan's own
+// shapes (3 spectra, the evaluation pipeline's batch) and keeps the fastest.
+// Runs on the plan's workspace; the data are whatever it holds.
+template <typename F> static float time_launches(hipStream_t s, int iters, F &&f) {
+    hipEvent_t e0, e1;
+    OFDFT_HIP(hipEventCreate(&e0));
+    OFDFT_HIP(hipEventCreate(&e1));
+    f();  // warm-up
+    OFDFT_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; ++i) f();
+    OFDFT_HIP(hipEventRecord(e1, s));
+    OFDFT_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    OFDFT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / iters;
+}
+
+template <typename T> static void autotune_t(Plan &P, hipStream_t s, float *times) {
+    const long long pitch = pipe_pitch(P);
+    const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+    cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
+    T *work = reinterpret_cast<T *>(P.work);
+    const int iters = 4;
+    auto pick = [&](int *var, int slot, auto &&run) {
+        float best = 1e30f;
+        int bv = 0;
+        for (int v = 0; v < 3; ++v) {
+            *var = v;
+            const float t = time_launches(s, iters, run);
+            if (times) times[slot * 3 + v] = t;
+            if (t < best) {
+                best = t;
+                bv = v;
+            }
+        }
+        *var = bv;
+    };
+    if (fixed_size(P.nr[2]) && P.nr[0] * P.nr[1] > 1) {
+        Prologue pro{};
+        pro.code[0] = OP_SQRT_RHOM;
+        pro.code[1] = OP_POW_BETA;
+        pro.beta = P.beta;
+        pro.b56 = 1;
+        pick(&P.var_r2c, 0, [&] { bench_rows_r2c<T>(P, work, spec, 3, &pro, s); });
+        pick(&P.var_c2r, 1, [&] { bench_rows_c2r<T>(P, spec, work, 3, s); });
+    }
+    for (int axis = 0; axis < 2; ++axis)
+        if (fixed_size(P.nr[axis]))
+            pick(&P.var_cols[axis], 2 + axis, [&] { fft_cols<T>(P, axis, false, false, spec, spec, 3, ncp, 1.0, s, pitch); });
+}
+
+void plan_autotune(Plan &P, hipStream_t s, float *times) {
+    if (!P.use_fixed) return;
+    if (P.prec == OFDFT_FP64)
+        autotune_t<double>(P, s, times);
+    else
+        autotune_t<float>(P, s, times);
 }
 
 template void bench_rows_r2c<float>(Plan &, const float *, float2 *, int, const Prologue *, hipStream_t);

@@ -1,457 +1,8 @@
-// Size-specialised FFT passes for the common 2/3/5-smooth lengths.
-//
-// The length N, the tile and the radix sequence are compile-time, so all
-// index arithmetic folds to shifts and constant multiplies.  The first radix
-// stage reads its inputs straight from HBM into registers and the last stage
-// (column passes, C2R rows) writes straight back, both coalesced across the
-// `TILE` adjacent pencils of a workgroup; intermediate stages exchange
-// through LDS.  Column passes address a flattened set of pencils
-// c in [0, ncols): base(c) = (c / P) * Q + c % P, element stride S, which
-// covers the y pass (P = nzc, Q = ny*nzc, S = nzc) and the x pass
-// (P = Q = S = ny*nzc) without partial tiles at plane boundaries.
-#include "dft.h"
-#include "gspace.h"
-#include "internal.h"
-#include "pointwise.h"
+// Size table of the size-specialised FFT passes and the tile count of the
+// R2C pass (its number of FP64 partial sums).
+#include "fft_fixed.h"
 
 namespace ofdft {
-
-struct StageList {
-    int nst;
-    int rad[kMaxStages];
-    int rmin;
-};
-
-// radices 8/4/2 for the power of two (8s first, 4 4 instead of 8 2), then 5, 3
-constexpr StageList factorize(int n) {
-    StageList s{};
-    s.nst = 0;
-    s.rmin = 64;
-    int m = n, e2 = 0;
-    while (m % 2 == 0) {
-        m /= 2;
-        ++e2;
-    }
-    int r8 = 0, r4 = 0, r2 = 0;
-    if (e2 % 3 == 0) {
-        r8 = e2 / 3;
-    } else if (e2 % 3 == 1) {
-        if (e2 >= 4) {
-            r8 = e2 / 3 - 1;
-            r4 = 2;
-        } else {
-            r2 = 1;
-        }
-    } else {
-        r8 = e2 / 3;
-        r4 = 1;
-    }
-    for (int i = 0; i < r8; ++i) s.rad[s.nst++] = 8;
-    for (int i = 0; i < r4; ++i) s.rad[s.nst++] = 4;
-    for (int i = 0; i < r2; ++i) s.rad[s.nst++] = 2;
-    while (m % 5 == 0) {
-        m /= 5;
-        s.rad[s.nst++] = 5;
-    }
-    while (m % 3 == 0) {
-        m /= 3;
-        s.rad[s.nst++] = 3;
-    }
-    for (int i = 0; i < s.nst; ++i) s.rmin = s.rad[i] < s.rmin ? s.rad[i] : s.rmin;
-    if (m != 1) s.nst = -1;
-    return s;
-}
-
-template <int N> struct SL {
-    static constexpr StageList s = factorize(N);
-    static constexpr int nst = s.nst;
-    static constexpr int rmin = s.rmin;
-    static constexpr int tpp = N / s.rmin;  // threads per pencil
-    template <int I> static constexpr int rad() { return s.rad[I]; }
-};
-
-template <typename C> struct Real;
-template <> struct Real<float2> { using type = float; };
-template <> struct Real<double2> { using type = double; };
-
-// ---------------------------------------------------------------- column pass
-// One stage for TILE pencils, pencil element e of pencil p at buf[e*TILE + p].
-template <typename C, int N, int TILE, int R, int NS, bool INV, bool FIRST, bool LAST>
-__device__ __forceinline__ void cstage(C *buf, const C *tw, const C *__restrict__ src, C *__restrict__ dst,
-                                       long long S, bool valid, int pen, int j, typename Real<C>::type scale) {
-    using T = typename Real<C>::type;
-    constexpr int NB = N / R;
-    const bool act = j < NB;
-    C v[R];
-    if (act) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (FIRST)
-                v[r] = valid ? src[(long long)(j + r * NB) * S] : C{(T)0, (T)0};
-            else
-                v[r] = buf[(j + r * NB) * TILE + pen];
-        }
-        if (NS > 1) {
-            const int k = j % NS;
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                C w = tw[(r * k * (N / (NS * R))) % N];
-                if (INV) w.y = -w.y;
-                v[r] = cmul(v[r], w);
-            }
-        }
-    }
-    if (!FIRST || LAST) __syncthreads();
-    if (act) {
-        DFT<R, INV, C>::run(v);
-        const int k = j % NS;
-        const int d = (j / NS) * NS * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (LAST) {
-                if (valid) dst[(long long)(d + r * NS) * S] = C{v[r].x * scale, v[r].y * scale};
-            } else {
-                buf[(d + r * NS) * TILE + pen] = v[r];
-            }
-        }
-    }
-    if (!LAST) __syncthreads();
-}
-
-template <typename C, int N, int TILE, bool INV, int I, int NS>
-__device__ __forceinline__ void cstages(C *buf, const C *tw, const C *src, C *dst, long long S, bool valid, int pen,
-                                        int j, typename Real<C>::type scale) {
-    if constexpr (I < SL<N>::nst) {
-        constexpr int R = SL<N>::template rad<I>();
-        cstage<C, N, TILE, R, NS, INV, I == 0, I == SL<N>::nst - 1>(buf, tw, src, dst, S, valid, pen, j, scale);
-        cstages<C, N, TILE, INV, I + 1, NS * R>(buf, tw, src, dst, S, valid, pen, j, scale);
-    }
-}
-
-template <typename T, int N, int TILE, bool INV>
-__global__ void __launch_bounds__(TILE *SL<N>::tpp)
-    k_cols_f(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, long long bs, long long ncols, long long P,
-             long long Q, long long S, long long vp, const cpx<T> *__restrict__ twg, T scale) {
-    using C = cpx<T>;
-    __shared__ C tw[N];
-    __shared__ C buf[N * TILE];
-    const int pen = threadIdx.x % TILE;
-    const int j = threadIdx.x / TILE;
-    for (int i = threadIdx.x; i < N; i += TILE * SL<N>::tpp) tw[i] = twg[i];
-    const long long c = (long long)blockIdx.x * TILE + pen;
-    const long long cp = c % P;
-    const bool valid = c < ncols && cp < vp;  // pad columns of a pitched layout are skipped
-    const long long base = valid ? (c / P) * Q + cp : 0;
-    const long long off = (long long)blockIdx.y * bs + base;
-    cstages<C, N, TILE, INV, 0, 1>(buf, tw, in + off, out + off, S, valid, pen, j, scale);
-}
-
-// Stages with a choice of source and destination: GIN reads the first stage
-// from HBM (else from LDS), GOUT writes the last stage to HBM (else to LDS in
-// natural order).
-template <typename C, int N, int TILE, bool INV, int I, int NS, bool GIN, bool GOUT>
-__device__ __forceinline__ void cstages_io(C *buf, const C *tw, const C *src, C *dst, long long S, bool valid,
-                                           int pen, int j, typename Real<C>::type scale) {
-    if constexpr (I < SL<N>::nst) {
-        constexpr int R = SL<N>::template rad<I>();
-        cstage<C, N, TILE, R, NS, INV, GIN && I == 0, GOUT && I == SL<N>::nst - 1>(buf, tw, src, dst, S, valid, pen,
-                                                                                    j, scale);
-        cstages_io<C, N, TILE, INV, I + 1, NS * R, GIN, GOUT>(buf, tw, src, dst, S, valid, pen, j, scale);
-    }
-}
-
-// x pass of the evaluation pipeline with the G-space multiply fused in: the
-// forward x transform of a column tile ends in LDS, each element is scaled by
-// its kernel value (gfactor: vW, WT or Hartree by spectrum slot), and the
-// inverse x transform starts from LDS and writes back in place.  Saves one
-// HBM round trip of every spectrum on each side of the multiply.
-template <typename T, int N, int TILE>
-__global__ void __launch_bounds__(TILE *SL<N>::tpp)
-    k_xfused_f(cpx<T> *__restrict__ data, long long bs, long long ncols, long long pitch, long long vp,
-               const cpx<T> *__restrict__ twg, GParams gp) {
-    using C = cpx<T>;
-    __shared__ C tw[N];
-    __shared__ C buf[N * TILE];
-    constexpr int NT = TILE * SL<N>::tpp;
-    const int pen = threadIdx.x % TILE;
-    const int j = threadIdx.x / TILE;
-    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
-    const long long c = (long long)blockIdx.x * TILE + pen;
-    const bool valid = c < ncols && c % pitch < vp;
-    C *col = data + (long long)blockIdx.y * bs + (valid ? c : 0);
-    const long long S = ncols;  // x stride of the (nx, ny, pitch) layout
-    double tkf = 1.0, wtfac = 1.0;
-    const int kind = gp.kind[blockIdx.y];
-    if (kind == 1) wt_consts(gp, tkf, wtfac);
-    cstages_io<C, N, TILE, false, 0, 1, true, false>(buf, tw, col, nullptr, S, valid, pen, j, (T)1);
-    for (int i = threadIdx.x; i < N * TILE; i += NT) {
-        const int e = i / TILE, pn = i - e * TILE;
-        const long long cc = (long long)blockIdx.x * TILE + pn;
-        if (cc >= ncols) continue;
-        const int iy = (int)(cc / pitch), iz = (int)(cc % pitch);
-        if (iz >= vp) continue;
-        const double f = gfactor<T>(gp, kind, e, iy, iz, tkf, wtfac);
-        C v = buf[i];
-        v.x = (T)(v.x * f);
-        v.y = (T)(v.y * f);
-        buf[i] = v;
-    }
-    __syncthreads();
-    cstages_io<C, N, TILE, true, 0, 1, false, true>(buf, tw, nullptr, col, S, valid, pen, j, (T)1);
-}
-
-// ---------------------------------------------------------------- rows passes
-// LDS stage for contiguous rows: pencil p element e at buf[p*LD + e].
-template <typename C, int N, int LD, int NPEN, int R, int NS, bool INV>
-__device__ __forceinline__ void rstage(C *buf, const C *tw) {
-    constexpr int NB = N / R;
-    const int p = threadIdx.x / NB;  // pencil
-    const int j = threadIdx.x % NB;
-    const bool act = p < NPEN;
-    C v[R];
-    if (act) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = buf[p * LD + j + r * NB];
-        if (NS > 1) {
-            const int k = j % NS;
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                C w = tw[(r * k * (N / (NS * R))) % N];
-                if (INV) w.y = -w.y;
-                v[r] = cmul(v[r], w);
-            }
-        }
-    }
-    __syncthreads();
-    if (act) {
-        DFT<R, INV, C>::run(v);
-        const int k = j % NS;
-        const int d = (j / NS) * NS * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) buf[p * LD + d + r * NS] = v[r];
-    }
-    __syncthreads();
-}
-
-template <typename C, int N, int LD, int NPEN, bool INV, int I, int NS>
-__device__ __forceinline__ void rstages(C *buf, const C *tw) {
-    if constexpr (I < SL<N>::nst) {
-        constexpr int R = SL<N>::template rad<I>();
-        rstage<C, N, LD, NPEN, R, NS, INV>(buf, tw);
-        rstages<C, N, LD, NPEN, INV, I + 1, NS * R>(buf, tw);
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ T prologue_f(int code, T x, T beta, bool b56, const T *aux, long long idx) {
-    switch (code) {
-        case OP_SQRT_RHOM: return sqrt(x > (T)0 ? x : (T)1e-30);
-        case OP_POW_BETA: return pow_beta<T>(x < (T)1e-30 ? (T)1e-30 : x, beta, b56);
-        case OP_DSQRT: {
-            const T r = aux[idx];
-            return x / ((T)2 * sqrt(r > (T)0 ? r : (T)1e-30));
-        }
-        case OP_DPOW_BETA: {
-            const T r = aux[idx];
-            const T rc = r < (T)1e-30 ? (T)1e-30 : r;
-            return beta * pow_beta_m1<T>(rc, beta, b56) * x;
-        }
-        default: return x;
-    }
-}
-
-// Rows R2C for NF spectra at once: NPEN packed pairs of rows per spectrum,
-// so the NF*NPEN pencils run through the same LDS stages.  When all spectra
-// transform the same input (in_bs == 0, each with its own prologue) the input
-// rows are read once.  Output rows have pitch opitch >= N/2+1.
-template <typename T, int N, int NPEN, int NF>
-__global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
-    k_rows_r2c_f(const T *__restrict__ in, cpx<T> *__restrict__ out, long long in_bs, long long out_bs, long long nrows,
-                 long long opitch, const cpx<T> *__restrict__ twg, T scale, Prologue pro, double *partials,
-                 int sum_slot) {
-    using C = cpx<T>;
-    constexpr int NT = NF * NPEN * SL<N>::tpp;
-    constexpr int ROWS = 2 * NPEN;
-    constexpr int NZC = N / 2 + 1;
-    constexpr int LD = N + 1;
-    __shared__ C tw[N];
-    __shared__ C buf[NF * NPEN * LD];
-    __shared__ double red[16];
-    const int slot0 = blockIdx.x * NF;
-    const long long row0 = (long long)blockIdx.y * ROWS;
-    const int nv = (int)min((long long)ROWS, nrows - row0);
-    const T beta = (T)pro.beta;
-    const T *aux = reinterpret_cast<const T *>(pro.aux);
-    const T *phi = reinterpret_cast<const T *>(pro.phi);
-    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
-    double acc = 0.0;
-    for (int e = threadIdx.x; e < ROWS * N; e += NT) {
-        const int r = e / N;
-        const int pos = e - r * N;
-        T x0 = (T)0;
-        if (r < nv) {
-            x0 = in[(long long)slot0 * in_bs + row0 * N + e];
-            if (phi) x0 = (T)(2.0 * (double)phi[row0 * N + e] * (double)x0);
-        }
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            T x = x0;
-            if (f > 0 && in_bs != 0 && r < nv) x = in[(long long)(slot0 + f) * in_bs + row0 * N + e];
-            if (partials != nullptr && slot0 + f == sum_slot) acc += (double)x;
-            if (r < nv) x = prologue_f<T>(pro.code[(slot0 + f) & 3], x, beta, pro.b56 != 0, aux, row0 * N + e);
-            reinterpret_cast<T *>(&buf[(f * NPEN + (r >> 1)) * LD + pos])[r & 1] = x;
-        }
-    }
-    const bool do_sum = partials != nullptr && sum_slot >= slot0 && sum_slot < slot0 + NF;
-    if (do_sum) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    }
-    __syncthreads();
-    if (do_sum && threadIdx.x == 0) {
-        double sm = 0.0;
-        for (int w = 0; w < (NT + 63) / 64; ++w) sm += red[w];
-        partials[blockIdx.y] = sm;
-    }
-    rstages<C, N, LD, NF * NPEN, false, 0, 1>(buf, tw);
-    for (int e = threadIdx.x; e < NF * ROWS * NZC; e += NT) {
-        const int fr = e / NZC, k = e - fr * NZC;
-        const int f = fr / ROWS, r = fr - f * ROWS;
-        if (r < nv) {
-            const int p = f * NPEN + (r >> 1);
-            const C z = buf[p * LD + k];
-            C zc = buf[p * LD + (k == 0 ? 0 : N - k)];
-            zc.y = -zc.y;
-            C o;
-            if ((r & 1) == 0)
-                o = C{(z.x + zc.x) * (T)0.5, (z.y + zc.y) * (T)0.5};
-            else
-                o = C{(z.y - zc.y) * (T)0.5, -(z.x - zc.x) * (T)0.5};
-            out[(long long)(slot0 + f) * out_bs + (row0 + r) * opitch + k] = C{o.x * scale, o.y * scale};
-        }
-    }
-}
-
-// Rows C2R (irfft semantics: imaginary parts of k = 0 and Nyquist ignored).
-template <typename T, int N, int NPEN>
-__global__ void __launch_bounds__(NPEN *SL<N>::tpp)
-    k_rows_c2r_f(const cpx<T> *__restrict__ in, T *__restrict__ out, long long in_bs, long long out_bs,
-                 long long nrows, long long ipitch, const cpx<T> *__restrict__ twg, T scale) {
-    using C = cpx<T>;
-    constexpr int NT = NPEN * SL<N>::tpp;
-    constexpr int ROWS = 2 * NPEN;
-    constexpr int NZC = N / 2 + 1;
-    constexpr int LD = N + 1;
-    __shared__ C tw[N];
-    __shared__ C stg[ROWS * NZC];
-    __shared__ C buf[NPEN * LD];
-    const long long row0 = (long long)blockIdx.y * ROWS;
-    const C *src = in + (long long)blockIdx.x * in_bs + row0 * ipitch;
-    T *dst = out + (long long)blockIdx.x * out_bs + row0 * N;
-    const int nv = (int)min((long long)ROWS, nrows - row0);
-    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
-    for (int e = threadIdx.x; e < ROWS * NZC; e += NT) {
-        const int r = e / NZC, k = e - r * NZC;
-        C v = (r < nv) ? src[r * ipitch + k] : C{(T)0, (T)0};
-        if (k == 0 || 2 * k == N) v.y = (T)0;
-        stg[e] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NPEN * N; e += NT) {
-        const int p = e / N, k = e - p * N;
-        C A, B;
-        if (k < NZC) {
-            A = stg[(2 * p) * NZC + k];
-            B = stg[(2 * p + 1) * NZC + k];
-        } else {
-            A = stg[(2 * p) * NZC + N - k];
-            B = stg[(2 * p + 1) * NZC + N - k];
-            A.y = -A.y;
-            B.y = -B.y;
-        }
-        buf[p * LD + k] = C{A.x - B.y, A.y + B.x};
-    }
-    __syncthreads();
-    rstages<C, N, LD, NPEN, true, 0, 1>(buf, tw);
-    for (int e = threadIdx.x; e < ROWS * N; e += NT) {
-        const int r = e / N;
-        if (r < nv) {
-            const C z = buf[(r >> 1) * LD + (e - r * N)];
-            dst[e] = ((r & 1) ? z.y : z.x) * scale;
-        }
-    }
-}
-
-// ---------------------------------------------------------------- dispatch
-// tile of the column pass: as many pencils as fit 1024 threads, power of 2,
-// at most 32 (FP32) / 16 (FP64), LDS <= 64 KB
-template <typename T, int N> constexpr int col_tile() {
-    int t = sizeof(T) == 4 ? 32 : 16;
-    while (t > 1 && (t * SL<N>::tpp > 1024 || (t + 1) * N * 2 * (int)sizeof(T) > 64 * 1024)) t /= 2;
-    return t;
-}
-template <typename T, int N> constexpr int row_pens() {
-    int p = 8;
-    while (p > 1 && (p * SL<N>::tpp > 1024 || (2 * p * (N / 2 + 1) + p * (N + 1) + N) * 2 * (int)sizeof(T) > 64 * 1024))
-        --p;
-    return p;
-}
-template <typename T, int N, int NF> constexpr int r2c_pens() {
-    int p = 8;
-    while (p > 1 && (NF * p * SL<N>::tpp > 1024 || (NF * p * (N + 1) + N) * 2 * (int)sizeof(T) + 128 > 64 * 1024))
-        --p;
-    return p;
-}
-
-template <typename T, int N>
-static void launch_cols(bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,
-                        long long Q, long long S, long long vp, const cpx<T> *tw, T scale, hipStream_t s) {
-    constexpr int TILE = col_tile<T, N>();
-    constexpr int NT = TILE * SL<N>::tpp;
-    dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);
-    if (inv)
-        hipLaunchKernelGGL((k_cols_f<T, N, TILE, true>), grid, dim3(NT), 0, s, in, out, bs, ncols, P, Q, S, vp, tw,
-                           scale);
-    else
-        hipLaunchKernelGGL((k_cols_f<T, N, TILE, false>), grid, dim3(NT), 0, s, in, out, bs, ncols, P, Q, S, vp, tw,
-                           scale);
-}
-
-template <typename T, int N, int NF>
-static void launch_r2c_nf(const T *in, cpx<T> *out, int nblk, long long in_bs, long long out_bs, long long nrows,
-                          long long opitch, const cpx<T> *tw, T scale, const Prologue &pro, double *partials,
-                          int sum_slot, hipStream_t s) {
-    constexpr int NP = r2c_pens<T, N, NF>();
-    const long long tiles = (nrows + 2 * NP - 1) / (2 * NP);
-    OFDFT_REQUIRE(tiles <= 65535, "too many rows for the R2C pass");
-    dim3 grid((unsigned)nblk, (unsigned)tiles);
-    hipLaunchKernelGGL((k_rows_r2c_f<T, N, NP, NF>), grid, dim3(NF * NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs,
-                       nrows, opitch, tw, scale, pro, partials, sum_slot);
-}
-
-template <typename T, int N>
-static void launch_r2c(const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, long long nrows,
-                       long long opitch, const cpx<T> *tw, T scale, const Prologue &pro, double *partials,
-                       int sum_slot, hipStream_t s) {
-    if (batch == 3)
-        launch_r2c_nf<T, N, 3>(in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
-    else if (batch == 2)
-        launch_r2c_nf<T, N, 2>(in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
-    else
-        launch_r2c_nf<T, N, 1>(in, out, batch, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
-}
-
-template <typename T, int N>
-static void launch_c2r(const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs, long long nrows,
-                       long long ipitch, const cpx<T> *tw, T scale, hipStream_t s) {
-    constexpr int NP = row_pens<T, N>();
-    dim3 grid((unsigned)batch, (unsigned)((nrows + 2 * NP - 1) / (2 * NP)));
-    hipLaunchKernelGGL((k_rows_c2r_f<T, N, NP>), grid, dim3(NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs, nrows,
-                       ipitch, tw, scale);
-}
-
-#define OFDFT_FIXED_SIZES(X) X(32) X(48) X(64) X(72) X(80) X(96) X(100) X(120) X(128) X(144) X(160) X(192) X(200) X(240) X(256)
 
 bool fixed_size(int n) {
     switch (n) {
@@ -463,109 +14,27 @@ bool fixed_size(int n) {
     }
 }
 
-template <typename T, int N>
-static void launch_xfused(cpx<T> *data, int batch, long long bs, long long ncols, long long pitch, long long vp,
-                          const cpx<T> *tw, const GParams &gp, hipStream_t s) {
-    constexpr int TILE = col_tile<T, N>();
-    dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);
-    hipLaunchKernelGGL((k_xfused_f<T, N, TILE>), grid, dim3(TILE * SL<N>::tpp), 0, s, data, bs, ncols, pitch, vp, tw,
-                       gp);
-}
-
-template <typename T>
-bool xfused_fixed(int n, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch, long long vp,
-                  const void *tw, const GParams &gp, hipStream_t s) {
-    switch (n) {
-#define CASE(N)                                                                                                   \
-    case N:                                                                                                       \
-        launch_xfused<T, N>(data, batch, bs, ncols, pitch, vp, reinterpret_cast<const cpx<T> *>(tw), gp, s);      \
-        return true;
-        OFDFT_FIXED_SIZES(CASE)
-#undef CASE
-        default: return false;
-    }
-}
-
-template bool xfused_fixed<float>(int, cpx<float> *, int, long long, long long, long long, long long, const void *,
-                                  const GParams &, hipStream_t);
-template bool xfused_fixed<double>(int, cpx<double> *, int, long long, long long, long long, long long, const void *,
-                                   const GParams &, hipStream_t);
-
-template <typename T>
-bool cols_fixed(int n, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols, long long P,
-                long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s) {
-    switch (n) {
-#define CASE(N)                                                                                                     \
-    case N:                                                                                                         \
-        launch_cols<T, N>(inv, in, out, batch, bs, ncols, P, Q, S, vp, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
-                          s);                                                                                       \
-        return true;
-        OFDFT_FIXED_SIZES(CASE)
-#undef CASE
-        default: return false;
-    }
-}
-
-template <typename T>
-bool r2c_fixed(int n, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               long long opitch, const void *tw, double scale, const Prologue &pro, double *partials, int sum_slot,
-               hipStream_t s) {
-    switch (n) {
-#define CASE(N)                                                                                                      \
-    case N:                                                                                                          \
-        launch_r2c<T, N>(in, out, batch, in_bs, out_bs, nrows, opitch, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
-                         pro, partials, sum_slot, s);                                                                \
-        return true;
-        OFDFT_FIXED_SIZES(CASE)
-#undef CASE
-        default: return false;
-    }
-}
-
-template <typename T>
-bool c2r_fixed(int n, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs, long long nrows,
-               long long ipitch, const void *tw, double scale, hipStream_t s) {
-    switch (n) {
-#define CASE(N)                                                                                                       \
-    case N:                                                                                                           \
-        launch_c2r<T, N>(in, out, batch, in_bs, out_bs, nrows, ipitch, reinterpret_cast<const cpx<T> *>(tw), (T)scale, \
-                         s);                                                                                          \
-        return true;
-        OFDFT_FIXED_SIZES(CASE)
-#undef CASE
-        default: return false;
-    }
-}
-
-// number of row tiles of the fixed R2C kernel for `batch` spectra (its partial-sum count)
-int r2c_fixed_tiles(int n, int prec, int batch, long long nrows) {
+// number of row tiles of the fixed R2C kernel for `batch` spectra and variant var
+int r2c_fixed_tiles(int n, int var, int prec, int batch, long long nrows) {
     int np = 1;
     switch (n) {
-#define CASE(N)                                                                                      \
-    case N:                                                                                          \
-        if (batch == 3)                                                                              \
-            np = prec == OFDFT_FP64 ? r2c_pens<double, N, 3>() : r2c_pens<float, N, 3>();            \
-        else if (batch == 2)                                                                         \
-            np = prec == OFDFT_FP64 ? r2c_pens<double, N, 2>() : r2c_pens<float, N, 2>();            \
-        else                                                                                         \
-            np = prec == OFDFT_FP64 ? r2c_pens<double, N, 1>() : r2c_pens<float, N, 1>();            \
+#define PENS(T, N, NF) \
+    (var == 1 ? r2c_pens_v<T, N, NF, 1>() : (var == 2 ? r2c_pens_v<T, N, NF, 2>() : r2c_pens_v<T, N, NF, 0>()))
+#define CASE(N)                                                                      \
+    case N:                                                                          \
+        if (batch == 3)                                                              \
+            np = prec == OFDFT_FP64 ? PENS(double, N, 3) : PENS(float, N, 3);        \
+        else if (batch == 2)                                                         \
+            np = prec == OFDFT_FP64 ? PENS(double, N, 2) : PENS(float, N, 2);        \
+        else                                                                         \
+            np = prec == OFDFT_FP64 ? PENS(double, N, 1) : PENS(float, N, 1);        \
         break;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
+#undef PENS
         default: return -1;
     }
     return (int)((nrows + 2 * np - 1) / (2 * np));
 }
-
-#define INST(T)                                                                                                      \
-    template bool cols_fixed<T>(int, bool, const cpx<T> *, cpx<T> *, int, long long, long long, long long, long long, \
-                                long long, long long, const void *, double, hipStream_t);                            \
-    template bool r2c_fixed<T>(int, const T *, cpx<T> *, int, long long, long long, long long, long long, const void *, \
-                               double, const Prologue &, double *, int, hipStream_t);                                \
-    template bool c2r_fixed<T>(int, const cpx<T> *, T *, int, long long, long long, long long, long long, const void *, \
-                               double, hipStream_t);
-INST(float)
-INST(double)
-#undef INST
 
 }  // namespace ofdft

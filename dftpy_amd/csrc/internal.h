@@ -102,6 +102,8 @@ struct Plan {
     LmgpState *lmgp = nullptr;
     bool rho0_cached_valid = false;
     bool use_fixed = true;  // size-specialised FFT kernels where available
+    // tile variants of the size-specialised passes (0 = default tile; set by the autotuner)
+    int var_r2c = 0, var_c2r = 0, var_cols[3] = {0, 0, 0};
 };
 
 int r2c_num_partials(const Plan &P, int batch);
@@ -138,6 +140,10 @@ template <typename T>
 void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue *pro, hipStream_t s);
 template <typename T>
 void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int batch, hipStream_t s);
+
+// autotuner of the tile variants (fft.hip); times[12]: per pass (r2c, c2r,
+// x cols, y cols) the mean ms of variants 0..2 (optional)
+void plan_autotune(Plan &P, hipStream_t s, float *times);
 
 // reductions (vec.hip)
 int reduce_blocks(long long n);

@@ -17,7 +17,7 @@ from .math_utils import ecut2nr, spacing2ecut
 class Plan:
     """Owning handle of one ofdft_plan."""
 
-    def __init__(self, nr, lattice, precision, full, max_batch=4):
+    def __init__(self, nr, lattice, precision, full, max_batch=4, autotune=True):
         _lib.require_gpu()
         h = ctypes.c_void_p()
         nr_c = np.ascontiguousarray(nr, dtype=np.int32)
@@ -27,6 +27,24 @@ class Plan:
         self.handle = h
         self.precision = precision
         self.full = full
+        self.variants = None
+        if autotune:
+            self.autotune()
+
+    def autotune(self):
+        """Pick the fastest tile variant of each FFT pass (ofdft_plan_autotune)."""
+        v = np.zeros(4, dtype=np.int32)
+        t = np.zeros(12, dtype=np.float32)
+        _lib.check(_lib.lib.ofdft_plan_autotune(self.handle, _lib.iptr(v), t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                _lib.stream()))
+        self.variants = v.tolist()
+        self.variant_ms = t.reshape(4, 3)
+        return self.variants
+
+    def set_variants(self, variants):
+        v = np.ascontiguousarray(variants, dtype=np.int32)
+        _lib.check(_lib.lib.ofdft_plan_set_variants(self.handle, _lib.iptr(v)))
+        self.variants = v.tolist()
 
     @property
     def workspace_bytes(self):

@@ -74,6 +74,14 @@ int ofdft_plan_set_lattice(ofdft_plan *plan, const double lattice[9]);
  * default) uses the size-specialised kernels for the lengths that have them,
  * 0 forces the generic mixed-radix kernels everywhere. */
 int ofdft_plan_set_fft_kernels(ofdft_plan *plan, int fixed);
+/* FFT autotuner (no counterpart in the reference): times the tile variants
+ * (0..2: default tile, halved, quartered) of each size-specialised pass on the
+ * plan's shapes and keeps the fastest.  variants (optional, 4 ints): chosen
+ * variant of the z R2C, z C2R, x and y passes; times (optional, 12 floats):
+ * mean ms of the three variants of each.  Synchronises stream. */
+int ofdft_plan_autotune(ofdft_plan *plan, int *variants, float *times, void *stream);
+/* Set the tile variants explicitly (same order as above). */
+int ofdft_plan_set_variants(ofdft_plan *plan, const int *variants);
 /* Bytes of device workspace the plan holds. */
 size_t ofdft_plan_workspace_bytes(const ofdft_plan *plan);
 

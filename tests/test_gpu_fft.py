@@ -109,3 +109,27 @@ def test_fixed_and_generic_kernels_agree(shape, dtype):
     assert np.max(np.abs(F1.cpu().numpy() - ref)) <= tol
     assert np.max(np.abs(F0.cpu().numpy() - ref)) <= tol
     assert np.max(np.abs(b1.cpu().numpy() - f)) <= _tol(dtype) * 8 * np.max(np.abs(f))
+
+
+@pytest.mark.parametrize("shape", [(160, 160, 160), (96, 72, 100), (64, 48, 32)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_tile_variants_agree(shape, dtype):
+    """Every tile variant of the size-specialised passes (the autotuner's
+    search space) gives the same transforms and the same WT evaluation."""
+    from dftpy_amd import DirectField
+    from dftpy_amd.functional import Functional
+    g = _grid(shape, dtype)
+    rng = np.random.default_rng(9)
+    f = 0.02 + 0.01 * rng.random(shape)
+    plan = g.plan(dtype)
+    print("autotuned", plan.variants, plan.variant_ms.tolist())
+    outs = []
+    for v in ([0, 0, 0, 0], [1, 1, 1, 1], [2, 2, 2, 2], plan.variants):
+        plan.set_variants(v)
+        F = DirectField(g, data=f).fft().data.cpu().numpy()
+        o = Functional(type="KEDF", name="WT")(DirectField(g, data=f))
+        outs.append((F, o.energy, o.potential.numpy()))
+    for F, e, p in outs[1:]:
+        assert np.array_equal(F, outs[0][0])
+        assert abs(e - outs[0][1]) <= 1e-12 * abs(outs[0][1])
+        assert np.max(np.abs(p - outs[0][2])) <= 1e-12 * np.max(np.abs(outs[0][2]))
