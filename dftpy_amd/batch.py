@@ -1,0 +1,160 @@
+"""Many cells of uneven grid sizes optimised concurrently on one GPU.
+
+    python -m dftpy_amd.batch [--ncell 48] [--workers 8] [--seed 0] [--sequential]
+
+Cells: fcc Al n x n x m supercells with n, m drawn from {2, 3, 4} (32-256 atoms),
+isotropic strain U(-3%, +3%) and Gaussian displacements of sigma = 0.1 bohr,
+grid spacing ~0.24 bohr rounded up to even 2/3/5-smooth sizes (64-144 per
+axis); TF + vW + WT + PZ-LDA + Hartree + the empty-core local PP.  Each cell
+is optimised by the FP32-bulk TN from the learned start with the reference
+stop rule (|dE| < 1e-6 Ha/atom * nat over 2 steps).
+
+Execution: W worker threads, each with its own HIP stream, its own plans (one
+per grid shape, autotuned at creation and moved to each cell's strained
+lattice) and its own network buffers, take cells from a shared queue.  The
+native optimizer releases the Python interpreter lock while it runs, so the
+cells' kernels from the W streams run concurrently on the GPU.  (The reference
+optimises one cell per process; there is no batch driver in it.)
+"""
+import argparse
+import json
+import math
+import queue
+import sys
+import threading
+import time
+
+import numpy as np
+import torch
+
+from . import DirectGrid, systems
+from .functional import Functional, TotalFunctional
+from .functional.pseudo import LocalPseudo, ReadPseudo
+from .learned import WarmStartNet, superposition_density
+from .math_utils import best_fft_size
+from .optimization import Optimization
+
+SPACING = 0.24
+
+
+def make_cells(ncell=48, seed=0):
+    """[(ions, nr)] of the batch (seeded)."""
+    rng = np.random.default_rng(seed)
+    cells = []
+    for k in range(ncell):
+        n, m = int(rng.choice([2, 3, 4])), int(rng.choice([2, 3, 4]))
+        strain = float(rng.uniform(-0.03, 0.03))
+        ions = systems.fcc_al((n, n, m), sigma=0.1, strain=strain, seed=seed * 1000 + k)
+        lengths = np.diag(ions.cell)
+        nr = tuple(int(best_fft_size(int(math.ceil(L / SPACING)), max_prime=5)) for L in lengths)
+        cells.append((ions, nr))
+    return cells
+
+
+class _Worker:
+    def __init__(self, readpp, weights=None):
+        self.stream = torch.cuda.Stream()
+        self.grids = {}
+        self.readpp = readpp
+        self.net = WarmStartNet.load() if weights is None else WarmStartNet.load(weights)
+
+    def grid_pair(self, ions, nr):
+        if nr not in self.grids:
+            self.grids[nr] = (DirectGrid(ions.cell, nr, full=False, dtype=torch.float64),
+                              DirectGrid(ions.cell, nr, full=False, dtype=torch.float32))
+        g64, g32 = self.grids[nr]
+        g64.set_lattice(ions.cell)
+        g32.set_lattice(ions.cell)
+        return g64, g32
+
+    def run(self, ions, nr):
+        with torch.cuda.stream(self.stream):
+            t0 = time.perf_counter()
+            g64, g32 = self.grid_pair(ions, nr)
+            pp = LocalPseudo(grid=g64, ions=ions, readpp=self.readpp)
+            ev = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                                 H=Functional(type="HARTREE"), PSEUDO=pp)
+            ne = float(ions.get_ncharges())
+            rho0 = ne / g64.volume
+            rho_s = superposition_density(g64, ions, self.readpp, rho0)
+            x0 = self.net.features(g64.plan(torch.float64), rho_s, pp.vreal_for(g64).data, rho0)
+            start = self.net.start_density(g32, x0, rho0, ne)
+            opt = Optimization("TN", optimization_options={"econv": 1e-6 * ions.nat, "maxiter": 60},
+                               EnergyEvaluator=ev)
+            opt.optimize_rho(start)
+            self.stream.synchronize()
+            return dict(nat=ions.nat, nr=list(nr), energy=float(opt.stats["energy"]), niter=opt.stats["niter"],
+                        status=opt.stats["converged"], evals=opt.stats["n_eval_v"] + opt.stats["n_eval_ev"],
+                        seconds=time.perf_counter() - t0)
+
+
+def run_batch(cells, workers=8, weights=None):
+    """Optimise all cells with `workers` concurrent streams; returns the
+    per-cell records (in input order) and the wall time."""
+    readpp = ReadPseudo({"Al": systems.empty_core_recpot()})
+    ws = [_Worker(readpp, weights) for _ in range(workers)]
+    # warm the plans of every shape (autotuning, tables) outside the timed region
+    shapes = {}
+    for ions, nr in cells:
+        shapes.setdefault(nr, ions)
+    for w in ws:
+        for nr, ions in shapes.items():
+            with torch.cuda.stream(w.stream):
+                w.grid_pair(ions, nr)[0].plan(torch.float64)
+                w.grid_pair(ions, nr)[1].plan(torch.float32)
+    torch.cuda.synchronize()
+    q = queue.Queue()
+    for i, c in enumerate(cells):
+        q.put((i, c))
+    out = [None] * len(cells)
+    errors = []
+
+    def loop(w):
+        while True:
+            try:
+                i, (ions, nr) = q.get_nowait()
+            except queue.Empty:
+                return
+            try:
+                out[i] = w.run(ions, nr)
+            except Exception as e:  # reported after the join
+                errors.append((i, repr(e)))
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=loop, args=(w,)) for w in ws]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if errors:
+        raise RuntimeError("batch cells failed: %s" % errors[:3])
+    return out, wall
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ncell", type=int, default=48)
+    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--sequential", action="store_true", help="also time the cells one after another")
+    args = ap.parse_args(argv)
+    cells = make_cells(args.ncell, args.seed)
+    res, wall = run_batch(cells, args.workers)
+    line = dict(workload="%d fcc Al cells (n,n,m in {2,3,4}, 32-256 atoms, strain +-3%%, sigma 0.1), grids %s" %
+                (len(cells), sorted({tuple(r["nr"]) for r in res})),
+                workers=args.workers, wall_s=wall, cells_per_s=len(cells) / wall,
+                atoms=int(sum(r["nat"] for r in res)), mean_iters=float(np.mean([r["niter"] for r in res])),
+                mean_evals=float(np.mean([r["evals"] for r in res])), all_converged=all(r["status"] == 0 for r in res))
+    if args.sequential:
+        res1, wall1 = run_batch(cells, 1)
+        line.update(sequential_wall_s=wall1, concurrency_speedup=wall1 / wall,
+                    max_energy_diff_vs_sequential=float(max(abs(a["energy"] - b["energy"]) for a, b in zip(res, res1))))
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
