@@ -82,6 +82,12 @@ template <typename T> __global__ void k_mu_sums(const T *pot, const T *rho, long
     write_partials<2>(acc, partials);
 }
 
+// scal[dst] = scal[a] / scal[b] (a device-side ratio, so the next kernel can
+// read it without a host round trip)
+__global__ void k_ratio(double *scal, int dst, int a, int b) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) scal[dst] = scal[a] / scal[b];
+}
+
 // res = (V - mu) phi ; sum res^2
 template <typename T>
 __global__ void k_residual(const T *pot, const T *phi, T *res, const double *mu, long long n, double *partials) {
@@ -304,17 +310,17 @@ template <typename T> struct Opt {
         eval<T>(P, rho, OFDFT_CALC_V, pot, nullptr, C.s);
         ++n_v;
     }
-    // mu = sum(V rho) / sum(rho), stored on device at scal[2]
-    double chemical_potential(const T *pot, const T *rho) {
+    // mu = sum(V rho) / sum(rho) (optimization.py:537-539) kept on device at
+    // scal[2], then res = (V - mu) phi; one host round trip for both
+    double mu_and_residual(const T *pot, const T *rho, const T *phi, T *res, double *rnorm) {
         hipLaunchKernelGGL((k_mu_sums<T>), C.grid(), dim3(256), 0, C.s, pot, rho, n, P.red);
-        const double *h = C.fetch(2);
-        const double mu = h[0] / h[1];
-        C.set_scalar(2, mu);
-        return mu;
-    }
-    double residual(const T *pot, const T *phi, T *res) {
+        C.reduce_final_cols(2, 18);
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 2, 18, 19);
         hipLaunchKernelGGL((k_residual<T>), C.grid(), dim3(256), 0, C.s, pot, phi, res, P.scal + 2, n, P.red);
-        return C.fetch(1)[0] * P.dV;
+        C.reduce_final_cols(1, 20);
+        const double *h = C.to_host(3, 18);
+        if (rnorm) *rnorm = h[2] * P.dV;
+        return h[0] / h[1];
     }
 
     // get_direction_TN (optimization.py:91-147); returns number of V evaluations.
@@ -398,14 +404,19 @@ template <typename T> struct Opt {
         return theta0;
     }
 
-    // value and derivative at theta (ValueAndDerivative, optimization.py:227-279)
+    // value and derivative at theta (ValueAndDerivative, optimization.py:227-279);
+    // the energies and the gradient sum come back in one copy
     void value_deriv(double th, double &e, double &g) {
         const double c = std::cos(th), s = std::sin(th);
         hipLaunchKernelGGL((k_rotate<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_P), c, s, F(F_NPHI), F(F_NRHO), n);
-        e = eval_ev(F(F_NRHO), F(F_NPOT));
+        eval<T>(P, F(F_NRHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_NPOT), energies_d, C.s);
+        ++n_ev;
         hipLaunchKernelGGL((k_theta_grad<T>), C.grid(), dim3(256), 0, C.s, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s,
                            n, P.red);
-        g = 2.0 * C.fetch(1)[0] * P.dV;
+        C.reduce_final_cols(1, 32 + OFDFT_E_COUNT);
+        const double *h = C.to_host(OFDFT_E_COUNT + 1, 32);
+        e = h[OFDFT_E_SUM];
+        g = 2.0 * h[OFDFT_E_COUNT] * P.dV;
     }
 
     // b takes a's contents (a becomes scratch)
@@ -424,8 +435,7 @@ template <typename T> struct Opt {
         const double factor = std::sqrt(h[0] / h[1]);
         hipLaunchKernelGGL((k_scale_square<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_RHO), factor, n);
         double energy = eval_ev(F(F_RHO), F(F_POT));
-        double mu = chemical_potential(F(F_POT), F(F_RHO));
-        residual(F(F_POT), F(F_PHI), F(F_RES));
+        double mu = mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES), nullptr);
         std::vector<double> hist{energy};
         std::vector<double> times{std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()};
         double theta = 0.1;
@@ -497,8 +507,7 @@ template <typename T> struct Opt {
                 swap_fields(F_P, F_DIR_PREV);
                 ++nres;
             }
-            mu = chemical_potential(F(F_POT), F(F_RHO));
-            residual(F(F_POT), F(F_PHI), F(F_RES));
+            mu = mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES), nullptr);
             hist.push_back(energy);
             times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
             (void)nd;
