@@ -486,6 +486,9 @@ void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long
     std::memset(&pr, 0, sizeof(pr));
     int sum_slot = -1;
     if (pro) pr = *pro;
+    for (int b = 0; b < 4; ++b)
+        OFDFT_REQUIRE(pr.aux != nullptr || (pr.code[b] != OP_DSQRT && pr.code[b] != OP_DPOW_BETA),
+                      "R2C prologue: a linear-response op needs rho");
     if (sum_partials) {
         sum_slot = 0;
         for (int b = 0; b < batch; ++b)

@@ -12,6 +12,9 @@
 // covers the y pass (P = nzc, Q = ny*nzc, S = nzc) and the x pass
 // (P = Q = S = ny*nzc) without partial tiles at plane boundaries.
 #pragma once
+#include <algorithm>
+#include <type_traits>
+
 #include "dft.h"
 #include "gspace.h"
 #include "internal.h"
@@ -206,10 +209,10 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
 // ---------------------------------------------------------------- rows passes
 // LDS stage for contiguous rows: pencil p element e at buf[p*LD + e].
 template <typename C, int N, int LD, int NPEN, int R, int NS, bool INV>
-__device__ __forceinline__ void rstage(C *buf, const C *tw) {
+__device__ __forceinline__ void rstage(C *buf, const C *tw, int tid) {
     constexpr int NB = N / R;
-    const int p = threadIdx.x / NB;  // pencil
-    const int j = threadIdx.x % NB;
+    const int p = tid / NB;  // pencil
+    const int j = tid % NB;
     const bool act = p < NPEN;
     C v[R];
     if (act) {
@@ -237,28 +240,44 @@ __device__ __forceinline__ void rstage(C *buf, const C *tw) {
 }
 
 template <typename C, int N, int LD, int NPEN, bool INV, int I, int NS>
-__device__ __forceinline__ void rstages(C *buf, const C *tw) {
+__device__ __forceinline__ void rstages(C *buf, const C *tw, int tid) {
     if constexpr (I < SL<N>::nst) {
         constexpr int R = SL<N>::template rad<I>();
-        rstage<C, N, LD, NPEN, R, NS, INV>(buf, tw);
-        rstages<C, N, LD, NPEN, INV, I + 1, NS * R>(buf, tw);
+        rstage<C, N, LD, NPEN, R, NS, INV>(buf, tw, tid);
+        rstages<C, N, LD, NPEN, INV, I + 1, NS * R>(buf, tw, tid);
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T prologue_f(int code, T x, T beta, bool b56, const T *aux, long long idx) {
+// Prologue of one spectrum of the R2C pass.  AUX and B56 are fixed per
+// launch (block-uniform branches select the instance), so the compiler never
+// evaluates the linear-response cases or the general power speculatively for
+// the element-wise ops that do not need them.  Without AUX the
+// linear-response codes (which need rho) cannot occur: the host rejects them.
+template <typename T, bool AUX, bool B56> __device__ __forceinline__ T prologue_t(int code, T x, T beta, T r) {
     switch (code) {
         case OP_SQRT_RHOM: return sqrt(x > (T)0 ? x : (T)1e-30);
-        case OP_POW_BETA: return pow_beta<T>(x < (T)1e-30 ? (T)1e-30 : x, beta, b56);
-        case OP_DSQRT: {
-            const T r = aux[idx];
-            return x / ((T)2 * sqrt(r > (T)0 ? r : (T)1e-30));
+        case OP_POW_BETA: {
+            const T xc = x < (T)1e-30 ? (T)1e-30 : x;
+            if constexpr (B56)
+                return sqrt(xc) * cbrt(xc);
+            else
+                return pow(xc, beta);
         }
-        case OP_DPOW_BETA: {
-            const T r = aux[idx];
-            const T rc = r < (T)1e-30 ? (T)1e-30 : r;
-            return beta * pow_beta_m1<T>(rc, beta, b56) * x;
-        }
+        case OP_DSQRT:
+            if constexpr (AUX)
+                return x / ((T)2 * sqrt(r > (T)0 ? r : (T)1e-30));
+            else
+                return x;
+        case OP_DPOW_BETA:
+            if constexpr (AUX) {
+                const T rc = r < (T)1e-30 ? (T)1e-30 : r;
+                if constexpr (B56)
+                    return beta * (cbrt(rc) / sqrt(rc)) * x;
+                else
+                    return beta * pow(rc, beta - (T)1) * x;
+            } else {
+                return x;
+            }
         default: return x;
     }
 }
@@ -267,6 +286,13 @@ __device__ __forceinline__ T prologue_f(int code, T x, T beta, bool b56, const T
 // so the NF*NPEN pencils run through the same LDS stages.  When all spectra
 // transform the same input (in_bs == 0, each with its own prologue) the input
 // rows are read once.  Output rows have pitch opitch >= N/2+1.
+//
+// Every load of the tile is issued at once, before the twiddles, with
+// addresses clamped into the tile instead of branches (the rows past the end
+// of a partial tile are zeroed afterwards), so a workgroup waits for one
+// memory round trip rather than one per element.  Consecutive lanes take
+// consecutive elements: the loads and the LDS writes stay unit-stride.
+// partials[blockIdx.y] receives the FP64 sum of slot sum_slot over the tile.
 template <typename T, int N, int NPEN, int NF>
 __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
     k_rows_r2c_f(const T *__restrict__ in, cpx<T> *__restrict__ out, long long in_bs, long long out_bs, long long nrows,
@@ -277,35 +303,81 @@ __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
     constexpr int ROWS = 2 * NPEN;
     constexpr int NZC = N / 2 + 1;
     constexpr int LD = N + 1;
+    constexpr int NE = ROWS * N;
+    constexpr int PER = (NE + NT - 1) / NT;
     __shared__ C tw[N];
     __shared__ C buf[NF * NPEN * LD];
     __shared__ double red[16];
     const int slot0 = blockIdx.x * NF;
     const long long row0 = (long long)blockIdx.y * ROWS;
     const int nv = (int)min((long long)ROWS, nrows - row0);
+    const int lim = nv * N;
     const T beta = (T)pro.beta;
+    const bool b56 = pro.b56 != 0;
     const T *aux = reinterpret_cast<const T *>(pro.aux);
     const T *phi = reinterpret_cast<const T *>(pro.phi);
-    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
-    double acc = 0.0;
-    for (int e = threadIdx.x; e < ROWS * N; e += NT) {
-        const int r = e / N;
-        const int pos = e - r * N;
-        T x0 = (T)0;
-        if (r < nv) {
-            x0 = in[(long long)slot0 * in_bs + row0 * N + e];
-            if (phi) x0 = (T)(2.0 * (double)phi[row0 * N + e] * (double)x0);
-        }
+    const bool sep = in_bs != 0;
+    const bool do_sum = partials != nullptr && sum_slot >= slot0 && sum_slot < slot0 + NF;
+    const T *src = in + slot0 * in_bs + row0 * N;
+    T x0[PER];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            T x = x0;
-            if (f > 0 && in_bs != 0 && r < nv) x = in[(long long)(slot0 + f) * in_bs + row0 * N + e];
-            if (partials != nullptr && slot0 + f == sum_slot) acc += (double)x;
-            if (r < nv) x = prologue_f<T>(pro.code[(slot0 + f) & 3], x, beta, pro.b56 != 0, aux, row0 * N + e);
-            reinterpret_cast<T *>(&buf[(f * NPEN + (r >> 1)) * LD + pos])[r & 1] = x;
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        x0[i] = src[e < lim ? e : lim - 1];
+    }
+    C twv = (threadIdx.x < N) ? twg[threadIdx.x] : C{};
+    T pv[PER], av[PER];
+    if (phi) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            pv[i] = phi[row0 * N + (e < lim ? e : lim - 1)];
         }
     }
-    const bool do_sum = partials != nullptr && sum_slot >= slot0 && sum_slot < slot0 + NF;
+    if (aux) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            av[i] = aux[row0 * N + (e < lim ? e : lim - 1)];
+        }
+    }
+    for (int i = threadIdx.x + NT; i < N; i += NT) tw[i] = twg[i];
+    if (threadIdx.x < N) tw[threadIdx.x] = twv;
+    double acc = 0.0;
+    auto write = [&](auto aux_c, auto b56_c) {
+        constexpr bool AUX = decltype(aux_c)::value, B56 = decltype(b56_c)::value;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = threadIdx.x + i * NT;
+            if (NE % NT == 0 || e < NE) {
+                const int r = e / N;
+                const int pos = e - r * N;
+                const bool live = e < lim;
+                T xa = live ? x0[i] : (T)0;
+                if (live && phi) xa = (T)(2.0 * (double)pv[i] * (double)xa);
+                const T a = AUX ? av[i] : (T)0;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    T x = xa;
+                    if (f > 0 && sep && live) x = in[(long long)(slot0 + f) * in_bs + row0 * N + e];
+                    if (do_sum && slot0 + f == sum_slot) acc += (double)x;
+                    if (live) x = prologue_t<T, AUX, B56>(pro.code[(slot0 + f) & 3], x, beta, a);
+                    reinterpret_cast<T *>(&buf[(f * NPEN + (r >> 1)) * LD + pos])[r & 1] = x;
+                }
+            }
+        }
+    };
+    if (aux) {
+        if (b56)
+            write(std::true_type{}, std::true_type{});
+        else
+            write(std::true_type{}, std::false_type{});
+    } else {
+        if (b56)
+            write(std::false_type{}, std::true_type{});
+        else
+            write(std::false_type{}, std::false_type{});
+    }
     if (do_sum) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
@@ -317,7 +389,7 @@ __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
         for (int w = 0; w < (NT + 63) / 64; ++w) sm += red[w];
         partials[blockIdx.y] = sm;
     }
-    rstages<C, N, LD, NF * NPEN, false, 0, 1>(buf, tw);
+    rstages<C, N, LD, NF * NPEN, false, 0, 1>(buf, tw, (int)threadIdx.x);
     for (int e = threadIdx.x; e < NF * ROWS * NZC; e += NT) {
         const int fr = e / NZC, k = e - fr * NZC;
         const int f = fr / ROWS, r = fr - f * ROWS;
@@ -337,6 +409,11 @@ __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
 }
 
 // Rows C2R (irfft semantics: imaginary parts of k = 0 and Nyquist ignored).
+// Each thread loads column k of a row pair (A of row 2p, B of row 2p+1) and
+// writes A + iB to column k and its Hermitian image conj(A) + i conj(B) to
+// column N - k of the packed pencil, so the half spectrum goes from registers
+// to the pencil in one LDS pass.  All loads of the tile are issued at once
+// with clamped addresses.
 template <typename T, int N, int NPEN>
 __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
     k_rows_c2r_f(const cpx<T> *__restrict__ in, T *__restrict__ out, long long in_bs, long long out_bs,
@@ -346,43 +423,49 @@ __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
     constexpr int ROWS = 2 * NPEN;
     constexpr int NZC = N / 2 + 1;
     constexpr int LD = N + 1;
+    constexpr int NW = NPEN * NZC;
+    constexpr int PER = (NW + NT - 1) / NT;
     __shared__ C tw[N];
-    __shared__ C stg[ROWS * NZC];
     __shared__ C buf[NPEN * LD];
     const long long row0 = (long long)blockIdx.y * ROWS;
     const C *src = in + (long long)blockIdx.x * in_bs + row0 * ipitch;
     T *dst = out + (long long)blockIdx.x * out_bs + row0 * N;
     const int nv = (int)min((long long)ROWS, nrows - row0);
-    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twg[i];
-    for (int e = threadIdx.x; e < ROWS * NZC; e += NT) {
-        const int r = e / NZC, k = e - r * NZC;
-        C v = (r < nv) ? src[r * ipitch + k] : C{(T)0, (T)0};
-        if (k == 0 || 2 * k == N) v.y = (T)0;
-        stg[e] = v;
+    C ra[PER], rb[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int w = threadIdx.x + i * NT;
+        const int wc = w < NW ? w : NW - 1;
+        const int p = wc / NZC, k = wc - p * NZC;
+        const int r0 = 2 * p < nv ? 2 * p : nv - 1;
+        const int r1 = 2 * p + 1 < nv ? 2 * p + 1 : nv - 1;
+        ra[i] = src[r0 * ipitch + k];
+        rb[i] = src[r1 * ipitch + k];
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NPEN * N; e += NT) {
-        const int p = e / N, k = e - p * N;
-        C A, B;
-        if (k < NZC) {
-            A = stg[(2 * p) * NZC + k];
-            B = stg[(2 * p + 1) * NZC + k];
-        } else {
-            A = stg[(2 * p) * NZC + N - k];
-            B = stg[(2 * p + 1) * NZC + N - k];
-            A.y = -A.y;
-            B.y = -B.y;
+    C twv = (threadIdx.x < N) ? twg[threadIdx.x] : C{};
+    for (int i = threadIdx.x + NT; i < N; i += NT) tw[i] = twg[i];
+    if (threadIdx.x < N) tw[threadIdx.x] = twv;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int w = threadIdx.x + i * NT;
+        if (NW % NT == 0 || w < NW) {
+            const int p = w / NZC, k = w - p * NZC;
+            C A = 2 * p < nv ? ra[i] : C{};
+            C B = 2 * p + 1 < nv ? rb[i] : C{};
+            if (k == 0 || 2 * k == N) {
+                A.y = (T)0;
+                B.y = (T)0;
+            }
+            buf[p * LD + k] = C{A.x - B.y, A.y + B.x};
+            if (k > 0 && 2 * k < N) buf[p * LD + N - k] = C{A.x + B.y, B.x - A.y};
         }
-        buf[p * LD + k] = C{A.x - B.y, A.y + B.x};
     }
     __syncthreads();
-    rstages<C, N, LD, NPEN, true, 0, 1>(buf, tw);
-    for (int e = threadIdx.x; e < ROWS * N; e += NT) {
+    rstages<C, N, LD, NPEN, true, 0, 1>(buf, tw, (int)threadIdx.x);
+    for (int e = threadIdx.x; e < nv * N; e += NT) {
         const int r = e / N;
-        if (r < nv) {
-            const C z = buf[(r >> 1) * LD + (e - r * N)];
-            dst[e] = ((r & 1) ? z.y : z.x) * scale;
-        }
+        const C z = buf[(r >> 1) * LD + (e - r * N)];
+        dst[e] = ((r & 1) ? z.y : z.x) * scale;
     }
 }
 
@@ -399,9 +482,8 @@ template <typename T, int N, int V> constexpr int col_tile_v() {
     return t < 1 ? 1 : t;
 }
 template <typename T, int N> constexpr int row_pens() {
-    int p = 8;
-    while (p > 1 && (p * SL<N>::tpp > 1024 || (2 * p * (N / 2 + 1) + p * (N + 1) + N) * 2 * (int)sizeof(T) > 64 * 1024))
-        --p;
+    int p = 16;
+    while (p > 1 && (p * SL<N>::tpp > 1024 || (p * (N + 1) + N) * 2 * (int)sizeof(T) > 64 * 1024)) --p;
     return p;
 }
 template <typename T, int N, int V> constexpr int row_pens_v() {
@@ -450,14 +532,14 @@ template <typename T, int N, int NF>
 static void launch_r2c_nf(int var, const T *in, cpx<T> *out, int nblk, long long in_bs, long long out_bs,
                           long long nrows, long long opitch, const cpx<T> *tw, T scale, const Prologue &pro,
                           double *partials, int sum_slot, hipStream_t s) {
-#define CALL(V)                                                                                                 \
-    {                                                                                                           \
-        constexpr int NP = r2c_pens_v<T, N, NF, V>();                                                           \
-        const long long tiles = (nrows + 2 * NP - 1) / (2 * NP);                                                \
-        OFDFT_REQUIRE(tiles <= 65535, "too many rows for the R2C pass");                                        \
-        dim3 grid((unsigned)nblk, (unsigned)tiles);                                                             \
+#define CALL(V)                                                                                                \
+    {                                                                                                          \
+        constexpr int NP = r2c_pens_v<T, N, NF, V>();                                                          \
+        const long long tiles = (nrows + 2 * NP - 1) / (2 * NP);                                               \
+        OFDFT_REQUIRE(tiles <= 65535, "too many rows for the R2C pass");                                       \
+        dim3 grid((unsigned)nblk, (unsigned)tiles);                                                            \
         hipLaunchKernelGGL((k_rows_r2c_f<T, N, NP, NF>), grid, dim3(NF * NP * SL<N>::tpp), 0, s, in, out, in_bs, \
-                           out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot);                          \
+                           out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot);                         \
     }
     OFDFT_VARIANT(var, CALL)
 #undef CALL
@@ -482,7 +564,9 @@ static void launch_c2r(int var, const cpx<T> *in, T *out, int batch, long long i
 #define CALL(V)                                                                                                \
     {                                                                                                          \
         constexpr int NP = row_pens_v<T, N, V>();                                                              \
-        dim3 grid((unsigned)batch, (unsigned)((nrows + 2 * NP - 1) / (2 * NP)));                               \
+        const long long tiles = (nrows + 2 * NP - 1) / (2 * NP);                                               \
+        OFDFT_REQUIRE(tiles <= 65535, "too many rows for the C2R pass");                                       \
+        dim3 grid((unsigned)batch, (unsigned)tiles);                                                           \
         hipLaunchKernelGGL((k_rows_c2r_f<T, N, NP>), grid, dim3(NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs, \
                            nrows, ipitch, tw, scale);                                                          \
     }
