@@ -78,13 +78,12 @@ __global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
 }
 
 // rho0 = sum / N with the kernel-reuse threshold of wt.py:64
-__global__ void k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache) {
-    __shared__ double sm[16];
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
-    acc = block_sum(acc, sm);
+__global__ void __launch_bounds__(kFinalThreads)
+    k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache) {
+    __shared__ double sm[kFinalThreads];
+    sum_columns(partials, nb, 1, sm);
     if (threadIdx.x == 0) {
-        const double r = acc * inv_n;
+        const double r = sm[0] * inv_n;
         if (fabs(*rho0_cache - r) > 1e-6) *rho0_cache = r;
     }
 }
@@ -218,28 +217,21 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
 
 // energies[0..5] = term sums * dV (TF also * x), [6] = their sum + E_ion,
 // [7] = electron count
-__global__ void k_energy_final(const double *__restrict__ partials, int nb, double *out, double dV, double x,
-                               double eion) {
-    __shared__ double sm[16];
-    double tot = 0.0;
-    for (int t = 0; t < 7; ++t) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[(long long)b * 7 + t];
-        acc = block_sum(acc, sm);
-        if (threadIdx.x == 0) {
-            double e = acc * dV;
-            if (t == 0) e *= x;
-            if (t < 6) {
-                out[t] = e;
-                tot += e;
-            } else {
-                out[7] = e;
-            }
+__global__ void __launch_bounds__(kFinalThreads)
+    k_energy_final(const double *__restrict__ partials, int nb, double *out, double dV, double x, double eion) {
+    __shared__ double sm[kFinalThreads];
+    sum_columns(partials, nb, 7, sm);
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int t = 0; t < 6; ++t) {
+            const double e = sm[t] * dV * (t == 0 ? x : 1.0);
+            out[t] = e;
+            tot += e;
         }
+        out[6] = tot + eion;
+        out[7] = sm[6] * dV;
     }
-    if (threadIdx.x == 0) out[6] = tot + eion;
 }
-
 
 template <typename T>
 void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s) {
@@ -265,7 +257,7 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch, fused);
         if (auto_rho0) {
             const int nb = r2c_num_partials(P, ns);
-            hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(256), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
+            hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
             OFDFT_HIP(hipGetLastError());
         }
         GParams gp;
@@ -309,7 +301,7 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
     hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, P.red, pp);
     OFDFT_HIP(hipGetLastError());
     if (pp.want_e) {
-        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
         OFDFT_HIP(hipGetLastError());
     }
     if (terms & OFDFT_TERM_LMGP) {

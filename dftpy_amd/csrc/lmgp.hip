@@ -286,14 +286,13 @@ __global__ void k_minmax_final(const double *partials, int nb, double *out) {
 }
 
 // energies[E_NL] = e dV, energies[E_SUM] += e dV
-__global__ void k_lmgp_energy(const double *partials, int nb, double dV, double *energies) {
-    __shared__ double sm[16];
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partials[b];
-    acc = block_sum(acc, sm);
+__global__ void __launch_bounds__(kFinalThreads)
+    k_lmgp_energy(const double *partials, int nb, double dV, double *energies) {
+    __shared__ double sm[kFinalThreads];
+    sum_columns(partials, nb, 1, sm);
     if (threadIdx.x == 0) {
-        energies[OFDFT_E_NL] = acc * dV;
-        energies[OFDFT_E_SUM] += acc * dV;
+        energies[OFDFT_E_NL] = sm[0] * dV;
+        energies[OFDFT_E_SUM] += sm[0] * dV;
     }
 }
 
@@ -426,7 +425,7 @@ template <typename T> void lmgp_eval(Plan &P, const T *rho, T *pot, double *ener
     hipLaunchKernelGGL((k_lmgp_final<T>), dim3(nb), dim3(256), 0, s, rho, n, kfmax, rmax, P.alpha, P.beta, 1.0 / 6.0,
                        b56, pot1, pot2, pot3, pot, pot != nullptr ? 1 : 0, P.red);
     OFDFT_HIP(hipGetLastError());
-    if (energies) hipLaunchKernelGGL(k_lmgp_energy, dim3(1), dim3(256), 0, s, P.red, nb, P.dV, energies);
+    if (energies) hipLaunchKernelGGL(k_lmgp_energy, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, P.dV, energies);
     OFDFT_HIP(hipGetLastError());
 }
 

@@ -38,20 +38,18 @@ struct Coefs {
 
 // out[t] = coef[t] * sum_b partials[b * nterms + t]; optionally out[nterms] =
 // sum_t out[t] + add.
-__global__ void k_reduce_final(const double *__restrict__ partials, int nblocks, int nterms, double *out,
-                               Coefs co) {
-    __shared__ double sm[16];
-    double tot = 0.0;
-    for (int t = 0; t < nterms; ++t) {
-        double acc = 0.0;
-        for (int bk = threadIdx.x; bk < nblocks; bk += blockDim.x) acc += partials[(long long)bk * nterms + t];
-        acc = block_sum(acc, sm);
-        if (threadIdx.x == 0) {
-            out[t] = acc * co.c[t];
-            tot += acc * co.c[t];
+__global__ void __launch_bounds__(kFinalThreads)
+    k_reduce_final(const double *__restrict__ partials, int nblocks, int nterms, double *out, Coefs co) {
+    __shared__ double sm[kFinalThreads];
+    sum_columns(partials, nblocks, nterms, sm);
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int t = 0; t < nterms; ++t) {
+            out[t] = sm[t] * co.c[t];
+            tot += sm[t] * co.c[t];
         }
+        if (co.sum_out) out[nterms] = tot + co.add;
     }
-    if (threadIdx.x == 0 && co.sum_out) out[nterms] = tot + co.add;
 }
 
 void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
@@ -59,7 +57,7 @@ void reduce_final(const double *partials, int nblocks, int nterms, double *out, 
     Coefs co{};
     for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
     co.sum_out = 0;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partials, nblocks, nterms, out, co);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co);
     OFDFT_HIP(hipGetLastError());
 }
 
@@ -69,7 +67,7 @@ void reduce_final_sum(const double *partials, int nblocks, int nterms, double *o
     for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
     co.sum_out = 1;
     co.add = add;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partials, nblocks, nterms, out, co);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co);
     OFDFT_HIP(hipGetLastError());
 }
 
