@@ -190,17 +190,54 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     const int kind = gp.kind[blockIdx.y];
     if (kind == 1) wt_consts(gp, tkf, wtfac);
     cstages_io<C, N, TILE, false, 0, 1, true, false>(buf, tw, col, nullptr, S, valid, pen, j, (T)1);
-    for (int i = threadIdx.x; i < N * TILE; i += NT) {
-        const int e = i / TILE, pn = i - e * TILE;
-        const long long cc = (long long)blockIdx.x * TILE + pn;
-        if (cc >= ncols) continue;
-        const int iy = (int)(cc / pitch), iz = (int)(cc % pitch);
-        if (iz >= vp) continue;
-        const double f = gfactor<T>(gp, kind, e, iy, iz, tkf, wtfac);
-        C v = buf[i];
-        v.x = (T)(v.x * f);
-        v.y = (T)(v.y * f);
-        buf[i] = v;
+    // multiply: this thread's pencil (pen) is one (iy, iz) column, its
+    // elements are e = j, j + tpp, ...; |G|^2 is built from the column's
+    // fixed part plus fx * b1 in the plan precision
+    if (valid) {
+        const int iy = (int)(c / pitch), iz = (int)(c - (long long)iy * pitch);
+        constexpr int TPP = SL<N>::tpp;
+        constexpr int PER = N / TPP;
+        if (gp.full) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int e = j + k * TPP;
+                const double f = gfactor<T>(gp, kind, e, iy, iz, tkf, wtfac);
+                C v = buf[e * TILE + pen];
+                v.x = (T)(v.x * f);
+                v.y = (T)(v.y * f);
+                buf[e * TILE + pen] = v;
+            }
+        } else {
+            const int fy = ffreq(iy, gp.ny);
+            const T cx = (T)(fy * gp.B[3] + iz * gp.B[6]);
+            const T cy = (T)(fy * gp.B[4] + iz * gp.B[7]);
+            const T cz = (T)(fy * gp.B[5] + iz * gp.B[8]);
+            const T bx = (T)gp.B[0], by = (T)gp.B[1], bz = (T)gp.B[2];
+            const T sc = (T)gp.scale;
+            const T itkf = (T)(1.0 / tkf), wsc = (T)(wtfac * gp.scale), lb = (T)gp.lbda;
+            const T sig = (T)(gp.sigma * gp.sigma / 4.0);
+            const bool yz0 = iy == 0 && iz == 0;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int e = j + k * TPP;
+                const T fx = (T)ffreq(e, N);
+                const T gx = fx * bx + cx, gy = fx * by + cy, gz = fx * bz + cz;
+                const T gg = gx * gx + gy * gy + gz * gz;
+                T f;
+                if (kind == 0) {
+                    f = (T)0.5 * gg * sc;
+                    if (gp.sigma > 0) f *= exp(-gg * sig);
+                } else if (kind == 1) {
+                    f = lindhard_t<T>(sqrt(gg) * itkf, lb, (T)1) * wsc;
+                } else {
+                    f = (yz0 && e == 0) ? (T)0 : (T)(4.0 * M_PI) * sc / gg;
+                }
+                C v = buf[e * TILE + pen];
+                v.x *= f;
+                v.y *= f;
+                buf[e * TILE + pen] = v;
+            }
+        }
     }
     __syncthreads();
     cstages_io<C, N, TILE, true, 0, 1, false, true>(buf, tw, nullptr, col, S, valid, pen, j, (T)1);

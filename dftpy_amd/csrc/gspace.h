@@ -14,7 +14,7 @@ namespace ofdft {
 // T = double for the FP64 pipeline, float for the FP32 one
 template <typename T> __device__ __forceinline__ T lindhard_t(T eta, T lbda, T mu) {
     const T tol = (T)1e-10;
-    if (eta > 3.65) {
+    if (eta > (T)3.65) {
         const T inv = (T)1 / (eta * eta);
         T s = (T)-3.00624946457977689e-3;
         s = (T)-3.38745061493813488e-3 + inv * s;
