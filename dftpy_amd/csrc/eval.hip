@@ -136,6 +136,14 @@ struct PostParams {
 constexpr double PZ_A = 0.0311, PZ_B = -0.048, PZ_C = 0.0020, PZ_D = -0.0116;
 constexpr double PZ_G = -0.1423, PZ_B1 = 1.0529, PZ_B2 = 0.3334;
 
+// Reciprocal and reciprocal square root: exact division for FP64; the
+// hardware approximations (about 1 ulp) for the FP32-bulk pipeline, whose
+// point-wise work is otherwise dominated by IEEE division sequences.
+template <typename CT> __device__ __forceinline__ CT rcp_p(CT x) { return (CT)1 / x; }
+template <> __device__ __forceinline__ float rcp_p<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename CT> __device__ __forceinline__ CT rsq_p(CT x) { return (CT)1 / sqrt(x); }
+template <> __device__ __forceinline__ float rsq_p<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+
 template <typename T, typename CT>
 __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, const T *__restrict__ vloc,
                        T *__restrict__ pot, double *partials, PostParams p) {
@@ -153,7 +161,6 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
         const CT cr = cbrt(r);
         const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
         const CT c_rc = r < (CT)1e-30 ? (CT)1e-10 : cr;  // cbrt(max(rho, 1e-30))
-        const CT s_rc = sqrt(rc);
         if (p.terms & OFDFT_TERM_TF) {
             const CT c2 = cr * cr;  // rho^(2/3)
             if (p.want_e) acc[0] += (double)(ctf * c2 * r);
@@ -161,15 +168,15 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
         }
         if (p.s_vw >= 0) {
             const CT a = (CT)work[p.s_vw * p.bs + i];
-            CT sq = (r > (CT)0 && r < (CT)1e-30) ? sqrt(r) : s_rc;  // sqrt(rho > 0 ? rho : 1e-30)
-            if (fabs(sq) < (CT)1e-30) sq = (CT)1e-30;
-            const CT vw = (CT)p.y * (a / sq);
+            // 1 / sqrt(rho > 0 ? rho : 1e-30)
+            const CT isq = rsq_p<CT>((r > (CT)0 && r < (CT)1e-30) ? r : rc);
+            const CT vw = (CT)p.y * (a * isq);
             if (p.want_e && r > (CT)0) acc[1] += (double)(r * vw);
             v += vw;
         }
         if (p.s_wt >= 0) {
             const CT c = (CT)work[p.s_wt * p.bs + i];
-            const CT rb_rc = p.b56 ? c_rc / s_rc : pow(rc, (CT)p.beta) / rc;  // rho^beta / rho
+            const CT rb_rc = p.b56 ? c_rc * rsq_p<CT>(rc) : pow(rc, (CT)p.beta) * rcp_p<CT>(rc);  // rho^beta / rho
             const CT vnl = (CT)(2.0 * p.alpha) * rb_rc * c;
             if (p.want_e) acc[2] += (double)(vnl * r) / (2.0 * p.alpha);
             v += vnl;
@@ -177,7 +184,7 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
         if (p.terms & OFDFT_TERM_XC) {
             CT rc3 = cr;
             if (rc3 < (CT)1e-30) rc3 = (CT)1e-30;
-            const CT rs = cbrt34pi / rc3;
+            const CT rs = cbrt34pi * rcp_p<CT>(rc3);
             CT ec, vc;
             if (rs < (CT)1) {
                 const CT lr = log(rs);
@@ -186,10 +193,10 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
                      (CT)(1.0 / 3.0 * (2 * PZ_D - PZ_C)) * rs;
             } else {
                 const CT sr = sqrt(rs);
-                const CT den = (CT)1 + (CT)PZ_B1 * sr + (CT)PZ_B2 * rs;
-                ec = (CT)PZ_G / den;
-                vc = ((CT)PZ_G + (CT)(7.0 / 6.0 * PZ_G * PZ_B1) * sr + (CT)(4.0 / 3.0 * PZ_G * PZ_B2) * rs) /
-                     (den * den);
+                const CT iden = rcp_p<CT>((CT)1 + (CT)PZ_B1 * sr + (CT)PZ_B2 * rs);
+                ec = (CT)PZ_G * iden;
+                vc = ((CT)PZ_G + (CT)(7.0 / 6.0 * PZ_G * PZ_B1) * sr + (CT)(4.0 / 3.0 * PZ_G * PZ_B2) * rs) *
+                     (iden * iden);
             }
             if (p.want_e) acc[3] += (double)((cx_e * rc3 + ec) * r);
             v += cx_v * rc3 + vc;
@@ -381,34 +388,36 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
         const T phi_i = phi[i], p_i = p[i];
         const CT r = (CT)rho[i], dr = (CT)(T)(2.0 * (double)phi_i * (double)p_i);  // drho = 2 phi p
         const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
-        const CT c_rp = cbrt(rp), s_rp = sqrt(rp);
+        const CT c_rp = cbrt(rp);
+        const CT ic = rcp_p<CT>(c_rp);  // rho^(-1/3)
+        const CT irp = ic * ic * ic;    // 1 / rho
         CT dv = (CT)0;
-        if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf / c_rp * dr;  // tf.py:38-40
+        if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf * ic * dr;  // tf.py:38-40
         if (pp.s_vw >= 0) {
-            const CT sq = (r > (CT)0 && r < (CT)1e-30) ? sqrt(r) : s_rp;  // sqrt(rho > 0 ? rho : 1e-30)
-            const CT ds = dr / ((CT)2 * sq);
-            const CT vwb = (CT)pp.y * (CT)base_a[i] / sq;
-            dv += ((CT)pp.y * (CT)work[pp.s_vw * pp.bs + i] - vwb * ds) / sq;
+            const CT isq = rsq_p<CT>((r > (CT)0 && r < (CT)1e-30) ? r : rp);  // 1 / sqrt(rho > 0 ? rho : 1e-30)
+            const CT ds = (CT)0.5 * dr * isq;
+            const CT vwb = (CT)pp.y * (CT)base_a[i] * isq;
+            dv += ((CT)pp.y * (CT)work[pp.s_vw * pp.bs + i] - vwb * ds) * isq;
         }
         if (pp.s_wt >= 0) {
-            const CT rb1 = pp.b56 ? c_rp / s_rp : pow(rp, (CT)pp.beta - (CT)1);  // rho^(beta-1)
-            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 / rp * dr * (CT)base_c[i] +
+            const CT rb1 = pp.b56 ? c_rp * rsq_p<CT>(rp) : pow(rp, (CT)pp.beta - (CT)1);  // rho^(beta-1)
+            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 * irp * dr * (CT)base_c[i] +
                                            rb1 * (CT)work[pp.s_wt * pp.bs + i]);
         }
         if (pp.terms & OFDFT_TERM_XC) {
             // semilocal_xc.py:291-303
             const CT rc3 = c_rp;
-            const CT rs = cbrt34pi / rc3;
+            const CT rs = cbrt34pi * ic;
             CT fc;
             if (rs < (CT)1) {
                 fc = -(CT)(PZ_A / 3.0) - ((CT)(PZ_C / 9.0) * (log(rs) * (CT)2 + (CT)1) + (CT)(PZ_D * 2.0 / 9.0)) * rs;
             } else {
                 const CT ta = (CT)PZ_B1 * sqrt(rs), tb = (CT)PZ_B2 * rs;
-                const CT de = (CT)1 + ta + tb;
+                const CT ide = rcp_p<CT>((CT)1 + ta + tb);
                 fc = (CT)(PZ_G / 36.0) *
-                     ((CT)5 * ta + (CT)7 * ta * ta + (CT)8 * tb + (CT)16 * tb * tb + (CT)21 * ta * tb) / (de * de * de);
+                     ((CT)5 * ta + (CT)7 * ta * ta + (CT)8 * tb + (CT)16 * tb * tb + (CT)21 * ta * tb) * (ide * ide * ide);
             }
-            dv += (cx3 * rc3 / rp + fc / rp) * dr;
+            dv += (cx3 * rc3 + fc) * irp * dr;
         }
         if (pp.s_h >= 0) dv += (CT)work[pp.s_h * pp.bs + i];
         const double a = ((double)pot[i] - m) * (double)p_i + (double)phi_i * (double)dv;
