@@ -73,13 +73,13 @@ def time_to_threshold(opt, wall, e_ref, tol):
     return None
 
 
-def run_fp64_plain(ev, rho64, e_ref, tol, nat):
+def run_fp64_plain(ev, rho64, e_ref, tol, nat, precond=False):
     import torch
     from dftpy_amd.optimization import Optimization
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     opt = Optimization(optimization_method="TN", EnergyEvaluator=ev,
-                       optimization_options={"econv": ECONV * nat, "maxiter": 100, "hvp": "fd"})
+                       optimization_options={"econv": ECONV * nat, "maxiter": 100, "hvp": "fd", "precond": precond})
     opt.optimize_rho(rho64)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -88,7 +88,7 @@ def run_fp64_plain(ev, rho64, e_ref, tol, nat):
     return time_to_threshold(opt, wall, e_ref, tol), wall, opt
 
 
-def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat):
+def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat, precond=False):
     """FP32-bulk TN from the density start_fn() returns; the start density is
     built inside the timed region.  Returns (seconds, info)."""
     import torch
@@ -100,7 +100,8 @@ def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat):
     torch.cuda.synchronize()
     t_start = time.perf_counter() - t0
     opt = Optimization(optimization_method="TN", EnergyEvaluator=ev,
-                       optimization_options={"econv": ECONV * nat, "maxiter": 60, "hvp": "analytic"})
+                       optimization_options={"econv": ECONV * nat, "maxiter": 60, "hvp": "analytic",
+                                             "precond": precond})
     r32 = opt.optimize_rho(rho_start32)
     r64 = DirectField(g64, data=r32.data.double())
     e64 = ev(r64, calcType={"E"}).energy
@@ -233,34 +234,46 @@ def main():
     from dftpy_amd.learned import WarmStartNet
     net = WarmStartNet.load()  # trained weights shipped with the package; fails loudly if absent
     pp = ev.PSEUDO
+    learned = learned_start_fn(ions, pp, g64, g32, net, use_net=True)
+    # FP32 forms: (start, preconditioned inner CG)
     forms = {
-        "learned": learned_start_fn(ions, pp, g64, g32, net, use_net=True),
-        "superposition": learned_start_fn(ions, pp, g64, g32, net, use_net=False),
-        "uniform": lambda: rho32,
+        "learned": (learned, True),
+        "superposition": (learned_start_fn(ions, pp, g64, g32, net, use_net=False), True),
+        "uniform": (lambda: rho32, True),
+        "learned_plain_cg": (learned, False),
     }
     for _ in range(args.warmup):
         run_fp64_plain(ev, rho64, e_ref, tol, nat)
-        for fn in forms.values():
-            run_fp32_form(ev, fn, g64, e_ref, tol, nat)
+        run_fp64_plain(ev, rho64, e_ref, tol, nat, precond=True)
+        for fn, pre in forms.values():
+            run_fp32_form(ev, fn, g64, e_ref, tol, nat, precond=pre)
     times = {k: [] for k in forms}
     infos = {k: [] for k in forms}
     t64, t64stop, opt64 = [], [], None
+    t64p, t64pstop = [], []
     torch.cuda.synchronize()
     for k in range(args.steps):
         tt64, wall64, opt64 = run_fp64_plain(ev, rho64, e_ref, tol, nat)
         t64.append(tt64 if tt64 is not None else float("nan"))
         t64stop.append(wall64)
-        for name, fn in forms.items():
-            tt, info = run_fp32_form(ev, fn, g64, e_ref, tol, nat)
+        tt64p, wall64p, opt64p = run_fp64_plain(ev, rho64, e_ref, tol, nat, precond=True)
+        t64p.append(tt64p if tt64p is not None else float("nan"))
+        t64pstop.append(wall64p)
+        for name, (fn, pre) in forms.items():
+            tt, info = run_fp32_form(ev, fn, g64, e_ref, tol, nat, precond=pre)
             times[name].append(tt)
             infos[name].append(info)
-        log("step %d: fp32+learned %.4f s (%s) | fp32 superposition %.4f s | fp32 uniform %.4f s | fp64 plain %.4f s "
-            "to its stop rule (%d iters, %d evals), %.4f s to threshold" %
-            (k, times["learned"][-1], infos["learned"][-1], times["superposition"][-1], times["uniform"][-1], wall64,
-             opt64.stats["niter"], opt64.stats["n_eval_v"] + opt64.stats["n_eval_ev"], t64[-1]))
+        log("step %d: fp32+learned %.4f s (%s) | fp32 superposition %.4f s | fp32 uniform %.4f s | fp32+learned "
+            "plain CG %.4f s | fp64 plain %.4f s to its stop rule (%d iters, %d evals), %.4f s to threshold | fp64 "
+            "preconditioned %.4f s (%d evals)" %
+            (k, times["learned"][-1], infos["learned"][-1], times["superposition"][-1], times["uniform"][-1],
+             times["learned_plain_cg"][-1], wall64, opt64.stats["niter"],
+             opt64.stats["n_eval_v"] + opt64.stats["n_eval_ev"], t64[-1], wall64p,
+             opt64p.stats["n_eval_v"] + opt64p.stats["n_eval_ev"]))
     v32 = float(np.mean(times["learned"]))
     v64 = float(np.mean(t64stop))
     v64_thr = float(np.mean(t64))
+    v64p = float(np.mean(t64pstop))
     # roofline of the dominant kernel: the x-axis C2C pass on 3 spectra (FP32)
     pt32 = pass_timings(g32, rho32)
     pt64 = pass_timings(g64, rho64)
@@ -284,14 +297,18 @@ def main():
                 dtype="fp32 (FP64 reductions/line search)", data="synthetic",
                 config=dict(workload="fcc Al 5x5x5 (500 atoms, 38.0 bohr, sigma 0.1 bohr seed %d), 160^3, WT+PZ-LDA+"
                                      "Hartree+empty-core PP (PME), TN to |E-E_ref| <= 1e-6 Ha/atom" % args.seed,
-                            form="FP32-bulk TN from the learned warm start (CNN inference at 160^3 inside the "
-                                 "timed region); plain form: FP64 TN (FD Hessian) from the uniform density",
+                            form="FP32-bulk TN (analytic Hessian product, inner CG preconditioned by the uniform-"
+                                 "gas Hessian) from the learned warm start (CNN inference at 160^3 inside the timed "
+                                 "region); plain form: the reference's FP64 TN (FD Hessian, plain inner CG) from the "
+                                 "uniform density",
                             e_ref=e_ref, e_ref_source=ref_src,
                             fp64_plain_s=round(v64, 5), speedup_vs_fp64_plain=round(v64 / v32, 3),
                             fp64_plain_first_within_threshold_s=round(v64_thr, 5),
                             speedup_vs_fp64_plain_first_within_threshold=round(v64_thr / v32, 3),
+                            fp64_preconditioned_s=round(v64p, 5), speedup_vs_fp64_preconditioned=round(v64p / v32, 3),
                             fp32_superposition_start_s=round(float(np.mean(times["superposition"])), 5),
                             fp32_uniform_start_s=round(float(np.mean(times["uniform"])), 5),
+                            fp32_learned_plain_cg_s=round(float(np.mean(times["learned_plain_cg"])), 5),
                             stop_rule="|dE| < 1e-6 Ha/atom * nat over 2 steps (all forms)",
                             runs=infos),
                 roofline=roof, cpu_baseline=cpu)

@@ -178,11 +178,33 @@ int ofdft_plan_destroy(ofdft_plan *P) {
         if (P->red) (void)hipFree(P->red);
         if (P->scal) (void)hipFree(P->scal);
         if (P->pinned) (void)hipHostFree(P->pinned);
+        if (P->tn_pre) (void)hipFree(P->tn_pre);
         delete P;
     });
 }
 
 size_t ofdft_plan_workspace_bytes(const ofdft_plan *P) { return P ? P->ws_bytes : 0; }
+
+int ofdft_set_tn_precond(ofdft_plan *P, int n, double dq, const double *table) {
+    return guard([&] {
+        OFDFT_REQUIRE(P != nullptr, "set_tn_precond: null plan");
+        if (n <= 0) {
+            P->tn_pre_n = 0;
+            return;
+        }
+        OFDFT_REQUIRE(n >= 2 && dq > 0.0 && table != nullptr, "set_tn_precond: bad table");
+        if (n > P->tn_pre_cap) {
+            if (P->tn_pre) OFDFT_HIP(hipFree(P->tn_pre));
+            P->tn_pre = nullptr;
+            P->tn_pre_cap = 0;
+            OFDFT_HIP(hipMalloc(&P->tn_pre, sizeof(double) * n));
+            P->tn_pre_cap = n;
+        }
+        OFDFT_HIP(hipMemcpy(P->tn_pre, table, sizeof(double) * n, hipMemcpyHostToDevice));
+        P->tn_pre_n = n;
+        P->tn_pre_dq = dq;
+    });
+}
 
 int ofdft_fft_r2c(ofdft_plan *P, const void *in, void *out, int batch, void *stream) {
     return guard([&] {

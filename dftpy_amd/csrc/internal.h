@@ -104,6 +104,11 @@ struct Plan {
     bool use_fixed = true;  // size-specialised FFT kernels where available
     // tile variants of the size-specialised passes (0 = default tile; set by the autotuner)
     int var_r2c = 0, var_c2r = 0, var_cols[3] = {0, 0, 0};
+    // radial table of the TN inner-CG preconditioner (M^-1(|G|) at |G| = i dq,
+    // i < tn_pre_n; tn_pre_n = 0: no preconditioner)
+    double *tn_pre = nullptr;
+    int tn_pre_n = 0, tn_pre_cap = 0;
+    double tn_pre_dq = 0.0;
 };
 
 int r2c_num_partials(const Plan &P, int batch);

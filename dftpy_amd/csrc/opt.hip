@@ -19,7 +19,7 @@ namespace ofdft {
 
 enum AuxField {
     F_PHI, F_RES, F_P, F_DIR, F_BEST, F_TRES, F_TP, F_PHI1, F_RHO1, F_POT1, F_AP, F_RES_PREV, F_DIR_PREV,
-    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_BASE_A, F_BASE_C, F_COUNT
+    F_POT, F_RHO, F_NPHI, F_NRHO, F_NPOT, F_BASE_A, F_BASE_C, F_Z, F_COUNT
 };
 
 void plan_aux_setup(Plan &P) {
@@ -243,6 +243,50 @@ __global__ void k_cg_sums(const T *r1, const T *r0, const T *d, long long n, dou
     write_partials<4>(acc, partials);
 }
 
+// sum a.b
+template <typename T> __global__ void k_dot2(const T *a, const T *b, long long n, double *partials) {
+    double acc[1] = {0};
+    GRID_LOOP(i, n) acc[0] += (double)a[i] * (double)b[i];
+    write_partials<1>(acc, partials);
+}
+
+struct PrecondParams {
+    double B[9];
+    const double *tab;
+    int n;
+    double inv_dq, scale;
+};
+
+// half spectrum (pitched rows, one workgroup per (ix, iy) row) times
+// M^-1(|G|), linearly interpolated in the radial table, and the 1/N of the
+// inverse transform
+template <typename T>
+__global__ void k_precond(cpx<T> *spec, long long pitch, int nzc, int nx, int ny, PrecondParams pp) {
+    const int row = blockIdx.x;
+    const int ix = row / ny, iy = row - ix * ny;
+    const int fx = ix <= (nx - 1) / 2 ? ix : ix - nx;
+    const int fy = iy <= (ny - 1) / 2 ? iy : iy - ny;
+    const double cx = fx * pp.B[0] + fy * pp.B[3], cy = fx * pp.B[1] + fy * pp.B[4], cz = fx * pp.B[2] + fy * pp.B[5];
+    cpx<T> *r = spec + (long long)row * pitch;
+    for (int iz = threadIdx.x; iz < nzc; iz += blockDim.x) {
+        const double gx = cx + iz * pp.B[6], gy = cy + iz * pp.B[7], gz = cz + iz * pp.B[8];
+        const double t = sqrt(gx * gx + gy * gy + gz * gz) * pp.inv_dq;
+        double m;
+        if (t >= (double)(pp.n - 1)) {
+            m = pp.tab[pp.n - 1];
+        } else {
+            const int i = (int)t;
+            const double f = t - i;
+            m = pp.tab[i] * (1.0 - f) + pp.tab[i + 1] * f;
+        }
+        const T c = (T)(m * pp.scale);
+        cpx<T> v = r[iz];
+        v.x *= c;
+        v.y *= c;
+        r[iz] = v;
+    }
+}
+
 #undef GRID_LOOP
 
 // ------------------------------------------------------------------ host helpers
@@ -323,6 +367,24 @@ template <typename T> struct Opt {
         return h[0] / h[1];
     }
 
+    // out = IFFT[M^-1(|G|) FFT[in]] with the plan's radial preconditioner table
+    void apply_precond(const T *in, T *out) {
+        const long long pitch = pipe_pitch(P);
+        const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+        cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
+        fft_r2c<T>(P, in, spec, 1, 0, ncp, 1.0, nullptr, nullptr, C.s, pitch);
+        PrecondParams pp;
+        for (int i = 0; i < 9; ++i) pp.B[i] = P.B[i];
+        pp.tab = P.tn_pre;
+        pp.n = P.tn_pre_n;
+        pp.inv_dq = 1.0 / P.tn_pre_dq;
+        pp.scale = 1.0 / (double)P.N;
+        hipLaunchKernelGGL((k_precond<T>), dim3((unsigned)(P.nr[0] * P.nr[1])), dim3(128), 0, C.s, spec, pitch, P.nzc,
+                           P.nr[0], P.nr[1], pp);
+        OFDFT_HIP(hipGetLastError());
+        fft_c2r<T>(P, spec, spec, out, 1, ncp, ncp, P.N, 1.0, C.s, pitch);
+    }
+
     // get_direction_TN (optimization.py:91-147); returns number of V evaluations.
     // hvp = 0: forward difference with eps (the reference); hvp = 1: exact
     // linear response (needs the last E+V evaluation's work slots at phi).
@@ -337,7 +399,21 @@ template <typename T> struct Opt {
             if (sl[1] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[1] * n, F(F_BASE_C), n);
         }
         hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
-        double r0 = C.fetch(1)[0];
+        double r0 = C.fetch(1)[0];  // |r|^2, for the stopping tests
+        // preconditioned CG (when the plan holds a preconditioner): z = M^-1 r,
+        // alpha = r.z / p.Ap, p <- z + beta p with beta = r1.z1 / r0.z0
+        const bool pre = P.tn_pre_n > 0;
+        T *z = F(F_Z);
+        double rz = r0;
+        auto precond_dot = [&]() {
+            apply_precond(tres, z);
+            hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, tres, z, n, P.red);
+        };
+        if (pre) {
+            precond_dot();
+            rz = C.fetch(1)[0];
+            hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, z, tp, n);
+        }
         const double rconv = r0 * 0.1;
         bool best_zero = true;
         std::vector<double> rl{1e10};
@@ -352,15 +428,19 @@ template <typename T> struct Opt {
                 hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps,
                                    Ap, n, P.red);
             }
-            // pAp -> scal[16]; the update takes alpha from there; r1 -> scal[17]
+            // pAp -> scal[16]; the update takes alpha from there; r1 -> scal[17]; r1.z1 -> scal[18]
             C.reduce_final_cols(1, 16);
-            hipLaunchKernelGGL((k_tn_update_dev<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, r0, P.scal + 16,
+            hipLaunchKernelGGL((k_tn_update_dev<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, rz, P.scal + 16,
                                n, P.red);
             C.reduce_final_cols(1, 17);
-            const double *hh = C.to_host(2, 16);
-            const double pAp = hh[0], r1 = hh[1];
+            if (pre) {
+                precond_dot();
+                C.reduce_final_cols(1, 18);
+            }
+            const double *hh = C.to_host(pre ? 3 : 2, 16);
+            const double pAp = hh[0], r1 = hh[1], rz1 = pre ? hh[2] : hh[1];
             if (pAp < 0.0) {
-                if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, r0 / pAp, tp, 0.0, dir, n);
+                if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, rz / pAp, tp, 0.0, dir, n);
                 break;
             }
             double rmin = rl[0];
@@ -381,9 +461,10 @@ template <typename T> struct Opt {
                     hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, best, dir, n);
                 break;
             }
-            const double beta = r1 / r0;
+            const double beta = rz1 / rz;
             r0 = r1;
-            hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, 1.0, tres, beta, tp, n);
+            rz = rz1;
+            hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, 1.0, pre ? z : tres, beta, tp, n);
         }
         return std::min(it + 1, maxfun);
     }

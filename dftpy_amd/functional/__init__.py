@@ -121,6 +121,7 @@ def configure(plan, funcs, rho_grid, eion=0.0):
         if f.terms & _lib.TERM_PP:
             vloc = f.vreal_for(rho_grid)
     rho0 = kedf["rho0"]
+    plan.kedf_settings = {k: kedf[k] for k in ("x", "y", "alpha", "beta")}
     _lib.check(_lib.lib.ofdft_set_kedf(plan.handle, float(kedf["x"]), float(kedf["y"]), float(kedf["alpha"]),
                                        float(kedf["beta"]), float(rho0) if rho0 else -1.0,
                                        float(kedf["sigma"]) if kedf["sigma"] else -1.0))

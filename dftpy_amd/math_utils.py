@@ -109,3 +109,46 @@ def lindhard(eta, lbda=1.0, mu=1.0):
         acc = c + inv * acc
     out[large] = 3.0 * (1.0 - mu) * e * e - lbda - 0.6 + inv * acc
     return out
+
+
+CTF = 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
+
+
+def lda_dv(rho0):
+    """d v_xc / d rho of the PZ LDA at a uniform density (semilocal_xc.py:291-303)."""
+    a, c, d = 0.0311, 0.0020, -0.0116
+    g, b1, b2 = -0.1423, 1.0529, 0.3334
+    rc = np.cbrt(rho0)
+    rs = np.cbrt(3.0 / (4.0 * np.pi)) / rc
+    fx = -np.cbrt(3.0 / np.pi) / 3.0 * rc / rho0
+    if rs < 1:
+        fc = -a / 3.0 - (c / 9.0 * (np.log(rs) * 2.0 + 1.0) + d * 2.0 / 9.0) * rs
+    else:
+        ta, tb = b1 * np.sqrt(rs), b2 * rs
+        fc = g / 36.0 * (5.0 * ta + 7.0 * ta * ta + 8.0 * tb + 16.0 * tb * tb + 21.0 * ta * tb) / (1 + ta + tb) ** 3
+    return fx + fc / rho0
+
+
+def uniform_response_kernel(q, rho0, tf=True, vw=True, wt=True, xc=True, hartree=True, x=1.0, y=1.0,
+                            alpha=5.0 / 6.0, beta=5.0 / 6.0):
+    """K(q) = d^2 E / d rho(q)^2 at the uniform density rho0 for the chosen
+    terms: TF x (10/9) C_TF rho0^(-1/3), vW y q^2 / (4 rho0), WT
+    2 alpha beta rho0^(alpha+beta-2) times the WT kernel (kernel.py:325-334),
+    PZ LDA dv/drho, Hartree 4 pi / q^2 (infinite at q = 0)."""
+    q = np.asarray(q, dtype=np.float64)
+    k = np.zeros_like(q)
+    if tf:
+        k = k + x * (10.0 / 9.0) * CTF * rho0 ** (-1.0 / 3.0)
+    if vw:
+        k = k + y * q * q / (4.0 * rho0)
+    if wt:
+        tkf = 2.0 * (3.0 * rho0 * np.pi ** 2) ** (1.0 / 3.0)
+        wtfac = 5.0 / (9.0 * alpha * beta * rho0 ** (alpha + beta - 5.0 / 3.0)) * CTF
+        k = k + 2.0 * alpha * beta * rho0 ** (alpha + beta - 2) * lindhard(q / tkf, x, 1.0) * wtfac
+    if hartree:
+        with np.errstate(divide="ignore"):
+            k = k + np.where(q > 0, 4.0 * np.pi / np.maximum(q, 1e-300) ** 2, np.inf)
+    if xc:
+        k = k + lda_dv(rho0)
+    return k
+

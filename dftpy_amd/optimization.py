@@ -8,8 +8,31 @@ from . import _lib
 from .field import DirectField
 from .functional import AbstractFunctional, configure
 from .functional.total_functional import TotalFunctional
+from .math_utils import uniform_response_kernel
 
 METHODS = {"TN": 0, "CG-HS": 1, "CG": 1, "CG-PR": 2, "CG-FR": 3}
+PRECOND_POINTS = 4096
+
+
+def tn_preconditioner(grid, rho0, terms, kedf):
+    """Radial table (values, dq) of M^-1(q), M(q) = 2 rho0 K(q): the Hessian
+    of the TN inner solve, (V - mu) + phi K phi-linearised, at the uniform
+    density rho0 (K from math_utils.uniform_response_kernel over the terms of
+    the functional; M^-1(0) = 0 when Hartree is present).  Covers |G| up to
+    half the sum of the reciprocal axes times the grid sizes."""
+    lat = np.asarray(grid.lattice, dtype=np.float64)
+    B = 2.0 * np.pi * np.linalg.inv(lat).T
+    nr = [int(n) for n in grid.nrR]
+    qmax = 0.5 * sum(nr[i] * np.linalg.norm(B[i]) for i in range(3)) * 1.01
+    q = np.linspace(0.0, qmax, PRECOND_POINTS)
+    K = uniform_response_kernel(q, rho0, tf=bool(terms & _lib.TERM_TF), vw=bool(terms & _lib.TERM_VW),
+                                wt=bool(terms & (_lib.TERM_WT | _lib.TERM_LMGP)), xc=bool(terms & _lib.TERM_XC),
+                                hartree=bool(terms & _lib.TERM_HARTREE), x=kedf["x"], y=kedf["y"],
+                                alpha=kedf["alpha"], beta=kedf["beta"])
+    M = 2.0 * rho0 * K
+    with np.errstate(divide="ignore"):
+        tab = np.where(np.isinf(M), 0.0, 1.0 / np.maximum(M, 1e-8))
+    return np.ascontiguousarray(tab), float(q[1] - q[0])
 
 
 class Optimization:
@@ -18,7 +41,7 @@ class Optimization:
         self.optimization_options = dict(optimization_options or {})
         defaults = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200, "maxls": 30,
                     "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
-                    "ncheck": 2, "hvp": "auto"}
+                    "ncheck": 2, "hvp": "auto", "precond": False}
         for k, v in defaults.items():
             self.optimization_options.setdefault(k, v)
         if EnergyEvaluator is None:
@@ -46,7 +69,7 @@ class Optimization:
         ev = self.EnergyEvaluator
         funcs = list(ev.funcDict.values()) if isinstance(ev, TotalFunctional) else [ev]
         eion = ev.eion() if isinstance(ev, TotalFunctional) else 0.0
-        _, vloc = configure(plan, funcs, grid, eion)
+        terms_all, vloc = configure(plan, funcs, grid, eion)
         o = self.optimization_options
         # Hessian-vector product of the TN inner solve: the reference's forward
         # difference (eps = 1e-9) needs FP64; FP32 grids use the exact linear
@@ -61,6 +84,16 @@ class Optimization:
         if hvp not in ("fd", "analytic"):
             raise ValueError("hvp must be 'auto', 'fd' or 'analytic'")
         hvp = 0 if hvp == "fd" else 1
+        # preconditioned inner CG of TN (not in the reference): opt-in
+        if o["precond"] and self.optimization_method == "TN":
+            key = (float(rho_in.data.mean()), int(terms_all), tuple(sorted(plan.kedf_settings.items())))
+            if getattr(plan, "tn_precond_key", None) != key:
+                tab, dq = tn_preconditioner(grid, key[0], key[1], plan.kedf_settings)
+                _lib.check(_lib.lib.ofdft_set_tn_precond(plan.handle, int(tab.size), dq, _lib.dptr(tab)))
+                plan.tn_precond_key = key
+        elif getattr(plan, "tn_precond_key", None) is not None:
+            _lib.check(_lib.lib.ofdft_set_tn_precond(plan.handle, 0, 0.0, None))
+            plan.tn_precond_key = None
         opts = np.array([o["econv"], o["maxiter"], o["maxfun"], o["maxls"], o["c1"], o["c2"], o["xtol"], o["ncheck"]],
                         dtype=np.float64)
         stats = np.zeros(16)

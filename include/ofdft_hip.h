@@ -170,6 +170,14 @@ int ofdft_ewald(ofdft_plan *plan, int nat, const double *pos_cart, const double 
                 const double *charges, int pme, int order, double precision, double *out,
                 void *stream);
 
+/* Preconditioner of the TN inner conjugate-gradient solve (no counterpart in
+ * the reference, whose get_direction_TN, optimization.py:91-147, runs plain
+ * CG).  table (host, n doubles): M^-1(q) at q = i dq, linearly interpolated
+ * in |G| and held at its last value beyond the table; while set (n > 0),
+ * ofdft_optimize's TN applies z = IFFT[M^-1(|G|) FFT[r]] to each CG residual
+ * (preconditioned CG; the stopping tests stay on the unpreconditioned
+ * residual).  n <= 0 removes it.  Setup call (copies the table). */
+int ofdft_set_tn_precond(ofdft_plan *plan, int n, double dq, const double *table);
 /* Density optimizer (Optimization.optimize_rho, optimization/optimization.py:281-520).
  * method: 0 = TN (get_direction_TN :91-147), 1 = CG-HS, 2 = CG-PR, 3 = CG-FR
  * (math_utils.py:395-423).  Line search: Moré-Thuente dcsrch on theta in

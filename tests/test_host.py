@@ -141,3 +141,52 @@ def test_fcc_builder():
     ref = systems.fcc_al((5, 5, 5))
     d = ions.positions - ref.positions
     assert 0.08 < d.std() < 0.12
+
+
+def test_uniform_response_kernel_is_the_oracle_linear_response():
+    """K(q) of math_utils.uniform_response_kernel (the TN preconditioner's
+    Hessian) equals the linear response of the oracle's total potential
+    (TF + vW + WT + PZ LDA + Hartree) to rho0 + eps cos(G.r), by central
+    differences, at three wave vectors and both LDA branches of rs."""
+    from oracle import functionals as OFN
+    from dftpy_amd.math_utils import uniform_response_kernel
+    L, n = 9.0, 18
+    g = OG.Grid(np.eye(3) * L, (n, n, n))
+    x = np.arange(n) * (L / n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    for rho0 in (0.03, 0.002):  # rs < 1 and rs > 1
+        kern = OFN.wt_kernel(g.q, rho0)
+        for m in ((1, 0, 0), (2, 1, 0), (3, 2, 1)):
+            G = 2.0 * np.pi / L * np.array(m, dtype=np.float64)
+            c = np.cos(G[0] * X + G[1] * Y + G[2] * Z)
+            eps = 1e-5 * rho0
+            v = []
+            for s in (1.0, -1.0):
+                r = rho0 + s * eps * c
+                _, vt, _ = OFN.total(r, g, np.zeros_like(r), kern, rho0=rho0, calc_e=False)
+                v.append(vt)
+            dv = (v[0] - v[1]) / (2.0 * eps)
+            k_num = float(np.sum(dv * c) / np.sum(c * c))
+            k = float(uniform_response_kernel(np.linalg.norm(G), rho0))
+            assert abs(k_num - k) < 1e-6 * abs(k), (rho0, m, k_num, k)
+
+
+def test_tn_preconditioner_table():
+    """The radial table is 1 / (2 rho0 K(q)): zero at q = 0 (Hartree),
+    2 / q^2 at large q (vW dominates), covering the grid's largest |G|."""
+    from types import SimpleNamespace
+    from dftpy_amd import _lib
+    from dftpy_amd.math_utils import uniform_response_kernel
+    from dftpy_amd.optimization import tn_preconditioner
+    grid = SimpleNamespace(lattice=np.diag([7.6, 7.6, 9.0]), nrR=np.array([32, 32, 36]))
+    terms = _lib.TERM_TF | _lib.TERM_VW | _lib.TERM_WT | _lib.TERM_XC | _lib.TERM_HARTREE
+    kedf = dict(x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0)
+    tab, dq = tn_preconditioner(grid, 0.0273, terms, kedf)
+    q = np.arange(tab.size) * dq
+    assert tab[0] == 0.0
+    assert q[-1] >= np.pi * np.sqrt((32 / 7.6) ** 2 * 2 + (36 / 9.0) ** 2)
+    np.testing.assert_allclose(tab[1:], 1.0 / (2 * 0.0273 * uniform_response_kernel(q[1:], 0.0273)), rtol=1e-14)
+    assert abs(tab[-1] * q[-1] ** 2 / 2.0 - 1.0) < 0.05
+    # without Hartree the q = 0 value is the finite local stiffness
+    tab2, _ = tn_preconditioner(grid, 0.0273, terms & ~_lib.TERM_HARTREE, kedf)
+    assert np.isfinite(tab2[0]) and tab2[0] > 0
