@@ -179,6 +179,7 @@ int ofdft_plan_destroy(ofdft_plan *P) {
         if (P->scal) (void)hipFree(P->scal);
         if (P->pinned) (void)hipHostFree(P->pinned);
         if (P->tn_pre) (void)hipFree(P->tn_pre);
+        plan_setup_free(*P);
         delete P;
     });
 }

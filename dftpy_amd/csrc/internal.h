@@ -109,7 +109,17 @@ struct Plan {
     double *tn_pre = nullptr;
     int tn_pre_n = 0, tn_pre_cap = 0;
     double tn_pre_dq = 0.0;
+    // staging of the setup routines' host inputs (PP tables, positions): a
+    // pinned host buffer and a device buffer, grown on demand and reused
+    char *stg_h = nullptr, *stg_d = nullptr;
+    size_t stg_cap = 0;
+    // PME B-spline moduli of this grid (x, y, z concatenated) for pme_order
+    void *pme_b = nullptr;
+    int pme_order = 0;
 };
+
+// pseudo.hip: release the staging buffers and PME tables of a plan
+void plan_setup_free(Plan &P);
 
 int r2c_num_partials(const Plan &P, int batch);
 // row pitch of the evaluation pipeline's spectra (128-byte aligned rows when

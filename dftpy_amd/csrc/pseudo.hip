@@ -11,6 +11,7 @@
 //    reciprocal sum on the G grid with the half-space mask
 //    (grid.py:657-695; ewald.py:448-465, 786-808), correction
 //    (ewald.py:467-482).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -338,31 +339,67 @@ static std::vector<double> bspline_bm(int n, int len, int order) {
     return out;
 }
 
-struct DevBuf {
-    void *p = nullptr;
-    DevBuf(size_t bytes) { OFDFT_HIP(hipMalloc(&p, bytes ? bytes : 8)); }
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
+// Host inputs of one setup call go through the plan's staging pair: filled
+// in pinned host memory and sent with one asynchronous copy.  Every routine
+// that stages synchronises its stream before returning, so the next call may
+// refill the host buffer.
+static char *stage_host(Plan &P, size_t bytes) {
+    if (bytes > P.stg_cap) {
+        if (P.stg_h) OFDFT_HIP(hipHostFree(P.stg_h));
+        if (P.stg_d) OFDFT_HIP(hipFree(P.stg_d));
+        P.stg_h = P.stg_d = nullptr;
+        P.stg_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+        OFDFT_HIP(hipHostMalloc((void **)&P.stg_h, cap));
+        OFDFT_HIP(hipMalloc((void **)&P.stg_d, cap));
+        P.stg_cap = cap;
     }
-    template <typename X> X *as() { return reinterpret_cast<X *>(p); }
+    return P.stg_h;
+}
+
+static char *stage_upload(Plan &P, size_t bytes, hipStream_t s) {
+    OFDFT_HIP(hipMemcpyAsync(P.stg_d, P.stg_h, bytes, hipMemcpyHostToDevice, s));
+    return P.stg_d;
+}
+
+// byte offsets of consecutive staged arrays (8-byte aligned)
+struct StageLayout {
+    size_t size = 0;
+    size_t add(size_t bytes) {
+        const size_t o = size;
+        size += (bytes + 7) & ~(size_t)7;
+        return o;
+    }
 };
 
-struct BTabHost {
-    DevBuf bx, by, bz;
-    BTab tab;
-    BTabHost(const Plan &P, int order)
-        : bx(sizeof(double2) * P.nr[0]), by(sizeof(double2) * P.nr[1]), bz(sizeof(double2) * P.nzc) {
+void plan_setup_free(Plan &P) {
+    if (P.stg_h) (void)hipHostFree(P.stg_h);
+    if (P.stg_d) (void)hipFree(P.stg_d);
+    if (P.pme_b) (void)hipFree(P.pme_b);
+    P.stg_h = P.stg_d = nullptr;
+    P.stg_cap = 0;
+    P.pme_b = nullptr;
+    P.pme_order = 0;
+}
+
+// B-spline moduli of the plan's grid for `order`, computed once per plan
+static BTab pme_tab(Plan &P, int order) {
+    if (P.pme_b == nullptr || P.pme_order != order) {
+        if (P.pme_b) OFDFT_HIP(hipFree(P.pme_b));
+        P.pme_b = nullptr;
         auto x = bspline_bm(P.nr[0], P.nr[0], order);
         auto y = bspline_bm(P.nr[1], P.nr[1], order);
         auto z = bspline_bm(P.nr[2], P.nzc, order);
-        OFDFT_HIP(hipMemcpy(bx.p, x.data(), x.size() * 8, hipMemcpyHostToDevice));
-        OFDFT_HIP(hipMemcpy(by.p, y.data(), y.size() * 8, hipMemcpyHostToDevice));
-        OFDFT_HIP(hipMemcpy(bz.p, z.data(), z.size() * 8, hipMemcpyHostToDevice));
-        tab.bx = bx.as<double2>();
-        tab.by = by.as<double2>();
-        tab.bz = bz.as<double2>();
+        std::vector<double> all(x);
+        all.insert(all.end(), y.begin(), y.end());
+        all.insert(all.end(), z.begin(), z.end());
+        OFDFT_HIP(hipMalloc(&P.pme_b, all.size() * sizeof(double)));
+        OFDFT_HIP(hipMemcpy(P.pme_b, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+        P.pme_order = order;
     }
-};
+    const double2 *b = reinterpret_cast<const double2 *>(P.pme_b);
+    return BTab{b, b + P.nr[0], b + P.nr[0] + P.nr[1]};
+}
 
 static int nblk(long long n) {
     long long b = (n + 255) / 256;
@@ -378,12 +415,19 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
     OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
     VlocTable t;
     std::memset(&t, 0, sizeof(t));
-    std::vector<DevBuf *> coefbufs;
+    StageLayout lay;
+    size_t coff[kMaxSpecies];
+    for (int sp = 0; sp < nspecies; ++sp) coff[sp] = lay.add(sizeof(double) * 4 * ncoef[sp]);
+    const size_t opos = lay.add(sizeof(double) * 3 * nat), ofrac = lay.add(sizeof(double) * 3 * nat),
+                 ospec = lay.add(sizeof(int) * nat);
+    char *h = stage_host(P, lay.size);
+    for (int sp = 0; sp < nspecies; ++sp) std::memcpy(h + coff[sp], coef[sp], sizeof(double) * 4 * ncoef[sp]);
+    std::memcpy(h + opos, pos_cart, sizeof(double) * 3 * nat);
+    std::memcpy(h + ofrac, pos_frac, sizeof(double) * 3 * nat);
+    std::memcpy(h + ospec, species, sizeof(int) * nat);
+    char *d = stage_upload(P, lay.size, s);
     for (int sp = 0; sp < nspecies; ++sp) {
-        DevBuf *b = new DevBuf(sizeof(double) * 4 * ncoef[sp]);
-        coefbufs.push_back(b);
-        OFDFT_HIP(hipMemcpy(b->p, coef[sp], sizeof(double) * 4 * ncoef[sp], hipMemcpyHostToDevice));
-        t.coef[sp] = b->as<double>();
+        t.coef[sp] = reinterpret_cast<const double *>(d + coff[sp]);
         t.q0[sp] = q0[sp];
         t.dq[sp] = dq[sp];
         t.qmax[sp] = qmax[sp];
@@ -393,41 +437,32 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
         t.v2[sp] = vsmall[3 * sp + 2];
         t.dqs[sp] = dqs[sp];
     }
-    DevBuf dpos(sizeof(double) * 3 * nat), dfrac(sizeof(double) * 3 * nat), dspec(sizeof(int) * nat);
-    OFDFT_HIP(hipMemcpy(dpos.p, pos_cart, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
-    OFDFT_HIP(hipMemcpy(dfrac.p, pos_frac, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
-    OFDFT_HIP(hipMemcpy(dspec.p, species, sizeof(int) * nat, hipMemcpyHostToDevice));
+    const double *dpos = reinterpret_cast<const double *>(d + opos);
+    const double *dfrac = reinterpret_cast<const double *>(d + ofrac);
+    const int *dspec = reinterpret_cast<const int *>(d + ospec);
     const GGeom g = geom_of(P);
     double2 *spec = reinterpret_cast<double2 *>(P.spec);  // slot 0: accumulated v(G), slot 1: species S(G)
     double *Q = reinterpret_cast<double *>(P.work);
-    try {
-        if (!pme) {
-            // vreal = irfftn(v) / dV
-            hipLaunchKernelGGL(k_vloc_exact, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, nat,
-                               dpos.as<double>(), dspec.as<int>(), 1.0);
+    if (!pme) {
+        // vreal = irfftn(v) / dV
+        hipLaunchKernelGGL(k_vloc_exact, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, nat, dpos, dspec, 1.0);
+        OFDFT_HIP(hipGetLastError());
+    } else {
+        const BTab bt = pme_tab(P, order);
+        for (int sp = 0; sp < nspecies; ++sp) {
+            OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
+            hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
+                               dfrac, nullptr, dspec, sp);
             OFDFT_HIP(hipGetLastError());
-        } else {
-            BTabHost bt(P, order);
-            for (int sp = 0; sp < nspecies; ++sp) {
-                OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
-                hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order,
-                                   nat, dfrac.as<double>(), nullptr, dspec.as<int>(), sp);
-                OFDFT_HIP(hipGetLastError());
-                fft_r2c<double>(P, Q, spec + P.Nc, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
-                // v = sum_s vloc_s * rfftn(Q_s) * dV * B * N / V = sum_s vloc_s rfftn(Q_s) B
-                hipLaunchKernelGGL(k_vloc_pme_acc, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc, g,
-                                   t, sp, bt.tab, sp == 0 ? 1 : 0);
-                OFDFT_HIP(hipGetLastError());
-            }
-            OFDFT_HIP(hipStreamSynchronize(s));
+            fft_r2c<double>(P, Q, spec + P.Nc, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
+            // v = sum_s vloc_s * rfftn(Q_s) * dV * B * N / V = sum_s vloc_s rfftn(Q_s) B
+            hipLaunchKernelGGL(k_vloc_pme_acc, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc, g, t, sp,
+                               bt, sp == 0 ? 1 : 0);
+            OFDFT_HIP(hipGetLastError());
         }
-        fft_c2r<double>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
-        OFDFT_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-        for (auto *b : coefbufs) delete b;
-        throw;
     }
-    for (auto *b : coefbufs) delete b;
+    fft_c2r<double>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
+    OFDFT_HIP(hipStreamSynchronize(s));
 }
 
 static double erfcinv_host(double y) {
@@ -473,15 +508,22 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
     const double rmax = erfcinv_host(precision / 3.0) / std::sqrt(eta);
     int Nimg[3];
     for (int d = 0; d < 3; ++d) Nimg[d] = (int)std::ceil(rmax / lengths[d]);
-    DevBuf dpos(sizeof(double) * 3 * nat), dfrac(sizeof(double) * 3 * nat), dq(sizeof(double) * nat);
-    OFDFT_HIP(hipMemcpy(dpos.p, pos_cart, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
-    OFDFT_HIP(hipMemcpy(dfrac.p, pos_frac, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
-    OFDFT_HIP(hipMemcpy(dq.p, charges, sizeof(double) * nat, hipMemcpyHostToDevice));
+    StageLayout lay;
+    const size_t opos = lay.add(sizeof(double) * 3 * nat), ofrac = lay.add(sizeof(double) * 3 * nat),
+                 oq = lay.add(sizeof(double) * nat);
+    char *hs = stage_host(P, lay.size);
+    std::memcpy(hs + opos, pos_cart, sizeof(double) * 3 * nat);
+    std::memcpy(hs + ofrac, pos_frac, sizeof(double) * 3 * nat);
+    std::memcpy(hs + oq, charges, sizeof(double) * nat);
+    char *ds = stage_upload(P, lay.size, s);
+    const double *dpos = reinterpret_cast<const double *>(ds + opos);
+    const double *dfrac = reinterpret_cast<const double *>(ds + ofrac);
+    const double *dq = reinterpret_cast<const double *>(ds + oq);
     double *res = P.scal + 8;
     const long long npair = (long long)nat * nat;
     const int nbr = nblk(npair);
     const double *L = P.lat;
-    hipLaunchKernelGGL(k_ewald_real, dim3(nbr), dim3(256), 0, s, nat, dpos.as<double>(), dq.as<double>(), L[0], L[1],
+    hipLaunchKernelGGL(k_ewald_real, dim3(nbr), dim3(256), 0, s, nat, dpos, dq, L[0], L[1],
                        L[2], L[3], L[4], L[5], L[6], L[7], L[8], Nimg[0], Nimg[1], Nimg[2], rmax, std::sqrt(eta),
                        P.red);
     OFDFT_HIP(hipGetLastError());
@@ -491,14 +533,14 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
     const int nbg = std::min(nblk(P.Nc), kRedBlocks);
     double2 *spec = reinterpret_cast<double2 *>(P.spec);
     if (pme) {
-        BTabHost bt(P, order);
+        const BTab bt = pme_tab(P, order);
         double *Q = reinterpret_cast<double *>(P.work);
         OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
         hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
-                           dfrac.as<double>(), dq.as<double>(), nullptr, -1);
+                           dfrac, dq, nullptr, -1);
         OFDFT_HIP(hipGetLastError());
         fft_r2c<double>(P, Q, spec, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
-        hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, spec, P.Nc, g, eta, bt.tab, nat, nullptr,
+        hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, spec, P.Nc, g, eta, bt, nat, nullptr,
                            nullptr, P.red);
         OFDFT_HIP(hipGetLastError());
         const double c = 4.0 * M_PI / P.volume;
@@ -507,7 +549,7 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
     } else {
         BTab none{nullptr, nullptr, nullptr};
         hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, nullptr, P.Nc, g, eta, none, nat,
-                           dpos.as<double>(), dq.as<double>(), P.red);
+                           dpos, dq, P.red);
         OFDFT_HIP(hipGetLastError());
         const double c = 4.0 * M_PI / P.volume;
         reduce_final(P.red, nbg, 1, res + 1, &c, s);

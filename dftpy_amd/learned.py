@@ -109,16 +109,19 @@ def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
     densities (the G = 0 term gives the electron count)."""
     plan = grid.plan(torch.float64)
     keys = sorted(readpp.tables)
-    tabs = [response_table(readpp.gp[k], readpp.vp[k], readpp.zval[k], rho0) for k in keys]
     ns = len(keys)
-    q0 = np.array([t.q0 for t in tabs])
-    dq = np.array([t.dq for t in tabs])
-    nco = np.array([t.coef.shape[0] for t in tabs], dtype=np.int32)
-    qmax = np.array([t.qmax for t in tabs])
-    vsmall = np.array([v for t in tabs for v in t.vsmall])
-    dqs = np.array([t.dqs for t in tabs])
-    coefs = [np.ascontiguousarray(t.coef.reshape(-1)) for t in tabs]
-    cptr = (_lib._DP * ns)(*[_lib.dptr(c) for c in coefs])
+    # the radial tables depend on the species and rho0 only: kept on the
+    # pseudopotential reader like its own tables
+    cache = readpp.__dict__.setdefault("_screening_tables", {})
+    ck = (tuple(keys), float(rho0))
+    if ck not in cache:
+        tabs = [response_table(readpp.gp[k], readpp.vp[k], readpp.zval[k], rho0) for k in keys]
+        coefs = [np.ascontiguousarray(t.coef.reshape(-1)) for t in tabs]
+        cache[ck] = (np.array([t.q0 for t in tabs]), np.array([t.dq for t in tabs]),
+                     np.array([t.coef.shape[0] for t in tabs], dtype=np.int32), np.array([t.qmax for t in tabs]),
+                     np.array([v for t in tabs for v in t.vsmall]), np.array([t.dqs for t in tabs]), coefs,
+                     (_lib._DP * ns)(*[_lib.dptr(c) for c in coefs]))
+    q0, dq, nco, qmax, vsmall, dqs, coefs, cptr = cache[ck]
     pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
     frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
     spec = np.array([keys.index(s) for s in ions.symbols], dtype=np.int32)
