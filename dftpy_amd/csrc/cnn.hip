@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(MID_THREADS)
                     v += bias[co];
                     const float r = fmaxf(v, 0.0f);
                     reinterpret_cast<__bf16 *>(out0)[e] = (__bf16)((float)xin[e] + r);
-                    out1[e] = (__bf16)r;
+                    if (out1) out1[e] = (__bf16)r;
                 } else {
                     v += resid[e];
                     reinterpret_cast<float *>(out0)[e] = v;
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(ROWS_THREADS)
                     v += bias[co];
                     const float r = fmaxf(v, 0.0f);
                     reinterpret_cast<__bf16 *>(out0)[e] = (__bf16)((float)xin[e] + r);
-                    out1[e] = (__bf16)r;
+                    if (out1) out1[e] = (__bf16)r;
                 } else {
                     v += resid[e];
                     reinterpret_cast<float *>(out0)[e] = v;
@@ -724,7 +724,7 @@ int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const 
                        const float *resid, const void *mask, void *out0, void *out1, void *stream) {
     return cnn_guard([&] {
         const Geo g = make_geo(nr, dil);
-        OFDFT_REQUIRE(x && w && out0 && out1, "null pointer");
+        OFDFT_REQUIRE(x && w && out0 && (out1 || mode == 0), "null pointer");
         const __bf16 *xb = reinterpret_cast<const __bf16 *>(x);
         const __bf16 *wb = reinterpret_cast<const __bf16 *>(w);
         OFDFT_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (forward) or 1 (backward-data)");

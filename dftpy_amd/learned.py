@@ -230,7 +230,6 @@ class WarmStartNet:
             b["loss"] = torch.zeros(1, dtype=torch.float64, device=dev)
         else:
             b["h"] = [bf(), bf()]
-            b["r"] = [bf()]
         self._bufs = {k: v for k, v in self._bufs.items() if k[1] != train}  # one grid shape per mode
         self._bufs[key] = b
         return b
@@ -249,12 +248,11 @@ class WarmStartNet:
                                        self.param("b1").data_ptr(), h[0].data_ptr(), s))
         for i, l in enumerate(range(2, 6)):
             if train:
-                hin, hout, r = h[i], h[i + 1], b["r"][i]
-            else:
-                hin, hout, r = h[i % 2], h[(i + 1) % 2], b["r"][0]
+                hin, hout, r = h[i], h[i + 1], b["r"][i].data_ptr()
+            else:  # inference keeps no ReLU outputs
+                hin, hout, r = h[i % 2], h[(i + 1) % 2], None
             _lib.check(L.ofdft_cnn_conv_mid(nrp, self.dil[l - 1], 0, hin.data_ptr(), self.wf[i].data_ptr(),
-                                            self.param("b%d" % l).data_ptr(), None, None, hout.data_ptr(),
-                                            r.data_ptr(), s))
+                                            self.param("b%d" % l).data_ptr(), None, None, hout.data_ptr(), r, s))
         hlast = h[4] if train else h[0]
         _lib.check(L.ofdft_cnn_conv_out(nrp, self.dil[5], hlast.data_ptr(), self.param("W6").data_ptr(),
                                         self.param("b6").data_ptr(), b["out"].data_ptr(), b["proj"].data_ptr(), s))

@@ -224,7 +224,8 @@ int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, 
 int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w, const float *b, void *h,
                       void *stream);
 /* 32 -> 32 layer on MFMA (bf16 in, FP32 accumulation), w [27][32 out][32 in] bf16.
- * mode 0 (forward): r = relu(conv(x) + bias); out0 (bf16 [N][32]) = x + r; out1 (bf16) = r.
+ * mode 0 (forward): r = relu(conv(x) + bias); out0 (bf16 [N][32]) = x + r; out1 (bf16) = r, or
+ *        NULL when r is not needed (inference).
  * mode 1 (backward-data, w from ofdft_cnn_prep_weights' wb): v = conv(x) + resid (FP32);
  *        out0 (FP32 [N][32]) = v; out1 (bf16) = mask > 0 ? v : 0.  out0 must not alias x
  *        (it may alias resid). */
