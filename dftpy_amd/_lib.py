@@ -56,6 +56,12 @@ SIGNATURES = {
     "ofdft_ewald": (_I, [_P, _I, _DP, _DP, _DP, _I, _I, _D, _DP, _P]),
     "ofdft_optimize": (_I, [_P, _P, _P, _I, _I, _DP, _DP, _DP, _I, _P]),
     "ofdft_set_tn_precond": (_I, [_P, _I, _D, _DP]),
+    "ofdft_local_pp_forces": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
+                                   _DP, _P]),
+    "ofdft_local_pp_stress": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
+                                   _D, _DP, _P]),
+    "ofdft_ewald_forces_stress": (_I, [_P, _I, _DP, _DP, _DP, _D, _DP, _DP, _P]),
+    "ofdft_functional_stress": (_I, [_P, _P, _DP, _DP, _P]),
     "ofdft_dcsrch": (_I, [_DP, _D, _D, _D, _D, _D, _D, _D, _IP, _DP]),
     "ofdft_cn_step": (_I, [_P, _P, _P, _D, _D, _D, _I, _P, _DP, _P]),
     "ofdft_cn_batch_workspace_bytes": (ctypes.c_size_t, [_P, _I]),

@@ -21,6 +21,16 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
               const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, double *vreal,
               hipStream_t s);
+void pp_forces(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+               const double *pos_cart, const double *pos_frac, const int *species, double *F, hipStream_t s);
+void pp_stress(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+               const double *pos_cart, const double *pos_frac, const int *species, double energy, double *S,
+               hipStream_t s);
+void ewald_fs(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges,
+              double precision, double *F, double *S, hipStream_t s);
+void functional_stress(Plan &P, const double *rho, const double *energies, double *out, hipStream_t s);
 void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges, int pme,
            int order, double precision, double *out, hipStream_t s);
 
@@ -395,6 +405,50 @@ int ofdft_ewald(ofdft_plan *P, int nat, const double *pos_cart, const double *po
     return guard([&] {
         OFDFT_REQUIRE(P && nat >= 1 && out, "ewald: bad arguments");
         ewald(*P, nat, pos_cart, pos_frac, charges, pme, order, precision, out, S(stream));
+    });
+}
+
+int ofdft_local_pp_forces(ofdft_plan *P, const void *rho, int nspecies, const double *q0, const double *dq,
+                          const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                          const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                          const int *species, double *forces, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho && forces && nat >= 1 && nspecies >= 1, "local_pp_forces: bad arguments");
+        for (int a = 0; a < nat; ++a)
+            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "local_pp_forces: species index out of range");
+        for (int k = 0; k < 3 * nat; ++k) forces[k] = 0.0;
+        pp_forces(*P, (const double *)rho, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac,
+                  species, forces, S(stream));
+    });
+}
+
+int ofdft_local_pp_stress(ofdft_plan *P, const void *rho, int nspecies, const double *q0, const double *dq,
+                          const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                          const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                          const int *species, double energy, double *stress, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho && stress && nat >= 1 && nspecies >= 1, "local_pp_stress: bad arguments");
+        for (int a = 0; a < nat; ++a)
+            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "local_pp_stress: species index out of range");
+        pp_stress(*P, (const double *)rho, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac,
+                  species, energy, stress, S(stream));
+    });
+}
+
+int ofdft_ewald_forces_stress(ofdft_plan *P, int nat, const double *pos_cart, const double *pos_frac,
+                              const double *charges, double precision, double *forces, double *stress, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && nat >= 1 && (forces || stress), "ewald_forces_stress: bad arguments");
+        if (forces)
+            for (int k = 0; k < 3 * nat; ++k) forces[k] = 0.0;
+        ewald_fs(*P, nat, pos_cart, pos_frac, charges, precision, forces, stress, S(stream));
+    });
+}
+
+int ofdft_functional_stress(ofdft_plan *P, const void *rho, const double *energies, double *out, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho && energies && out, "functional_stress: bad arguments");
+        functional_stress(*P, (const double *)rho, energies, out, S(stream));
     });
 }
 

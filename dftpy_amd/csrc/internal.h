@@ -116,6 +116,9 @@ struct Plan {
     // PME B-spline moduli of this grid (x, y, z concatenated) for pme_order
     void *pme_b = nullptr;
     int pme_order = 0;
+    // workspace of the force and stress routines (grown on demand)
+    void *fs_ws = nullptr;
+    size_t fs_cap = 0;
 };
 
 // pseudo.hip: release the staging buffers and PME tables of a plan

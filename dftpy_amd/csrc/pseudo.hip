@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "pointwise.h"
 #include "reduce.h"
 
 namespace ofdft {
@@ -376,6 +377,9 @@ void plan_setup_free(Plan &P) {
     if (P.stg_h) (void)hipHostFree(P.stg_h);
     if (P.stg_d) (void)hipFree(P.stg_d);
     if (P.pme_b) (void)hipFree(P.pme_b);
+    if (P.fs_ws) (void)hipFree(P.fs_ws);
+    P.fs_ws = nullptr;
+    P.fs_cap = 0;
     P.stg_h = P.stg_d = nullptr;
     P.stg_cap = 0;
     P.pme_b = nullptr;
@@ -563,6 +567,666 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
     out[2] = e_real;
     out[3] = e_rec;
     out[4] = eta;
+}
+
+
+// ================================================================== forces and stress
+// Local-PP and Ewald forces and stress with exact structure factors, and the
+// stress of the TF, vW, WT, PZ-LDA and Hartree terms (ewald.py:541-760,
+// functional/pseudo/__init__.py:414-470, kedf/tf.py:52-62, kedf/vw.py:97-111,
+// kedf/wt.py:75-105, kedf/kernel.py:197-211, hartree.py:67-91,
+// xc/semilocal_xc.py:308-322).  G-space sums run over the half-space mask
+// with the reference's factors.  exp(-i G.R_a) is the product of per-axis
+// tables exp(-2 pi i k f_{a,d}) (G.R = 2 pi k.f), so no sincos is evaluated
+// per (atom, G) pair.
+
+// phase tables, per atom [x: nx | y: ny | z: nzc], frequencies as gvec uses them
+__global__ void k_phase_tables(int nat, const double *__restrict__ frac, int nx, int ny, int nz, int nzc, int full,
+                               double2 *__restrict__ tab) {
+    const int len = nx + ny + nzc;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)nat * len;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(e / len), k = (int)(e - (long long)a * len);
+        int d, kk;
+        if (k < nx) {
+            d = 0;
+            kk = ffreq_(k, nx);
+        } else if (k < nx + ny) {
+            d = 1;
+            kk = ffreq_(k - nx, ny);
+        } else {
+            d = 2;
+            kk = k - nx - ny;
+            if (full) kk = wrapf_(kk, nz);
+        }
+        double sn, cs;
+        sincospi(2.0 * kk * frac[3 * a + d], &sn, &cs);
+        tab[e] = double2{cs, -sn};
+    }
+}
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) { return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+
+__device__ __forceinline__ void half_index(long long i, int ny, int nzc, int &ix, int &iy, int &iz) {
+    iz = (int)(i % nzc);
+    const long long rest = i / nzc;
+    iy = (int)(rest % ny);
+    ix = (int)(rest / ny);
+}
+
+// S(G) = sum_a w_a exp(-i G.R_a) on the half grid
+__global__ void k_strf_w(long long nc, int nx, int ny, int nzc, int nat, const double2 *__restrict__ tab,
+                         const double *__restrict__ w, double2 *__restrict__ S) {
+    const int len = nx + ny + nzc;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, ny, nzc, ix, iy, iz);
+        double re = 0.0, im = 0.0;
+        for (int a = 0; a < nat; ++a) {
+            const double wa = w[a];
+            if (wa == 0.0) continue;
+            const double2 *t = tab + (long long)a * len;
+            const double2 ph = cmul2(cmul2(t[ix], t[nx + iy]), t[nx + ny + iz]);
+            re += wa * ph.x;
+            im += wa * ph.y;
+        }
+        S[i] = double2{re, im};
+    }
+}
+
+// d vloc / dq of the cubic table (no small-q patch: the spline's own
+// derivative, extrapolated below q0), zero beyond qmax
+__device__ __forceinline__ double vloc_deriv(const VlocTable &t, int s, double q) {
+    if (!(q < t.qmax[s])) return 0.0;
+    int i = (int)floor((q - t.q0[s]) / t.dq[s]);
+    if (i < 0) i = 0;
+    if (i >= t.ncoef[s]) i = t.ncoef[s] - 1;
+    const double x = q - (t.q0[s] + i * t.dq[s]);
+    const double *c = t.coef[s] + 4 * (long long)i;
+    return c[1] + x * (2.0 * c[2] + x * 3.0 * c[3]);
+}
+
+// X(G) = cp vloc_s(|G|) rho(G) + ce exp(-G^2/(4 eta)) / G^2 S_Z(G)  (G != 0)
+__global__ void k_force_x(long long nc, GGeom g, VlocTable t, int s, double cp, const double2 *__restrict__ rhoG,
+                          double ce, double eta, const double2 *__restrict__ SZ, double2 *__restrict__ X) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, g.ny, g.nzc, ix, iy, iz);
+        int f[3], fp[3];
+        freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        const double gg = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        double2 x{0.0, 0.0};
+        if (rhoG) {
+            const double v = cp * vloc_eval(t, s, sqrt(gg));
+            x.x += v * rhoG[i].x;
+            x.y += v * rhoG[i].y;
+        }
+        if (SZ && gg > 0.0) {
+            const double w = ce * exp(-gg / (4.0 * eta)) / gg;
+            x.x += w * SZ[i].x;
+            x.y += w * SZ[i].y;
+        }
+        X[i] = x;
+    }
+}
+
+// partials[(a * gridDim.x + blk) * 3 + d] = sum over masked G of G_d Im(exp(iG.R_a) X(G))
+// for atom alist[blockIdx.y]
+__global__ void k_force_sum(long long nc, GGeom g, const double2 *__restrict__ tab, const double2 *__restrict__ X,
+                            const int *__restrict__ alist, double *__restrict__ partials) {
+    __shared__ double sm[16];
+    const int a = alist[blockIdx.y];
+    const int len = g.nx + g.ny + g.nzc;
+    const double2 *t = tab + (long long)a * len;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, g.ny, g.nzc, ix, iy, iz);
+        if (!gmask(ix, iy, iz, g.nx, g.ny, g.nz, g.full)) continue;
+        int f[3], fp[3];
+        freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        const double2 ph = cmul2(cmul2(t[ix], t[g.nx + iy]), t[g.nx + g.ny + iz]);  // exp(-iG.R)
+        const double2 x = X[i];
+        const double im = ph.x * x.y - ph.y * x.x;  // Im(conj(ph) x)
+        acc[0] += G[0] * im;
+        acc[1] += G[1] * im;
+        acc[2] += G[2] * im;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double v = block_sum(acc[d], sm);
+        if (threadIdx.x == 0) partials[((long long)blockIdx.y * gridDim.x + blockIdx.x) * 3 + d] = v;
+    }
+}
+
+// eta dG/deta of the inverse Lindhard function (kernel.py:197-211)
+__device__ __forceinline__ double lindhard_deriv(double eta, double mu) {
+    const double atol = 1.0e-10;
+    double d;
+    if (eta < atol) {
+        d = -2.0 * eta * (1.0 / 3.0 - 3.0 * mu);
+    } else if (fabs(eta - 1.0) < atol) {
+        d = -48.0;
+    } else {
+        const double A = log(fabs((1.0 + eta) / (1.0 - eta)));
+        const double den = 0.5 + 0.25 * (1.0 - eta * eta) / eta * A;
+        d = (0.5 / eta - 0.25 * (eta * eta + 1.0) / (eta * eta) * A) / (den * den) + 6.0 * eta * mu;
+    }
+    return d * eta;
+}
+
+struct StressGIn {
+    const double2 *rhoG;  // Hartree and PP (null: neither)
+    const double2 *rA, *rB;  // FFT(rho^alpha), FFT(rho^beta) (null: no WT)
+    double itkf, mu;
+    const double2 *Ss;  // nsp species structure factors, stride nc (null: no PP)
+    int nsp;
+    VlocTable t;
+    const double2 *SZ;  // charge structure factor (null: no Ewald)
+    double eta;
+};
+
+// per masked G != 0, the six g_i g_j sums of: Hartree |rho|^2/G^4,
+// PP Re(rho v')/|G| (v' = sum_s dvloc_s/dq conj(S_s)), WT (g_i g_j/G^2 -
+// delta/3) eta G'(eta) Re(rA conj(rB)), Ewald |S_Z|^2 exp(-G^2/4eta)/G^2
+// (2 g_i g_j (1 + G^2/4eta)/G^2 - delta); partials [block][24]
+__global__ void k_stress_g(long long nc, GGeom g, StressGIn in, double *__restrict__ partials) {
+    __shared__ double sm[16];
+    double acc[24];
+#pragma unroll
+    for (int k = 0; k < 24; ++k) acc[k] = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, g.ny, g.nzc, ix, iy, iz);
+        if (!gmask(ix, iy, iz, g.nx, g.ny, g.nz, g.full)) continue;
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        int f[3], fp[3];
+        freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        const double gg = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        const double q = sqrt(gg), ig = 1.0 / gg;
+        double h = 0.0, pp = 0.0, wt = 0.0, ew = 0.0, ewd = 0.0;
+        if (in.rhoG) {
+            const double2 r = in.rhoG[i];
+            h = (r.x * r.x + r.y * r.y) * ig * ig;
+            if (in.Ss) {
+                double vx = 0.0, vy = 0.0;
+                for (int s = 0; s < in.nsp; ++s) {
+                    const double d = vloc_deriv(in.t, s, q);
+                    const double2 S = in.Ss[(long long)s * nc + i];
+                    vx += d * S.x;
+                    vy -= d * S.y;
+                }
+                pp = (r.x * vx - r.y * vy) / q;
+            }
+        }
+        if (in.rA) {
+            const double2 a = in.rA[i], b = in.rB[i];
+            wt = lindhard_deriv(q * in.itkf, in.mu) * (a.x * b.x + a.y * b.y);
+        }
+        if (in.SZ) {
+            const double2 S = in.SZ[i];
+            const double e = (S.x * S.x + S.y * S.y) * exp(-gg / (4.0 * in.eta)) * ig;
+            ew = e * 2.0 * ig * (1.0 + gg / (4.0 * in.eta));
+            ewd = e;
+        }
+        int k = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b, ++k) {
+                const double gab = G[a] * G[b];
+                acc[k] += gab * h;
+                acc[6 + k] += gab * pp;
+                acc[12 + k] += (gab * ig - (a == b ? 1.0 / 3.0 : 0.0)) * wt;
+                acc[18 + k] += gab * ew - (a == b ? ewd : 0.0);
+            }
+    }
+    for (int k = 0; k < 24; ++k) {
+        const double v = block_sum(acc[k], sm);
+        if (threadIdx.x == 0) partials[(long long)blockIdx.x * 24 + k] = v;
+    }
+}
+
+// spectra i G_d rho(G), d = 0..2 (dense, stride nc)
+__global__ void k_grad_spectra(long long nc, GGeom g, const double2 *__restrict__ rhoG, double2 *__restrict__ out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, g.ny, g.nzc, ix, iy, iz);
+        int f[3], fp[3];
+        freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        const double2 r = rhoG[i];
+        for (int d = 0; d < 3; ++d) out[(long long)d * nc + i] = double2{-G[d] * r.y, G[d] * r.x};
+    }
+}
+
+// [block][7]: six sums of d_i d_j / rho, and the sum of v_xc rho
+__global__ void k_real_stress(long long n, const double *__restrict__ rho, const double *__restrict__ d,
+                              double *__restrict__ partials) {
+    __shared__ double sm[16];
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double r = rho[i];
+        const double g0 = d[i], g1 = d[n + i], g2 = d[2 * n + i];
+        const double ir = 1.0 / r;
+        acc[0] += g0 * g0 * ir;
+        acc[1] += g0 * g1 * ir;
+        acc[2] += g0 * g2 * ir;
+        acc[3] += g1 * g1 * ir;
+        acc[4] += g1 * g2 * ir;
+        acc[5] += g2 * g2 * ir;
+        double e, v;
+        pz_lda<double>(r < 1e-30 ? 1e-10 : cbrt(r), e, v);
+        acc[6] += v * r;
+    }
+    for (int k = 0; k < 7; ++k) {
+        const double v = block_sum(acc[k], sm);
+        if (threadIdx.x == 0) partials[(long long)blockIdx.x * 7 + k] = v;
+    }
+}
+
+// Ewald real-space forces and stress, one workgroup per atom i over the
+// atoms j and the lattice images within rmax: partials[i][0..2] force,
+// [3..8] the six sums of q_i q_j (2 sqrt(eta/pi) exp(-eta d^2) + erfc(sqrt(eta) d)/d) R_a R_b / d^2
+__global__ void k_ewald_real_fs(int nat, const double *__restrict__ pos, const double *__restrict__ q, GGeom lat,
+                                int N0, int N1, int N2, double rmax, double eta, double *__restrict__ partials) {
+    __shared__ double sm[16];
+    const int i = blockIdx.x;
+    const double se = sqrt(eta);
+    const int n0 = 2 * N0 + 1, n1 = 2 * N1 + 1, n2 = 2 * N2 + 1;
+    const long long nimg = (long long)n0 * n1 * n2;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long e = threadIdx.x; e < nimg * nat; e += blockDim.x) {
+        const int j = (int)(e % nat);
+        const long long im = e / nat;
+        const int c = (int)(im % n2) - N2, b = (int)((im / n2) % n1) - N1, a = (int)(im / ((long long)n1 * n2)) - N0;
+        // R = r_i - (r_j - T), T = a L0 + b L1 + c L2 (lat.B holds the lattice rows here)
+        double R[3];
+        for (int d = 0; d < 3; ++d)
+            R[d] = pos[3 * i + d] - pos[3 * j + d] + a * lat.B[d] + b * lat.B[3 + d] + c * lat.B[6 + d];
+        const double dd = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        if (!(dd < rmax && dd > 0.001)) continue;
+        const double x = se * dd;
+        const double ex = exp(-x * x), ec = erfc(x);
+        const double qq = q[i] * q[j];
+        const double ff = qq * (ec / (x * x * x) + 2.0 / sqrt(M_PI) * ex / (x * x)) * se * se * se;
+        acc[0] += ff * R[0];
+        acc[1] += ff * R[1];
+        acc[2] += ff * R[2];
+        const double ss = qq * (2.0 * se / sqrt(M_PI) * ex + ec / dd) / (dd * dd);
+        int k = 3;
+        for (int u = 0; u < 3; ++u)
+            for (int v = u; v < 3; ++v, ++k) acc[k] += ss * R[u] * R[v];
+    }
+    for (int k = 0; k < 9; ++k) {
+        const double v = block_sum(acc[k], sm);
+        if (threadIdx.x == 0) partials[(long long)i * 9 + k] = v;
+    }
+}
+
+// ------------------------------------------------------------------ host
+static double *fs_workspace(Plan &P, size_t bytes) {
+    if (bytes > P.fs_cap) {
+        if (P.fs_ws) OFDFT_HIP(hipFree(P.fs_ws));
+        P.fs_ws = nullptr;
+        P.fs_cap = 0;
+        OFDFT_HIP(hipMalloc(&P.fs_ws, bytes));
+        P.fs_cap = bytes;
+    }
+    return reinterpret_cast<double *>(P.fs_ws);
+}
+
+// the host inputs of one force / stress call, staged in one upload
+struct IonInputs {
+    VlocTable t;
+    const double *pos = nullptr, *frac = nullptr, *w = nullptr;  // w: nw weight vectors of nat
+    const int *spec = nullptr, *alist = nullptr;                // alist: atom indices grouped by species
+};
+
+static IonInputs stage_ions(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                            const double *const *coef, const double *qmax, const double *vsmall, const double *dqs,
+                            int nat, const double *pos_cart, const double *pos_frac, const int *species,
+                            const std::vector<double> &weights, const std::vector<int> &alist, hipStream_t s) {
+    StageLayout lay;
+    size_t coff[kMaxSpecies] = {};
+    for (int sp = 0; sp < nspecies; ++sp) coff[sp] = lay.add(sizeof(double) * 4 * ncoef[sp]);
+    const size_t opos = lay.add(sizeof(double) * 3 * nat), ofrac = lay.add(sizeof(double) * 3 * nat),
+                 ospec = lay.add(sizeof(int) * nat), ow = lay.add(sizeof(double) * weights.size()),
+                 oal = lay.add(sizeof(int) * alist.size());
+    char *h = stage_host(P, lay.size);
+    for (int sp = 0; sp < nspecies; ++sp) std::memcpy(h + coff[sp], coef[sp], sizeof(double) * 4 * ncoef[sp]);
+    std::memcpy(h + opos, pos_cart, sizeof(double) * 3 * nat);
+    std::memcpy(h + ofrac, pos_frac, sizeof(double) * 3 * nat);
+    if (species) std::memcpy(h + ospec, species, sizeof(int) * nat);
+    if (!weights.empty()) std::memcpy(h + ow, weights.data(), sizeof(double) * weights.size());
+    if (!alist.empty()) std::memcpy(h + oal, alist.data(), sizeof(int) * alist.size());
+    char *d = stage_upload(P, lay.size, s);
+    IonInputs in;
+    std::memset(&in.t, 0, sizeof(in.t));
+    for (int sp = 0; sp < nspecies; ++sp) {
+        in.t.coef[sp] = reinterpret_cast<const double *>(d + coff[sp]);
+        in.t.q0[sp] = q0[sp];
+        in.t.dq[sp] = dq[sp];
+        in.t.qmax[sp] = qmax[sp];
+        in.t.ncoef[sp] = ncoef[sp];
+        in.t.v0[sp] = vsmall[3 * sp];
+        in.t.v1[sp] = vsmall[3 * sp + 1];
+        in.t.v2[sp] = vsmall[3 * sp + 2];
+        in.t.dqs[sp] = dqs[sp];
+    }
+    in.pos = reinterpret_cast<const double *>(d + opos);
+    in.frac = reinterpret_cast<const double *>(d + ofrac);
+    in.spec = reinterpret_cast<const int *>(d + ospec);
+    in.w = reinterpret_cast<const double *>(d + ow);
+    in.alist = reinterpret_cast<const int *>(d + oal);
+    return in;
+}
+
+static void phase_tables(Plan &P, int nat, const double *frac, double2 *tab, hipStream_t s) {
+    const long long n = (long long)nat * (P.nr[0] + P.nr[1] + P.nzc);
+    hipLaunchKernelGGL(k_phase_tables, dim3(nblk(n)), dim3(256), 0, s, nat, frac, P.nr[0], P.nr[1], P.nr[2], P.nzc,
+                       P.full, tab);
+    OFDFT_HIP(hipGetLastError());
+}
+
+// Ewald parameters of the reference (ewald.py:256-279 eta, 280-320 cut-off)
+struct EwaldParams {
+    double eta, rmax, gmax;
+    int nimg[3];
+};
+
+static EwaldParams ewald_params(const Plan &P, int nat, const double *charges, double precision) {
+    double lengths[3], sp_max = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        lengths[d] = std::sqrt(P.lat[3 * d] * P.lat[3 * d] + P.lat[3 * d + 1] * P.lat[3 * d + 1] +
+                               P.lat[3 * d + 2] * P.lat[3 * d + 2]);
+        const double sp1 = lengths[d] / P.nr[d], sp2 = lengths[d] / (P.nr[d] - 1);
+        sp_max = std::max(sp_max, 0.5 * (sp1 + sp2));
+    }
+    EwaldParams e;
+    e.gmax = std::sqrt(2.0 * (M_PI * M_PI / (2.0 * sp_max * sp_max)));
+    double q2 = 0.0;
+    for (int a = 0; a < nat; ++a) q2 += charges[a] * charges[a];
+    e.eta = 1.6;
+    while (true) {
+        const double ub =
+            4.0 * M_PI * nat * q2 * std::sqrt(e.eta / M_PI) * std::erfc(e.gmax / 2.0 * std::sqrt(1.0 / e.eta));
+        if (ub < precision) break;
+        e.eta -= 0.01;
+        OFDFT_REQUIRE(e.eta > 0.0, "ewald: no eta satisfies the precision");
+    }
+    e.rmax = erfcinv_host(precision / 3.0) / std::sqrt(e.eta);
+    for (int d = 0; d < 3; ++d) e.nimg[d] = (int)std::ceil(e.rmax / lengths[d]);
+    return e;
+}
+
+// sum the force partials [na][nb][3] of k_force_sum into F[alist[k]]
+static void reduce_forces(const double *dpart, int na, int nb, const std::vector<int> &alist, double scale, double *F,
+                          hipStream_t s) {
+    std::vector<double> h((size_t)na * nb * 3);
+    OFDFT_HIP(hipMemcpyAsync(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OFDFT_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < na; ++k)
+        for (int d = 0; d < 3; ++d) {
+            double acc = 0.0;
+            for (int b = 0; b < nb; ++b) acc += h[((size_t)k * nb + b) * 3 + d];
+            F[3 * alist[k] + d] += scale * acc;
+        }
+}
+
+static std::vector<double> reduce_cols(const double *dpart, int nb, int K, hipStream_t s) {
+    std::vector<double> h((size_t)nb * K), out(K, 0.0);
+    OFDFT_HIP(hipMemcpyAsync(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OFDFT_HIP(hipStreamSynchronize(s));
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < K; ++k) out[k] += h[(size_t)b * K + k];
+    return out;
+}
+
+static void sym_add(double *S, const double *v6, double scale) {
+    int k = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b, ++k) {
+            S[3 * a + b] += scale * v6[k];
+            if (a != b) S[3 * b + a] += scale * v6[k];
+        }
+}
+
+constexpr int kForceChunks = 64;
+
+// F (host, nat x 3) += local-PP forces (pseudo/__init__.py:456-470)
+void pp_forces(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+               const double *pos_cart, const double *pos_frac, const int *species, double *F, hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "forces need an FP64 plan");
+    std::vector<int> alist;
+    std::vector<int> cnt(nspecies, 0);
+    for (int sp = 0; sp < nspecies; ++sp)
+        for (int a = 0; a < nat; ++a)
+            if (species[a] == sp) {
+                alist.push_back(a);
+                ++cnt[sp];
+            }
+    IonInputs in = stage_ions(P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species,
+                              {}, alist, s);
+    const long long nc = P.Nc;
+    const int len = P.nr[0] + P.nr[1] + P.nzc;
+    double *ws = fs_workspace(P, sizeof(double) * (2 * nc * 2 + (size_t)nat * len * 2 + (size_t)nat * kForceChunks * 3));
+    double2 *rhoG = reinterpret_cast<double2 *>(ws);
+    double2 *X = rhoG + nc;
+    double2 *tab = X + nc;
+    double *part = reinterpret_cast<double *>(tab + (size_t)nat * len);
+    fft_r2c<double>(P, rho, rhoG, 1, P.N, nc, P.dV, nullptr, nullptr, s);
+    phase_tables(P, nat, in.frac, tab, s);
+    const GGeom g = geom_of(P);
+    int off = 0;
+    for (int sp = 0; sp < nspecies; ++sp) {
+        if (cnt[sp] == 0) continue;
+        hipLaunchKernelGGL(k_force_x, dim3(nblk(nc)), dim3(256), 0, s, nc, g, in.t, sp, 2.0 / P.volume, rhoG, 0.0,
+                           1.0, (const double2 *)nullptr, X);
+        OFDFT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_force_sum, dim3(kForceChunks, cnt[sp]), dim3(256), 0, s, nc, g, tab, X,
+                           in.alist + off, part + (size_t)off * kForceChunks * 3);
+        OFDFT_HIP(hipGetLastError());
+        off += cnt[sp];
+    }
+    std::vector<int> order(alist.begin(), alist.end());
+    reduce_forces(part, nat, kForceChunks, order, 1.0, F, s);
+}
+
+// S (host, 3 x 3) = local-PP stress (pseudo/__init__.py:414-454)
+void pp_stress(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+               const double *pos_cart, const double *pos_frac, const int *species, double energy, double *S,
+               hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "stress needs an FP64 plan");
+    std::vector<double> w((size_t)nspecies * nat, 0.0);
+    for (int a = 0; a < nat; ++a) w[(size_t)species[a] * nat + a] = 1.0;
+    IonInputs in = stage_ions(P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species, w,
+                              {}, s);
+    const long long nc = P.Nc;
+    const int len = P.nr[0] + P.nr[1] + P.nzc;
+    const int nb = 1024;
+    double *ws = fs_workspace(P, sizeof(double) * ((1 + nspecies) * nc * 2 + (size_t)nat * len * 2 + nb * 24));
+    double2 *rhoG = reinterpret_cast<double2 *>(ws);
+    double2 *Ss = rhoG + nc;
+    double2 *tab = Ss + (size_t)nspecies * nc;
+    double *part = reinterpret_cast<double *>(tab + (size_t)nat * len);
+    fft_r2c<double>(P, rho, rhoG, 1, P.N, nc, P.dV, nullptr, nullptr, s);
+    phase_tables(P, nat, in.frac, tab, s);
+    for (int sp = 0; sp < nspecies; ++sp) {
+        hipLaunchKernelGGL(k_strf_w, dim3(nblk(nc)), dim3(256), 0, s, nc, P.nr[0], P.nr[1], P.nzc, nat, tab,
+                           in.w + (size_t)sp * nat, Ss + (size_t)sp * nc);
+        OFDFT_HIP(hipGetLastError());
+    }
+    StressGIn si;
+    std::memset(&si, 0, sizeof(si));
+    si.rhoG = rhoG;
+    si.Ss = Ss;
+    si.nsp = nspecies;
+    si.t = in.t;
+    hipLaunchKernelGGL(k_stress_g, dim3(nb), dim3(256), 0, s, nc, geom_of(P), si, part);
+    OFDFT_HIP(hipGetLastError());
+    const std::vector<double> v = reduce_cols(part, nb, 24, s);
+    for (int k = 0; k < 9; ++k) S[k] = 0.0;
+    sym_add(S, v.data() + 6, -2.0 / (P.volume * P.volume));
+    for (int d = 0; d < 3; ++d) S[4 * d] -= energy / P.volume;
+}
+
+// Ewald forces (F, nat x 3, optional) and stress (S, 3 x 3, optional), real
+// space over images plus exact reciprocal sums (ewald.py:541-760)
+void ewald_fs(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges,
+              double precision, double *F, double *S, hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "Ewald forces need an FP64 plan");
+    const EwaldParams ep = ewald_params(P, nat, charges, precision);
+    std::vector<double> w(charges, charges + nat);
+    std::vector<int> alist(nat);
+    for (int a = 0; a < nat; ++a) alist[a] = a;
+    IonInputs in = stage_ions(P, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nat, pos_cart,
+                              pos_frac, nullptr, w, alist, s);
+    const long long nc = P.Nc;
+    const int len = P.nr[0] + P.nr[1] + P.nzc;
+    const int nb = 1024;
+    double *ws = fs_workspace(P, sizeof(double) * (2 * nc * 2 + (size_t)nat * len * 2 +
+                                                   std::max((size_t)nat * kForceChunks * 3, (size_t)nb * 24) +
+                                                   (size_t)nat * 9));
+    double2 *SZ = reinterpret_cast<double2 *>(ws);
+    double2 *X = SZ + nc;
+    double2 *tab = X + nc;
+    double *part = reinterpret_cast<double *>(tab + (size_t)nat * len);
+    double *rpart = part + std::max((size_t)nat * kForceChunks * 3, (size_t)nb * 24);
+    phase_tables(P, nat, in.frac, tab, s);
+    hipLaunchKernelGGL(k_strf_w, dim3(nblk(nc)), dim3(256), 0, s, nc, P.nr[0], P.nr[1], P.nzc, nat, tab, in.w, SZ);
+    OFDFT_HIP(hipGetLastError());
+    const GGeom g = geom_of(P);
+    // real space
+    GGeom lat = g;
+    for (int k = 0; k < 9; ++k) lat.B[k] = P.lat[k];
+    hipLaunchKernelGGL(k_ewald_real_fs, dim3(nat), dim3(256), 0, s, nat, in.pos, in.w, lat, ep.nimg[0], ep.nimg[1],
+                       ep.nimg[2], ep.rmax, ep.eta, rpart);
+    OFDFT_HIP(hipGetLastError());
+    std::vector<double> hr((size_t)nat * 9);
+    OFDFT_HIP(hipMemcpyAsync(hr.data(), rpart, hr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OFDFT_HIP(hipStreamSynchronize(s));
+    if (F) {
+        for (int a = 0; a < nat; ++a)
+            for (int d = 0; d < 3; ++d) F[3 * a + d] += hr[(size_t)a * 9 + d];
+        // reciprocal: F_a = (8 pi / V) Z_a sum_mask G exp(-G^2/4eta)/G^2 Im(exp(iG.R_a) S_Z(G))
+        hipLaunchKernelGGL(k_force_x, dim3(nblk(nc)), dim3(256), 0, s, nc, g, VlocTable{}, 0, 0.0,
+                           (const double2 *)nullptr, 8.0 * M_PI / P.volume, ep.eta, SZ, X);
+        OFDFT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_force_sum, dim3(kForceChunks, nat), dim3(256), 0, s, nc, g, tab, X, in.alist, part);
+        OFDFT_HIP(hipGetLastError());
+        std::vector<double> Fr((size_t)nat * 3, 0.0);
+        reduce_forces(part, nat, kForceChunks, alist, 1.0, Fr.data(), s);
+        for (int a = 0; a < nat; ++a)
+            for (int d = 0; d < 3; ++d) F[3 * a + d] += charges[a] * Fr[3 * a + d];
+    }
+    if (S) {
+        for (int k = 0; k < 9; ++k) S[k] = 0.0;
+        std::vector<double> r6(6, 0.0);
+        for (int a = 0; a < nat; ++a)
+            for (int k = 0; k < 6; ++k) r6[k] += hr[(size_t)a * 9 + 3 + k];
+        sym_add(S, r6.data(), -0.5 / P.volume);
+        StressGIn si;
+        std::memset(&si, 0, sizeof(si));
+        si.SZ = SZ;
+        si.eta = ep.eta;
+        hipLaunchKernelGGL(k_stress_g, dim3(nb), dim3(256), 0, s, nc, g, si, part);
+        OFDFT_HIP(hipGetLastError());
+        const std::vector<double> v = reduce_cols(part, nb, 24, s);
+        sym_add(S, v.data() + 18, 4.0 * M_PI / (P.volume * P.volume));
+        double qsum = 0.0;
+        for (int a = 0; a < nat; ++a) qsum += charges[a];
+        for (int d = 0; d < 3; ++d) S[4 * d] += qsum * qsum * 4.0 * M_PI * (1.0 / (4.0 * ep.eta * P.volume * P.volume) / 2.0);
+    }
+}
+
+// stress of TF, vW, WT (nonlocal part), PZ LDA and Hartree at rho, out [5][9]
+// in that order; energies are the evaluation's (OFDFT_E_* order)
+void functional_stress(Plan &P, const double *rho, const double *energies, double *out, hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "stress needs an FP64 plan");
+    const long long nc = P.Nc, n = P.N;
+    const int nb = 1024;
+    for (int k = 0; k < 45; ++k) out[k] = 0.0;
+    double *ws = fs_workspace(P, sizeof(double) * (3 * nc * 2 + (size_t)nb * 24));
+    double2 *rhoG = reinterpret_cast<double2 *>(ws);
+    double2 *rA = rhoG + nc, *rB = rA + nc;
+    double *part = reinterpret_cast<double *>(rB + nc);
+    fft_r2c<double>(P, rho, rhoG, 1, n, nc, P.dV, nullptr, nullptr, s);
+    const GGeom g = geom_of(P);
+    const double V = P.volume;
+    // rho0 = mean(rho) for the WT kernel (wt.py:79)
+    double rho0 = 0.0;
+    {
+        double2 h0;
+        OFDFT_HIP(hipMemcpyAsync(&h0, rhoG, sizeof(h0), hipMemcpyDeviceToHost, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+        rho0 = h0.x / V;
+    }
+    const bool wt = (P.terms & OFDFT_TERM_WT) != 0;
+    StressGIn si;
+    std::memset(&si, 0, sizeof(si));
+    si.rhoG = (P.terms & OFDFT_TERM_HARTREE) ? rhoG : nullptr;
+    const double tkf = 2.0 * std::cbrt(3.0 * rho0 * M_PI * M_PI);
+    if (wt) {
+        Prologue pa{}, pb{};
+        pa.code[0] = OP_POW_BETA;
+        pa.beta = P.alpha;
+        pa.b56 = beta_is_5_6(P.alpha);
+        fft_r2c<double>(P, rho, rA, 1, n, nc, P.dV, &pa, nullptr, s);
+        si.rA = rA;
+        si.rB = rA;
+        if (std::fabs(P.alpha - P.beta) > 1e-14) {
+            pb.code[0] = OP_POW_BETA;
+            pb.beta = P.beta;
+            pb.b56 = beta_is_5_6(P.beta);
+            fft_r2c<double>(P, rho, rB, 1, n, nc, P.dV, &pb, nullptr, s);
+            si.rB = rB;
+        }
+        si.itkf = 1.0 / tkf;
+        si.mu = P.y;
+    }
+    if (si.rhoG || si.rA) {
+        hipLaunchKernelGGL(k_stress_g, dim3(nb), dim3(256), 0, s, nc, g, si, part);
+        OFDFT_HIP(hipGetLastError());
+        const std::vector<double> v = reduce_cols(part, nb, 24, s);
+        if (si.rhoG) {
+            sym_add(out + 36, v.data(), 8.0 * M_PI / (V * V));
+            for (int d = 0; d < 3; ++d) out[36 + 4 * d] -= energies[OFDFT_E_H] / V;
+        }
+        if (wt) {
+            const double a = P.alpha, b = P.beta;
+            sym_add(out + 18, v.data() + 12, M_PI * M_PI / (a * b * std::pow(rho0, a + b - 2) * tkf / 2.0) / (V * V));
+            for (int d = 0; d < 3; ++d) out[18 + 4 * d] -= 2.0 / 3.0 * energies[OFDFT_E_NL] / V;
+        }
+    }
+    if (P.terms & OFDFT_TERM_TF)
+        for (int d = 0; d < 3; ++d) out[4 * d] = -2.0 / 3.0 * energies[OFDFT_E_TF] / V;
+    if (P.terms & (OFDFT_TERM_VW | OFDFT_TERM_XC)) {
+        // gradients of rho: C2R of i G_d rho(G) into the work fields
+        double2 *gs = reinterpret_cast<double2 *>(P.spec);
+        double *d = reinterpret_cast<double *>(P.work);
+        hipLaunchKernelGGL(k_grad_spectra, dim3(nblk(nc)), dim3(256), 0, s, nc, g, rhoG, gs);
+        OFDFT_HIP(hipGetLastError());
+        fft_c2r<double>(P, gs, gs, d, 3, nc, nc, n, 1.0 / ((double)n * P.dV), s);
+        hipLaunchKernelGGL(k_real_stress, dim3(nb), dim3(256), 0, s, n, rho, d, part);
+        OFDFT_HIP(hipGetLastError());
+        const std::vector<double> v = reduce_cols(part, nb, 7, s);
+        if (P.terms & OFDFT_TERM_VW) sym_add(out + 9, v.data(), -0.25 / V * P.dV * P.y);
+        if (P.terms & OFDFT_TERM_XC)
+            for (int d2 = 0; d2 < 3; ++d2) out[27 + 4 * d2] = (energies[OFDFT_E_XC] - v[6] * P.dV) / V;
+    }
 }
 
 }  // namespace ofdft

@@ -167,6 +167,36 @@ def evaluate(funcs, rho, calcType, name="TOTAL", eion=0.0):
     return out
 
 
+def _fp64(rho):
+    """rho on the FP64 grid of its shape (forces and stress run in FP64)."""
+    if rho.grid.dtype == torch.float64:
+        return rho
+    return DirectField(rho.grid.with_dtype(torch.float64), data=rho.data.double())
+
+
+def functional_stress(funcs, rho):
+    """Stress tensors of the TF, vW, WT-nonlocal, LDA and Hartree terms of
+    ``funcs`` at rho (ofdft_functional_stress; kedf/tf.py:52-62,
+    kedf/vw.py:97-111, kedf/wt.py:75-105, xc/semilocal_xc.py:308-322,
+    hartree.py:67-91).  Returns a dict of 3x3 arrays (zero for absent terms)."""
+    r = _fp64(rho)
+    grid = r.grid
+    plan = grid.plan(torch.float64)
+    terms, vloc = configure(plan, funcs, grid)
+    if terms & _lib.TERM_LMGP:
+        raise NotImplementedError("stress of the LMGP functional is not implemented")
+    src = r.data.contiguous()
+    en = torch.empty(_lib.E_COUNT, dtype=torch.float64, device=src.device)
+    _lib.check(_lib.lib.ofdft_eval(plan.handle, src.data_ptr(), _lib.CALC_E, None, en.data_ptr(), _lib.stream()))
+    e = np.ascontiguousarray(en.cpu().numpy())
+    out = np.zeros(45)
+    _lib.check(_lib.lib.ofdft_functional_stress(plan.handle, src.data_ptr(), _lib.dptr(e), _lib.dptr(out),
+                                                _lib.stream()))
+    del vloc
+    p = out.reshape(5, 3, 3)
+    return dict(TF=p[0].copy(), VW=p[1].copy(), NL=p[2].copy(), XC=p[3].copy(), HARTREE=p[4].copy())
+
+
 class ZeroFunctional(AbstractFunctional):
     name = "ZERO"
 
@@ -216,6 +246,21 @@ class KEDF(AbstractFunctional):
             raise NotImplementedError("split KEDF output is not supported")
         return out
 
+    def stress(self, density, split=False, **kwargs):
+        """Sum of the stresses of the KEDF's parts (kedf/__init__.py:186-226);
+        split: dict keyed TF / VW / WT-NL."""
+        st = functional_stress([self], density)
+        parts = {}
+        if self.terms & _lib.TERM_TF:
+            parts["TF"] = st["TF"]
+        if self.terms & _lib.TERM_VW:
+            parts["VW"] = st["VW"]
+        if self.terms & _lib.TERM_WT:
+            parts["WT-NL"] = st["NL"]
+        if split:
+            return parts
+        return sum(parts.values(), np.zeros((3, 3)))
+
 
 class XC(AbstractFunctional):
     """``XC(xc='LDA')``: the reference's built-in Perdew-Zunger LDA
@@ -236,6 +281,10 @@ class XC(AbstractFunctional):
         self.terms = _lib.TERM_XC
         self.options = kwargs
 
+    def stress(self, density, **kwargs):
+        """LDAStress (xc/semilocal_xc.py:308-322)."""
+        return functional_stress([self], density)["XC"]
+
 
 class Hartree(AbstractFunctional):
     """Hartree functional (functional/hartree.py:11-56)."""
@@ -245,6 +294,10 @@ class Hartree(AbstractFunctional):
         self.name = "HARTREE"
         self.terms = _lib.TERM_HARTREE
         self.options = kwargs
+
+    def stress(self, density, **kwargs):
+        """HartreeFunctionalStress (hartree.py:67-91)."""
+        return functional_stress([self], density)["HARTREE"]
 
 
 from .pseudo import LocalPseudo  # noqa: E402

@@ -98,6 +98,33 @@ class Ewald:
         self.energy
         return self._parts["rec"]
 
+    def _forces_stress(self):
+        plan = self.grid.plan(torch.float64)
+        pos = np.ascontiguousarray(self.ions.positions, dtype=np.float64)
+        frac = np.ascontiguousarray(self.ions.get_scaled_positions(), dtype=np.float64)
+        q = np.ascontiguousarray(self.ions.charges, dtype=np.float64)
+        f = np.zeros((self.ions.nat, 3))
+        st = np.zeros((3, 3))
+        _lib.check(_lib.lib.ofdft_ewald_forces_stress(plan.handle, self.ions.nat, _lib.dptr(pos), _lib.dptr(frac),
+                                                      _lib.dptr(q), float(self.precision), _lib.dptr(f),
+                                                      _lib.dptr(st), _lib.stream()))
+        self._fs = (f, st)
+
+    @property
+    def forces(self):
+        """Ewald forces (ewald.py:510-521): real space plus the exact
+        reciprocal sum, nat x 3 in Ha/bohr."""
+        if getattr(self, "_fs", None) is None:
+            self._forces_stress()
+        return self._fs[0].copy()
+
+    @property
+    def stress(self):
+        """Ewald stress (ewald.py:523-539), 3 x 3 in Ha/bohr^3."""
+        if getattr(self, "_fs", None) is None:
+            self._forces_stress()
+        return self._fs[1].copy()
+
 
 class LocalPseudo(AbstractFunctional):
     """``LocalPseudo(grid, ions, PP_list, PME=True)`` (pseudo/__init__.py:80-246)."""
@@ -156,9 +183,9 @@ class LocalPseudo(AbstractFunctional):
                                BsplineOrder=self.BsplineOrder)
         return self.ewald
 
-    def _compute_vreal64(self):
-        grid = self.grid
-        plan = grid.plan(torch.float64)
+    def _table_args(self):
+        """(keep-alive list, ctypes arguments) of the radial tables and the
+        ions, in the order of ofdft_local_pp's nspecies .. species."""
         keys = sorted(self.readpp.tables)
         tabs = [self.readpp.tables[k] for k in keys]
         ns = len(keys)
@@ -174,13 +201,54 @@ class LocalPseudo(AbstractFunctional):
         pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
         frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
         spec = np.array([keys.index(s) for s in ions.symbols], dtype=np.int32)
+        keep = [q0, dq, nco, qmax, vsmall, dqs, coefs, cptr, pos, frac, spec]
+        args = (ns, _lib.dptr(q0), _lib.dptr(dq), _lib.iptr(nco), cptr, _lib.dptr(qmax), _lib.dptr(vsmall),
+                _lib.dptr(dqs), ions.nat, _lib.dptr(pos), _lib.dptr(frac), _lib.iptr(spec))
+        return keep, args
+
+    def _compute_vreal64(self):
+        grid = self.grid
+        plan = grid.plan(torch.float64)
+        keep, args = self._table_args()
         nr = tuple(int(n) for n in grid.nrR)
         out = torch.empty(nr, dtype=torch.float64, device=grid.device)
-        _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, _lib.dptr(q0), _lib.dptr(dq), _lib.iptr(nco), cptr,
-                                           _lib.dptr(qmax), _lib.dptr(vsmall), _lib.dptr(dqs), ions.nat,
-                                           _lib.dptr(pos), _lib.dptr(frac), _lib.iptr(spec), int(bool(self.PME)),
-                                           int(self.BsplineOrder), out.data_ptr(), _lib.stream()))
+        _lib.check(_lib.lib.ofdft_local_pp(plan.handle, *args, int(bool(self.PME)), int(self.BsplineOrder),
+                                           out.data_ptr(), _lib.stream()))
+        del keep
         return out
+
+    def forces(self, rho):
+        """LocalPseudo.forces (pseudo/__init__.py:260-270,456-470): nat x 3
+        in Ha/bohr, exact structure factors (the reference's PME form agrees
+        with them to 1e-4, test_pme.py:34-36)."""
+        from . import _fp64
+        if not isinstance(rho, DirectField):
+            raise TypeError("rho must be DirectField")
+        r = _fp64(rho)
+        plan = r.grid.plan(torch.float64)
+        keep, args = self._table_args()
+        f = np.zeros((self.ions.nat, 3))
+        _lib.check(_lib.lib.ofdft_local_pp_forces(plan.handle, r.data.contiguous().data_ptr(), *args, _lib.dptr(f),
+                                                  _lib.stream()))
+        del keep
+        return f
+
+    def stress(self, rho, energy=None):
+        """LocalPseudo.stress (pseudo/__init__.py:248-258,414-454), 3 x 3 in
+        Ha/bohr^3; energy: the PP energy at rho (computed if None)."""
+        from . import _fp64
+        if not isinstance(rho, DirectField):
+            raise TypeError("rho must be DirectField")
+        r = _fp64(rho)
+        if energy is None:
+            energy = float((self.vreal_for(r.grid).data * r.data).sum().item() * r.grid.dV)
+        plan = r.grid.plan(torch.float64)
+        keep, args = self._table_args()
+        st = np.zeros((3, 3))
+        _lib.check(_lib.lib.ofdft_local_pp_stress(plan.handle, r.data.contiguous().data_ptr(), *args, float(energy),
+                                                  _lib.dptr(st), _lib.stream()))
+        del keep
+        return st
 
     def vreal_for(self, grid):
         """vreal as a DirectField in the precision of ``grid``."""

@@ -1,6 +1,8 @@
 """TotalFunctional (functional/total_functional.py:5-146 of the reference):
 sum of member functionals plus the Ewald ion-ion energy of the PSEUDO
 member, evaluated in one fused HIP call."""
+import numpy as np
+
 from . import AbstractFunctional, evaluate
 
 
@@ -65,3 +67,44 @@ class TotalFunctional(AbstractFunctional):
 
     def Energy(self, rho, **kwargs):
         return self.compute(rho, calcType={"E"}).energy
+
+    def get_forces(self, rho, ions=None, split=False, **kwargs):
+        """Forces of the members that have them (the local PP) plus the Ewald
+        forces (total_functional.py:148-174), nat x 3 in Ha/bohr; split: dict
+        by member key with 'II' and 'TOTAL'."""
+        ew = self.ewald
+        if ions is None:
+            if ew is None:
+                raise ValueError("Please provide ions for forces")
+            ions = ew.ions
+        forces = {"TOTAL": np.zeros((ions.nat, 3))}
+        for key, func in self.funcDict.items():
+            if hasattr(func, "forces"):
+                f = func.forces(rho)
+                forces[key] = f
+                forces["TOTAL"] += f
+        if ew is not None:
+            forces["II"] = ew.forces
+            forces["TOTAL"] += forces["II"]
+        return forces if split else forces["TOTAL"]
+
+    def get_stress(self, rho, split=False, **kwargs):
+        """Stress tensor (total_functional.py:176-209): KEDF parts as
+        'KEDF-TF', 'KEDF-VW', 'KEDF-NL', other members by type, Ewald as 'II'."""
+        stress = {"TOTAL": np.zeros((3, 3))}
+        for key, func in self.funcDict.items():
+            if func.type == "KEDF":
+                for k2, v in func.stress(rho, split=True).items():
+                    stress["TOTAL"] += v
+                    k = "KEDF-" + k2.split("-")[-1]
+                    stress[k] = stress.get(k, 0.0) + v
+            elif hasattr(func, "stress"):
+                v = func.stress(rho)
+                stress["TOTAL"] += v
+                stress[func.type] = v
+        ew = self.ewald
+        if ew is not None:
+            stress["II"] = ew.stress
+            stress["TOTAL"] += stress["II"]
+        return stress if split else stress["TOTAL"]
+

@@ -170,6 +170,36 @@ int ofdft_ewald(ofdft_plan *plan, int nat, const double *pos_cart, const double 
                 const double *charges, int pme, int order, double precision, double *out,
                 void *stream);
 
+/* Forces and stress (FP64 plans; host outputs; synchronise stream).  G-space
+ * sums run over the half-space mask with exact structure factors (the
+ * reference's PME forms agree with its exact forms to 1e-4, test_pme.py).
+ *
+ * ofdft_local_pp_forces: LocalPseudo.forces (pseudo/__init__.py:456-470),
+ *   F_a = (2/V) sum_mask G vloc_s(|G|) Im(rho(G) exp(iG.R_a)); table arguments
+ *   as ofdft_local_pp.  forces: nat x 3.
+ * ofdft_local_pp_stress: LocalPseudo.stress (pseudo/__init__.py:414-454) with
+ *   the spline derivative of vloc; energy = the PP energy at rho.  stress: 3 x 3.
+ * ofdft_ewald_forces_stress: ewald.forces / ewald.stress (ewald.py:510-539):
+ *   real-space sums over images (Forces_real 541-578, Stress_real 609-662)
+ *   plus Forces_rec (580-607) / Stress_rec (719-760) with the G = 0 term;
+ *   forces (nat x 3) and stress (3 x 3) each optional.
+ * ofdft_functional_stress: out (5 x 9) = stress of TF (kedf/tf.py:52-62), vW
+ *   (kedf/vw.py:97-111), WT nonlocal (kedf/wt.py:75-105), PZ LDA
+ *   (xc/semilocal_xc.py:308-322) and Hartree (hartree.py:67-91) for the plan's
+ *   terms (zero for absent ones); energies (host, OFDFT_E_COUNT) are those of
+ *   an evaluation at rho. */
+int ofdft_local_pp_forces(ofdft_plan *plan, const void *rho, int nspecies, const double *q0, const double *dq,
+                          const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                          const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                          const int *species, double *forces, void *stream);
+int ofdft_local_pp_stress(ofdft_plan *plan, const void *rho, int nspecies, const double *q0, const double *dq,
+                          const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                          const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                          const int *species, double energy, double *stress, void *stream);
+int ofdft_ewald_forces_stress(ofdft_plan *plan, int nat, const double *pos_cart, const double *pos_frac,
+                              const double *charges, double precision, double *forces, double *stress, void *stream);
+int ofdft_functional_stress(ofdft_plan *plan, const void *rho, const double *energies, double *out, void *stream);
+
 /* Preconditioner of the TN inner conjugate-gradient solve (no counterpart in
  * the reference, whose get_direction_TN, optimization.py:91-147, runs plain
  * CG).  table (host, n doubles): M^-1(q) at q = i dq, linearly interpolated

@@ -1,0 +1,102 @@
+"""Forces and stress on the GPU against the oracle (oracle/forces.py, itself
+pinned by the Ewald goldens and by central differences of its energies) and
+against the reference's Ewald goldens."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import data
+from oracle import forces as FO
+from oracle import functionals as F
+from oracle import grid as G
+from oracle import io as OIO
+from oracle import pseudo as OP
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(nr=(24, 24, 24), seed=3):
+    """Displaced 4-atom fcc Al in a sheared cell and a smooth positive density
+    (the oracle's finite-difference test system)."""
+    from dftpy_amd import DirectField, DirectGrid, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    from dftpy_amd.ions import Ions
+    rng = np.random.default_rng(seed)
+    a = 7.6
+    lat = np.array([[a, 0.0, 0.0], [0.3, a, 0.0], [0.0, -0.2, a]])
+    frac = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5]]) + rng.normal(0, 0.02, (4, 3))
+    ions = Ions(["Al"] * 4, frac @ lat, lat)
+    grid = DirectGrid(lat, nr, full=False)
+    x = np.indices(nr).reshape(3, -1).T / np.array(nr)
+    rho_np = (12.0 / grid.volume * (1.0 + 0.3 * np.cos(2 * np.pi * (x @ np.array([1, 0, 1]))) +
+                                    0.2 * np.sin(2 * np.pi * (x @ np.array([0, 2, 1]))))).reshape(nr)
+    rho = DirectField(grid, data=torch.as_tensor(rho_np, device="cuda"))
+    pp = Functional(type="PSEUDO", grid=grid, ions=ions, PP_list={"Al": systems.empty_core_recpot()}, PME=False)
+    ev = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                         H=Functional(type="HARTREE"), PSEUDO=pp)
+    og = G.Grid(lat, nr)
+    oat = OIO.Atoms(ions.symbols, ions.positions, lat)
+    gp, vp, z = OIO.read_recpot(systems.empty_core_recpot())
+    oat.set_charges({"Al": z})
+    return ions, grid, rho, rho_np, pp, ev, og, oat, {"Al": (gp, vp)}
+
+
+def _close(a, b, rtol, atol=0.0):
+    a, b = np.asarray(a), np.asarray(b)
+    assert np.max(np.abs(a - b)) <= rtol * np.max(np.abs(b)) + atol, (a, b)
+
+
+def test_ewald_goldens_gpu():
+    # examples/test/test_ewald.py:21-52 (forces and stress of the exact sums)
+    from dftpy_amd import DirectGrid, formats
+    from dftpy_amd.functional.pseudo import Ewald
+    ions, rho, lat = formats.read_qepp(data("Al_fde_rho.pp"))
+    ions.set_charges({"Al": 3.0})
+    ew = Ewald(ions, DirectGrid(lat, rho.shape, full=True), PME=False)
+    f, s = ew.forces, ew.stress
+    assert np.isclose(f[0, 0], 0.0, atol=1e-5)
+    # Stress_real[0, 0] = 0 and Stress_rec[0, 0] = 8.04967853e-03
+    assert np.isclose(s[0, 0], 8.04967853e-03, atol=1e-5)
+    og = G.Grid(lat, rho.shape, full=True)
+    oat = OIO.Atoms(ions.symbols, ions.positions, ions.cell, np.full(ions.nat, 3.0))
+    _close(s, FO.ewald_stress(og, oat), 1e-9, 1e-12)
+    _close(f, FO.ewald_forces(og, oat), 1e-9, 1e-10)
+
+
+def test_forces_vs_oracle():
+    ions, grid, rho, rho_np, pp, ev, og, oat, opp = _system()
+    f_pp = pp.forces(rho)
+    _close(f_pp, FO.pp_forces(rho_np, og, oat, opp), 1e-9)
+    f_ew = pp.get_ewald().forces
+    _close(f_ew, FO.ewald_forces(og, oat), 1e-9)
+    tot = ev.get_forces(rho, split=True)
+    _close(tot["TOTAL"], f_pp + f_ew, 1e-12)
+    assert set(tot) == {"TOTAL", "PSEUDO", "II"}
+
+
+def test_stress_vs_oracle():
+    ions, grid, rho, rho_np, pp, ev, og, oat, opp = _system()
+    r0 = rho_np.mean()
+    e_wt, _, _ = F.wt(rho_np, og, rho0=r0, kernel=F.wt_kernel(og.q, r0))
+    e_xc, v_xc = F.lda(rho_np, og)
+    e_pp = (OP.local_pp(og, oat, opp, pme=False) * rho_np).sum() * og.dV
+    ref = {"KEDF-TF": FO.tf_stress(F.tf(rho_np, og)[0], og.volume), "KEDF-VW": FO.vw_stress(rho_np, og),
+           "KEDF-NL": FO.wt_stress(rho_np, og, e_wt), "XC": FO.lda_stress(rho_np, og, e_xc, v_xc),
+           "HARTREE": FO.hartree_stress(rho_np, og, F.hartree(rho_np, og)[0]),
+           "PSEUDO": FO.pp_stress(rho_np, og, oat, opp, e_pp), "II": FO.ewald_stress(og, oat)}
+    st = ev.get_stress(rho, split=True)
+    for k, v in ref.items():
+        _close(st[k], v, 1e-8, 1e-13)
+    _close(st["TOTAL"], sum(ref.values()), 1e-8)
+
+
+def test_stress_fp32_field_runs_in_fp64():
+    """A density on an FP32 grid is promoted: the stress equals that of the
+    same density on the FP64 grid to FP32 input precision."""
+    from dftpy_amd import DirectField
+    ions, grid, rho, rho_np, pp, ev, og, oat, opp = _system(nr=(20, 20, 20))
+    g32 = grid.with_dtype(torch.float32)
+    r32 = DirectField(g32, data=rho.data.float())
+    s32 = ev["H"].stress(r32)
+    s64 = ev["H"].stress(DirectField(grid, data=r32.data.double()))
+    _close(s32, s64, 1e-14, 1e-16)
