@@ -243,6 +243,12 @@ __global__ void k_cg_sums(const T *r1, const T *r0, const T *d, long long n, dou
     write_partials<4>(acc, partials);
 }
 
+// p *= sqrt(scal[a] / scal[b])  (normalise p to sum p^2 = scal[a] with scal[b] = sum p^2)
+template <typename T> __global__ void k_scale_norm(T *p, const double *scal, int a, int b, long long n) {
+    const double f = sqrt(scal[a] / scal[b]);
+    GRID_LOOP(i, n) p[i] = (T)(f * (double)p[i]);
+}
+
 // sum a.b
 template <typename T> __global__ void k_dot2(const T *a, const T *b, long long n, double *partials) {
     double acc[1] = {0};
@@ -398,22 +404,24 @@ template <typename T> struct Opt {
             if (sl[0] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[0] * n, F(F_BASE_A), n);
             if (sl[1] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[1] * n, F(F_BASE_C), n);
         }
-        hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
-        double r0 = C.fetch(1)[0];  // |r|^2, for the stopping tests
         // preconditioned CG (when the plan holds a preconditioner): z = M^-1 r,
         // alpha = r.z / p.Ap, p <- z + beta p with beta = r1.z1 / r0.z0
         const bool pre = P.tn_pre_n > 0;
         T *z = F(F_Z);
-        double rz = r0;
         auto precond_dot = [&]() {
             apply_precond(tres, z);
             hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, tres, z, n, P.red);
         };
+        hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
+        C.reduce_final_cols(1, 16);  // |r|^2, for the stopping tests
         if (pre) {
             precond_dot();
-            rz = C.fetch(1)[0];
+            C.reduce_final_cols(1, 17);  // r.z
             hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, z, tp, n);
         }
+        const double *h0 = C.to_host(pre ? 2 : 1, 16);
+        double r0 = h0[0];
+        double rz = pre ? h0[1] : r0;
         const double rconv = r0 * 0.1;
         bool best_zero = true;
         std::vector<double> rl{1e10};
@@ -471,18 +479,24 @@ template <typename T> struct Opt {
 
     // OrthogonalNormalization (optimization.py:204-225) of F_P against phi;
     // returns theta0.
-    double orthonormalize() {
+    // OrthogonalNormalization (optimization.py:204-225): p -= (sum p phi / sum phi^2) phi,
+    // then scaled to sum p^2 = sum phi^2, with the projection and the scale taken
+    // from device scalars.  Returns theta0 = sqrt(|p|^2 / N) before scaling and
+    // sum(V phi p) after it (the line search's slope at theta = 0 is 2 dV times
+    // that), in one host round trip.
+    void orthonormalize(double &theta0, double &vphip) {
         T *p = F(F_P), *phi = F(F_PHI);
         hipLaunchKernelGGL((k_pphi<T>), C.grid(), dim3(256), 0, C.s, p, phi, n, P.red);
-        const double *h = C.fetch(2);
-        const double Ne = h[1] * P.dV;
-        const double f = h[0] * P.dV / Ne;
-        C.set_scalar(3, f);
+        C.reduce_final_cols(2, 21);  // scal[21] = sum p phi, scal[22] = sum phi^2
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 3, 21, 22);
         hipLaunchKernelGGL((k_orth<T>), C.grid(), dim3(256), 0, C.s, p, phi, P.scal + 3, n, P.red);
-        const double pnorm = C.fetch(1)[0] * P.dV;
-        const double theta0 = std::sqrt(pnorm / Ne);
-        hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, 0.0, p, std::sqrt(Ne / pnorm), p, n);
-        return theta0;
+        C.reduce_final_cols(1, 23);  // scal[23] = sum p^2
+        hipLaunchKernelGGL((k_scale_norm<T>), C.grid(), dim3(256), 0, C.s, p, P.scal, 22, 23, n);
+        hipLaunchKernelGGL((k_dot3<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), phi, p, n, P.red);
+        C.reduce_final_cols(1, 24);  // scal[24] = sum V phi p
+        const double *h = C.to_host(4, 21);
+        theta0 = std::sqrt(h[2] / h[1]);
+        vphip = h[3];
     }
 
     // value and derivative at theta (ValueAndDerivative, optimization.py:227-279);
@@ -545,18 +559,16 @@ template <typename T> struct Opt {
                     hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, -1.0, F(F_RES), beta, F(F_P), n);
                 }
             }
-            double theta0 = orthonormalize();
-            hipLaunchKernelGGL((k_dot3<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), F(F_PHI), F(F_P), n, P.red);
-            const double d0 = C.fetch(1)[0] * 2.0;
-            if (d0 > 0) {
+            double theta0, vphip;
+            orthonormalize(theta0, vphip);
+            if (vphip > 0) {  // uphill: restart from the steepest descent direction
                 hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
-                theta0 = orthonormalize();
+                orthonormalize(theta0, vphip);
             }
             theta = std::min(theta0, theta);
-            // line search (math_utils.py:76-106) with func0 at theta = 0
-            hipLaunchKernelGGL((k_theta_grad<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), F(F_PHI), F(F_P), F(F_PHI), 1.0,
-                               0.0, n, P.red);
-            double f = energy, g = 2.0 * C.fetch(1)[0] * P.dV;
+            // line search (math_utils.py:76-106) with func0 at theta = 0, where
+            // dE/dtheta = 2 dV sum(V phi p)
+            double f = energy, g = 2.0 * vphip * P.dV;
             double st[16];
             LsTask task = LS_START;
             double stp = theta;

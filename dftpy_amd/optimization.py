@@ -86,7 +86,14 @@ class Optimization:
         hvp = 0 if hvp == "fd" else 1
         # preconditioned inner CG of TN (not in the reference): opt-in
         if o["precond"] and self.optimization_method == "TN":
-            key = (float(rho_in.data.mean()), int(terms_all), tuple(sorted(plan.kedf_settings.items())))
+            # rho0 = N_e / V from the ions when the functional has them (no device read)
+            rho0 = None
+            for f in funcs:
+                if getattr(f, "type", "") == "PSEUDO":
+                    rho0 = float(f.ions.get_ncharges()) / grid.volume
+            if rho0 is None:
+                rho0 = float(rho_in.data.mean())
+            key = (float("%.8e" % rho0), int(terms_all), tuple(sorted(plan.kedf_settings.items())))
             if getattr(plan, "tn_precond_key", None) != key:
                 tab, dq = tn_preconditioner(grid, key[0], key[1], plan.kedf_settings)
                 _lib.check(_lib.lib.ofdft_set_tn_precond(plan.handle, int(tab.size), dq, _lib.dptr(tab)))
