@@ -178,8 +178,9 @@ __global__ void __launch_bounds__(MID_THREADS)
 // once in LDS (6 nz points x 64 bytes) and the 9 (dy, dz) taps read their A
 // operands from there, so each input point is read from HBM/L2 about 4.5
 // times per layer instead of 27.  Rows in a residue class mod d keep the
-// dilated y taps inside the staged rows.  Weights (B operands) are read from
-// global memory (L1/L2 resident).  8 waves; wave w owns M-tiles w, w+8, ...
+// dilated y taps inside the staged rows.  The 9 taps' weights (B operands,
+// 18 KB) are staged with the rows; LDS per workgroup stays under 80 KB so
+// two workgroups share a CU.  8 waves; wave w owns M-tiles w, w+8, ...
 constexpr int ROWS_THREADS = 512;
 constexpr int ROWS_MAX_MT = 5;  // M-tiles per wave for nz = 160 (40 tiles / 8 waves)
 
@@ -203,6 +204,10 @@ __global__ void __launch_bounds__(ROWS_THREADS)
     const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
     const bf16x8 *wv = reinterpret_cast<const bf16x8 *>(w);
     const int nload = 6 * nz * 4;
+    // the weights of the 9 taps of x offset ix ([9][32 out][4 x 8 in] bf16)
+    // are staged next to the rows, so the B operands come from LDS
+    bf16x8 *wl = rows + nload;
+    constexpr int NWL = 9 * CH * 4;
 #pragma unroll 1
     for (int ix = 0; ix < 3; ++ix) {
         const int xp = wrapi(x + (ix - 1) * d, g.nx);
@@ -212,24 +217,22 @@ __global__ void __launch_bounds__(ROWS_THREADS)
             const int yr = wrapi(y0 + (r - 1) * d, g.ny);
             rows[e] = xv[((long long)xp * nyz + (long long)yr * nz) * 4 + rem];
         }
+        for (int e = threadIdx.x; e < NWL; e += ROWS_THREADS) wl[e] = wv[ix * NWL + e];
         __syncthreads();
 #pragma unroll
-        for (int iy = 0; iy < 3; ++iy) {
+        for (int tt = 0; tt < 9; ++tt) {
+            const int iy = tt / 3, iz = tt % 3;
+            const bf16x8 b0 = wl[(tt * CH + m) * 4 + kq];
+            const bf16x8 b1 = wl[(tt * CH + 16 + m) * 4 + kq];
 #pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-                const int t = (ix * 3 + iy) * 3 + iz;
-                const bf16x8 b0 = wv[(t * CH + m) * 4 + kq];
-                const bf16x8 b1 = wv[(t * CH + 16 + m) * 4 + kq];
-#pragma unroll
-                for (int i = 0; i < MT; ++i) {
-                    const int tile = wave + 8 * i;  // tile = k * tiles_per_row + zt
-                    const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
-                    int z = zt * 16 + m + (iz - 1) * d;
-                    z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-                    const bf16x8 a = rows[((k + iy) * nz + z) * 4 + kq];
-                    acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[i][0], 0, 0, 0);
-                    acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[i][1], 0, 0, 0);
-                }
+            for (int i = 0; i < MT; ++i) {
+                const int tile = wave + 8 * i;  // tile = k * tiles_per_row + zt
+                const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
+                int z = zt * 16 + m + (iz - 1) * d;
+                z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+                const bf16x8 a = rows[((k + iy) * nz + z) * 4 + kq];
+                acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[i][0], 0, 0, 0);
+                acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[i][1], 0, 0, 0);
             }
         }
     }
@@ -720,6 +723,12 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
     });
 }
 
+// allow a row-tiled launch above the default 64 KB of dynamic LDS
+static void rows_lds_limit(const void *kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        OFDFT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const void *w, const float *bias,
                        const float *resid, const void *mask, void *out0, void *out1, void *stream) {
     return cnn_guard([&] {
@@ -733,10 +742,12 @@ int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const 
         const int tiles = 4 * g.nz / 16;
         if (g.nz % 16 == 0 && g.nz <= 160 && g.ny % (4 * g.d) == 0 && tiles % 8 == 0 && tiles / 8 <= ROWS_MAX_MT) {
             const dim3 grid((unsigned)(g.ny / 4), (unsigned)g.nx);
-            const size_t lds = (size_t)6 * g.nz * CH * sizeof(__bf16);
+            const size_t lds = (size_t)6 * g.nz * CH * sizeof(__bf16) + (size_t)9 * CH * CH * sizeof(__bf16);
             const int mt = tiles / 8;
 #define ROWS_LAUNCH(MTV)                                                                                            \
     do {                                                                                                            \
+        rows_lds_limit((const void *)&k_cnn_mid_rows<0, MTV>, lds);                                                \
+        rows_lds_limit((const void *)&k_cnn_mid_rows<1, MTV>, lds);                                                \
         if (mode == 0)                                                                                              \
             hipLaunchKernelGGL((k_cnn_mid_rows<0, MTV>), grid, dim3(ROWS_THREADS), lds, St(stream), g, xb, wb, bias, \
                                nullptr, nullptr, out0, reinterpret_cast<__bf16 *>(out1));                            \
