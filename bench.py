@@ -173,18 +173,35 @@ def cpu_baseline(nr, seed, n_evals, budget_s=25.0):
     vreal = P.local_pp(g, at, {"Al": (gp, vp)}, pme=True)
     rho = np.full(tuple(nr), at.ncharge() / g.volume)
     kern = F.wt_kernel(g.q, rho.mean())
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < 3 and time.perf_counter() - t_start < budget_s:
-        t0 = time.perf_counter()
-        F.total(rho, g, vreal, kern)
-        times.append(time.perf_counter() - t0)
-    per = float(np.median(times))
+
+    def per_eval(workers):
+        g.workers = workers
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < 3 and time.perf_counter() - t_start < budget_s:
+            t0 = time.perf_counter()
+            F.total(rho, g, vreal, kern)
+            times.append(time.perf_counter() - t0)
+        g.workers = None
+        return float(np.median(times)), len(times)
+
+    per, nt = per_eval(None)
+    ncores = len(os.sched_getaffinity(0))
+    per_all, nt_all = per_eval(ncores)
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=per * n_evals, unit="s", cores=1, kind="port",
                 sample="%d FP64 oracle E+V evaluations at 160^3 (numpy pocketfft, 1 thread), %.3f s each, x %d "
-                       "evaluations of the TN run (FD Hessian, as the reference)" % (len(times), per, n_evals),
-                seconds_per_eval=per, cpu_threads_available=len(os.sched_getaffinity(0)),
-                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+                       "evaluations of the TN run (FD Hessian, as the reference)" % (nt, per, n_evals),
+                seconds_per_eval=per, value_all_cores=per_all * n_evals, cores_all=ncores,
+                seconds_per_eval_all_cores=per_all,
+                sample_all_cores="%d evaluations with scipy.fft on %d workers (FFTs threaded, point-wise numpy "
+                                 "on one thread)" % (nt_all, ncores),
+                cpu_model=cpu_model, omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
 
 
 def main():

@@ -16,7 +16,7 @@ everything those leave open, by the mathematics: each force against central
 differences of the oracle's energy under a displacement of one atom, each
 stress term against central differences of its energy under homogeneous
 strain (density carried along, electron count fixed), in
-tests/test_oracle.py.
+tests/test_oracle_forces.py.
 """
 import numpy as np
 from scipy import special as sp

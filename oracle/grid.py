@@ -169,8 +169,17 @@ class Grid:
         return np.pi ** 2 / (2 * sp.max() ** 2)
 
     # --- transforms (field.py) ---
+    # FFT threads: None = numpy's pocketfft (one thread, the reference's
+    # default); an integer = scipy.fft with that many workers
+    workers = None
+
     def fft(self, f):
         """R2C (full=False) or complex fftn (full=True, ``field.py:159-172``)."""
+        if self.workers:
+            import scipy.fft as sf
+            if self.full:
+                return sf.fftn(f, workers=self.workers) * self.dV
+            return sf.rfftn(f, workers=self.workers) * self.dV
         if self.full:
             return np.fft.fftn(f) * self.dV
         return np.fft.rfftn(f) * self.dV
@@ -178,6 +187,11 @@ class Grid:
     def ifft(self, F):
         """Real field back; on a full grid the real part of ifftn is kept
         (``force_real``, ``field.py:739-764``)."""
+        if self.workers:
+            import scipy.fft as sf
+            if self.full:
+                return np.real(sf.ifftn(F, workers=self.workers)) / self.dV
+            return sf.irfftn(F, s=tuple(self.nr), axes=(0, 1, 2), workers=self.workers) / self.dV
         if self.full:
             return np.real(np.fft.ifftn(F)) / self.dV
         return np.fft.irfftn(F, s=tuple(self.nr), axes=(0, 1, 2)) / self.dV
