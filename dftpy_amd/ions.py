@@ -3,13 +3,27 @@ import numpy as np
 
 from .constants import Units
 
+# Element symbols indexed by atomic number (0 is a dummy), for formats that
+# store atomic numbers instead of symbols.
+chemical_symbols = ["X"] + """
+H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn
+Ga Ge As Se Br Kr Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce
+Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At Rn
+Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf Es Fm Md No Lr Rf Db Sg Bh Hs Mt Ds Rg Cn Nh Fl
+Mc Lv Ts Og""".split()
+atomic_numbers = {s: z for z, s in enumerate(chemical_symbols)}
+
 
 class Ions:
     """Atoms with positions and cell in bohr.  ``units='ase'`` converts from
     Angstrom input like ``Ions(..., units='ase')`` of the reference."""
 
     def __init__(self, symbols=None, positions=None, cell=None, scaled_positions=None, charges=None, units="au",
-                 pbc=None, **kwargs):
+                 pbc=None, numbers=None, **kwargs):
+        if symbols is None:
+            if numbers is None:
+                raise ValueError("Ions need symbols or atomic numbers")
+            symbols = [chemical_symbols[int(z)] for z in np.asarray(numbers).ravel()]
         self.symbols = list(symbols)
         self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3).copy()
         if scaled_positions is not None:
@@ -20,6 +34,11 @@ class Ions:
             self.cell /= Units.Bohr
             self.positions /= Units.Bohr
         self._charges = None if charges is None else np.asarray(charges, dtype=np.float64)
+
+    @property
+    def numbers(self):
+        """Atomic numbers (0 for a symbol that is not an element)."""
+        return np.array([atomic_numbers.get(s, 0) for s in self.symbols], dtype=np.int64)
 
     @property
     def nat(self):
