@@ -221,6 +221,22 @@ KEDF_TERMS = {
 }
 
 
+KEDF_PARTS = (("TF", _lib.TERM_TF), ("VW", _lib.TERM_VW), ("WT-NL", _lib.TERM_WT), ("LMGP-NL", _lib.TERM_LMGP))
+
+
+class _TermSubset(AbstractFunctional):
+    """Some of a KEDF's terms with the KEDF's settings (split output)."""
+
+    def __init__(self, terms, options, name):
+        self.type = "KEDF"
+        self.name = name
+        self.terms = terms
+        self.options = options
+
+    def kedf_options(self):
+        return self.options
+
+
 class KEDF(AbstractFunctional):
     """``KEDF(name='WT', **options)`` (functional/kedf/__init__.py:90-184).
     Options: x (TF scale), y (vW scale), alpha, beta, rho0, sigma."""
@@ -241,10 +257,15 @@ class KEDF(AbstractFunctional):
         return self.options
 
     def compute(self, rho, calcType={"E", "V"}, split=False, **kwargs):
-        out = evaluate([self], rho, calcType, name="KEDF")
-        if split:
-            raise NotImplementedError("split KEDF output is not supported")
-        return out
+        """split: dict of the parts keyed as kedf/__init__.py:144-172 names
+        them ('TF', 'VW', 'WT-NL' or 'LMGP-NL'), one evaluation each."""
+        if not split:
+            return evaluate([self], rho, calcType, name="KEDF")
+        parts = {}
+        for key, bit in KEDF_PARTS:
+            if self.terms & bit:
+                parts[key] = evaluate([_TermSubset(bit, self.options, key)], rho, calcType, name=key)
+        return parts
 
     def stress(self, density, split=False, **kwargs):
         """Sum of the stresses of the KEDF's parts (kedf/__init__.py:186-226);

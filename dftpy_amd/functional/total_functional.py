@@ -3,7 +3,7 @@ sum of member functionals plus the Ewald ion-ion energy of the PSEUDO
 member, evaluated in one fused HIP call."""
 import numpy as np
 
-from . import AbstractFunctional, evaluate
+from . import AbstractFunctional, FunctionalOutput, evaluate
 
 
 class TotalFunctional(AbstractFunctional):
@@ -58,12 +58,34 @@ class TotalFunctional(AbstractFunctional):
 
     def compute(self, rho, calcType={"E", "V"}, split=False, **kwargs):
         if split:
-            raise NotImplementedError("split output is not supported")
+            return self.get_energy_potential(rho, calcType, split=True)
         eion = self.eion() if "E" in set(calcType) else 0.0
         return evaluate(list(self.funcDict.values()), rho, calcType, name="TOTAL", eion=eion)
 
-    def get_energy_potential(self, rho, calcType={"E", "V"}, **kwargs):
-        return self.compute(rho, calcType, **kwargs)
+    def get_energy_potential(self, rho, calcType={"E", "V"}, split=False, **kwargs):
+        """Without split: one fused evaluation of all members.  With split
+        (total_functional.py:113-146): a dict with 'KEDF-TF', 'KEDF-VW',
+        'KEDF-NL', the other members by type, 'II' (Ewald energy) and
+        'TOTAL', one evaluation per part."""
+        if not split:
+            return self.compute(rho, calcType)
+        out = {"TOTAL": FunctionalOutput(name="TOTAL", energy=0.0)}
+        for func in self.funcDict.values():
+            if func.type == "KEDF":
+                for k2, r in func.compute(rho, calcType, split=True).items():
+                    key = "KEDF-" + k2.split("-")[-1]
+                    out[key] = r + out.get(key)
+                    out["TOTAL"] += r
+            else:
+                r = func(rho, calcType)
+                out[func.type] = r
+                out["TOTAL"] += r
+        ew = self.ewald
+        if ew is not None:
+            ii = FunctionalOutput(name="II", energy=ew.energy)
+            out["II"] = ii
+            out["TOTAL"] += ii
+        return out
 
     def Energy(self, rho, **kwargs):
         return self.compute(rho, calcType={"E"}).energy
