@@ -172,94 +172,207 @@ __global__ void __launch_bounds__(MID_THREADS)
     }
 }
 
-// Row-tiled variant for nz % 16 == 0, nz <= 160 and ny % (4 d) == 0: a
-// workgroup owns 4 output rows y0 + k d (k = 0..3) of one x plane, all of z.
-// For each x offset the 6 input rows y0 + (k-1) d of plane x + dx are staged
-// once in LDS (6 nz points x 64 bytes) and the 9 (dy, dz) taps read their A
-// operands from there, so each input point is read from HBM/L2 about 4.5
-// times per layer instead of 27.  Rows in a residue class mod d keep the
-// dilated y taps inside the staged rows.  The 9 taps' weights (B operands,
-// 18 KB) are staged with the rows; LDS per workgroup stays under 80 KB so
-// two workgroups share a CU.  8 waves; wave w owns M-tiles w, w+8, ...
-constexpr int ROWS_THREADS = 512;
-constexpr int ROWS_MAX_MT = 5;  // M-tiles per wave for nz = 160 (40 tiles / 8 waves)
+// Row-tiled variant for nz % 32 == 0, nz <= 160 and ny % (4 d) == 0.  A work
+// item is 4 output rows y0 + k d (k = 0..3) of one x plane, all of z.  For
+// each x offset (a stage) the 6 input rows y0 + (r-1) d of plane x + dx are
+// staged in LDS (6 nz points x 64 bytes) with the 9 taps' weights, so each
+// input point is read from HBM/L2 about 4.5 times per layer instead of 27.
+// Rows in a residue class mod d keep the dilated y taps inside the staged
+// rows.
+//
+// One wave per 16-point z tile, owning that tile in all 4 output rows: an A
+// operand read from staged row r serves the output rows k = r - dy of every
+// dy tap (up to 3 rows), so each A fragment feeds up to 6 MFMAs and the
+// B operands of one dz column (3 dy x 2 channel halves) stay in registers.
+//
+// The kernel is persistent (one workgroup per CU) and double-buffers the
+// stages in LDS: the global loads of the next stage are in flight while the
+// current one is computed, then land in the other buffer.  Work items are
+// dealt to the 8 XCDs in contiguous chunks (workgroup b runs on XCD b % 8),
+// so the planes an XCD re-reads stay in its own L2.
+constexpr int ROWS_MAX_MT = 5;                       // nz = 32 MT <= 160
+constexpr int ROWS_MAX_THREADS = 128 * ROWS_MAX_MT;  // 64 lanes x nz / 16 waves
+constexpr int ROWS_XCDS = 8;
 
 template <int MODE, int MT>
-__global__ void __launch_bounds__(ROWS_THREADS)
+__global__ void __launch_bounds__(ROWS_MAX_THREADS)
     k_cnn_mid_rows(Geo g, const __bf16 *__restrict__ xin, const __bf16 *__restrict__ w,
                    const float *__restrict__ bias, const float *__restrict__ resid, const __bf16 *__restrict__ mask,
                    void *__restrict__ out0, __bf16 *__restrict__ out1) {
-    extern __shared__ bf16x8 rows[];  // [6][nz][4]
-    const int nz = g.nz, d = g.d;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    extern __shared__ bf16x8 lds[];  // 2 x ([6][nz][4] rows + [9][32][4] weights)
+    constexpr int NZ = 32 * MT;      // the launcher picks MT = nz / 32
+    constexpr int NT = 128 * MT;     // threads: one wave per z tile
+    constexpr int NLOAD = 6 * NZ * 4;
+    constexpr int NWL = 9 * CH * 4;
+    constexpr int BUF = NLOAD + NWL;
+    constexpr int PER = (NLOAD + NT - 1) / NT, PERW = (NWL + NT - 1) / NT;
+    const int nz = NZ, d = g.d;
+    const int lane = threadIdx.x & 63, zt = threadIdx.x >> 6;
     const int m = lane & 15, kq = lane >> 4;
-    const int x = blockIdx.y;
-    const int gy = blockIdx.x;  // row group: residue gy % d, block gy / d
-    const int y0 = (gy / d) * 4 * d + gy % d;
-    const int tiles_per_row = nz / 16;
     const long long nyz = (long long)g.ny * nz;
-    floatx4 acc[MT][2];
-#pragma unroll
-    for (int i = 0; i < MT; ++i) acc[i][0] = acc[i][1] = floatx4{0.f, 0.f, 0.f, 0.f};
     const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
     const bf16x8 *wv = reinterpret_cast<const bf16x8 *>(w);
-    const int nload = 6 * nz * 4;
-    // the weights of the 9 taps of x offset ix ([9][32 out][4 x 8 in] bf16)
-    // are staged next to the rows, so the B operands come from LDS
-    bf16x8 *wl = rows + nload;
-    constexpr int NWL = 9 * CH * 4;
-#pragma unroll 1
-    for (int ix = 0; ix < 3; ++ix) {
-        const int xp = wrapi(x + (ix - 1) * d, g.nx);
-        __syncthreads();
-        for (int e = threadIdx.x; e < nload; e += ROWS_THREADS) {
-            const int r = e / (nz * 4), rem = e - r * nz * 4;
-            const int yr = wrapi(y0 + (r - 1) * d, g.ny);
-            rows[e] = xv[((long long)xp * nyz + (long long)yr * nz) * 4 + rem];
-        }
-        for (int e = threadIdx.x; e < NWL; e += ROWS_THREADS) wl[e] = wv[ix * NWL + e];
-        __syncthreads();
+    // this workgroup's items: slot s of its XCD's chunk, then every
+    // (workgroups per XCD)-th item of the chunk
+    const int ngroups = g.ny / 4;
+    const long long items = (long long)g.nx * ngroups;
+    const int nwg = gridDim.x, per_xcd = nwg / ROWS_XCDS;
+    const int xcd = blockIdx.x % ROWS_XCDS, slot = blockIdx.x / ROWS_XCDS;
+    const long long c0 = items * xcd / ROWS_XCDS, c1 = items * (xcd + 1) / ROWS_XCDS;
+    const long long first = c0 + slot;
+    const int nmine = first < c1 ? (int)((c1 - 1 - first) / per_xcd + 1) : 0;
+    const int nq = 3 * nmine;  // stages: x offsets -d, +d, 0 of each item (centre last)
+    auto item_of = [&](int q, int &x, int &y0, int &xp, int &ix) {
+        const long long it = first + (long long)(q / 3) * per_xcd;
+        const int st = q % 3;
+        x = (int)(it / ngroups);
+        const int gy = (int)(it - (long long)x * ngroups);  // residue gy % d, block gy / d
+        y0 = (gy / d) * 4 * d + gy % d;
+        ix = st == 0 ? 0 : (st == 1 ? 2 : 1);
+        xp = wrapi(x + (ix - 1) * d, g.nx);
+    };
+    bf16x8 st_r[PER], st_w[PERW];
+    auto fetch = [&](int q) {
+        int x, y0, xp, ix;
+        item_of(q, x, y0, xp, ix);
 #pragma unroll
-        for (int tt = 0; tt < 9; ++tt) {
-            const int iy = tt / 3, iz = tt % 3;
-            const bf16x8 b0 = wl[(tt * CH + m) * 4 + kq];
-            const bf16x8 b1 = wl[(tt * CH + 16 + m) * 4 + kq];
-#pragma unroll
-            for (int i = 0; i < MT; ++i) {
-                const int tile = wave + 8 * i;  // tile = k * tiles_per_row + zt
-                const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
-                int z = zt * 16 + m + (iz - 1) * d;
-                z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-                const bf16x8 a = rows[((k + iy) * nz + z) * 4 + kq];
-                acc[i][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[i][0], 0, 0, 0);
-                acc[i][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[i][1], 0, 0, 0);
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + j * NT;
+            if (e < NLOAD) {
+                const int r = e / (NZ * 4), rem = e - r * NZ * 4;
+                const int yr = wrapi(y0 + (r - 1) * d, g.ny);
+                st_r[j] = xv[((long long)xp * nyz + (long long)yr * NZ) * 4 + rem];
             }
         }
+#pragma unroll
+        for (int j = 0; j < PERW; ++j) {
+            const int e = threadIdx.x + j * NT;
+            if (e < NWL) st_w[j] = wv[ix * NWL + e];
+        }
+    };
+    auto land = [&](int buf) {
+        bf16x8 *dst = lds + buf * BUF;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + j * NT;
+            if (e < NLOAD) dst[e] = st_r[j];
+        }
+#pragma unroll
+        for (int j = 0; j < PERW; ++j) {
+            const int e = threadIdx.x + j * NT;
+            if (e < NWL) dst[NLOAD + e] = st_w[j];
+        }
+    };
+    if (nq > 0) {
+        fetch(0);
+        land(0);
     }
+    __syncthreads();
+    floatx4 acc[4][2];
 #pragma unroll
-    for (int i = 0; i < MT; ++i) {
-        const int tile = wave + 8 * i;
-        const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
-        const long long rowbase = (long long)x * nyz + (long long)(y0 + k * d) * nz + zt * 16;
+    for (int k = 0; k < 4; ++k) acc[k][0] = acc[k][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int q = 0; q < nq; ++q) {
+        const int cur = q & 1;
+        if (q + 1 < nq) fetch(q + 1);
+        const bf16x8 *rows = lds + cur * BUF;
+        const bf16x8 *wl = rows + NLOAD;
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const int co = nt * 16 + m;
+        for (int iz = 0; iz < 3; ++iz) {
+            bf16x8 b[3][2];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const long long e = (rowbase + kq * 4 + j) * CH + co;
-                float v = acc[i][nt][j];
-                if (MODE == 0) {
-                    v += bias[co];
-                    const float r = fmaxf(v, 0.0f);
-                    reinterpret_cast<__bf16 *>(out0)[e] = (__bf16)((float)xin[e] + r);
-                    if (out1) out1[e] = (__bf16)r;
-                } else {
-                    v += resid[e];
-                    reinterpret_cast<float *>(out0)[e] = v;
-                    out1[e] = (float)mask[e] > 0.0f ? (__bf16)v : (__bf16)0.0f;
+            for (int iy = 0; iy < 3; ++iy) {
+                const int tt = iy * 3 + iz;
+                b[iy][0] = wl[(tt * CH + m) * 4 + kq];
+                b[iy][1] = wl[(tt * CH + 16 + m) * 4 + kq];
+            }
+            int z = zt * 16 + m + (iz - 1) * d;
+            z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) {
+                const bf16x8 a = rows[(r * nz + z) * 4 + kq];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int iy = r - k;  // output row k reads staged row k + dy
+                    if (iy >= 0 && iy < 3) {
+                        acc[k][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[iy][0], acc[k][0], 0, 0, 0);
+                        acc[k][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[iy][1], acc[k][1], 0, 0, 0);
+                    }
                 }
             }
         }
+        if (q % 3 == 2) {  // the item's last stage (centre plane): write it out
+            int x, y0, xp, ix;
+            item_of(q, x, y0, xp, ix);
+            if (MODE == 0) {
+                // forward: out = x + relu(acc + b).  The residual x at an
+                // output point is LDS row k + 1 of this (centre) stage; each
+                // output replaces its own residual in LDS, then the 4 rows go
+                // out as coalesced 16-byte stores
+                __syncthreads();  // every wave is done reading the stage
+                __bf16 *rs = reinterpret_cast<__bf16 *>(lds + cur * BUF);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) {
+                        const int co = nt * 16 + m;
+                        const float bco = bias[co];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int z = zt * 16 + kq * 4 + j;
+                            const int li = ((k + 1) * nz + z) * CH + co;
+                            const float r = fmaxf(acc[k][nt][j] + bco, 0.0f);
+                            if (out1) {
+                                const long long e =
+                                    ((long long)x * nyz + (long long)(y0 + k * d) * nz + z) * CH + co;
+                                out1[e] = (__bf16)r;
+                            }
+                            rs[li] = (__bf16)((float)rs[li] + r);
+                        }
+                    }
+                }
+                __syncthreads();
+                bf16x8 *ov = reinterpret_cast<bf16x8 *>(out0);
+                const bf16x8 *src = lds + cur * BUF;
+                for (int e = threadIdx.x; e < 4 * NZ * 4; e += NT) {
+                    const int k = e / (NZ * 4), rem = e - k * NZ * 4;
+                    ov[((long long)x * nyz + (long long)(y0 + k * d) * NZ) * 4 + rem] = src[(k + 1) * NZ * 4 + rem];
+                }
+            } else {
+                // backward-data: out0 = acc + resid (FP32), out1 = out0 where
+                // the mask is positive (bf16); per row, resid and mask are
+                // loaded before the stores
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const long long rowbase = (long long)x * nyz + (long long)(y0 + k * d) * nz + zt * 16;
+                    float rv[2][4];
+                    bool keep[2][4];
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const long long e = (rowbase + kq * 4 + j) * CH + nt * 16 + m;
+                            rv[nt][j] = resid[e];
+                            keep[nt][j] = (float)mask[e] > 0.0f;
+                        }
+                    }
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const long long e = (rowbase + kq * 4 + j) * CH + nt * 16 + m;
+                            const float v = acc[k][nt][j] + rv[nt][j];
+                            reinterpret_cast<float *>(out0)[e] = v;
+                            out1[e] = keep[nt][j] ? (__bf16)v : (__bf16)0.0f;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k][0] = acc[k][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        if (q + 1 < nq) land(cur ^ 1);
+        __syncthreads();
     }
 }
 
@@ -724,6 +837,15 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
 }
 
 // allow a row-tiled launch above the default 64 KB of dynamic LDS
+// workgroups of the persistent row kernel: one per CU, a multiple of the 8
+// XCDs
+static int rows_workgroups() {
+    int dev = 0, cus = 0;
+    OFDFT_HIP(hipGetDevice(&dev));
+    OFDFT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max(ROWS_XCDS, cus / ROWS_XCDS * ROWS_XCDS);
+}
+
 static void rows_lds_limit(const void *kernel, size_t bytes) {
     if (bytes > 64 * 1024)
         OFDFT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -739,22 +861,22 @@ int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const 
         OFDFT_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (forward) or 1 (backward-data)");
         OFDFT_REQUIRE(mode == 1 || bias, "forward mode needs the bias");
         OFDFT_REQUIRE(mode == 0 || (resid && mask), "backward mode needs resid and mask");
-        const int tiles = 4 * g.nz / 16;
-        if (g.nz % 16 == 0 && g.nz <= 160 && g.ny % (4 * g.d) == 0 && tiles % 8 == 0 && tiles / 8 <= ROWS_MAX_MT) {
-            const dim3 grid((unsigned)(g.ny / 4), (unsigned)g.nx);
-            const size_t lds = (size_t)6 * g.nz * CH * sizeof(__bf16) + (size_t)9 * CH * CH * sizeof(__bf16);
-            const int mt = tiles / 8;
+        if (g.nz % 32 == 0 && g.nz <= 32 * ROWS_MAX_MT && g.ny % (4 * g.d) == 0) {
+            // two stage buffers of 6 rows and 9 taps' weights each
+            const size_t lds = 2 * ((size_t)6 * g.nz * CH * sizeof(__bf16) + (size_t)9 * CH * CH * sizeof(__bf16));
+            const int mt = g.nz / 32;
+            const dim3 block((unsigned)(4 * g.nz));  // one wave per 16-point z tile
+            const dim3 grid((unsigned)rows_workgroups());
 #define ROWS_LAUNCH(MTV)                                                                                            \
     do {                                                                                                            \
         rows_lds_limit((const void *)&k_cnn_mid_rows<0, MTV>, lds);                                                \
         rows_lds_limit((const void *)&k_cnn_mid_rows<1, MTV>, lds);                                                \
         if (mode == 0)                                                                                              \
-            hipLaunchKernelGGL((k_cnn_mid_rows<0, MTV>), grid, dim3(ROWS_THREADS), lds, St(stream), g, xb, wb, bias, \
-                               nullptr, nullptr, out0, reinterpret_cast<__bf16 *>(out1));                            \
+            hipLaunchKernelGGL((k_cnn_mid_rows<0, MTV>), grid, block, lds, St(stream), g, xb, wb, bias, nullptr,      \
+                               nullptr, out0, reinterpret_cast<__bf16 *>(out1));                                    \
         else                                                                                                        \
-            hipLaunchKernelGGL((k_cnn_mid_rows<1, MTV>), grid, dim3(ROWS_THREADS), lds, St(stream), g, xb, wb,      \
-                               nullptr, resid, reinterpret_cast<const __bf16 *>(mask), out0,                         \
-                               reinterpret_cast<__bf16 *>(out1));                                                   \
+            hipLaunchKernelGGL((k_cnn_mid_rows<1, MTV>), grid, block, lds, St(stream), g, xb, wb, nullptr, resid,     \
+                               reinterpret_cast<const __bf16 *>(mask), out0, reinterpret_cast<__bf16 *>(out1));      \
     } while (0)
             switch (mt) {
                 case 1: ROWS_LAUNCH(1); break;

@@ -25,6 +25,16 @@ namespace ofdft {
 
 void set_error(const std::string &msg);
 
+// Wait for the work queued on a stream by polling it.  The host round trips
+// of the solvers are a few microseconds of GPU work each; a blocking wait
+// adds an interrupt-driven wake-up to every one of them.
+inline void stream_wait(hipStream_t s) {
+    hipError_t e;
+    while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+    }
+    OFDFT_HIP(e);
+}
+
 template <typename T> struct cpx_t;
 template <> struct cpx_t<float> { using type = float2; };
 template <> struct cpx_t<double> { using type = double2; };

@@ -308,10 +308,11 @@ struct Ctx {
 
     dim3 grid() const { return dim3(nb); }
     // reduce K partial columns into scal[slot..slot+K) and copy them to the host
+    void wait() const { stream_wait(s); }
     const double *fetch(int K, int slot = 16) {
         reduce_final_cols(K, slot);
         OFDFT_HIP(hipMemcpyAsync(host, P.scal + slot, K * sizeof(double), hipMemcpyDeviceToHost, s));
-        OFDFT_HIP(hipStreamSynchronize(s));
+        wait();
         return host;
     }
     void reduce_final_cols(int K, int slot) {
@@ -322,13 +323,13 @@ struct Ctx {
     // copy scal[slot..slot+K) to the host
     const double *to_host(int K, int slot) {
         OFDFT_HIP(hipMemcpyAsync(host, P.scal + slot, K * sizeof(double), hipMemcpyDeviceToHost, s));
-        OFDFT_HIP(hipStreamSynchronize(s));
+        wait();
         return host;
     }
     void set_scalar(int slot, double v) {
         host[32] = v;
         OFDFT_HIP(hipMemcpyAsync(P.scal + slot, host + 32, sizeof(double), hipMemcpyHostToDevice, s));
-        OFDFT_HIP(hipStreamSynchronize(s));
+        wait();
     }
 };
 
@@ -351,8 +352,11 @@ template <typename T> struct Opt {
 
     double eval_ev(const T *rho, T *pot) {
         eval<T>(P, rho, OFDFT_CALC_E | OFDFT_CALC_V, pot, energies_d, C.s);
-        OFDFT_HIP(hipMemcpyAsync(energies_h, energies_d, sizeof(energies_h), hipMemcpyDeviceToHost, C.s));
-        OFDFT_HIP(hipStreamSynchronize(C.s));
+        // through the pinned buffer (past the slots the other round trips use)
+        double *pin = C.host + 48;
+        OFDFT_HIP(hipMemcpyAsync(pin, energies_d, sizeof(energies_h), hipMemcpyDeviceToHost, C.s));
+        C.wait();
+        std::memcpy(energies_h, pin, sizeof(energies_h));
         ++n_ev;
         return energies_h[OFDFT_E_SUM];
     }
@@ -616,7 +620,7 @@ template <typename T> struct Opt {
         }
         hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_RHO), rho_io, n);
         if (pot_out) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), pot_out, n);
-        OFDFT_HIP(hipStreamSynchronize(C.s));
+        C.wait();
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) {
             for (int i = 0; i < 16; ++i) stats[i] = 0.0;

@@ -466,7 +466,7 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
         }
     }
     fft_c2r<double>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
-    OFDFT_HIP(hipStreamSynchronize(s));
+    stream_wait(s);  // the staging buffer is reused by the next call
 }
 
 static double erfcinv_host(double y) {

@@ -120,11 +120,17 @@ class Optimization:
         self.converged = int(stats[0])
         self.mu = stats[3]
         self.rho = DirectField(grid, data=rho)
-        self.phi = DirectField(grid, data=torch.sqrt(torch.clamp(rho, min=0)))
         self.functional = type("Out", (), {})()
         self.functional.energy = stats[2]
         self.functional.potential = DirectField(grid, data=pot)
         return self.rho
+
+    @property
+    def phi(self):
+        """sqrt(rho) of the last result (computed when asked for)."""
+        if self.rho is None:
+            return None
+        return DirectField(self.rho.grid, data=torch.sqrt(torch.clamp(self.rho.data, min=0)))
 
     def __call__(self, guess_rho=None, calcType={"E", "V"}, guess_phi=None, lphi=False):
         return self.optimize_rho(guess_rho=guess_rho, guess_phi=guess_phi, lphi=lphi)
