@@ -267,6 +267,8 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
         fetch(0);
         land(0);
     }
+    // the forward bias of this lane's two output channels
+    const float bias_lane[2] = {MODE == 0 ? bias[m] : 0.0f, MODE == 0 ? bias[16 + m] : 0.0f};
     __syncthreads();
     floatx4 acc[4][2];
 #pragma unroll
@@ -316,12 +318,11 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
 #pragma unroll
                     for (int nt = 0; nt < 2; ++nt) {
                         const int co = nt * 16 + m;
-                        const float bco = bias[co];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const int z = zt * 16 + kq * 4 + j;
                             const int li = ((k + 1) * nz + z) * CH + co;
-                            const float r = fmaxf(acc[k][nt][j] + bco, 0.0f);
+                            const float r = fmaxf(acc[k][nt][j] + bias_lane[nt], 0.0f);
                             if (out1) {
                                 const long long e =
                                     ((long long)x * nyz + (long long)(y0 + k * d) * nz + z) * CH + co;

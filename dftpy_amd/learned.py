@@ -340,3 +340,45 @@ def learned_start(grid, ions, pseudo, net=None, use_net=True):
     vloc = pseudo.vreal_for(g64).data
     x0 = net.features(g64.plan(torch.float64), rho_s, vloc, rho0)
     return net.start_density(grid, x0, rho0, ne, use_net=use_net)
+
+
+def time_network(sizes=(96, 160), train_size=96, reps=5):
+    """HIP-event timings of the network: inference at each size and one
+    training step (forward, backward, weight gradients, Adam) at train_size,
+    random inputs and weights.  Returns {name: milliseconds}."""
+    out = {}
+    net = WarmStartNet()
+    for n in sizes:
+        nr = (n, n, n)
+        x = torch.randn(2, n ** 3, device=net.device)
+        net.forward(x, nr)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            net.forward(x, nr)
+        e1.record()
+        torch.cuda.synchronize()
+        out["inference_%d" % n] = e0.elapsed_time(e1) / reps
+        if n == train_size:
+            from .grid import DirectGrid
+            g = DirectGrid(np.eye(3) * 0.24 * n, nr, full=False, dtype=torch.float32)
+            plan = g.plan(torch.float32)
+            t = torch.randn(n ** 3, device=net.device) * 0.1
+            net.train_step(plan, x, nr, t, 1e-3)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                net.train_step(plan, x, nr, t, 1e-3)
+            e1.record()
+            torch.cuda.synchronize()
+            out["train_step_%d" % n] = e0.elapsed_time(e1) / reps
+    return out
+
+
+if __name__ == "__main__":
+    # python -m dftpy_amd.learned: timings of the network (see time_network)
+    for k, ms in time_network().items():
+        n = int(k.rsplit("_", 1)[1])
+        flops = 2 * NTAP * n ** 3 * (2 * C + 4 * C * C + C) * (3 if k.startswith("train") else 1)
+        print("%-16s %8.3f ms  %6.1f TFLOP/s" % (k, ms, flops / ms / 1e9), flush=True)
