@@ -67,8 +67,171 @@ __device__ __forceinline__ void decompose(const Geo &g, long long p, int &x, int
     x = (int)(r / g.ny);
 }
 
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+
 // ------------------------------------------------------------ layer 1 (2 -> 32)
-// The weights are read with workgroup-uniform addresses (scalar loads).
+// MFMA form: K = 27 taps x 2 channels = 54 (zero-padded to 64), FP16
+// operands (the inputs are FP32 features; FP16 keeps 11 bits, finer than the
+// bf16 the output is stored in) and FP32 accumulation.  The weights are the A
+// operand (rows = output channels) and the gathered neighbours the B operand
+// (columns = 16 consecutive points), so each lane ends with 4 consecutive
+// channels of one point and stores them as 8 bytes.  A wave strides over
+// 16-point tiles; the weights stay in registers.
+constexpr int KIN_PAD = 64;
+
+__global__ void __launch_bounds__(256) k_cnn_in_mfma(Geo g, const float *__restrict__ x0, const float *__restrict__ w,
+                                                     const float *__restrict__ b, __bf16 *__restrict__ h,
+                                                     long long ntiles) {
+    const int lane = threadIdx.x & 63, m = lane & 15, kq = lane >> 4;
+    const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    // A operands: W1[t][c][channel] at k = 2 t + c, rows = channels nt*16 + m
+    f16x8 wa[2][2];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kb * 32 + kq * 8 + e;
+                wa[kb][nt][e] = k < 2 * NTAP ? (_Float16)w[(k >> 1) * 64 + (k & 1) * 32 + nt * 16 + m] : (_Float16)0.0f;
+            }
+    float bl[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bl[nt][j] = b[nt * 16 + kq * 4 + j];
+    for (long long tile = wave0; tile < ntiles; tile += nwaves) {
+        const long long p = tile * 16 + m;
+        const long long pc = p < g.n ? p : g.n - 1;  // tail lanes compute a copy and store nothing
+        int x, y, z;
+        decompose(g, pc, x, y, z);
+        Nbr nb;
+        nb.init(g, x, y, z);
+        f16x8 xb[2];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kb * 32 + kq * 8 + e;
+                xb[kb][e] = k < 2 * NTAP ? (_Float16)x0[(k & 1) * g.n + nb.at(k >> 1)] : (_Float16)0.0f;
+            }
+        floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wa[kb][nt], xb[kb], acc[nt], 0, 0, 0);
+        if (p < g.n) {
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                bf16x4 o;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) o[j] = (__bf16)fmaxf(acc[nt][j] + bl[nt][j], 0.0f);
+                *reinterpret_cast<bf16x4 *>(h + p * CH + nt * 16 + kq * 4) = o;
+            }
+        }
+    }
+}
+
+// Row-staged form for nz % 16 == 0 and ny % (4 d) == 0: a workgroup owns 4
+// output rows y0 + k d of one x plane (as the middle-layer kernel) and stages
+// the 3 planes x 6 rows x 2 channels of features it reads as FP16 in LDS, so
+// the 54 neighbour values of a point come from LDS and no per-point index
+// division is needed.
+constexpr int IN_ROWS_THREADS = 256;
+constexpr int IN_ROWS_MAX_NZ = 512;
+
+__global__ void __launch_bounds__(IN_ROWS_THREADS)
+    k_cnn_in_rows(Geo g, const float *__restrict__ x0, const float *__restrict__ w, const float *__restrict__ b,
+                  __bf16 *__restrict__ h) {
+    extern __shared__ _Float16 xs[];  // [3 planes][6 rows][2 channels][nz]
+    const int nz = g.nz, d = g.d;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
+    const int x = blockIdx.y, gy = blockIdx.x;
+    const int y0 = (gy / d) * 4 * d + gy % d;
+    const long long nyz = (long long)g.ny * nz;
+    // stage: element e = ((i * 6 + r) * 2 + c) * nz + z
+    const int nstage = 36 * nz;
+    for (int e = threadIdx.x; e < nstage; e += IN_ROWS_THREADS) {
+        const int z = e % nz, q = e / nz;
+        const int c = q & 1, r = (q >> 1) % 6, i = q / 12;
+        const int xp = wrapi(x + (i - 1) * d, g.nx), yr = wrapi(y0 + (r - 1) * d, g.ny);
+        xs[e] = (_Float16)x0[(long long)c * g.n + (long long)xp * nyz + (long long)yr * nz + z];
+    }
+    f16x8 wa[2][2];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = kb * 32 + kq * 8 + e;
+                wa[kb][nt][e] = k < 2 * NTAP ? (_Float16)w[(k >> 1) * 64 + (k & 1) * 32 + nt * 16 + m] : (_Float16)0.0f;
+            }
+    float bl[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bl[nt][j] = b[nt * 16 + kq * 4 + j];
+    // this lane's 16 operand slots k = 2 t + c: LDS offset of (plane i, row
+    // j, channel c) relative to output row 0, and which z shift l they take
+    int off[2][8], lsel[2][8];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int kk = kb * 32 + kq * 8 + e;
+            const int t = kk >> 1, c = kk & 1;
+            off[kb][e] = kk < 2 * NTAP ? (((t / 9) * 6 + (t / 3) % 3) * 2 + c) * nz : -1;
+            lsel[kb][e] = t % 3;
+        }
+    // per-wave staging of one output tile (16 points x 32 channels, bf16)
+    // for coalesced 16-byte stores
+    bf16x4 *ot = reinterpret_cast<bf16x4 *>(xs + 36 * nz) + wave * 128;
+    __syncthreads();
+    const int tiles_per_row = nz / 16, ntile = 4 * tiles_per_row;
+    for (int tile = wave; tile < ntile; tile += IN_ROWS_THREADS / 64) {
+        const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
+        const int zc = zt * 16 + m;
+        const int zm = wrapi(zc - d, nz), zp = wrapi(zc + d, nz);
+        f16x8 xb[2];
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int zz = lsel[kb][e] == 0 ? zm : (lsel[kb][e] == 1 ? zc : zp);
+                xb[kb][e] = off[kb][e] >= 0 ? xs[off[kb][e] + k * 2 * nz + zz] : (_Float16)0.0f;
+            }
+        floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wa[kb][nt], xb[kb], acc[nt], 0, 0, 0);
+        // lane (m, kq) holds channels nt*16 + kq*4 .. +3 of point m
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            bf16x4 o;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[j] = (__bf16)fmaxf(acc[nt][j] + bl[nt][j], 0.0f);
+            ot[m * 8 + nt * 4 + kq] = o;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // lane l stores 16 bytes: point l / 4, channels (l % 4) * 8 .. +7
+        const long long p0 = (long long)x * nyz + (long long)(y0 + k * d) * nz + zt * 16;
+        const bf16x8 v = reinterpret_cast<const bf16x8 *>(ot)[lane];
+        reinterpret_cast<bf16x8 *>(h)[(p0 + (lane >> 2)) * 4 + (lane & 3)] = v;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// FP32 FMA form (the weights are read with workgroup-uniform addresses)
 __global__ void __launch_bounds__(256) k_cnn_in(Geo g, const float *__restrict__ x0, const float *__restrict__ w,
                                                 const float *__restrict__ b, __bf16 *__restrict__ h) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -831,13 +994,23 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
     return cnn_guard([&] {
         const Geo g = make_geo(nr, dil);
         OFDFT_REQUIRE(x0 && w && b && h, "null pointer");
-        hipLaunchKernelGGL(k_cnn_in, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g, x0, w, b,
-                           reinterpret_cast<__bf16 *>(h));
+        if (g.nz % 16 == 0 && g.nz <= IN_ROWS_MAX_NZ && g.ny % (4 * g.d) == 0) {
+            // staged features + one 1 KB output tile per wave
+            const size_t lds = (size_t)36 * g.nz * sizeof(_Float16) + (size_t)(IN_ROWS_THREADS / 64) * 1024;
+            hipLaunchKernelGGL(k_cnn_in_rows, dim3((unsigned)(g.ny / 4), (unsigned)g.nx), dim3(IN_ROWS_THREADS), lds,
+                               St(stream), g, x0, w, b, reinterpret_cast<__bf16 *>(h));
+        } else {
+            const long long ntiles = (g.n + 15) / 16;
+            // 8 tiles per wave amortise the weight loads
+            const unsigned blocks =
+                (unsigned)std::max<long long>(1, std::min<long long>((ntiles + 31) / 32, 1 << 20));
+            hipLaunchKernelGGL(k_cnn_in_mfma, dim3(blocks), dim3(256), 0, St(stream), g, x0, w, b,
+                               reinterpret_cast<__bf16 *>(h), ntiles);
+        }
         OFDFT_HIP(hipGetLastError());
     });
 }
 
-// allow a row-tiled launch above the default 64 KB of dynamic LDS
 // workgroups of the persistent row kernel: one per CU, a multiple of the 8
 // XCDs
 static int rows_workgroups() {
@@ -847,6 +1020,7 @@ static int rows_workgroups() {
     return std::max(ROWS_XCDS, cus / ROWS_XCDS * ROWS_XCDS);
 }
 
+// allow a row-tiled launch above the default 64 KB of dynamic LDS
 static void rows_lds_limit(const void *kernel, size_t bytes) {
     if (bytes > 64 * 1024)
         OFDFT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));

@@ -20,7 +20,9 @@ W6[t, ci].
 ``bf16=True`` rounds to bfloat16 (round to nearest even) exactly where the
 HIP path stores bfloat16: the mid-layer weights, the hidden activations
 h_l and the relu outputs r_l, and the gradients fed to the mid-layer
-products.  Everything else is FP64 here (FP32 accumulation on the GPU).
+products; and to FP16 where the HIP path's first layer takes FP16 matrix
+operands (the input features and W1 in the forward product).  Everything
+else is FP64 here (FP32 accumulation on the GPU).
 
 Pinned by: central finite differences of the loss against ``backward``
 in pure FP64 (``tests/test_oracle.py``).
@@ -37,6 +39,11 @@ def to_bf16(a):
     u = f.view(np.uint32).astype(np.uint64)
     r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def to_f16(a):
+    """Round to IEEE half precision (nearest even) and return as float64."""
+    return np.asarray(a, dtype=np.float64).astype(np.float16).astype(np.float64)
 
 
 def taps(d):
@@ -64,11 +71,13 @@ def init_params(seed=0, scale=1.0):
 def forward(p, x0, dil=(1, 1, 1, 1, 1, 1), bf16=False):
     """x0: (2, nx, ny, nz).  Returns (out (nx,ny,nz), cache)."""
     rb = to_bf16 if bf16 else (lambda a: a)
+    rh = to_f16 if bf16 else (lambda a: a)
     T1 = taps(dil[0])
+    xin, W1 = rh(x0), rh(p["W1"])
     pre = np.zeros(x0.shape[1:] + (C,))
     for t, off in enumerate(T1):
         for c in range(2):
-            pre += shift(x0[c], off)[..., None] * p["W1"][t, c][None, None, None, :]
+            pre += shift(xin[c], off)[..., None] * W1[t, c][None, None, None, :]
     h = rb(np.maximum(pre + p["b1"], 0.0))
     hs, rs = [h], [h]
     for l in range(2, 6):
