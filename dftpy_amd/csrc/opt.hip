@@ -63,8 +63,10 @@ template <typename T> __global__ void k_init_phi(const T *rho, T *phi, long long
     write_partials<2>(acc, partials);
 }
 
-// phi *= f ; rho = phi^2
-template <typename T> __global__ void k_scale_square(T *phi, T *rho, double f, long long n) {
+// phi *= sqrt(s[0] / s[1]) ; rho = phi^2 (the normalisation factor from
+// device sums, optimization.py:305-316)
+template <typename T> __global__ void k_scale_square(T *phi, T *rho, const double *s, long long n) {
+    const double f = sqrt(s[0] / s[1]);
     GRID_LOOP(i, n) {
         const T v = (T)((double)phi[i] * f);
         phi[i] = v;
@@ -365,16 +367,14 @@ template <typename T> struct Opt {
         ++n_v;
     }
     // mu = sum(V rho) / sum(rho) (optimization.py:537-539) kept on device at
-    // scal[2], then res = (V - mu) phi; one host round trip for both
-    double mu_and_residual(const T *pot, const T *rho, const T *phi, T *res, double *rnorm) {
+    // scal[2] (sums at scal[18], scal[19]), then res = (V - mu) phi.  No host
+    // round trip: the host needs mu only for the final report.
+    void mu_and_residual(const T *pot, const T *rho, const T *phi, T *res) {
         hipLaunchKernelGGL((k_mu_sums<T>), C.grid(), dim3(256), 0, C.s, pot, rho, n, P.red);
         C.reduce_final_cols(2, 18);
         hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 2, 18, 19);
         hipLaunchKernelGGL((k_residual<T>), C.grid(), dim3(256), 0, C.s, pot, phi, res, P.scal + 2, n, P.red);
         C.reduce_final_cols(1, 20);
-        const double *h = C.to_host(3, 18);
-        if (rnorm) *rnorm = h[2] * P.dV;
-        return h[0] / h[1];
     }
 
     // out = IFFT[M^-1(|G|) FFT[in]] with the plan's radial preconditioner table
@@ -488,7 +488,9 @@ template <typename T> struct Opt {
     // from device scalars.  Returns theta0 = sqrt(|p|^2 / N) before scaling and
     // sum(V phi p) after it (the line search's slope at theta = 0 is 2 dV times
     // that), in one host round trip.
-    void orthonormalize(double &theta0, double &vphip) {
+    // The same copy also brings back the energies of the last E+V
+    // evaluation (scal[32..]), which the first line search needs.
+    void orthonormalize(double &theta0, double &vphip, double *energies = nullptr) {
         T *p = F(F_P), *phi = F(F_PHI);
         hipLaunchKernelGGL((k_pphi<T>), C.grid(), dim3(256), 0, C.s, p, phi, n, P.red);
         C.reduce_final_cols(2, 21);  // scal[21] = sum p phi, scal[22] = sum phi^2
@@ -498,9 +500,11 @@ template <typename T> struct Opt {
         hipLaunchKernelGGL((k_scale_norm<T>), C.grid(), dim3(256), 0, C.s, p, P.scal, 22, 23, n);
         hipLaunchKernelGGL((k_dot3<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), phi, p, n, P.red);
         C.reduce_final_cols(1, 24);  // scal[24] = sum V phi p
-        const double *h = C.to_host(4, 21);
+        const double *h = C.to_host(energies ? 32 + OFDFT_E_COUNT - 21 : 4, 21);
         theta0 = std::sqrt(h[2] / h[1]);
         vphip = h[3];
+        if (energies)
+            for (int i = 0; i < OFDFT_E_COUNT; ++i) energies[i] = h[32 - 21 + i];
     }
 
     // value and derivative at theta (ValueAndDerivative, optimization.py:227-279);
@@ -528,13 +532,16 @@ template <typename T> struct Opt {
         const double c1 = opts[4], c2 = opts[5], xtol = opts[6];
         const int ncheck = (int)opts[7];
         const auto t0 = std::chrono::steady_clock::now();
-        // initial phi (optimization.py:305-316)
+        // initial phi (optimization.py:305-316), normalised on the device; the
+        // starting energy reaches the host with the first orthonormalisation
         hipLaunchKernelGGL((k_init_phi<T>), C.grid(), dim3(256), 0, C.s, rho_io, F(F_PHI), n, P.red);
-        const double *h = C.fetch(2);
-        const double factor = std::sqrt(h[0] / h[1]);
-        hipLaunchKernelGGL((k_scale_square<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_RHO), factor, n);
-        double energy = eval_ev(F(F_RHO), F(F_POT));
-        double mu = mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES), nullptr);
+        C.reduce_final_cols(2, 16);
+        hipLaunchKernelGGL((k_scale_square<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_RHO), P.scal + 16, n);
+        eval<T>(P, F(F_RHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_POT), energies_d, C.s);
+        ++n_ev;
+        mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES));
+        bool energy_pending = true;
+        double energy = 0.0;
         std::vector<double> hist{energy};
         std::vector<double> times{std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()};
         double theta = 0.1;
@@ -564,7 +571,11 @@ template <typename T> struct Opt {
                 }
             }
             double theta0, vphip;
-            orthonormalize(theta0, vphip);
+            orthonormalize(theta0, vphip, energy_pending ? energies_h : nullptr);
+            if (energy_pending) {
+                energy = hist[0] = energies_h[OFDFT_E_SUM];
+                energy_pending = false;
+            }
             if (vphip > 0) {  // uphill: restart from the steepest descent direction
                 hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
                 orthonormalize(theta0, vphip);
@@ -604,7 +615,7 @@ template <typename T> struct Opt {
                 swap_fields(F_P, F_DIR_PREV);
                 ++nres;
             }
-            mu = mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES), nullptr);
+            mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES));
             hist.push_back(energy);
             times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
             (void)nd;
@@ -620,7 +631,11 @@ template <typename T> struct Opt {
         }
         hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_RHO), rho_io, n);
         if (pot_out) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), pot_out, n);
-        C.wait();
+        // mu of the final density (scal[2], set by the last residual) and, when
+        // no iteration ran, the starting energy
+        const double *hm = C.to_host(32 + OFDFT_E_COUNT - 2, 2);
+        const double mu = hm[0];
+        if (energy_pending) energy = hist[0] = hm[32 - 2 + OFDFT_E_SUM];
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (stats) {
             for (int i = 0; i < 16; ++i) stats[i] = 0.0;
