@@ -145,9 +145,9 @@ __global__ void k_tn_ap(const T *pot1, const T *phi1, const T *res0, const T *p,
 // from the device (pAp reduced by the previous launch), so the inner CG needs one host round trip per iteration;
 // negative curvature (pAp < 0) leaves dir and tres untouched for the host.
 template <typename T>
-__global__ void k_tn_update_dev(T *dir, T *tres, const T *p, const T *Ap, double r0, const double *pap, long long n,
-                                double *partials) {
-    const double pa = *pap;
+__global__ void k_tn_update_dev(T *dir, T *tres, const T *p, const T *Ap, const double *rzp, const double *pap,
+                                long long n, double *partials) {
+    const double pa = *pap, r0 = *rzp;
     double acc[1] = {0};
     if (!(pa < 0.0)) {
         const double alpha = r0 / pa;
@@ -416,17 +416,16 @@ template <typename T> struct Opt {
             apply_precond(tres, z);
             hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, tres, z, n, P.red);
         };
+        // |r|^2 -> scal[25] (stopping tests), r.z -> scal[26]; the host reads
+        // both with the first iteration's scalars
         hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
-        C.reduce_final_cols(1, 16);  // |r|^2, for the stopping tests
+        C.reduce_final_cols(1, 25);
         if (pre) {
             precond_dot();
-            C.reduce_final_cols(1, 17);  // r.z
+            C.reduce_final_cols(1, 26);
             hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, z, tp, n);
         }
-        const double *h0 = C.to_host(pre ? 2 : 1, 16);
-        double r0 = h0[0];
-        double rz = pre ? h0[1] : r0;
-        const double rconv = r0 * 0.1;
+        double r0 = 0.0, rz = 0.0, rconv = 0.0;
         bool best_zero = true;
         std::vector<double> rl{1e10};
         int it = 0;
@@ -440,16 +439,24 @@ template <typename T> struct Opt {
                 hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps,
                                    Ap, n, P.red);
             }
-            // pAp -> scal[16]; the update takes alpha from there; r1 -> scal[17]; r1.z1 -> scal[18]
+            // pAp -> scal[16]; the update takes alpha = r.z / pAp from the
+            // device (r.z of the start, then of the previous iteration);
+            // r1 -> scal[17]; r1.z1 -> scal[18]
             C.reduce_final_cols(1, 16);
-            hipLaunchKernelGGL((k_tn_update_dev<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, rz, P.scal + 16,
-                               n, P.red);
+            const int rz_slot = it == 0 ? (pre ? 26 : 25) : (pre ? 18 : 17);
+            hipLaunchKernelGGL((k_tn_update_dev<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, P.scal + rz_slot,
+                               P.scal + 16, n, P.red);
             C.reduce_final_cols(1, 17);
             if (pre) {
                 precond_dot();
                 C.reduce_final_cols(1, 18);
             }
-            const double *hh = C.to_host(pre ? 3 : 2, 16);
+            const double *hh = C.to_host(it == 0 ? 11 : (pre ? 3 : 2), 16);
+            if (it == 0) {
+                r0 = hh[25 - 16];
+                rz = pre ? hh[26 - 16] : r0;
+                rconv = r0 * 0.1;
+            }
             const double pAp = hh[0], r1 = hh[1], rz1 = pre ? hh[2] : hh[1];
             if (pAp < 0.0) {
                 if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, rz / pAp, tp, 0.0, dir, n);
