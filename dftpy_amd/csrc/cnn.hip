@@ -9,9 +9,11 @@
 // fields.  Tap t = (i*3 + j)*3 + k reads the input at p + d*(i-1, j-1, k-1)
 // with periodic wrap (d = dilation of the layer).
 //
-//   h1 = relu(conv(x0; W1) + b1)                          FP32 FMA (K = 54)
+//   h1 = relu(conv(x0; W1) + b1)                          FP16 MFMA (K = 54)
 //   h_l = h_{l-1} + relu(conv(h_{l-1}; W_l) + b_l), l=2..5  BF16 MFMA, FP32 acc
-//   out = conv(h5; W6) + b6                               FP32 FMA (K = 864)
+//   out = conv(h5; W6) + b6                               FP16 MFMA projection
+//                                                         onto 27 tap planes
+//                                                         (FP16) + gather
 //
 // The 32 -> 32 layers are an implicit GEMM per tap: D[p][co] += X[p+off][ci]
 // W[co][ci] with v_mfma_f32_16x16x32_bf16 (16 points x 16 channels x 32 input
@@ -231,34 +233,6 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
     }
 }
 
-// FP32 FMA form (the weights are read with workgroup-uniform addresses)
-__global__ void __launch_bounds__(256) k_cnn_in(Geo g, const float *__restrict__ x0, const float *__restrict__ w,
-                                                const float *__restrict__ b, __bf16 *__restrict__ h) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.n) return;
-    int x, y, z;
-    decompose(g, p, x, y, z);
-    Nbr nb;
-    nb.init(g, x, y, z);
-    float acc[CH];
-#pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = b[c];
-#pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
-        const int q = nb.at(t);
-        const float v0 = x0[q], v1 = x0[g.n + q];
-#pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] += w[t * 64 + c] * v0 + w[t * 64 + 32 + c] * v1;
-    }
-    bf16x8 *dst = reinterpret_cast<bf16x8 *>(h + p * CH);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        bf16x8 o;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) o[e] = (__bf16)fmaxf(acc[q * 8 + e], 0.0f);
-        dst[q] = o;
-    }
-}
 
 // ------------------------------------------------------- layers 2..5 (MFMA)
 // MODE 0 (forward):  r = relu(conv(x) + bias); out0 (bf16) = x + r; out1 (bf16) = r
@@ -567,42 +541,92 @@ __global__ void __launch_bounds__(256) k_cnn_out(Geo g, const __bf16 *__restrict
     out[p] = acc;
 }
 
-// Output layer as projection + gather: u[t][q] = sum_c W6[t][c] h[q][c]
-// (planar, 27 fields), then out[p] = b + sum_t u[t][p + off_t].  Each hidden
-// point is read once and every u value once, instead of 27 reads of h.
-__global__ void __launch_bounds__(256) k_cnn_out_proj(Geo g, const __bf16 *__restrict__ h,
-                                                      const float *__restrict__ w, float *__restrict__ u) {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= g.n) return;
-    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h) + q * 4;
-    float hc[CH];
+// MFMA projection: u[t][q] = sum_c W6[t][c] h[q][c] as 27 FP16 planes (the
+// hidden state converts to FP16 exactly in its range, W6 is rounded to FP16).
+// The hidden points are the A operand (rows = 16 consecutive points), W6 the
+// B operand (columns = taps).  A wave takes 64 consecutive points (4 tiles),
+// gathers the 27 x 64 results in LDS and writes each tap plane's 128 bytes
+// with 16-byte stores.  W6 stays in registers.
+typedef __attribute__((__vector_size__(4 * sizeof(_Float16)))) _Float16 f16x4;
+constexpr int PROJ_THREADS = 256;
+
+__global__ void __launch_bounds__(PROJ_THREADS) k_cnn_out_proj16(Geo g, const __bf16 *__restrict__ h,
+                                                                 const float *__restrict__ w,
+                                                                 _Float16 *__restrict__ u, long long ngroups) {
+    __shared__ f16x8 stage[PROJ_THREADS / 64][NTAP * 8];  // [wave][tap][8 x 8 points]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
+    const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    f16x8 wb[2];
 #pragma unroll
-    for (int c4 = 0; c4 < 4; ++c4) {
-        const bf16x8 v = hv[c4];
+    for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) hc[c4 * 8 + e] = (float)v[e];
-    }
+        for (int e = 0; e < 8; ++e) {
+            const int t = nt * 16 + m;
+            wb[nt][e] = t < NTAP ? (_Float16)w[t * CH + kq * 8 + e] : (_Float16)0.0f;
+        }
+    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h);
+    _Float16 *st = reinterpret_cast<_Float16 *>(stage[wave]);
+    const bool vec = (g.n & 7) == 0;
+    for (long long grp = wave0; grp < ngroups; grp += nwaves) {
+        const long long p0 = grp * 64;
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
-        float s = 0.0f;
+        for (int i = 0; i < 4; ++i) {
+            const long long p = p0 + i * 16 + m;
+            const bf16x8 a8 = hv[(p < g.n ? p : g.n - 1) * 4 + kq];
+            f16x8 a;
 #pragma unroll
-        for (int c = 0; c < CH; ++c) s += w[t * CH + c] * hc[c];
-        u[(long long)t * g.n + q] = s;
+            for (int e = 0; e < 8; ++e) a[e] = (_Float16)(float)a8[e];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const floatx4 d =
+                    __builtin_amdgcn_mfma_f32_16x16x32_f16(a, wb[nt], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                const int t = nt * 16 + m;  // lane holds points i*16 + kq*4 + j of tap t
+                if (t < NTAP)
+                    *reinterpret_cast<f16x4 *>(st + t * 64 + i * 16 + kq * 4) =
+                        f16x4{(_Float16)d[0], (_Float16)d[1], (_Float16)d[2], (_Float16)d[3]};
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // chunk c = tap * 8 + k8 holds points k8*8 .. +7 of that tap
+        for (int c = lane; c < NTAP * 8; c += 64) {
+            const int t = c >> 3, k8 = c & 7;
+            const long long q = p0 + k8 * 8;
+            const f16x8 v = stage[wave][c];
+            _Float16 *dst = u + (long long)t * g.n + q;
+            if (vec && q + 7 < g.n) {
+                *reinterpret_cast<f16x8 *>(dst) = v;
+            } else {
+                for (int j = 0; j < 8; ++j)
+                    if (q + j < g.n) dst[j] = v[j];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
-__global__ void __launch_bounds__(256) k_cnn_out_gather(Geo g, const float *__restrict__ u, const float *__restrict__ b,
-                                                        float *__restrict__ out) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.n) return;
-    int x, y, z;
-    decompose(g, p, x, y, z);
-    Nbr nb;
-    nb.init(g, x, y, z);
-    float acc = b[0];
+// out[p] = b + sum_t u[t][p + off_t]: one workgroup per (x, y) row, threads
+// along z, so the neighbour rows are workgroup-uniform
+__global__ void __launch_bounds__(256) k_cnn_out_gather16(Geo g, const _Float16 *__restrict__ u,
+                                                          const float *__restrict__ b, float *__restrict__ out) {
+    const int y = blockIdx.x, x = blockIdx.y;
+    const long long nyz = (long long)g.ny * g.nz;
+    const long long bx[3] = {wrapi(x - g.d, g.nx) * nyz, x * nyz, wrapi(x + g.d, g.nx) * nyz};
+    const long long by[3] = {(long long)wrapi(y - g.d, g.ny) * g.nz, (long long)y * g.nz,
+                             (long long)wrapi(y + g.d, g.ny) * g.nz};
+    const float b0 = b[0];
+    for (int z = threadIdx.x; z < g.nz; z += blockDim.x) {
+        const int bz[3] = {wrapi(z - g.d, g.nz), z, wrapi(z + g.d, g.nz)};
+        float acc = b0;
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) acc += u[(long long)t * g.n + nb.at(t)];
-    out[p] = acc;
+        for (int t = 0; t < NTAP; ++t)
+            acc += (float)u[(long long)t * g.n + bx[t / 9] + by[(t / 3) % 3] + bz[t % 3]];
+        out[bx[1] + by[1] + z] = acc;
+    }
 }
 
 // dh[q][c] = sum_t W6[t][c] dout[q - off_t];  g = mask > 0 ? dh : 0 (bf16)
@@ -1086,10 +1110,16 @@ int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, 
         const Geo g = make_geo(nr, dil);
         OFDFT_REQUIRE(h && w && b && out, "null pointer");
         if (work) {
-            hipLaunchKernelGGL(k_cnn_out_proj, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g,
-                               reinterpret_cast<const __bf16 *>(h), w, work);
+            // the work buffer (27 N floats) holds the 27 FP16 tap planes
+            _Float16 *u = reinterpret_cast<_Float16 *>(work);
+            const long long ngroups = (g.n + 63) / 64;  // 64-point groups, 4 per wave on average
+            const unsigned blocks = (unsigned)std::max<long long>(1, std::min<long long>((ngroups + 15) / 16, 1 << 20));
+            hipLaunchKernelGGL(k_cnn_out_proj16, dim3(blocks), dim3(PROJ_THREADS), 0, St(stream), g,
+                               reinterpret_cast<const __bf16 *>(h), w, u, ngroups);
             OFDFT_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_cnn_out_gather, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g, work, b, out);
+            const unsigned threads = (unsigned)std::min(256, (g.nz + 63) / 64 * 64);
+            hipLaunchKernelGGL(k_cnn_out_gather16, dim3((unsigned)g.ny, (unsigned)g.nx), dim3(threads), 0, St(stream),
+                               g, u, b, out);
         } else {
             hipLaunchKernelGGL(k_cnn_out, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g,
                                reinterpret_cast<const __bf16 *>(h), w, b, out);

@@ -250,7 +250,8 @@ int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, 
  * channel (FP32).  Tap t = (i*3+j)*3+k reads p + dil*(i-1, j-1, k-1) with
  * periodic wrap.  nr: grid shape; every pointer is device memory.
  * ------------------------------------------------------------------------- */
-/* h = relu(conv(x0; w) + b): x0 [2][N] FP32, w [27][2][32] FP32, b [32], h [N][32] bf16 */
+/* h = relu(conv(x0; w) + b): x0 [2][N] FP32, w [27][2][32] FP32, b [32], h [N][32] bf16.
+ * Computed on MFMA with x0 and w rounded to FP16 and FP32 accumulation. */
 int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w, const float *b, void *h,
                       void *stream);
 /* 32 -> 32 layer on MFMA (bf16 in, FP32 accumulation), w [27][32 out][32 in] bf16.
@@ -262,7 +263,9 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
 int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const void *w, const float *bias,
                        const float *resid, const void *mask, void *out0, void *out1, void *stream);
 /* out = conv(h; w) + b: h [N][32] bf16, w [27][32] FP32, b [1], out [N] FP32.
- * work (27 N floats) selects the projection + gather form; NULL the direct one. */
+ * work (27 N floats of capacity) selects the projection + gather form: the 27
+ * projected tap planes are kept there in FP16 (h and w rounded to FP16 on MFMA);
+ * NULL selects the direct FP32 form. */
 int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, const float *b, float *out,
                        float *work, void *stream);
 /* dh [N][32] FP32 = d(sum dout*out)/dh; gout (bf16) = mask > 0 ? dh : 0 */

@@ -20,9 +20,10 @@ W6[t, ci].
 ``bf16=True`` rounds to bfloat16 (round to nearest even) exactly where the
 HIP path stores bfloat16: the mid-layer weights, the hidden activations
 h_l and the relu outputs r_l, and the gradients fed to the mid-layer
-products; and to FP16 where the HIP path's first layer takes FP16 matrix
-operands (the input features and W1 in the forward product).  Everything
-else is FP64 here (FP32 accumulation on the GPU).
+products; and to FP16 where the HIP path's forward uses FP16: the matrix
+operands of the first layer (input features and W1) and of the output
+projection (h5 and W6), and the 27 projected tap planes of the output layer.
+Everything else is FP64 here (FP32 accumulation on the GPU).
 
 Pinned by: central finite differences of the loss against ``backward``
 in pure FP64 (``tests/test_oracle.py``).
@@ -90,8 +91,10 @@ def forward(p, x0, dil=(1, 1, 1, 1, 1, 1), bf16=False):
         hs.append(h)
         rs.append(rb(r))
     out = np.zeros(x0.shape[1:])
+    h16, W6 = rh(h), rh(p["W6"])
     for t, off in enumerate(taps(dil[5])):
-        out += shift(h, off) @ p["W6"][t]
+        # tap plane u_t = h @ W6[t] at every point, then shifted
+        out += shift(rh(h16 @ W6[t]), off)
     out += p["b6"][0]
     return out, dict(hs=hs, rs=rs, x0=x0, dil=dil, bf16=bf16)
 
