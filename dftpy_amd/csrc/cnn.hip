@@ -410,9 +410,23 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
     floatx4 acc[4][2];
 #pragma unroll
     for (int k = 0; k < 4; ++k) acc[k][0] = acc[k][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // the 16-byte stores of a finished item (forward): issued at the top of
+    // the next stage, before its loads, so the wait that lands those loads in
+    // LDS does not also wait for fresh stores
+    auto copy_out = [&](int q) {
+        int x, y0, xp, ix;
+        item_of(q, x, y0, xp, ix);
+        bf16x8 *ov = reinterpret_cast<bf16x8 *>(out0);
+        const bf16x8 *src = lds + (q & 1) * BUF;
+        for (int e = threadIdx.x; e < 4 * NZ * 4; e += NT) {
+            const int k = e / (NZ * 4), rem = e - k * NZ * 4;
+            ov[((long long)x * nyz + (long long)(y0 + k * d) * NZ) * 4 + rem] = src[(k + 1) * NZ * 4 + rem];
+        }
+    };
 #pragma unroll 1
     for (int q = 0; q < nq; ++q) {
         const int cur = q & 1;
+        if (MODE == 0 && q % 3 == 0 && q > 0) copy_out(q - 1);
         if (q + 1 < nq) fetch(q + 1);
         const bf16x8 *rows = lds + cur * BUF;
         const bf16x8 *wl = rows + NLOAD;
@@ -469,13 +483,7 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
                         }
                     }
                 }
-                __syncthreads();
-                bf16x8 *ov = reinterpret_cast<bf16x8 *>(out0);
-                const bf16x8 *src = lds + cur * BUF;
-                for (int e = threadIdx.x; e < 4 * NZ * 4; e += NT) {
-                    const int k = e / (NZ * 4), rem = e - k * NZ * 4;
-                    ov[((long long)x * nyz + (long long)(y0 + k * d) * NZ) * 4 + rem] = src[(k + 1) * NZ * 4 + rem];
-                }
+                // the rows go out at the top of the next stage (copy_out)
             } else {
                 // backward-data: out0 = acc + resid (FP32), out1 = out0 where
                 // the mask is positive (bf16); per row, resid and mask are
@@ -509,9 +517,15 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
 #pragma unroll
             for (int k = 0; k < 4; ++k) acc[k][0] = acc[k][1] = floatx4{0.f, 0.f, 0.f, 0.f};
         }
-        if (q + 1 < nq) land(cur ^ 1);
+        if (q + 1 < nq) {
+            // the buffer landed now is the one copy_out read at the top of
+            // this stage: every wave must be past those reads
+            if (MODE == 0 && q % 3 == 0 && q > 0) __syncthreads();
+            land(cur ^ 1);
+        }
         __syncthreads();
     }
+    if (MODE == 0 && nq > 0) copy_out(nq - 1);
 }
 
 // ------------------------------------------------------------ layer 6 (32 -> 1)
