@@ -1,6 +1,6 @@
 """Many cells of uneven grid sizes optimised concurrently on one GPU.
 
-    python -m dftpy_amd.batch [--ncell 48] [--workers 8] [--seed 0] [--sequential]
+    python -m dftpy_amd.batch [--ncell 48] [--workers 4] [--seed 0] [--sequential]
 
 Cells: fcc Al n x n x m supercells with n, m drawn from {2, 3, 4} (32-256 atoms),
 isotropic strain U(-3%, +3%) and Gaussian displacements of sigma = 0.1 bohr,
@@ -91,20 +91,31 @@ class _Worker:
                         seconds=time.perf_counter() - t0)
 
 
-def run_batch(cells, workers=8, weights=None):
+def run_batch(cells, workers=4, weights=None):
     """Optimise all cells with `workers` concurrent streams; returns the
     per-cell records (in input order) and the wall time."""
     readpp = ReadPseudo({"Al": systems.empty_core_recpot()})
     ws = [_Worker(readpp, weights) for _ in range(workers)]
-    # warm the plans of every shape (autotuning, tables) outside the timed region
+    # warm every worker on every shape outside the timed region: plans
+    # (autotuning, tables) and one full cell, so the device buffers of each
+    # shape are in the stream's allocator cache and no allocation (which
+    # synchronises the device) happens while the cells run
     shapes = {}
     for ions, nr in cells:
         shapes.setdefault(nr, ions)
-    for w in ws:
-        for nr, ions in shapes.items():
+
+    def warm(w):
+        for nr, ions in sorted(shapes.items(), key=lambda kv: -int(np.prod(kv[0]))):
             with torch.cuda.stream(w.stream):
                 w.grid_pair(ions, nr)[0].plan(torch.float64)
                 w.grid_pair(ions, nr)[1].plan(torch.float32)
+            w.run(ions, nr)
+
+    warmers = [threading.Thread(target=warm, args=(w,)) for w in ws]
+    for t in warmers:
+        t.start()
+    for t in warmers:
+        t.join()
     torch.cuda.synchronize()
     q = queue.Queue()
     for i, c in enumerate(cells):
@@ -140,7 +151,7 @@ def run_batch(cells, workers=8, weights=None):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--ncell", type=int, default=48)
-    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--sequential", action="store_true", help="also time the cells one after another")
     args = ap.parse_args(argv)
