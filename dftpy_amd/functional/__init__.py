@@ -122,6 +122,12 @@ def configure(plan, funcs, rho_grid, eion=0.0):
             vloc = f.vreal_for(rho_grid)
     rho0 = kedf["rho0"]
     plan.kedf_settings = {k: kedf[k] for k in ("x", "y", "alpha", "beta")}
+    # the plan keeps its settings between calls: skip the library calls when
+    # nothing changed (vloc is read through its pointer at evaluation time)
+    vptr = vloc.data.data_ptr() if vloc is not None else None
+    key = (terms, tuple((k, kedf[k]) for k in sorted(kedf)), vptr, float(eion))
+    if getattr(plan, "config_key", None) == key:
+        return terms, vloc
     _lib.check(_lib.lib.ofdft_set_kedf(plan.handle, float(kedf["x"]), float(kedf["y"]), float(kedf["alpha"]),
                                        float(kedf["beta"]), float(rho0) if rho0 else -1.0,
                                        float(kedf["sigma"]) if kedf["sigma"] else -1.0))
@@ -132,8 +138,9 @@ def configure(plan, funcs, rho_grid, eion=0.0):
                                            1.15))
         plan.lmgp_ready = True
     _lib.check(_lib.lib.ofdft_set_terms(plan.handle, terms))
-    _lib.check(_lib.lib.ofdft_set_vloc(plan.handle, vloc.data.data_ptr() if vloc is not None else None))
+    _lib.check(_lib.lib.ofdft_set_vloc(plan.handle, vptr))
     _lib.check(_lib.lib.ofdft_set_eion(plan.handle, float(eion)))
+    plan.config_key = key
     return terms, vloc
 
 
