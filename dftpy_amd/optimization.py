@@ -6,7 +6,7 @@ import torch
 
 from . import _lib
 from .field import DirectField
-from .functional import AbstractFunctional, configure
+from .functional import AbstractFunctional, FunctionalOutput, configure
 from .functional.total_functional import TotalFunctional
 from .math_utils import uniform_response_kernel
 
@@ -120,9 +120,8 @@ class Optimization:
         self.converged = int(stats[0])
         self.mu = stats[3]
         self.rho = DirectField(grid, data=rho)
-        self.functional = type("Out", (), {})()
-        self.functional.energy = stats[2]
-        self.functional.potential = DirectField(grid, data=pot)
+        # the last evaluation, as optimization.py:517 keeps it
+        self.functional = FunctionalOutput(name="TOTAL", energy=stats[2], potential=DirectField(grid, data=pot))
         return self.rho
 
     @property
