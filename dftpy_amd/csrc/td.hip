@@ -155,7 +155,7 @@ template <typename T> struct CN {
         std::vector<double> ones(K, 1.0);
         reduce_final(P.red, nb, K, P.scal + 64, ones.data(), s);
         OFDFT_HIP(hipMemcpyAsync(host, P.scal + 64, K * sizeof(double), hipMemcpyDeviceToHost, s));
-        OFDFT_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
         return host;
     }
     // kin = -1/2 lap_sigma(x)
@@ -213,7 +213,7 @@ template <typename T> struct CN {
                 rr_i = nr_i;
             }
         }
-        OFDFT_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
         if (info) {
             info[0] = status;
             info[1] = k;
@@ -359,7 +359,7 @@ template <typename T> struct CNBatch {
         hipLaunchKernelGGL(k_tdb_reduce, dim3(nb * K), dim3(256), 0, s, P.red, nblk, K, P.scal + 128);
         OFDFT_HIP(hipGetLastError());
         OFDFT_HIP(hipMemcpyAsync(host, P.scal + 128, (size_t)nb * K * sizeof(double), hipMemcpyDeviceToHost, s));
-        OFDFT_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
         return host;
     }
     void kinetic(const cpx<T> *x) {
@@ -439,7 +439,7 @@ template <typename T> struct CNBatch {
                 nactive += bs.active[r];
             }
         }
-        OFDFT_HIP(hipStreamSynchronize(s));
+        stream_wait(s);
         if (info)
             for (int r = 0; r < nb; ++r) {
                 info[4 * r] = status[r];
