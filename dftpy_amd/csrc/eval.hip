@@ -122,6 +122,47 @@ static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long
     OFDFT_HIP(hipGetLastError());
 }
 
+// Energy-only evaluation: the spectral terms as sums over the half spectrum,
+// E = sum_G w(kz) f(G) |F(G)|^2 / N (Parseval; w = 2 for the kz columns whose
+// mirror is not stored), with f the same multiplier the potential path
+// applies.  One workgroup per stride of (x, y) rows, threads along kz.  The
+// sums land in the k_post partial columns of vW (times y), WT and Hartree
+// (times 1/2), which k_post leaves at zero when it has no fields.
+template <typename T>
+__global__ void __launch_bounds__(128) k_gspace_energy(const cpx<T> *__restrict__ spec, int ns, GParams p,
+                                                       double y, double *partials) {
+    __shared__ double sm[16];
+    double tkf = 0.0, wtfac = 0.0;
+    if (p.s_wt >= 0) wt_consts(p, tkf, wtfac);
+    double acc_vw = 0.0, acc_wt = 0.0, acc_h = 0.0;
+    const int rows = p.nx * p.ny;
+    for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+        const int ix = row / p.ny, iy = row - ix * p.ny;
+        for (int iz = threadIdx.x; iz < p.nzc; iz += blockDim.x) {
+            const double w = (iz == 0 || ((p.nz & 1) == 0 && iz == p.nz / 2)) ? 1.0 : 2.0;
+            const long long e = (long long)row * p.pitch + iz;
+            for (int sl = 0; sl < ns; ++sl) {
+                const int kind = p.kind[sl];
+                const cpx<T> v = spec[sl * p.bs + e];
+                const double a2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                const double t = w * gfactor<T>(p, kind, ix, iy, iz, tkf, wtfac) * a2;
+                if (kind == 0)
+                    acc_vw += t;
+                else if (kind == 1)
+                    acc_wt += t;
+                else
+                    acc_h += t;
+            }
+        }
+    }
+    const double sums[3] = {block_sum(acc_vw, sm), block_sum(acc_wt, sm), block_sum(acc_h, sm)};
+    if (threadIdx.x == 0) {
+        partials[(long long)blockIdx.x * 7 + 1] = y * sums[0];
+        partials[(long long)blockIdx.x * 7 + 2] = sums[1];
+        partials[(long long)blockIdx.x * 7 + 4] = 0.5 * sums[2];
+    }
+}
+
 struct PostParams {
     int s_vw, s_wt, s_h;
     int b56;  // beta == 5/6
@@ -240,6 +281,74 @@ __global__ void __launch_bounds__(kFinalThreads)
     }
 }
 
+// E only: forward transforms of the pipeline spectra, their energies summed
+// in G space (k_gspace_energy) and the point-wise terms by k_post without
+// fields; no inverse transforms and no potential
+template <typename T>
+static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s_h, double *energies,
+                        hipStream_t s) {
+    cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
+    double *rho0_cache = P.scal + 0;
+    Prologue pro{};
+    if (s_vw >= 0) pro.code[s_vw] = OP_SQRT_RHOM;
+    if (s_wt >= 0) pro.code[s_wt] = OP_POW_BETA;
+    if (s_h >= 0) pro.code[s_h] = OP_IDENTITY;
+    pro.beta = P.beta;
+    pro.b56 = beta_is_5_6(P.beta);
+    const bool auto_rho0 = (s_wt >= 0) && !(P.rho0 > 0);
+    const long long pitch = pipe_pitch(P);
+    const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+    fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch, false);
+    if (auto_rho0) {
+        const int nb0 = r2c_num_partials(P, ns);
+        hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb0, 1.0 / (double)P.N, rho0_cache);
+        OFDFT_HIP(hipGetLastError());
+    }
+    GParams gp;
+    for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
+    for (int i = 0; i < 4; ++i) gp.kind[i] = -1;
+    if (s_vw >= 0) gp.kind[s_vw] = 0;
+    if (s_wt >= 0) gp.kind[s_wt] = 1;
+    if (s_h >= 0) gp.kind[s_h] = 2;
+    gp.nx = P.nr[0];
+    gp.ny = P.nr[1];
+    gp.nz = P.nr[2];
+    gp.nzc = P.nzc;
+    gp.full = P.full;
+    gp.scale = 1.0 / (double)P.N;
+    gp.s_vw = s_vw;
+    gp.s_wt = s_wt;
+    gp.s_h = s_h;
+    gp.sigma = P.sigma;
+    gp.lbda = P.x;
+    gp.alpha = P.alpha;
+    gp.beta = P.beta;
+    gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
+    gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
+    gp.bs = ncp;
+    gp.pitch = (int)pitch;
+    PostParams pp;
+    pp.s_vw = pp.s_wt = pp.s_h = -1;
+    pp.terms = P.terms;
+    pp.x = P.x;
+    pp.y = P.y;
+    pp.alpha = P.alpha;
+    pp.beta = P.beta;
+    pp.b56 = beta_is_5_6(P.beta);
+    pp.n = P.N;
+    pp.bs = P.N;
+    pp.want_v = 0;
+    pp.want_e = 1;
+    const T *vloc = (P.terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
+    const int nb = std::min(reduce_blocks(P.N), 1024);
+    hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, nullptr, vloc, nullptr, P.red, pp);
+    OFDFT_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_gspace_energy<T>), dim3(nb), dim3(128), 0, s, spec, ns, gp, P.y, P.red);
+    OFDFT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
+    OFDFT_HIP(hipGetLastError());
+}
+
 template <typename T>
 void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s) {
     const unsigned terms = P.terms;
@@ -250,6 +359,12 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
     cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
     T *work = reinterpret_cast<T *>(P.work);
     double *rho0_cache = P.scal + 0;
+    const bool energy_only = (calc & OFDFT_CALC_E) && energies != nullptr && !((calc & OFDFT_CALC_V) && pot) &&
+                             !(terms & OFDFT_TERM_LMGP);
+    if (energy_only && ns > 0) {
+        eval_energy<T>(P, rho, ns, s_vw, s_wt, s_h, energies, s);
+        return;
+    }
     if (ns > 0) {
         Prologue pro{};
         if (s_vw >= 0) pro.code[s_vw] = OP_SQRT_RHOM;

@@ -188,3 +188,33 @@ def test_fused_x_pass_matches_separate_passes(nr, full, dtype):
         assert abs(e - e_ref) <= etol * e_scale
         assert np.max(np.abs(v - v_ref)) <= vtol * np.max(np.abs(v_ref))
     assert abs(outs[0][0] - outs[1][0]) <= etol * e_scale
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("full", [True, False])
+@pytest.mark.parametrize("nr", [(24, 24, 24), (30, 27, 25)])
+def test_energy_only_matches_energy_and_potential(dtype, full, nr):
+    """calcType={'E'} sums the FFT terms in G space (Parseval) instead of
+    transforming back; every part equals the E+V path's (and, in FP64, the
+    oracle's).  Covers even and odd sizes (fixed and generic FFTs), full and
+    half grids, automatic and explicit rho0."""
+    from dftpy_amd import DirectField, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    ions, _ = systems.c1_system()
+    lat = ions.cell
+    rho = _c1_density(nr, lat, seed=3)
+    g = _dg(lat, nr, full, dtype)
+    pp = Functional(type="PSEUDO", grid=g, ions=ions, PP_list={"Al": systems.empty_core_recpot()})
+    for ke in (Functional(type="KEDF", name="WT"), Functional(type="KEDF", name="WT", rho0=0.03),
+               Functional(type="KEDF", name="TFVW")):
+        tot = TotalFunctional(KE=ke, XC=Functional(type="XC", name="LDA"), HARTREE=Functional(type="HARTREE"),
+                              PSEUDO=pp)
+        f = DirectField(g, data=torch.as_tensor(rho, dtype=dtype, device="cuda"))
+        e_only = tot(f, calcType={"E"})
+        both = tot(f, calcType={"E", "V"})
+        tol = 1e-11 if dtype == torch.float64 else 2e-6
+        for k in ("TF", "vW", "NL", "XC", "HARTREE", "PSEUDO", "N"):
+            assert abs(e_only.parts[k] - both.parts[k]) <= tol * max(1.0, abs(both.parts[k])), (k, e_only.parts[k],
+                                                                                                 both.parts[k])
+        assert abs(e_only.energy - both.energy) <= tol * max(1.0, abs(both.energy))
+        assert not hasattr(e_only, "potential")
