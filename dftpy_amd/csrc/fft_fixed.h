@@ -229,6 +229,8 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
                     if (gp.sigma > 0) f *= exp(-gg * sig);
                 } else if (kind == 1) {
                     f = lindhard_t<T>(sqrt(gg) * itkf, lb, (T)1) * wsc;
+                } else if (kind == 3) {
+                    f = (T)(radial_table(gp, sqrt((double)gg)) * gp.scale);
                 } else {
                     f = (yz0 && e == 0) ? (T)0 : (T)(4.0 * M_PI) * sc / gg;
                 }

@@ -43,7 +43,7 @@ __device__ __forceinline__ double lindhard(double eta, double lbda, double mu) {
 
 struct GParams {
     double B[9];
-    int kind[4];  // per spectrum slot: 0 vW, 1 WT, 2 Hartree, -1 none
+    int kind[4];  // per spectrum slot: 0 vW, 1 WT, 2 Hartree, 3 radial table, -1 none
     int nx, ny, nz, nzc;
     int pitch;  // row pitch of the spectra (>= nzc)
     int full;
@@ -54,7 +54,20 @@ struct GParams {
     double rho0;             // used when rho0_dev == nullptr
     const double *rho0_dev;  // cached rho0 on device
     long long bs;
+    // kind 3: a radial table of |G| (linear interpolation, constant past the
+    // end), e.g. the optimizer's preconditioner
+    const double *tab;
+    int tab_n;
+    double tab_inv_dq;
 };
+
+__device__ __forceinline__ double radial_table(const GParams &p, double g) {
+    const double t = g * p.tab_inv_dq;
+    if (t >= (double)(p.tab_n - 1)) return p.tab[p.tab_n - 1];
+    const int i = (int)t;
+    const double f = t - i;
+    return p.tab[i] * (1.0 - f) + p.tab[i + 1] * f;
+}
 
 __device__ __forceinline__ int ffreq(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
 __device__ __forceinline__ int wrapf(int f, int n) { return ((n & 1) == 0 && f == n / 2) ? -n / 2 : f; }
@@ -90,6 +103,7 @@ __device__ __forceinline__ double kernel_val(const GParams &p, int kind, double 
         return v;
     }
     if (kind == 1) return (double)lindhard_t<T>((T)(sqrt(gg) / tkf), (T)p.lbda, (T)1) * wtfac;
+    if (kind == 3) return radial_table(p, sqrt(gg));
     return g0 ? 0.0 : 4.0 * M_PI / gg;
 }
 

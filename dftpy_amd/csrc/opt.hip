@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "dcsrch.h"
+#include "gspace.h"
 #include "internal.h"
 #include "reduce.h"
 
@@ -36,6 +37,10 @@ void plan_aux_free(Plan &P) {
 template <typename T> static T *fld(Plan &P, int f) { return reinterpret_cast<T *>(P.aux) + (size_t)f * P.N; }
 
 template <typename T> __global__ void k_fill(T *y, T v, long long n);
+
+template <typename T>
+bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
+                  long long vp, const void *tw, const GParams &gp, hipStream_t s);
 
 // ------------------------------------------------------------------ kernels
 #define GRID_LOOP(i, n) \
@@ -377,11 +382,39 @@ template <typename T> struct Opt {
         C.reduce_final_cols(1, 20);
     }
 
-    // out = IFFT[M^-1(|G|) FFT[in]] with the plan's radial preconditioner table
+    // out = IFFT[M^-1(|G|) FFT[in]] with the plan's radial preconditioner table;
+    // for size-specialised x lengths the multiply runs inside the x pass
+    // (forward x, multiply, inverse x in one kernel), as in the evaluation
     void apply_precond(const T *in, T *out) {
         const long long pitch = pipe_pitch(P);
         const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
         cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
+        const bool fused = P.use_fixed && P.nr[0] > 1 && P.nr[1] * P.nr[2] > 1 && fixed_size(P.nr[0]);
+        if (fused) {
+            GParams gp{};
+            for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
+            gp.kind[0] = 3;
+            gp.kind[1] = gp.kind[2] = gp.kind[3] = -1;
+            gp.nx = P.nr[0];
+            gp.ny = P.nr[1];
+            gp.nz = P.nr[2];
+            gp.nzc = P.nzc;
+            gp.pitch = (int)pitch;
+            gp.full = P.full;
+            gp.scale = 1.0 / (double)P.N;
+            gp.s_vw = gp.s_wt = gp.s_h = -1;
+            gp.bs = ncp;
+            gp.tab = P.tn_pre;
+            gp.tab_n = P.tn_pre_n;
+            gp.tab_inv_dq = 1.0 / P.tn_pre_dq;
+            fft_r2c<T>(P, in, spec, 1, 0, ncp, 1.0, nullptr, nullptr, C.s, pitch, true);
+            OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], P.var_cols[0], spec, 1, ncp, (long long)P.nr[1] * pitch, pitch,
+                                          P.nzc, P.tw[0], gp, C.s),
+                          "fused x pass unavailable");
+            OFDFT_HIP(hipGetLastError());
+            fft_c2r<T>(P, spec, spec, out, 1, ncp, ncp, P.N, 1.0, C.s, pitch, true);
+            return;
+        }
         fft_r2c<T>(P, in, spec, 1, 0, ncp, 1.0, nullptr, nullptr, C.s, pitch);
         PrecondParams pp;
         for (int i = 0; i < 9; ++i) pp.B[i] = P.B[i];
