@@ -333,11 +333,6 @@ struct Ctx {
         wait();
         return host;
     }
-    void set_scalar(int slot, double v) {
-        host[32] = v;
-        OFDFT_HIP(hipMemcpyAsync(P.scal + slot, host + 32, sizeof(double), hipMemcpyHostToDevice, s));
-        wait();
-    }
 };
 
 template <typename T> struct Opt {
@@ -357,16 +352,6 @@ template <typename T> struct Opt {
     // swaps pointers instead of copying
     T *F(int f) { return fld<T>(P, map[f]); }
 
-    double eval_ev(const T *rho, T *pot) {
-        eval<T>(P, rho, OFDFT_CALC_E | OFDFT_CALC_V, pot, energies_d, C.s);
-        // through the pinned buffer (past the slots the other round trips use)
-        double *pin = C.host + 48;
-        OFDFT_HIP(hipMemcpyAsync(pin, energies_d, sizeof(energies_h), hipMemcpyDeviceToHost, C.s));
-        C.wait();
-        std::memcpy(energies_h, pin, sizeof(energies_h));
-        ++n_ev;
-        return energies_h[OFDFT_E_SUM];
-    }
     void eval_v(const T *rho, T *pot) {
         eval<T>(P, rho, OFDFT_CALC_V, pot, nullptr, C.s);
         ++n_v;
