@@ -441,7 +441,9 @@ template <typename T> struct Opt {
         if (pre) {
             precond_dot();
             C.reduce_final_cols(1, 26);
-            hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, z, tp, n);
+            swap_fields(F_Z, F_TP);  // p0 = z0 (z is recomputed before its next use)
+            tp = F(F_TP);
+            z = F(F_Z);
         }
         double r0 = 0.0, rz = 0.0, rconv = 0.0;
         bool best_zero = true;
@@ -575,7 +577,7 @@ template <typename T> struct Opt {
             int nd = 1;
             if (method == 0) {
                 nd = direction_tn(maxfun, 1.0e-9, hvp);
-                hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR), F(F_P), n);
+                swap_fields(F_DIR, F_P);  // p takes the direction (dir is reset by the next TN start)
             } else {
                 // CG (math_utils.py:395-423); first step and missing history: -res
                 if (nres < 1) {
