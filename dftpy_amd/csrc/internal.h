@@ -123,6 +123,7 @@ struct Plan {
     // pinned host buffer and a device buffer, grown on demand and reused
     char *stg_h = nullptr, *stg_d = nullptr;
     size_t stg_cap = 0;
+    hipEvent_t stg_evt = nullptr;  // recorded after each staging upload
     // PME B-spline moduli of this grid (x, y, z concatenated) for pme_order
     void *pme_b = nullptr;
     int pme_order = 0;

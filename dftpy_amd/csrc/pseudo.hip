@@ -341,11 +341,20 @@ static std::vector<double> bspline_bm(int n, int len, int order) {
 }
 
 // Host inputs of one setup call go through the plan's staging pair: filled
-// in pinned host memory and sent with one asynchronous copy.  Every routine
-// that stages synchronises its stream before returning, so the next call may
-// refill the host buffer.
+// in pinned host memory and sent with one asynchronous copy.  An event
+// recorded after each upload guards the host buffer: the next call waits for
+// it (normally long complete) before refilling, so a routine that only
+// produces device results need not wait for its stream.
 static char *stage_host(Plan &P, size_t bytes) {
+    if (P.stg_evt) {
+        hipError_t e;
+        while ((e = hipEventQuery(P.stg_evt)) == hipErrorNotReady) {
+        }
+        OFDFT_HIP(e);
+    }
     if (bytes > P.stg_cap) {
+        // kernels of an earlier call may still read the device copy
+        if (P.stg_d) OFDFT_HIP(hipDeviceSynchronize());
         if (P.stg_h) OFDFT_HIP(hipHostFree(P.stg_h));
         if (P.stg_d) OFDFT_HIP(hipFree(P.stg_d));
         P.stg_h = P.stg_d = nullptr;
@@ -360,6 +369,8 @@ static char *stage_host(Plan &P, size_t bytes) {
 
 static char *stage_upload(Plan &P, size_t bytes, hipStream_t s) {
     OFDFT_HIP(hipMemcpyAsync(P.stg_d, P.stg_h, bytes, hipMemcpyHostToDevice, s));
+    if (!P.stg_evt) OFDFT_HIP(hipEventCreateWithFlags(&P.stg_evt, hipEventDisableTiming));
+    OFDFT_HIP(hipEventRecord(P.stg_evt, s));
     return P.stg_d;
 }
 
@@ -374,6 +385,8 @@ struct StageLayout {
 };
 
 void plan_setup_free(Plan &P) {
+    if (P.stg_evt) (void)hipEventSynchronize(P.stg_evt);
+    if (P.stg_evt) (void)hipEventDestroy(P.stg_evt);
     if (P.stg_h) (void)hipHostFree(P.stg_h);
     if (P.stg_d) (void)hipFree(P.stg_d);
     if (P.pme_b) (void)hipFree(P.pme_b);
@@ -466,7 +479,6 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
         }
     }
     fft_c2r<double>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
-    stream_wait(s);  // the staging buffer is reused by the next call
 }
 
 static double erfcinv_host(double y) {

@@ -200,7 +200,7 @@ class LocalPseudo(AbstractFunctional):
         ions = self.ions
         pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
         frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
-        spec = np.array([keys.index(s) for s in ions.symbols], dtype=np.int32)
+        spec = ions.species_index(keys)
         keep = [q0, dq, nco, qmax, vsmall, dqs, coefs, cptr, pos, frac, spec]
         args = (ns, _lib.dptr(q0), _lib.dptr(dq), _lib.iptr(nco), cptr, _lib.dptr(qmax), _lib.dptr(vsmall),
                 _lib.dptr(dqs), ions.nat, _lib.dptr(pos), _lib.dptr(frac), _lib.iptr(spec))

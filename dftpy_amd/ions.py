@@ -54,6 +54,16 @@ class Ions:
     def get_positions(self):
         return self.positions.copy()
 
+    def species_index(self, keys):
+        """int32 index of each atom's symbol in the sorted list ``keys``."""
+        keys = np.asarray(keys)
+        sym = np.asarray(self.symbols)
+        idx = np.searchsorted(keys, sym)
+        if idx.size and (idx.max() >= keys.size or not np.array_equal(keys[idx], sym)):
+            missing = sorted(set(self.symbols) - set(keys.tolist()))
+            raise ValueError("no table for species %s" % missing)
+        return idx.astype(np.int32)
+
     @property
     def symbols_uniq(self):
         return np.sort(np.unique(self.symbols))

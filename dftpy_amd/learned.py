@@ -124,7 +124,7 @@ def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
     q0, dq, nco, qmax, vsmall, dqs, coefs, cptr = cache[ck]
     pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
     frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
-    spec = np.array([keys.index(s) for s in ions.symbols], dtype=np.int32)
+    spec = ions.species_index(keys)
     out = torch.empty(tuple(int(n) for n in grid.nrR), dtype=torch.float64, device=grid.device)
     _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, _lib.dptr(q0), _lib.dptr(dq), _lib.iptr(nco), cptr,
                                        _lib.dptr(qmax), _lib.dptr(vsmall), _lib.dptr(dqs), ions.nat, _lib.dptr(pos),
