@@ -236,18 +236,90 @@ template <typename T> __global__ void k_neg(const T *x, T *y, long long n) {
     GRID_LOOP(i, n) y[i] = -x[i];
 }
 
-// CG directions (math_utils.py:395-423): sums r1(r1 - r0), d(r1 - r0), r1^2, r0^2
+// CG directions (math_utils.py:395-423): sums r1(r1 - r0), d(r1 - r0), r1^2,
+// r0^2, d r0
 template <typename T>
 __global__ void k_cg_sums(const T *r1, const T *r0, const T *d, long long n, double *partials) {
-    double acc[4] = {0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     GRID_LOOP(i, n) {
         const double a = r1[i], b = r0[i], dd = d[i];
         acc[0] += a * (a - b);
         acc[1] += dd * (a - b);
         acc[2] += a * a;
         acc[3] += b * b;
+        acc[4] += dd * b;
     }
-    write_partials<4>(acc, partials);
+    write_partials<5>(acc, partials);
+}
+
+// CG-CD as math_utils.py:411-412 writes it: beta = -(r1 ** 2) / sum(d r0) is a
+// field (the numerator is not summed), p = -r1 + beta d
+template <typename T> __global__ void k_cg_cd(const T *r1, const T *d, double sdr0, T *p, long long n) {
+    GRID_LOOP(i, n) {
+        const double r = r1[i];
+        const double beta = -(r * r) / sdr0;
+        p[i] = (T)(-r + beta * (double)d[i]);
+    }
+}
+
+// L-BFGS (math_utils.py:374-455).  A pair: s = phi - phi_old,
+// y = res - res_old, and the partial sums of y s
+template <typename T>
+__global__ void k_lbfgs_pair(const T *phi, const T *phi_old, const T *res, const T *res_old, T *s, T *y, long long n,
+                             double *partials) {
+    double acc[1] = {0};
+    GRID_LOOP(i, n) {
+        const T a = phi[i] - phi_old[i], b = res[i] - res_old[i];
+        s[i] = a;
+        y[i] = b;
+        acc[0] += (double)b * (double)a;
+    }
+    write_partials<1>(acc, partials);
+}
+
+// scal[k] = 1 / scal[k]
+__global__ void k_recip(double *scal, int k) {
+    if (threadIdx.x == 0) scal[k] = 1.0 / scal[k];
+}
+
+// first loop: alpha = rho_i sum(s_i q) (scal[rho] * scal[dot]), q -= alpha y_i;
+// alpha is kept at scal[alpha_out] for the second loop
+template <typename T>
+__global__ void k_lbfgs_q(T *q, const T *y, double *scal, int rho, int dot, int alpha_out, long long n) {
+    const double alpha = scal[rho] * scal[dot];
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[alpha_out] = alpha;
+    GRID_LOOP(i, n) q[i] = (T)((double)q[i] - alpha * (double)y[i]);
+}
+
+// second loop: beta = rho_i sum(y_i d), d += s_i (alpha_i - beta); alpha_in < 0
+// stands for alpha = 0 (the oldest pair, which the first loop does not visit)
+template <typename T>
+__global__ void k_lbfgs_d(T *d, const T *s, const double *scal, int rho, int dot, int alpha_in, long long n) {
+    const double beta = scal[rho] * scal[dot];
+    const double a = (alpha_in < 0 ? 0.0 : scal[alpha_in]) - beta;
+    GRID_LOOP(i, n) d[i] = (T)((double)d[i] + (double)s[i] * a);
+}
+
+// sums s y and y y (gamma = s.y / y.y of the newest pair)
+template <typename T> __global__ void k_sy_yy(const T *s, const T *y, long long n, double *partials) {
+    double acc[2] = {0, 0};
+    GRID_LOOP(i, n) {
+        const double a = s[i], b = y[i];
+        acc[0] += a * b;
+        acc[1] += b * b;
+    }
+    write_partials<2>(acc, partials);
+}
+
+// x *= scal[a] / scal[b]
+template <typename T> __global__ void k_scale_ratio(T *x, const double *scal, int a, int b, long long n) {
+    const double f = scal[a] / scal[b];
+    GRID_LOOP(i, n) x[i] = (T)(f * (double)x[i]);
+}
+
+// x *= f
+template <typename T> __global__ void k_scale(T *x, double f, long long n) {
+    GRID_LOOP(i, n) x[i] = (T)(f * (double)x[i]);
 }
 
 // p *= sqrt(scal[a] / scal[b])  (normalise p to sum p^2 = scal[a] with scal[b] = sum p^2)
@@ -552,12 +624,84 @@ template <typename T> struct Opt {
     // b takes a's contents (a becomes scratch)
     void swap_fields(int a, int b) { std::swap(map[a], map[b]); }
 
+    // L-BFGS memory (math_utils.py:374-392): up to bound + 1 (s, y) pairs in
+    // slots of one device buffer, oldest first in `order`; rho_i = 1 / (y_i s_i)
+    // of slot k sits at scal[LB_RHO + k], the first loop's alpha_i at
+    // scal[LB_ALPHA + i], dot products at scal[LB_DOT..]
+    static constexpr int LB_RHO = 160, LB_ALPHA = 176, LB_DOT = 192, LB_MAX_PAIRS = 16;
+    struct Lbfgs {
+        T *buf = nullptr;
+        int bound = 0;
+        double h0 = 0.0;
+        std::vector<int> order, free_slots;
+        ~Lbfgs() {
+            if (buf) (void)hipFree(buf);
+        }
+    };
+    T *lb_s(Lbfgs &L, int k) { return L.buf + (size_t)(2 * k) * n; }
+    T *lb_y(Lbfgs &L, int k) { return L.buf + (size_t)(2 * k + 1) * n; }
+
+    void lbfgs_update(Lbfgs &L) {
+        if ((int)L.order.size() > L.bound) {
+            L.free_slots.push_back(L.order.front());
+            L.order.erase(L.order.begin());
+        }
+        const int k = L.free_slots.back();
+        L.free_slots.pop_back();
+        // phi_old is F_NPHI and res_old F_RES_PREV after the accept swaps
+        hipLaunchKernelGGL((k_lbfgs_pair<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_NPHI), F(F_RES),
+                           F(F_RES_PREV), lb_s(L, k), lb_y(L, k), n, P.red);
+        C.reduce_final_cols(1, LB_RHO + k);
+        hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, C.s, P.scal, LB_RHO + k);
+        L.order.push_back(k);
+    }
+
+    // two-loop recursion (math_utils.py:431-455) into F_P, scalars on the device
+    void lbfgs_direction(Lbfgs &L) {
+        T *q = F(F_P);
+        hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), q, n);
+        const int m = (int)L.order.size();
+        for (int i = m - 1; i >= 1; --i) {  // the reference's first loop stops before the oldest pair
+            const int k = L.order[i];
+            hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, lb_s(L, k), q, n, P.red);
+            C.reduce_final_cols(1, LB_DOT);
+            hipLaunchKernelGGL((k_lbfgs_q<T>), C.grid(), dim3(256), 0, C.s, q, lb_y(L, k), P.scal, LB_RHO + k,
+                               LB_DOT, LB_ALPHA + i, n);
+        }
+        if (L.h0 == 0.0) {
+            if (m >= 1) {  // gamma = s.y / y.y of the newest pair; gamma = 1 without pairs
+                const int k = L.order[m - 1];
+                hipLaunchKernelGGL((k_sy_yy<T>), C.grid(), dim3(256), 0, C.s, lb_s(L, k), lb_y(L, k), n, P.red);
+                C.reduce_final_cols(2, LB_DOT + 1);
+                hipLaunchKernelGGL((k_scale_ratio<T>), C.grid(), dim3(256), 0, C.s, q, P.scal, LB_DOT + 1,
+                                   LB_DOT + 2, n);
+            }
+        } else {
+            hipLaunchKernelGGL((k_scale<T>), C.grid(), dim3(256), 0, C.s, q, L.h0, n);
+        }
+        for (int i = 0; i < m; ++i) {
+            const int k = L.order[i];
+            hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, lb_y(L, k), q, n, P.red);
+            C.reduce_final_cols(1, LB_DOT);
+            hipLaunchKernelGGL((k_lbfgs_d<T>), C.grid(), dim3(256), 0, C.s, q, lb_s(L, k), P.scal, LB_RHO + k,
+                               LB_DOT, i == 0 ? -1 : LB_ALPHA + i, n);
+        }
+    }
+
     int run(T *rho_io, T *pot_out, int method, int hvp, const double *opts, double *stats, double *history,
             int hcap) {
         const double econv = opts[0];
         const int maxiter = (int)opts[1], maxfun = (int)opts[2], maxls = (int)opts[3];
         const double c1 = opts[4], c2 = opts[5], xtol = opts[6];
         const int ncheck = (int)opts[7];
+        Lbfgs lb;
+        if (method == OFDFT_OPT_LBFGS) {
+            lb.bound = (int)opts[8];
+            lb.h0 = opts[9];
+            OFDFT_REQUIRE(lb.bound >= 0 && lb.bound + 1 <= LB_MAX_PAIRS, "optimize: maxcor must be in [0, 15]");
+            OFDFT_HIP(hipMalloc(&lb.buf, (size_t)2 * (lb.bound + 1) * n * sizeof(T)));
+            for (int k = lb.bound; k >= 0; --k) lb.free_slots.push_back(k);
+        }
         const auto t0 = std::chrono::steady_clock::now();
         // initial phi (optimization.py:305-316), normalised on the device; the
         // starting energy reaches the host with the first orthonormalisation
@@ -575,26 +719,38 @@ template <typename T> struct Opt {
         int converged = 2, it = 0, nres = 0;
         for (it = 1; it < maxiter; ++it) {
             int nd = 1;
-            if (method == 0) {
+            if (method == OFDFT_OPT_TN) {
                 nd = direction_tn(maxfun, 1.0e-9, hvp);
                 swap_fields(F_DIR, F_P);  // p takes the direction (dir is reset by the next TN start)
+            } else if (method == OFDFT_OPT_LBFGS) {
+                lbfgs_direction(lb);
             } else {
-                // CG (math_utils.py:395-423); first step and missing history: -res
-                if (nres < 1) {
+                // CG (math_utils.py:395-423); first step and missing history: -res;
+                // GD (math_utils.py:426-428) always -res
+                if (nres < 1 || method == OFDFT_OPT_GD) {
                     hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
                 } else {
                     hipLaunchKernelGGL((k_cg_sums<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_RES_PREV),
                                        F(F_DIR_PREV), n, P.red);
-                    const double *q = C.fetch(4);
-                    double beta;
-                    if (method == 1)
-                        beta = q[0] / q[1];
-                    else if (method == 2)
-                        beta = std::max(q[0] / q[3], 0.0);
-                    else
-                        beta = q[2] / q[3];
-                    hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR_PREV), F(F_P), n);
-                    hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, -1.0, F(F_RES), beta, F(F_P), n);
+                    const double *q = C.fetch(5);
+                    if (method == OFDFT_OPT_CG_CD) {
+                        hipLaunchKernelGGL((k_cg_cd<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_DIR_PREV), q[4],
+                                           F(F_P), n);
+                    } else {
+                        double beta;
+                        if (method == OFDFT_OPT_CG_HS)
+                            beta = q[0] / q[1];
+                        else if (method == OFDFT_OPT_CG_PR)
+                            beta = std::max(q[0] / q[3], 0.0);
+                        else if (method == OFDFT_OPT_CG_DY)
+                            beta = q[2] / q[1];
+                        else if (method == OFDFT_OPT_CG_LS)
+                            beta = q[0] / q[4];
+                        else
+                            beta = q[2] / q[3];
+                        hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR_PREV), F(F_P), n);
+                        hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, -1.0, F(F_RES), beta, F(F_P), n);
+                    }
                 }
             }
             double theta0, vphip;
@@ -637,12 +793,13 @@ template <typename T> struct Opt {
             swap_fields(F_NRHO, F_RHO);
             swap_fields(F_NPOT, F_POT);
             energy = f;
-            if (method != 0) {
+            if (method != OFDFT_OPT_TN) {
                 swap_fields(F_RES, F_RES_PREV);
                 swap_fields(F_P, F_DIR_PREV);
                 ++nres;
             }
             mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES));
+            if (method == OFDFT_OPT_LBFGS) lbfgs_update(lb);
             hist.push_back(energy);
             times.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
             (void)nd;
@@ -692,7 +849,7 @@ extern "C" int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method
     Plan *P = reinterpret_cast<Plan *>(plan);
     try {
         OFDFT_REQUIRE(P && rho && opts, "optimize: bad arguments");
-        OFDFT_REQUIRE(method >= 0 && method <= 3, "optimize: unknown method");
+        OFDFT_REQUIRE(method >= OFDFT_OPT_TN && method <= OFDFT_OPT_LBFGS, "optimize: unknown method");
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
         if (P->prec == OFDFT_FP64) {
             Opt<double> o(*P, s);

@@ -27,7 +27,7 @@ from .math_utils import ecut2nr
 from .optimization import Optimization
 
 KEDF_KEYS = ("x", "y", "alpha", "beta", "sigma", "rho0")
-OPT_KEYS = ("econv", "maxiter", "maxfun", "maxls", "c1", "c2", "xtol", "algorithm", "vector")
+OPT_KEYS = ("econv", "maxiter", "maxfun", "maxls", "c1", "c2", "xtol", "algorithm", "vector", "h0")
 HA2EV = Units.Ha
 # Ha/bohr^3 -> eV/A^3 -> GPa (1 eV/A^3 = e * 1e21 GPa with e in coulomb)
 HA_BOHR3_TO_GPA = Units.Ha / Units.Bohr ** 3 * Units._e * 1.0e21

@@ -10,7 +10,18 @@ from .functional import AbstractFunctional, FunctionalOutput, configure
 from .functional.total_functional import TotalFunctional
 from .math_utils import uniform_response_kernel
 
-METHODS = {"TN": 0, "CG-HS": 1, "CG": 1, "CG-PR": 2, "CG-FR": 3}
+METHODS = {"TN": 0, "CG-HS": 1, "CG": 1, "CG-PR": 2, "CG-FR": 3, "CG-DY": 4, "CG-CD": 5, "CG-LS": 6, "GD": 7,
+           "LBFGS": 8}
+
+
+def _method_code(method):
+    """Method name -> ofdft_optimize code.  Any other "CG..." name takes the
+    Fletcher-Reeves beta, the fall-through branch of math_utils.py:420-421."""
+    if method in METHODS:
+        return METHODS[method]
+    if method[0:2] == "CG":
+        return METHODS["CG-FR"]
+    raise NotImplementedError("optimization method %s is not implemented" % method)
 PRECOND_POINTS = 4096
 
 
@@ -48,8 +59,7 @@ class Optimization:
             raise AttributeError("Must provide an energy evaluator")
         if not isinstance(EnergyEvaluator, AbstractFunctional):
             raise TypeError("EnergyEvaluator must be a dftpy_amd functional")
-        if optimization_method not in METHODS:
-            raise NotImplementedError("optimization method %s is not implemented" % optimization_method)
+        _method_code(optimization_method)
         o = self.optimization_options
         if o["algorithm"] != "EMM" or o["vector"] != "Orthogonalization":
             raise NotImplementedError("only the EMM algorithm with orthogonalised rotations is implemented")
@@ -101,15 +111,19 @@ class Optimization:
         elif getattr(plan, "tn_precond_key", None) is not None:
             _lib.check(_lib.lib.ofdft_set_tn_precond(plan.handle, 0, 0.0, None))
             plan.tn_precond_key = None
-        opts = np.array([o["econv"], o["maxiter"], o["maxfun"], o["maxls"], o["c1"], o["c2"], o["xtol"], o["ncheck"]],
-                        dtype=np.float64)
+        h0 = 0.0
+        if self.optimization_method == "LBFGS":
+            # optimization.py:346 reads optimization_options["h0"], which has no default
+            h0 = float(o["h0"]) if o["h0"] else 0.0
+        opts = np.array([o["econv"], o["maxiter"], o["maxfun"], o["maxls"], o["c1"], o["c2"], o["xtol"], o["ncheck"],
+                         o["maxcor"], h0], dtype=np.float64)
         stats = np.zeros(16)
         hcap = int(o["maxiter"]) + 2
         hist = np.zeros(2 * hcap)
         rho = rho_in.data.clone().contiguous()
         pot = torch.empty_like(rho)
         _lib.check(_lib.lib.ofdft_optimize(plan.handle, rho.data_ptr(), pot.data_ptr(),
-                                           METHODS[self.optimization_method], hvp, _lib.dptr(opts), _lib.dptr(stats),
+                                           _method_code(self.optimization_method), hvp, _lib.dptr(opts), _lib.dptr(stats),
                                            _lib.dptr(hist), hcap, _lib.stream()))
         del vloc
         n = int(stats[1]) + 1

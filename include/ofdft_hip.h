@@ -208,14 +208,31 @@ int ofdft_functional_stress(ofdft_plan *plan, const void *rho, const double *ene
  * (preconditioned CG; the stopping tests stay on the unpreconditioned
  * residual).  n <= 0 removes it.  Setup call (copies the table). */
 int ofdft_set_tn_precond(ofdft_plan *plan, int n, double dq, const double *table);
+enum {
+    OFDFT_OPT_TN = 0,
+    OFDFT_OPT_CG_HS,
+    OFDFT_OPT_CG_PR,
+    OFDFT_OPT_CG_FR,
+    OFDFT_OPT_CG_DY,
+    OFDFT_OPT_CG_CD,
+    OFDFT_OPT_CG_LS,
+    OFDFT_OPT_GD,
+    OFDFT_OPT_LBFGS
+};
 /* Density optimizer (Optimization.optimize_rho, optimization/optimization.py:281-520).
- * method: 0 = TN (get_direction_TN :91-147), 1 = CG-HS, 2 = CG-PR, 3 = CG-FR
- * (math_utils.py:395-423).  Line search: Moré-Thuente dcsrch on theta in
+ * method: OFDFT_OPT_TN (get_direction_TN :91-147), the CG directions of
+ * math_utils.py:395-423 (CG-HS, -PR, -FR, -DY, -CD, -LS, with the reference's
+ * CG-LS denominator sum(d r0) and its field-valued CG-CD beta), OFDFT_OPT_GD
+ * (-residual, :426-428) or OFDFT_OPT_LBFGS (two-loop recursion :431-455 with
+ * at most maxcor + 1 pairs, :374-392; its buffer of 2 (maxcor + 1) fields is
+ * allocated for the call).  Line search: Moré-Thuente dcsrch on theta in
  * [0, pi] (math_utils.py:76-106).  hvp: 0 = forward difference with
  * eps = 1e-9 as the reference; 1 = analytic linearised potential (exact
  * Hessian-vector product, used in FP32).  rho: device, N reals, overwritten
  * with the optimised density; pot (device, N) receives the final potential.
- * opts (host, 8 doubles): econv, maxiter, maxfun, maxls, c1, c2, xtol, ncheck.
+ * opts (host, 10 doubles): econv, maxiter, maxfun, maxls, c1, c2, xtol, ncheck,
+ * maxcor, h0 (the last two read by L-BFGS only; h0 = 0 scales the first loop's
+ * result by s.y / y.y of the newest pair, as a falsy H0 does in the reference).
  * stats (host, 16 doubles): [converged, niter, energy, mu, n_eval_v, n_eval_ev,
  * seconds, ...]; converged: 0 converged, 1 line search failed, 2 maxiter
  * reached (optimization.py:440-444,509-511), 3 FP32 plan stopped at single-

@@ -47,7 +47,7 @@ def test_tn_gaas_golden():
     assert np.isclose(new_rho.integral(), ions.get_ncharges(), rtol=1e-12)
 
 
-def _c1_oracle(econv):
+def _c1_oracle(econv, method="TN", **kw):
     from dftpy_amd import systems
     ions, nr = systems.c1_system()
     og = OG.Grid(ions.cell, nr)
@@ -66,10 +66,49 @@ def _c1_oracle(econv):
         return e, v
 
     rho = np.full(tuple(nr), oat.ncharge() / og.volume)
-    return OO.optimize(ev, rho, og.dV, method="TN", econv=econv, maxiter=100, maxfun=50, maxls=30)
+    o = dict(maxiter=100, maxfun=50, maxls=30)
+    o.update(kw)
+    return OO.optimize(ev, rho, og.dV, method=method, econv=econv, **o)
 
 
-@pytest.mark.parametrize("method", ["TN", "CG-HS"])
+def _c1_gpu(method, **opts):
+    from dftpy_amd import DirectField, DirectGrid, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    from dftpy_amd.optimization import Optimization
+    ions, nr = systems.c1_system()
+    grid = DirectGrid(ions.cell, nr, full=False)
+    pp = Functional(type="PSEUDO", grid=grid, ions=ions, PP_list={"Al": systems.empty_core_recpot()})
+    ev = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                         H=Functional(type="HARTREE"), PSEUDO=pp)
+    rho = DirectField(grid)
+    rho[:] = ions.get_ncharges() / grid.volume
+    opt = Optimization(optimization_method=method, EnergyEvaluator=ev, optimization_options=opts)
+    return opt, opt.optimize_rho(rho), ions
+
+
+@pytest.mark.parametrize("method,extra", [("CG-PR", {}), ("CG-FR", {}), ("CG-DY", {}), ("CG-CD", {}), ("CG-LS", {}),
+                                          ("GD", {}), ("LBFGS", {"h0": 1.0}), ("LBFGS", {"h0": 0.0}),
+                                          ("LBFGS", {"h0": 1.0, "maxcor": 1})])
+def test_c1_direction_methods_follow_the_oracle(method, extra):
+    """C1 in FP64, 12 iterations of the same method on the GPU and in the
+    oracle (CG betas of math_utils.py:395-423, steepest descent, L-BFGS
+    :374-455, whose memory of maxcor + 1 pairs wraps within the run): the
+    energy after every step agrees to 1e-9 Ha and the densities to 1e-8."""
+    print("method", method, extra, flush=True)
+    opt, out, ions = _c1_gpu(method, econv=1e-14, maxiter=12, **extra)
+    ref = _c1_oracle(1e-14, method=method, maxiter=12, **extra)
+    assert len(opt.history) == len(ref["history"]) == 12
+    assert np.max(np.abs(opt.history - np.asarray(ref["history"]))) < 1e-9
+    assert np.max(np.abs(out.numpy() - ref["rho"])) < 1e-8
+
+
+def test_lbfgs_needs_h0():
+    """optimization.py:346 reads optimization_options['h0'], which has no default."""
+    with pytest.raises(KeyError):
+        _c1_gpu("LBFGS", maxiter=3)
+
+
+@pytest.mark.parametrize("method", ["TN", "CG-HS", "CG-DY", "LBFGS"])
 def test_c1_vs_oracle(method):
     """C1: fcc Al 4 atoms, a = 7.60, 32^3, WT + LDA + H + empty core, from
     the uniform density; energy within 1e-6 Ha/atom of the oracle."""
@@ -83,8 +122,10 @@ def test_c1_vs_oracle(method):
                          H=Functional(type="HARTREE"), PSEUDO=pp)
     rho = DirectField(grid)
     rho[:] = ions.get_ncharges() / grid.volume
-    opt = Optimization(optimization_method=method, EnergyEvaluator=ev,
-                       optimization_options={"econv": 1e-8, "maxiter": 200})
+    opts = {"econv": 1e-8, "maxiter": 200}
+    if method == "LBFGS":
+        opts["h0"] = 1.0
+    opt = Optimization(optimization_method=method, EnergyEvaluator=ev, optimization_options=opts)
     out = opt.optimize_rho(rho)
     assert opt.converged == 0
     ref = _c1_oracle(1e-10)

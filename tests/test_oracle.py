@@ -194,6 +194,88 @@ def test_tn_gaas_golden():
     assert np.isclose(ev(res["rho"], True)[0], -43.3046, rtol=1e-5)
 
 
+def _quadratic_cg_step(seed=0, shape=(2, 2, 3)):
+    """One exact line search on f(x) = x.Ax/2 - b.x from x0 along -g0;
+    returns (A, g0, g1, d0) with gradients and direction as fields."""
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    M = rng.standard_normal((n, n))
+    A = M @ M.T + n * np.eye(n)
+    b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    g0 = A @ x0 - b
+    d0 = -g0
+    alpha = -(g0 @ d0) / (d0 @ A @ d0)
+    g1 = A @ (x0 + alpha * d0) - b
+    return A, g0.reshape(shape), g1.reshape(shape), d0.reshape(shape)
+
+
+@pytest.mark.parametrize("method", ["CG-HS", "CG", "CG-PR", "CG-FR", "CG-DY"])
+def test_cg_betas_conjugate_on_a_quadratic(method):
+    """With an exact line search on a quadratic, the HS, PR, FR and DY betas
+    all give the linear-CG direction, which is A-conjugate to the previous one."""
+    A, g0, g1, d0 = _quadratic_cg_step()
+    d1 = optimize.get_direction_CG([g0, g1], [d0], method)
+    assert abs(d1.ravel() @ A @ d0.ravel()) < 1e-10 * np.linalg.norm(d1) * np.linalg.norm(A @ d0.ravel())
+    assert np.array_equal(optimize.get_direction_CG([g0], [], method), -g0)
+
+
+def test_cg_ls_and_cd_as_the_reference_writes_them():
+    """math_utils.py:411-414: CG-LS divides by sum(d r0) (textbook Liu-Storey
+    divides by -sum(d r0)), so on the quadratic its beta is minus the linear-CG
+    beta; CG-CD's beta is the field -(r1 ** 2) / sum(d r0), whose sum is the
+    textbook conjugate-descent beta, equal to Fletcher-Reeves there."""
+    A, g0, g1, d0 = _quadratic_cg_step(1)
+    beta_cg = np.sum(g1 ** 2) / np.sum(g0 ** 2)
+    d_ls = optimize.get_direction_CG([g0, g1], [d0], "CG-LS")
+    assert np.allclose(d_ls, -g1 - beta_cg * d0, rtol=1e-12, atol=1e-12)
+    d_cd = optimize.get_direction_CG([g0, g1], [d0], "CG-CD")
+    beta_field = (d_cd + g1) / d0
+    assert np.ptp(beta_field) > 0.1 * beta_cg  # a field, not a scalar
+    assert np.isclose(beta_field.sum(), beta_cg, rtol=1e-10)
+
+
+@pytest.mark.parametrize("m", [0, 1, 2, 4])
+@pytest.mark.parametrize("h0", [0.0, 0.7])
+def test_lbfgs_two_loop_against_dense_update(m, h0):
+    """The reference's two-loop recursion (math_utils.py:431-455, first loop
+    over pairs m-1..1) equals the dense BFGS inverse update over pairs 1..m-1
+    started from V0^T H0, V_i = I - rho_i y_i s_i^T, H0 = h0 I or gamma I."""
+    rng = np.random.default_rng(10 * m + int(h0 * 10))
+    shape = (3, 2, 2)
+    n = int(np.prod(shape))
+    lb = optimize.LBFGS(h0=h0, bound=6)
+    for _ in range(m):
+        s = rng.standard_normal(shape)
+        y = s + 0.3 * rng.standard_normal(shape)  # s.y > 0
+        lb.update(s, y)
+    r = rng.standard_normal(shape)
+    d = optimize.get_direction_LBFGS(r, lb)
+    S = [v.ravel() for v in lb.s]
+    Y = [v.ravel() for v in lb.y]
+    rho = [1.0 / (a @ b) for a, b in zip(S, Y)]
+    I = np.eye(n)
+    if h0:
+        H = h0 * I
+    else:
+        H = (S[-1] @ Y[-1]) / (Y[-1] @ Y[-1]) * I if m else I
+    if m:
+        H = (I - rho[0] * np.outer(S[0], Y[0])) @ H
+    for j in range(1, m):
+        V = I - rho[j] * np.outer(Y[j], S[j])
+        H = V.T @ H @ V + rho[j] * np.outer(S[j], S[j])
+    assert np.allclose(d.ravel(), H @ (-r.ravel()), rtol=1e-11, atol=1e-12)
+
+
+def test_lbfgs_memory_keeps_bound_plus_one_pairs():
+    """math_utils.py:383-388: the oldest pair is dropped only once more than
+    Bound pairs are held."""
+    lb = optimize.LBFGS(h0=1.0, bound=2)
+    for k in range(5):
+        lb.update(np.full(3, k + 1.0), np.ones(3))
+    assert [v[0] for v in lb.s] == [3.0, 4.0, 5.0]
+    assert np.allclose(lb.rho, [1 / 9.0, 1 / 12.0, 1 / 15.0])
+
+
 def test_td_dipole_golden():
     # examples/test/test_propagator.py:28-81 (CN part)
     atoms, rho0, lat = io.read_xsf(data("GaAs_random.xsf"))
