@@ -17,7 +17,7 @@ SRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libofdft_hip.so")
 
-SOURCES = ["fft.hip", "fft_fixed.hip", "fft_fixed_f32.hip", "fft_fixed_f64.hip", "vec.hip", "eval.hip", "pseudo.hip", "opt.hip", "td.hip", "cnn.hip", "lmgp.hip", "capi.cpp", "dcsrch.cpp"]
+SOURCES = ["fft.hip", "fft_fixed.hip", "fft_fixed_f32.hip", "fft_fixed_f64.hip", "vec.hip", "eval.hip", "pseudo.hip", "opt.hip", "td.hip", "cnn.hip", "lmgp.hip", "density.hip", "capi.cpp", "dcsrch.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("OFDFT_ARCH", "gfx950")

@@ -200,6 +200,30 @@ int ofdft_ewald_forces_stress(ofdft_plan *plan, int nat, const double *pos_cart,
                               const double *charges, double precision, double *forces, double *stress, void *stream);
 int ofdft_functional_stress(ofdft_plan *plan, const void *rho, const double *energies, double *out, void *stream);
 
+/* Start densities from atomic radial functions (density/density.py).
+ * ofdft_atomic_density: the real-space placement of AtomicDensity.step
+ *   (:18-67) as guess_rho_atom (:209-249, kind 0) and build_pseudo_density /
+ *   gen_gaussian_density (:353-406, kind 1) use it, added into rho (device,
+ *   N doubles, FP64 plan).  Per species k: rcut[k], rtol[k] (kind 0: points
+ *   with r <= rtol skipped when rtol > 0), scale[k] and, kind 1, sigma[k]:
+ *   value scale * exp(-r^2 / 2 sigma^2) / (sqrt(2 pi) sigma)^3; kind 0: the
+ *   piecewise cubic of nbreak[k] increasing breakpoints breaks[k] and
+ *   4 (nbreak[k] - 1) coefficients coef[k] (c0..c3 in powers of r - x_i).
+ *   Each atom covers min(int(rcut / spacing) + 1, nr / 2) points per side of
+ *   the grid point below it; displacements are formed as density.py:55 does
+ *   (lattice[k][j] s_j); where the box reaches a grid point twice the later
+ *   (+border) value is kept, as the reference's fancy-index += does.
+ *   pos_frac: nat x 3 scaled positions; species: nat indices.  Setup call
+ *   (returns after the placement).
+ * ofdft_clamp_normalize: rho < dtol -> dtol when clamp, then, if
+ *   ncharge > 1e-6, rho *= ncharge / (sum rho dV) (density.py:236-248,
+ *   300-311); FP64 plan, stream-ordered. */
+int ofdft_atomic_density(ofdft_plan *plan, int kind, int nspecies, const int *nbreak, const double *const *breaks,
+                         const double *const *coef, const double *rcut, const double *rtol, const double *sigma,
+                         const double *scale, int nat, const double *pos_frac, const int *species, void *rho,
+                         void *stream);
+int ofdft_clamp_normalize(ofdft_plan *plan, void *rho, int clamp, double dtol, double ncharge, void *stream);
+
 /* Preconditioner of the TN inner conjugate-gradient solve (no counterpart in
  * the reference, whose get_direction_TN, optimization.py:91-147, runs plain
  * CG).  table (host, n doubles): M^-1(q) at q = i dq, linearly interpolated

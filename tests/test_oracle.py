@@ -8,6 +8,7 @@ from conftest import data
 from oracle import dcsrch as D
 from oracle import functionals as F
 from oracle import grid as G
+from oracle import density as OD
 from oracle import io, optimize, pseudo, td, units
 
 
@@ -274,6 +275,49 @@ def test_lbfgs_memory_keeps_bound_plus_one_pairs():
         lb.update(np.full(3, k + 1.0), np.ones(3))
     assert [v[0] for v in lb.s] == [3.0, 4.0, 5.0]
     assert np.allclose(lb.rho, [1 / 9.0, 1 / 12.0, 1 / 15.0])
+
+
+def _periodic_gaussians(g, pos, sigma, z):
+    """sum over atoms and neighbouring images of z * normalised Gaussians."""
+    r = np.stack(np.meshgrid(*[np.arange(n) / n for n in g.nr], indexing="ij"), axis=-1) @ g.lattice
+    out = np.zeros(tuple(g.nr))
+    for R in pos:
+        for t in np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1], indexing="ij")).reshape(3, -1).T:
+            d = np.linalg.norm(r - R - t @ g.lattice, axis=-1)
+            out += z * OD.gaussian(d, sigma)
+    return out
+
+
+def test_gaussian_atoms_closed_forms():
+    """A resolved Gaussian (sigma = 3 h, rcut < L/2) integrates to its scale and
+    every placed value is the closed form at the true distance (cubic cell)."""
+    g = G.Grid(np.eye(3) * 12.0, (60, 60, 60))
+    pos = np.array([[3.13, 7.71, 5.02]])
+    rho, nc = OD.gen_gaussian_density(g, ["Al"], pos / 12.0, {"Al": {"rcut": 5.5, "sigma": 0.6, "scale": 3.0}})
+    assert nc == 3.0
+    assert np.isclose(rho.sum() * g.dV, 3.0, rtol=1e-12)
+    r = np.stack(np.meshgrid(*[np.arange(60) * 0.2] * 3, indexing="ij"), axis=-1)
+    d = np.linalg.norm((r - pos[0] + 6.0) % 12.0 - 6.0, axis=-1)
+    ref = np.where(d < 5.5, 3.0 * OD.gaussian(d, 0.6), 0.0)
+    assert np.max(np.abs(rho - ref)) < 1e-12 * ref.max()
+
+
+def test_recipe_of_a_gaussian_transform_is_the_periodic_gaussian():
+    """get_3d_value_recipe with the table z exp(-q^2 sigma^2 / 2) (the transform
+    of a normalised Gaussian) gives the periodic sum of the Gaussians; PME and
+    exact structure factors agree; the real-space spline path agrees too."""
+    L, sigma, z = 10.0, 0.8, 3.0
+    g = G.Grid(np.eye(3) * L, (48, 48, 48))
+    pos = np.array([[1.3, 2.2, 7.9], [6.1, 5.4, 0.7]])
+    q = np.arange(0, 3001) * 0.01
+    tab = {"Al": z * np.exp(-0.5 * (q * sigma) ** 2)}
+    ref = _periodic_gaussians(g, pos, sigma, z)
+    for pme in (False, True):
+        rho = OD.get_3d_value_recipe(g, ["Al", "Al"], pos, pos / L, {"Al": q}, tab, 2 * z, pme=pme)
+        assert np.max(np.abs(rho - ref)) < (1e-9 if not pme else 1e-6) * ref.max()
+    rr = np.linspace(0.0, 6.0, 601)
+    rho_a = OD.guess_rho_atom(g, ["Al", "Al"], pos / L, {"Al": rr}, {"Al": z * OD.gaussian(rr, sigma)}, 2 * z)
+    assert np.max(np.abs(rho_a - ref)) < 1e-6 * ref.max()
 
 
 def test_td_dipole_golden():
