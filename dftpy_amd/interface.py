@@ -7,7 +7,7 @@ Optdensity`` or ``Calculation``.
 
 Every computation runs through libofdft_hip on the GPU; this module only
 wires the pieces together.  Configuration features outside the GPU path
-(multi-step grids, spin, OESCF, atomic initial densities, complex grids)
+(multi-step grids, spin, OESCF, complex grids)
 raise NotImplementedError.
 """
 import os
@@ -18,6 +18,7 @@ import torch
 
 from .config import ReadConf
 from .constants import Units
+from .density import DensityGenerator
 from .field import DirectField
 from .formats import io
 from .functional import Functional, FunctionalOutput
@@ -89,8 +90,12 @@ def ConfigParser(config, ions=None, rhoini=None, pseudo=None, grid=None, dtype=t
         src = None
     elif ini == "read":
         src = io.read_density(config["DENSITY"]["densityfile"], dtype=dtype, device=device)
+    elif ini == "atomic":
+        # interface.py:123-125; recpot files carry no atomic density, so this is
+        # the uniform gas unless the reader provides one
+        src = DensityGenerator(pseudo=pseudo).guess_rho(ions, grid=grid)
     else:
-        raise NotImplementedError("DENSITY.densityini = %s is not implemented (heg, read are)" % ini)
+        raise NotImplementedError("DENSITY.densityini = %s is not implemented (heg, read, atomic are)" % ini)
     if src is None:
         rho = DirectField(grid, data=torch.full(tuple(grid.nrR), nelec / grid.volume, dtype=grid.dtype,
                                                 device=grid.device))

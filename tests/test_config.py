@@ -162,7 +162,7 @@ def test_config_parser_reads_density_and_rejects_what_is_missing(ini, tmp_path):
     _, others = ConfigParser(conf, device="cpu")
     assert float(others["rho"].data.max()) == float(others["rho"].data.min())
     for sec, key, val in (("MATH", "multistep", 2), ("DENSITY", "nspin", 2), ("OPT", "algorithm", "OESCF"),
-                          ("DENSITY", "densityini", "atomic")):
+                          ("DENSITY", "densityini", "random")):
         c2 = C.ReadConf(ini)
         c2[sec][key] = val
         with pytest.raises(NotImplementedError):
