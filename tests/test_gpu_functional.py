@@ -239,3 +239,23 @@ def test_plan_settings_follow_the_functional():
     for _ in range(2):
         for f, e in zip(reversed(funcs), reversed(first)):
             assert f(rho).energy == e
+
+
+def test_vw_gaussian_sigma_vs_oracle():
+    """vW with the Gaussian filter exp(-G^2 sigma^2 / 4) of vw.py:45-49 (the
+    KEDF's sigma option, applied to its vW part only), FP64, WT total."""
+    from dftpy_amd import DirectField, systems
+    from dftpy_amd.functional import Functional
+    ions, nr = systems.c1_system()
+    lat = ions.cell
+    rho = _c1_density(nr, lat, seed=3)
+    og = OG.Grid(lat, nr)
+    g = _dg(lat, nr, False)
+    for name, parts in (("vW", (OF.vw,)), ("WT", (OF.tf, OF.vw, OF.wt))):
+        out = Functional(type="KEDF", name=name, sigma=0.3)(DirectField(g, data=rho))
+        e_ref = sum((f(rho, og, sigma=0.3) if f is OF.vw else f(rho, og))[0] for f in parts)
+        v_ref = sum((f(rho, og, sigma=0.3) if f is OF.vw else f(rho, og))[1] for f in parts)
+        assert np.isclose(out.energy, e_ref, rtol=1e-12, atol=0)
+        assert np.max(np.abs(out.potential.numpy() - v_ref)) < 1e-10 * np.max(np.abs(v_ref))
+    e0 = Functional(type="KEDF", name="vW")(DirectField(g, data=rho)).energy
+    assert abs(e0 - OF.vw(rho, og)[0]) < 1e-12 * abs(e0) and abs(e0 - out.energy) > 1e-6

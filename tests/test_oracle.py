@@ -130,6 +130,19 @@ def test_hartree_parseval():
     assert np.isclose(e, e2, rtol=1e-12)
 
 
+def test_vw_sigma_on_a_plane_wave():
+    """sqrt(rho) = a + b cos(G0 x): the filtered vW potential is
+    (1/2) b |G0|^2 exp(-|G0|^2 sigma^2 / 4) cos(G0 x) / sqrt(rho) (vw.py:45-49)."""
+    L, n, a, b, sigma = 8.0, 24, 0.2, 0.05, 0.7
+    g = G.Grid(np.eye(3) * L, (n, n, n))
+    x = np.arange(n) * L / n
+    c = np.cos(2 * np.pi * 2 * x / L)[:, None, None] * np.ones((n, n, n))
+    sq = a + b * c
+    _, v = F.vw(sq * sq, g, sigma=sigma)
+    G0 = 2 * np.pi * 2 / L
+    assert np.allclose(v, 0.5 * b * G0 ** 2 * np.exp(-G0 ** 2 * sigma ** 2 / 4) * c / sq, rtol=1e-10, atol=1e-13)
+
+
 def test_vw_uniform_is_zero():
     g = G.Grid(np.eye(3) * 5, (8, 8, 8))
     e, v = F.vw(np.full((8, 8, 8), 0.03), g)
