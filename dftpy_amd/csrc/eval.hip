@@ -129,16 +129,20 @@ static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long
 // sums land in the k_post partial columns of vW (times y), WT and Hartree
 // (times 1/2), which k_post leaves at zero when it has no fields.
 template <typename T>
-__global__ void __launch_bounds__(128) k_gspace_energy(const cpx<T> *__restrict__ spec, int ns, GParams p,
+__global__ void __launch_bounds__(256) k_gspace_energy(const cpx<T> *__restrict__ spec, int ns, GParams p,
                                                        double y, double *partials) {
     __shared__ double sm[16];
     double tkf = 0.0, wtfac = 0.0;
     if (p.s_wt >= 0) wt_consts(p, tkf, wtfac);
     double acc_vw = 0.0, acc_wt = 0.0, acc_h = 0.0;
-    const int rows = p.nx * p.ny;
-    for (int row = blockIdx.x; row < rows; row += gridDim.x) {
+    // flat grid-stride over the stored half spectrum (every lane busy; rows
+    // are shorter than a workgroup)
+    const long long total = (long long)p.nx * p.ny * p.nzc;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int row = (int)(idx / p.nzc), iz = (int)(idx - (long long)row * p.nzc);
         const int ix = row / p.ny, iy = row - ix * p.ny;
-        for (int iz = threadIdx.x; iz < p.nzc; iz += blockDim.x) {
+        {
             const double w = (iz == 0 || ((p.nz & 1) == 0 && iz == p.nz / 2)) ? 1.0 : 2.0;
             const long long e = (long long)row * p.pitch + iz;
             for (int sl = 0; sl < ns; ++sl) {
@@ -343,7 +347,7 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     const int nb = std::min(reduce_blocks(P.N), 1024);
     hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, nullptr, vloc, nullptr, P.red, pp);
     OFDFT_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_gspace_energy<T>), dim3(nb), dim3(128), 0, s, spec, ns, gp, P.y, P.red);
+    hipLaunchKernelGGL((k_gspace_energy<T>), dim3(nb), dim3(256), 0, s, spec, ns, gp, P.y, P.red);
     OFDFT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
     OFDFT_HIP(hipGetLastError());
