@@ -18,7 +18,7 @@ from two cheap fields:
 Network input x0 = [c (rho_s - rho0) / rho0, sv * vloc]; output delta; start
 density rho = rho0 exp(x0[0] + delta), normalised to the electron count.
 The optimizer then finishes from that density, so the plain method remains
-both the fallback and the judge of the result.
+both the fallback and the arbiter of the result.
 """
 import os
 

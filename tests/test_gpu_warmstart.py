@@ -2,7 +2,7 @@
 on held-out cells (108-atom fcc Al, 96^3, seeds outside the training range
 0..1999): the start density's energy error, and the evaluations the FP32 TN
 optimizer needs under the reference stop rule.  Both starts must reach the
-same minimum (within 1e-6 Ha/atom): the optimizer remains the judge."""
+same minimum (within 1e-6 Ha/atom): the optimizer remains the arbiter."""
 import pytest
 
 pytestmark = pytest.mark.gpu
