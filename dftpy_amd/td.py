@@ -6,7 +6,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .field import DirectField
+from .field import DirectField, ReciprocalField
 
 
 class Hamiltonian:
@@ -130,3 +130,92 @@ def dipole(rho, rho0):
     d = (rho.data - rho0.data).to(torch.float64)
     r = rho.grid.r.to(torch.float64)
     return (torch.einsum("xyz,kxyz->k", d, r) * rho.grid.dV).cpu().numpy()
+
+
+def calc_j(psi, N=1):
+    """Current density (utils/utils.py:44-47): Re(-i/2 (conj(psi) grad psi -
+    psi grad conj(psi))) with the spectral gradients IFFT(i G FFT(.)) of
+    field.py:267-281 on the full grid (HIP C2C transforms).  grad conj(psi)
+    is transformed on its own, as the reference does: on an even grid the
+    Nyquist plane makes it differ from conj(grad psi).  Returns a
+    (3, nx, ny, nz) real tensor."""
+    # conj_physical: a lazy conj() view would hand the library the unconjugated memory
+    pc = DirectField(psi.grid, data=psi.data.conj_physical().contiguous(), cplx=True)
+    f, fc = psi.fft(), pc.fft()
+    G = f.grid.g
+    out = []
+    for k in range(3):
+        ig = (1j * torch.as_tensor(G[k], dtype=torch.float64, device=f.data.device)).to(f.data.dtype)
+        gp = ReciprocalField(f.grid, data=(ig * f.data).contiguous(), cplx=True).ifft(force_real=False).data
+        gc = ReciprocalField(f.grid, data=(ig * fc.data).contiguous(), cplx=True).ifft(force_real=False).data
+        out.append(torch.real(-0.5j * (pc.data * gp - psi.data * gc)) * N)
+    return torch.stack(out)
+
+
+def _norm_rho(rho, dV):
+    return float(torch.sqrt((rho.to(torch.float64) ** 2).sum() * dV))
+
+
+def _norm_j(j, dV):
+    return float(torch.sqrt((j.to(torch.float64) ** 2).sum(dim=(1, 2, 3)) * dV).max())
+
+
+class PredictorCorrector:
+    """Predictor-corrector time step (td/predictor_corrector.py:10-150):
+    repeat up to ``max_steps`` times: psi_pred = CN(psi) with the current
+    Hamiltonian potential, then the potential of ``functionals`` at
+    (rho + rho_pred) / 2 becomes the Hamiltonian's; stop when rho_pred and
+    j_pred change by less than tol * |rho| and tol * max|j_k| (or atol)
+    between two predictions.  ``run()`` returns the convergence flag; the new
+    state is ``psi_pred``, ``rho_pred``, ``j_pred``.  Vector potentials are
+    not implemented."""
+
+    def __init__(self, psi, propagator=None, tol=1.0e-8, atol=None, max_steps=2, propagate_vector_potential=False,
+                 functionals=None, Omega=None, A_t=None, A_tm1=None, N0=None):
+        if propagate_vector_potential or A_t is not None:
+            raise NotImplementedError("vector potentials are not implemented")
+        self.propagator, self.functionals = propagator, functionals
+        self.max_steps, self.tol, self.atol = max_steps, tol, atol
+        self.nsteps = 0
+        self.psi = psi
+        self.dV = psi.grid.dV
+        self.rho = calc_rho(psi).data
+        self.j = calc_j(psi)
+        self.atol_rho = _get_atol(tol, atol, _norm_rho(self.rho, self.dV))
+        self.atol_j = _get_atol(tol, atol, _norm_j(self.j, self.dV))
+        self.old_rho_pred = torch.zeros_like(self.rho)
+        self.rho_pred = self.rho
+        self.old_j_pred = torch.zeros_like(self.j)
+        self.j_pred = self.j
+        self.psi_pred = None
+        self.diff_rho = self.diff_j = None
+
+    def step(self):
+        if self.nsteps > 0:
+            self.old_rho_pred = self.rho_pred
+            self.old_j_pred = self.j_pred
+        self.psi_pred, info = self.propagator(self.psi)
+        self.rho_pred = calc_rho(self.psi_pred).data
+        self.j_pred = calc_j(self.psi_pred)
+        rho_corr = DirectField(self.psi.grid, data=((self.rho + self.rho_pred) * 0.5).contiguous())
+        self.propagator.hamiltonian.v = self.functionals(rho_corr, calcType={"V"}).potential
+
+    def converged(self):
+        self.diff_rho = _norm_rho(self.old_rho_pred - self.rho_pred, self.dV)
+        self.diff_j = _norm_j(self.old_j_pred - self.j_pred, self.dV)
+        return self.diff_rho < self.atol_rho and self.diff_j < self.atol_j
+
+    def run(self):
+        # Dynamics.irun (optimize.py:85-128)
+        while not self.converged() and self.nsteps < self.max_steps:
+            self.step()
+            self.nsteps += 1
+        return self.converged()
+
+
+def _get_atol(tol, atol, bnrm2):
+    """linear_solver.py:8-17."""
+    tol = float(tol)
+    if atol is None:
+        return tol if bnrm2 == 0 else tol * float(bnrm2)
+    return max(float(atol), tol * float(bnrm2))

@@ -2,7 +2,8 @@
 ``td/hamiltonian.py:70-114``, ``field.py:343-350`` (smoothed Laplacian),
 ``td/propagator/crank_nicholson.py:44-56,147-188``, ``linear_solver.py:8-47``
 (CG without conjugation), ``utils/utils.py:40-41`` (calc_rho) and
-``td/utils.py:10-16`` (delta kick)."""
+``td/utils.py:10-16`` (delta kick), ``utils/utils.py:44-47`` (current
+density) and ``td/predictor_corrector.py:84-150`` (predictor-corrector step)."""
 import numpy as np
 
 
@@ -61,3 +62,47 @@ def dipole(rho, rho0, grid):
     """delta mu = int (rho - rho0) r dV (``real_time_runner.py:251-253``)."""
     d = rho - rho0
     return np.array([(d * grid.r[i]).sum() * grid.dV for i in range(3)])
+
+
+def calc_j(psi, grid):
+    """``utils/utils.py:44-47``: Re(-i/2 (conj(psi) grad psi - psi grad conj(psi)))
+    with the spectral gradients of ``field.py:267-281`` on the full grid."""
+    k = np.meshgrid(*[np.fft.fftfreq(int(n), 1.0 / int(n)) for n in grid.nr], indexing="ij")
+    G = np.einsum("jxyz,jk->kxyz", np.array(k), grid.B)
+    f, fc = np.fft.fftn(psi), np.fft.fftn(np.conj(psi))
+    out = []
+    for i in range(3):
+        gp, gc = np.fft.ifftn(1j * G[i] * f), np.fft.ifftn(1j * G[i] * fc)
+        out.append(np.real(-0.5j * (np.conj(psi) * gp - psi * gc)))
+    return np.array(out)
+
+
+def predictor_corrector(psi, v, grid, dt, potential, tol=1e-8, atol=None, max_steps=2):
+    """``td/predictor_corrector.py:84-150`` driven as ``optimize.py:85-128``
+    does: returns (psi_pred, v, nsteps, converged), v being the potential of
+    ``potential`` at (rho + rho_pred) / 2 after the last prediction."""
+    def nrm(x):
+        return np.sqrt((x ** 2).sum() * grid.dV)
+
+    def nrm_j(j):
+        return np.max(np.sqrt((j ** 2).sum(axis=(1, 2, 3)) * grid.dV))
+
+    def get_atol(b):
+        return (tol * b if b != 0 else tol) if atol is None else max(atol, tol * b)
+
+    rho, j = np.abs(psi) ** 2, calc_j(psi, grid)
+    at_rho, at_j = get_atol(nrm(rho)), get_atol(nrm_j(j))
+    old_rho, old_j, rho_pred, j_pred = np.zeros_like(rho), np.zeros_like(j), rho, j
+    psi_pred, n = None, 0
+
+    def conv():
+        return nrm(old_rho - rho_pred) < at_rho and nrm_j(old_j - j_pred) < at_j
+
+    while not conv() and n < max_steps:
+        if n > 0:
+            old_rho, old_j = rho_pred, j_pred
+        psi_pred, info, _ = cn_step(psi, v, grid, dt, tol)
+        rho_pred, j_pred = np.abs(psi_pred) ** 2, calc_j(psi_pred, grid)
+        v = potential((rho + rho_pred) * 0.5)
+        n += 1
+    return psi_pred, v, n, conv()

@@ -123,3 +123,45 @@ def test_cn_batch_matches_single_runs_and_oracle(dtype):
             for _ in range(nsteps):
                 ps, st, _ = OTD.cn_step(ps, pot(np.real(ps * np.conj(ps))), og, dt, tol=tol, maxiter=200)
             assert np.max(np.abs(batch[r].cpu().numpy() - ps)) < 1e-8
+
+
+def test_predictor_corrector_and_current_vs_oracle():
+    """Three runner steps (real_time_runner.py:127-144): PredictorCorrector
+    with max_steps = 2 around the HIP Crank-Nicolson solve, the corrector
+    potential from Hartree + PZ-LDA at (rho + rho_pred) / 2; psi, the current
+    density and the potential against the complex128 oracle."""
+    from dftpy_amd import DirectField, DirectGrid
+    from dftpy_amd.functional import Functional, TotalFunctional
+    from dftpy_amd.td import CrankNicholson, Hamiltonian, PredictorCorrector, calc_j
+    from oracle import functionals as OF2
+    from oracle import grid as OG2
+    from oracle import td as OT
+    lat = np.eye(3) * 7.6
+    nr = (24, 24, 24)
+    g = DirectGrid(lat, nr, full=True)
+    og = OG2.Grid(lat, nr, full=True)
+    r = og.r
+    rho0 = 0.027 * (1 + 0.3 * np.cos(2 * np.pi * r[0] / 7.6) * np.cos(2 * np.pi * r[2] / 7.6))
+    psi_np = np.sqrt(rho0) * np.exp(1j * 1e-2 * r[1])
+    F = TotalFunctional(XC=Functional(type="XC", name="LDA"), H=Functional(type="HARTREE"))
+
+    def pot_ref(rr):
+        return OF2.lda(rr, og)[1] + OF2.hartree(rr, og)[1]
+
+    v0 = F(DirectField(g, data=torch.as_tensor(rho0, device="cuda")), calcType={"V"}).potential
+    H = Hamiltonian(v=v0)
+    cn = CrankNicholson(H, 0.1)
+    psi = DirectField(g, data=torch.as_tensor(psi_np, device="cuda"), cplx=True)
+    v_np = pot_ref(rho0)
+    for step in range(3):
+        print("pc step", step, flush=True)
+        pc = PredictorCorrector(psi, propagator=cn, functionals=F, max_steps=2)
+        pc.run()
+        psi = pc.psi_pred
+        psi_np, v_np, n, _ = OT.predictor_corrector(psi_np, v_np, og, 0.1, pot_ref)
+        assert pc.nsteps == n == 2
+        assert np.max(np.abs(psi.data.cpu().numpy() - psi_np)) < 1e-9
+        assert np.max(np.abs(H.v.data.cpu().numpy() - v_np)) < 1e-9 * np.max(np.abs(v_np))
+    j = calc_j(psi).cpu().numpy()
+    j_ref = OT.calc_j(psi_np, og)
+    assert np.max(np.abs(j - j_ref)) < 1e-8 * np.max(np.abs(j_ref))

@@ -333,6 +333,28 @@ def test_recipe_of_a_gaussian_transform_is_the_periodic_gaussian():
     assert np.max(np.abs(rho_a - ref)) < 1e-6 * ref.max()
 
 
+def test_current_density_of_a_plane_wave():
+    """psi = a exp(i G0.r): j = |a|^2 G0 everywhere (utils/utils.py:44-47)."""
+    g = G.Grid(np.array([[7.0, 0, 0], [0.5, 6.0, 0], [0, 0.3, 8.0]]), (12, 10, 14), full=True)
+    G0 = np.array([2, -1, 3]) @ g.B
+    a = 0.3 - 0.2j
+    psi = a * np.exp(1j * np.einsum("kxyz,k->xyz", g.r, G0))
+    j = td.calc_j(psi, g)
+    assert np.allclose(j, (abs(a) ** 2 * G0)[:, None, None, None], rtol=1e-11, atol=1e-13)
+
+
+def test_predictor_corrector_with_a_fixed_potential_is_one_cn_step():
+    """With a potential that does not depend on rho the second prediction
+    repeats the first: two CN solves, converged, the state of one CN step."""
+    g = G.Grid(np.eye(3) * 6.0, (12, 12, 12), full=True)
+    rng = np.random.default_rng(4)
+    v = 0.1 * rng.random((12, 12, 12))
+    psi = np.sqrt(0.02 * (1 + 0.3 * rng.random((12, 12, 12)))) * np.exp(1j * 0.01 * g.r[0])
+    out, v2, n, conv = td.predictor_corrector(psi, v, g, 0.1, lambda r: v)
+    ref, _, _ = td.cn_step(psi, v, g, 0.1)
+    assert n == 2 and conv and np.array_equal(out, ref) and v2 is v
+
+
 def test_td_dipole_golden():
     # examples/test/test_propagator.py:28-81 (CN part)
     atoms, rho0, lat = io.read_xsf(data("GaAs_random.xsf"))
