@@ -36,10 +36,10 @@ sys.path.insert(0, ROOT)
 METRIC = "s to 1e-6 Ha/atom, 500-atom Al WT: FP32+learned start vs plain FP64 GPU"
 THRESH = 1e-6  # Ha/atom
 ECONV = 1e-6  # the reference's default stop rule, per atom
-# HBM bytes per launch of the FP32 column pass on 3 spectra of 160x160x81
-# (pitch 96), from the FETCH_SIZE / WRITE_SIZE counter runs of
-# `bench.py --passes-only`: 28833 KiB * 2.0 + 50400 KiB * 1.0
-TRAFFIC_X_C2C_FP32 = (28833 * 2.0 + 50400 * 1.0) * 1024
+# HBM bytes per launch of the CNN middle layer at 160^3, from the FETCH_SIZE /
+# WRITE_SIZE counter runs of `bench.py --cnn-only` (None until measured)
+TRAFFIC_CNN_MID = None
+TRAFFIC_CNN_MID_SOURCE = "not measured"
 
 
 def log(*a):
@@ -146,13 +146,19 @@ def learned_start_fn(ions, pp, g64, g32, net, use_net=True):
     return fn
 
 
+PASSES = ((0, "z_r2c_prologue"), (1, "y_c2c"), (6, "x_fused_gspace"), (3, "z_c2r"), (7, "post_pointwise"),
+          (5, "evaluation_E_V"))
+
+
 def pass_timings(grid, rho, iters=50):
-    """Mean duration of each pipeline pass (HIP events on torch's stream)."""
+    """Mean duration (ms) of each pass of the E+V pipeline with 3 spectra (vW,
+    WT, Hartree), timed with HIP events on torch's stream, the stream the
+    passes run on."""
     import ctypes
     from dftpy_amd import _lib
     plan = grid.plan(grid.dtype)
     out = {}
-    for which, name in ((0, "z_r2c_prologue"), (1, "y_c2c"), (2, "x_c2c"), (3, "z_c2r"), (5, "evaluation_E_V")):
+    for which, name in PASSES:
         ms = ctypes.c_double()
         _lib.check(_lib.lib.ofdft_bench_pass(plan.handle, which, 3, iters, rho.data.data_ptr(), ctypes.byref(ms),
                                              _lib.stream()))
@@ -160,51 +166,138 @@ def pass_timings(grid, rho, iters=50):
     return out
 
 
-def cpu_baseline(nr, seed, n_evals, budget_s=25.0):
-    """The FP64 numpy oracle (pocketfft, one thread) timed per E+V evaluation
-    at the same 160^3 size, projected onto the evaluation count of the FP64
-    GPU run (same algorithm)."""
+def pass_roofline(nr, ms, rs):
+    """Algorithmic bytes, GB/s and fraction of the 8 TB/s HBM peak of each
+    pipeline pass (3 spectra; rs = bytes per real).  Bytes per pass: each
+    array of the pass's contract read once and written once."""
+    n = nr[0] * nr[1] * nr[2]
+    nc = nr[0] * nr[1] * (nr[2] // 2 + 1)
+    cs = 2 * rs
+    alg = {"z_r2c_prologue": n * rs + 3 * nc * cs,  # rho in, 3 half spectra out
+           "y_c2c": 2 * 3 * nc * cs,
+           "x_fused_gspace": 2 * 3 * nc * cs,  # forward, multiply and inverse in place
+           "z_c2r": 3 * nc * cs + 3 * n * rs,
+           "post_pointwise": 6 * n * rs,  # rho, 3 inverse outputs, vloc in; V out
+           "evaluation_E_V": 29 * n * rs}  # compulsory traffic of one evaluation
+    out = {}
+    for k, b in alg.items():
+        gbs = b / (ms[k] * 1e-3) / 1e9
+        out[k] = dict(avg_ms=round(ms[k], 5), algorithmic_bytes=int(b), achieved_gbs=round(gbs, 1),
+                      frac=round(gbs / 8000.0, 4))
+    return out
+
+
+def cnn_layer_timings(net, nr, iters=20):
+    """Mean duration (ms) of each layer kernel of the warm-start network's
+    inference at grid nr, on random inputs of the network's shape, timed with
+    events on torch's current stream (the stream the layers run on)."""
+    import torch
+    from dftpy_amd import _lib
+    n = int(np.prod(nr))
+    nr_c = np.ascontiguousarray(nr, dtype=np.int32)
+    nrp = _lib.iptr(nr_c)
+    b = net._buffers(nr, False)
+    h = b["h"]
+    x0 = torch.randn(2, n, device="cuda", dtype=torch.float32)
+    h[0].copy_(torch.randn(n, 32, device="cuda").to(torch.bfloat16))
+    L = _lib.lib
+    s = _lib.stream()
+    calls = {"conv_in": lambda: L.ofdft_cnn_conv_in(nrp, net.dil[0], x0.data_ptr(), net.param("W1").data_ptr(),
+                                                     net.param("b1").data_ptr(), h[1].data_ptr(), s)}
+    for i, l in enumerate(range(2, 6)):
+        calls["mid%d_d%d" % (l, net.dil[l - 1])] = (
+            lambda i=i, l=l: L.ofdft_cnn_conv_mid(nrp, net.dil[l - 1], 0, h[0].data_ptr(), net.wf[i].data_ptr(),
+                                                  net.param("b%d" % l).data_ptr(), None, None, h[1].data_ptr(), None,
+                                                  s))
+    calls["conv_out"] = lambda: L.ofdft_cnn_conv_out(nrp, net.dil[5], h[0].data_ptr(), net.param("W6").data_ptr(),
+                                                      net.param("b6").data_ptr(), b["out"].data_ptr(),
+                                                      b["proj"].data_ptr(), s)
+    out = {}
+    for name, fn in calls.items():
+        _lib.check(fn())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            fn()
+        e1.record()
+        e1.synchronize()
+        out[name] = e0.elapsed_time(e1) / iters
+    return out
+
+
+def cpu_baseline(nr, seed, gpu_evals, n_iter_sample=2):
+    """The FP64 numpy oracle (pocketfft, one thread), the reference's
+    algorithm (TN with the forward-difference Hessian): the first
+    n_iter_sample TN iterations from the uniform density at the same 160^3
+    size are timed (a bounded sample, ~10-20 s), and projected onto the
+    evaluation count of the FP64 GPU run of the same algorithm to the same
+    stop.  The full oracle TN to convergence is timed on the 4-atom 32^3
+    cell beside it."""
     from dftpy_amd import systems
     from oracle import functionals as F
     from oracle import grid as G
     from oracle import io as OIO
+    from oracle import optimize as O
     from oracle import pseudo as P
+
+    def setup(ions, nr):
+        g = G.Grid(ions.cell, nr)
+        at = OIO.Atoms(ions.symbols, ions.positions, ions.cell)
+        gp, vp, z = OIO.read_recpot(systems.empty_core_recpot())
+        at.set_charges({"Al": z})
+        vreal = P.local_pp(g, at, {"Al": (gp, vp)}, pme=True)
+        eII = P.ewald_energy(g, at, pme=True)
+        count = [0]
+        cache = {}
+
+        def ev(r, need_e):
+            count[0] += 1
+            r0 = r.mean()
+            if "k" not in cache or abs(cache["r0"] - r0) > 1e-6:
+                cache["k"], cache["r0"] = F.wt_kernel(g.q, r0), r0
+            e, v, _ = F.total(r, g, vreal, cache["k"], ewald=eII)
+            return e, v
+        return g, at, ev, count
+
+    class Stop(Exception):
+        pass
+
     ions, nr = systems.c2_system(seed)
-    g = G.Grid(ions.cell, nr)
-    at = OIO.Atoms(ions.symbols, ions.positions, ions.cell)
-    gp, vp, z = OIO.read_recpot(systems.empty_core_recpot())
-    at.set_charges({"Al": z})
-    vreal = P.local_pp(g, at, {"Al": (gp, vp)}, pme=True)
+    g, at, ev, count = setup(ions, nr)
     rho = np.full(tuple(nr), at.ncharge() / g.volume)
-    kern = F.wt_kernel(g.q, rho.mean())
 
-    def per_eval(workers):
-        g.workers = workers
-        times = []
-        t_start = time.perf_counter()
-        while len(times) < 3 and time.perf_counter() - t_start < budget_s:
-            t0 = time.perf_counter()
-            F.total(rho, g, vreal, kern)
-            times.append(time.perf_counter() - t0)
-        g.workers = None
-        return float(np.median(times)), len(times)
-
-    per, nt = per_eval(None)
-    ncores = len(os.sched_getaffinity(0))
-    per_all, nt_all = per_eval(ncores)
+    def stop_after(it, e):
+        if it >= n_iter_sample:
+            raise Stop()
+    t0 = time.perf_counter()
+    try:
+        O.optimize(ev, rho, g.dV, method="TN", econv=ECONV * ions.nat, maxiter=100, callback=stop_after)
+    except Stop:
+        pass
+    t_sample = time.perf_counter() - t0
+    n_sample = count[0]
+    per = t_sample / n_sample
+    # the whole reference algorithm on the 4-atom cell, to its stop rule
+    ions1, nr1 = systems.c1_system()
+    g1, at1, ev1, count1 = setup(ions1, nr1)
+    t0 = time.perf_counter()
+    res1 = O.optimize(ev1, np.full(tuple(nr1), at1.ncharge() / g1.volume), g1.dV, method="TN", econv=1e-8,
+                      maxiter=100)
+    t_c1 = time.perf_counter() - t0
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
             cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
-    return dict(value=per * n_evals, unit="s", cores=1, kind="port",
-                sample="%d FP64 oracle E+V evaluations at 160^3 (numpy pocketfft, 1 thread), %.3f s each, x %d "
-                       "evaluations of the TN run (FD Hessian, as the reference)" % (nt, per, n_evals),
-                seconds_per_eval=per, value_all_cores=per_all * n_evals, cores_all=ncores,
-                seconds_per_eval_all_cores=per_all,
-                sample_all_cores="%d evaluations with scipy.fft on %d workers (FFTs threaded, point-wise numpy "
-                                 "on one thread)" % (nt_all, ncores),
+    return dict(value=round(per * gpu_evals, 3), unit="s", cores=1, kind="port",
+                sample="first %d TN iterations of the FP64 oracle (numpy pocketfft, one thread) from the uniform "
+                       "density at 160^3, 500 atoms: %d evaluations in %.2f s (%.3f s each), projected onto the %d "
+                       "evaluations the same algorithm (FD Hessian) takes on the GPU to the same stop" %
+                       (n_iter_sample, n_sample, t_sample, per, gpu_evals),
+                timed_sample_s=round(t_sample, 3), sample_evaluations=n_sample, seconds_per_eval=round(per, 4),
+                c1_full_tn_s=round(t_c1, 3), c1_full_tn_evaluations=count1[0], c1_energy=res1["energy"],
+                c1_workload="4-atom fcc Al, 32^3, TN from the uniform density to |dE| < 1e-8 Ha over 2 steps",
                 cpu_model=cpu_model, omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
 
 
@@ -217,6 +310,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--passes-only", action="store_true",
                     help="only launch the evaluation passes (and a reference copy kernel) for counter collection")
+    ap.add_argument("--cnn-only", action="store_true",
+                    help="only launch the warm-start network's layers at 160^3 (and a reference copy kernel) for "
+                         "counter collection")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the single-cell path on one device; "
@@ -225,10 +321,14 @@ def main():
     import torch
     log("device:", torch.cuda.get_device_name(0))
     ions, nr, g64, g32, ev, rho64, rho32 = build_system(args.seed)
-    if args.passes_only:
-        ev(rho32, calcType={"E", "V"})
-        log("fp32 passes (ms):", pass_timings(g32, rho32, iters=args.steps))
-        log("fp64 passes (ms):", pass_timings(g64, rho64, iters=args.steps))
+    if args.passes_only or args.cnn_only:
+        if args.passes_only:
+            ev(rho32, calcType={"E", "V"})
+            log("fp32 passes (ms):", pass_timings(g32, rho32, iters=args.steps))
+            log("fp64 passes (ms):", pass_timings(g64, rho64, iters=args.steps))
+        else:
+            from dftpy_amd.learned import WarmStartNet
+            log("cnn layers (ms):", cnn_layer_timings(WarmStartNet.load(), tuple(nr), iters=args.steps))
         # a plain device copy of 256 MiB, to calibrate the HBM byte counters
         a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
@@ -295,17 +395,23 @@ def main():
     v64 = float(np.mean(t64stop))
     v64_thr = float(np.mean(t64))
     v64p = float(np.mean(t64pstop))
-    # roofline of the dominant kernel: the x-axis C2C pass on 3 spectra (FP32)
+    # roofline of the kernel with the most time per headline step: the CNN
+    # middle layer (4 launches per step inside the learned start)
     pt32 = pass_timings(g32, rho32)
     pt64 = pass_timings(g64, rho64)
-    nc = nr[0] * nr[1] * (nr[2] // 2 + 1)
-    dom = max(("y_c2c", "x_c2c"), key=lambda k: pt32[k])
-    bytes_dom = 6 * nc * 8  # read + write 3 complex64 half spectra
-    achieved = bytes_dom / (pt32[dom] * 1e-3) / 1e9
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=8000.0, unit="GB/s", frac=round(achieved / 8000.0, 4),
-                traffic=TRAFFIC_X_C2C_FP32, traffic_source="rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch of "
-                "k_cols_f<float,160,16,false> (3 spectra), calibrated on a 256 MiB copy", kernel="k_cols<float> %s pass, 3 spectra of 160x160x81 complex64" % dom,
-                algorithmic_bytes=bytes_dom, avg_ms=pt32[dom], pass_ms_fp32=pt32, pass_ms_fp64=pt64)
+    cnn_ms = cnn_layer_timings(net, tuple(nr))
+    mid_ms = float(np.mean([v for k, v in cnn_ms.items() if k.startswith("mid")]))
+    npts = nr[0] * nr[1] * nr[2]
+    flop_mid = 2 * 27 * 32 * 32 * npts  # 27 taps x 32 x 32 channels, multiply + add
+    achieved = flop_mid / (mid_ms * 1e-3) / 1e12
+    roof = dict(bound="mfma", achieved=round(achieved, 1), peak=2500.0, unit="TFLOP/s",
+                frac=round(achieved / 2500.0, 4), traffic=TRAFFIC_CNN_MID,
+                traffic_source=TRAFFIC_CNN_MID_SOURCE,
+                kernel="k_cnn_mid_rows<0,5>: warm-start CNN middle layer (32 -> 32 channels, 3^3 dilated taps, "
+                       "BF16 MFMA with FP32 accumulation) at 160^3",
+                algorithmic_flop=flop_mid, algorithmic_bytes=2 * 64 * npts, avg_ms=round(mid_ms, 5),
+                per_step_ms=round(4 * mid_ms, 4), cnn_layer_ms={k: round(v, 5) for k, v in cnn_ms.items()},
+                passes_fp32=pass_roofline(nr, pt32, 4), passes_fp64=pass_roofline(nr, pt64, 8))
     n_evals_to_thr = None
     if opt64 is not None:
         n_evals_to_thr = opt64.stats["n_eval_v"] + opt64.stats["n_eval_ev"]
@@ -330,6 +436,9 @@ def main():
                             fp32_superposition_start_s=round(float(np.mean(times["superposition"])), 5),
                             fp32_uniform_start_s=round(float(np.mean(times["uniform"])), 5),
                             fp32_learned_plain_cg_s=round(float(np.mean(times["learned_plain_cg"])), 5),
+                            speedup_learned_vs_uniform_start=round(float(np.mean(times["uniform"])) / v32, 3),
+                            speedup_learned_vs_superposition_start=round(
+                                float(np.mean(times["superposition"])) / v32, 3),
                             stop_rule="|dE| < 1e-6 Ha/atom * nat over 2 steps (all forms)",
                             runs=infos),
                 roofline=roof, cpu_baseline=cpu)

@@ -44,11 +44,13 @@ SIGNATURES = {
     "ofdft_set_kedf": (_I, [_P, _D, _D, _D, _D, _D, _D]),
     "ofdft_set_terms": (_I, [_P, ctypes.c_uint]),
     "ofdft_set_vloc": (_I, [_P, _P]),
+    "ofdft_set_core": (_I, [_P, _P]),
     "ofdft_set_eion": (_I, [_P, _D]),
     "ofdft_mgp_kernel_table": (_I, [_I, _D, _I, _I, _DP, _DP, _DP]),
     "ofdft_set_lmgp": (_I, [_P, _I, _D, _DP, _DP, _D]),
     "ofdft_lmgp_reset": (_I, [_P]),
     "ofdft_eval": (_I, [_P, _P, ctypes.c_uint, _P, _P, _P]),
+    "ofdft_eval_fields": (_I, [_P, _P, ctypes.c_uint, _P, _P, _P, _P, _P]),
     "ofdft_dot": (_I, [_P, _P, _P, _P, _P]),
     "ofdft_bench_pass": (_I, [_P, _I, _I, _I, _P, _DP, _P]),
     "ofdft_atomic_density": (_I, [_P, _I, _I, _IP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), _DP, _DP, _DP, _DP, _I,
@@ -59,6 +61,7 @@ SIGNATURES = {
     "ofdft_ewald": (_I, [_P, _I, _DP, _DP, _DP, _I, _I, _D, _DP, _P]),
     "ofdft_optimize": (_I, [_P, _P, _P, _I, _I, _DP, _DP, _DP, _I, _P]),
     "ofdft_set_tn_precond": (_I, [_P, _I, _D, _DP]),
+    "ofdft_plan_reserve_lbfgs": (_I, [_P, _I]),
     "ofdft_local_pp_forces": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
                                    _DP, _P]),
     "ofdft_local_pp_stress": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
@@ -67,6 +70,8 @@ SIGNATURES = {
     "ofdft_functional_stress": (_I, [_P, _P, _DP, _DP, _P]),
     "ofdft_dcsrch": (_I, [_DP, _D, _D, _D, _D, _D, _D, _D, _IP, _DP]),
     "ofdft_cn_step": (_I, [_P, _P, _P, _D, _D, _D, _I, _P, _DP, _P]),
+    "ofdft_cn_solver_workspace_bytes": (ctypes.c_size_t, [_P, _I, _I]),
+    "ofdft_cn_step_solver": (_I, [_P, _I, _I, _P, _P, _D, _D, _D, _I, _P, _DP, _P, ctypes.c_size_t, _P]),
     "ofdft_cn_batch_workspace_bytes": (ctypes.c_size_t, [_P, _I]),
     "ofdft_cn_step_batch": (_I, [_P, _I, _P, _P, _D, _D, _D, _I, _P, _DP, _P, ctypes.c_size_t, _P]),
     "ofdft_cnn_conv_in": (_I, [_IP, _I, _P, _P, _P, _P, _P]),

@@ -189,6 +189,7 @@ int ofdft_plan_destroy(ofdft_plan *P) {
         if (P->scal) (void)hipFree(P->scal);
         if (P->pinned) (void)hipHostFree(P->pinned);
         if (P->tn_pre) (void)hipFree(P->tn_pre);
+        if (P->lbfgs_buf) (void)hipFree(P->lbfgs_buf);
         plan_setup_free(*P);
         delete P;
     });
@@ -214,6 +215,21 @@ int ofdft_set_tn_precond(ofdft_plan *P, int n, double dq, const double *table) {
         OFDFT_HIP(hipMemcpy(P->tn_pre, table, sizeof(double) * n, hipMemcpyHostToDevice));
         P->tn_pre_n = n;
         P->tn_pre_dq = dq;
+    });
+}
+
+int ofdft_plan_reserve_lbfgs(ofdft_plan *P, int maxcor) {
+    return guard([&] {
+        OFDFT_REQUIRE(P != nullptr, "plan_reserve_lbfgs: null plan");
+        OFDFT_REQUIRE(maxcor >= 0 && maxcor + 1 <= 16, "plan_reserve_lbfgs: maxcor must be in [0, 15]");
+        const int pairs = maxcor + 1;
+        if (pairs <= P->lbfgs_pairs) return;
+        const size_t rs = P->prec == OFDFT_FP64 ? 8 : 4;
+        if (P->lbfgs_buf) OFDFT_HIP(hipFree(P->lbfgs_buf));
+        P->lbfgs_buf = nullptr;
+        P->lbfgs_pairs = 0;
+        OFDFT_HIP(hipMalloc(&P->lbfgs_buf, (size_t)2 * pairs * P->N * rs));
+        P->lbfgs_pairs = pairs;
     });
 }
 
@@ -265,7 +281,6 @@ int ofdft_plan_set_fft_kernels(ofdft_plan *P, int fixed) {
 int ofdft_set_kedf(ofdft_plan *P, double x, double y, double alpha, double beta, double rho0, double sigma) {
     return guard([&] {
         OFDFT_REQUIRE(P, "null plan");
-        OFDFT_REQUIRE(std::fabs(alpha - beta) < 1e-9, "WT with alpha != beta is not supported");
         P->x = x;
         P->y = y;
         P->alpha = alpha;
@@ -281,6 +296,13 @@ int ofdft_set_terms(ofdft_plan *P, unsigned terms) {
         OFDFT_REQUIRE((terms & ~127u) == 0, "unknown term bits");
         OFDFT_REQUIRE(!((terms & OFDFT_TERM_WT) && (terms & OFDFT_TERM_LMGP)), "WT and LMGP are exclusive");
         P->terms = terms;
+    });
+}
+
+int ofdft_set_core(ofdft_plan *P, const void *core) {
+    return guard([&] {
+        OFDFT_REQUIRE(P != nullptr, "set_core: null plan");
+        P->core = core;
     });
 }
 
@@ -309,6 +331,20 @@ int ofdft_eval(ofdft_plan *P, const void *rho, unsigned calc, void *pot, double 
     });
 }
 
+int ofdft_eval_fields(ofdft_plan *P, const void *rho, unsigned calc, void *pot, double *energies, void *edens,
+                      void *v2rho2, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho && (edens || v2rho2), "eval_fields: bad arguments");
+        OFDFT_REQUIRE(!((P->terms & OFDFT_TERM_PP) && P->vloc == nullptr), "eval_fields: PP term set but no vloc");
+        if (P->prec == OFDFT_FP64)
+            eval_fields<double>(*P, (const double *)rho, calc, (double *)pot, energies, (double *)edens,
+                                (double *)v2rho2, S(stream));
+        else
+            eval_fields<float>(*P, (const float *)rho, calc, (float *)pot, energies, (float *)edens, (float *)v2rho2,
+                               S(stream));
+    });
+}
+
 int ofdft_dot(ofdft_plan *P, const void *a, const void *b, double *out, void *stream) {
     return guard([&] {
         OFDFT_REQUIRE(P && a && out, "dot: bad arguments");
@@ -323,8 +359,8 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                      void *stream) {
     return guard([&] {
         OFDFT_REQUIRE(P && avg_ms && iters >= 1 && batch >= 1 && batch <= 3, "bench_pass: bad arguments");
-        OFDFT_REQUIRE(which >= 0 && which <= 5, "bench_pass: unknown pass");
-        OFDFT_REQUIRE((which != 0 && which != 5) || rho != nullptr, "bench_pass: the z pass needs rho");
+        OFDFT_REQUIRE(which >= 0 && which <= 7, "bench_pass: unknown pass");
+        OFDFT_REQUIRE((which != 0 && which != 5 && which != 7) || rho != nullptr, "bench_pass: this pass needs rho");
         hipStream_t s = S(stream);
         hipEvent_t e0, e1;
         OFDFT_HIP(hipEventCreate(&e0));
@@ -348,6 +384,8 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                 } else if (which == 5) {
                     eval<double>(*P, (const double *)rho, OFDFT_CALC_E | OFDFT_CALC_V, (double *)P->aux,
                                  P->scal + 100, s);
+                } else if (which == 6 || which == 7) {
+                    bench_eval_stage<double>(*P, which - 6, (const double *)rho, (double *)P->aux, s);
                 } else {
                     fft_c2c<double>(*P, spec, spec, false, 1, P->N, 1.0, s);
                 }
@@ -369,6 +407,8 @@ int ofdft_bench_pass(ofdft_plan *P, int which, int batch, int iters, const void 
                 } else if (which == 5) {
                     eval<float>(*P, (const float *)rho, OFDFT_CALC_E | OFDFT_CALC_V, (float *)P->aux, P->scal + 100,
                                 s);
+                } else if (which == 6 || which == 7) {
+                    bench_eval_stage<float>(*P, which - 6, (const float *)rho, (float *)P->aux, s);
                 } else {
                     fft_c2c<float>(*P, spec, spec, false, 1, P->N, 1.0, s);
                 }

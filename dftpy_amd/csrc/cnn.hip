@@ -528,6 +528,179 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
     if (MODE == 0 && nq > 0) copy_out(nq - 1);
 }
 
+// Ring-staged forward (inference: no ReLU output kept) for nz % 32 == 0,
+// ny % (4 d) == 0, nx % d == 0 and d <= 4.  A step computes 4 output rows
+// y0 + k d (k = 0..3) of one x plane for one 32-point z block.  The steps of
+// a walk visit x = c, c + d, c + 2d, ... (one residue class mod d, nx / d
+// planes), so the input planes x - d, x, x + d of step j are planes j - 1,
+// j, j + 1 of the walk: a ring of 4 LDS plane slots (6 staged rows y0 +
+// (r - 1) d, 32 + 2 DMAX z points with the dz halo, 32 channels) holds the
+// three in use and receives the next one, whose global loads are issued at
+// the top of the step and land after its MFMAs.  Each input row is staged
+// 1.5 (y) x 1.25 (z halo) times per layer instead of 4.5 times, and the
+// weights never pass through LDS: wave (z tile, channel half) keeps the B
+// operands of all 27 taps for its 16 output channels in registers.
+//
+// The concatenated walks (XCD k takes the k-th eighth of them, its
+// workgroups contiguous sub-ranges) are split into equal contiguous step
+// ranges, one per workgroup; a range start or a walk boundary loads the
+// walk's first three planes before computing.  Results go through an LDS
+// output stage (two, alternating) and leave as 16-byte stores at the top of
+// the next step.
+constexpr int RING_Z = 32;                               // output z points per step
+constexpr int RING_DMAX = 4;                             // largest dilation
+constexpr int RING_ZS = RING_Z + 2 * RING_DMAX;          // staged z points
+constexpr int RING_PLANE = 6 * RING_ZS * 4;              // bf16x8 per plane slot
+constexpr int RING_OUT = 4 * RING_Z * 4;                 // bf16x8 per output stage
+constexpr int RING_THREADS = 256;                        // 2 z tiles x 2 channel halves
+constexpr int RING_PER = (RING_PLANE + RING_THREADS - 1) / RING_THREADS;
+constexpr size_t RING_LDS = (size_t)(4 * RING_PLANE + 2 * RING_OUT) * 16;
+
+__global__ void __launch_bounds__(RING_THREADS, 2)
+    k_cnn_mid_ring(Geo g, const __bf16 *__restrict__ xin, const __bf16 *__restrict__ w,
+                   const float *__restrict__ bias, __bf16 *__restrict__ out) {
+    extern __shared__ bf16x8 lds[];  // [4][6][RING_ZS][4] plane slots, [2][4][RING_Z][4] output stages
+    bf16x8 *ostage = lds + 4 * RING_PLANE;
+    const int d = g.d, nx = g.nx, ny = g.ny, nz = g.nz;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int zt = wv & 1, hh = wv >> 1;  // z tile, channel half
+    const int m = lane & 15, kq = lane >> 4;
+    const long long nyz = (long long)ny * nz;
+    const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
+    const bf16x8 *wvv = reinterpret_cast<const bf16x8 *>(w);
+    // this wave's B operands: W[t][co = hh*16 + m][ci = kq*8 .. kq*8+7]
+    bf16x8 bw[NTAP];
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) bw[t] = wvv[(t * CH + hh * 16 + m) * 4 + kq];
+    const float bco = bias[hh * 16 + m];
+    // walks: w = (c * nzb + zb) * ngroups + gy, steps j = 0 .. nx/d - 1
+    const int ngroups = ny / 4, nzb = nz / RING_Z, walk = nx / d;
+    const long long total = (long long)ngroups * nzb * d * walk;
+    const int nwg = gridDim.x, per_xcd = nwg / ROWS_XCDS;
+    const int xcd = blockIdx.x % ROWS_XCDS, slotw = blockIdx.x / ROWS_XCDS;
+    const long long xa = total * xcd / ROWS_XCDS, xb = total * (xcd + 1) / ROWS_XCDS;
+    const long long s_begin = xa + (xb - xa) * slotw / per_xcd, s_end = xa + (xb - xa) * (slotw + 1) / per_xcd;
+    struct Step {
+        int c, zb, y0, j;
+    };
+    auto step_of = [&](long long s) {
+        Step st;
+        const long long wk = s / walk;
+        st.j = (int)(s - wk * walk);
+        const int gy = (int)(wk % ngroups);
+        const long long r = wk / ngroups;
+        st.zb = (int)(r % nzb);
+        st.c = (int)(r / nzb);
+        st.y0 = (gy / d) * 4 * d + gy % d;
+        return st;
+    };
+    // global loads of walk plane i (x = c + d i mod nx) of a step's rows
+    bf16x8 stg[RING_PER];
+    auto fetch = [&](const Step &st, int i) {
+        const int xp = ((st.c + d * i) % nx + nx) % nx;
+        const int z0 = st.zb * RING_Z - RING_DMAX;
+#pragma unroll
+        for (int q = 0; q < RING_PER; ++q) {
+            const int e = tid + q * RING_THREADS;
+            if (e < RING_PLANE) {
+                const int r = e / (RING_ZS * 4), rem = e - r * (RING_ZS * 4);
+                const int zz = rem >> 2, kk = rem & 3;
+                const int yr = wrapi(st.y0 + (r - 1) * d, ny);
+                int z = z0 + zz;
+                z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+                stg[q] = xv[((long long)xp * nyz + (long long)yr * nz + z) * 4 + kk];
+            }
+        }
+    };
+    auto land = [&](int slot) {
+        bf16x8 *dst = lds + slot * RING_PLANE;
+#pragma unroll
+        for (int q = 0; q < RING_PER; ++q) {
+            const int e = tid + q * RING_THREADS;
+            if (e < RING_PLANE) dst[e] = stg[q];
+        }
+    };
+    auto copy_out = [&](const Step &st, int buf) {
+        const int x = st.c + d * st.j;
+        const bf16x8 *src = ostage + buf * RING_OUT;
+        bf16x8 *ov = reinterpret_cast<bf16x8 *>(out);
+#pragma unroll
+        for (int q = 0; q < RING_OUT / RING_THREADS; ++q) {
+            const int e = tid + q * RING_THREADS;
+            const int k = e / (RING_Z * 4), rem = e - k * (RING_Z * 4);
+            ov[((long long)x * nyz + (long long)(st.y0 + k * d) * nz + st.zb * RING_Z) * 4 + rem] = src[e];
+        }
+    };
+    int head = 0;  // slot of walk plane j - 1; j at head + 1, j + 1 at head + 2
+#pragma unroll 1
+    for (long long s = s_begin; s < s_end; ++s) {
+        const Step st = step_of(s);
+        if (s > s_begin) copy_out(step_of(s - 1), (int)((s - 1) & 1));
+        if (s == s_begin || st.j == 0) {
+            // the walk's first three planes (the previous step's barrier
+            // freed every slot)
+#pragma unroll 1
+            for (int i = -1; i <= 1; ++i) {
+                fetch(st, st.j + i);
+                land((head + 1 + i) & 3);
+            }
+            __syncthreads();
+        }
+        const bool pre = s + 1 < s_end && st.j + 1 < walk;
+        if (pre) fetch(st, st.j + 2);
+        floatx4 acc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // the 6 staged rows of one (dx, dz) tap column are read together, one
+        // column ahead of its 12 MFMAs (two register sets), and the MFMAs
+        // cycle through the 4 output rows so that consecutive ones never
+        // depend on each other
+        bf16x8 av[2][6];
+        auto load_a = [&](int set, int t) {
+            const int dx = t / 3, dz = t % 3;
+            const bf16x8 *base = lds + ((head + dx) & 3) * RING_PLANE;
+            const int zi = RING_DMAX + zt * 16 + m + (dz - 1) * d;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) av[set][r] = base[(r * RING_ZS + zi) * 4 + kq];
+        };
+        load_a(0, 0);
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int cur = t & 1;
+            if (t + 1 < 9) load_a(cur ^ 1, t + 1);
+            const int dx = t / 3, dz = t % 3;
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[cur][k + dy], bw[(dx * 3 + dy) * 3 + dz],
+                                                                     acc[k], 0, 0, 0);
+            }
+        }
+        // out = x + relu(acc + b); the residual x is staged row k + 1 of the
+        // centre plane
+        {
+            const __bf16 *rs = reinterpret_cast<const __bf16 *>(lds + ((head + 1) & 3) * RING_PLANE);
+            __bf16 *os = reinterpret_cast<__bf16 *>(ostage + (int)(s & 1) * RING_OUT);
+            const int co = hh * 16 + m;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int z = zt * 16 + kq * 4 + jj;
+                    const float r = fmaxf(acc[k][jj] + bco, 0.0f);
+                    const float x0 = (float)rs[((k + 1) * RING_ZS + RING_DMAX + z) * CH + co];
+                    os[(k * RING_Z + z) * CH + co] = (__bf16)(x0 + r);
+                }
+            }
+        }
+        if (pre) land((head + 3) & 3);
+        __syncthreads();
+        head = (head + 1) & 3;
+    }
+    if (s_end > s_begin) copy_out(step_of(s_end - 1), (int)((s_end - 1) & 1));
+}
+
 // ------------------------------------------------------------ layer 6 (32 -> 1)
 __global__ void __launch_bounds__(256) k_cnn_out(Geo g, const __bf16 *__restrict__ h, const float *__restrict__ w,
                                                  const float *__restrict__ b, float *__restrict__ out) {
@@ -1074,6 +1247,14 @@ int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const 
         OFDFT_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (forward) or 1 (backward-data)");
         OFDFT_REQUIRE(mode == 1 || bias, "forward mode needs the bias");
         OFDFT_REQUIRE(mode == 0 || (resid && mask), "backward mode needs resid and mask");
+        if (mode == 0 && out1 == nullptr && g.nz % RING_Z == 0 && g.ny % (4 * g.d) == 0 && g.nx % g.d == 0 &&
+            g.d <= RING_DMAX) {
+            rows_lds_limit((const void *)&k_cnn_mid_ring, RING_LDS);
+            hipLaunchKernelGGL(k_cnn_mid_ring, dim3((unsigned)(2 * rows_workgroups())), dim3(RING_THREADS), RING_LDS,
+                               St(stream), g, xb, wb, bias, reinterpret_cast<__bf16 *>(out0));
+            OFDFT_HIP(hipGetLastError());
+            return;
+        }
         if (g.nz % 32 == 0 && g.nz <= 32 * ROWS_MAX_MT && g.ny % (4 * g.d) == 0) {
             // two stage buffers of 6 rows and 9 taps' weights each
             const size_t lds = 2 * ((size_t)6 * g.nz * CH * sizeof(__bf16) + (size_t)9 * CH * CH * sizeof(__bf16));

@@ -74,6 +74,12 @@ __global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
             v.x = (T)(v.x * f);
             v.y = (T)(v.y * f);
         }
+        if (p.s_wt2 >= 0) {
+            cpx<T> &v = spec[p.s_wt2 * p.bs + i];
+            const double f = k.wt * p.scale;
+            v.x = (T)(v.x * f);
+            v.y = (T)(v.y * f);
+        }
     }
 }
 
@@ -99,6 +105,9 @@ template <typename T>
 bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
                   long long vp, const void *tw, const GParams &gp, hipStream_t s);
 
+// the WT exponents take the single-spectrum form of wt.py:18-22
+static bool wt_alpha_is_beta(const Plan &P) { return std::fabs(P.alpha - P.beta) < 1e-9; }
+
 // the x pass can run fused with the G-space multiply (size-specialised x kernel)
 static bool x_fusable(const Plan &P) { return P.use_fixed && P.nr[0] > 1 && P.nr[1] * P.nr[2] > 1 && fixed_size(P.nr[0]); }
 
@@ -112,6 +121,7 @@ static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long
     if (gp.s_vw >= 0) gp.kind[gp.s_vw] = 0;
     if (gp.s_wt >= 0) gp.kind[gp.s_wt] = 1;
     if (gp.s_h >= 0) gp.kind[gp.s_h] = 2;
+    if (gp.s_wt2 >= 0) gp.kind[gp.s_wt2] = 1;
     if (fused) {
         const long long ncols = (long long)P.nr[1] * pitch;
         OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], P.var_cols[0], spec, ns, ncp, ncols, pitch, P.nzc, P.tw[0], gp, s),
@@ -169,6 +179,8 @@ __global__ void __launch_bounds__(256) k_gspace_energy(const cpx<T> *__restrict_
 
 struct PostParams {
     int s_vw, s_wt, s_h;
+    int s_wt2 = -1;  // alpha != beta: IFFT(K FFT(rho^alpha))
+    const void *core = nullptr;  // NLCC: the XC term sees rho + core
     int b56;  // beta == 5/6
     unsigned terms;
     double x, y, alpha, beta;
@@ -219,15 +231,29 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             if (p.want_e && r > (CT)0) acc[1] += (double)(r * vw);
             v += vw;
         }
-        if (p.s_wt >= 0) {
+        if (p.s_wt >= 0 && p.s_wt2 < 0) {
             const CT c = (CT)work[p.s_wt * p.bs + i];
             const CT rb_rc = p.b56 ? c_rc * rsq_p<CT>(rc) : pow(rc, (CT)p.beta) * rcp_p<CT>(rc);  // rho^beta / rho
             const CT vnl = (CT)(2.0 * p.alpha) * rb_rc * c;
             if (p.want_e) acc[2] += (double)(vnl * r) / (2.0 * p.alpha);
             v += vnl;
+        } else if (p.s_wt >= 0) {
+            // alpha != beta (wt.py:23-25,44-56): cb = K * rho^beta, ca = K * rho^alpha
+            const CT cb = (CT)work[p.s_wt * p.bs + i];
+            const CT ca = (CT)work[p.s_wt2 * p.bs + i];
+            const CT ra = pow(rc, (CT)p.alpha), rb = pow(rc, (CT)p.beta);
+            const CT irc = rcp_p<CT>(rc);
+            if (p.want_e) acc[2] += (double)(ra * cb);
+            v += (CT)p.alpha * ra * irc * cb + (CT)p.beta * rb * irc * ca;
         }
         if (p.terms & OFDFT_TERM_XC) {
-            CT rc3 = cr;
+            // xc/__init__.py:59-74: with a core density the XC term is
+            // evaluated at rho + core (energy sum_r eps(rho+core)(rho+core) dV)
+            CT rx = r, rc3 = cr;
+            if (p.core) {
+                rx = r + (CT)reinterpret_cast<const T *>(p.core)[i];
+                rc3 = cbrt(rx);
+            }
             if (rc3 < (CT)1e-30) rc3 = (CT)1e-30;
             const CT rs = cbrt34pi * rcp_p<CT>(rc3);
             CT ec, vc;
@@ -243,7 +269,7 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
                 vc = ((CT)PZ_G + (CT)(7.0 / 6.0 * PZ_G * PZ_B1) * sr + (CT)(4.0 / 3.0 * PZ_G * PZ_B2) * rs) *
                      (iden * iden);
             }
-            if (p.want_e) acc[3] += (double)((cx_e * rc3 + ec) * r);
+            if (p.want_e) acc[3] += (double)((cx_e * rc3 + ec) * rx);
             v += cx_v * rc3 + vc;
         }
         if (p.s_h >= 0) {
@@ -332,7 +358,8 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     gp.bs = ncp;
     gp.pitch = (int)pitch;
     PostParams pp;
-    pp.s_vw = pp.s_wt = pp.s_h = -1;
+    pp.s_vw = pp.s_wt = pp.s_h = pp.s_wt2 = -1;
+    pp.core = (P.terms & OFDFT_TERM_XC) ? P.core : nullptr;
     pp.terms = P.terms;
     pp.x = P.x;
     pp.y = P.y;
@@ -353,6 +380,53 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     OFDFT_HIP(hipGetLastError());
 }
 
+// G-space parameters of the pipeline's spectra (slots s_vw, s_wt, s_h of ns)
+static GParams pipeline_gparams(const Plan &P, int s_vw, int s_wt, int s_h, bool auto_rho0, double *rho0_cache,
+                                long long ncp, long long pitch, int s_wt2 = -1) {
+    GParams gp;
+    gp.s_wt2 = s_wt2;
+    for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
+    gp.nx = P.nr[0];
+    gp.ny = P.nr[1];
+    gp.nz = P.nr[2];
+    gp.nzc = P.nzc;
+    gp.full = P.full;
+    gp.scale = 1.0 / (double)P.N;
+    gp.s_vw = s_vw;
+    gp.s_wt = s_wt;
+    gp.s_h = s_h;
+    gp.sigma = P.sigma;
+    gp.lbda = P.x;  // WT kernel lambda = x, mu = 1 (wt.py:66, kernel.py:331-334)
+    gp.alpha = P.alpha;
+    gp.beta = P.beta;
+    gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
+    gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
+    gp.bs = ncp;
+    gp.pitch = (int)pitch;
+    return gp;
+}
+
+static PostParams pipeline_post(const Plan &P, int s_vw, int s_wt, int s_h, bool want_v, bool want_e,
+                                int s_wt2 = -1) {
+    PostParams pp;
+    pp.s_wt2 = s_wt2;
+    pp.core = (P.terms & OFDFT_TERM_XC) ? P.core : nullptr;
+    pp.s_vw = s_vw;
+    pp.s_wt = s_wt;
+    pp.s_h = s_h;
+    pp.terms = P.terms;
+    pp.x = P.x;
+    pp.y = P.y;
+    pp.alpha = P.alpha;
+    pp.beta = P.beta;
+    pp.b56 = beta_is_5_6(P.beta);
+    pp.n = P.N;
+    pp.bs = P.N;
+    pp.want_v = want_v;
+    pp.want_e = want_e;
+    return pp;
+}
+
 template <typename T>
 void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStream_t s) {
     const unsigned terms = P.terms;
@@ -360,11 +434,13 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
     const int s_vw = (terms & OFDFT_TERM_VW) ? ns++ : -1;
     const int s_wt = (terms & OFDFT_TERM_WT) ? ns++ : -1;
     const int s_h = (terms & OFDFT_TERM_HARTREE) ? ns++ : -1;
+    // WT with alpha != beta transforms rho^alpha as a fourth spectrum
+    const int s_wt2 = (s_wt >= 0 && !wt_alpha_is_beta(P)) ? ns : -1;
     cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
     T *work = reinterpret_cast<T *>(P.work);
     double *rho0_cache = P.scal + 0;
     const bool energy_only = (calc & OFDFT_CALC_E) && energies != nullptr && !((calc & OFDFT_CALC_V) && pot) &&
-                             !(terms & OFDFT_TERM_LMGP);
+                             !(terms & OFDFT_TERM_LMGP) && s_wt2 < 0;
     if (energy_only && ns > 0) {
         eval_energy<T>(P, rho, ns, s_vw, s_wt, s_h, energies, s);
         return;
@@ -386,42 +462,21 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
             hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
             OFDFT_HIP(hipGetLastError());
         }
-        GParams gp;
-        for (int i = 0; i < 9; ++i) gp.B[i] = P.B[i];
-        gp.nx = P.nr[0];
-        gp.ny = P.nr[1];
-        gp.nz = P.nr[2];
-        gp.nzc = P.nzc;
-        gp.full = P.full;
-        gp.scale = 1.0 / (double)P.N;
-        gp.s_vw = s_vw;
-        gp.s_wt = s_wt;
-        gp.s_h = s_h;
-        gp.sigma = P.sigma;
-        gp.lbda = P.x;  // WT kernel lambda = x, mu = 1 (wt.py:66, kernel.py:331-334)
-        gp.alpha = P.alpha;
-        gp.beta = P.beta;
-        gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
-        gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
-        gp.bs = ncp;
-        gp.pitch = (int)pitch;
-        gspace_apply<T>(P, spec, ns, ncp, pitch, gp, fused, s);
-        fft_c2r<T>(P, spec, spec, work, ns, ncp, ncp, P.N, 1.0, s, pitch, fused);
+        int nsa = ns;
+        if (s_wt2 >= 0) {
+            Prologue pa{};
+            pa.code[0] = OP_POW_BETA;  // max(rho, 1e-30)^alpha
+            pa.beta = P.alpha;
+            pa.b56 = beta_is_5_6(P.alpha);
+            fft_r2c<T>(P, rho, spec + (long long)s_wt2 * ncp, 1, 0, ncp, 1.0, &pa, nullptr, s, pitch, fused);
+            nsa = ns + 1;
+        }
+        GParams gp = pipeline_gparams(P, s_vw, s_wt, s_h, auto_rho0, rho0_cache, ncp, pitch, s_wt2);
+        gspace_apply<T>(P, spec, nsa, ncp, pitch, gp, fused, s);
+        fft_c2r<T>(P, spec, spec, work, nsa, ncp, ncp, P.N, 1.0, s, pitch, fused);
     }
-    PostParams pp;
-    pp.s_vw = s_vw;
-    pp.s_wt = s_wt;
-    pp.s_h = s_h;
-    pp.terms = terms;
-    pp.x = P.x;
-    pp.y = P.y;
-    pp.alpha = P.alpha;
-    pp.beta = P.beta;
-    pp.b56 = beta_is_5_6(P.beta);
-    pp.n = P.N;
-    pp.bs = P.N;
-    pp.want_v = (calc & OFDFT_CALC_V) && pot != nullptr;
-    pp.want_e = (calc & OFDFT_CALC_E) && energies != nullptr;
+    PostParams pp = pipeline_post(P, s_vw, s_wt, s_h, (calc & OFDFT_CALC_V) && pot != nullptr,
+                                  (calc & OFDFT_CALC_E) && energies != nullptr, s_wt2);
     const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
     const int nb = std::min(reduce_blocks(P.N), 1024);
     hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, P.red, pp);
@@ -438,6 +493,124 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
 
 template void eval<float>(Plan &, const float *, unsigned, float *, double *, hipStream_t);
 template void eval<double>(Plan &, const double *, unsigned, double *, double *, hipStream_t);
+
+// Energy density ('D') and second derivative ('V2') fields of the terms that
+// have them in the reference, from rho and the inverse transforms an E+V
+// evaluation left in the work slots: TF C rho^(5/3) (tf.py:26-35,68-73,
+// unscaled by x as there) and x 10/9 C rho^(-1/3) (tf.py:47-50,77-79); vW
+// y v_vW rho (vw.py:125-126); WT rho^alpha (K * rho^beta) (wt.py:44-56,
+// 123-129, the alpha == beta form v rho / (2 alpha)); PZ-LDA d v_xc / d rho
+// (semilocal_xc.py:291-303).  Hartree and the local PP have neither.
+struct FieldParams {
+    int s_vw, s_wt, s_wt2;
+    const void *core;  // XC at rho + core
+    unsigned terms;
+    double x, y, alpha, beta;
+    long long n, bs;
+};
+
+template <typename T>
+__global__ void k_fields(const T *__restrict__ rho, const T *__restrict__ work, T *__restrict__ edens,
+                         T *__restrict__ v2, FieldParams p) {
+    const double ctf = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double r = (double)rho[i];
+        const double rc = r < 1e-30 ? 1e-30 : r;
+        double ed = 0.0, f2 = 0.0;
+        if (p.terms & OFDFT_TERM_TF) {
+            const double c = cbrt(r);
+            ed += ctf * r * c * c;
+            f2 += p.x * 10.0 / 9.0 * ctf / c;
+        }
+        if (p.s_vw >= 0) {
+            const double isq = 1.0 / sqrt((r > 0.0 && r < 1e-30) ? r : rc);
+            ed += p.y * (double)work[p.s_vw * p.bs + i] * isq * r;
+        }
+        if (p.s_wt >= 0) {
+            const double cb = (double)work[p.s_wt * p.bs + i];
+            if (p.s_wt2 < 0)
+                ed += pow(rc, p.beta - 1.0) * cb * r;
+            else
+                ed += pow(rc, p.alpha) * cb;
+        }
+        if (p.terms & OFDFT_TERM_XC) {
+            const double rx = p.core ? r + (double)reinterpret_cast<const T *>(p.core)[i] : r;
+            const double cr = cbrt(rx);
+            const double rs = cbrt(3.0 / (4.0 * M_PI)) / (cr < 1e-30 ? 1e-30 : cr);
+            const double fx = -cbrt(3.0 / M_PI) / 3.0 * cr / rx;
+            double fc;
+            if (rs < 1.0) {
+                fc = -PZ_A / 3.0 - (PZ_C / 9.0 * (log(rs) * 2.0 + 1.0) + PZ_D * 2.0 / 9.0) * rs;
+            } else {
+                const double ta = PZ_B1 * sqrt(rs), tb = PZ_B2 * rs, de = 1.0 + ta + tb;
+                fc = PZ_G / 36.0 * (5.0 * ta + 7.0 * ta * ta + 8.0 * tb + 16.0 * tb * tb + 21.0 * ta * tb) /
+                     (de * de * de);
+            }
+            f2 += fx + fc / rx;
+        }
+        if (edens) edens[i] = (T)ed;
+        if (v2) v2[i] = (T)f2;
+    }
+}
+
+template <typename T>
+void eval_fields(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, T *edens, T *v2, hipStream_t s) {
+    OFDFT_REQUIRE(!(P.terms & OFDFT_TERM_LMGP) || edens == nullptr,
+                  "the energy density of the LMGP functional is not implemented");
+    // the full pipeline (inverse transforms in the work slots), the potential
+    // into the caller's buffer or the plan's scratch field
+    T *vout = pot ? pot : reinterpret_cast<T *>(P.aux);
+    eval<T>(P, rho, calc | OFDFT_CALC_V, vout, energies, s);
+    FieldParams fp;
+    int ns = 0;
+    fp.s_vw = (P.terms & OFDFT_TERM_VW) ? ns++ : -1;
+    fp.s_wt = (P.terms & OFDFT_TERM_WT) ? ns++ : -1;
+    if (P.terms & OFDFT_TERM_HARTREE) ns++;
+    fp.s_wt2 = (fp.s_wt >= 0 && !wt_alpha_is_beta(P)) ? ns : -1;
+    fp.terms = P.terms;
+    fp.core = (P.terms & OFDFT_TERM_XC) ? P.core : nullptr;
+    fp.x = P.x;
+    fp.y = P.y;
+    fp.alpha = P.alpha;
+    fp.beta = P.beta;
+    fp.n = P.N;
+    fp.bs = P.N;
+    hipLaunchKernelGGL((k_fields<T>), dim3(grid_blocks(P.N)), dim3(256), 0, s, rho,
+                       reinterpret_cast<const T *>(P.work), edens, v2, fp);
+    OFDFT_HIP(hipGetLastError());
+}
+template void eval_fields<float>(Plan &, const float *, unsigned, float *, double *, float *, float *, hipStream_t);
+template void eval_fields<double>(Plan &, const double *, unsigned, double *, double *, double *, double *,
+                                  hipStream_t);
+
+// One stage of the E+V pipeline on the plan's workspace, for timing (the
+// spectra and work fields hold whatever the last evaluation left): stage 0 =
+// the x pass with the G-space multiply of every spectrum, in place; stage 1 =
+// the point-wise post pass (potential into pot, energy partials).
+template <typename T> void bench_eval_stage(Plan &P, int stage, const T *rho, T *pot, hipStream_t s) {
+    const unsigned terms = P.terms;
+    int ns = 0;
+    const int s_vw = (terms & OFDFT_TERM_VW) ? ns++ : -1;
+    const int s_wt = (terms & OFDFT_TERM_WT) ? ns++ : -1;
+    const int s_h = (terms & OFDFT_TERM_HARTREE) ? ns++ : -1;
+    const long long pitch = pipe_pitch(P);
+    const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+    if (stage == 0) {
+        OFDFT_REQUIRE(ns > 0, "bench: no spectra in the pipeline");
+        GParams gp = pipeline_gparams(P, s_vw, s_wt, s_h, false, nullptr, ncp, pitch);
+        gspace_apply<T>(P, reinterpret_cast<cpx<T> *>(P.spec), ns, ncp, pitch, gp, x_fusable(P), s);
+    } else {
+        PostParams pp = pipeline_post(P, s_vw, s_wt, s_h, true, true);
+        const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
+        const int nb = std::min(reduce_blocks(P.N), 1024);
+        hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, reinterpret_cast<const T *>(P.work),
+                           vloc, pot, P.red, pp);
+    }
+    OFDFT_HIP(hipGetLastError());
+}
+template void bench_eval_stage<float>(Plan &, int, const float *, float *, hipStream_t);
+template void bench_eval_stage<double>(Plan &, int, const double *, double *, hipStream_t);
 
 // ------------------------------------------------------------------ linear response
 // dV[drho] of the same functional at rho: the FFT terms are linear in their
@@ -552,6 +725,10 @@ void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T 
          const double *mu, T *Ap, int nb, hipStream_t s) {
     OFDFT_REQUIRE(!(P.terms & OFDFT_TERM_LMGP), "the analytic Hessian product does not cover LMGP; use the "
                                                  "finite-difference one");
+    OFDFT_REQUIRE(!(P.terms & OFDFT_TERM_WT) || wt_alpha_is_beta(P),
+                  "the analytic Hessian product covers WT with alpha == beta only; use the finite-difference one");
+    OFDFT_REQUIRE(!((P.terms & OFDFT_TERM_XC) && P.core),
+                  "the analytic Hessian product does not cover a core density in XC; use the finite-difference one");
     eval_linear<T>(P, rho, phi, p, s);
     const unsigned terms = P.terms;
     int ns = 0;

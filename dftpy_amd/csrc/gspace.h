@@ -49,6 +49,7 @@ struct GParams {
     int full;
     double scale;
     int s_vw, s_wt, s_h;
+    int s_wt2 = -1;  // WT with alpha != beta: the rho^alpha spectrum (same kernel)
     double sigma;
     double lbda, alpha, beta;
     double rho0;             // used when rho0_dev == nullptr

@@ -99,7 +99,7 @@ struct Plan {
     double *red;  // reduction partials
     int red_blocks;
     double *scal;  // device scalars
-    void *aux;     // optimizer / propagator fields (allocated lazily at first use? no: at create)
+    void *aux;     // optimizer / propagator fields (allocated at create)
     size_t aux_bytes;
     size_t ws_bytes;
     double *pinned;  // 128 doubles of pinned host memory for scalar read-backs (optimizer, propagator)
@@ -108,6 +108,7 @@ struct Plan {
     double x = 1.0, y = 1.0, alpha = 5.0 / 6.0, beta = 5.0 / 6.0, rho0 = -1.0, sigma = -1.0;
     unsigned terms = 63;
     const void *vloc = nullptr;
+    const void *core = nullptr;  // NLCC core density added to rho in the XC term (N reals) or null
     double eion = 0.0;
     LmgpState *lmgp = nullptr;
     bool rho0_cached_valid = false;
@@ -117,6 +118,10 @@ struct Plan {
     // radial table of the TN inner-CG preconditioner (M^-1(|G|) at |G| = i dq,
     // i < tn_pre_n; tn_pre_n = 0: no preconditioner)
     double *tn_pre = nullptr;
+    // L-BFGS pair memory of the optimizer: 2 lbfgs_pairs fields (s_i, y_i) of
+    // the plan precision, reserved by ofdft_plan_reserve_lbfgs
+    void *lbfgs_buf = nullptr;
+    int lbfgs_pairs = 0;
     int tn_pre_n = 0, tn_pre_cap = 0;
     double tn_pre_dq = 0.0;
     // staging of the setup routines' host inputs (PP tables, positions): a
@@ -169,6 +174,12 @@ template <typename T>
 void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue *pro, hipStream_t s);
 template <typename T>
 void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int batch, hipStream_t s);
+// an evaluation with the energy-density / second-derivative fields (eval.hip)
+template <typename T>
+void eval_fields(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, T *edens, T *v2, hipStream_t s);
+// a stage of the E+V pipeline (0 = fused x pass with the G-space multiply,
+// 1 = point-wise post pass), eval.hip
+template <typename T> void bench_eval_stage(Plan &P, int stage, const T *rho, T *pot, hipStream_t s);
 
 // autotuner of the tile variants (fft.hip); times[12]: per pass (r2c, c2r,
 // x cols, y cols) the mean ms of variants 0..2 (optional)

@@ -337,6 +337,7 @@ size_t lmgp_ws_bytes(const Plan &P) {
 }
 
 template <typename T> void lmgp_eval(Plan &P, const T *rho, T *pot, double *energies, hipStream_t s) {
+    OFDFT_REQUIRE(std::fabs(P.alpha - P.beta) < 1e-9, "LMGP with alpha != beta is not supported");
     LmgpState &L = *P.lmgp;
     const long long n = P.N, nc = P.Nc;
     // rho range -> k_F range (k_F is monotone in rho)
@@ -361,12 +362,7 @@ template <typename T> void lmgp_eval(Plan &P, const T *rho, T *pot, double *ener
     for (int j = 0; j < nsp; ++j) kfl[j] = kfmin * std::pow(L.ratio, (double)j);
     kfl[0] -= 1e-16;
     kfl[nsp - 1] += 1e-16;
-    OFDFT_REQUIRE(nsp >= 2 && nsp <= 4096, "LMGP: k_F ladder out of range");
-    if (nsp > L.kfl_cap) {
-        if (L.kfl) (void)hipFree(L.kfl);
-        L.kfl_cap = std::max(nsp, 64);
-        OFDFT_HIP(hipMalloc(&L.kfl, L.kfl_cap * sizeof(double)));
-    }
+    OFDFT_REQUIRE(nsp >= 2 && nsp <= L.kfl_cap, "LMGP: k_F ladder out of range");
     OFDFT_HIP(hipMemcpyAsync(L.kfl, kfl.data(), nsp * sizeof(double), hipMemcpyHostToDevice, s));
     L.last_nsp = nsp;
     // workspace
@@ -492,6 +488,14 @@ int ofdft_set_lmgp(ofdft_plan *plan, int ntab, double etamax, const double *kcoe
         OFDFT_HIP(hipMemcpy(L.kcoef, kcoef, cb, hipMemcpyHostToDevice));
         OFDFT_HIP(hipMemcpy(L.dcoef, dcoef, cb, hipMemcpyHostToDevice));
         L.ntab = ntab;
+        // the k_F ladder at its largest (4096 samples): evaluations allocate nothing
+        if (L.kfl_cap < 4096) {
+            if (L.kfl) (void)hipFree(L.kfl);
+            L.kfl = nullptr;
+            L.kfl_cap = 0;
+            OFDFT_HIP(hipMalloc(&L.kfl, 4096 * sizeof(double)));
+            L.kfl_cap = 4096;
+        }
         L.etamax = etamax;
         L.deta = etamax / (ntab - 1);
         // values at the last knot: the end of the last cubic piece

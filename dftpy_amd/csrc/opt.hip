@@ -630,13 +630,10 @@ template <typename T> struct Opt {
     // scal[LB_ALPHA + i], dot products at scal[LB_DOT..]
     static constexpr int LB_RHO = 160, LB_ALPHA = 176, LB_DOT = 192, LB_MAX_PAIRS = 16;
     struct Lbfgs {
-        T *buf = nullptr;
+        T *buf = nullptr;  // the plan's pair memory (ofdft_plan_reserve_lbfgs)
         int bound = 0;
         double h0 = 0.0;
         std::vector<int> order, free_slots;
-        ~Lbfgs() {
-            if (buf) (void)hipFree(buf);
-        }
     };
     T *lb_s(Lbfgs &L, int k) { return L.buf + (size_t)(2 * k) * n; }
     T *lb_y(Lbfgs &L, int k) { return L.buf + (size_t)(2 * k + 1) * n; }
@@ -699,7 +696,10 @@ template <typename T> struct Opt {
             lb.bound = (int)opts[8];
             lb.h0 = opts[9];
             OFDFT_REQUIRE(lb.bound >= 0 && lb.bound + 1 <= LB_MAX_PAIRS, "optimize: maxcor must be in [0, 15]");
-            OFDFT_HIP(hipMalloc(&lb.buf, (size_t)2 * (lb.bound + 1) * n * sizeof(T)));
+            OFDFT_REQUIRE(P.lbfgs_buf != nullptr && P.lbfgs_pairs >= lb.bound + 1,
+                          "optimize: L-BFGS needs the plan's pair memory; call ofdft_plan_reserve_lbfgs(plan, maxcor) "
+                          "first");
+            lb.buf = reinterpret_cast<T *>(P.lbfgs_buf);
             for (int k = lb.bound; k >= 0; --k) lb.free_slots.push_back(k);
         }
         const auto t0 = std::chrono::steady_clock::now();
@@ -771,20 +771,28 @@ template <typename T> struct Opt {
             LsTask task = LS_START;
             double stp = theta;
             bool ok = false, evaluated = false;
-            for (int i = 0; i < maxls; ++i) {
-                task = dcsrch(stp, f, g, c1, c2, xtol, 0.0, M_PI, task, st);
-                if (task == LS_FG) {
-                    value_deriv(stp, f, g);
-                    evaluated = true;
-                } else {
-                    ok = (task == LS_CONVERGENCE);
-                    break;
+            if (P.prec == OFDFT_FP32 && std::fabs(g) * theta < 0.1 * econv) {
+                // FP32: the first-order energy change of the trial step is
+                // below a tenth of the stop threshold, so the Wolfe tests
+                // would compare energy differences at single-precision
+                // resolution and could only exhaust maxls; the trial (for TN
+                // the Newton) step is taken as it is, with one evaluation
+                value_deriv(stp, f, g);
+                ok = evaluated = true;
+            } else {
+                for (int i = 0; i < maxls; ++i) {
+                    task = dcsrch(stp, f, g, c1, c2, xtol, 0.0, M_PI, task, st);
+                    if (task == LS_FG) {
+                        value_deriv(stp, f, g);
+                        evaluated = true;
+                    } else {
+                        ok = (task == LS_CONVERGENCE);
+                        break;
+                    }
                 }
             }
             if (!ok || !evaluated) {
-                // FP32: past the first steps a failed search means the energy
-                // differences fell below single-precision resolution
-                converged = (P.prec == OFDFT_FP32 && it > ncheck) ? 3 : 1;
+                converged = 1;  // optimization.py:440-444
                 break;
             }
             theta = stp;

@@ -5,6 +5,10 @@
 // where lap_sigma = IFFT(-G^2 exp(-G^2 sigma^2 / 2) FFT) on the full complex
 // grid (field.py:343-350).  Inner products accumulate in FP64.
 #include <cmath>
+#include <complex>
+#include <initializer_list>
+#include <tuple>
+#include <vector>
 
 #include "internal.h"
 #include "reduce.h"
@@ -218,6 +222,320 @@ template <typename T> struct CN {
             info[0] = status;
             info[1] = k;
             info[2] = rnorm;
+            info[3] = atol;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- Krylov solvers
+// The other linear solvers of the Crank-Nicolson step
+// (crank_nicholson.py:65-77): bicg and bicgstab as linear_solver.py:49-110
+// writes them, and GMRES(restart) (reached there through scipy's gmres),
+// on the same operator A = 1 + i H dt/2.  Complex vector kernels with FP64
+// sums; the scalar recursions run on the host on FP64 values.
+struct ZPairs {
+    const void *a[4];
+    const void *b[4];
+    int conj[4];  // pair t sums conj(a) b (1) or a b (0)
+    int k;
+};
+
+// up to 4 complex sums, columns (2t, 2t+1) = (re, im) of pair t
+template <typename T> __global__ void k_zsums(ZPairs z, long long n, double *partials) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (t < z.k) {
+                const cpx<T> a = reinterpret_cast<const cpx<T> *>(z.a[t])[i];
+                const cpx<T> b = reinterpret_cast<const cpx<T> *>(z.b[t])[i];
+                const double ar = (double)a.x, ai = z.conj[t] ? -(double)a.y : (double)a.y;
+                acc[2 * t] += ar * (double)b.x - ai * (double)b.y;
+                acc[2 * t + 1] += ar * (double)b.y + ai * (double)b.x;
+            }
+        }
+    }
+    td_partials<8>(acc, partials);
+}
+
+// out = ca a + cb b (complex scalars; b may be null); out may alias a or b
+template <typename T>
+__global__ void k_zlin(const cpx<T> *a, double car, double cai, const cpx<T> *b, double cbr, double cbi, cpx<T> *out,
+                       long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const cpx<T> x = a[i];
+        double r = car * x.x - cai * x.y, m = car * x.y + cai * x.x;
+        if (b) {
+            const cpx<T> y = b[i];
+            r += cbr * y.x - cbi * y.y;
+            m += cbr * y.y + cbi * y.x;
+        }
+        out[i] = cpx<T>{(T)r, (T)m};
+    }
+}
+
+constexpr int kZChunk = 8;  // basis vectors per projection launch
+struct ZBasis {
+    const void *v[kZChunk];
+    double cr[kZChunk], ci[kZChunk];
+    int nv;
+};
+
+// sums conj(v_j) w for the chunk's vectors, columns (2j, 2j+1)
+template <typename T> __global__ void k_zproj(ZBasis zb, const cpx<T> *w, long long n, double *partials) {
+    double acc[2 * kZChunk];
+#pragma unroll
+    for (int j = 0; j < 2 * kZChunk; ++j) acc[j] = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const cpx<T> x = w[i];
+#pragma unroll
+        for (int j = 0; j < kZChunk; ++j) {
+            if (j < zb.nv) {
+                const cpx<T> v = reinterpret_cast<const cpx<T> *>(zb.v[j])[i];
+                acc[2 * j] += (double)v.x * x.x + (double)v.y * x.y;
+                acc[2 * j + 1] += (double)v.x * x.y - (double)v.y * x.x;
+            }
+        }
+    }
+    td_partials<2 * kZChunk>(acc, partials);
+}
+
+// w += sum_j c_j v_j over the chunk's vectors
+template <typename T> __global__ void k_zcomb(ZBasis zb, cpx<T> *w, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const cpx<T> x = w[i];
+        double r = x.x, m = x.y;
+#pragma unroll
+        for (int j = 0; j < kZChunk; ++j) {
+            if (j < zb.nv) {
+                const cpx<T> v = reinterpret_cast<const cpx<T> *>(zb.v[j])[i];
+                r += zb.cr[j] * v.x - zb.ci[j] * v.y;
+                m += zb.cr[j] * v.y + zb.ci[j] * v.x;
+            }
+        }
+        w[i] = cpx<T>{(T)r, (T)m};
+    }
+}
+
+enum { CN_CG = 0, CN_BICG = 1, CN_BICGSTAB = 2, CN_GMRES = 3 };
+constexpr int kMaxRestart = 64;
+
+template <typename T> struct CNK : CN<T> {
+    using C = cpx<T>;
+    using CN<T>::P;
+    using CN<T>::s;
+    using CN<T>::n;
+    using CN<T>::nb;
+    char *ws;
+    CNK(Plan &p, hipStream_t st, double sg, void *w) : CN<T>(p, st, sg), ws(reinterpret_cast<char *>(w)) {}
+    C *field(int k) { return reinterpret_cast<C *>(ws) + (size_t)k * n; }
+    typedef std::complex<double> Z;
+
+    // complex sums of up to 4 pairs, fetched to the host
+    std::vector<Z> sums(std::initializer_list<std::tuple<const C *, const C *, int>> pairs) {
+        ZPairs z{};
+        z.k = 0;
+        for (auto &pr : pairs) {
+            z.a[z.k] = std::get<0>(pr);
+            z.b[z.k] = std::get<1>(pr);
+            z.conj[z.k] = std::get<2>(pr);
+            ++z.k;
+        }
+        hipLaunchKernelGGL((k_zsums<T>), dim3(nb), dim3(256), 0, s, z, n, P.red);
+        OFDFT_HIP(hipGetLastError());
+        const double *h = this->fetch(8);
+        std::vector<Z> out(z.k);
+        for (int t = 0; t < z.k; ++t) out[t] = Z(h[2 * t], h[2 * t + 1]);
+        return out;
+    }
+    void lin(const C *a, Z ca, const C *b, Z cb, C *out) {
+        hipLaunchKernelGGL((k_zlin<T>), dim3(nb), dim3(256), 0, s, a, ca.real(), ca.imag(), b, cb.real(), cb.imag(),
+                           out, n);
+        OFDFT_HIP(hipGetLastError());
+    }
+    double norm(const C *x) { return std::sqrt(sums({{x, x, 1}})[0].real()); }
+    void A(const C *x, const T *v, double dt, C *out) { this->template apply<0>(x, v, 0.5 * dt, out); }
+
+    // linear_solver.py:49-75 (sums conjugated)
+    int bicg(const C *b, const T *v, double dt, double atol, int maxiter, C *x, int &k, double &rn) {
+        C *res = field(1), *p = field(2), *w = field(3);
+        A(x, v, dt, res);
+        lin(b, 1.0, res, -1.0, res);
+        rn = norm(res);
+        k = 0;
+        if (rn < atol) return 0;
+        lin(res, 1.0, nullptr, 0.0, p);
+        Z rr = sums({{res, res, 1}})[0];
+        while (k < maxiter) {
+            A(p, v, dt, w);
+            const Z alpha = rr / sums({{p, w, 1}})[0];
+            lin(x, 1.0, p, alpha, x);
+            lin(res, 1.0, w, -alpha, res);
+            const Z rn2 = sums({{res, res, 1}})[0];
+            rn = std::sqrt(rn2.real());
+            ++k;
+            if (rn < atol) return 0;
+            lin(res, 1.0, p, rn2 / rr, p);
+            rr = rn2;
+        }
+        return 1;
+    }
+
+    // linear_solver.py:78-110 (sums not conjugated; res and r follow the
+    // same recursion, so one field carries both)
+    int bicgstab(const C *b, const T *v, double dt, double atol, int maxiter, C *x, int &k, double &rn) {
+        C *r0 = field(1), *r = field(2), *p = field(3), *w = field(4), *sv = field(5), *t = field(6);
+        A(x, v, dt, r0);
+        lin(b, 1.0, r0, -1.0, r0);
+        rn = norm(r0);
+        k = 0;
+        if (rn < atol) return 0;
+        lin(r0, 1.0, nullptr, 0.0, r);
+        hipLaunchKernelGGL((k_zlin<T>), dim3(nb), dim3(256), 0, s, r0, 0.0, 0.0, (const C *)nullptr, 0.0, 0.0, p, n);
+        hipLaunchKernelGGL((k_zlin<T>), dim3(nb), dim3(256), 0, s, r0, 0.0, 0.0, (const C *)nullptr, 0.0, 0.0, w, n);
+        Z rho_old = 1.0, omega = 1.0, alpha = 1.0;
+        while (k < maxiter) {
+            const Z rho = sums({{r0, r, 0}})[0];
+            const Z beta = rho / rho_old * alpha / omega;
+            lin(p, 1.0, w, -omega, p);  // p - omega v
+            lin(r, 1.0, p, beta, p);    // r + beta (p - omega v)
+            A(p, v, dt, w);
+            alpha = rho / sums({{r0, w, 0}})[0];
+            lin(x, 1.0, p, alpha, x);  // h = x + alpha p
+            lin(r, 1.0, w, -alpha, sv);  // s = r - alpha v (= res)
+            rn = norm(sv);
+            if (rn < atol) {
+                ++k;
+                return 0;
+            }
+            A(sv, v, dt, t);
+            const std::vector<Z> ts = sums({{t, sv, 0}, {t, t, 0}});
+            omega = ts[0] / ts[1];
+            lin(x, 1.0, sv, omega, x);
+            lin(sv, 1.0, t, -omega, r);
+            rn = norm(r);
+            ++k;
+            if (rn < atol) return 0;
+            rho_old = rho;
+        }
+        return 1;
+    }
+
+    // GMRES(m): Arnoldi by classical Gram-Schmidt applied twice, Givens
+    // rotations on the host; the residual estimate |g_{j+1}| against atol,
+    // the true residual recomputed at each restart.  maxiter counts Arnoldi
+    // steps (operator applications).
+    void project(int nv, C *w, std::vector<Z> &h) {
+        h.assign(nv, Z(0.0, 0.0));
+        for (int c0 = 0; c0 < nv; c0 += kZChunk) {
+            ZBasis zb{};
+            zb.nv = std::min(kZChunk, nv - c0);
+            for (int j = 0; j < zb.nv; ++j) zb.v[j] = field(4 + c0 + j);
+            hipLaunchKernelGGL((k_zproj<T>), dim3(nb), dim3(256), 0, s, zb, (const C *)w, n, P.red);
+            OFDFT_HIP(hipGetLastError());
+            const double *hh = this->fetch(2 * kZChunk);
+            for (int j = 0; j < zb.nv; ++j) h[c0 + j] = Z(hh[2 * j], hh[2 * j + 1]);
+        }
+    }
+    void combine(int nv, const std::vector<Z> &c, double sign, C *w) {
+        for (int c0 = 0; c0 < nv; c0 += kZChunk) {
+            ZBasis zb{};
+            zb.nv = std::min(kZChunk, nv - c0);
+            for (int j = 0; j < zb.nv; ++j) {
+                zb.v[j] = field(4 + c0 + j);
+                zb.cr[j] = sign * c[c0 + j].real();
+                zb.ci[j] = sign * c[c0 + j].imag();
+            }
+            hipLaunchKernelGGL((k_zcomb<T>), dim3(nb), dim3(256), 0, s, zb, w, n);
+            OFDFT_HIP(hipGetLastError());
+        }
+    }
+    int gmres(const C *b, const T *v, double dt, double atol, int maxiter, int m, C *x, int &k, double &rn) {
+        C *r = field(1), *w = field(2);
+        k = 0;
+        std::vector<std::vector<Z>> H(m + 1, std::vector<Z>(m, Z(0, 0)));
+        std::vector<Z> g(m + 1), cs(m), sn(m), h, h2;
+        while (true) {
+            A(x, v, dt, r);
+            lin(b, 1.0, r, -1.0, r);
+            const double beta = norm(r);
+            rn = beta;
+            if (beta < atol) return 0;
+            if (k >= maxiter) return 1;
+            lin(r, 1.0 / beta, nullptr, 0.0, field(4));
+            std::fill(g.begin(), g.end(), Z(0, 0));
+            g[0] = beta;
+            int j = 0;
+            bool done = false;
+            for (; j < m && k < maxiter; ++j) {
+                A(field(4 + j), v, dt, w);
+                ++k;
+                project(j + 1, w, h);
+                combine(j + 1, h, -1.0, w);
+                project(j + 1, w, h2);
+                combine(j + 1, h2, -1.0, w);
+                for (int i = 0; i <= j; ++i) H[i][j] = h[i] + h2[i];
+                const double hn = norm(w);
+                H[j + 1][j] = hn;
+                for (int i = 0; i < j; ++i) {  // earlier rotations
+                    const Z a = H[i][j], c = H[i + 1][j];
+                    H[i][j] = std::conj(cs[i]) * a + std::conj(sn[i]) * c;
+                    H[i + 1][j] = -sn[i] * a + cs[i] * c;
+                }
+                const Z a = H[j][j], c = H[j + 1][j];
+                const double den = std::sqrt(std::norm(a) + std::norm(c));
+                cs[j] = den > 0 ? a / den : Z(1, 0);
+                sn[j] = den > 0 ? c / den : Z(0, 0);
+                H[j][j] = den;
+                H[j + 1][j] = 0.0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] = std::conj(cs[j]) * g[j];
+                rn = std::abs(g[j + 1]);
+                if (rn < atol || hn == 0.0) {
+                    ++j;
+                    done = true;
+                    break;
+                }
+                if (j + 1 < m) lin(w, 1.0 / hn, nullptr, 0.0, field(4 + j + 1));
+            }
+            // y = H^-1 g (upper triangular j x j), x += V y
+            std::vector<Z> y(j);
+            for (int i = j - 1; i >= 0; --i) {
+                Z acc = g[i];
+                for (int l = i + 1; l < j; ++l) acc -= H[i][l] * y[l];
+                y[i] = acc / H[i][i];
+            }
+            combine(j, y, 1.0, x);
+            if (done) {
+                // the estimate met the tolerance: confirm with the true residual
+                A(x, v, dt, r);
+                lin(b, 1.0, r, -1.0, r);
+                rn = norm(r);
+                if (rn < atol) return 0;
+            }
+        }
+    }
+
+    void solve(int solver, int restart, const C *psi0, const T *v, double dt, double tol, int maxiter, C *x,
+               double *info) {
+        C *b = field(0);
+        this->template apply<2>(psi0, v, -0.5 * dt, b);
+        const double bn = std::sqrt(this->fetch(1)[0]);
+        const double atol = bn == 0.0 ? tol : tol * bn;
+        if (x != psi0) lin(psi0, 1.0, nullptr, 0.0, x);
+        int k = 0, st = 1;
+        double rn = 0.0;
+        if (solver == CN_BICG)
+            st = bicg(b, v, dt, atol, maxiter, x, k, rn);
+        else if (solver == CN_BICGSTAB)
+            st = bicgstab(b, v, dt, atol, maxiter, x, k, rn);
+        else
+            st = gmres(b, v, dt, atol, maxiter, restart, x, k, rn);
+        stream_wait(s);
+        if (info) {
+            info[0] = st;
+            info[1] = k;
+            info[2] = rn;
             info[3] = atol;
         }
     }
@@ -476,6 +794,54 @@ extern "C" int ofdft_cn_step_batch(ofdft_plan *plan, int nb, const void *psi0, c
         } else {
             CNBatch<float> c(*P, s, sigma, nb, work);
             c.step((const float2 *)psi0, (const float *)v, dt, tol, maxiter, (float2 *)psi1, info);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+extern "C" size_t ofdft_cn_solver_workspace_bytes(const ofdft_plan *plan, int solver, int restart) {
+    const Plan *P = reinterpret_cast<const Plan *>(plan);
+    if (!P) return 0;
+    const size_t f = (size_t)P->N * (P->prec == OFDFT_FP64 ? 16 : 8);
+    if (solver == CN_GMRES) return (size_t)(4 + std::max(restart, 1)) * f;
+    return (size_t)7 * f;
+}
+
+extern "C" int ofdft_cn_step_solver(ofdft_plan *plan, int solver, int restart, const void *psi0, const void *v,
+                                    double dt, double sigma, double tol, int maxiter, void *psi1, double *info,
+                                    void *work, size_t work_bytes, void *stream) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    try {
+        OFDFT_REQUIRE(P && psi0 && v && psi1, "cn_step_solver: bad arguments");
+        OFDFT_REQUIRE(solver >= CN_CG && solver <= CN_GMRES, "cn_step_solver: solver is 0 cg, 1 bicg, 2 bicgstab, "
+                                                             "3 gmres");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (solver == CN_CG) {
+            OFDFT_REQUIRE(P->aux_bytes >= (size_t)4 * P->N * (P->prec == OFDFT_FP64 ? 16 : 8), "cn_step: workspace");
+            if (P->prec == OFDFT_FP64) {
+                CN<double> c(*P, s, sigma);
+                c.step((const double2 *)psi0, (const double *)v, dt, tol, maxiter, (double2 *)psi1, info);
+            } else {
+                CN<float> c(*P, s, sigma);
+                c.step((const float2 *)psi0, (const float *)v, dt, tol, maxiter, (float2 *)psi1, info);
+            }
+            return 0;
+        }
+        OFDFT_REQUIRE(solver != CN_GMRES || (restart >= 1 && restart <= kMaxRestart),
+                      "cn_step_solver: restart must be in [1, 64]");
+        OFDFT_REQUIRE(work != nullptr && work_bytes >= ofdft_cn_solver_workspace_bytes(plan, solver, restart),
+                      "cn_step_solver: workspace too small");
+        OFDFT_REQUIRE(psi1 != psi0, "cn_step_solver: psi1 must not alias psi0");
+        if (P->prec == OFDFT_FP64) {
+            CNK<double> c(*P, s, sigma, work);
+            c.solve(solver, restart, (const double2 *)psi0, (const double *)v, dt, tol, maxiter, (double2 *)psi1,
+                    info);
+        } else {
+            CNK<float> c(*P, s, sigma, work);
+            c.solve(solver, restart, (const float2 *)psi0, (const float *)v, dt, tol, maxiter, (float2 *)psi1, info);
         }
         return 0;
     } catch (const std::exception &e) {

@@ -32,10 +32,12 @@ class FunctionalOutput:
             self.potential = potential
         if energydensity is not None:
             self.energydensity = energydensity
+        if kwargs.get("v2rho2") is not None:
+            self.v2rho2 = kwargs["v2rho2"]
         self.parts = kwargs.get("parts")
 
     def _items(self):
-        for k in ("energy", "potential", "energydensity"):
+        for k in ("energy", "potential", "energydensity", "v2rho2"):
             if hasattr(self, k):
                 yield k, getattr(self, k)
 
@@ -113,6 +115,7 @@ def configure(plan, funcs, rho_grid, eion=0.0):
     terms = 0
     kedf = dict(x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0=None, sigma=None)
     vloc = None
+    core = None
     for f in funcs:
         terms |= f.terms
         o = f.kedf_options()
@@ -120,14 +123,20 @@ def configure(plan, funcs, rho_grid, eion=0.0):
             kedf.update({k: v for k, v in o.items() if k in kedf})
         if f.terms & _lib.TERM_PP:
             vloc = f.vreal_for(rho_grid)
+        if f.terms & _lib.TERM_XC and getattr(f, "core_density", None) is not None:
+            core = f.core_for(rho_grid)
     rho0 = kedf["rho0"]
     plan.kedf_settings = {k: kedf[k] for k in ("x", "y", "alpha", "beta")}
+    plan.has_core = core is not None
     # the plan keeps its settings between calls: skip the library calls when
-    # nothing changed (vloc is read through its pointer at evaluation time)
+    # nothing changed (vloc and the core density are read through their
+    # pointers at evaluation time)
     vptr = vloc.data.data_ptr() if vloc is not None else None
-    key = (terms, tuple((k, kedf[k]) for k in sorted(kedf)), vptr, float(eion))
+    cptr = core.data_ptr() if core is not None else None
+    keep = (vloc, core)
+    key = (terms, tuple((k, kedf[k]) for k in sorted(kedf)), vptr, cptr, float(eion))
     if getattr(plan, "config_key", None) == key:
-        return terms, vloc
+        return terms, keep
     _lib.check(_lib.lib.ofdft_set_kedf(plan.handle, float(kedf["x"]), float(kedf["y"]), float(kedf["alpha"]),
                                        float(kedf["beta"]), float(rho0) if rho0 else -1.0,
                                        float(kedf["sigma"]) if kedf["sigma"] else -1.0))
@@ -139,9 +148,10 @@ def configure(plan, funcs, rho_grid, eion=0.0):
         plan.lmgp_ready = True
     _lib.check(_lib.lib.ofdft_set_terms(plan.handle, terms))
     _lib.check(_lib.lib.ofdft_set_vloc(plan.handle, vptr))
+    _lib.check(_lib.lib.ofdft_set_core(plan.handle, cptr))
     _lib.check(_lib.lib.ofdft_set_eion(plan.handle, float(eion)))
     plan.config_key = key
-    return terms, vloc
+    return terms, keep
 
 
 def evaluate(funcs, rho, calcType, name="TOTAL", eion=0.0):
@@ -151,8 +161,9 @@ def evaluate(funcs, rho, calcType, name="TOTAL", eion=0.0):
     if rho.rank > 1:
         raise NotImplementedError("spin-polarised densities are not supported")
     calc = set(calcType)
-    if calc - {"E", "V"}:
-        raise NotImplementedError("calcType %s not supported (only 'E' and 'V')" % sorted(calc - {"E", "V"}))
+    if calc - {"E", "V", "D", "V2"}:
+        raise NotImplementedError("calcType %s not supported (only 'E', 'V', 'D' and 'V2')" %
+                                  sorted(calc - {"E", "V", "D", "V2"}))
     grid = rho.grid
     plan = grid.plan(grid.dtype)
     terms, vloc = configure(plan, funcs, grid, eion)
@@ -160,9 +171,27 @@ def evaluate(funcs, rho, calcType, name="TOTAL", eion=0.0):
     src = rho.data.contiguous()
     pot = torch.empty_like(src) if "V" in calc else None
     en = torch.empty(_lib.E_COUNT, dtype=torch.float64, device=src.device) if "E" in calc else None
-    _lib.check(_lib.lib.ofdft_eval(plan.handle, src.data_ptr(), flags, pot.data_ptr() if pot is not None else None,
-                                   en.data_ptr() if en is not None else None, _lib.stream()))
+    # energy density (kedf/tf.py:68-73, vw.py:125-126, wt.py:123-129) and
+    # second derivative (tf.py:77-79, semilocal_xc.py:291-303) of the terms
+    # that have them
+    dens = torch.empty_like(src) if "D" in calc and terms & (_lib.TERM_TF | _lib.TERM_VW | _lib.TERM_WT |
+                                                              _lib.TERM_LMGP) else None
+    v2 = torch.empty_like(src) if "V2" in calc and terms & (_lib.TERM_TF | _lib.TERM_XC) else None
+    if dens is not None or v2 is not None:
+        _lib.check(_lib.lib.ofdft_eval_fields(plan.handle, src.data_ptr(), flags,
+                                              pot.data_ptr() if pot is not None else None,
+                                              en.data_ptr() if en is not None else None,
+                                              dens.data_ptr() if dens is not None else None,
+                                              v2.data_ptr() if v2 is not None else None, _lib.stream()))
+    else:
+        _lib.check(_lib.lib.ofdft_eval(plan.handle, src.data_ptr(), flags,
+                                       pot.data_ptr() if pot is not None else None,
+                                       en.data_ptr() if en is not None else None, _lib.stream()))
     out = FunctionalOutput(name=name)
+    if dens is not None:
+        out.energydensity = DirectField(grid, data=dens)
+    if v2 is not None:
+        out.v2rho2 = DirectField(grid, data=v2)
     if en is not None:
         e = en.cpu().numpy()
         out.energy = _energy_of(terms, e) + eion
@@ -192,6 +221,8 @@ def functional_stress(funcs, rho):
     terms, vloc = configure(plan, funcs, grid)
     if terms & _lib.TERM_LMGP:
         raise NotImplementedError("stress of the LMGP functional is not implemented")
+    if plan.has_core:
+        raise NotImplementedError("the XC stress with a core density is not implemented")
     src = r.data.contiguous()
     en = torch.empty(_lib.E_COUNT, dtype=torch.float64, device=src.device)
     _lib.check(_lib.lib.ofdft_eval(plan.handle, src.data_ptr(), _lib.CALC_E, None, en.data_ptr(), _lib.stream()))
@@ -256,9 +287,6 @@ class KEDF(AbstractFunctional):
         self.terms = KEDF_TERMS[self.name]
         self.options = dict(kwargs)
         self.options["kedf"] = self.name
-        a, b = self.options.get("alpha", 5.0 / 6.0), self.options.get("beta", 5.0 / 6.0)
-        if abs(a - b) > 1e-9 and self.terms & _lib.TERM_WT:
-            raise NotImplementedError("WT with alpha != beta is not implemented")
 
     def kedf_options(self):
         return self.options
@@ -302,12 +330,27 @@ class XC(AbstractFunctional):
                 xc = "LDA"
         if xc is None or xc.upper() != "LDA":
             raise NotImplementedError("only the PZ LDA is implemented (got %r)" % (xc or libxc))
-        if core_density is not None:
-            raise NotImplementedError("non-linear core correction is not implemented")
         self.type = "XC"
         self.name = "LDA"
         self.terms = _lib.TERM_XC
         self.options = kwargs
+        # non-linear core correction (xc/__init__.py:49-74): XC of rho + core
+        self.core_density = core_density
+        self._core = {}
+
+    def core_for(self, grid):
+        """The core density as a contiguous device tensor in the precision of
+        ``grid`` (the DirectField given, or an array of the grid's shape)."""
+        c = self.core_density
+        data = c.data if isinstance(c, DirectField) else torch.as_tensor(np.asarray(c))
+        key = (grid.dtype, data.data_ptr() if data.is_cuda else id(c))
+        if key not in self._core:
+            t = data.to(device=grid.device, dtype=grid.dtype).contiguous()
+            if tuple(t.shape) != tuple(int(n) for n in grid.nrR):
+                raise ValueError("core density shape %s does not match the grid %s" % (tuple(t.shape),
+                                                                                        tuple(grid.nrR)))
+            self._core = {key: t}
+        return self._core[key]
 
     def stress(self, density, **kwargs):
         """LDAStress (xc/semilocal_xc.py:308-322)."""

@@ -93,6 +93,13 @@ class Optimization:
                                       "optimise LMGP on an FP64 grid")
         if hvp not in ("fd", "analytic"):
             raise ValueError("hvp must be 'auto', 'fd' or 'analytic'")
+        ks = plan.kedf_settings
+        if hvp == "analytic" and getattr(plan, "has_core", False):
+            raise NotImplementedError("the analytic Hessian product does not cover a core density in XC; optimise "
+                                      "on an FP64 grid (forward-difference Hessian)")
+        if hvp == "analytic" and ev.terms & _lib.TERM_WT and abs(ks["alpha"] - ks["beta"]) > 1e-9:
+            raise NotImplementedError("the analytic Hessian product covers WT with alpha == beta only; optimise WT "
+                                      "with alpha != beta on an FP64 grid (forward-difference Hessian)")
         hvp = 0 if hvp == "fd" else 1
         # preconditioned inner CG of TN (not in the reference): opt-in
         if o["precond"] and self.optimization_method == "TN":
@@ -115,6 +122,7 @@ class Optimization:
         if self.optimization_method == "LBFGS":
             # optimization.py:346 reads optimization_options["h0"], which has no default
             h0 = float(o["h0"]) if o["h0"] else 0.0
+            _lib.check(_lib.lib.ofdft_plan_reserve_lbfgs(plan.handle, int(o["maxcor"])))
         opts = np.array([o["econv"], o["maxiter"], o["maxfun"], o["maxls"], o["c1"], o["c2"], o["xtol"], o["ncheck"],
                          o["maxcor"], h0], dtype=np.float64)
         stats = np.zeros(16)

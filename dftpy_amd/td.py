@@ -30,22 +30,32 @@ class Hamiltonian:
         return (torch.real(psi.data.conj() * hp.data).sum() * psi.grid.dV).item()
 
 
+# linear solvers of crank_nicholson.py:65-77 available here -> ofdft_cn_step_solver code
+LINEAR_SOLVERS = {"cg": 0, "bicg": 1, "bicgstab": 2, "gmres": 3, "gmres_scipy": 3}
+
+
 class CrankNicholson:
-    """(1 + i H dt/2) psi1 = (1 - i H dt/2) psi0 with the complex-symmetric CG
-    (linear_solver.py:24-47); ``linearsolver='cg'`` only."""
+    """(1 + i H dt/2) psi1 = (1 - i H dt/2) psi0 (crank_nicholson.py:59-188)
+    with the linear solver ``linearsolver``: 'cg' (complex-symmetric CG,
+    linear_solver.py:24-47), 'bicg', 'bicgstab' (linear_solver.py:49-110) or
+    'gmres' / 'gmres_scipy' (restarted GMRES, restart 20 as scipy's)."""
 
     def __init__(self, hamiltonian, interval, linearsolver="cg", tol=1e-8, maxiter=100, atol=None, sigma=0.025,
-                 **kwargs):
-        if linearsolver != "cg":
-            raise NotImplementedError("only the 'cg' linear solver is implemented")
+                 restart=20, **kwargs):
+        if linearsolver not in LINEAR_SOLVERS:
+            raise NotImplementedError("linear solver %r is not implemented (choose from %s)" %
+                                      (linearsolver, sorted(LINEAR_SOLVERS)))
         if atol is not None:
             raise NotImplementedError("atol is not supported")
         self.hamiltonian = hamiltonian
         self.interval = interval
+        self.linear_solver = linearsolver
         self.tol = tol
         self.maxiter = maxiter
         self.sigma = sigma
+        self.restart = int(restart)
         self.last_info = None
+        self._ws = None
 
     def __call__(self, psi0):
         grid = psi0.grid
@@ -55,9 +65,19 @@ class CrankNicholson:
         v = self.hamiltonian.v.data.to(grid.dtype).contiguous()
         out = torch.empty_like(src)
         info = np.zeros(4)
-        _lib.check(_lib.lib.ofdft_cn_step(plan.handle, src.data_ptr(), v.data_ptr(), float(self.interval),
-                                          float(self.sigma), float(self.tol), int(self.maxiter), out.data_ptr(),
-                                          _lib.dptr(info), _lib.stream()))
+        code = LINEAR_SOLVERS[self.linear_solver]
+        if code == 0:
+            _lib.check(_lib.lib.ofdft_cn_step(plan.handle, src.data_ptr(), v.data_ptr(), float(self.interval),
+                                              float(self.sigma), float(self.tol), int(self.maxiter), out.data_ptr(),
+                                              _lib.dptr(info), _lib.stream()))
+        else:
+            nbytes = int(_lib.lib.ofdft_cn_solver_workspace_bytes(plan.handle, code, self.restart))
+            if self._ws is None or self._ws.numel() < nbytes or self._ws.device != src.device:
+                self._ws = torch.empty(nbytes, dtype=torch.uint8, device=src.device)
+            _lib.check(_lib.lib.ofdft_cn_step_solver(plan.handle, code, self.restart, src.data_ptr(), v.data_ptr(),
+                                                     float(self.interval), float(self.sigma), float(self.tol),
+                                                     int(self.maxiter), out.data_ptr(), _lib.dptr(info),
+                                                     self._ws.data_ptr(), nbytes, _lib.stream()))
         self.last_info = dict(status=int(info[0]), iterations=int(info[1]), residual=info[2], atol=info[3])
         return DirectField(grid, data=out, cplx=True), int(info[0])
 

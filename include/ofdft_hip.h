@@ -17,8 +17,10 @@
  *    the message (thread-local).
  *  - The plan owns its workspace (allocated at create, freed at destroy);
  *    the FFT, evaluation, reduction, optimizer and propagator entry points
- *    allocate no device memory; the one-time setup calls (local_pp, ewald)
- *    allocate small per-call tables.
+ *    allocate no device memory.  Setup calls allocate: plan-owned tables
+ *    that grow only (set_lmgp, set_tn_precond, plan_reserve_lbfgs, the PP
+ *    staging and PME moduli, the force/stress workspace) and the per-call
+ *    upload of ofdft_atomic_density.
  *
  * Each declaration names the reference interface it replaces, as
  * path:line relative to the reference's src/dftpy.
@@ -107,6 +109,12 @@ int ofdft_set_terms(ofdft_plan *plan, unsigned terms);
  * pseudo/__init__.py:236-246): device array of N reals in the plan
  * precision, kept by pointer (caller keeps it alive) or NULL. */
 int ofdft_set_vloc(ofdft_plan *plan, const void *vreal);
+/* Core density of the non-linear core correction (XC(core_density=...),
+ * xc/__init__.py:49-74): the XC term is evaluated at rho + core (energy and
+ * potential); device array of N reals in the plan precision, kept by pointer,
+ * or NULL.  Not covered by the analytic Hessian product (the forward-difference
+ * one applies) nor by the XC stress. */
+int ofdft_set_core(ofdft_plan *plan, const void *core);
 /* LMGP nonlocal kinetic functional (LWT with the MGP kernel, lwt.py:501-578;
  * OFDFT_TERM_LMGP).  ofdft_mgp_kernel_table computes the MGP kernel table
  * K(eta_k), eta_k = k etamax / (neta - 1) (kernel.py:266-322, t-integral with
@@ -131,6 +139,17 @@ int ofdft_set_eion(ofdft_plan *plan, double eion);
  * (device, N reals).  rho is not modified (unlike wt.py:15-26). */
 int ofdft_eval(ofdft_plan *plan, const void *rho, unsigned calc, void *pot, double *energies,
                void *stream);
+/* The same evaluation with the calcType 'D' and 'V2' fields of the terms
+ * that have them in the reference (device, N reals each, either may be
+ * NULL): energy density edens = TF C rho^(5/3) (kedf/tf.py:68-73, without the
+ * x factor, as there) + vW y v_vW rho (kedf/vw.py:125-126) + WT rho^alpha
+ * (K * rho^beta) (kedf/wt.py:44-56,123-129); v2rho2 = TF x 10/9 C
+ * rho^(-1/3) (kedf/tf.py:47-50,77-79) + PZ-LDA d v_xc / d rho
+ * (xc/semilocal_xc.py:291-303).  Hartree and the local PP contribute to
+ * neither (the reference's return no such fields); LMGP 'D' is refused.
+ * Runs the full E+V pipeline (pot may be NULL). */
+int ofdft_eval_fields(ofdft_plan *plan, const void *rho, unsigned calc, void *pot, double *energies, void *edens,
+                      void *v2rho2, void *stream);
 
 /* Sum_r a(r) b(r) in FP64 (MP.einsum, mpi/mpi.py:199-206); b == NULL gives
  * Sum_r a(r).  Not multiplied by dV. */
@@ -141,8 +160,10 @@ int ofdft_dot(ofdft_plan *plan, const void *a, const void *b, double *out, void 
  * events and returns the mean duration in ms.  which: 0 z-pass R2C with the
  * vW/WT prologue (reads rho), 1 y-pass, 2 x-pass (C2C on the half grid),
  * 3 z-pass C2R, 4 full-grid C2C 3D transform, 5 one complete E+V evaluation
- * of rho with the plan's functional settings; batch 1..3 spectra.  Uses the
- * plan's workspace as data. */
+ * of rho with the plan's functional settings, 6 the evaluation's x pass
+ * (forward C2C, G-space multiply of every spectrum, inverse C2C, in place),
+ * 7 the evaluation's point-wise post pass (reads rho); batch 1..3 spectra
+ * (passes 0-4).  Uses the plan's workspace as data. */
 int ofdft_bench_pass(ofdft_plan *plan, int which, int batch, int iters, const void *rho, double *avg_ms,
                      void *stream);
 
@@ -223,7 +244,6 @@ int ofdft_atomic_density(ofdft_plan *plan, int kind, int nspecies, const int *nb
                          const double *scale, int nat, const double *pos_frac, const int *species, void *rho,
                          void *stream);
 int ofdft_clamp_normalize(ofdft_plan *plan, void *rho, int clamp, double dtol, double ncharge, void *stream);
-
 /* Preconditioner of the TN inner conjugate-gradient solve (no counterpart in
  * the reference, whose get_direction_TN, optimization.py:91-147, runs plain
  * CG).  table (host, n doubles): M^-1(q) at q = i dq, linearly interpolated
@@ -248,8 +268,8 @@ enum {
  * math_utils.py:395-423 (CG-HS, -PR, -FR, -DY, -CD, -LS, with the reference's
  * CG-LS denominator sum(d r0) and its field-valued CG-CD beta), OFDFT_OPT_GD
  * (-residual, :426-428) or OFDFT_OPT_LBFGS (two-loop recursion :431-455 with
- * at most maxcor + 1 pairs, :374-392; its buffer of 2 (maxcor + 1) fields is
- * allocated for the call).  Line search: Moré-Thuente dcsrch on theta in
+ * at most maxcor + 1 pairs, :374-392, kept in the plan's pair memory that
+ * ofdft_plan_reserve_lbfgs sets up; an L-BFGS call without it fails).  Line search: Moré-Thuente dcsrch on theta in
  * [0, pi] (math_utils.py:76-106).  hvp: 0 = forward difference with
  * eps = 1e-9 as the reference; 1 = analytic linearised potential (exact
  * Hessian-vector product, used in FP32).  rho: device, N reals, overwritten
@@ -259,12 +279,19 @@ enum {
  * result by s.y / y.y of the newest pair, as a falsy H0 does in the reference).
  * stats (host, 16 doubles): [converged, niter, energy, mu, n_eval_v, n_eval_ev,
  * seconds, ...]; converged: 0 converged, 1 line search failed, 2 maxiter
- * reached (optimization.py:440-444,509-511), 3 FP32 plan stopped at single-
- * precision resolution (line search failure after ncheck steps).
+ * reached (optimization.py:440-444,509-511).  FP32 plans skip the line
+ * search when |dE/dtheta| theta of the trial step is below econv / 10 (its
+ * energy differences would sit at single-precision resolution) and take the
+ * trial step.
  * history (host, 2*hcap doubles): energies per step in [0, hcap), seconds since
  * the call started at each step in [hcap, 2*hcap). */
 int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method, int hvp, const double *opts,
                    double *stats, double *history, int hcap, void *stream);
+/* Setup call for OFDFT_OPT_LBFGS (no separate counterpart: the reference's
+ * LBFGS class, math_utils.py:374-392, keeps its pairs in Python lists): makes
+ * the plan hold memory for maxcor + 1 (s, y) pairs, 2 (maxcor + 1) fields of
+ * the plan precision (0 <= maxcor <= 15); grows only, freed with the plan. */
+int ofdft_plan_reserve_lbfgs(ofdft_plan *plan, int maxcor);
 
 /* Moré-Thuente line search step (MINPACK-2 dcsrch, reached through
  * scipy in math_utils.py:6-9,76-106): reverse-communication state machine
@@ -281,6 +308,17 @@ int ofdft_dcsrch(double *stp, double f, double g, double ftol, double gtol, doub
  * info (host, 4 doubles): [status, iterations, final residual, atol]. */
 int ofdft_cn_step(ofdft_plan *plan, const void *psi0, const void *v, double dt, double sigma,
                   double tol, int maxiter, void *psi1, double *info, void *stream);
+/* The same step with the linear solver chosen as crank_nicholson.py:65-77
+ * offers: solver 0 cg (as ofdft_cn_step), 1 bicg and 2 bicgstab
+ * (linear_solver.py:49-110, restated), 3 GMRES(restart) (the gmres_scipy
+ * choice; Arnoldi by classical Gram-Schmidt applied twice, FP64 Givens
+ * rotations on the host, maxiter counting operator applications).  Solvers
+ * 1-3 take a workspace of ofdft_cn_solver_workspace_bytes(plan, solver,
+ * restart) bytes (device, caller-owned); psi1 must not alias psi0. */
+size_t ofdft_cn_solver_workspace_bytes(const ofdft_plan *plan, int solver, int restart);
+int ofdft_cn_step_solver(ofdft_plan *plan, int solver, int restart, const void *psi0, const void *v, double dt,
+                         double sigma, double tol, int maxiter, void *psi1, double *info, void *work,
+                         size_t work_bytes, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Learned warm start (no counterpart in the reference: it produces the start

@@ -2,8 +2,9 @@
 Hartree and PZ-LDA.  Test infrastructure only (tests/, never the product).
 
 Restates ``ewald.py:541-760`` (Forces_real, Forces_rec, Stress_real,
-Stress_rec with the G = 0 term), ``functional/pseudo/__init__.py:406-470``
-(_PP_Derivative, _Stress, _Force), ``functional/kedf/tf.py:52-62``,
+Stress_rec with the G = 0 term), ``ewald.py:810-960`` (Forces_rec_PME,
+Stress_rec_PME), ``functional/pseudo/__init__.py:406-563`` (_PP_Derivative,
+_Stress, _Force, _ForcePME, _StressPME), ``functional/kedf/tf.py:52-62``,
 ``functional/kedf/vw.py:97-111``, ``functional/kedf/wt.py:75-105`` with
 ``functional/kedf/kernel.py:197-211`` (LindhardDerivative),
 ``functional/hartree.py:67-91`` and ``functional/xc/semilocal_xc.py:308-322``
@@ -24,7 +25,7 @@ from scipy.interpolate import splev, splrep
 from scipy.spatial.distance import cdist
 
 from .functionals import CTF
-from .pseudo import ewald_eta, strf, vloc_line
+from .pseudo import barray, calc_Mn, ewald_eta, pme_qarray, strf, vloc_line
 
 
 def _images(grid, atoms, rmax):
@@ -178,6 +179,118 @@ def pp_stress(rho, grid, atoms, pp, energy):
             S[i, j] = S[j, i] = -(np.sum(g[i] * g[j] * x)).real / grid.volume * 2.0
     S -= np.eye(3) * energy
     return S / grid.volume
+
+
+# ---------------------------------------------------------------- PME forms
+def _pme_stencil_forces(field, grid, frac, order):
+    """-sum over each atom's B-spline stencil of grad_R Q(l) field(l) (the
+    Q_derivativeA contraction of ``pseudo/__init__.py:500-520`` and
+    ``ewald.py:833-862``), cartesian via the inverse lattice."""
+    nr = grid.nr
+    cell_inv = np.linalg.inv(grid.lattice)
+    ix = np.arange(order)
+    F = np.zeros((len(frac), 3))
+    for a, s in enumerate(frac):
+        Up = s * nr
+        fl = np.floor(Up).astype(np.int64)
+        M = [calc_Mn(Up[j] - fl[j], order) for j in range(3)]
+        # order - 1 splines in an array of order + 1 entries, as calc_Mn
+        # (ewald.py:66-86) returns them
+        M2 = [np.append(calc_Mn(Up[j] - fl[j], order - 1), 0.0) for j in range(3)]
+        d = [M2[j][1:order + 1] - M2[j][0:order] for j in range(3)]
+        m = [M[j][1:order + 1] for j in range(3)]
+        Qd = np.stack([nr[0] * np.einsum("i,j,k->ijk", d[0], m[1], m[2]),
+                       nr[1] * np.einsum("i,j,k->ijk", m[0], d[1], m[2]),
+                       nr[2] * np.einsum("i,j,k->ijk", m[0], m[1], d[2])])
+        sub = field[np.ix_(np.mod(1 + fl[0] - ix, nr[0]), np.mod(1 + fl[1] - ix, nr[1]),
+                           np.mod(1 + fl[2] - ix, nr[2]))]
+        F[a] = -np.einsum("dijk,ijk->d", Qd, sub) @ cell_inv
+    return F
+
+
+def pp_forces_pme(rho, grid, atoms, pp, order=10):
+    """``pseudo/__init__.py:475-520`` (_ForcePME): per species the field
+    IFFT(conj(B) vloc rho(G)) (G = 0 dropped) contracted with the derivative
+    of each atom's B-spline stencil."""
+    rhoG = grid.fft(rho) * np.conjugate(barray(grid, order))
+    frac = atoms.scaled_positions()
+    F = np.zeros((atoms.nat, 3))
+    for key in sorted(pp):
+        idx = [i for i, s in enumerate(atoms.symbols) if s == key]
+        if not idx:
+            continue
+        x = rhoG * vloc_line(grid.q, *pp[key])
+        x[0, 0, 0] = 0.0
+        F[idx] = _pme_stencil_forces(grid.ifft(x), grid, frac[idx], order)
+    return F
+
+
+def pp_stress_pme(rho, grid, atoms, pp, energy, order=10):
+    """``pseudo/__init__.py:522-563`` (_StressPME): the exact form with each
+    species' structure factor replaced by B rfftn(Q_s)."""
+    rhoG = grid.fft(rho)
+    B = barray(grid, order)
+    frac = atoms.scaled_positions()
+    v = np.zeros(grid.nrG, dtype=np.complex128)
+    for key in sorted(pp):
+        idx = [i for i, s in enumerate(atoms.symbols) if s == key]
+        if not idx:
+            continue
+        S = B * grid.fft(pme_qarray(grid, frac[idx], np.ones(len(idx)), order)) / grid.dV
+        v += vloc_deriv_line(grid.q, *pp[key]) * np.conjugate(S)
+    qq = grid.q.copy()
+    qq[0, 0, 0] = 1.0
+    invq = 1.0 / qq
+    invq[0, 0, 0] = 0.0
+    m = grid.mask
+    x = (rhoG * v * invq)[m]
+    g = grid.g[:, m]
+    S = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(i, 3):
+            S[i, j] = S[j, i] = -(np.sum(g[i] * g[j] * x)).real / grid.volume * 2.0
+    S -= np.eye(3) * energy
+    return S / grid.volume
+
+
+def ewald_forces_rec_pme(grid, atoms, eta, order=10):
+    """``ewald.py:810-864`` (Forces_rec_PME)."""
+    Q = pme_qarray(grid, atoms.scaled_positions(), atoms.charges, order)
+    B = barray(grid, order)
+    x = grid.fft(Q) * (B * np.conjugate(B)) * np.exp(-grid.gg / (4.0 * eta)) * grid.invgg
+    F = _pme_stencil_forces(grid.ifft(x), grid, atoms.scaled_positions(), order)
+    return F * atoms.charges[:, None] * 4.0 * np.pi / grid.dV
+
+
+def ewald_stress_rec_pme(grid, atoms, eta, order=10):
+    """``ewald.py:915-960`` (Stress_rec_PME): the exact form with S_Z(G)
+    replaced by B rfftn(Q_Z)."""
+    Q = pme_qarray(grid, atoms.scaled_positions(), atoms.charges, order)
+    s = grid.fft(Q) * barray(grid, order) / grid.dV
+    s2 = (np.conjugate(s) * s).real
+    m = grid.mask
+    gg, invgg, g = grid.gg[m], grid.invgg[m], grid.g[:, m]
+    w = s2[m] * np.exp(-gg / (4.0 * eta)) * invgg
+    st = []
+    for i in range(3):
+        for j in range(i, 3):
+            f = g[i] * g[j] * 2.0 * invgg * (1 + gg / (4.0 * eta))
+            if i == j:
+                f = f - 1.0
+            st.append(np.sum(w * f))
+    S = _sym(np.array(st) * 4.0 * np.pi / grid.volume ** 2)
+    S += np.eye(3) * atoms.ncharge() ** 2 * 4.0 * np.pi * (1.0 / (4.0 * eta * grid.volume ** 2) / 2.0)
+    return S
+
+
+def ewald_forces_pme(grid, atoms, precision=1e-8, order=10):
+    eta = ewald_eta(grid, atoms, precision)
+    return ewald_forces_real(grid, atoms, eta, precision) + ewald_forces_rec_pme(grid, atoms, eta, order)
+
+
+def ewald_stress_pme(grid, atoms, precision=1e-8, order=10):
+    eta = ewald_eta(grid, atoms, precision)
+    return ewald_stress_real(grid, atoms, eta, precision) + ewald_stress_rec_pme(grid, atoms, eta, order)
 
 
 # ---------------------------------------------------------------- KEDF, Hartree, LDA

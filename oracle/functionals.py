@@ -85,8 +85,12 @@ def wt_kernel(q, rho0, x=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0):
 
 
 def wt(rho, grid, rho0=None, x=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, kernel=None, calc_e=True):
-    """WT nonlocal part for alpha == beta (``functional/kedf/wt.py:12-28,108-132``).
-    Returns (energy, potential, kernel)."""
+    """WT nonlocal part (``functional/kedf/wt.py:12-56,108-132``).  Returns
+    (energy, potential, kernel).  alpha == beta: v = 2 alpha rho^(beta-1)
+    IFFT(K FFT(rho^beta)) and E = sum v rho / (2 alpha) dV (``:18-22,124-125``);
+    otherwise v = alpha rho^(alpha-1) IFFT(K FFT(rho^beta)) + beta rho^(beta-1)
+    IFFT(K FFT(rho^alpha)) (``:23-25``) and E = sum rho^alpha IFFT(K
+    FFT(rho^beta)) dV (``:44-56``), rho clamped at 1e-30 in both."""
     if rho0 is None:
         rho0 = rho.mean()
     if kernel is None:
@@ -94,9 +98,15 @@ def wt(rho, grid, rho0=None, x=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, kernel=None
     r = rho.copy()
     r[r < ZERO] = ZERO
     rb = r ** beta
-    ra1 = rb / r
-    pot = 2.0 * alpha * ra1 * grid.ifft(grid.fft(rb) * kernel)
-    ene = (pot * rho / (2 * alpha)).sum() * grid.dV if calc_e else 0.0
+    cb = grid.ifft(grid.fft(rb) * kernel)
+    if abs(beta - alpha) < 1e-9:
+        pot = 2.0 * alpha * (rb / r) * cb
+        ene = (pot * rho / (2 * alpha)).sum() * grid.dV if calc_e else 0.0
+    else:
+        ra = r ** alpha
+        ca = grid.ifft(grid.fft(ra) * kernel)
+        pot = alpha * r ** (alpha - 1.0) * cb + beta * r ** (beta - 1.0) * ca
+        ene = (ra * cb).sum() * grid.dV if calc_e else 0.0
     return ene, pot, kernel
 
 
@@ -148,6 +158,33 @@ def lda_v2(rho):
     fc[r2] = g / 36.0 * (5.0 * ta + 7.0 * ta * ta + 8.0 * tb + 16.0 * tb * tb + 21.0 * ta * tb) / de ** 3
     fc /= rho
     return fx + fc
+
+
+# ---------------------------------------------------------------- 'D' and 'V2'
+def energy_density(rho, grid, terms=("TF", "vW", "NL"), x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0,
+                   kernel=None):
+    """calcType 'D' of the KEDF terms: TF C rho^(5/3) (``kedf/tf.py:26-35,
+    68-73``, without x), vW y v rho (``kedf/vw.py:125-126``), WT v rho / (2
+    alpha) for alpha == beta (``kedf/wt.py:123-129``), else rho^alpha (K *
+    rho^beta) (``:44-56``).  Hartree and the local PP have none."""
+    d = np.zeros_like(rho)
+    if "TF" in terms:
+        d += np.cbrt(rho ** 5) * CTF
+    if "vW" in terms:
+        d += vw(rho, grid, y=y)[1] * rho
+    if "NL" in terms:
+        if abs(alpha - beta) < 1e-9:
+            d += wt(rho, grid, alpha=alpha, beta=beta, kernel=kernel)[1] * rho / (2 * alpha)
+        else:
+            r = np.maximum(rho, ZERO)
+            K = wt_kernel(grid.q, rho.mean(), alpha=alpha, beta=beta) if kernel is None else kernel
+            d += r ** alpha * grid.ifft(grid.fft(r ** beta) * K)
+    return d
+
+
+def tf_v2(rho, x=1.0):
+    """``kedf/tf.py:47-50,77-79``: x 10/9 C rho^(-1/3)."""
+    return x * 10.0 / 9.0 * CTF / np.cbrt(rho)
 
 
 # ---------------------------------------------------------------- total

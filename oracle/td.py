@@ -2,7 +2,8 @@
 ``td/hamiltonian.py:70-114``, ``field.py:343-350`` (smoothed Laplacian),
 ``td/propagator/crank_nicholson.py:44-56,147-188``, ``linear_solver.py:8-47``
 (CG without conjugation), ``utils/utils.py:40-41`` (calc_rho) and
-``td/utils.py:10-16`` (delta kick), ``utils/utils.py:44-47`` (current
+``td/utils.py:10-16`` (delta kick), ``linear_solver.py:49-110`` (bicg,
+bicgstab) and scipy's gmres (the ``gmres_scipy`` solver), ``utils/utils.py:44-47`` (current
 density) and ``td/predictor_corrector.py:84-150`` (predictor-corrector step)."""
 import numpy as np
 
@@ -42,12 +43,84 @@ def cg(A, b, x0, tol=1e-8, maxiter=100, atol=None):
     return x, 1, k
 
 
-def cn_step(psi0, v, grid, dt, tol=1e-8, maxiter=100, sigma=0.025):
+def bicg(A, b, x0, tol=1e-8, maxiter=100):
+    """``linear_solver.py:49-75``: CG with conjugated sums."""
+    bn = _norm(b)
+    atol = tol * bn if bn != 0 else tol
+    res = b - A(x0)
+    if _norm(res) < atol:
+        return x0, 0, 0
+    p, x, k = res, x0, 0
+    while k < maxiter:
+        v = A(p)
+        rr = np.sum(np.conj(res) * res)
+        alpha = rr / np.sum(np.conj(p) * v)
+        x = x + alpha * p
+        new = res - alpha * v
+        if _norm(new) < atol:
+            return x, 0, k + 1
+        beta = np.sum(np.conj(new) * new) / rr
+        p = new + beta * p
+        res = new
+        k += 1
+    return x, 1, k
+
+
+def bicgstab(A, b, x0, tol=1e-8, maxiter=100):
+    """``linear_solver.py:78-110`` (sums without conjugation)."""
+    bn = _norm(b)
+    atol = tol * bn if bn != 0 else tol
+    r0 = b - A(x0)
+    if _norm(r0) < atol:
+        return x0, 0, 0
+    rho_old, omega, alpha = 1.0, 1.0, 1.0
+    v = 0 * b
+    p = v
+    x, res, r, k = x0, r0, r0, 0
+    while k < maxiter:
+        rho = np.sum(r0 * r)
+        beta = rho / rho_old * alpha / omega
+        p = r + beta * (p - omega * v)
+        v = A(p)
+        alpha = rho / np.sum(r0 * v)
+        h = x + alpha * p
+        res = res - alpha * v
+        if _norm(res) < atol:
+            return h, 0, k + 1
+        s = r - alpha * v
+        t = A(s)
+        omega = np.sum(t * s) / np.sum(t * t)
+        x = h + omega * s
+        res = res - omega * t
+        if _norm(res) < atol:
+            return x, 0, k + 1
+        r = s - omega * t
+        rho_old = rho
+        k += 1
+    return x, 1, k
+
+
+def gmres(A, b, x0, tol=1e-8, maxiter=100, restart=20):
+    """The ``gmres_scipy`` choice of ``crank_nicholson.py:65-77``: scipy's
+    restarted GMRES on the operator, relative tolerance tol (maxiter in
+    restart cycles, as scipy counts them)."""
+    import scipy.sparse.linalg as sla
+    shape = b.shape
+    n = b.size
+    op = sla.LinearOperator((n, n), matvec=lambda y: A(y.reshape(shape)).ravel(), dtype=np.complex128)
+    x, info = sla.gmres(op, b.ravel(), x0=x0.ravel(), rtol=tol, atol=0.0, restart=restart, maxiter=maxiter)
+    return x.reshape(shape), info, None
+
+
+SOLVERS = {"cg": cg, "bicg": bicg, "bicgstab": bicgstab, "gmres": gmres}
+
+
+def cn_step(psi0, v, grid, dt, tol=1e-8, maxiter=100, sigma=0.025, solver="cg"):
     """``crank_nicholson.py:147-188``: (1 + i H dt/2) psi1 = (1 - i H dt/2) psi0."""
     H = lambda x: hamiltonian_apply(x, v, grid, sigma)
     A = lambda x: x + 1j * H(x) * dt / 2.0
     b = psi0 - 1j * H(psi0) * dt / 2.0
-    return cg(A, b, psi0, tol, maxiter)
+    return SOLVERS[solver](A, b, psi0, tol, maxiter)
 
 
 def initial_kick(k, direction, psi, grid):

@@ -10,10 +10,11 @@ pytestmark = pytest.mark.gpu
 
 SHAPE = (12, 10, 15)
 DIL = (1, 2, 1, 3, 2, 1)
-# (12, 10, 15): every layer on the point-tiled kernel; (6, 8, 32) and
-# (4, 16, 64): layers with ny % 4d == 0 on the row-tiled kernel, the rest
-# on the point-tiled one
-SHAPES = [(12, 10, 15), (6, 8, 32), (4, 16, 64)]
+# (12, 10, 15): every layer on the point-tiled kernel; (6, 8, 32),
+# (4, 16, 64), (4, 8, 96) and (4, 8, 160): layers with ny % 4d == 0 on the
+# row-tiled kernel (one instance per nz / 32 = 1, 2, 3, 5: 96 is the
+# training grid, 160 the 500-atom cell's), the rest on the point-tiled one
+SHAPES = [(12, 10, 15), (6, 8, 32), (4, 16, 64), (4, 8, 96), (4, 8, 160)]
 
 
 def _setup(seed=0, scale=1.5, shape=SHAPE):
@@ -29,9 +30,9 @@ def _setup(seed=0, scale=1.5, shape=SHAPE):
     return p, x0, target
 
 
-def _net(p):
+def _net(p, dil=DIL):
     from dftpy_amd.learned import WarmStartNet
-    net = WarmStartNet(dil=DIL)
+    net = WarmStartNet(dil=dil)
     net.set_params(p)
     return net
 
@@ -73,6 +74,65 @@ def test_backward_matches_oracle(shape):
         got = net.param(k, net.grad).cpu().numpy().astype(np.float64)
         print("%s rel %.2e" % (k, _rel(got, g)))
         assert _rel(got, g) < 1e-2, k
+
+
+@pytest.mark.parametrize("shape", [(8, 16, 96), (8, 16, 160)])
+def test_shipped_dilations_match_oracle(shape):
+    """The shipped network's dilations (1, 2, 4, 4, 2, 1) with ny a multiple
+    of 4d and nx of d for every layer: the inference forward runs all four
+    middle layers on the ring-staged kernel, the training forward and the
+    backward on the row-tiled instances of the 96^3 training grid
+    (nz / 32 = 3) and of the 160^3 cell (nz / 32 = 5)."""
+    from oracle import cnn
+    from dftpy_amd import DirectGrid
+    from dftpy_amd.learned import WarmStartNet
+    dil = WarmStartNet().dil
+    assert dil == (1, 2, 4, 4, 2, 1)
+    p, x0, target = _setup(3, shape=shape)
+    ref, cache = cnn.forward(p, x0, dil, bf16=True)
+    net = _net(p, dil)
+    x = torch.as_tensor(x0.reshape(2, -1), dtype=torch.float32, device="cuda")
+    for train in (False, True):
+        out = net.forward(x, shape, train=train).cpu().numpy().astype(np.float64).reshape(shape)
+        print("forward (train=%s) rel L2 %.2e" % (train, _rel(out, ref)))
+        assert _rel(out, ref) < 2e-3
+        assert np.abs(out - ref).max() < 2e-2 * np.abs(ref).max()
+    ref_loss, dout = cnn.loss(ref, target)
+    gref = cnn.backward(p, cache, dout)
+    grid = DirectGrid(np.eye(3) * 5.0, shape, full=False, dtype=torch.float32)
+    tgt = torch.as_tensor(target.reshape(-1), dtype=torch.float32, device="cuda")
+    loss = net.train_step(grid.plan(torch.float32), x, shape, tgt, lr=0.0)
+    assert abs(float(loss.item()) - ref_loss) < 1e-2 * ref_loss
+    for k, g in gref.items():
+        got = net.param(k, net.grad).cpu().numpy().astype(np.float64)
+        print("%s rel %.2e" % (k, _rel(got, g)))
+        assert _rel(got, g) < 1e-2, k
+
+
+def test_ring_forward_matches_row_kernel_at_160():
+    """At the 500-atom cell's full 160^3 grid (where the FP64 oracle is too
+    slow to run), the inference forward (ring-staged middle layers) against
+    the training forward (row-tiled middle layers, the instance the oracle
+    tests above pin at nz = 160): the same network, the same bf16 rounding
+    points, FP32 accumulation in a different order."""
+    from dftpy_amd.learned import WarmStartNet
+    from oracle import cnn
+    shape = (160, 160, 160)
+    p = cnn.init_params(21, scale=1.5)
+    rng = np.random.default_rng(21)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = rng.normal(0, 0.2, p[k].shape)
+    net = WarmStartNet()
+    net.set_params({k: v.astype(np.float32).astype(np.float64) for k, v in p.items()})
+    x = torch.randn((2, int(np.prod(shape))), dtype=torch.float32, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(5))
+    a = net.forward(x, shape, train=False).clone()
+    b = net.forward(x, shape, train=True).clone()
+    rel = float((a - b).norm() / b.norm())
+    print("ring vs row kernel at 160^3: rel L2 %.2e, max %.2e" % (rel, float((a - b).abs().max())))
+    assert rel < 1e-3
+    assert float((a - b).abs().max()) < 1e-2 * float(b.abs().max())
 
 
 def _oracle_adam(p, x0, tgt, lr, steps):

@@ -41,6 +41,35 @@ def test_wt_golden_and_potential(al_pp, full):
     assert np.max(np.abs(v - v_ref)) < 1e-10 * max(1.0, np.max(np.abs(v_ref)))
 
 
+@pytest.mark.parametrize("full,dtype", [(True, torch.float64), (False, torch.float64), (False, torch.float32)])
+def test_wt_alpha_ne_beta_vs_oracle(al_pp, full, dtype):
+    """WT nonlocal term with alpha != beta (wt.py:23-25,44-56; here the
+    alpha + beta = 5/3 pair (5 +- sqrt 5) / 6) on Al_fde_rho.pp: energy and
+    potential against the oracle (FP64 1e-11, FP32 stated), and the FP64
+    stress (wt.py:75-105 with rho^alpha and rho^beta) against the oracle."""
+    from dftpy_amd import DirectField
+    from dftpy_amd.functional import Functional
+    from oracle import forces as FO
+    atoms, rho, lat = al_pp
+    a, b = (5 + np.sqrt(5)) / 6, (5 - np.sqrt(5)) / 6
+    g = _dg(lat, rho.shape, full, dtype)
+    f = Functional(type="KEDF", name="WT-NL", alpha=a, beta=b)
+    out = f(DirectField(g, data=rho))
+    og = OG.Grid(lat, rho.shape, full=full)
+    e_ref, v_ref, _ = OF.wt(rho, og, alpha=a, beta=b)
+    v = out.potential.numpy().astype(np.float64)
+    de, dv = abs(out.energy - e_ref), float(np.max(np.abs(v - v_ref)))
+    print("WT alpha %.4f beta %.4f %s: E %.12f oracle %.12f |dE| %.2e max|dV| %.2e" %
+          (a, b, dtype, out.energy, e_ref, de, dv))
+    if dtype == torch.float64:
+        assert de <= 1e-11 * abs(e_ref) and dv <= 1e-10 * max(1.0, np.max(np.abs(v_ref)))
+        st = f.stress(DirectField(g, data=rho), split=True)["WT-NL"]
+        st_ref = FO.wt_stress(rho, og, e_ref, alpha=a, beta=b)
+        assert np.max(np.abs(st - st_ref)) <= 1e-9 * np.max(np.abs(st_ref))
+    else:
+        assert de <= 1e-5 * abs(e_ref) and dv <= 1e-5 * max(1.0, np.max(np.abs(v_ref)))
+
+
 def _c1_density(nr, lat, seed=0):
     rng = np.random.default_rng(seed)
     og = OG.Grid(lat, nr)
@@ -80,6 +109,74 @@ def test_total_terms_vs_oracle(dtype):
     else:
         assert abs(out.energy - e_ref) / nat < 2e-5
         assert np.max(np.abs(out.potential.numpy() - v_ref)) < 2e-4
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_energy_density_and_v2_vs_oracle(dtype):
+    """calcType {'E', 'V', 'D', 'V2'} of a TotalFunctional (WT KEDF, PZ-LDA,
+    Hartree, local PP): the energy density of the KEDF terms and the V2 of TF
+    and LDA against the oracle; E and V equal the plain evaluation's; WT with
+    alpha != beta gives its own energy density."""
+    from dftpy_amd import DirectField, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    ions, nr = systems.c1_system()
+    lat = ions.cell
+    rho = _c1_density(nr, lat)
+    g = _dg(lat, nr, False, dtype)
+    pp = Functional(type="PSEUDO", grid=g, ions=ions, PP_list={"Al": systems.empty_core_recpot()})
+    tot = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                          H=Functional(type="HARTREE"), PSEUDO=pp)
+    r = DirectField(g, data=rho)
+    out = tot(r, calcType={"E", "V", "D", "V2"})
+    plain = tot(r, calcType={"E", "V"})
+    og = OG.Grid(lat, nr)
+    d_ref = OF.energy_density(rho, og, ("TF", "vW", "NL"))
+    v2_ref = OF.tf_v2(rho) + OF.lda_v2(rho)
+    tol = 1e-10 if dtype == torch.float64 else 2e-5
+    dd = np.max(np.abs(out.energydensity.numpy() - d_ref)) / np.max(np.abs(d_ref))
+    dv2 = np.max(np.abs(out.v2rho2.numpy() - v2_ref)) / np.max(np.abs(v2_ref))
+    print("%s: energy density rel %.2e, v2rho2 rel %.2e" % (dtype, dd, dv2))
+    assert dd < tol and dv2 < tol
+    assert out.energy == plain.energy
+    assert torch.equal(out.potential.data, plain.potential.data)
+    a, b = (5 + np.sqrt(5)) / 6, (5 - np.sqrt(5)) / 6
+    nl = Functional(type="KEDF", name="WT-NL", alpha=a, beta=b)(r, calcType={"E", "D"})
+    d_ab = OF.energy_density(rho, og, ("NL",), alpha=a, beta=b)
+    assert np.max(np.abs(nl.energydensity.numpy() - d_ab)) < tol * np.max(np.abs(d_ab))
+    assert np.isclose(float(nl.energydensity.data.double().sum().item()) * og.dV, nl.energy,
+                      rtol=1e-12 if dtype == torch.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_xc_core_density_vs_oracle(dtype):
+    """Non-linear core correction (xc/__init__.py:59-74): XC(core_density=c)
+    evaluates PZ-LDA at rho + c (energy, potential and V2), alone and inside
+    a TotalFunctional, against the oracle on the shifted density; the other
+    terms still see rho."""
+    from dftpy_amd import DirectField, systems
+    from dftpy_amd.functional import Functional, TotalFunctional
+    ions, nr = systems.c1_system()
+    lat = ions.cell
+    rho = _c1_density(nr, lat)
+    og = OG.Grid(lat, nr)
+    r = og.r
+    core = 0.05 * np.exp(-((r[0] - 1.9) ** 2 + (r[1] - 3.8) ** 2 + (r[2] - 1.9) ** 2) / 0.8)
+    g = _dg(lat, nr, False, dtype)
+    cf = DirectField(g, data=core)
+    xc = Functional(type="XC", name="LDA", core_density=cf)
+    out = xc(DirectField(g, data=rho), calcType={"E", "V", "V2"})
+    e_ref, v_ref = OF.lda(rho + core, og)
+    tol = 1e-11 if dtype == torch.float64 else 2e-5
+    print("%s: XC with core |dE| %.2e, max|dV| %.2e" % (dtype, abs(out.energy - e_ref),
+                                                        np.max(np.abs(out.potential.numpy() - v_ref))))
+    assert abs(out.energy - e_ref) <= tol * abs(e_ref)
+    assert np.max(np.abs(out.potential.numpy() - v_ref)) <= tol * np.max(np.abs(v_ref))
+    v2_ref = OF.lda_v2(rho + core)
+    assert np.max(np.abs(out.v2rho2.numpy() - v2_ref)) <= tol * np.max(np.abs(v2_ref))
+    tot = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=xc, H=Functional(type="HARTREE"))
+    o2 = tot(DirectField(g, data=rho))
+    e2 = OF.tf(rho, og)[0] + OF.vw(rho, og)[0] + OF.wt(rho, og)[0] + e_ref + OF.hartree(rho, og)[0]
+    assert abs(o2.energy - e2) <= tol * abs(e2)
 
 
 @pytest.mark.parametrize("pme", [True, False])

@@ -165,3 +165,40 @@ def test_predictor_corrector_and_current_vs_oracle():
     j = calc_j(psi).cpu().numpy()
     j_ref = OT.calc_j(psi_np, og)
     assert np.max(np.abs(j - j_ref)) < 1e-8 * np.max(np.abs(j_ref))
+
+
+@pytest.mark.parametrize("solver,dtype", [("bicgstab", torch.float64), ("gmres", torch.float64), ("bicg", torch.float64),
+                                          ("bicgstab", torch.float32), ("gmres", torch.float32)])
+def test_cn_linear_solvers_vs_oracle(solver, dtype):
+    """One Crank-Nicolson step of a kicked 4-atom Al orbital (32^3, dt 0.1)
+    with the linear solvers of crank_nicholson.py:65-77 against the complex128
+    oracle with the same solver: bicgstab follows linear_solver.py:78-110 step
+    for step; GMRES(20) reaches scipy's gmres solution within the tolerance;
+    bicg (CG with conjugated sums) is compared after 6 iterations, where both
+    still fall short of the tolerance on this non-Hermitian operator."""
+    from dftpy_amd import DirectField, DirectGrid, systems
+    from dftpy_amd.td import CrankNicholson, Hamiltonian, initial_kick
+    ions, nr = systems.c1_system()
+    g = DirectGrid(ions.cell, nr, full=True, dtype=dtype)
+    og = OG.Grid(ions.cell, nr, full=True)
+    r = og.r
+    rho = 0.027 * (1 + 0.3 * np.cos(2 * np.pi * r[0] / 7.6) * np.cos(2 * np.pi * r[2] / 7.6))
+    v = 0.2 * np.cos(2 * np.pi * r[1] / 7.6) - 0.1
+    psi = initial_kick(1e-2, 0, DirectField(g, data=torch.as_tensor(np.sqrt(rho), device="cuda")))
+    ham = Hamiltonian(v=DirectField(g, data=torch.as_tensor(v, device="cuda")))
+    tol = 1e-10 if dtype == torch.float64 else 1e-6
+    maxiter = 6 if solver == "bicg" else 200
+    cn = CrankNicholson(ham, 0.1, linearsolver=solver, tol=tol, maxiter=maxiter)
+    print("cn solver", solver, dtype, flush=True)
+    out, info = cn(psi)
+    ps0 = OTD.initial_kick(1e-2, 0, np.sqrt(rho), og)
+    ref, st, _ = OTD.cn_step(ps0, v, og, 0.1, tol=tol, maxiter=maxiter if solver != "gmres" else 10,
+                             solver=solver)
+    err = np.max(np.abs(out.data.cpu().numpy() - ref)) / np.max(np.abs(ref))
+    print("  status %d (oracle %d), %d iterations, rel. max error %.2e" % (info, st, cn.last_info["iterations"], err))
+    if solver == "bicg":
+        assert info == 1 and st == 1
+        assert err < 1e-10
+    else:
+        assert info == 0 and st == 0
+        assert err < (1e-8 if dtype == torch.float64 else 1e-5)

@@ -115,6 +115,53 @@ def test_lda_potential_is_derivative():
     assert np.isclose((ep - em) / (2e-6 * g.dV), v[1, 2, 3], rtol=1e-6)
 
 
+def test_wt_alpha_ne_beta_is_a_derivative():
+    """WT with alpha != beta (wt.py:23-25,44-56): the potential is dE/drho by
+    central differences (kernel held fixed), E and V are symmetric in
+    (alpha, beta) (K is even), and alpha -> beta approaches the alpha == beta
+    branch.  A dropped cross term or a swapped exponent fails one of them."""
+    lat = np.diag([5.0, 5.5, 6.0])
+    g = G.Grid(lat, (6, 8, 10))
+    rng = np.random.default_rng(3)
+    rho = 0.02 + 0.02 * rng.random((6, 8, 10))
+    a, b = 0.7, 1.0 - 0.7 / 3.0 + 0.4
+    K = F.wt_kernel(g.q, rho.mean(), alpha=a, beta=b)
+    e, v, _ = F.wt(rho, g, alpha=a, beta=b, kernel=K)
+    for idx in ((1, 2, 3), (4, 7, 0), (5, 0, 9)):
+        d = np.zeros_like(rho)
+        d[idx] = 1e-7
+        ep = F.wt(rho + d, g, alpha=a, beta=b, kernel=K)[0]
+        em = F.wt(rho - d, g, alpha=a, beta=b, kernel=K)[0]
+        assert np.isclose((ep - em) / (2e-7 * g.dV), v[idx], rtol=1e-6)
+    e2, v2, _ = F.wt(rho, g, alpha=b, beta=a, kernel=K)
+    assert np.isclose(e2, e, rtol=1e-12) and np.allclose(v2, v, rtol=1e-12)
+    c = 5.0 / 6.0
+    Kc = F.wt_kernel(g.q, rho.mean(), alpha=c, beta=c)
+    e3, v3, _ = F.wt(rho, g, alpha=c, beta=c, kernel=Kc)
+    e4, v4, _ = F.wt(rho, g, alpha=c * (1 + 1e-7), beta=c, kernel=Kc)
+    assert np.isclose(e4, e3, rtol=1e-5) and np.allclose(v4, v3, rtol=1e-5)
+
+
+def test_energy_density_and_v2_pins():
+    """calcType 'D' integrates to each term's energy (TF, vW, WT with
+    alpha == beta and alpha != beta), and 'V2' is dV/drho of the local terms
+    (TF, PZ-LDA on both rs branches) by central differences."""
+    lat = np.diag([5.0, 5.5, 6.0])
+    g = G.Grid(lat, (6, 8, 10))
+    rng = np.random.default_rng(4)
+    rho = 0.01 + 0.5 * rng.random((6, 8, 10))
+    for terms, e in ((("TF",), F.tf(rho, g)[0]), (("vW",), F.vw(rho, g)[0]), (("NL",), F.wt(rho, g)[0])):
+        assert np.isclose(F.energy_density(rho, g, terms).sum() * g.dV, e, rtol=1e-12), terms
+    a, b = (5 + np.sqrt(5)) / 6, (5 - np.sqrt(5)) / 6
+    e_ab = F.wt(rho, g, alpha=a, beta=b)[0]
+    assert np.isclose(F.energy_density(rho, g, ("NL",), alpha=a, beta=b).sum() * g.dV, e_ab, rtol=1e-12)
+    h = 1e-6 * rho
+    fd_tf = (F.tf(rho + h, g, x=0.7)[1] - F.tf(rho - h, g, x=0.7)[1]) / (2 * h)
+    assert np.allclose(F.tf_v2(rho, x=0.7), fd_tf, rtol=1e-7)
+    fd_xc = (F.lda(rho + h, g)[1] - F.lda(rho - h, g)[1]) / (2 * h)
+    assert np.allclose(F.lda_v2(rho), fd_xc, rtol=1e-6)
+
+
 def test_hartree_parseval():
     lat = np.diag([6.0, 7.0, 8.0])
     g = G.Grid(lat, (12, 14, 16))
@@ -355,6 +402,41 @@ def test_predictor_corrector_with_a_fixed_potential_is_one_cn_step():
     assert n == 2 and conv and np.array_equal(out, ref) and v2 is v
 
 
+@pytest.mark.parametrize("solver", ["cg", "bicg", "bicgstab", "gmres"])
+def test_cn_solvers_against_a_dense_solve(solver):
+    """The linear solvers of the CN step (linear_solver.py:24-110 and scipy's
+    gmres) reach the dense solution of (1 + i H dt/2) psi1 = (1 - i H dt/2)
+    psi0 on a 4x4x4 grid (H built column by column from the oracle's
+    Hamiltonian), to their tolerance.  The reference's bicg is CG with
+    conjugated sums: it needs a Hermitian positive-definite operator, so it
+    is pinned on 1 + H dt/2 instead (on the CN operator it stalls, as the
+    reference's does)."""
+    lat = np.diag([3.0, 3.5, 4.0])
+    g = G.Grid(lat, (4, 4, 4), full=True)
+    rng = np.random.default_rng(8)
+    v = 0.3 * rng.random((4, 4, 4))
+    psi0 = (1 + 0.2 * rng.random((4, 4, 4))) * np.exp(1j * rng.random((4, 4, 4)))
+    dt = 0.1
+    n = 64
+    Hm = np.zeros((n, n), complex)
+    for j in range(n):
+        e = np.zeros(n, complex)
+        e[j] = 1.0
+        Hm[:, j] = td.hamiltonian_apply(e.reshape(4, 4, 4), v, g).ravel()
+    if solver == "bicg":
+        A = np.eye(n) + 0.5 * dt * Hm
+        b = psi0.ravel()
+        x, info, _ = td.bicg(lambda y: y + 0.5 * dt * td.hamiltonian_apply(y, v, g), psi0, 0 * psi0, tol=1e-12,
+                             maxiter=200)
+        ref = np.linalg.solve(A, b).reshape(4, 4, 4)
+    else:
+        A = np.eye(n) + 0.5j * dt * Hm
+        ref = np.linalg.solve(A, (np.eye(n) - 0.5j * dt * Hm) @ psi0.ravel()).reshape(4, 4, 4)
+        x, info, _ = td.cn_step(psi0, v, g, dt, tol=1e-12, maxiter=200, solver=solver)
+    assert info == 0
+    assert np.max(np.abs(x - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
 def test_td_dipole_golden():
     # examples/test/test_propagator.py:28-81 (CN part)
     atoms, rho0, lat = io.read_xsf(data("GaAs_random.xsf"))
@@ -427,6 +509,42 @@ def test_cnn_oracle_gradient_is_the_derivative():
             q2[k] = p[k] - eps * v
             errs.append(abs((L(q1) - L(q2)) / (2 * eps) - an))
         assert min(errs) <= 2e-8 * max(1.0, abs(an)) + 5e-8, (k, an, errs)
+
+
+def test_cnn_oracle_forward_is_torch_conv3d():
+    """The FP64 oracle network equals torch's conv3d with circular padding
+    and dilation (layer by layer: ReLU, residual sums, the 1-channel output
+    layer), on an uneven grid with dilations 1, 2 and 4 that wrap around
+    the cell.  A transposed weight, a mirrored tap or a wrong shift sign
+    fails it."""
+    import torch
+    import torch.nn.functional as TF
+    from oracle import cnn
+    rng = np.random.default_rng(5)
+    shape = (9, 10, 11)
+    dil = (1, 2, 4, 4, 2, 1)
+    p = cnn.init_params(7, scale=2.0)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = rng.normal(0, 0.1, p[k].shape)
+    x0 = rng.normal(0, 1, (2,) + shape)
+    ref, _ = cnn.forward(p, x0, dil)
+
+    def conv(x, w, b, d):
+        # x (C, nx, ny, nz); w (27, ci, co) with tap t = (i*3 + j)*3 + k
+        # reading x[p + d (i-1, j-1, k-1)]
+        ci, co = w.shape[1], w.shape[2]
+        wt = torch.as_tensor(w).reshape(3, 3, 3, ci, co).permute(4, 3, 0, 1, 2).contiguous()
+        xp = TF.pad(x[None], (d,) * 6, mode="circular")
+        return TF.conv3d(xp, wt, torch.as_tensor(b), dilation=d)[0]
+
+    x = torch.as_tensor(x0)
+    h = torch.relu(conv(x, p["W1"], p["b1"], dil[0]))
+    for l in range(2, 6):
+        w = np.transpose(p["W%d" % l], (0, 2, 1))  # W_l[t, co, ci] -> (27, ci, co)
+        h = h + torch.relu(conv(h, w, p["b%d" % l], dil[l - 1]))
+    out = conv(h, p["W6"][:, :, None], p["b6"], dil[5])[0].numpy()
+    assert np.abs(out - ref).max() < 1e-12 * np.abs(ref).max()
 
 
 def test_lmgp_golden_and_heg(al_pp):

@@ -99,3 +99,67 @@ def test_stresses_are_strain_derivatives():
         for k in S:
             fd = (Ep[k] - Em[k]) / (2 * h) / g.volume / (1.0 if i == j else 2.0)
             assert abs(fd - S[k][i, j]) < 2e-6 * max(1e-3, np.abs(S[k]).max()), (k, i, j, fd, S[k][i, j])
+
+
+@pytest.mark.parametrize("skew", [False, True])
+def test_pme_forces_are_minus_pme_energy_gradients(skew):
+    """PME forms (pseudo/__init__.py:475-520, ewald.py:810-864) against
+    central differences of the PME energies (order-10 B-splines are smooth,
+    so the PME energy is differentiable).  On an orthogonal cell the PP and
+    Ewald forces are -dE/dR.  The reference contracts the stencil derivative
+    d/d(frac) with the inverse lattice as matmul(dQ^T, A^-1), which equals
+    A^-T A (-dE/dR) on a skewed cell (rows of A = lattice vectors): the
+    restatement keeps that form and is pinned by it there.  Both stay within
+    the PME-vs-exact tolerance of the reference (test_pme.py:34-36) on the
+    orthogonal cell."""
+    g, at, rho, pp = _cell()
+    if not skew:
+        A = np.diag(np.diag(g.lattice))
+        g = G.Grid(A, g.nr)
+        at = io.Atoms(at.symbols, at.scaled_positions() @ A, A, at.charges)
+    A = g.lattice
+    M = np.linalg.inv(A).T @ A
+    f_pp = FO.pp_forces_pme(rho, g, at, pp)
+    f_ew = FO.ewald_forces_pme(g, at)
+    h = 1e-4
+    for a in (0, 2):
+        fd = np.zeros((2, 3))
+        for d in range(3):
+            e = []
+            for s in (1.0, -1.0):
+                pos = at.positions.copy()
+                pos[a, d] += s * h
+                at2 = io.Atoms(at.symbols, pos, at.cell, at.charges)
+                e.append(((pseudo.local_pp(g, at2, pp, pme=True) * rho).sum() * g.dV,
+                          pseudo.ewald_energy(g, at2, pme=True)))
+            fd[0, d] = -(e[0][0] - e[1][0]) / (2 * h)
+            fd[1, d] = -(e[0][1] - e[1][1]) / (2 * h)
+        assert np.max(np.abs(M @ fd[0] - f_pp[a])) < 1e-6 * max(1.0, np.abs(f_pp[a]).max())
+        assert np.max(np.abs(M @ fd[1] - f_ew[a])) < 1e-6 * max(1.0, np.abs(f_ew[a]).max())
+    if not skew:
+        assert np.max(np.abs(f_pp - FO.pp_forces(rho, g, at, pp))) < 1e-4
+        assert np.max(np.abs(f_ew - FO.ewald_forces(g, at))) < 1e-4
+
+
+def test_pme_stresses_are_strain_derivatives():
+    """The PME stresses (pseudo/__init__.py:522-563, ewald.py:915-960) are
+    (1/V) dE/d(eps_ij) of the PME energies under homogeneous strain."""
+    g, at, rho, pp = _cell(n=20)
+
+    def energies(g, at, rho):
+        return dict(PP=(pseudo.local_pp(g, at, pp, pme=True) * rho).sum() * g.dV,
+                    II=pseudo.ewald_energy(g, at, pme=True))
+    E = energies(g, at, rho)
+    S = dict(PP=FO.pp_stress_pme(rho, g, at, pp, E["PP"]), II=FO.ewald_stress_pme(g, at))
+    h = 1e-5
+    for (i, j) in ((0, 0), (1, 1), (0, 1), (1, 2)):
+        eps = np.zeros((3, 3))
+        eps[i, j] = eps[j, i] = h
+        Ep = energies(*_strained(g, at, rho, eps))
+        Em = energies(*_strained(g, at, rho, -eps))
+        for k in S:
+            fd = (Ep[k] - Em[k]) / (2 * h) / g.volume / (1.0 if i == j else 2.0)
+            assert abs(fd - S[k][i, j]) < 2e-6 * max(1e-3, np.abs(S[k]).max()), (k, i, j, fd, S[k][i, j])
+    E0 = _energies(g, at, rho, pp)
+    assert np.max(np.abs(S["PP"] - FO.pp_stress(rho, g, at, pp, E0["PP"]))) < 1e-4
+    assert np.max(np.abs(S["II"] - FO.ewald_stress(g, at))) < 1e-4
