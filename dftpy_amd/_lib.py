@@ -55,6 +55,8 @@ SIGNATURES = {
     "ofdft_bench_pass": (_I, [_P, _I, _I, _I, _P, _DP, _P]),
     "ofdft_atomic_density": (_I, [_P, _I, _I, _IP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), _DP, _DP, _DP, _DP, _I,
                                   _DP, _IP, _P, _P]),
+    "ofdft_atomic_density_noalloc": (_I, [_P, _I, _I, _IP, ctypes.POINTER(_DP), ctypes.POINTER(_DP), _DP, _DP, _DP,
+                                          _DP, _I, _DP, _IP, _P, _P]),
     "ofdft_clamp_normalize": (_I, [_P, _P, _I, _D, _D, _P]),
     "ofdft_local_pp": (_I, [_P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP, _I, _I,
                             _P, _P]),

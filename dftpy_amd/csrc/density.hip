@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -116,6 +118,104 @@ __global__ void k_norm_scale(double *__restrict__ rho, long long n, const double
 constexpr int kMaxPlaceSpecies = 16;
 constexpr int kNormSlot = 240;
 
+// Species records and the upload layout shared by both placement entry
+// points: species records, breakpoints, coefficients, scaled positions,
+// species indices and the lattice, each part 256-byte aligned.
+struct PlaceUpload {
+    PlaceSpecies sp[kMaxPlaceSpecies];
+    long long nbk = 0;
+    size_t b_sp = 0, b_x = 0, b_c = 0, b_f = 0, b_s = 0, b_l = 0;
+    size_t o_x = 0, o_c = 0, o_f = 0, o_s = 0, o_l = 0, total = 0;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// validates the arguments and fills U (no device work)
+static void place_prepare(const Plan *P, int kind, int nspecies, const int *nbreak, const double *const *breaks,
+                          const double *const *coef, const double *rcut, const double *rtol, const double *sigma,
+                          const double *scale, int nat, const double *pos_frac, const int *species, const void *rho,
+                          PlaceUpload &U) {
+    OFDFT_REQUIRE(P && rho && rcut && scale && pos_frac && species, "atomic_density: bad arguments");
+    OFDFT_REQUIRE(P->prec == OFDFT_FP64, "atomic_density: needs an FP64 plan");
+    OFDFT_REQUIRE(kind == 0 || kind == 1, "atomic_density: kind is 0 (radial spline) or 1 (Gaussian)");
+    OFDFT_REQUIRE(nspecies >= 1 && nspecies <= kMaxPlaceSpecies, "atomic_density: 1..16 species");
+    OFDFT_REQUIRE(nat >= 0, "atomic_density: nat < 0");
+    if (kind == 0) OFDFT_REQUIRE(nbreak && breaks && coef, "atomic_density: spline tables missing");
+    if (kind == 1) OFDFT_REQUIRE(sigma, "atomic_density: sigma missing");
+    const int nr[3] = {P->nr[0], P->nr[1], P->nr[2]};
+    U.nbk = 0;
+    for (int k = 0; k < nspecies; ++k) {
+        PlaceSpecies &S = U.sp[k];
+        S.rcut = rcut[k];
+        S.rtol = rtol ? rtol[k] : 0.0;
+        S.sigma = kind == 1 ? sigma[k] : 1.0;
+        S.scale = scale[k];
+        OFDFT_REQUIRE(S.rcut > 0.0, "atomic_density: rcut must be positive");
+        if (kind == 1) OFDFT_REQUIRE(S.sigma > 0.0, "atomic_density: sigma must be positive");
+        for (int d = 0; d < 3; ++d) {
+            // border = min(int(rcut / spacing) + 1, nr // 2), spacing = |a_d| / nr_d
+            const double len = std::sqrt(P->lat[3 * d] * P->lat[3 * d] + P->lat[3 * d + 1] * P->lat[3 * d + 1] +
+                                         P->lat[3 * d + 2] * P->lat[3 * d + 2]);
+            const int b = (int)(S.rcut / (len / nr[d])) + 1;
+            S.border[d] = std::min(b, nr[d] / 2);
+        }
+        S.nbreak = kind == 0 ? nbreak[k] : 2;
+        if (kind == 0) OFDFT_REQUIRE(S.nbreak >= 2 && breaks[k] && coef[k], "atomic_density: bad spline table");
+        S.off = U.nbk;
+        U.nbk += S.nbreak;
+    }
+    for (int a = 0; a < nat; ++a)
+        OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "atomic_density: species index out of range");
+    U.b_sp = sizeof(PlaceSpecies) * nspecies;
+    U.b_x = sizeof(double) * U.nbk;
+    U.b_c = sizeof(double) * 4 * (U.nbk - nspecies);
+    U.b_f = sizeof(double) * 3 * (size_t)nat;
+    U.b_s = sizeof(int) * (size_t)nat;
+    U.b_l = sizeof(double) * 9;
+    U.o_x = align256(U.b_sp);
+    U.o_c = U.o_x + align256(U.b_x);
+    U.o_f = U.o_c + align256(U.b_c);
+    U.o_s = U.o_f + align256(U.b_f);
+    U.o_l = U.o_s + align256(U.b_s);
+    U.total = U.o_l + align256(U.b_l);
+}
+
+static void place_launch(const Plan *P, int kind, int nat, const PlaceUpload &U, const char *d, double *rho,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_place_atoms, dim3(nat), dim3(256), 0, s, rho, kind, nat,
+                       reinterpret_cast<const double *>(d + U.o_f), reinterpret_cast<const int *>(d + U.o_s),
+                       reinterpret_cast<const PlaceSpecies *>(d), reinterpret_cast<const double *>(d + U.o_x),
+                       reinterpret_cast<const double *>(d + U.o_c), P->nr[0], P->nr[1], P->nr[2],
+                       reinterpret_cast<const double *>(d + U.o_l));
+    OFDFT_HIP(hipGetLastError());
+}
+
+// Device memory of ofdft_atomic_density_noalloc: its upload lives here
+// instead of in a per-call allocation.  Each GPU holds its own copy until the
+// process ends; every call on a GPU reuses that GPU's copy.
+constexpr size_t kPlaceScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) char scratch[kPlaceScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+
+// device address of `scratch` on the current device, looked up once per
+// device (the symbol's host-side name is no device address)
+static char *place_scratch() {
+    static std::mutex mtx;
+    static char *addr[kMaxScratchDevices] = {};
+    int dev = -1;
+    OFDFT_HIP(hipGetDevice(&dev));
+    OFDFT_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, "atomic_density_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(mtx);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        OFDFT_REQUIRE(e == hipSuccess && p != nullptr,
+                      std::string("atomic_density_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+        addr[dev] = reinterpret_cast<char *>(p);
+    }
+    return addr[dev];
+}
+
 }  // namespace ofdft
 
 using namespace ofdft;
@@ -132,78 +232,70 @@ extern "C" int ofdft_atomic_density(ofdft_plan *plan, int kind, int nspecies, co
         }
     } buf;
     try {
-        OFDFT_REQUIRE(P && rho && rcut && scale && pos_frac && species, "atomic_density: bad arguments");
-        OFDFT_REQUIRE(P->prec == OFDFT_FP64, "atomic_density: needs an FP64 plan");
-        OFDFT_REQUIRE(kind == 0 || kind == 1, "atomic_density: kind is 0 (radial spline) or 1 (Gaussian)");
-        OFDFT_REQUIRE(nspecies >= 1 && nspecies <= kMaxPlaceSpecies, "atomic_density: 1..16 species");
-        OFDFT_REQUIRE(nat >= 0, "atomic_density: nat < 0");
-        if (kind == 0) OFDFT_REQUIRE(nbreak && breaks && coef, "atomic_density: spline tables missing");
-        if (kind == 1) OFDFT_REQUIRE(sigma, "atomic_density: sigma missing");
+        PlaceUpload U;
+        place_prepare(P, kind, nspecies, nbreak, breaks, coef, rcut, rtol, sigma, scale, nat, pos_frac, species, rho,
+                      U);
         if (nat == 0) return 0;
         hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-        const int nr[3] = {P->nr[0], P->nr[1], P->nr[2]};
-        std::vector<PlaceSpecies> sp(nspecies);
-        long long nbk = 0;
-        for (int k = 0; k < nspecies; ++k) {
-            PlaceSpecies &S = sp[k];
-            S.rcut = rcut[k];
-            S.rtol = rtol ? rtol[k] : 0.0;
-            S.sigma = kind == 1 ? sigma[k] : 1.0;
-            S.scale = scale[k];
-            OFDFT_REQUIRE(S.rcut > 0.0, "atomic_density: rcut must be positive");
-            if (kind == 1) OFDFT_REQUIRE(S.sigma > 0.0, "atomic_density: sigma must be positive");
-            for (int d = 0; d < 3; ++d) {
-                // border = min(int(rcut / spacing) + 1, nr // 2), spacing = |a_d| / nr_d
-                const double len = std::sqrt(P->lat[3 * d] * P->lat[3 * d] + P->lat[3 * d + 1] * P->lat[3 * d + 1] +
-                                             P->lat[3 * d + 2] * P->lat[3 * d + 2]);
-                const int b = (int)(S.rcut / (len / nr[d])) + 1;
-                S.border[d] = std::min(b, nr[d] / 2);
-            }
-            S.nbreak = kind == 0 ? nbreak[k] : 2;
-            if (kind == 0) OFDFT_REQUIRE(S.nbreak >= 2 && breaks[k] && coef[k], "atomic_density: bad spline table");
-            S.off = nbk;
-            nbk += S.nbreak;
-        }
-        for (int a = 0; a < nat; ++a)
-            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "atomic_density: species index out of range");
-        // one upload: species records, breakpoints, coefficients, positions, species, lattice
-        const size_t b_sp = sizeof(PlaceSpecies) * nspecies, b_x = sizeof(double) * nbk,
-                     b_c = sizeof(double) * 4 * (nbk - nspecies), b_f = sizeof(double) * 3 * nat,
-                     b_s = sizeof(int) * nat, b_l = sizeof(double) * 9;
-        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-        const size_t total = al(b_sp) + al(b_x) + al(b_c) + al(b_f) + al(b_s) + al(b_l);
-        std::vector<char> h(total, 0);
+        // one upload from one host image of the layout
+        std::vector<char> h(U.total, 0);
         char *o = h.data();
-        std::memcpy(o, sp.data(), b_sp);
-        size_t off = al(b_sp);
-        const size_t o_x = off;
-        off += al(b_x);
-        const size_t o_c = off;
-        off += al(b_c);
+        std::memcpy(o, U.sp, U.b_sp);
         if (kind == 0)
             for (int k = 0; k < nspecies; ++k) {
-                std::memcpy(o + o_x + sizeof(double) * sp[k].off, breaks[k], sizeof(double) * sp[k].nbreak);
-                std::memcpy(o + o_c + sizeof(double) * 4 * (sp[k].off - k), coef[k],
-                            sizeof(double) * 4 * (sp[k].nbreak - 1));
+                std::memcpy(o + U.o_x + sizeof(double) * U.sp[k].off, breaks[k], sizeof(double) * U.sp[k].nbreak);
+                std::memcpy(o + U.o_c + sizeof(double) * 4 * (U.sp[k].off - k), coef[k],
+                            sizeof(double) * 4 * (U.sp[k].nbreak - 1));
             }
-        const size_t o_f = off;
-        std::memcpy(o + o_f, pos_frac, b_f);
-        off += al(b_f);
-        const size_t o_s = off;
-        std::memcpy(o + o_s, species, b_s);
-        off += al(b_s);
-        const size_t o_l = off;
-        std::memcpy(o + o_l, P->lat, b_l);
-        OFDFT_HIP(hipMalloc(&buf.p, total));
+        std::memcpy(o + U.o_f, pos_frac, U.b_f);
+        std::memcpy(o + U.o_s, species, U.b_s);
+        std::memcpy(o + U.o_l, P->lat, U.b_l);
+        OFDFT_HIP(hipMalloc(&buf.p, U.total));
         char *d = reinterpret_cast<char *>(buf.p);
-        OFDFT_HIP(hipMemcpyAsync(d, h.data(), total, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_place_atoms, dim3(nat), dim3(256), 0, s, reinterpret_cast<double *>(rho), kind, nat,
-                           reinterpret_cast<const double *>(d + o_f), reinterpret_cast<const int *>(d + o_s),
-                           reinterpret_cast<const PlaceSpecies *>(d), reinterpret_cast<const double *>(d + o_x),
-                           reinterpret_cast<const double *>(d + o_c), nr[0], nr[1], nr[2],
-                           reinterpret_cast<const double *>(d + o_l));
-        OFDFT_HIP(hipGetLastError());
+        OFDFT_HIP(hipMemcpyAsync(d, h.data(), U.total, hipMemcpyHostToDevice, s));
+        place_launch(P, kind, nat, U, d, reinterpret_cast<double *>(rho), s);
         // the upload buffer is freed (after the stream drains) when buf goes out of scope
+        OFDFT_HIP(hipStreamSynchronize(s));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+
+extern "C" int ofdft_atomic_density_noalloc(ofdft_plan *plan, int kind, int nspecies, const int *nbreak,
+                                            const double *const *breaks, const double *const *coef,
+                                            const double *rcut, const double *rtol, const double *sigma,
+                                            const double *scale, int nat, const double *pos_frac, const int *species,
+                                            void *rho, void *stream) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    try {
+        PlaceUpload U;
+        place_prepare(P, kind, nspecies, nbreak, breaks, coef, rcut, rtol, sigma, scale, nat, pos_frac, species, rho,
+                      U);
+        OFDFT_REQUIRE(U.total <= kPlaceScratchBytes,
+                      "atomic_density_noalloc: tables and positions need " + std::to_string(U.total) +
+                          " bytes, more than the " + std::to_string(kPlaceScratchBytes) +
+                          " bytes of its static device buffer; use ofdft_atomic_density");
+        if (nat == 0) return 0;
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        char *d = place_scratch();
+        // the parts go straight from the caller's arrays into the static
+        // buffer (every byte the kernel reads is written first, so nothing is
+        // cleared); the host waits below, so the arrays may be released after
+        // the call
+        OFDFT_HIP(hipMemcpyAsync(d, U.sp, U.b_sp, hipMemcpyHostToDevice, s));
+        if (kind == 0)
+            for (int k = 0; k < nspecies; ++k) {
+                OFDFT_HIP(hipMemcpyAsync(d + U.o_x + sizeof(double) * U.sp[k].off, breaks[k],
+                                         sizeof(double) * U.sp[k].nbreak, hipMemcpyHostToDevice, s));
+                OFDFT_HIP(hipMemcpyAsync(d + U.o_c + sizeof(double) * 4 * (U.sp[k].off - k), coef[k],
+                                         sizeof(double) * 4 * (U.sp[k].nbreak - 1), hipMemcpyHostToDevice, s));
+            }
+        OFDFT_HIP(hipMemcpyAsync(d + U.o_f, pos_frac, U.b_f, hipMemcpyHostToDevice, s));
+        OFDFT_HIP(hipMemcpyAsync(d + U.o_s, species, U.b_s, hipMemcpyHostToDevice, s));
+        OFDFT_HIP(hipMemcpyAsync(d + U.o_l, P->lat, U.b_l, hipMemcpyHostToDevice, s));
+        place_launch(P, kind, nat, U, d, reinterpret_cast<double *>(rho), s);
         OFDFT_HIP(hipStreamSynchronize(s));
         return 0;
     } catch (const std::exception &e) {

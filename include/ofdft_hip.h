@@ -244,6 +244,20 @@ int ofdft_atomic_density(ofdft_plan *plan, int kind, int nspecies, const int *nb
                          const double *scale, int nat, const double *pos_frac, const int *species, void *rho,
                          void *stream);
 int ofdft_clamp_normalize(ofdft_plan *plan, void *rho, int clamp, double dtol, double ncharge, void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): the same placement
+ * as ofdft_atomic_density, same arguments and result, but the upload of the
+ * species records, spline tables, positions and lattice goes into a static
+ * device buffer of 4 MiB (one per GPU, held until the process ends) instead
+ * of a per-call allocation; no hipMalloc/hipFree.  Inputs whose upload needs
+ * more than 4 MiB (about 150,000 atoms) are rejected with an error.
+ * Every call on one GPU shares that buffer: calls that overlap on one GPU
+ * (other threads or streams) race; each call returns after its placement. */
+int ofdft_atomic_density_noalloc(ofdft_plan *plan, int kind, int nspecies, const int *nbreak,
+                                 const double *const *breaks, const double *const *coef, const double *rcut,
+                                 const double *rtol, const double *sigma, const double *scale, int nat,
+                                 const double *pos_frac, const int *species, void *rho, void *stream);
 /* Preconditioner of the TN inner conjugate-gradient solve (no counterpart in
  * the reference, whose get_direction_TN, optimization.py:91-147, runs plain
  * CG).  table (host, n doubles): M^-1(q) at q = i dq, linearly interpolated
