@@ -69,6 +69,11 @@ SIGNATURES = {
     "ofdft_local_pp_stress": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
                                    _D, _DP, _P]),
     "ofdft_ewald_forces_stress": (_I, [_P, _I, _DP, _DP, _DP, _D, _DP, _DP, _P]),
+    "ofdft_local_pp_forces_pme": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
+                                       _I, _DP, _P]),
+    "ofdft_local_pp_stress_pme": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
+                                       _I, _D, _DP, _P]),
+    "ofdft_ewald_forces_stress_pme": (_I, [_P, _I, _DP, _DP, _DP, _D, _I, _DP, _DP, _P]),
     "ofdft_functional_stress": (_I, [_P, _P, _DP, _DP, _P]),
     "ofdft_dcsrch": (_I, [_DP, _D, _D, _D, _D, _D, _D, _D, _IP, _DP]),
     "ofdft_cn_step": (_I, [_P, _P, _P, _D, _D, _D, _I, _P, _DP, _P]),

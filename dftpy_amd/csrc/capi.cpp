@@ -30,6 +30,16 @@ void pp_stress(Plan &P, const double *rho, int nspecies, const double *q0, const
                hipStream_t s);
 void ewald_fs(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges,
               double precision, double *F, double *S, hipStream_t s);
+void pp_forces_pme(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                   const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+                   const double *pos_cart, const double *pos_frac, const int *species, int order, double *F,
+                   hipStream_t s);
+void pp_stress_pme(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                   const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+                   const double *pos_cart, const double *pos_frac, const int *species, int order, double energy,
+                   double *S, hipStream_t s);
+void ewald_fs_pme(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges,
+                  double precision, int order, double *F, double *S, hipStream_t s);
 void functional_stress(Plan &P, const double *rho, const double *energies, double *out, hipStream_t s);
 void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges, int pme,
            int order, double precision, double *out, hipStream_t s);
@@ -472,6 +482,46 @@ int ofdft_local_pp_stress(ofdft_plan *P, const void *rho, int nspecies, const do
             OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "local_pp_stress: species index out of range");
         pp_stress(*P, (const double *)rho, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac,
                   species, energy, stress, S(stream));
+    });
+}
+
+int ofdft_local_pp_forces_pme(ofdft_plan *P, const void *rho, int nspecies, const double *q0, const double *dq,
+                              const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                              const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                              const int *species, int order, double *forces, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho && forces && nat >= 1 && nspecies >= 1, "local_pp_forces_pme: bad arguments");
+        for (int a = 0; a < nat; ++a)
+            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies,
+                          "local_pp_forces_pme: species index out of range");
+        for (int k = 0; k < 3 * nat; ++k) forces[k] = 0.0;
+        pp_forces_pme(*P, (const double *)rho, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart,
+                      pos_frac, species, order, forces, S(stream));
+    });
+}
+
+int ofdft_local_pp_stress_pme(ofdft_plan *P, const void *rho, int nspecies, const double *q0, const double *dq,
+                              const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                              const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                              const int *species, int order, double energy, double *stress, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && rho && stress && nat >= 1 && nspecies >= 1, "local_pp_stress_pme: bad arguments");
+        for (int a = 0; a < nat; ++a)
+            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies,
+                          "local_pp_stress_pme: species index out of range");
+        pp_stress_pme(*P, (const double *)rho, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart,
+                      pos_frac, species, order, energy, stress, S(stream));
+    });
+}
+
+int ofdft_ewald_forces_stress_pme(ofdft_plan *P, int nat, const double *pos_cart, const double *pos_frac,
+                                  const double *charges, double precision, int order, double *forces, double *stress,
+                                  void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && nat >= 1 && (forces || stress), "ewald_forces_stress_pme: bad arguments");
+        if (forces)
+            for (int k = 0; k < 3 * nat; ++k) forces[k] = 0.0;
+        ewald_fs_pme(*P, nat, pos_cart, pos_frac, charges, precision, order, forces, stress, S(stream));
     });
 }
 

@@ -297,6 +297,11 @@ __global__ void k_ewald_real(int nat, const double *__restrict__ pos, const doub
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
+__global__ void k_fill_zero(double *__restrict__ x, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
 // ------------------------------------------------------------------ host
 static GGeom geom_of(const Plan &P) {
     GGeom g;
@@ -1161,6 +1166,294 @@ void ewald_fs(Plan &P, int nat, const double *pos_cart, const double *pos_frac, 
         double qsum = 0.0;
         for (int a = 0; a < nat; ++a) qsum += charges[a];
         for (int d = 0; d < 3; ++d) S[4 * d] += qsum * qsum * 4.0 * M_PI * (1.0 / (4.0 * ep.eta * P.volume * P.volume) / 2.0);
+    }
+}
+
+// ------------------------------------------------------------------ PME forms
+// Forces and stress with PME structure factors (pseudo/__init__.py:475-563,
+// ewald.py:810-864,915-960).  The stress is the exact form with each
+// structure factor replaced by B(G) rfftn(Q)(G); the forces contract the
+// derivative of each atom's B-spline stencil with a smoothed field.
+
+// S *= B (dense half grid)
+__global__ void k_bmul(double2 *__restrict__ S, long long nc, GGeom g, BTab b) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, g.ny, g.nzc, ix, iy, iz);
+        const double2 bb = cmul2(cmul2(b.bx[ix], b.by[iy]), b.bz[iz]);
+        S[i] = cmul2(S[i], bb);
+    }
+}
+
+// X = conj(B) vloc_s(|G|) rhoG (mode 0, local PP) or |B|^2 exp(-G^2/4eta)/G^2 SQ
+// (mode 1, Ewald); X(0) = 0
+__global__ void k_pme_x(long long nc, GGeom g, VlocTable t, int s, const double2 *__restrict__ in, double eta, BTab b,
+                        int mode, double2 *__restrict__ X) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        half_index(i, g.ny, g.nzc, ix, iy, iz);
+        if (ix == 0 && iy == 0 && iz == 0) {
+            X[i] = double2{0.0, 0.0};
+            continue;
+        }
+        int f[3], fp[3];
+        freqs_of(g, ix, iy, iz, f, fp);
+        double G[3];
+        gvec(g, f[0], f[1], f[2], G);
+        const double gg = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+        const double2 bb = cmul2(cmul2(b.bx[ix], b.by[iy]), b.bz[iz]);
+        const double2 v = in[i];
+        if (mode == 0) {
+            const double w = vloc_eval(t, s, sqrt(gg));
+            const double2 cb{bb.x, -bb.y};
+            const double2 r = cmul2(cb, v);
+            X[i] = double2{w * r.x, w * r.y};
+        } else {
+            const double w = (bb.x * bb.x + bb.y * bb.y) * exp(-gg / (4.0 * eta)) / gg;
+            X[i] = double2{w * v.x, w * v.y};
+        }
+    }
+}
+
+// F_a = -scale_a (sum_l grad_frac Q_a(l) field(l)) . Ainv (the reference's
+// matmul(Q_derivative^T, cell_inv)); one workgroup per atom alist[blockIdx.x]
+__global__ void k_pme_force(const double *__restrict__ field, int nx, int ny, int nz, int order,
+                            const double *__restrict__ frac, const int *__restrict__ alist,
+                            const double *__restrict__ scale, GGeom ainv, double *__restrict__ F) {
+    __shared__ double mv[3][kMaxOrder + 1], dv[3][kMaxOrder + 1];
+    __shared__ int fl[3];
+    __shared__ double sm[16];
+    const int a = alist[blockIdx.x];
+    const int nr[3] = {nx, ny, nz};
+    if (threadIdx.x < 3) {
+        const int d = threadIdx.x;
+        const double u = frac[3 * a + d] * nr[d];
+        const double f = floor(u);
+        fl[d] = (int)f;
+        double m[kMaxOrder + 1], m2[kMaxOrder + 1];
+        calc_mn(u - f, order, m);
+        calc_mn(u - f, order - 1, m2);
+        m2[order] = 0.0;
+        for (int k = 0; k < order; ++k) {
+            mv[d][k] = m[k + 1];
+            dv[d][k] = nr[d] * (m2[k + 1] - m2[k]);
+        }
+    }
+    __syncthreads();
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int o3 = order * order * order;
+    for (int e = threadIdx.x; e < o3; e += blockDim.x) {
+        const int i = e / (order * order), j = (e / order) % order, k = e % order;
+        int lx = (1 + fl[0] - i) % nx;
+        if (lx < 0) lx += nx;
+        int ly = (1 + fl[1] - j) % ny;
+        if (ly < 0) ly += ny;
+        int lz = (1 + fl[2] - k) % nz;
+        if (lz < 0) lz += nz;
+        const double v = field[((long long)lx * ny + ly) * nz + lz];
+        acc[0] += dv[0][i] * mv[1][j] * mv[2][k] * v;
+        acc[1] += mv[0][i] * dv[1][j] * mv[2][k] * v;
+        acc[2] += mv[0][i] * mv[1][j] * dv[2][k] * v;
+    }
+    double t[3];
+    for (int d = 0; d < 3; ++d) t[d] = block_sum(acc[d], sm);
+    if (threadIdx.x == 0) {
+        const double sc = scale ? scale[a] : 1.0;
+        for (int k = 0; k < 3; ++k)
+            F[3 * a + k] = -sc * (t[0] * ainv.B[k] + t[1] * ainv.B[3 + k] + t[2] * ainv.B[6 + k]);
+    }
+}
+
+// inverse of the lattice matrix (rows = lattice vectors), row-major in B[9]
+static GGeom inverse_lattice(const Plan &P) {
+    const double *A = P.lat;
+    const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                       A[2] * (A[3] * A[7] - A[4] * A[6]);
+    GGeom g = geom_of(P);
+    g.B[0] = (A[4] * A[8] - A[5] * A[7]) / det;
+    g.B[1] = (A[2] * A[7] - A[1] * A[8]) / det;
+    g.B[2] = (A[1] * A[5] - A[2] * A[4]) / det;
+    g.B[3] = (A[5] * A[6] - A[3] * A[8]) / det;
+    g.B[4] = (A[0] * A[8] - A[2] * A[6]) / det;
+    g.B[5] = (A[2] * A[3] - A[0] * A[5]) / det;
+    g.B[6] = (A[3] * A[7] - A[4] * A[6]) / det;
+    g.B[7] = (A[1] * A[6] - A[0] * A[7]) / det;
+    g.B[8] = (A[0] * A[4] - A[1] * A[3]) / det;
+    return g;
+}
+
+// field = irfftn(X) into `field` (N doubles), X on the dense half grid
+static void pme_field(Plan &P, double2 *X, double *field, hipStream_t s) {
+    fft_c2r<double>(P, X, X, field, 1, P.Nc, P.Nc, P.N, 1.0 / (double)P.N, s);
+}
+
+// F (host, nat x 3) += local-PP forces, PME form (pseudo/__init__.py:475-520)
+void pp_forces_pme(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                   const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+                   const double *pos_cart, const double *pos_frac, const int *species, int order, double *F,
+                   hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "forces need an FP64 plan");
+    OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
+    std::vector<int> alist;
+    std::vector<int> cnt(nspecies, 0);
+    for (int sp = 0; sp < nspecies; ++sp)
+        for (int a = 0; a < nat; ++a)
+            if (species[a] == sp) {
+                alist.push_back(a);
+                ++cnt[sp];
+            }
+    IonInputs in = stage_ions(P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species,
+                              {}, alist, s);
+    const long long nc = P.Nc;
+    double *ws = fs_workspace(P, sizeof(double) * (2 * nc * 2 + P.N + (size_t)3 * nat));
+    double2 *rhoG = reinterpret_cast<double2 *>(ws);
+    double2 *X = rhoG + nc;
+    double *field = reinterpret_cast<double *>(X + nc);
+    double *Fd = field + P.N;
+    const BTab bt = pme_tab(P, order);
+    const GGeom g = geom_of(P);
+    const GGeom ai = inverse_lattice(P);
+    fft_r2c<double>(P, rho, rhoG, 1, P.N, nc, 1.0, nullptr, nullptr, s);
+    int off = 0;
+    for (int sp = 0; sp < nspecies; ++sp) {
+        if (cnt[sp] == 0) continue;
+        hipLaunchKernelGGL(k_pme_x, dim3(nblk(nc)), dim3(256), 0, s, nc, g, in.t, sp, (const double2 *)rhoG, 0.0, bt,
+                           0, X);
+        OFDFT_HIP(hipGetLastError());
+        pme_field(P, X, field, s);
+        hipLaunchKernelGGL(k_pme_force, dim3(cnt[sp]), dim3(256), 0, s, (const double *)field, P.nr[0], P.nr[1],
+                           P.nr[2], order, in.frac, in.alist + off, (const double *)nullptr, ai, Fd);
+        OFDFT_HIP(hipGetLastError());
+        off += cnt[sp];
+    }
+    std::vector<double> h((size_t)3 * nat);
+    OFDFT_HIP(hipMemcpyAsync(h.data(), Fd, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OFDFT_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < 3 * nat; ++k) F[k] += h[k];
+}
+
+// S (host, 3 x 3) = local-PP stress, PME form (pseudo/__init__.py:522-563)
+void pp_stress_pme(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                   const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
+                   const double *pos_cart, const double *pos_frac, const int *species, int order, double energy,
+                   double *S, hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "stress needs an FP64 plan");
+    OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
+    IonInputs in = stage_ions(P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species,
+                              {}, {}, s);
+    const long long nc = P.Nc;
+    const int nb = 1024;
+    double *ws = fs_workspace(P, sizeof(double) * ((1 + nspecies) * nc * 2 + P.N + nb * 24));
+    double2 *rhoG = reinterpret_cast<double2 *>(ws);
+    double2 *Ss = rhoG + nc;
+    double *Q = reinterpret_cast<double *>(Ss + (size_t)nspecies * nc);
+    double *part = Q + P.N;
+    const BTab bt = pme_tab(P, order);
+    const GGeom g = geom_of(P);
+    fft_r2c<double>(P, rho, rhoG, 1, P.N, nc, P.dV, nullptr, nullptr, s);
+    for (int sp = 0; sp < nspecies; ++sp) {
+        hipLaunchKernelGGL(k_fill_zero, dim3(nblk(P.N)), dim3(256), 0, s, Q, P.N);
+        hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat, in.frac,
+                           (const double *)nullptr, in.spec, sp);
+        OFDFT_HIP(hipGetLastError());
+        fft_r2c<double>(P, Q, Ss + (size_t)sp * nc, 1, P.N, nc, 1.0, nullptr, nullptr, s);
+        hipLaunchKernelGGL(k_bmul, dim3(nblk(nc)), dim3(256), 0, s, Ss + (size_t)sp * nc, nc, g, bt);
+        OFDFT_HIP(hipGetLastError());
+    }
+    StressGIn si;
+    std::memset(&si, 0, sizeof(si));
+    si.rhoG = rhoG;
+    si.Ss = Ss;
+    si.nsp = nspecies;
+    si.t = in.t;
+    hipLaunchKernelGGL(k_stress_g, dim3(nb), dim3(256), 0, s, nc, g, si, part);
+    OFDFT_HIP(hipGetLastError());
+    const std::vector<double> v = reduce_cols(part, nb, 24, s);
+    for (int k = 0; k < 9; ++k) S[k] = 0.0;
+    sym_add(S, v.data() + 6, -2.0 / (P.volume * P.volume));
+    for (int d = 0; d < 3; ++d) S[4 * d] -= energy / P.volume;
+}
+
+// Ewald forces and stress with the PME reciprocal forms (ewald.py:810-864,
+// 915-960) and the real-space sums of ewald_fs
+void ewald_fs_pme(Plan &P, int nat, const double *pos_cart, const double *pos_frac, const double *charges,
+                  double precision, int order, double *F, double *S, hipStream_t s) {
+    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "Ewald forces need an FP64 plan");
+    OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
+    const EwaldParams ep = ewald_params(P, nat, charges, precision);
+    std::vector<double> w(charges, charges + nat);
+    std::vector<int> alist(nat);
+    for (int a = 0; a < nat; ++a) alist[a] = a;
+    IonInputs in = stage_ions(P, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nat, pos_cart,
+                              pos_frac, nullptr, w, alist, s);
+    const long long nc = P.Nc;
+    const int nb = 1024;
+    double *ws = fs_workspace(P, sizeof(double) * (2 * nc * 2 + P.N + (size_t)3 * nat + (size_t)nb * 24 +
+                                                   (size_t)nat * 9 + nat));
+    double2 *SZ = reinterpret_cast<double2 *>(ws);
+    double2 *X = SZ + nc;
+    double *Q = reinterpret_cast<double *>(X + nc);
+    double *Fd = Q + P.N;
+    double *part = Fd + 3 * nat;
+    double *rpart = part + (size_t)nb * 24;
+    double *qs = rpart + (size_t)nat * 9;
+    const BTab bt = pme_tab(P, order);
+    const GGeom g = geom_of(P);
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblk(P.N)), dim3(256), 0, s, Q, P.N);
+    hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat, in.frac,
+                       in.w, (const int *)nullptr, -1);
+    OFDFT_HIP(hipGetLastError());
+    fft_r2c<double>(P, Q, SZ, 1, P.N, nc, 1.0, nullptr, nullptr, s);  // rfftn(Q_Z)
+    // real space, as ewald_fs
+    GGeom lat = g;
+    for (int k = 0; k < 9; ++k) lat.B[k] = P.lat[k];
+    hipLaunchKernelGGL(k_ewald_real_fs, dim3(nat), dim3(256), 0, s, nat, in.pos, in.w, lat, ep.nimg[0], ep.nimg[1],
+                       ep.nimg[2], ep.rmax, ep.eta, rpart);
+    OFDFT_HIP(hipGetLastError());
+    std::vector<double> hr((size_t)nat * 9);
+    OFDFT_HIP(hipMemcpyAsync(hr.data(), rpart, hr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    OFDFT_HIP(hipStreamSynchronize(s));
+    if (F) {
+        for (int a = 0; a < nat; ++a)
+            for (int d = 0; d < 3; ++d) F[3 * a + d] += hr[(size_t)a * 9 + d];
+        // F_a = -q_a (4 pi / dV) stencil(irfftn(rfftn(Q) |B|^2 exp(-G^2/4eta)/G^2))
+        std::vector<double> sc(nat);
+        for (int a = 0; a < nat; ++a) sc[a] = charges[a] * 4.0 * M_PI / P.dV;
+        OFDFT_HIP(hipMemcpyAsync(qs, sc.data(), sizeof(double) * nat, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pme_x, dim3(nblk(nc)), dim3(256), 0, s, nc, g, VlocTable{}, 0, (const double2 *)SZ, ep.eta,
+                           bt, 1, X);
+        OFDFT_HIP(hipGetLastError());
+        pme_field(P, X, Q, s);
+        hipLaunchKernelGGL(k_pme_force, dim3(nat), dim3(256), 0, s, (const double *)Q, P.nr[0], P.nr[1], P.nr[2],
+                           order, in.frac, in.alist, (const double *)qs, inverse_lattice(P), Fd);
+        OFDFT_HIP(hipGetLastError());
+        std::vector<double> h((size_t)3 * nat);
+        OFDFT_HIP(hipMemcpyAsync(h.data(), Fd, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        OFDFT_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < 3 * nat; ++k) F[k] += h[k];
+    }
+    if (S) {
+        for (int k = 0; k < 9; ++k) S[k] = 0.0;
+        std::vector<double> r6(6, 0.0);
+        for (int a = 0; a < nat; ++a)
+            for (int k = 0; k < 6; ++k) r6[k] += hr[(size_t)a * 9 + 3 + k];
+        sym_add(S, r6.data(), -0.5 / P.volume);
+        hipLaunchKernelGGL(k_bmul, dim3(nblk(nc)), dim3(256), 0, s, SZ, nc, g, bt);  // B rfftn(Q_Z)
+        OFDFT_HIP(hipGetLastError());
+        StressGIn si;
+        std::memset(&si, 0, sizeof(si));
+        si.SZ = SZ;
+        si.eta = ep.eta;
+        hipLaunchKernelGGL(k_stress_g, dim3(nb), dim3(256), 0, s, nc, g, si, part);
+        OFDFT_HIP(hipGetLastError());
+        const std::vector<double> v = reduce_cols(part, nb, 24, s);
+        sym_add(S, v.data() + 18, 4.0 * M_PI / (P.volume * P.volume));
+        double qsum = 0.0;
+        for (int a = 0; a < nat; ++a) qsum += charges[a];
+        for (int d = 0; d < 3; ++d)
+            S[4 * d] += qsum * qsum * 4.0 * M_PI * (1.0 / (4.0 * ep.eta * P.volume * P.volume) / 2.0);
     }
 }
 

@@ -105,22 +105,30 @@ class Ewald:
         q = np.ascontiguousarray(self.ions.charges, dtype=np.float64)
         f = np.zeros((self.ions.nat, 3))
         st = np.zeros((3, 3))
-        _lib.check(_lib.lib.ofdft_ewald_forces_stress(plan.handle, self.ions.nat, _lib.dptr(pos), _lib.dptr(frac),
-                                                      _lib.dptr(q), float(self.precision), _lib.dptr(f),
-                                                      _lib.dptr(st), _lib.stream()))
+        if self.PME:
+            _lib.check(_lib.lib.ofdft_ewald_forces_stress_pme(plan.handle, self.ions.nat, _lib.dptr(pos),
+                                                              _lib.dptr(frac), _lib.dptr(q), float(self.precision),
+                                                              int(self.order), _lib.dptr(f), _lib.dptr(st),
+                                                              _lib.stream()))
+        else:
+            _lib.check(_lib.lib.ofdft_ewald_forces_stress(plan.handle, self.ions.nat, _lib.dptr(pos), _lib.dptr(frac),
+                                                          _lib.dptr(q), float(self.precision), _lib.dptr(f),
+                                                          _lib.dptr(st), _lib.stream()))
         self._fs = (f, st)
 
     @property
     def forces(self):
-        """Ewald forces (ewald.py:510-521): real space plus the exact
-        reciprocal sum, nat x 3 in Ha/bohr."""
+        """Ewald forces (ewald.py:510-521): real space plus the reciprocal
+        part, PME (Forces_rec_PME) or exact (Forces_rec) as the Ewald was
+        built, nat x 3 in Ha/bohr."""
         if getattr(self, "_fs", None) is None:
             self._forces_stress()
         return self._fs[0].copy()
 
     @property
     def stress(self):
-        """Ewald stress (ewald.py:523-539), 3 x 3 in Ha/bohr^3."""
+        """Ewald stress (ewald.py:523-539; Stress_rec_PME or Stress_rec),
+        3 x 3 in Ha/bohr^3."""
         if getattr(self, "_fs", None) is None:
             self._forces_stress()
         return self._fs[1].copy()
@@ -218,9 +226,9 @@ class LocalPseudo(AbstractFunctional):
         return out
 
     def forces(self, rho):
-        """LocalPseudo.forces (pseudo/__init__.py:260-270,456-470): nat x 3
-        in Ha/bohr, exact structure factors (the reference's PME form agrees
-        with them to 1e-4, test_pme.py:34-36)."""
+        """LocalPseudo.forces (pseudo/__init__.py:260-270): nat x 3 in
+        Ha/bohr, the PME form (_ForcePME, :475-520) when PME is set, as in
+        the reference, else exact structure factors (_Force, :456-470)."""
         from . import _fp64
         if not isinstance(rho, DirectField):
             raise TypeError("rho must be DirectField")
@@ -228,14 +236,20 @@ class LocalPseudo(AbstractFunctional):
         plan = r.grid.plan(torch.float64)
         keep, args = self._table_args()
         f = np.zeros((self.ions.nat, 3))
-        _lib.check(_lib.lib.ofdft_local_pp_forces(plan.handle, r.data.contiguous().data_ptr(), *args, _lib.dptr(f),
-                                                  _lib.stream()))
+        if self.PME:
+            _lib.check(_lib.lib.ofdft_local_pp_forces_pme(plan.handle, r.data.contiguous().data_ptr(), *args,
+                                                          int(self.BsplineOrder), _lib.dptr(f), _lib.stream()))
+        else:
+            _lib.check(_lib.lib.ofdft_local_pp_forces(plan.handle, r.data.contiguous().data_ptr(), *args,
+                                                      _lib.dptr(f), _lib.stream()))
         del keep
         return f
 
     def stress(self, rho, energy=None):
-        """LocalPseudo.stress (pseudo/__init__.py:248-258,414-454), 3 x 3 in
-        Ha/bohr^3; energy: the PP energy at rho (computed if None)."""
+        """LocalPseudo.stress (pseudo/__init__.py:248-258), 3 x 3 in
+        Ha/bohr^3: the PME form (_StressPME, :522-563) when PME is set, else
+        the exact one (_Stress, :414-454); energy: the PP energy at rho
+        (computed if None)."""
         from . import _fp64
         if not isinstance(rho, DirectField):
             raise TypeError("rho must be DirectField")
@@ -245,8 +259,13 @@ class LocalPseudo(AbstractFunctional):
         plan = r.grid.plan(torch.float64)
         keep, args = self._table_args()
         st = np.zeros((3, 3))
-        _lib.check(_lib.lib.ofdft_local_pp_stress(plan.handle, r.data.contiguous().data_ptr(), *args, float(energy),
-                                                  _lib.dptr(st), _lib.stream()))
+        if self.PME:
+            _lib.check(_lib.lib.ofdft_local_pp_stress_pme(plan.handle, r.data.contiguous().data_ptr(), *args,
+                                                          int(self.BsplineOrder), float(energy), _lib.dptr(st),
+                                                          _lib.stream()))
+        else:
+            _lib.check(_lib.lib.ofdft_local_pp_stress(plan.handle, r.data.contiguous().data_ptr(), *args,
+                                                      float(energy), _lib.dptr(st), _lib.stream()))
         del keep
         return st
 

@@ -192,8 +192,9 @@ int ofdft_ewald(ofdft_plan *plan, int nat, const double *pos_cart, const double 
                 void *stream);
 
 /* Forces and stress (FP64 plans; host outputs; synchronise stream).  G-space
- * sums run over the half-space mask with exact structure factors (the
- * reference's PME forms agree with its exact forms to 1e-4, test_pme.py).
+ * sums run over the half-space mask; these entry points use exact structure
+ * factors (the reference's PME=False forms), the *_pme ones below the PME
+ * forms.
  *
  * ofdft_local_pp_forces: LocalPseudo.forces (pseudo/__init__.py:456-470),
  *   F_a = (2/V) sum_mask G vloc_s(|G|) Im(rho(G) exp(iG.R_a)); table arguments
@@ -219,6 +220,28 @@ int ofdft_local_pp_stress(ofdft_plan *plan, const void *rho, int nspecies, const
                           const int *species, double energy, double *stress, void *stream);
 int ofdft_ewald_forces_stress(ofdft_plan *plan, int nat, const double *pos_cart, const double *pos_frac,
                               const double *charges, double precision, double *forces, double *stress, void *stream);
+/* PME forms of the same (the reference's defaults, PME=True): order-`order`
+ * cardinal B-splines on the plan's grid.
+ * ofdft_local_pp_forces_pme: LocalPseudo._ForcePME (pseudo/__init__.py:475-520),
+ *   per species the field irfftn(conj(B) vloc_s rho(G)) (G = 0 dropped)
+ *   contracted with each atom's stencil derivative as the reference does,
+ *   matmul(dQ/d(frac)^T, A^-1) (rows of A = lattice vectors).
+ * ofdft_local_pp_stress_pme: LocalPseudo._StressPME (pseudo/__init__.py:522-563):
+ *   the exact form with each species' structure factor B rfftn(Q_s).
+ * ofdft_ewald_forces_stress_pme: ewald.forces / ewald.stress with PME=True
+ *   (ewald.py:510-539): real-space sums plus Forces_rec_PME (810-864) and
+ *   Stress_rec_PME (915-960). */
+int ofdft_local_pp_forces_pme(ofdft_plan *plan, const void *rho, int nspecies, const double *q0, const double *dq,
+                              const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                              const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                              const int *species, int order, double *forces, void *stream);
+int ofdft_local_pp_stress_pme(ofdft_plan *plan, const void *rho, int nspecies, const double *q0, const double *dq,
+                              const int *ncoef, const double *const *coef, const double *qmax, const double *vsmall,
+                              const double *dqs, int nat, const double *pos_cart, const double *pos_frac,
+                              const int *species, int order, double energy, double *stress, void *stream);
+int ofdft_ewald_forces_stress_pme(ofdft_plan *plan, int nat, const double *pos_cart, const double *pos_frac,
+                                  const double *charges, double precision, int order, double *forces, double *stress,
+                                  void *stream);
 int ofdft_functional_stress(ofdft_plan *plan, const void *rho, const double *energies, double *out, void *stream);
 
 /* Start densities from atomic radial functions (density/density.py).
@@ -258,6 +281,7 @@ int ofdft_atomic_density_noalloc(ofdft_plan *plan, int kind, int nspecies, const
                                  const double *const *breaks, const double *const *coef, const double *rcut,
                                  const double *rtol, const double *sigma, const double *scale, int nat,
                                  const double *pos_frac, const int *species, void *rho, void *stream);
+
 /* Preconditioner of the TN inner conjugate-gradient solve (no counterpart in
  * the reference, whose get_direction_TN, optimization.py:91-147, runs plain
  * CG).  table (host, n doubles): M^-1(q) at q = i dq, linearly interpolated

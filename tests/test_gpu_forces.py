@@ -123,3 +123,21 @@ def test_pme_forms_agree_with_exact():
     assert np.isclose(e0.energy, e1.energy, atol=1e-5)
     assert np.allclose(e0.forces, e1.forces, atol=1e-5)
     assert np.allclose(e0.stress, e1.stress, atol=1e-5)
+
+
+def test_pme_forces_stress_vs_oracle():
+    """PME=True (the reference's default) forces and stress of the local PP
+    (pseudo/__init__.py:475-563) and the Ewald sum (ewald.py:810-960) on the
+    sheared 4-atom cell against the oracle's PME forms, which are pinned by
+    central differences of the PME energies."""
+    from dftpy_amd import systems
+    from dftpy_amd.functional import Functional
+    ions, grid, rho, rho_np, _, _, og, oat, opp = _system()
+    pp = Functional(type="PSEUDO", grid=grid, ions=ions, PP_list={"Al": systems.empty_core_recpot()}, PME=True)
+    e_pp = (OP.local_pp(og, oat, opp, pme=True) * rho_np).sum() * og.dV
+    _close(pp.forces(rho), FO.pp_forces_pme(rho_np, og, oat, opp), 1e-9)
+    _close(pp.stress(rho, energy=e_pp), FO.pp_stress_pme(rho_np, og, oat, opp, e_pp), 1e-9, 1e-13)
+    ew = pp.get_ewald()
+    assert ew.PME
+    _close(ew.forces, FO.ewald_forces_pme(og, oat), 1e-9)
+    _close(ew.stress, FO.ewald_stress_pme(og, oat), 1e-9, 1e-13)
