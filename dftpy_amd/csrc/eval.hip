@@ -19,6 +19,7 @@
 #include "gspace.h"
 #include "internal.h"
 #include "pointwise.h"
+#include "stream.h"
 #include "reduce.h"
 
 namespace ofdft {
@@ -201,30 +202,40 @@ template <> __device__ __forceinline__ float rcp_p<float>(float x) { return __bu
 template <typename CT> __device__ __forceinline__ CT rsq_p(CT x) { return (CT)1 / sqrt(x); }
 template <> __device__ __forceinline__ float rsq_p<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 
-template <typename T, typename CT>
+template <typename T, typename CT, int VEC>
 __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, const T *__restrict__ vloc,
                        T *__restrict__ pot, double *partials, PostParams p) {
-    __shared__ double sm[16];
     const CT ctf = (CT)(0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0));
     const CT cbrt34pi = (CT)cbrt(3.0 / (4.0 * M_PI));
     const CT cx_e = (CT)(-3.0 / 4.0 * cbrt(3.0 / M_PI));
     const CT cx_v = (CT)cbrt(-3.0 / M_PI);
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const CT r = (CT)rho[i];
-        CT v = (CT)0;
-        // one cube root and one square root serve every term (for beta = 5/6)
-        const CT cr = cbrt(r);
-        const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
-        const CT c_rc = r < (CT)1e-30 ? (CT)1e-10 : cr;  // cbrt(max(rho, 1e-30))
-        if (p.terms & OFDFT_TERM_TF) {
-            const CT c2 = cr * cr;  // rho^(2/3)
-            if (p.want_e) acc[0] += (double)(ctf * c2 * r);
-            v += (CT)p.x * ((CT)(5.0 / 3.0) * ctf * c2);
+    const bool xc_core = (p.terms & OFDFT_TERM_XC) && p.core;
+    pw_groups<VEC>(p.n, [&](long long i0) {
+        // this group's inputs, one vector access per field present
+        const PwVec<VEC, T> rv = pw_ld<VEC>(rho, i0);
+        PwVec<VEC, T> av{}, cv{}, c2v{}, hv{}, lv{}, corev{}, out{};
+        if (p.s_vw >= 0) av = pw_ld<VEC>(work + p.s_vw * p.bs, i0);
+        if (p.s_wt >= 0) cv = pw_ld<VEC>(work + p.s_wt * p.bs, i0);
+        if (p.s_wt >= 0 && p.s_wt2 >= 0) c2v = pw_ld<VEC>(work + p.s_wt2 * p.bs, i0);
+        if (p.s_h >= 0) hv = pw_ld<VEC>(work + p.s_h * p.bs, i0);
+        if (vloc) lv = pw_ld<VEC>(vloc, i0);
+        if (xc_core) corev = pw_ld<VEC>(reinterpret_cast<const T *>(p.core), i0);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const CT r = (CT)rv.v[k];
+            CT v = (CT)0;
+            // one cube root and one square root serve every term (for beta = 5/6)
+            const CT cr = cbrt(r);
+            const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
+            const CT c_rc = r < (CT)1e-30 ? (CT)1e-10 : cr;  // cbrt(max(rho, 1e-30))
+            if (p.terms & OFDFT_TERM_TF) {
+                const CT c2 = cr * cr;  // rho^(2/3)
+                if (p.want_e) acc[0] += (double)(ctf * c2 * r);
+                v += (CT)p.x * ((CT)(5.0 / 3.0) * ctf * c2);
         }
         if (p.s_vw >= 0) {
-            const CT a = (CT)work[p.s_vw * p.bs + i];
+            const CT a = (CT)av.v[k];
             // 1 / sqrt(rho > 0 ? rho : 1e-30)
             const CT isq = rsq_p<CT>((r > (CT)0 && r < (CT)1e-30) ? r : rc);
             const CT vw = (CT)p.y * (a * isq);
@@ -232,15 +243,15 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             v += vw;
         }
         if (p.s_wt >= 0 && p.s_wt2 < 0) {
-            const CT c = (CT)work[p.s_wt * p.bs + i];
+            const CT c = (CT)cv.v[k];
             const CT rb_rc = p.b56 ? c_rc * rsq_p<CT>(rc) : pow(rc, (CT)p.beta) * rcp_p<CT>(rc);  // rho^beta / rho
             const CT vnl = (CT)(2.0 * p.alpha) * rb_rc * c;
             if (p.want_e) acc[2] += (double)(vnl * r) / (2.0 * p.alpha);
             v += vnl;
         } else if (p.s_wt >= 0) {
             // alpha != beta (wt.py:23-25,44-56): cb = K * rho^beta, ca = K * rho^alpha
-            const CT cb = (CT)work[p.s_wt * p.bs + i];
-            const CT ca = (CT)work[p.s_wt2 * p.bs + i];
+            const CT cb = (CT)cv.v[k];
+            const CT ca = (CT)c2v.v[k];
             const CT ra = pow(rc, (CT)p.alpha), rb = pow(rc, (CT)p.beta);
             const CT irc = rcp_p<CT>(rc);
             if (p.want_e) acc[2] += (double)(ra * cb);
@@ -251,7 +262,7 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             // evaluated at rho + core (energy sum_r eps(rho+core)(rho+core) dV)
             CT rx = r, rc3 = cr;
             if (p.core) {
-                rx = r + (CT)reinterpret_cast<const T *>(p.core)[i];
+                rx = r + (CT)corev.v[k];
                 rc3 = cbrt(rx);
             }
             if (rc3 < (CT)1e-30) rc3 = (CT)1e-30;
@@ -273,24 +284,31 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             v += cx_v * rc3 + vc;
         }
         if (p.s_h >= 0) {
-            const CT vh = (CT)work[p.s_h * p.bs + i];
+            const CT vh = (CT)hv.v[k];
             if (p.want_e) acc[4] += 0.5 * (double)vh * (double)r;
             v += vh;
         }
         if (vloc) {
-            const CT vl = (CT)vloc[i];
+            const CT vl = (CT)lv.v[k];
             if (p.want_e) acc[5] += (double)vl * (double)r;
             v += vl;
         }
         acc[6] += (double)r;
-        if (p.want_v) pot[i] = (T)v;
-    }
-    if (p.want_e) {
-        for (int t = 0; t < 7; ++t) {
-            const double s = block_sum(acc[t], sm);
-            if (threadIdx.x == 0) partials[(long long)blockIdx.x * 7 + t] = s;
+        out.v[k] = (T)v;
         }
-    }
+        if (p.want_v) pw_st<VEC>(pot, i0, out);
+    });
+    if (p.want_e) block_sums<7>(acc, partials);
+}
+
+// k_post with the element width the field length allows (stream.h)
+template <typename T>
+static void launch_post(int nb, hipStream_t s, const T *rho, const T *work, const T *vloc, T *pot, double *partials,
+                        const PostParams &pp) {
+    if (pw_vec(pp.n) == 4 && pw_vec(pp.bs) == 4 && pw_aligned({rho, work, vloc, pot, pp.core}))
+        hipLaunchKernelGGL((k_post<T, T, 4>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, partials, pp);
+    else
+        hipLaunchKernelGGL((k_post<T, T, 1>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, partials, pp);
 }
 
 // energies[0..5] = term sums * dV (TF also * x), [6] = their sum + E_ion,
@@ -372,7 +390,7 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     pp.want_e = 1;
     const T *vloc = (P.terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
     const int nb = std::min(reduce_blocks(P.N), 1024);
-    hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, nullptr, vloc, nullptr, P.red, pp);
+    launch_post<T>(nb, s, rho, nullptr, vloc, nullptr, P.red, pp);
     OFDFT_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_gspace_energy<T>), dim3(nb), dim3(256), 0, s, spec, ns, gp, P.y, P.red);
     OFDFT_HIP(hipGetLastError());
@@ -479,7 +497,7 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
                                   (calc & OFDFT_CALC_E) && energies != nullptr, s_wt2);
     const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
     const int nb = std::min(reduce_blocks(P.N), 1024);
-    hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, P.red, pp);
+    launch_post<T>(nb, s, rho, work, vloc, pot, P.red, pp);
     OFDFT_HIP(hipGetLastError());
     if (pp.want_e) {
         hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
@@ -604,7 +622,7 @@ template <typename T> void bench_eval_stage(Plan &P, int stage, const T *rho, T 
         PostParams pp = pipeline_post(P, s_vw, s_wt, s_h, true, true);
         const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
         const int nb = std::min(reduce_blocks(P.N), 1024);
-        hipLaunchKernelGGL((k_post<T, T>), dim3(nb), dim3(256), 0, s, rho, reinterpret_cast<const T *>(P.work),
+        launch_post<T>(nb, s, rho, reinterpret_cast<const T *>(P.work),
                            vloc, pot, P.red, pp);
     }
     OFDFT_HIP(hipGetLastError());
@@ -664,37 +682,49 @@ void eval_linear(Plan &P, const T *rho, const T *phi, const T *p, hipStream_t s)
 
 // Ap = (V - mu) p + phi dV[2 phi p]  and  sum p.Ap.
 // base_a = 1/2 L(sqrt(rho)) and base_c = C(rho^beta) of the evaluation at rho.
-template <typename T, typename CT>
+template <typename T, typename CT, int VEC>
 __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
                       const T *__restrict__ p, const T *__restrict__ pot, const T *__restrict__ work,
                       const T *__restrict__ base_a, const T *__restrict__ base_c, const double *__restrict__ mu,
                       T *__restrict__ Ap, double *partials, PostParams pp) {
-    __shared__ double sm[16];
     const CT ctf = (CT)(0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0));
     const CT cbrt34pi = (CT)cbrt(3.0 / (4.0 * M_PI));
     const CT cx3 = (CT)(-cbrt(3.0 / M_PI) / 3.0);
     const double m = *mu;
-    double acc = 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < pp.n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const T phi_i = phi[i], p_i = p[i];
-        const CT r = (CT)rho[i], dr = (CT)(T)(2.0 * (double)phi_i * (double)p_i);  // drho = 2 phi p
-        const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
-        const CT c_rp = cbrt(rp);
-        const CT ic = rcp_p<CT>(c_rp);  // rho^(-1/3)
-        const CT irp = ic * ic * ic;    // 1 / rho
-        CT dv = (CT)0;
-        if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf * ic * dr;  // tf.py:38-40
+    double acc[1] = {0.0};
+    pw_groups<VEC>(pp.n, [&](long long i0) {
+        const PwVec<VEC, T> phv = pw_ld<VEC>(phi, i0), pv = pw_ld<VEC>(p, i0), rv = pw_ld<VEC>(rho, i0),
+                            potv = pw_ld<VEC>(pot, i0);
+        PwVec<VEC, T> bav{}, avw{}, bcv{}, cwt{}, hv{}, out{};
         if (pp.s_vw >= 0) {
-            const CT isq = rsq_p<CT>((r > (CT)0 && r < (CT)1e-30) ? r : rp);  // 1 / sqrt(rho > 0 ? rho : 1e-30)
-            const CT ds = (CT)0.5 * dr * isq;
-            const CT vwb = (CT)pp.y * (CT)base_a[i] * isq;
-            dv += ((CT)pp.y * (CT)work[pp.s_vw * pp.bs + i] - vwb * ds) * isq;
+            bav = pw_ld<VEC>(base_a, i0);
+            avw = pw_ld<VEC>(work + pp.s_vw * pp.bs, i0);
+        }
+        if (pp.s_wt >= 0) {
+            bcv = pw_ld<VEC>(base_c, i0);
+            cwt = pw_ld<VEC>(work + pp.s_wt * pp.bs, i0);
+        }
+        if (pp.s_h >= 0) hv = pw_ld<VEC>(work + pp.s_h * pp.bs, i0);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const T phi_i = phv.v[k], p_i = pv.v[k];
+            const CT r = (CT)rv.v[k], dr = (CT)(T)(2.0 * (double)phi_i * (double)p_i);  // drho = 2 phi p
+            const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
+            const CT c_rp = cbrt(rp);
+            const CT ic = rcp_p<CT>(c_rp);  // rho^(-1/3)
+            const CT irp = ic * ic * ic;    // 1 / rho
+            CT dv = (CT)0;
+            if (pp.terms & OFDFT_TERM_TF) dv += (CT)pp.x * (CT)(10.0 / 9.0) * ctf * ic * dr;  // tf.py:38-40
+            if (pp.s_vw >= 0) {
+                const CT isq = rsq_p<CT>((r > (CT)0 && r < (CT)1e-30) ? r : rp);  // 1 / sqrt(rho > 0 ? rho : 1e-30)
+                const CT ds = (CT)0.5 * dr * isq;
+                const CT vwb = (CT)pp.y * (CT)bav.v[k] * isq;
+                dv += ((CT)pp.y * (CT)avw.v[k] - vwb * ds) * isq;
         }
         if (pp.s_wt >= 0) {
             const CT rb1 = pp.b56 ? c_rp * rsq_p<CT>(rp) : pow(rp, (CT)pp.beta - (CT)1);  // rho^(beta-1)
-            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 * irp * dr * (CT)base_c[i] +
-                                           rb1 * (CT)work[pp.s_wt * pp.bs + i]);
+            dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 * irp * dr * (CT)bcv.v[k] +
+                                           rb1 * (CT)cwt.v[k]);
         }
         if (pp.terms & OFDFT_TERM_XC) {
             // semilocal_xc.py:291-303
@@ -707,17 +737,19 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
                 const CT ta = (CT)PZ_B1 * sqrt(rs), tb = (CT)PZ_B2 * rs;
                 const CT ide = rcp_p<CT>((CT)1 + ta + tb);
                 fc = (CT)(PZ_G / 36.0) *
-                     ((CT)5 * ta + (CT)7 * ta * ta + (CT)8 * tb + (CT)16 * tb * tb + (CT)21 * ta * tb) * (ide * ide * ide);
+                     ((CT)5 * ta + (CT)7 * ta * ta + (CT)8 * tb + (CT)16 * tb * tb + (CT)21 * ta * tb) *
+                     (ide * ide * ide);
             }
             dv += (cx3 * rc3 + fc) * irp * dr;
         }
-        if (pp.s_h >= 0) dv += (CT)work[pp.s_h * pp.bs + i];
-        const double a = ((double)pot[i] - m) * (double)p_i + (double)phi_i * (double)dv;
-        Ap[i] = (T)a;
-        acc += (double)p_i * (double)(T)a;
-    }
-    const double sum = block_sum(acc, sm);
-    if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+        if (pp.s_h >= 0) dv += (CT)hv.v[k];
+        const double a = ((double)potv.v[k] - m) * (double)p_i + (double)phi_i * (double)dv;
+        out.v[k] = (T)a;
+        acc[0] += (double)p_i * (double)(T)a;
+        }
+        pw_st<VEC>(Ap, i0, out);
+    });
+    block_sums<1>(acc, partials);
 }
 
 template <typename T>
@@ -746,8 +778,12 @@ void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T 
     pp.bs = P.N;
     pp.want_v = 1;
     pp.want_e = 0;
-    hipLaunchKernelGGL((k_hvp<T, T>), dim3(nb), dim3(256), 0, s, rho, phi, p, pot,
-                       reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
+    if (pw_vec(pp.n) == 4 && pw_aligned({rho, phi, p, pot, P.work, base_a, base_c, Ap}))
+        hipLaunchKernelGGL((k_hvp<T, T, 4>), dim3(nb), dim3(256), 0, s, rho, phi, p, pot,
+                           reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
+    else
+        hipLaunchKernelGGL((k_hvp<T, T, 1>), dim3(nb), dim3(256), 0, s, rho, phi, p, pot,
+                           reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
     OFDFT_HIP(hipGetLastError());
 }
 

@@ -15,6 +15,7 @@
 #include "gspace.h"
 #include "internal.h"
 #include "reduce.h"
+#include "stream.h"
 
 namespace ofdft {
 
@@ -36,57 +37,52 @@ void plan_aux_free(Plan &P) {
 
 template <typename T> static T *fld(Plan &P, int f) { return reinterpret_cast<T *>(P.aux) + (size_t)f * P.N; }
 
-template <typename T> __global__ void k_fill(T *y, T v, long long n);
 
 template <typename T>
 bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
                   long long vp, const void *tw, const GParams &gp, hipStream_t s);
 
 // ------------------------------------------------------------------ kernels
-#define GRID_LOOP(i, n) \
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (long long)gridDim.x * blockDim.x)
-
-template <int K> __device__ __forceinline__ void write_partials(double *acc, double *partials) {
-    __shared__ double sm[16];
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-        const double s = block_sum(acc[t], sm);
-        if (threadIdx.x == 0) partials[(long long)blockIdx.x * K + t] = s;
-    }
-}
+// Streaming kernels: template VEC (4 consecutive elements per thread when the
+// field length allows, else 1), see stream.h.  The reducing ones write one
+// row of FP64 partial sums per workgroup.
+#define PW_BODY(i) [&](long long i)
 
 // phi = rho > 0 ? sqrt(rho) : 1e-300 ; sums rho and phi^2 (optimization.py:305-315)
-template <typename T> __global__ void k_init_phi(const T *rho, T *phi, long long n, double *partials) {
+template <typename T, int VEC> __global__ void k_init_phi(const T *__restrict__ rho, T *__restrict__ phi,
+                                                          long long n, double *partials) {
     double acc[2] = {0, 0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T r = rho[i];
         const T f = r > (T)0 ? sqrt(r) : (T)1e-300;
         phi[i] = f;
         acc[0] += (double)r;
         acc[1] += (double)f * (double)f;
-    }
-    write_partials<2>(acc, partials);
+    });
+    block_sums<2>(acc, partials);
 }
 
 // phi *= sqrt(s[0] / s[1]) ; rho = phi^2 (the normalisation factor from
 // device sums, optimization.py:305-316)
-template <typename T> __global__ void k_scale_square(T *phi, T *rho, const double *s, long long n) {
+template <typename T, int VEC> __global__ void k_scale_square(T *__restrict__ phi, T *__restrict__ rho,
+                                                              const double *s, long long n) {
     const double f = sqrt(s[0] / s[1]);
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)phi[i] * f);
         phi[i] = v;
         rho[i] = v * v;
-    }
+    });
 }
 
 // sums  V*rho  and  rho  (mu = int V rho / N, optimization.py:537-539)
-template <typename T> __global__ void k_mu_sums(const T *pot, const T *rho, long long n, double *partials) {
+template <typename T, int VEC> __global__ void k_mu_sums(const T *__restrict__ pot, const T *__restrict__ rho,
+                                                         long long n, double *partials) {
     double acc[2] = {0, 0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         acc[0] += (double)pot[i] * (double)rho[i];
         acc[1] += (double)rho[i];
-    }
-    write_partials<2>(acc, partials);
+    });
+    block_sums<2>(acc, partials);
 }
 
 // scal[dst] = scal[a] / scal[b] (a device-side ratio, so the next kernel can
@@ -96,185 +92,199 @@ __global__ void k_ratio(double *scal, int dst, int a, int b) {
 }
 
 // res = (V - mu) phi ; sum res^2
-template <typename T>
-__global__ void k_residual(const T *pot, const T *phi, T *res, const double *mu, long long n, double *partials) {
+template <typename T, int VEC>
+__global__ void k_residual(const T *__restrict__ pot, const T *__restrict__ phi, T *__restrict__ res,
+                           const double *mu, long long n, double *partials) {
     const double m = *mu;
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T r = (T)(((double)pot[i] - m) * (double)phi[i]);
         res[i] = r;
         acc[0] += (double)r * (double)r;
-    }
-    write_partials<1>(acc, partials);
+    });
+    block_sums<1>(acc, partials);
 }
 
 // TN start: tres = -res, tp = tres, dir = 0 ; sum tres^2
-template <typename T>
-__global__ void k_tn_start(const T *res, T *tres, T *tp, T *dir, long long n, double *partials) {
+template <typename T, int VEC>
+__global__ void k_tn_start(const T *__restrict__ res, T *__restrict__ tres, T *__restrict__ tp, T *__restrict__ dir,
+                           long long n, double *partials) {
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T r = -res[i];
         tres[i] = r;
         tp[i] = r;
         dir[i] = (T)0;
         acc[0] += (double)r * (double)r;
-    }
-    write_partials<1>(acc, partials);
+    });
+    block_sums<1>(acc, partials);
 }
 
 // phi1 = phi + eps p ; rho1 = phi1^2
-template <typename T>
-__global__ void k_tn_shift(const T *phi, const T *p, double eps, T *phi1, T *rho1, long long n) {
-    GRID_LOOP(i, n) {
+template <typename T, int VEC>
+__global__ void k_tn_shift(const T *__restrict__ phi, const T *__restrict__ p, double eps, T *__restrict__ phi1,
+                           T *__restrict__ rho1, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)phi[i] + eps * (double)p[i]);
         phi1[i] = v;
         rho1[i] = v * v;
-    }
+    });
 }
 
 // Ap = ((V1 - mu) phi1 - res0) / eps ; sum p.Ap   (optimization.py:105-114)
-template <typename T>
-__global__ void k_tn_ap(const T *pot1, const T *phi1, const T *res0, const T *p, const double *mu, double eps, T *Ap,
+template <typename T, int VEC>
+__global__ void k_tn_ap(const T *__restrict__ pot1, const T *__restrict__ phi1, const T *__restrict__ res0,
+                        const T *__restrict__ p, const double *mu, double eps, T *__restrict__ Ap,
                         long long n, double *partials) {
     const double m = *mu;
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const double a = (((double)pot1[i] - m) * (double)phi1[i] - (double)res0[i]) / eps;
         Ap[i] = (T)a;
         acc[0] += (double)p[i] * (double)(T)a;
-    }
-    write_partials<1>(acc, partials);
+    });
+    block_sums<1>(acc, partials);
 }
 
 // dir += alpha p ; tres -= alpha Ap ; sum tres^2, with alpha = r0 / pAp taken
 // from the device (pAp reduced by the previous launch), so the inner CG needs one host round trip per iteration;
 // negative curvature (pAp < 0) leaves dir and tres untouched for the host.
-template <typename T>
-__global__ void k_tn_update_dev(T *dir, T *tres, const T *p, const T *Ap, const double *rzp, const double *pap,
+template <typename T, int VEC>
+__global__ void k_tn_update_dev(T *__restrict__ dir, T *__restrict__ tres, const T *__restrict__ p,
+                                const T *__restrict__ Ap, const double *rzp, const double *pap,
                                 long long n, double *partials) {
     const double pa = *pap, r0 = *rzp;
     double acc[1] = {0};
     if (!(pa < 0.0)) {
         const double alpha = r0 / pa;
-        GRID_LOOP(i, n) {
+        pw_loop<VEC>(n, PW_BODY(i) {
             dir[i] = (T)((double)dir[i] + alpha * (double)p[i]);
             const T r = (T)((double)tres[i] - alpha * (double)Ap[i]);
             tres[i] = r;
             acc[0] += (double)r * (double)r;
-        }
+        });
     }
-    write_partials<1>(acc, partials);
+    block_sums<1>(acc, partials);
 }
 
 // y = a x + b y
-template <typename T> __global__ void k_axpby(double a, const T *x, double b, T *y, long long n) {
-    GRID_LOOP(i, n) y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+template <typename T, int VEC> __global__ void k_axpby(double a, const T *__restrict__ x, double b,
+                                                       T *__restrict__ y, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) { y[i] = (T)(a * (double)x[i] + b * (double)y[i]); });
 }
 
-template <typename T> __global__ void k_copy(const T *x, T *y, long long n) {
-    GRID_LOOP(i, n) y[i] = x[i];
+template <typename T, int VEC> __global__ void k_copy(const T *__restrict__ x, T *__restrict__ y, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) { y[i] = x[i]; });
 }
 
-template <typename T> __global__ void k_fill(T *y, T v, long long n) {
-    GRID_LOOP(i, n) y[i] = v;
+template <typename T, int VEC> __global__ void k_fill(T *__restrict__ y, T v, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) { y[i] = v; });
 }
 
 // sums p*phi and phi*phi
-template <typename T> __global__ void k_pphi(const T *p, const T *phi, long long n, double *partials) {
+template <typename T, int VEC> __global__ void k_pphi(const T *__restrict__ p, const T *__restrict__ phi,
+                                                      long long n, double *partials) {
     double acc[2] = {0, 0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         acc[0] += (double)p[i] * (double)phi[i];
         acc[1] += (double)phi[i] * (double)phi[i];
-    }
-    write_partials<2>(acc, partials);
+    });
+    block_sums<2>(acc, partials);
 }
 
 // p -= f phi ; sum p^2
-template <typename T> __global__ void k_orth(T *p, const T *phi, const double *f, long long n, double *partials) {
+template <typename T, int VEC>
+__global__ void k_orth(T *__restrict__ p, const T *__restrict__ phi, const double *f, long long n, double *partials) {
     const double ff = *f;
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)p[i] - ff * (double)phi[i]);
         p[i] = v;
         acc[0] += (double)v * (double)v;
-    }
-    write_partials<1>(acc, partials);
+    });
+    block_sums<1>(acc, partials);
 }
 
 // sum V phi p
-template <typename T>
-__global__ void k_dot3(const T *a, const T *b, const T *c, long long n, double *partials) {
+template <typename T, int VEC>
+__global__ void k_dot3(const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c, long long n,
+                       double *partials) {
     double acc[1] = {0};
-    GRID_LOOP(i, n) acc[0] += (double)a[i] * (double)b[i] * (double)c[i];
-    write_partials<1>(acc, partials);
+    pw_loop<VEC>(n, PW_BODY(i) { acc[0] += (double)a[i] * (double)b[i] * (double)c[i]; });
+    block_sums<1>(acc, partials);
 }
 
 // newphi = phi cos + p sin ; newrho = newphi^2   (optimization.py:236-238)
-template <typename T>
-__global__ void k_rotate(const T *phi, const T *p, double c, double s, T *nphi, T *nrho, long long n) {
-    GRID_LOOP(i, n) {
+template <typename T, int VEC>
+__global__ void k_rotate(const T *__restrict__ phi, const T *__restrict__ p, double c, double s,
+                         T *__restrict__ nphi, T *__restrict__ nrho, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)phi[i] * c + (double)p[i] * s);
         nphi[i] = v;
         nrho[i] = v * v;
-    }
+    });
 }
 
 // sum V * newphi * (p cos - phi sin)   (optimization.py:270-276)
-template <typename T>
-__global__ void k_theta_grad(const T *pot, const T *nphi, const T *p, const T *phi, double c, double s, long long n,
+template <typename T, int VEC>
+__global__ void k_theta_grad(const T *__restrict__ pot, const T *__restrict__ nphi, const T *__restrict__ p,
+                             const T *__restrict__ phi, double c, double s, long long n,
                              double *partials) {
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T p2 = (T)((double)p[i] * c - (double)phi[i] * s);
         acc[0] += (double)pot[i] * (double)nphi[i] * (double)p2;
-    }
-    write_partials<1>(acc, partials);
+    });
+    block_sums<1>(acc, partials);
 }
 
 // p = -res (steepest descent)
-template <typename T> __global__ void k_neg(const T *x, T *y, long long n) {
-    GRID_LOOP(i, n) y[i] = -x[i];
+template <typename T, int VEC> __global__ void k_neg(const T *__restrict__ x, T *__restrict__ y, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) { y[i] = -x[i]; });
 }
 
 // CG directions (math_utils.py:395-423): sums r1(r1 - r0), d(r1 - r0), r1^2,
 // r0^2, d r0
-template <typename T>
-__global__ void k_cg_sums(const T *r1, const T *r0, const T *d, long long n, double *partials) {
+template <typename T, int VEC>
+__global__ void k_cg_sums(const T *__restrict__ r1, const T *__restrict__ r0, const T *__restrict__ d, long long n,
+                          double *partials) {
     double acc[5] = {0, 0, 0, 0, 0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const double a = r1[i], b = r0[i], dd = d[i];
         acc[0] += a * (a - b);
         acc[1] += dd * (a - b);
         acc[2] += a * a;
         acc[3] += b * b;
         acc[4] += dd * b;
-    }
-    write_partials<5>(acc, partials);
+    });
+    block_sums<5>(acc, partials);
 }
 
 // CG-CD as math_utils.py:411-412 writes it: beta = -(r1 ** 2) / sum(d r0) is a
 // field (the numerator is not summed), p = -r1 + beta d
-template <typename T> __global__ void k_cg_cd(const T *r1, const T *d, double sdr0, T *p, long long n) {
-    GRID_LOOP(i, n) {
+template <typename T, int VEC> __global__ void k_cg_cd(const T *__restrict__ r1, const T *__restrict__ d,
+                                                       double sdr0, T *__restrict__ p, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const double r = r1[i];
         const double beta = -(r * r) / sdr0;
         p[i] = (T)(-r + beta * (double)d[i]);
-    }
+    });
 }
 
 // L-BFGS (math_utils.py:374-455).  A pair: s = phi - phi_old,
 // y = res - res_old, and the partial sums of y s
-template <typename T>
-__global__ void k_lbfgs_pair(const T *phi, const T *phi_old, const T *res, const T *res_old, T *s, T *y, long long n,
+template <typename T, int VEC>
+__global__ void k_lbfgs_pair(const T *__restrict__ phi, const T *__restrict__ phi_old, const T *__restrict__ res,
+                             const T *__restrict__ res_old, T *__restrict__ s, T *__restrict__ y, long long n,
                              double *partials) {
     double acc[1] = {0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const T a = phi[i] - phi_old[i], b = res[i] - res_old[i];
         s[i] = a;
         y[i] = b;
         acc[0] += (double)b * (double)a;
-    }
-    write_partials<1>(acc, partials);
+    });
+    block_sums<1>(acc, partials);
 }
 
 // scal[k] = 1 / scal[k]
@@ -284,56 +294,64 @@ __global__ void k_recip(double *scal, int k) {
 
 // first loop: alpha = rho_i sum(s_i q) (scal[rho] * scal[dot]), q -= alpha y_i;
 // alpha is kept at scal[alpha_out] for the second loop
-template <typename T>
-__global__ void k_lbfgs_q(T *q, const T *y, double *scal, int rho, int dot, int alpha_out, long long n) {
+template <typename T, int VEC>
+__global__ void k_lbfgs_q(T *__restrict__ q, const T *__restrict__ y, double *scal, int rho, int dot, int alpha_out,
+                          long long n) {
     const double alpha = scal[rho] * scal[dot];
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[alpha_out] = alpha;
-    GRID_LOOP(i, n) q[i] = (T)((double)q[i] - alpha * (double)y[i]);
+    pw_loop<VEC>(n, PW_BODY(i) { q[i] = (T)((double)q[i] - alpha * (double)y[i]); });
 }
 
 // second loop: beta = rho_i sum(y_i d), d += s_i (alpha_i - beta); alpha_in < 0
 // stands for alpha = 0 (the oldest pair, which the first loop does not visit)
-template <typename T>
-__global__ void k_lbfgs_d(T *d, const T *s, const double *scal, int rho, int dot, int alpha_in, long long n) {
+template <typename T, int VEC>
+__global__ void k_lbfgs_d(T *__restrict__ d, const T *__restrict__ s, const double *scal, int rho, int dot,
+                          int alpha_in, long long n) {
     const double beta = scal[rho] * scal[dot];
     const double a = (alpha_in < 0 ? 0.0 : scal[alpha_in]) - beta;
-    GRID_LOOP(i, n) d[i] = (T)((double)d[i] + (double)s[i] * a);
+    pw_loop<VEC>(n, PW_BODY(i) { d[i] = (T)((double)d[i] + (double)s[i] * a); });
 }
 
 // sums s y and y y (gamma = s.y / y.y of the newest pair)
-template <typename T> __global__ void k_sy_yy(const T *s, const T *y, long long n, double *partials) {
+template <typename T, int VEC> __global__ void k_sy_yy(const T *__restrict__ s, const T *__restrict__ y, long long n,
+                                                       double *partials) {
     double acc[2] = {0, 0};
-    GRID_LOOP(i, n) {
+    pw_loop<VEC>(n, PW_BODY(i) {
         const double a = s[i], b = y[i];
         acc[0] += a * b;
         acc[1] += b * b;
-    }
-    write_partials<2>(acc, partials);
+    });
+    block_sums<2>(acc, partials);
 }
 
 // x *= scal[a] / scal[b]
-template <typename T> __global__ void k_scale_ratio(T *x, const double *scal, int a, int b, long long n) {
+template <typename T, int VEC> __global__ void k_scale_ratio(T *__restrict__ x, const double *scal, int a, int b,
+                                                             long long n) {
     const double f = scal[a] / scal[b];
-    GRID_LOOP(i, n) x[i] = (T)(f * (double)x[i]);
+    pw_loop<VEC>(n, PW_BODY(i) { x[i] = (T)(f * (double)x[i]); });
 }
 
 // x *= f
-template <typename T> __global__ void k_scale(T *x, double f, long long n) {
-    GRID_LOOP(i, n) x[i] = (T)(f * (double)x[i]);
+template <typename T, int VEC> __global__ void k_scale(T *__restrict__ x, double f, long long n) {
+    pw_loop<VEC>(n, PW_BODY(i) { x[i] = (T)(f * (double)x[i]); });
 }
 
 // p *= sqrt(scal[a] / scal[b])  (normalise p to sum p^2 = scal[a] with scal[b] = sum p^2)
-template <typename T> __global__ void k_scale_norm(T *p, const double *scal, int a, int b, long long n) {
+template <typename T, int VEC> __global__ void k_scale_norm(T *__restrict__ p, const double *scal, int a, int b,
+                                                            long long n) {
     const double f = sqrt(scal[a] / scal[b]);
-    GRID_LOOP(i, n) p[i] = (T)(f * (double)p[i]);
+    pw_loop<VEC>(n, PW_BODY(i) { p[i] = (T)(f * (double)p[i]); });
 }
 
 // sum a.b
-template <typename T> __global__ void k_dot2(const T *a, const T *b, long long n, double *partials) {
+template <typename T, int VEC> __global__ void k_dot2(const T *__restrict__ a, const T *__restrict__ b, long long n,
+                                                      double *partials) {
     double acc[1] = {0};
-    GRID_LOOP(i, n) acc[0] += (double)a[i] * (double)b[i];
-    write_partials<1>(acc, partials);
+    pw_loop<VEC>(n, PW_BODY(i) { acc[0] += (double)a[i] * (double)b[i]; });
+    block_sums<1>(acc, partials);
 }
+
+#undef PW_BODY
 
 struct PrecondParams {
     double B[9];
@@ -372,7 +390,6 @@ __global__ void k_precond(cpx<T> *spec, long long pitch, int nzc, int nx, int ny
     }
 }
 
-#undef GRID_LOOP
 
 // ------------------------------------------------------------------ host helpers
 struct Ctx {
@@ -380,8 +397,12 @@ struct Ctx {
     hipStream_t s;
     int nb;
     double *host;  // pinned
+    int vec;  // element width of the streaming kernels (stream.h)
+    int nbw;  // workgroups of the non-reducing streaming kernels
     Ctx(Plan &p, hipStream_t st) : P(p), s(st) {
         nb = std::min(reduce_blocks(P.N), 1024);
+        vec = pw_vec(P.N);
+        nbw = pw_blocks(P.N);
         host = P.pinned;
     }
 
@@ -406,6 +427,19 @@ struct Ctx {
         return host;
     }
 };
+
+// Launch the streaming kernel template K<T, VEC> with the context's element
+// width: the reducing kernels on the C.nb partial-sum workgroups, the others
+// on C.nbw workgroups.
+#define PW_LAUNCH_(K, NB, ...)                                                                \
+    do {                                                                                      \
+        if (C.vec == 4)                                                                       \
+            hipLaunchKernelGGL((K<T, 4>), dim3(NB), dim3(256), 0, C.s, __VA_ARGS__);          \
+        else                                                                                  \
+            hipLaunchKernelGGL((K<T, 1>), dim3(NB), dim3(256), 0, C.s, __VA_ARGS__);          \
+    } while (0)
+#define PW_RED(K, ...) PW_LAUNCH_(K, C.nb, __VA_ARGS__)
+#define PW_STREAM(K, ...) PW_LAUNCH_(K, C.nbw, __VA_ARGS__)
 
 template <typename T> struct Opt {
     Plan &P;
@@ -432,10 +466,10 @@ template <typename T> struct Opt {
     // scal[2] (sums at scal[18], scal[19]), then res = (V - mu) phi.  No host
     // round trip: the host needs mu only for the final report.
     void mu_and_residual(const T *pot, const T *rho, const T *phi, T *res) {
-        hipLaunchKernelGGL((k_mu_sums<T>), C.grid(), dim3(256), 0, C.s, pot, rho, n, P.red);
+        PW_RED(k_mu_sums, pot, rho, n, P.red);
         C.reduce_final_cols(2, 18);
         hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 2, 18, 19);
-        hipLaunchKernelGGL((k_residual<T>), C.grid(), dim3(256), 0, C.s, pot, phi, res, P.scal + 2, n, P.red);
+        PW_RED(k_residual, pot, phi, res, P.scal + 2, n, P.red);
         C.reduce_final_cols(1, 20);
     }
 
@@ -495,8 +529,8 @@ template <typename T> struct Opt {
             int sl[3];
             work_slots(P, sl);
             const T *work = reinterpret_cast<const T *>(P.work);
-            if (sl[0] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[0] * n, F(F_BASE_A), n);
-            if (sl[1] >= 0) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, work + sl[1] * n, F(F_BASE_C), n);
+            if (sl[0] >= 0) PW_STREAM(k_copy, work + sl[0] * n, F(F_BASE_A), n);
+            if (sl[1] >= 0) PW_STREAM(k_copy, work + sl[1] * n, F(F_BASE_C), n);
         }
         // preconditioned CG (when the plan holds a preconditioner): z = M^-1 r,
         // alpha = r.z / p.Ap, p <- z + beta p with beta = r1.z1 / r0.z0
@@ -504,11 +538,11 @@ template <typename T> struct Opt {
         T *z = F(F_Z);
         auto precond_dot = [&]() {
             apply_precond(tres, z);
-            hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, tres, z, n, P.red);
+            PW_RED(k_dot2, tres, z, n, P.red);
         };
         // |r|^2 -> scal[25] (stopping tests), r.z -> scal[26]; the host reads
         // both with the first iteration's scalars
-        hipLaunchKernelGGL((k_tn_start<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), tres, tp, dir, n, P.red);
+        PW_RED(k_tn_start, F(F_RES), tres, tp, dir, n, P.red);
         C.reduce_final_cols(1, 25);
         if (pre) {
             precond_dot();
@@ -526,9 +560,9 @@ template <typename T> struct Opt {
                 hvp<T>(P, F(F_RHO), F(F_PHI), tp, F(F_POT), F(F_BASE_A), F(F_BASE_C), P.scal + 2, Ap, C.nb, C.s);
                 ++n_v;
             } else {
-                hipLaunchKernelGGL((k_tn_shift<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), tp, eps, phi1, rho1, n);
+                PW_STREAM(k_tn_shift, F(F_PHI), tp, eps, phi1, rho1, n);
                 eval_v(rho1, pot1);
-                hipLaunchKernelGGL((k_tn_ap<T>), C.grid(), dim3(256), 0, C.s, pot1, phi1, F(F_RES), tp, P.scal + 2, eps,
+                PW_RED(k_tn_ap, pot1, phi1, F(F_RES), tp, P.scal + 2, eps,
                                    Ap, n, P.red);
             }
             // pAp -> scal[16]; the update takes alpha = r.z / pAp from the
@@ -536,7 +570,7 @@ template <typename T> struct Opt {
             // r1 -> scal[17]; r1.z1 -> scal[18]
             C.reduce_final_cols(1, 16);
             const int rz_slot = it == 0 ? (pre ? 26 : 25) : (pre ? 18 : 17);
-            hipLaunchKernelGGL((k_tn_update_dev<T>), C.grid(), dim3(256), 0, C.s, dir, tres, tp, Ap, P.scal + rz_slot,
+            PW_RED(k_tn_update_dev, dir, tres, tp, Ap, P.scal + rz_slot,
                                P.scal + 16, n, P.red);
             C.reduce_final_cols(1, 17);
             if (pre) {
@@ -551,13 +585,13 @@ template <typename T> struct Opt {
             }
             const double pAp = hh[0], r1 = hh[1], rz1 = pre ? hh[2] : hh[1];
             if (pAp < 0.0) {
-                if (it == 0) hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, rz / pAp, tp, 0.0, dir, n);
+                if (it == 0) PW_STREAM(k_axpby, rz / pAp, tp, 0.0, dir, n);
                 break;
             }
             double rmin = rl[0];
             for (double v : rl) rmin = std::min(rmin, v);
             if (r1 < rmin) {
-                hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, dir, best, n);
+                PW_STREAM(k_copy, dir, best, n);
                 best_zero = false;
             }
             rl.push_back(r1);
@@ -567,15 +601,15 @@ template <typename T> struct Opt {
             if (r1 > 1000.0 * rmin_prev || (it > 10 && std::fabs(r0 - r1) < 0.1 * r0)) {
                 // fall back to the best direction seen
                 if (best_zero)
-                    hipLaunchKernelGGL((k_fill<T>), C.grid(), dim3(256), 0, C.s, dir, (T)0, n);
+                    PW_STREAM(k_fill, dir, (T)0, n);
                 else
-                    hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, best, dir, n);
+                    PW_STREAM(k_copy, best, dir, n);
                 break;
             }
             const double beta = rz1 / rz;
             r0 = r1;
             rz = rz1;
-            hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, 1.0, pre ? z : tres, beta, tp, n);
+            PW_STREAM(k_axpby, 1.0, pre ? z : tres, beta, tp, n);
         }
         return std::min(it + 1, maxfun);
     }
@@ -591,13 +625,13 @@ template <typename T> struct Opt {
     // evaluation (scal[32..]), which the first line search needs.
     void orthonormalize(double &theta0, double &vphip, double *energies = nullptr) {
         T *p = F(F_P), *phi = F(F_PHI);
-        hipLaunchKernelGGL((k_pphi<T>), C.grid(), dim3(256), 0, C.s, p, phi, n, P.red);
+        PW_RED(k_pphi, p, phi, n, P.red);
         C.reduce_final_cols(2, 21);  // scal[21] = sum p phi, scal[22] = sum phi^2
         hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 3, 21, 22);
-        hipLaunchKernelGGL((k_orth<T>), C.grid(), dim3(256), 0, C.s, p, phi, P.scal + 3, n, P.red);
+        PW_RED(k_orth, p, phi, P.scal + 3, n, P.red);
         C.reduce_final_cols(1, 23);  // scal[23] = sum p^2
-        hipLaunchKernelGGL((k_scale_norm<T>), C.grid(), dim3(256), 0, C.s, p, P.scal, 22, 23, n);
-        hipLaunchKernelGGL((k_dot3<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), phi, p, n, P.red);
+        PW_STREAM(k_scale_norm, p, P.scal, 22, 23, n);
+        PW_RED(k_dot3, F(F_POT), phi, p, n, P.red);
         C.reduce_final_cols(1, 24);  // scal[24] = sum V phi p
         const double *h = C.to_host(energies ? 32 + OFDFT_E_COUNT - 21 : 4, 21);
         theta0 = std::sqrt(h[2] / h[1]);
@@ -610,10 +644,10 @@ template <typename T> struct Opt {
     // the energies and the gradient sum come back in one copy
     void value_deriv(double th, double &e, double &g) {
         const double c = std::cos(th), s = std::sin(th);
-        hipLaunchKernelGGL((k_rotate<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_P), c, s, F(F_NPHI), F(F_NRHO), n);
+        PW_STREAM(k_rotate, F(F_PHI), F(F_P), c, s, F(F_NPHI), F(F_NRHO), n);
         eval<T>(P, F(F_NRHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_NPOT), energies_d, C.s);
         ++n_ev;
-        hipLaunchKernelGGL((k_theta_grad<T>), C.grid(), dim3(256), 0, C.s, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s,
+        PW_RED(k_theta_grad, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s,
                            n, P.red);
         C.reduce_final_cols(1, 32 + OFDFT_E_COUNT);
         const double *h = C.to_host(OFDFT_E_COUNT + 1, 32);
@@ -646,7 +680,7 @@ template <typename T> struct Opt {
         const int k = L.free_slots.back();
         L.free_slots.pop_back();
         // phi_old is F_NPHI and res_old F_RES_PREV after the accept swaps
-        hipLaunchKernelGGL((k_lbfgs_pair<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_NPHI), F(F_RES),
+        PW_RED(k_lbfgs_pair, F(F_PHI), F(F_NPHI), F(F_RES),
                            F(F_RES_PREV), lb_s(L, k), lb_y(L, k), n, P.red);
         C.reduce_final_cols(1, LB_RHO + k);
         hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, C.s, P.scal, LB_RHO + k);
@@ -656,31 +690,31 @@ template <typename T> struct Opt {
     // two-loop recursion (math_utils.py:431-455) into F_P, scalars on the device
     void lbfgs_direction(Lbfgs &L) {
         T *q = F(F_P);
-        hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), q, n);
+        PW_STREAM(k_neg, F(F_RES), q, n);
         const int m = (int)L.order.size();
         for (int i = m - 1; i >= 1; --i) {  // the reference's first loop stops before the oldest pair
             const int k = L.order[i];
-            hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, lb_s(L, k), q, n, P.red);
+            PW_RED(k_dot2, lb_s(L, k), q, n, P.red);
             C.reduce_final_cols(1, LB_DOT);
-            hipLaunchKernelGGL((k_lbfgs_q<T>), C.grid(), dim3(256), 0, C.s, q, lb_y(L, k), P.scal, LB_RHO + k,
+            PW_STREAM(k_lbfgs_q, q, lb_y(L, k), P.scal, LB_RHO + k,
                                LB_DOT, LB_ALPHA + i, n);
         }
         if (L.h0 == 0.0) {
             if (m >= 1) {  // gamma = s.y / y.y of the newest pair; gamma = 1 without pairs
                 const int k = L.order[m - 1];
-                hipLaunchKernelGGL((k_sy_yy<T>), C.grid(), dim3(256), 0, C.s, lb_s(L, k), lb_y(L, k), n, P.red);
+                PW_RED(k_sy_yy, lb_s(L, k), lb_y(L, k), n, P.red);
                 C.reduce_final_cols(2, LB_DOT + 1);
-                hipLaunchKernelGGL((k_scale_ratio<T>), C.grid(), dim3(256), 0, C.s, q, P.scal, LB_DOT + 1,
+                PW_STREAM(k_scale_ratio, q, P.scal, LB_DOT + 1,
                                    LB_DOT + 2, n);
             }
         } else {
-            hipLaunchKernelGGL((k_scale<T>), C.grid(), dim3(256), 0, C.s, q, L.h0, n);
+            PW_STREAM(k_scale, q, L.h0, n);
         }
         for (int i = 0; i < m; ++i) {
             const int k = L.order[i];
-            hipLaunchKernelGGL((k_dot2<T>), C.grid(), dim3(256), 0, C.s, lb_y(L, k), q, n, P.red);
+            PW_RED(k_dot2, lb_y(L, k), q, n, P.red);
             C.reduce_final_cols(1, LB_DOT);
-            hipLaunchKernelGGL((k_lbfgs_d<T>), C.grid(), dim3(256), 0, C.s, q, lb_s(L, k), P.scal, LB_RHO + k,
+            PW_STREAM(k_lbfgs_d, q, lb_s(L, k), P.scal, LB_RHO + k,
                                LB_DOT, i == 0 ? -1 : LB_ALPHA + i, n);
         }
     }
@@ -705,9 +739,9 @@ template <typename T> struct Opt {
         const auto t0 = std::chrono::steady_clock::now();
         // initial phi (optimization.py:305-316), normalised on the device; the
         // starting energy reaches the host with the first orthonormalisation
-        hipLaunchKernelGGL((k_init_phi<T>), C.grid(), dim3(256), 0, C.s, rho_io, F(F_PHI), n, P.red);
+        PW_RED(k_init_phi, rho_io, F(F_PHI), n, P.red);
         C.reduce_final_cols(2, 16);
-        hipLaunchKernelGGL((k_scale_square<T>), C.grid(), dim3(256), 0, C.s, F(F_PHI), F(F_RHO), P.scal + 16, n);
+        PW_STREAM(k_scale_square, F(F_PHI), F(F_RHO), P.scal + 16, n);
         eval<T>(P, F(F_RHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_POT), energies_d, C.s);
         ++n_ev;
         mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES));
@@ -728,13 +762,13 @@ template <typename T> struct Opt {
                 // CG (math_utils.py:395-423); first step and missing history: -res;
                 // GD (math_utils.py:426-428) always -res
                 if (nres < 1 || method == OFDFT_OPT_GD) {
-                    hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
+                    PW_STREAM(k_neg, F(F_RES), F(F_P), n);
                 } else {
-                    hipLaunchKernelGGL((k_cg_sums<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_RES_PREV),
+                    PW_RED(k_cg_sums, F(F_RES), F(F_RES_PREV),
                                        F(F_DIR_PREV), n, P.red);
                     const double *q = C.fetch(5);
                     if (method == OFDFT_OPT_CG_CD) {
-                        hipLaunchKernelGGL((k_cg_cd<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_DIR_PREV), q[4],
+                        PW_STREAM(k_cg_cd, F(F_RES), F(F_DIR_PREV), q[4],
                                            F(F_P), n);
                     } else {
                         double beta;
@@ -748,8 +782,8 @@ template <typename T> struct Opt {
                             beta = q[0] / q[4];
                         else
                             beta = q[2] / q[3];
-                        hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_DIR_PREV), F(F_P), n);
-                        hipLaunchKernelGGL((k_axpby<T>), C.grid(), dim3(256), 0, C.s, -1.0, F(F_RES), beta, F(F_P), n);
+                        PW_STREAM(k_copy, F(F_DIR_PREV), F(F_P), n);
+                        PW_STREAM(k_axpby, -1.0, F(F_RES), beta, F(F_P), n);
                     }
                 }
             }
@@ -760,7 +794,7 @@ template <typename T> struct Opt {
                 energy_pending = false;
             }
             if (vphip > 0) {  // uphill: restart from the steepest descent direction
-                hipLaunchKernelGGL((k_neg<T>), C.grid(), dim3(256), 0, C.s, F(F_RES), F(F_P), n);
+                PW_STREAM(k_neg, F(F_RES), F(F_P), n);
                 orthonormalize(theta0, vphip);
             }
             theta = std::min(theta0, theta);
@@ -821,8 +855,8 @@ template <typename T> struct Opt {
                 }
             }
         }
-        hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_RHO), rho_io, n);
-        if (pot_out) hipLaunchKernelGGL((k_copy<T>), C.grid(), dim3(256), 0, C.s, F(F_POT), pot_out, n);
+        PW_STREAM(k_copy, F(F_RHO), rho_io, n);
+        if (pot_out) PW_STREAM(k_copy, F(F_POT), pot_out, n);
         // mu of the final density (scal[2], set by the last residual) and, when
         // no iteration ran, the starting energy
         const double *hm = C.to_host(32 + OFDFT_E_COUNT - 2, 2);
@@ -847,6 +881,10 @@ template <typename T> struct Opt {
         return converged;
     }
 };
+
+#undef PW_STREAM
+#undef PW_RED
+#undef PW_LAUNCH_
 
 }  // namespace ofdft
 
