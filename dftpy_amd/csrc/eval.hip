@@ -197,10 +197,8 @@ constexpr double PZ_G = -0.1423, PZ_B1 = 1.0529, PZ_B2 = 0.3334;
 // Reciprocal and reciprocal square root: exact division for FP64; the
 // hardware approximations (about 1 ulp) for the FP32-bulk pipeline, whose
 // point-wise work is otherwise dominated by IEEE division sequences.
-template <typename CT> __device__ __forceinline__ CT rcp_p(CT x) { return (CT)1 / x; }
-template <> __device__ __forceinline__ float rcp_p<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-template <typename CT> __device__ __forceinline__ CT rsq_p(CT x) { return (CT)1 / sqrt(x); }
-template <> __device__ __forceinline__ float rsq_p<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <typename CT> __device__ __forceinline__ CT rcp_p(CT x) { return fm::rcp(x); }
+template <typename CT> __device__ __forceinline__ CT rsq_p(CT x) { return fm::rsq(x); }
 
 template <typename T, typename CT, int VEC>
 __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, const T *__restrict__ vloc,
@@ -226,7 +224,7 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             const CT r = (CT)rv.v[k];
             CT v = (CT)0;
             // one cube root and one square root serve every term (for beta = 5/6)
-            const CT cr = cbrt(r);
+            const CT cr = fm::cbrt(r);
             const CT rc = r < (CT)1e-30 ? (CT)1e-30 : r;
             const CT c_rc = r < (CT)1e-30 ? (CT)1e-10 : cr;  // cbrt(max(rho, 1e-30))
             if (p.terms & OFDFT_TERM_TF) {
@@ -244,7 +242,7 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
         }
         if (p.s_wt >= 0 && p.s_wt2 < 0) {
             const CT c = (CT)cv.v[k];
-            const CT rb_rc = p.b56 ? c_rc * rsq_p<CT>(rc) : pow(rc, (CT)p.beta) * rcp_p<CT>(rc);  // rho^beta / rho
+            const CT rb_rc = p.b56 ? c_rc * rsq_p<CT>(rc) : fm::pow(rc, (CT)p.beta) * rcp_p<CT>(rc);  // rho^beta / rho
             const CT vnl = (CT)(2.0 * p.alpha) * rb_rc * c;
             if (p.want_e) acc[2] += (double)(vnl * r) / (2.0 * p.alpha);
             v += vnl;
@@ -252,7 +250,7 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             // alpha != beta (wt.py:23-25,44-56): cb = K * rho^beta, ca = K * rho^alpha
             const CT cb = (CT)cv.v[k];
             const CT ca = (CT)c2v.v[k];
-            const CT ra = pow(rc, (CT)p.alpha), rb = pow(rc, (CT)p.beta);
+            const CT ra = fm::pow(rc, (CT)p.alpha), rb = fm::pow(rc, (CT)p.beta);
             const CT irc = rcp_p<CT>(rc);
             if (p.want_e) acc[2] += (double)(ra * cb);
             v += (CT)p.alpha * ra * irc * cb + (CT)p.beta * rb * irc * ca;
@@ -263,18 +261,18 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
             CT rx = r, rc3 = cr;
             if (p.core) {
                 rx = r + (CT)corev.v[k];
-                rc3 = cbrt(rx);
+                rc3 = fm::cbrt(rx);
             }
             if (rc3 < (CT)1e-30) rc3 = (CT)1e-30;
             const CT rs = cbrt34pi * rcp_p<CT>(rc3);
             CT ec, vc;
             if (rs < (CT)1) {
-                const CT lr = log(rs);
+                const CT lr = fm::log(rs);
                 ec = (CT)PZ_A * lr + (CT)PZ_B + (CT)PZ_C * rs * lr + (CT)PZ_D * rs;
                 vc = lr * ((CT)PZ_A + (CT)(2.0 / 3.0 * PZ_C) * rs) + (CT)PZ_B - (CT)(1.0 / 3.0 * PZ_A) +
                      (CT)(1.0 / 3.0 * (2 * PZ_D - PZ_C)) * rs;
             } else {
-                const CT sr = sqrt(rs);
+                const CT sr = fm::sqrt(rs);
                 const CT iden = rcp_p<CT>((CT)1 + (CT)PZ_B1 * sr + (CT)PZ_B2 * rs);
                 ec = (CT)PZ_G * iden;
                 vc = ((CT)PZ_G + (CT)(7.0 / 6.0 * PZ_G * PZ_B1) * sr + (CT)(4.0 / 3.0 * PZ_G * PZ_B2) * rs) *
@@ -710,7 +708,7 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
             const T phi_i = phv.v[k], p_i = pv.v[k];
             const CT r = (CT)rv.v[k], dr = (CT)(T)(2.0 * (double)phi_i * (double)p_i);  // drho = 2 phi p
             const CT rp = r > (CT)1e-30 ? r : (CT)1e-30;
-            const CT c_rp = cbrt(rp);
+            const CT c_rp = fm::cbrt(rp);
             const CT ic = rcp_p<CT>(c_rp);  // rho^(-1/3)
             const CT irp = ic * ic * ic;    // 1 / rho
             CT dv = (CT)0;
@@ -722,7 +720,7 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
                 dv += ((CT)pp.y * (CT)avw.v[k] - vwb * ds) * isq;
         }
         if (pp.s_wt >= 0) {
-            const CT rb1 = pp.b56 ? c_rp * rsq_p<CT>(rp) : pow(rp, (CT)pp.beta - (CT)1);  // rho^(beta-1)
+            const CT rb1 = pp.b56 ? c_rp * rsq_p<CT>(rp) : fm::pow(rp, (CT)pp.beta - (CT)1);  // rho^(beta-1)
             dv += (CT)(2.0 * pp.alpha) * (((CT)pp.beta - (CT)1) * rb1 * irp * dr * (CT)bcv.v[k] +
                                            rb1 * (CT)cwt.v[k]);
         }
@@ -732,9 +730,10 @@ __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
             const CT rs = cbrt34pi * ic;
             CT fc;
             if (rs < (CT)1) {
-                fc = -(CT)(PZ_A / 3.0) - ((CT)(PZ_C / 9.0) * (log(rs) * (CT)2 + (CT)1) + (CT)(PZ_D * 2.0 / 9.0)) * rs;
+                fc = -(CT)(PZ_A / 3.0) -
+                     ((CT)(PZ_C / 9.0) * (fm::log(rs) * (CT)2 + (CT)1) + (CT)(PZ_D * 2.0 / 9.0)) * rs;
             } else {
-                const CT ta = (CT)PZ_B1 * sqrt(rs), tb = (CT)PZ_B2 * rs;
+                const CT ta = (CT)PZ_B1 * fm::sqrt(rs), tb = (CT)PZ_B2 * rs;
                 const CT ide = rcp_p<CT>((CT)1 + ta + tb);
                 fc = (CT)(PZ_G / 36.0) *
                      ((CT)5 * ta + (CT)7 * ta * ta + (CT)8 * tb + (CT)16 * tb * tb + (CT)21 * ta * tb) *

@@ -228,11 +228,11 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
                     f = (T)0.5 * gg * sc;
                     if (gp.sigma > 0) f *= exp(-gg * sig);
                 } else if (kind == 1) {
-                    f = lindhard_t<T>(sqrt(gg) * itkf, lb, (T)1) * wsc;
+                    f = lindhard_t<T>(fm::sqrt(gg) * itkf, lb, (T)1) * wsc;
                 } else if (kind == 3) {
                     f = (T)(radial_table(gp, sqrt((double)gg)) * gp.scale);
                 } else {
-                    f = (yz0 && e == 0) ? (T)0 : (T)(4.0 * M_PI) * sc / gg;
+                    f = (yz0 && e == 0) ? (T)0 : fm::div((T)(4.0 * M_PI) * sc, gg);
                 }
                 C v = buf[e * TILE + pen];
                 v.x *= f;
@@ -294,26 +294,26 @@ __device__ __forceinline__ void rstages(C *buf, const C *tw, int tid) {
 // linear-response codes (which need rho) cannot occur: the host rejects them.
 template <typename T, bool AUX, bool B56> __device__ __forceinline__ T prologue_t(int code, T x, T beta, T r) {
     switch (code) {
-        case OP_SQRT_RHOM: return sqrt(x > (T)0 ? x : (T)1e-30);
+        case OP_SQRT_RHOM: return fm::sqrt(x > (T)0 ? x : (T)1e-30);
         case OP_POW_BETA: {
             const T xc = x < (T)1e-30 ? (T)1e-30 : x;
             if constexpr (B56)
-                return sqrt(xc) * cbrt(xc);
+                return fm::sqrt(xc) * fm::cbrt(xc);
             else
-                return pow(xc, beta);
+                return fm::pow(xc, beta);
         }
         case OP_DSQRT:
             if constexpr (AUX)
-                return x / ((T)2 * sqrt(r > (T)0 ? r : (T)1e-30));
+                return fm::div(x, (T)2 * fm::sqrt(r > (T)0 ? r : (T)1e-30));
             else
                 return x;
         case OP_DPOW_BETA:
             if constexpr (AUX) {
                 const T rc = r < (T)1e-30 ? (T)1e-30 : r;
                 if constexpr (B56)
-                    return beta * (cbrt(rc) / sqrt(rc)) * x;
+                    return beta * fm::div(fm::cbrt(rc), fm::sqrt(rc)) * x;
                 else
-                    return beta * pow(rc, beta - (T)1) * x;
+                    return beta * fm::pow(rc, beta - (T)1) * x;
             } else {
                 return x;
             }

@@ -6,6 +6,8 @@
 
 #include <cmath>
 
+#include "fastmath.h"
+
 namespace ofdft {
 
 // ------------------------------------------------------------------ Lindhard
@@ -15,7 +17,7 @@ namespace ofdft {
 template <typename T> __device__ __forceinline__ T lindhard_t(T eta, T lbda, T mu) {
     const T tol = (T)1e-10;
     if (eta > (T)3.65) {
-        const T inv = (T)1 / (eta * eta);
+        const T inv = fm::rcp(eta * eta);
         T s = (T)-3.00624946457977689e-3;
         s = (T)-3.38745061493813488e-3 + inv * s;
         s = (T)-3.84763737842981233e-3 + inv * s;
@@ -35,8 +37,9 @@ template <typename T> __device__ __forceinline__ T lindhard_t(T eta, T lbda, T m
     }
     if (fabs(eta - (T)1) < tol) return (T)2 - lbda - (T)3 * mu + (T)20 * (eta - (T)1);
     if (eta < tol) return (T)1 - lbda + eta * eta * ((T)(1.0 / 3.0) - (T)3 * mu);
-    return (T)1 / ((T)0.5 + (T)0.25 * ((T)1 - eta * eta) * log(((T)1 + eta) / fabs((T)1 - eta)) / eta) -
-           (T)3 * mu * eta * eta - lbda;
+    // the logarithm stays the library's: its argument tends to 1 as eta -> 0
+    const T lg = log(fm::div((T)1 + eta, fabs((T)1 - eta)));
+    return fm::rcp((T)0.5 + fm::div((T)0.25 * ((T)1 - eta * eta) * lg, eta)) - (T)3 * mu * eta * eta - lbda;
 }
 
 __device__ __forceinline__ double lindhard(double eta, double lbda, double mu) { return lindhard_t<double>(eta, lbda, mu); }
