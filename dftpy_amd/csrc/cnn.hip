@@ -535,8 +535,10 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
 // planes), so the input planes x - d, x, x + d of step j are planes j - 1,
 // j, j + 1 of the walk: a ring of 4 LDS plane slots (6 staged rows y0 +
 // (r - 1) d, 32 + 2 DMAX z points with the dz halo, 32 channels) holds the
-// three in use and receives the next one, whose global loads are issued at
-// the top of the step and land after its MFMAs.  Each input row is staged
+// three in use and receives the next one.  The global loads of a plane are
+// issued two steps before it is needed, at the top of step j for plane j + 3,
+// into one of two register sets; the other set (plane j + 2, issued a step
+// earlier) lands in the free slot after the step's MFMAs.  Each input row is staged
 // 1.5 (y) x 1.25 (z halo) times per layer instead of 4.5 times, and the
 // weights never pass through LDS: wave (z tile, channel half) keeps the B
 // operands of all 27 taps for its 16 output channels in registers.
@@ -559,21 +561,19 @@ constexpr size_t RING_LDS = (size_t)(4 * RING_PLANE + 2 * RING_OUT) * 16;
 __global__ void __launch_bounds__(RING_THREADS, 2)
     k_cnn_mid_ring(Geo g, const __bf16 *__restrict__ xin, const __bf16 *__restrict__ w,
                    const float *__restrict__ bias, __bf16 *__restrict__ out) {
-    extern __shared__ bf16x8 lds[];  // [4][6][RING_ZS][4] plane slots, [2][4][RING_Z][4] output stages
+    extern __shared__ bf16x8 lds[];
     bf16x8 *ostage = lds + 4 * RING_PLANE;
     const int d = g.d, nx = g.nx, ny = g.ny, nz = g.nz;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int zt = wv & 1, hh = wv >> 1;  // z tile, channel half
+    const int zt = wv & 1, hh = wv >> 1;
     const int m = lane & 15, kq = lane >> 4;
     const long long nyz = (long long)ny * nz;
     const bf16x8 *xv = reinterpret_cast<const bf16x8 *>(xin);
     const bf16x8 *wvv = reinterpret_cast<const bf16x8 *>(w);
-    // this wave's B operands: W[t][co = hh*16 + m][ci = kq*8 .. kq*8+7]
     bf16x8 bw[NTAP];
 #pragma unroll
     for (int t = 0; t < NTAP; ++t) bw[t] = wvv[(t * CH + hh * 16 + m) * 4 + kq];
     const float bco = bias[hh * 16 + m];
-    // walks: w = (c * nzb + zb) * ngroups + gy, steps j = 0 .. nx/d - 1
     const int ngroups = ny / 4, nzb = nz / RING_Z, walk = nx / d;
     const long long total = (long long)ngroups * nzb * d * walk;
     const int nwg = gridDim.x, per_xcd = nwg / ROWS_XCDS;
@@ -594,25 +594,29 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
         st.y0 = (gy / d) * 4 * d + gy % d;
         return st;
     };
-    // global loads of walk plane i (x = c + d i mod nx) of a step's rows
-    bf16x8 stg[RING_PER];
-    auto fetch = [&](const Step &st, int i) {
+    // per-thread element offsets of a plane slot (row, z, 16-byte chunk),
+    // the last element repeated for the threads past the plane's end
+    int eo_r[RING_PER], eo_z[RING_PER], eo_k[RING_PER];
+#pragma unroll
+    for (int q = 0; q < RING_PER; ++q) {
+        const int e = min(tid + q * RING_THREADS, RING_PLANE - 1);
+        eo_r[q] = e / (RING_ZS * 4);
+        const int rem = e - eo_r[q] * (RING_ZS * 4);
+        eo_z[q] = rem >> 2;
+        eo_k[q] = rem & 3;
+    }
+    auto fetch = [&](const Step &st, int i, bf16x8 *stg) {
         const int xp = ((st.c + d * i) % nx + nx) % nx;
         const int z0 = st.zb * RING_Z - RING_DMAX;
 #pragma unroll
         for (int q = 0; q < RING_PER; ++q) {
-            const int e = tid + q * RING_THREADS;
-            if (e < RING_PLANE) {
-                const int r = e / (RING_ZS * 4), rem = e - r * (RING_ZS * 4);
-                const int zz = rem >> 2, kk = rem & 3;
-                const int yr = wrapi(st.y0 + (r - 1) * d, ny);
-                int z = z0 + zz;
-                z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-                stg[q] = xv[((long long)xp * nyz + (long long)yr * nz + z) * 4 + kk];
-            }
+            const int yr = wrapi(st.y0 + (eo_r[q] - 1) * d, ny);
+            int z = z0 + eo_z[q];
+            z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+            stg[q] = xv[((long long)xp * nyz + (long long)yr * nz + z) * 4 + eo_k[q]];
         }
     };
-    auto land = [&](int slot) {
+    auto land = [&](int slot, const bf16x8 *stg) {
         bf16x8 *dst = lds + slot * RING_PLANE;
 #pragma unroll
         for (int q = 0; q < RING_PER; ++q) {
@@ -631,30 +635,27 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
             ov[((long long)x * nyz + (long long)(st.y0 + k * d) * nz + st.zb * RING_Z) * 4 + rem] = src[e];
         }
     };
-    int head = 0;  // slot of walk plane j - 1; j at head + 1, j + 1 at head + 2
-#pragma unroll 1
-    for (long long s = s_begin; s < s_end; ++s) {
+    int head = 0;  // slot of walk plane j - 1; j at head + 1, j + 1 at head + 2, j + 2 lands at head + 3
+    bf16x8 sa[RING_PER], sb[RING_PER];
+    // one step: `landing` holds plane j + 2 (issued a step earlier, or at the
+    // walk's start), `issuing` receives plane j + 3
+    auto do_step = [&](long long s, bf16x8 *landing, bf16x8 *issuing) {
         const Step st = step_of(s);
         if (s > s_begin) copy_out(step_of(s - 1), (int)((s - 1) & 1));
         if (s == s_begin || st.j == 0) {
-            // the walk's first three planes (the previous step's barrier
-            // freed every slot)
+            // the walk's first three planes, then plane j + 2 into the landing set
 #pragma unroll 1
             for (int i = -1; i <= 1; ++i) {
-                fetch(st, st.j + i);
-                land((head + 1 + i) & 3);
+                fetch(st, st.j + i, issuing);
+                land((head + 1 + i) & 3, issuing);
             }
+            fetch(st, st.j + 2, landing);
             __syncthreads();
         }
-        const bool pre = s + 1 < s_end && st.j + 1 < walk;
-        if (pre) fetch(st, st.j + 2);
+        fetch(st, st.j + 3, issuing);
         floatx4 acc[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[k] = floatx4{0.f, 0.f, 0.f, 0.f};
-        // the 6 staged rows of one (dx, dz) tap column are read together, one
-        // column ahead of its 12 MFMAs (two register sets), and the MFMAs
-        // cycle through the 4 output rows so that consecutive ones never
-        // depend on each other
         bf16x8 av[2][6];
         auto load_a = [&](int set, int t) {
             const int dx = t / 3, dz = t % 3;
@@ -677,8 +678,6 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
                                                                      acc[k], 0, 0, 0);
             }
         }
-        // out = x + relu(acc + b); the residual x is staged row k + 1 of the
-        // centre plane
         {
             const __bf16 *rs = reinterpret_cast<const __bf16 *>(lds + ((head + 1) & 3) * RING_PLANE);
             __bf16 *os = reinterpret_cast<__bf16 *>(ostage + (int)(s & 1) * RING_OUT);
@@ -694,9 +693,17 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
                 }
             }
         }
-        if (pre) land((head + 3) & 3);
+        land((head + 3) & 3, landing);
         __syncthreads();
         head = (head + 1) & 3;
+    };
+    // unrolled by two so the register sets swap roles statically; with every
+    // global load unconditional the compiler waits only for the set landing
+#pragma unroll 1
+    for (long long s = s_begin; s < s_end; s += 2) {
+        do_step(s, sa, sb);
+        if (s + 1 >= s_end) break;
+        do_step(s + 1, sb, sa);
     }
     if (s_end > s_begin) copy_out(step_of(s_end - 1), (int)((s_end - 1) & 1));
 }
