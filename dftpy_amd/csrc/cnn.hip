@@ -573,7 +573,10 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
     bf16x8 bw[NTAP];
 #pragma unroll
     for (int t = 0; t < NTAP; ++t) bw[t] = wvv[(t * CH + hh * 16 + m) * 4 + kq];
-    const float bco = bias[hh * 16 + m];
+    // the accumulators are D[co][point] (weights as the A operand): lane
+    // (m, kq) ends with channels hh*16 + 4kq .. +3 of point zt*16 + m
+    const int co0 = hh * 16 + kq * 4;
+    const float4 bco = float4{bias[co0], bias[co0 + 1], bias[co0 + 2], bias[co0 + 3]};
     const int ngroups = ny / 4, nzb = nz / RING_Z, walk = nx / d;
     const long long total = (long long)ngroups * nzb * d * walk;
     const int nwg = gridDim.x, per_xcd = nwg / ROWS_XCDS;
@@ -674,23 +677,25 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
             for (int dy = 0; dy < 3; ++dy) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[cur][k + dy], bw[(dx * 3 + dy) * 3 + dz],
+                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[(dx * 3 + dy) * 3 + dz], av[cur][k + dy],
                                                                      acc[k], 0, 0, 0);
             }
         }
         {
             const __bf16 *rs = reinterpret_cast<const __bf16 *>(lds + ((head + 1) & 3) * RING_PLANE);
             __bf16 *os = reinterpret_cast<__bf16 *>(ostage + (int)(s & 1) * RING_OUT);
-            const int co = hh * 16 + m;
+            const int z = zt * 16 + m;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                for (int jj = 0; jj < 4; ++jj) {
-                    const int z = zt * 16 + kq * 4 + jj;
-                    const float r = fmaxf(acc[k][jj] + bco, 0.0f);
-                    const float x0 = (float)rs[((k + 1) * RING_ZS + RING_DMAX + z) * CH + co];
-                    os[(k * RING_Z + z) * CH + co] = (__bf16)(x0 + r);
-                }
+                // 4 channels of one point: one 8-byte LDS read and write
+                const bf16x4 x4 =
+                    *reinterpret_cast<const bf16x4 *>(rs + ((k + 1) * RING_ZS + RING_DMAX + z) * CH + co0);
+                bf16x4 o4;
+                o4[0] = (__bf16)((float)x4[0] + fmaxf(acc[k][0] + bco.x, 0.0f));
+                o4[1] = (__bf16)((float)x4[1] + fmaxf(acc[k][1] + bco.y, 0.0f));
+                o4[2] = (__bf16)((float)x4[2] + fmaxf(acc[k][2] + bco.z, 0.0f));
+                o4[3] = (__bf16)((float)x4[3] + fmaxf(acc[k][3] + bco.w, 0.0f));
+                *reinterpret_cast<bf16x4 *>(os + (k * RING_Z + z) * CH + co0) = o4;
             }
         }
         land((head + 3) & 3, landing);
