@@ -71,6 +71,7 @@ __device__ __forceinline__ void decompose(const Geo &g, long long p, int &x, int
 
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(_Float16)))) _Float16 f16x4;
 
 // ------------------------------------------------------------ layer 1 (2 -> 32)
 // MFMA form: K = 27 taps x 2 channels = 54 (zero-padded to 64), FP16
@@ -154,13 +155,22 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
     const int x = blockIdx.y, gy = blockIdx.x;
     const int y0 = (gy / d) * 4 * d + gy % d;
     const long long nyz = (long long)g.ny * nz;
-    // stage: element e = ((i * 6 + r) * 2 + c) * nz + z
-    const int nstage = 36 * nz;
-    for (int e = threadIdx.x; e < nstage; e += IN_ROWS_THREADS) {
-        const int z = e % nz, q = e / nz;
+    // stage: row q = (i * 6 + r) * 2 + c holds xs[q * nz + z]; wave w takes
+    // rows w, w + 4, ..., 16 bytes per lane per load (nz % 16 == 0)
+    constexpr int NW = IN_ROWS_THREADS / 64;
+    const int nz4 = nz >> 2;
+#pragma unroll
+    for (int u = 0; u < 36 / NW; ++u) {
+        const int q = wave + NW * u;
         const int c = q & 1, r = (q >> 1) % 6, i = q / 12;
         const int xp = wrapi(x + (i - 1) * d, g.nx), yr = wrapi(y0 + (r - 1) * d, g.ny);
-        xs[e] = (_Float16)x0[(long long)c * g.n + (long long)xp * nyz + (long long)yr * nz + z];
+        const float4 *src = reinterpret_cast<const float4 *>(x0 + (long long)c * g.n + (long long)xp * nyz +
+                                                             (long long)yr * nz);
+        f16x4 *dst = reinterpret_cast<f16x4 *>(xs + q * nz);
+        for (int z4 = lane; z4 < nz4; z4 += 64) {
+            const float4 v = src[z4];
+            dst[z4] = f16x4{(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
+        }
     }
     f16x8 wa[2][2];
 #pragma unroll
@@ -556,7 +566,7 @@ constexpr int RING_PLANE = 6 * RING_ZS * 4;              // bf16x8 per plane slo
 constexpr int RING_OUT = 4 * RING_Z * 4;                 // bf16x8 per output stage
 constexpr int RING_THREADS = 256;                        // 2 z tiles x 2 channel halves
 constexpr int RING_PER = (RING_PLANE + RING_THREADS - 1) / RING_THREADS;
-constexpr size_t RING_LDS = (size_t)(4 * RING_PLANE + 2 * RING_OUT) * 16;
+constexpr size_t RING_LDS = (size_t)(4 * RING_PLANE + 2 * RING_OUT) * 16 + CH * sizeof(float);  // + the bias
 
 __global__ void __launch_bounds__(RING_THREADS, 2)
     k_cnn_mid_ring(Geo g, const __bf16 *__restrict__ xin, const __bf16 *__restrict__ w,
@@ -576,48 +586,69 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
     // the accumulators are D[co][point] (weights as the A operand): lane
     // (m, kq) ends with channels hh*16 + 4kq .. +3 of point zt*16 + m
     const int co0 = hh * 16 + kq * 4;
-    const float4 bco = float4{bias[co0], bias[co0 + 1], bias[co0 + 2], bias[co0 + 3]};
+    // the bias lives in LDS after the output stages (read once per step)
+    float *bias_s = reinterpret_cast<float *>(ostage + 2 * RING_OUT);
+    if (tid < CH) bias_s[tid] = bias[tid];
     const int ngroups = ny / 4, nzb = nz / RING_Z, walk = nx / d;
     const long long total = (long long)ngroups * nzb * d * walk;
     const int nwg = gridDim.x, per_xcd = nwg / ROWS_XCDS;
     const int xcd = blockIdx.x % ROWS_XCDS, slotw = blockIdx.x / ROWS_XCDS;
     const long long xa = total * xcd / ROWS_XCDS, xb = total * (xcd + 1) / ROWS_XCDS;
     const long long s_begin = xa + (xb - xa) * slotw / per_xcd, s_end = xa + (xb - xa) * (slotw + 1) / per_xcd;
+    // position of a step: walk (c, zb, gy), its first row y0, plane j; the
+    // divisions happen once, at the range start, later steps advance it
     struct Step {
-        int c, zb, y0, j;
+        int c, zb, gy, y0, j;
     };
     auto step_of = [&](long long s) {
         Step st;
         const long long wk = s / walk;
         st.j = (int)(s - wk * walk);
-        const int gy = (int)(wk % ngroups);
+        st.gy = (int)(wk % ngroups);
         const long long r = wk / ngroups;
         st.zb = (int)(r % nzb);
         st.c = (int)(r / nzb);
-        st.y0 = (gy / d) * 4 * d + gy % d;
+        st.y0 = (st.gy / d) * 4 * d + st.gy % d;
         return st;
     };
-    // per-thread element offsets of a plane slot (row, z, 16-byte chunk),
-    // the last element repeated for the threads past the plane's end
-    int eo_r[RING_PER], eo_z[RING_PER], eo_k[RING_PER];
-#pragma unroll
-    for (int q = 0; q < RING_PER; ++q) {
-        const int e = min(tid + q * RING_THREADS, RING_PLANE - 1);
-        eo_r[q] = e / (RING_ZS * 4);
-        const int rem = e - eo_r[q] * (RING_ZS * 4);
-        eo_z[q] = rem >> 2;
-        eo_k[q] = rem & 3;
-    }
-    auto fetch = [&](const Step &st, int i, bf16x8 *stg) {
-        const int xp = ((st.c + d * i) % nx + nx) % nx;
+    auto advance = [&](Step &st) {
+        if (++st.j == walk) {
+            st.j = 0;
+            if (++st.gy == ngroups) {
+                st.gy = 0;
+                if (++st.zb == nzb) {
+                    st.zb = 0;
+                    ++st.c;
+                }
+            }
+            st.y0 = (st.gy / d) * 4 * d + st.gy % d;
+        }
+    };
+    // this thread's offsets within an x plane of the walk's staged elements
+    // (row, z, 16-byte chunk; the last element repeated past the plane's end),
+    // set at each walk start
+    int in_off[RING_PER];
+    auto walk_setup = [&](const Step &st) {
         const int z0 = st.zb * RING_Z - RING_DMAX;
 #pragma unroll
         for (int q = 0; q < RING_PER; ++q) {
-            const int yr = wrapi(st.y0 + (eo_r[q] - 1) * d, ny);
-            int z = z0 + eo_z[q];
+            const int e = min(tid + q * RING_THREADS, RING_PLANE - 1);
+            const int r = e / (RING_ZS * 4), rem = e - r * (RING_ZS * 4);
+            const int yr = wrapi(st.y0 + (r - 1) * d, ny);
+            int z = z0 + (rem >> 2);
             z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-            stg[q] = xv[((long long)xp * nyz + (long long)yr * nz + z) * 4 + eo_k[q]];
+            in_off[q] = (yr * nz + z) * 4 + (rem & 3);
         }
+    };
+    auto fetch = [&](const Step &st, int i, bf16x8 *stg) {
+        // plane i in [-1, walk + 2]: x = c + d i wrapped into [0, nx)
+        int xp = st.c + d * i;
+        if (xp >= nx) xp -= nx;
+        if (xp >= nx) xp -= nx;
+        if (xp < 0) xp += nx;
+        const bf16x8 *src = xv + (long long)xp * nyz * 4;
+#pragma unroll
+        for (int q = 0; q < RING_PER; ++q) stg[q] = src[in_off[q]];
     };
     auto land = [&](int slot, const bf16x8 *stg) {
         bf16x8 *dst = lds + slot * RING_PLANE;
@@ -627,25 +658,31 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
             if (e < RING_PLANE) dst[e] = stg[q];
         }
     };
+    // output stage element e of this thread: row k, 16-byte chunk rem; its
+    // offset from the step's first output point
+    int oc[RING_OUT / RING_THREADS];
+#pragma unroll
+    for (int q = 0; q < RING_OUT / RING_THREADS; ++q) {
+        const int e = tid + q * RING_THREADS;
+        const int k = e / (RING_Z * 4), rem = e - k * (RING_Z * 4);
+        oc[q] = k * d * nz * 4 + rem;
+    }
     auto copy_out = [&](const Step &st, int buf) {
         const int x = st.c + d * st.j;
         const bf16x8 *src = ostage + buf * RING_OUT;
-        bf16x8 *ov = reinterpret_cast<bf16x8 *>(out);
+        bf16x8 *ov =
+            reinterpret_cast<bf16x8 *>(out) + ((long long)x * nyz + (long long)st.y0 * nz + st.zb * RING_Z) * 4;
 #pragma unroll
-        for (int q = 0; q < RING_OUT / RING_THREADS; ++q) {
-            const int e = tid + q * RING_THREADS;
-            const int k = e / (RING_Z * 4), rem = e - k * (RING_Z * 4);
-            ov[((long long)x * nyz + (long long)(st.y0 + k * d) * nz + st.zb * RING_Z) * 4 + rem] = src[e];
-        }
+        for (int q = 0; q < RING_OUT / RING_THREADS; ++q) ov[oc[q]] = src[tid + q * RING_THREADS];
     };
     int head = 0;  // slot of walk plane j - 1; j at head + 1, j + 1 at head + 2, j + 2 lands at head + 3
     bf16x8 sa[RING_PER], sb[RING_PER];
     // one step: `landing` holds plane j + 2 (issued a step earlier, or at the
     // walk's start), `issuing` receives plane j + 3
-    auto do_step = [&](long long s, bf16x8 *landing, bf16x8 *issuing) {
-        const Step st = step_of(s);
-        if (s > s_begin) copy_out(step_of(s - 1), (int)((s - 1) & 1));
+    auto do_step = [&](long long s, const Step &st, const Step &prev, bf16x8 *landing, bf16x8 *issuing) {
+        if (s > s_begin) copy_out(prev, (int)((s - 1) & 1));
         if (s == s_begin || st.j == 0) {
+            walk_setup(st);
             // the walk's first three planes, then plane j + 2 into the landing set
 #pragma unroll 1
             for (int i = -1; i <= 1; ++i) {
@@ -685,6 +722,7 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
             const __bf16 *rs = reinterpret_cast<const __bf16 *>(lds + ((head + 1) & 3) * RING_PLANE);
             __bf16 *os = reinterpret_cast<__bf16 *>(ostage + (int)(s & 1) * RING_OUT);
             const int z = zt * 16 + m;
+            const float4 bco = *reinterpret_cast<const float4 *>(bias_s + co0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 // 4 channels of one point: one 8-byte LDS read and write
@@ -704,13 +742,18 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
     };
     // unrolled by two so the register sets swap roles statically; with every
     // global load unconditional the compiler waits only for the set landing
+    Step st = step_of(s_begin), prev = st;
 #pragma unroll 1
     for (long long s = s_begin; s < s_end; s += 2) {
-        do_step(s, sa, sb);
+        do_step(s, st, prev, sa, sb);
+        prev = st;
+        advance(st);
         if (s + 1 >= s_end) break;
-        do_step(s + 1, sb, sa);
+        do_step(s + 1, st, prev, sb, sa);
+        prev = st;
+        advance(st);
     }
-    if (s_end > s_begin) copy_out(step_of(s_end - 1), (int)((s_end - 1) & 1));
+    if (s_end > s_begin) copy_out(prev, (int)((s_end - 1) & 1));
 }
 
 // ------------------------------------------------------------ layer 6 (32 -> 1)
@@ -746,7 +789,6 @@ __global__ void __launch_bounds__(256) k_cnn_out(Geo g, const __bf16 *__restrict
 // B operand (columns = taps).  A wave takes 64 consecutive points (4 tiles),
 // gathers the 27 x 64 results in LDS and writes each tap plane's 128 bytes
 // with 16-byte stores.  W6 stays in registers.
-typedef __attribute__((__vector_size__(4 * sizeof(_Float16)))) _Float16 f16x4;
 constexpr int PROJ_THREADS = 256;
 
 __global__ void __launch_bounds__(PROJ_THREADS) k_cnn_out_proj16(Geo g, const __bf16 *__restrict__ h,
@@ -1217,8 +1259,9 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
     return cnn_guard([&] {
         const Geo g = make_geo(nr, dil);
         OFDFT_REQUIRE(x0 && w && b && h, "null pointer");
-        if (g.nz % 16 == 0 && g.nz <= IN_ROWS_MAX_NZ && g.ny % (4 * g.d) == 0) {
-            // staged features + one 1 KB output tile per wave
+        if (g.nz % 16 == 0 && g.nz <= IN_ROWS_MAX_NZ && g.ny % (4 * g.d) == 0 &&
+            (reinterpret_cast<uintptr_t>(x0) & 15) == 0) {
+            // staged features (16-byte row loads) + one 1 KB output tile per wave
             const size_t lds = (size_t)36 * g.nz * sizeof(_Float16) + (size_t)(IN_ROWS_THREADS / 64) * 1024;
             hipLaunchKernelGGL(k_cnn_in_rows, dim3((unsigned)(g.ny / 4), (unsigned)g.nx), dim3(IN_ROWS_THREADS), lds,
                                St(stream), g, x0, w, b, reinterpret_cast<__bf16 *>(h));
