@@ -37,9 +37,12 @@ METRIC = "s to 1e-6 Ha/atom, 500-atom Al WT: FP32+learned start vs plain FP64 GP
 THRESH = 1e-6  # Ha/atom
 ECONV = 1e-6  # the reference's default stop rule, per atom
 # HBM bytes per launch of the CNN middle layer at 160^3, from the FETCH_SIZE /
-# WRITE_SIZE counter runs of `bench.py --cnn-only` (None until measured)
-TRAFFIC_CNN_MID = None
-TRAFFIC_CNN_MID_SOURCE = "not measured"
+# WRITE_SIZE counter runs of `bench.py --cnn-only --steps 5` (24 launches):
+# FETCH_SIZE 267,993 KiB x 2 (a plain device copy reads half its bytes on this
+# counter) + WRITE_SIZE 256,000 KiB x 1
+TRAFFIC_CNN_MID = int((2 * 267993 + 256000) * 1024)
+TRAFFIC_CNN_MID_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --cnn-only (mean of 24 launches of "
+                          "k_cnn_mid_ring), FETCH scaled by 2 and WRITE by 1 from a 256 MiB copy")
 
 
 def log(*a):
@@ -407,7 +410,7 @@ def main():
     roof = dict(bound="mfma", achieved=round(achieved, 1), peak=2500.0, unit="TFLOP/s",
                 frac=round(achieved / 2500.0, 4), traffic=TRAFFIC_CNN_MID,
                 traffic_source=TRAFFIC_CNN_MID_SOURCE,
-                kernel="k_cnn_mid_rows<0,5>: warm-start CNN middle layer (32 -> 32 channels, 3^3 dilated taps, "
+                kernel="k_cnn_mid_ring: warm-start CNN middle layer (32 -> 32 channels, 3^3 dilated taps, "
                        "BF16 MFMA with FP32 accumulation) at 160^3",
                 algorithmic_flop=flop_mid, algorithmic_bytes=2 * 64 * npts, avg_ms=round(mid_ms, 5),
                 per_step_ms=round(4 * mid_ms, 4), cnn_layer_ms={k: round(v, 5) for k, v in cnn_ms.items()},
