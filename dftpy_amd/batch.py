@@ -28,7 +28,7 @@ import time
 import numpy as np
 import torch
 
-from . import DirectGrid, systems
+from . import DirectField, DirectGrid, systems
 from .functional import Functional, TotalFunctional
 from .functional.pseudo import LocalPseudo, ReadPseudo
 from .learned import WarmStartNet, superposition_density
@@ -53,7 +53,8 @@ def make_cells(ncell=48, seed=0):
 
 
 class _Worker:
-    def __init__(self, readpp, weights=None):
+    def __init__(self, readpp, weights=None, check64=False):
+        self.check64 = check64
         self.stream = torch.cuda.Stream()
         self.grids = {}
         self.readpp = readpp
@@ -82,19 +83,38 @@ class _Worker:
             start = self.net.start_density(g32, x0, rho0, ne)
             opt = Optimization("TN", optimization_options={"econv": 1e-6 * ions.nat, "maxiter": 60, "precond": True},
                                EnergyEvaluator=ev)
-            opt.optimize_rho(start)
+            rho = opt.optimize_rho(start)
+            e64 = None
+            if self.check64:  # the FP64 energy of the FP32 result
+                e64 = float(ev(DirectField(g64, data=rho.data.double()), calcType={"E"}).energy)
             self.stream.synchronize()
             # status 0: the reference stop rule
             return dict(nat=ions.nat, nr=list(nr), energy=float(opt.stats["energy"]), niter=opt.stats["niter"],
                         status=opt.stats["converged"], evals=opt.stats["n_eval_v"] + opt.stats["n_eval_ev"],
-                        seconds=time.perf_counter() - t0)
+                        seconds=time.perf_counter() - t0, energy64=e64)
 
 
-def run_batch(cells, workers=4, weights=None):
-    """Optimise all cells with `workers` concurrent streams; returns the
-    per-cell records (in input order) and the wall time."""
+def fp64_reference(ions, nr, econv_atom=1e-9):
+    """The cell's ground-state energy from the FP64 TN (the reference's
+    algorithm, FD Hessian) from the uniform density, converged tightly."""
     readpp = ReadPseudo({"Al": systems.empty_core_recpot()})
-    ws = [_Worker(readpp, weights) for _ in range(workers)]
+    g64 = DirectGrid(ions.cell, nr, full=False, dtype=torch.float64)
+    ev = TotalFunctional(KE=Functional(type="KEDF", name="WT"), XC=Functional(type="XC", name="LDA"),
+                         H=Functional(type="HARTREE"), PSEUDO=LocalPseudo(grid=g64, ions=ions, readpp=readpp))
+    rho = DirectField(g64)
+    rho[:] = ions.get_ncharges() / g64.volume
+    opt = Optimization("TN", optimization_options={"econv": econv_atom * ions.nat, "maxiter": 100},
+                       EnergyEvaluator=ev)
+    opt.optimize_rho(rho)
+    return float(opt.stats["energy"]), int(opt.stats["converged"])
+
+
+def run_batch(cells, workers=4, weights=None, check64=False):
+    """Optimise all cells with `workers` concurrent streams; returns the
+    per-cell records (in input order) and the wall time.  check64 adds each
+    result's FP64 energy (`energy64`, inside the timed region)."""
+    readpp = ReadPseudo({"Al": systems.empty_core_recpot()})
+    ws = [_Worker(readpp, weights, check64) for _ in range(workers)]
     # warm every worker on every shape outside the timed region: plans
     # (autotuning, tables) and one full cell, so the device buffers of each
     # shape are in the stream's allocator cache and no allocation (which

@@ -170,17 +170,59 @@ def evaluate_gain(fac, net, seeds, estar, log):
     return rec
 
 
-def summarize(rec):
+def evaluate_starts(fac, net, seeds, estar, log, starts=("uniform", "superposition", "learned")):
+    """The benchmark's form on held-out cells: FP32 TN with the preconditioned
+    inner CG and the reference stop rule from each start; per-cell records."""
+    from . import DirectField
+    rec = []
+    for i, seed in enumerate(seeds):
+        ions, ev, ne, rho0, x0 = fac.cell(seed, net)
+        nat = ions.nat
+        r = dict(seed=int(seed), e_star=float(estar[i]))
+        for name in starts:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "uniform":
+                start = DirectField(fac.g32)
+                start[:] = ne / fac.g32.volume
+            else:
+                start = net.start_density(fac.g32, x0, rho0, ne, use_net=name == "learned")
+            torch.cuda.synchronize()
+            t_start = time.perf_counter() - t0
+            e_start = energy64(ev, fac.g64, start)
+            opt = Optimization(optimization_method="TN", EnergyEvaluator=ev,
+                               optimization_options={"econv": 1e-6 * nat, "maxiter": 60, "precond": True})
+            t1 = time.perf_counter()
+            rho = opt.optimize_rho(start)
+            torch.cuda.synchronize()
+            t_opt = time.perf_counter() - t1
+            e_end = energy64(ev, fac.g64, rho)
+            r[name] = dict(start_err_per_atom=(e_start - estar[i]) / nat, final_err_per_atom=(e_end - estar[i]) / nat,
+                           niter=opt.stats["niter"], n_eval=opt.stats["n_eval_v"] + opt.stats["n_eval_ev"],
+                           status=opt.stats["converged"], seconds=t_start + t_opt, start_seconds=t_start)
+        rec.append(r)
+        if (i + 1) % 20 == 0 or i == 0:
+            log("held-out %d/%d: " % (i + 1, len(seeds)) +
+                ", ".join("%s %d iters %.1f ms" % (n, r[n]["niter"], 1e3 * r[n]["seconds"]) for n in starts))
+    return rec
+
+
+def summarize(rec, names=("superposition", "learned")):
     out = {}
-    for name in ("superposition", "learned"):
+    for name in names:
         a = lambda k: np.array([r[name][k] for r in rec], dtype=float)  # noqa: E731
         out[name] = dict(niter_mean=float(a("niter").mean()), n_eval_mean=float(a("n_eval").mean()),
                          seconds_mean=float(a("seconds").mean()),
                          start_err_per_atom_median=float(np.median(a("start_err_per_atom"))),
                          final_err_per_atom_max=float(np.abs(a("final_err_per_atom")).max()),
-                         within_1e6_fraction=float(np.mean(np.abs(a("final_err_per_atom")) <= 1e-6)))
-    out["time_ratio"] = out["superposition"]["seconds_mean"] / out["learned"]["seconds_mean"]
-    out["eval_ratio"] = out["superposition"]["n_eval_mean"] / out["learned"]["n_eval_mean"]
+                         within_1e6_fraction=float(np.mean(np.abs(a("final_err_per_atom")) <= 1e-6)),
+                         status0_fraction=float(np.mean(a("status") == 0)) if "status" in rec[0][name] else None)
+    for base in names:
+        if base != "learned" and "learned" in names:
+            out["time_ratio" if base == "superposition" else "time_ratio_vs_" + base] = (
+                out[base]["seconds_mean"] / out["learned"]["seconds_mean"])
+            out["eval_ratio" if base == "superposition" else "eval_ratio_vs_" + base] = (
+                out[base]["n_eval_mean"] / out["learned"]["n_eval_mean"])
     return out
 
 
@@ -194,11 +236,33 @@ def main(argv=None):
     ap.add_argument("--dil", default="1,2,4,4,2,1")
     ap.add_argument("--c", type=float, default=0.7)
     ap.add_argument("--loss", choices=("metric", "mse"), default="metric")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="no training: evaluate the shipped network on the held-out cells with the preconditioned "
+                         "optimizer from the uniform, superposition and learned starts, and add the result to "
+                         "the shipped evaluation record (written to --out)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
 
     def log(msg):
         print(msg, flush=True)
+
+    if args.eval_only:
+        net = WarmStartNet.load()
+        fac = CellFactory()
+        te_seeds = list(range(10000, 10000 + args.ntest))
+        _, _, estar = generate(fac, net, te_seeds, log)
+        rec = evaluate_starts(fac, net, te_seeds, estar, log)
+        summ = summarize(rec, ("uniform", "superposition", "learned"))
+        log(json.dumps(summ))
+        src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "warmstart_eval.json")
+        with open(src) as f:
+            shipped = json.load(f)
+        shipped["preconditioned"] = dict(summary=summ, cells=rec,
+                                         form="FP32 TN, preconditioned inner CG, reference stop rule "
+                                              "(the benchmark's form)")
+        with open(os.path.join(args.out, "warmstart_eval.json"), "w") as f:
+            json.dump(shipped, f, indent=1)
+        return 0
 
     net = WarmStartNet(dil=tuple(int(d) for d in args.dil.split(",")), c=args.c, sv=1.0, seed=0)
     fac = CellFactory()

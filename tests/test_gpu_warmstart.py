@@ -31,3 +31,37 @@ def test_learned_start_gains_over_superposition(seed):
     assert 0 <= lrn["start_err"] < 0.05 * sup["start_err"]
     assert lrn["evals"] < sup["evals"]
     assert abs(lrn["final_err"]) <= 1e-6 and abs(sup["final_err"]) <= 1e-6
+
+
+@pytest.mark.parametrize("seed", [10003])
+def test_learned_start_gains_over_uniform_with_preconditioner(seed):
+    """Against the uniform density, both runs with the preconditioned inner CG
+    (the benchmark's form): the learned start needs fewer evaluations and
+    both reach the same minimum."""
+    import torch
+    from dftpy_amd import DirectField
+    from dftpy_amd.learned import WarmStartNet
+    from dftpy_amd.optimization import Optimization
+    from dftpy_amd.warmstart_train import CellFactory, converge, energy64
+    net = WarmStartNet.load()
+    fac = CellFactory()
+    ions, ev, ne, rho0, x0 = fac.cell(seed, net)
+    nat = ions.nat
+    ref_rho, _, _ = converge(ev, net.start_density(fac.g32, x0, rho0, ne, use_net=True), 1e-9, nat, 100)
+    estar = energy64(ev, fac.g64, ref_rho)
+    uniform = DirectField(fac.g32)
+    uniform[:] = ne / fac.g32.volume
+    res = {}
+    for name, start in (("uniform", uniform), ("learned", net.start_density(fac.g32, x0, rho0, ne, use_net=True))):
+        opt = Optimization(optimization_method="TN", EnergyEvaluator=ev,
+                           optimization_options={"econv": 1e-6 * nat, "maxiter": 60, "precond": True})
+        torch.cuda.synchronize()
+        rho = opt.optimize_rho(start)
+        e1 = energy64(ev, fac.g64, rho)
+        res[name] = dict(final_err=(e1 - estar) / nat, iters=opt.stats["niter"], status=opt.stats["converged"],
+                         evals=opt.stats["n_eval_v"] + opt.stats["n_eval_ev"])
+    print(seed, res)
+    uni, lrn = res["uniform"], res["learned"]
+    assert uni["status"] == 0 and lrn["status"] == 0
+    assert lrn["evals"] < uni["evals"]
+    assert abs(lrn["final_err"]) <= 1e-6 and abs(uni["final_err"]) <= 1e-6
