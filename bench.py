@@ -91,9 +91,10 @@ def run_fp64_plain(ev, rho64, e_ref, tol, nat, precond=False):
     return time_to_threshold(opt, wall, e_ref, tol), wall, opt
 
 
-def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat, precond=False):
+def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat, precond=False, options=None):
     """FP32-bulk TN from the density start_fn() returns; the start density is
-    built inside the timed region.  Returns (seconds, info)."""
+    built inside the timed region (options: more optimizer options, e.g. the
+    device-driven iteration).  Returns (seconds, info)."""
     import torch
     from dftpy_amd import DirectField
     from dftpy_amd.optimization import Optimization
@@ -106,8 +107,8 @@ def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat, precond=False):
     rho_start32 = start_fn()
     ev1.record()
     opt = Optimization(optimization_method="TN", EnergyEvaluator=ev,
-                       optimization_options={"econv": ECONV * nat, "maxiter": 60, "hvp": "analytic",
-                                             "precond": precond})
+                       optimization_options=dict({"econv": ECONV * nat, "maxiter": 60, "hvp": "analytic",
+                                                  "precond": precond}, **(options or {})))
     r32 = opt.optimize_rho(rho_start32)
     r64 = DirectField(g64, data=r32.data.double())
     e64 = ev(r64, calcType={"E"}).energy

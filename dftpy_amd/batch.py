@@ -1,6 +1,6 @@
 """Many cells of uneven grid sizes optimised concurrently on one GPU.
 
-    python -m dftpy_amd.batch [--ncell 48] [--workers 4] [--seed 0] [--sequential]
+    python -m dftpy_amd.batch [--ncell 48] [--workers 4] [--seed 0] [--sequential] [--graph]
 
 Cells: fcc Al n x n x m supercells with n, m drawn from {2, 3, 4} (32-256 atoms),
 isotropic strain U(-3%, +3%) and Gaussian displacements of sigma = 0.1 bohr,
@@ -8,7 +8,8 @@ grid spacing ~0.24 bohr rounded up to even 2/3/5-smooth sizes (64-144 per
 axis); TF + vW + WT + PZ-LDA + Hartree + the empty-core local PP.  Each cell
 is optimised by the FP32-bulk TN (uniform-gas preconditioned inner CG) from
 the learned start with the reference stop rule (|dE| < 1e-6 Ha/atom * nat
-over 2 steps).
+over 2 steps); with --graph the TN iterations run device-driven, one
+captured hipGraph launch per superstep (ofdft_optimize_tn_graph).
 
 Execution: W worker threads, each with its own HIP stream, its own plans (one
 per grid shape, autotuned at creation and moved to each cell's strained
@@ -53,8 +54,9 @@ def make_cells(ncell=48, seed=0):
 
 
 class _Worker:
-    def __init__(self, readpp, weights=None, check64=False):
+    def __init__(self, readpp, weights=None, check64=False, graph=False):
         self.check64 = check64
+        self.graph = graph
         self.stream = torch.cuda.Stream()
         self.grids = {}
         self.readpp = readpp
@@ -81,7 +83,8 @@ class _Worker:
             rho_s = superposition_density(g64, ions, self.readpp, rho0)
             x0 = self.net.features(g64.plan(torch.float64), rho_s, pp.vreal_for(g64).data, rho0)
             start = self.net.start_density(g32, x0, rho0, ne)
-            opt = Optimization("TN", optimization_options={"econv": 1e-6 * ions.nat, "maxiter": 60, "precond": True},
+            opt = Optimization("TN", optimization_options={"econv": 1e-6 * ions.nat, "maxiter": 60, "precond": True,
+                                                           "graph": self.graph},
                                EnergyEvaluator=ev)
             rho = opt.optimize_rho(start)
             e64 = None
@@ -109,12 +112,13 @@ def fp64_reference(ions, nr, econv_atom=1e-9):
     return float(opt.stats["energy"]), int(opt.stats["converged"])
 
 
-def run_batch(cells, workers=4, weights=None, check64=False):
+def run_batch(cells, workers=4, weights=None, check64=False, graph=False):
     """Optimise all cells with `workers` concurrent streams; returns the
     per-cell records (in input order) and the wall time.  check64 adds each
-    result's FP64 energy (`energy64`, inside the timed region)."""
+    result's FP64 energy (`energy64`, inside the timed region); graph runs the
+    device-driven TN."""
     readpp = ReadPseudo({"Al": systems.empty_core_recpot()})
-    ws = [_Worker(readpp, weights, check64) for _ in range(workers)]
+    ws = [_Worker(readpp, weights, check64, graph) for _ in range(workers)]
     # warm every worker on every shape outside the timed region: plans
     # (autotuning, tables) and one full cell, so the device buffers of each
     # shape are in the stream's allocator cache and no allocation (which
@@ -173,18 +177,19 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--sequential", action="store_true", help="also time the cells one after another")
+    ap.add_argument("--graph", action="store_true", help="device-driven TN (one hipGraph launch per superstep)")
     args = ap.parse_args(argv)
     cells = make_cells(args.ncell, args.seed)
-    res, wall = run_batch(cells, args.workers)
+    res, wall = run_batch(cells, args.workers, graph=args.graph)
     line = dict(workload="%d fcc Al cells (n,n,m in {2,3,4}, 32-256 atoms, strain +-3%%, sigma 0.1), grids %s" %
                 (len(cells), sorted({tuple(r["nr"]) for r in res})),
-                workers=args.workers, wall_s=wall, cells_per_s=len(cells) / wall,
+                workers=args.workers, graph=args.graph, wall_s=wall, cells_per_s=len(cells) / wall,
                 atoms=int(sum(r["nat"] for r in res)), mean_iters=float(np.mean([r["niter"] for r in res])),
                 mean_evals=float(np.mean([r["evals"] for r in res])),
                 status_counts={str(k): int(sum(r["status"] == k for r in res)) for k in sorted({r["status"] for r in res})},
                 all_converged=all(r["status"] == 0 for r in res))
     if args.sequential:
-        res1, wall1 = run_batch(cells, 1)
+        res1, wall1 = run_batch(cells, 1, graph=args.graph)
         line.update(sequential_wall_s=wall1, concurrency_speedup=wall1 / wall,
                     max_energy_diff_vs_sequential=float(max(abs(a["energy"] - b["energy"]) for a, b in zip(res, res1))))
     print(json.dumps(line), flush=True)

@@ -17,6 +17,11 @@ static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
 
+std::recursive_mutex &legacy_lock() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
               const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, double *vreal,
@@ -107,6 +112,7 @@ const char *ofdft_last_error(void) { return g_err.c_str(); }
 int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, int full, int max_batch,
                       ofdft_plan **out) {
     return guard([&] {
+        OFDFT_LEGACY_LOCK;
         OFDFT_REQUIRE(out != nullptr, "plan_create: null output pointer");
         OFDFT_REQUIRE(precision == OFDFT_FP64 || precision == OFDFT_FP32, "precision must be FP64 or FP32");
         for (int d = 0; d < 3; ++d) OFDFT_REQUIRE(nr[d] >= 1 && nr[d] <= 4096, "grid size out of range");
@@ -141,10 +147,10 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             OFDFT_HIP(hipMalloc(&P->work, work_bytes));
             P->red_blocks = 4096;
             OFDFT_HIP(hipMalloc(&P->red, sizeof(double) * 4096 * 16));
-            OFDFT_HIP(hipMalloc(&P->scal, sizeof(double) * 256));
-            OFDFT_HIP(hipMemset(P->scal, 0, sizeof(double) * 256));
+            OFDFT_HIP(hipMalloc(&P->scal, sizeof(double) * kScalSlots));
+            OFDFT_HIP(hipMemset(P->scal, 0, sizeof(double) * kScalSlots));
             OFDFT_HIP(hipHostMalloc((void **)&P->pinned, 128 * sizeof(double), hipHostMallocDefault));
-            P->ws_bytes = spec_bytes + work_bytes + sizeof(double) * (4096 * 16 + 256);
+            P->ws_bytes = spec_bytes + work_bytes + sizeof(double) * (4096 * 16 + kScalSlots);
             plan_aux_setup(*P);
             P->ws_bytes += P->aux_bytes;
             OFDFT_HIP(hipDeviceSynchronize());
@@ -190,6 +196,7 @@ int ofdft_plan_set_lattice(ofdft_plan *plan, const double lattice[9]) {
 int ofdft_plan_destroy(ofdft_plan *P) {
     if (!P) return 0;
     return guard([&] {
+        OFDFT_LEGACY_LOCK;
         plan_fft_free(*P);
         plan_aux_free(*P);
         lmgp_free(*P);
@@ -209,6 +216,7 @@ size_t ofdft_plan_workspace_bytes(const ofdft_plan *P) { return P ? P->ws_bytes 
 
 int ofdft_set_tn_precond(ofdft_plan *P, int n, double dq, const double *table) {
     return guard([&] {
+        OFDFT_LEGACY_LOCK;
         OFDFT_REQUIRE(P != nullptr, "set_tn_precond: null plan");
         if (n <= 0) {
             P->tn_pre_n = 0;
@@ -230,6 +238,7 @@ int ofdft_set_tn_precond(ofdft_plan *P, int n, double dq, const double *table) {
 
 int ofdft_plan_reserve_lbfgs(ofdft_plan *P, int maxcor) {
     return guard([&] {
+        OFDFT_LEGACY_LOCK;
         OFDFT_REQUIRE(P != nullptr, "plan_reserve_lbfgs: null plan");
         OFDFT_REQUIRE(maxcor >= 0 && maxcor + 1 <= 16, "plan_reserve_lbfgs: maxcor must be in [0, 15]");
         const int pairs = maxcor + 1;

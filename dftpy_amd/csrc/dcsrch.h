@@ -1,9 +1,9 @@
-// Moré-Thuente line search state machine (host).
+// Moré-Thuente line search state machine (host entry; the algorithm is in
+// dcsrch_core.h, shared with the device-driven optimizer iteration).
 #pragma once
+#include "dcsrch_core.h"
 
 namespace ofdft {
-
-enum LsTask { LS_START = 0, LS_FG = 1, LS_CONVERGENCE = 2, LS_WARNING = 3, LS_ERROR = 4 };
 
 // One reverse-communication step: with task LS_START initialise from
 // (f, g) at stp = 0 and the first trial step stp; afterwards pass the value

@@ -225,6 +225,7 @@ extern "C" int ofdft_atomic_density(ofdft_plan *plan, int kind, int nspecies, co
                                     const double *rtol, const double *sigma, const double *scale, int nat,
                                     const double *pos_frac, const int *species, void *rho, void *stream) {
     Plan *P = reinterpret_cast<Plan *>(plan);
+    OFDFT_LEGACY_LOCK;  // the upload buffer is allocated and freed per call
     struct DevBuf {
         void *p = nullptr;
         ~DevBuf() {

@@ -25,7 +25,8 @@
 namespace ofdft {
 
 template <typename T>
-__global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
+__global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p, const double *gate) {
+    if (gate_off(gate)) return;
     // nc counts the pitched layout; columns iz >= nzc are padding
     const double rho0 = p.rho0_dev ? *p.rho0_dev : p.rho0;
     const double tkf = 2.0 * cbrt(3.0 * rho0 * M_PI * M_PI);
@@ -86,7 +87,8 @@ __global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p) {
 
 // rho0 = sum / N with the kernel-reuse threshold of wt.py:64
 __global__ void __launch_bounds__(kFinalThreads)
-    k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache) {
+    k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache, const double *gate) {
+    if (gate_off(gate)) return;
     __shared__ double sm[kFinalThreads];
     sum_columns(partials, nb, 1, sm);
     if (threadIdx.x == 0) {
@@ -104,7 +106,7 @@ static int grid_blocks(long long n, int per = 256 * 4) {
 
 template <typename T>
 bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
-                  long long vp, const void *tw, const GParams &gp, hipStream_t s);
+                  long long vp, const void *tw, const GParams &gp, hipStream_t s, const double *gate);
 
 // the WT exponents take the single-spectrum form of wt.py:18-22
 static bool wt_alpha_is_beta(const Plan &P) { return std::fabs(P.alpha - P.beta) < 1e-9; }
@@ -125,10 +127,11 @@ static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long
     if (gp.s_wt2 >= 0) gp.kind[gp.s_wt2] = 1;
     if (fused) {
         const long long ncols = (long long)P.nr[1] * pitch;
-        OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], P.var_cols[0], spec, ns, ncp, ncols, pitch, P.nzc, P.tw[0], gp, s),
+        OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], P.var_cols[0], spec, ns, ncp, ncols, pitch, P.nzc, P.tw[0], gp, s,
+                                      P.gate),
                       "fused x pass unavailable");
     } else {
-        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp);
+        hipLaunchKernelGGL((k_gspace<T>), dim3(grid_blocks(ncp)), dim3(256), 0, s, spec, ncp, gp, P.gate);
     }
     OFDFT_HIP(hipGetLastError());
 }
@@ -136,38 +139,54 @@ static void gspace_apply(Plan &P, cpx<T> *spec, int ns, long long ncp, long long
 // Energy-only evaluation: the spectral terms as sums over the half spectrum,
 // E = sum_G w(kz) f(G) |F(G)|^2 / N (Parseval; w = 2 for the kz columns whose
 // mirror is not stored), with f the same multiplier the potential path
-// applies.  One workgroup per stride of (x, y) rows, threads along kz.  The
+// applies.  A flat grid stride over the stored points.  The
 // sums land in the k_post partial columns of vW (times y), WT and Hartree
 // (times 1/2), which k_post leaves at zero when it has no fields.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gspace_energy(const cpx<T> *__restrict__ spec, int ns, GParams p,
-                                                       double y, double *partials) {
+                                                       double y, double *partials, const double *gate) {
+    if (gate_off(gate)) return;
     __shared__ double sm[16];
     double tkf = 0.0, wtfac = 0.0;
     if (p.s_wt >= 0) wt_consts(p, tkf, wtfac);
     double acc_vw = 0.0, acc_wt = 0.0, acc_h = 0.0;
     // flat grid-stride over the stored half spectrum (every lane busy; rows
-    // are shorter than a workgroup)
-    const long long total = (long long)p.nx * p.ny * p.nzc;
-    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int row = (int)(idx / p.nzc), iz = (int)(idx - (long long)row * p.nzc);
-        const int ix = row / p.ny, iy = row - ix * p.ny;
-        {
-            const double w = (iz == 0 || ((p.nz & 1) == 0 && iz == p.nz / 2)) ? 1.0 : 2.0;
-            const long long e = (long long)row * p.pitch + iz;
-            for (int sl = 0; sl < ns; ++sl) {
-                const int kind = p.kind[sl];
-                const cpx<T> v = spec[sl * p.bs + e];
-                const double a2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-                const double t = w * gfactor<T>(p, kind, ix, iy, iz, tkf, wtfac) * a2;
-                if (kind == 0)
-                    acc_vw += t;
-                else if (kind == 1)
-                    acc_wt += t;
-                else
-                    acc_h += t;
-            }
+    // are shorter than a workgroup), 32-bit index arithmetic (the half
+    // spectrum of a plan has far fewer than 2^31 points); |G|^2 once per
+    // point for all spectra
+    const unsigned nzc = (unsigned)p.nzc, ny = (unsigned)p.ny;
+    const unsigned total = (unsigned)p.nx * ny * nzc;
+    for (unsigned idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+        const unsigned row = idx / nzc, iz = idx - row * nzc;
+        const unsigned ix = row / ny, iy = row - ix * ny;
+        const int fx = ffreq((int)ix, p.nx), fy = ffreq((int)iy, p.ny);
+        const bool g0 = (idx == 0);
+        double gg, gg2 = 0.0;
+        bool nyq = false;
+        if (p.full) {
+            const int fz = wrapf((int)iz, p.nz);
+            gg = gsq(p.B, fx, fy, fz);
+            nyq = ((p.nx & 1) == 0 && (int)ix == p.nx / 2) || ((p.ny & 1) == 0 && (int)iy == p.ny / 2) ||
+                  ((p.nz & 1) == 0 && (int)iz == p.nz / 2);
+            if (nyq) gg2 = gsq(p.B, wrapf(-fx, p.nx), wrapf(-fy, p.ny), wrapf(-fz, p.nz));
+        } else {
+            gg = gsq(p.B, fx, fy, (int)iz);
+        }
+        const double w = (iz == 0 || ((p.nz & 1) == 0 && (int)iz == p.nz / 2)) ? 1.0 : 2.0;
+        const long long e = (long long)row * p.pitch + iz;
+        for (int sl = 0; sl < ns; ++sl) {
+            const int kind = p.kind[sl];
+            const cpx<T> v = spec[sl * p.bs + e];
+            const double a2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            double f = kernel_val<T>(p, kind, gg, g0, tkf, wtfac);
+            if (nyq) f = 0.5 * (f + kernel_val<T>(p, kind, gg2, g0, tkf, wtfac));
+            const double t = w * (f * p.scale) * a2;
+            if (kind == 0)
+                acc_vw += t;
+            else if (kind == 1)
+                acc_wt += t;
+            else
+                acc_h += t;
         }
     }
     const double sums[3] = {block_sum(acc_vw, sm), block_sum(acc_wt, sm), block_sum(acc_h, sm)};
@@ -202,7 +221,8 @@ template <typename CT> __device__ __forceinline__ CT rsq_p(CT x) { return fm::rs
 
 template <typename T, typename CT, int VEC>
 __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, const T *__restrict__ vloc,
-                       T *__restrict__ pot, double *partials, PostParams p) {
+                       T *__restrict__ pot, double *partials, PostParams p, const double *gate) {
+    if (gate_off(gate)) return;
     const CT ctf = (CT)(0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0));
     const CT cbrt34pi = (CT)cbrt(3.0 / (4.0 * M_PI));
     const CT cx_e = (CT)(-3.0 / 4.0 * cbrt(3.0 / M_PI));
@@ -302,17 +322,19 @@ __global__ void k_post(const T *__restrict__ rho, const T *__restrict__ work, co
 // k_post with the element width the field length allows (stream.h)
 template <typename T>
 static void launch_post(int nb, hipStream_t s, const T *rho, const T *work, const T *vloc, T *pot, double *partials,
-                        const PostParams &pp) {
+                        const PostParams &pp, const double *gate) {
     if (pw_vec(pp.n) == 4 && pw_vec(pp.bs) == 4 && pw_aligned({rho, work, vloc, pot, pp.core}))
-        hipLaunchKernelGGL((k_post<T, T, 4>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, partials, pp);
+        hipLaunchKernelGGL((k_post<T, T, 4>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, partials, pp, gate);
     else
-        hipLaunchKernelGGL((k_post<T, T, 1>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, partials, pp);
+        hipLaunchKernelGGL((k_post<T, T, 1>), dim3(nb), dim3(256), 0, s, rho, work, vloc, pot, partials, pp, gate);
 }
 
 // energies[0..5] = term sums * dV (TF also * x), [6] = their sum + E_ion,
 // [7] = electron count
 __global__ void __launch_bounds__(kFinalThreads)
-    k_energy_final(const double *__restrict__ partials, int nb, double *out, double dV, double x, double eion) {
+    k_energy_final(const double *__restrict__ partials, int nb, double *out, double dV, double x, double eion,
+                   const double *gate) {
+    if (gate_off(gate)) return;
     __shared__ double sm[kFinalThreads];
     sum_columns(partials, nb, 7, sm);
     if (threadIdx.x == 0) {
@@ -347,7 +369,8 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch, false);
     if (auto_rho0) {
         const int nb0 = r2c_num_partials(P, ns);
-        hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb0, 1.0 / (double)P.N, rho0_cache);
+        hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb0, 1.0 / (double)P.N, rho0_cache,
+                           P.gate);
         OFDFT_HIP(hipGetLastError());
     }
     GParams gp;
@@ -388,11 +411,13 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     pp.want_e = 1;
     const T *vloc = (P.terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
     const int nb = std::min(reduce_blocks(P.N), 1024);
-    launch_post<T>(nb, s, rho, nullptr, vloc, nullptr, P.red, pp);
+    launch_post<T>(nb, s, rho, nullptr, vloc, nullptr, P.red, pp, P.gate);
     OFDFT_HIP(hipGetLastError());
-    hipLaunchKernelGGL((k_gspace_energy<T>), dim3(nb), dim3(256), 0, s, spec, ns, gp, P.y, P.red);
+    OFDFT_REQUIRE((long long)P.nr[0] * P.nr[1] * P.nzc < (1LL << 31), "energy pass: grid too large");
+    hipLaunchKernelGGL((k_gspace_energy<T>), dim3(nb), dim3(256), 0, s, spec, ns, gp, P.y, P.red, P.gate);
     OFDFT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion,
+                       P.gate);
     OFDFT_HIP(hipGetLastError());
 }
 
@@ -475,7 +500,8 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         fft_r2c<T>(P, rho, spec, ns, 0, ncp, 1.0, &pro, auto_rho0 ? P.red : nullptr, s, pitch, fused);
         if (auto_rho0) {
             const int nb = r2c_num_partials(P, ns);
-            hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, 1.0 / (double)P.N, rho0_cache);
+            hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, 1.0 / (double)P.N,
+                               rho0_cache, P.gate);
             OFDFT_HIP(hipGetLastError());
         }
         int nsa = ns;
@@ -495,14 +521,16 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
                                   (calc & OFDFT_CALC_E) && energies != nullptr, s_wt2);
     const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
     const int nb = std::min(reduce_blocks(P.N), 1024);
-    launch_post<T>(nb, s, rho, work, vloc, pot, P.red, pp);
+    launch_post<T>(nb, s, rho, work, vloc, pot, P.red, pp, P.gate);
     OFDFT_HIP(hipGetLastError());
     if (pp.want_e) {
-        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion);
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, energies, P.dV, P.x, P.eion,
+                           P.gate);
         OFDFT_HIP(hipGetLastError());
     }
     if (terms & OFDFT_TERM_LMGP) {
         OFDFT_REQUIRE(P.lmgp != nullptr, "LMGP term without tables (ofdft_set_lmgp)");
+        OFDFT_REQUIRE(P.gate == nullptr, "gated launch of the LMGP evaluation");
         lmgp_eval<T>(P, rho, pp.want_v ? pot : nullptr, pp.want_e ? energies : nullptr, s);
     }
 }
@@ -592,6 +620,7 @@ void eval_fields(Plan &P, const T *rho, unsigned calc, T *pot, double *energies,
     fp.beta = P.beta;
     fp.n = P.N;
     fp.bs = P.N;
+    OFDFT_REQUIRE(P.gate == nullptr, "gated launch of the field evaluation");
     hipLaunchKernelGGL((k_fields<T>), dim3(grid_blocks(P.N)), dim3(256), 0, s, rho,
                        reinterpret_cast<const T *>(P.work), edens, v2, fp);
     OFDFT_HIP(hipGetLastError());
@@ -620,8 +649,7 @@ template <typename T> void bench_eval_stage(Plan &P, int stage, const T *rho, T 
         PostParams pp = pipeline_post(P, s_vw, s_wt, s_h, true, true);
         const T *vloc = (terms & OFDFT_TERM_PP) ? reinterpret_cast<const T *>(P.vloc) : nullptr;
         const int nb = std::min(reduce_blocks(P.N), 1024);
-        launch_post<T>(nb, s, rho, reinterpret_cast<const T *>(P.work),
-                           vloc, pot, P.red, pp);
+        launch_post<T>(nb, s, rho, reinterpret_cast<const T *>(P.work), vloc, pot, P.red, pp, P.gate);
     }
     OFDFT_HIP(hipGetLastError());
 }
@@ -684,7 +712,8 @@ template <typename T, typename CT, int VEC>
 __global__ void k_hvp(const T *__restrict__ rho, const T *__restrict__ phi,
                       const T *__restrict__ p, const T *__restrict__ pot, const T *__restrict__ work,
                       const T *__restrict__ base_a, const T *__restrict__ base_c, const double *__restrict__ mu,
-                      T *__restrict__ Ap, double *partials, PostParams pp) {
+                      T *__restrict__ Ap, double *partials, PostParams pp, const double *gate) {
+    if (gate_off(gate)) return;
     const CT ctf = (CT)(0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0));
     const CT cbrt34pi = (CT)cbrt(3.0 / (4.0 * M_PI));
     const CT cx3 = (CT)(-cbrt(3.0 / M_PI) / 3.0);
@@ -779,10 +808,10 @@ void hvp(Plan &P, const T *rho, const T *phi, const T *p, const T *pot, const T 
     pp.want_e = 0;
     if (pw_vec(pp.n) == 4 && pw_aligned({rho, phi, p, pot, P.work, base_a, base_c, Ap}))
         hipLaunchKernelGGL((k_hvp<T, T, 4>), dim3(nb), dim3(256), 0, s, rho, phi, p, pot,
-                           reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
+                           reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp, P.gate);
     else
         hipLaunchKernelGGL((k_hvp<T, T, 1>), dim3(nb), dim3(256), 0, s, rho, phi, p, pot,
-                           reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp);
+                           reinterpret_cast<const T *>(P.work), base_a, base_c, mu, Ap, P.red, pp, P.gate);
     OFDFT_HIP(hipGetLastError());
 }
 

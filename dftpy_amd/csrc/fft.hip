@@ -286,14 +286,15 @@ __global__ void k_rows_c2c(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ o
 bool fixed_size(int n);
 template <typename T>
 bool cols_fixed(int n, int var, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols,
-                long long P, long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s);
+                long long P, long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s,
+                const double *gate);
 template <typename T>
 bool r2c_fixed(int n, int var, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
                long long nrows, long long opitch, const void *tw, double scale, const Prologue &pro, double *partials,
-               int sum_slot, hipStream_t s);
+               int sum_slot, hipStream_t s, const double *gate);
 template <typename T>
 bool c2r_fixed(int n, int var, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs,
-               long long nrows, long long ipitch, const void *tw, double scale, hipStream_t s);
+               long long nrows, long long ipitch, const void *tw, double scale, hipStream_t s, const double *gate);
 int r2c_fixed_tiles(int n, int var, int prec, int batch, long long nrows);
 
 int r2c_num_partials(const Plan &P, int batch) {
@@ -449,11 +450,13 @@ void fft_cols(Plan &P, int axis, bool full_grid, bool inverse, const cpx<T> *in,
             Pp = Qq = pitch;
             Ss = ncols;
         }
-        cols_fixed<T>(n, P.var_cols[axis], inverse, in, out, batch, bs, ncols, Pp, Qq, Ss, zlen, P.tw[axis], scale, s);
+        cols_fixed<T>(n, P.var_cols[axis], inverse, in, out, batch, bs, ncols, Pp, Qq, Ss, zlen, P.tw[axis], scale, s,
+                      P.gate);
         OFDFT_HIP(hipGetLastError());
         return;
     }
     OFDFT_REQUIRE(pitch == zlen, "generic column pass needs an unpadded layout");
+    OFDFT_REQUIRE(P.gate == nullptr, "gated launch of a generic FFT pass");
     long long M, outer;
     PassShape sh;
     if (axis == 1) {
@@ -499,8 +502,9 @@ void fft_r2c(Plan &P, const T *in, cpx<T> *out, int batch, long long in_bs, long
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
     const bool last_z = (P.nr[0] == 1 && P.nr[1] == 1);
     if (!(P.use_fixed && r2c_fixed<T>(P.nr[2], P.var_r2c, in, out, batch, in_bs, out_bs, nrows, pitch, P.tw[2],
-                                      last_z ? scale : 1.0, pr, sum_partials, sum_slot, s))) {
+                                      last_z ? scale : 1.0, pr, sum_partials, sum_slot, s, P.gate))) {
         OFDFT_REQUIRE(pitch == P.nzc, "generic R2C pass needs an unpadded layout");
+        OFDFT_REQUIRE(P.gate == nullptr, "gated launch of a generic FFT pass");
         dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
         hipLaunchKernelGGL((k_rows_r2c<T>), grid, dim3(sh.threads), sh.lds, s, in, out, in_bs, out_bs, nrows, P.nr[2],
                            P.nzc, P.ax[2], tw, sh.tile, (T)(last_z ? scale : 1.0), pr, sum_partials, sum_slot);
@@ -527,8 +531,10 @@ void fft_c2r(Plan &P, const cpx<T> *in, cpx<T> *tmp, T *out, int batch, long lon
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const PassShape sh = P.rows_c2r;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
-    if (!(P.use_fixed && c2r_fixed<T>(P.nr[2], P.var_c2r, src, out, batch, src_bs, out_bs, nrows, pitch, P.tw[2], scale, s))) {
+    if (!(P.use_fixed && c2r_fixed<T>(P.nr[2], P.var_c2r, src, out, batch, src_bs, out_bs, nrows, pitch, P.tw[2],
+                                      scale, s, P.gate))) {
         OFDFT_REQUIRE(pitch == P.nzc, "generic C2R pass needs an unpadded layout");
+        OFDFT_REQUIRE(P.gate == nullptr, "gated launch of a generic FFT pass");
         dim3 grid((unsigned)((nrows + sh.tile - 1) / sh.tile), (unsigned)batch);
         hipLaunchKernelGGL((k_rows_c2r<T>), grid, dim3(sh.threads), sh.lds, s, src, out, src_bs, out_bs, nrows,
                            P.nr[2], P.nzc, P.ax[2], tw, sh.tile, (T)scale);
@@ -545,6 +551,7 @@ void fft_c2c(Plan &P, const cpx<T> *in, cpx<T> *out, bool inverse, int batch, lo
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(P.tw[2]);
     const bool last_z = (P.nr[0] == 1 && P.nr[1] == 1);
     const T sz = (T)(last_z ? scale : 1.0);
+    OFDFT_REQUIRE(P.gate == nullptr, "gated launch of a generic FFT pass");
     if (inverse)
         hipLaunchKernelGGL((k_rows_c2c<T, true>), grid, dim3(sh.threads), sh.lds, s, in, out, bs, nrows, P.nr[2],
                            P.ax[2], tw, sh.tile, sz);
@@ -562,7 +569,7 @@ void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const long long pitch = pipe_pitch(P);
     if (P.use_fixed && r2c_fixed<T>(P.nr[2], P.var_r2c, in, out, batch, 0LL, (long long)P.nr[0] * P.nr[1] * pitch, nrows, pitch,
-                                    P.tw[2], 1.0, *pro, nullptr, -1, s)) {
+                                    P.tw[2], 1.0, *pro, nullptr, -1, s, nullptr)) {
         OFDFT_HIP(hipGetLastError());
         return;
     }
@@ -578,7 +585,7 @@ template <typename T> void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int
     const long long nrows = (long long)P.nr[0] * P.nr[1];
     const long long pitch = pipe_pitch(P);
     if (P.use_fixed && c2r_fixed<T>(P.nr[2], P.var_c2r, in, out, batch, (long long)P.nr[0] * P.nr[1] * pitch, P.N, nrows, pitch,
-                                    P.tw[2], 1.0, s)) {
+                                    P.tw[2], 1.0, s, nullptr)) {
         OFDFT_HIP(hipGetLastError());
         return;
     }

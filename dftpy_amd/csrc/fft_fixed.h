@@ -137,7 +137,8 @@ __device__ __forceinline__ void cstages(C *buf, const C *tw, const C *src, C *ds
 template <typename T, int N, int TILE, bool INV>
 __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     k_cols_f(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, long long bs, long long ncols, long long P,
-             long long Q, long long S, long long vp, const cpx<T> *__restrict__ twg, T scale) {
+             long long Q, long long S, long long vp, const cpx<T> *__restrict__ twg, T scale, const double *gate) {
+    if (gate_off(gate)) return;
     using C = cpx<T>;
     __shared__ C tw[N];
     __shared__ C buf[N * TILE];
@@ -174,7 +175,8 @@ __device__ __forceinline__ void cstages_io(C *buf, const C *tw, const C *src, C 
 template <typename T, int N, int TILE>
 __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     k_xfused_f(cpx<T> *__restrict__ data, long long bs, long long ncols, long long pitch, long long vp,
-               const cpx<T> *__restrict__ twg, GParams gp) {
+               const cpx<T> *__restrict__ twg, GParams gp, const double *gate) {
+    if (gate_off(gate)) return;
     using C = cpx<T>;
     __shared__ C tw[N];
     __shared__ C buf[N * TILE];
@@ -336,7 +338,8 @@ template <typename T, int N, int NPEN, int NF>
 __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
     k_rows_r2c_f(const T *__restrict__ in, cpx<T> *__restrict__ out, long long in_bs, long long out_bs, long long nrows,
                  long long opitch, const cpx<T> *__restrict__ twg, T scale, Prologue pro, double *partials,
-                 int sum_slot) {
+                 int sum_slot, const double *gate) {
+    if (gate_off(gate)) return;
     using C = cpx<T>;
     constexpr int NT = NF * NPEN * SL<N>::tpp;
     constexpr int ROWS = 2 * NPEN;
@@ -456,7 +459,8 @@ __global__ void __launch_bounds__(NF *NPEN *SL<N>::tpp)
 template <typename T, int N, int NPEN>
 __global__ void __launch_bounds__(NPEN *SL<N>::tpp)
     k_rows_c2r_f(const cpx<T> *__restrict__ in, T *__restrict__ out, long long in_bs, long long out_bs,
-                 long long nrows, long long ipitch, const cpx<T> *__restrict__ twg, T scale) {
+                 long long nrows, long long ipitch, const cpx<T> *__restrict__ twg, T scale, const double *gate) {
+    if (gate_off(gate)) return;
     using C = cpx<T>;
     constexpr int NT = NPEN * SL<N>::tpp;
     constexpr int ROWS = 2 * NPEN;
@@ -551,17 +555,18 @@ constexpr int kTileVariants = 3;  // variants 0, 1, 2 of every tiled pass
 
 template <typename T, int N>
 static void launch_cols(int var, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols,
-                        long long P, long long Q, long long S, long long vp, const cpx<T> *tw, T scale, hipStream_t s) {
+                        long long P, long long Q, long long S, long long vp, const cpx<T> *tw, T scale, hipStream_t s,
+                        const double *gate) {
 #define CALL(V)                                                                                                      \
     {                                                                                                                \
         constexpr int TILE = col_tile_v<T, N, V>();                                                                  \
         dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);                                           \
         if (inv)                                                                                                     \
             hipLaunchKernelGGL((k_cols_f<T, N, TILE, true>), grid, dim3(TILE * SL<N>::tpp), 0, s, in, out, bs, ncols, \
-                               P, Q, S, vp, tw, scale);                                                              \
+                               P, Q, S, vp, tw, scale, gate);                                                        \
         else                                                                                                         \
             hipLaunchKernelGGL((k_cols_f<T, N, TILE, false>), grid, dim3(TILE * SL<N>::tpp), 0, s, in, out, bs,       \
-                               ncols, P, Q, S, vp, tw, scale);                                                       \
+                               ncols, P, Q, S, vp, tw, scale, gate);                                                 \
     }
     OFDFT_VARIANT(var, CALL)
 #undef CALL
@@ -570,7 +575,7 @@ static void launch_cols(int var, bool inv, const cpx<T> *in, cpx<T> *out, int ba
 template <typename T, int N, int NF>
 static void launch_r2c_nf(int var, const T *in, cpx<T> *out, int nblk, long long in_bs, long long out_bs,
                           long long nrows, long long opitch, const cpx<T> *tw, T scale, const Prologue &pro,
-                          double *partials, int sum_slot, hipStream_t s) {
+                          double *partials, int sum_slot, hipStream_t s, const double *gate) {
 #define CALL(V)                                                                                                \
     {                                                                                                          \
         constexpr int NP = r2c_pens_v<T, N, NF, V>();                                                          \
@@ -578,7 +583,7 @@ static void launch_r2c_nf(int var, const T *in, cpx<T> *out, int nblk, long long
         OFDFT_REQUIRE(tiles <= 65535, "too many rows for the R2C pass");                                       \
         dim3 grid((unsigned)nblk, (unsigned)tiles);                                                            \
         hipLaunchKernelGGL((k_rows_r2c_f<T, N, NP, NF>), grid, dim3(NF * NP * SL<N>::tpp), 0, s, in, out, in_bs, \
-                           out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot);                         \
+                           out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, gate);                   \
     }
     OFDFT_VARIANT(var, CALL)
 #undef CALL
@@ -587,19 +592,22 @@ static void launch_r2c_nf(int var, const T *in, cpx<T> *out, int nblk, long long
 template <typename T, int N>
 static void launch_r2c(int var, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
                        long long nrows, long long opitch, const cpx<T> *tw, T scale, const Prologue &pro,
-                       double *partials, int sum_slot, hipStream_t s) {
+                       double *partials, int sum_slot, hipStream_t s, const double *gate) {
     if (batch == 3)
-        launch_r2c_nf<T, N, 3>(var, in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
+        launch_r2c_nf<T, N, 3>(var, in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s,
+                               gate);
     else if (batch == 2)
-        launch_r2c_nf<T, N, 2>(var, in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s);
+        launch_r2c_nf<T, N, 2>(var, in, out, 1, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot, s,
+                               gate);
     else
         launch_r2c_nf<T, N, 1>(var, in, out, batch, in_bs, out_bs, nrows, opitch, tw, scale, pro, partials, sum_slot,
-                               s);
+                               s, gate);
 }
 
 template <typename T, int N>
 static void launch_c2r(int var, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs,
-                       long long nrows, long long ipitch, const cpx<T> *tw, T scale, hipStream_t s) {
+                       long long nrows, long long ipitch, const cpx<T> *tw, T scale, hipStream_t s,
+                       const double *gate) {
 #define CALL(V)                                                                                                \
     {                                                                                                          \
         constexpr int NP = row_pens_v<T, N, V>();                                                              \
@@ -607,7 +615,7 @@ static void launch_c2r(int var, const cpx<T> *in, T *out, int batch, long long i
         OFDFT_REQUIRE(tiles <= 65535, "too many rows for the C2R pass");                                       \
         dim3 grid((unsigned)batch, (unsigned)tiles);                                                           \
         hipLaunchKernelGGL((k_rows_c2r_f<T, N, NP>), grid, dim3(NP * SL<N>::tpp), 0, s, in, out, in_bs, out_bs, \
-                           nrows, ipitch, tw, scale);                                                          \
+                           nrows, ipitch, tw, scale, gate);                                                    \
     }
     OFDFT_VARIANT(var, CALL)
 #undef CALL
@@ -615,13 +623,13 @@ static void launch_c2r(int var, const cpx<T> *in, T *out, int batch, long long i
 
 template <typename T, int N>
 static void launch_xfused(int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
-                          long long vp, const cpx<T> *tw, const GParams &gp, hipStream_t s) {
+                          long long vp, const cpx<T> *tw, const GParams &gp, hipStream_t s, const double *gate) {
 #define CALL(V)                                                                                                      \
     {                                                                                                                \
         constexpr int TILE = col_tile_v<T, N, V>();                                                                  \
         dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);                                           \
         hipLaunchKernelGGL((k_xfused_f<T, N, TILE>), grid, dim3(TILE * SL<N>::tpp), 0, s, data, bs, ncols, pitch, vp, \
-                           tw, gp);                                                                                  \
+                           tw, gp, gate);                                                                            \
     }
     OFDFT_VARIANT(var, CALL)
 #undef CALL
@@ -631,11 +639,12 @@ static void launch_xfused(int var, cpx<T> *data, int batch, long long bs, long l
 
 template <typename T>
 bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
-                  long long vp, const void *tw, const GParams &gp, hipStream_t s) {
+                  long long vp, const void *tw, const GParams &gp, hipStream_t s, const double *gate) {
     switch (n) {
 #define CASE(N)                                                                                                       \
     case N:                                                                                                           \
-        launch_xfused<T, N>(var, data, batch, bs, ncols, pitch, vp, reinterpret_cast<const cpx<T> *>(tw), gp, s);     \
+        launch_xfused<T, N>(var, data, batch, bs, ncols, pitch, vp, reinterpret_cast<const cpx<T> *>(tw), gp, s,       \
+                            gate);                                                                                    \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -645,12 +654,13 @@ bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long lo
 
 template <typename T>
 bool cols_fixed(int n, int var, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols,
-                long long P, long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s) {
+                long long P, long long Q, long long S, long long vp, const void *tw, double scale, hipStream_t s,
+                const double *gate) {
     switch (n) {
 #define CASE(N)                                                                                                        \
     case N:                                                                                                            \
-        launch_cols<T, N>(var, inv, in, out, batch, bs, ncols, P, Q, S, vp, reinterpret_cast<const cpx<T> *>(tw),       \
-                          (T)scale, s);                                                                                \
+        launch_cols<T, N>(var, inv, in, out, batch, bs, ncols, P, Q, S, vp, reinterpret_cast<const cpx<T> *>(tw),      \
+                          (T)scale, s, gate);                                                                         \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -661,12 +671,12 @@ bool cols_fixed(int n, int var, bool inv, const cpx<T> *in, cpx<T> *out, int bat
 template <typename T>
 bool r2c_fixed(int n, int var, const T *in, cpx<T> *out, int batch, long long in_bs, long long out_bs,
                long long nrows, long long opitch, const void *tw, double scale, const Prologue &pro, double *partials,
-               int sum_slot, hipStream_t s) {
+               int sum_slot, hipStream_t s, const double *gate) {
     switch (n) {
 #define CASE(N)                                                                                                       \
     case N:                                                                                                           \
         launch_r2c<T, N>(var, in, out, batch, in_bs, out_bs, nrows, opitch, reinterpret_cast<const cpx<T> *>(tw),     \
-                         (T)scale, pro, partials, sum_slot, s);                                                       \
+                         (T)scale, pro, partials, sum_slot, s, gate);                                                 \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -676,12 +686,12 @@ bool r2c_fixed(int n, int var, const T *in, cpx<T> *out, int batch, long long in
 
 template <typename T>
 bool c2r_fixed(int n, int var, const cpx<T> *in, T *out, int batch, long long in_bs, long long out_bs,
-               long long nrows, long long ipitch, const void *tw, double scale, hipStream_t s) {
+               long long nrows, long long ipitch, const void *tw, double scale, hipStream_t s, const double *gate) {
     switch (n) {
 #define CASE(N)                                                                                                       \
     case N:                                                                                                           \
         launch_c2r<T, N>(var, in, out, batch, in_bs, out_bs, nrows, ipitch, reinterpret_cast<const cpx<T> *>(tw),     \
-                         (T)scale, s);                                                                                \
+                         (T)scale, s, gate);                                                                          \
         return true;
         OFDFT_FIXED_SIZES(CASE)
 #undef CASE
@@ -690,13 +700,13 @@ bool c2r_fixed(int n, int var, const cpx<T> *in, T *out, int batch, long long in
 }
 
 #define OFDFT_FIXED_INST(T)                                                                                            \
-    template bool xfused_fixed<T>(int, int, cpx<T> *, int, long long, long long, long long, long long, const void *,    \
-                                  const GParams &, hipStream_t);                                                      \
-    template bool cols_fixed<T>(int, int, bool, const cpx<T> *, cpx<T> *, int, long long, long long, long long,         \
-                                long long, long long, long long, const void *, double, hipStream_t);                  \
-    template bool r2c_fixed<T>(int, int, const T *, cpx<T> *, int, long long, long long, long long, long long,          \
-                               const void *, double, const Prologue &, double *, int, hipStream_t);                   \
-    template bool c2r_fixed<T>(int, int, const cpx<T> *, T *, int, long long, long long, long long, long long,          \
-                               const void *, double, hipStream_t);
+    template bool xfused_fixed<T>(int, int, cpx<T> *, int, long long, long long, long long, long long, const void *,   \
+                                  const GParams &, hipStream_t, const double *);                                       \
+    template bool cols_fixed<T>(int, int, bool, const cpx<T> *, cpx<T> *, int, long long, long long, long long,        \
+                                long long, long long, long long, const void *, double, hipStream_t, const double *);   \
+    template bool r2c_fixed<T>(int, int, const T *, cpx<T> *, int, long long, long long, long long, long long,         \
+                               const void *, double, const Prologue &, double *, int, hipStream_t, const double *);    \
+    template bool c2r_fixed<T>(int, int, const cpx<T> *, T *, int, long long, long long, long long, long long,         \
+                               const void *, double, hipStream_t, const double *);
 
 }  // namespace ofdft

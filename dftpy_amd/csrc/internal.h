@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -25,6 +26,13 @@ namespace ofdft {
 
 void set_error(const std::string &msg);
 
+// Runtime calls that synchronise with the legacy default stream (blocking
+// copies and sets, frees, device synchronisation) fail, and invalidate the
+// capture, while another thread captures a stream; the library's setup code
+// takes this lock around them and the optimizer's graph capture holds it.
+std::recursive_mutex &legacy_lock();
+#define OFDFT_LEGACY_LOCK std::lock_guard<std::recursive_mutex> legacy_lk_(legacy_lock())
+
 // Wait for the work queued on a stream by polling it.  The host round trips
 // of the solvers are a few microseconds of GPU work each; a blocking wait
 // adds an interrupt-driven wake-up to every one of them.
@@ -34,6 +42,11 @@ inline void stream_wait(hipStream_t s) {
     }
     OFDFT_HIP(e);
 }
+
+// Gated launches: a kernel given a gate (a device double) returns at once
+// when the gate holds 0.  The optimizer's device-driven iteration queues its
+// kernels ahead of its control decisions this way (null: always run).
+__device__ __forceinline__ bool gate_off(const double *gate) { return gate != nullptr && *gate == 0.0; }
 
 template <typename T> struct cpx_t;
 template <> struct cpx_t<float> { using type = float2; };
@@ -75,6 +88,10 @@ struct Prologue {
 };
 
 struct LmgpState;  // LMGP nonlocal KEDF tables and workspace (lmgp.hip)
+
+// device scalars of a plan (FP64): [0, 256) the solvers' reduction results
+// and step scalars, [256, 1024) the state of the device-driven TN iteration
+constexpr int kScalSlots = 1024;
 
 // One plan: grid geometry, FFT plans, workspace and functional settings.
 struct Plan {
@@ -135,6 +152,11 @@ struct Plan {
     // workspace of the force and stress routines (grown on demand)
     void *fs_ws = nullptr;
     size_t fs_cap = 0;
+    // gate of the kernels launched for this plan (device double, see
+    // gate_off; null outside the optimizer's device-driven iteration)
+    const double *gate = nullptr;
+    // captured graphs of the device-driven TN iteration (opt.hip)
+    struct TnGraphs *tn_graphs = nullptr;
 };
 
 // pseudo.hip: release the staging buffers and PME tables of a plan
@@ -190,7 +212,7 @@ int reduce_blocks(long long n);
 template <typename T>
 void dot(Plan &P, const T *a, const T *b, const T *c, long long n, double *out, double scale, hipStream_t s);
 void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
-                  hipStream_t s);
+                  hipStream_t s, const double *gate = nullptr);
 
 // eval.hip
 template <typename T>

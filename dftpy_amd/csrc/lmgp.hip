@@ -439,6 +439,7 @@ extern "C" {
 int ofdft_mgp_kernel_table(int neta, double etamax, int maxpoints, int ng, const double *gp, const double *gcoef,
                            double *out) {
     double *d = nullptr, *dgp = nullptr, *dgc = nullptr;
+    OFDFT_LEGACY_LOCK;
     try {
         OFDFT_REQUIRE(neta >= 4 && etamax > 0 && maxpoints >= 1 && ng >= 4 && gp && gcoef && out,
                       "mgp_kernel_table: bad arguments");
@@ -469,6 +470,7 @@ int ofdft_set_lmgp(ofdft_plan *plan, int ntab, double etamax, const double *kcoe
                    double ratio) {
     try {
         OFDFT_REQUIRE(plan != nullptr, "set_lmgp: null plan");
+        OFDFT_LEGACY_LOCK;
         Plan &P = *plan;
         if (ntab <= 0) {
             lmgp_free(P);

@@ -30,9 +30,23 @@ void plan_aux_setup(Plan &P) {
     OFDFT_HIP(hipMalloc(&P.aux, P.aux_bytes));
 }
 
+// the cached superstep graph of the device-driven TN iteration
+struct TnGraphs {
+    std::vector<char> key;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;  // capture stream
+    int captures = 0, updates = 0;
+};
+
 void plan_aux_free(Plan &P) {
     if (P.aux) (void)hipFree(P.aux);
     P.aux = nullptr;
+    if (P.tn_graphs) {
+        if (P.tn_graphs->exec) (void)hipGraphExecDestroy(P.tn_graphs->exec);
+        if (P.tn_graphs->cap) (void)hipStreamDestroy(P.tn_graphs->cap);
+        delete P.tn_graphs;
+        P.tn_graphs = nullptr;
+    }
 }
 
 template <typename T> static T *fld(Plan &P, int f) { return reinterpret_cast<T *>(P.aux) + (size_t)f * P.N; }
@@ -40,7 +54,7 @@ template <typename T> static T *fld(Plan &P, int f) { return reinterpret_cast<T 
 
 template <typename T>
 bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
-                  long long vp, const void *tw, const GParams &gp, hipStream_t s);
+                  long long vp, const void *tw, const GParams &gp, hipStream_t s, const double *gate);
 
 // ------------------------------------------------------------------ kernels
 // Streaming kernels: template VEC (4 consecutive elements per thread when the
@@ -49,8 +63,10 @@ bool xfused_fixed(int n, int var, cpx<T> *data, int batch, long long bs, long lo
 #define PW_BODY(i) [&](long long i)
 
 // phi = rho > 0 ? sqrt(rho) : 1e-300 ; sums rho and phi^2 (optimization.py:305-315)
-template <typename T, int VEC> __global__ void k_init_phi(const T *__restrict__ rho, T *__restrict__ phi,
+template <typename T, int VEC> __global__ void k_init_phi(const double *gate,
+                                                          const T *__restrict__ rho, T *__restrict__ phi,
                                                           long long n, double *partials) {
+    if (gate_off(gate)) return;
     double acc[2] = {0, 0};
     pw_loop<VEC>(n, PW_BODY(i) {
         const T r = rho[i];
@@ -64,8 +80,10 @@ template <typename T, int VEC> __global__ void k_init_phi(const T *__restrict__ 
 
 // phi *= sqrt(s[0] / s[1]) ; rho = phi^2 (the normalisation factor from
 // device sums, optimization.py:305-316)
-template <typename T, int VEC> __global__ void k_scale_square(T *__restrict__ phi, T *__restrict__ rho,
+template <typename T, int VEC> __global__ void k_scale_square(const double *gate,
+                                                              T *__restrict__ phi, T *__restrict__ rho,
                                                               const double *s, long long n) {
+    if (gate_off(gate)) return;
     const double f = sqrt(s[0] / s[1]);
     pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)phi[i] * f);
@@ -75,8 +93,10 @@ template <typename T, int VEC> __global__ void k_scale_square(T *__restrict__ ph
 }
 
 // sums  V*rho  and  rho  (mu = int V rho / N, optimization.py:537-539)
-template <typename T, int VEC> __global__ void k_mu_sums(const T *__restrict__ pot, const T *__restrict__ rho,
+template <typename T, int VEC> __global__ void k_mu_sums(const double *gate,
+                                                         const T *__restrict__ pot, const T *__restrict__ rho,
                                                          long long n, double *partials) {
+    if (gate_off(gate)) return;
     double acc[2] = {0, 0};
     pw_loop<VEC>(n, PW_BODY(i) {
         acc[0] += (double)pot[i] * (double)rho[i];
@@ -87,14 +107,17 @@ template <typename T, int VEC> __global__ void k_mu_sums(const T *__restrict__ p
 
 // scal[dst] = scal[a] / scal[b] (a device-side ratio, so the next kernel can
 // read it without a host round trip)
-__global__ void k_ratio(double *scal, int dst, int a, int b) {
+__global__ void k_ratio(const double *gate, double *scal, int dst, int a, int b) {
+    if (gate_off(gate)) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) scal[dst] = scal[a] / scal[b];
 }
 
 // res = (V - mu) phi ; sum res^2
 template <typename T, int VEC>
-__global__ void k_residual(const T *__restrict__ pot, const T *__restrict__ phi, T *__restrict__ res,
+__global__ void k_residual(const double *gate,
+                           const T *__restrict__ pot, const T *__restrict__ phi, T *__restrict__ res,
                            const double *mu, long long n, double *partials) {
+    if (gate_off(gate)) return;
     const double m = *mu;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) {
@@ -107,8 +130,10 @@ __global__ void k_residual(const T *__restrict__ pot, const T *__restrict__ phi,
 
 // TN start: tres = -res, tp = tres, dir = 0 ; sum tres^2
 template <typename T, int VEC>
-__global__ void k_tn_start(const T *__restrict__ res, T *__restrict__ tres, T *__restrict__ tp, T *__restrict__ dir,
+__global__ void k_tn_start(const double *gate,
+                           const T *__restrict__ res, T *__restrict__ tres, T *__restrict__ tp, T *__restrict__ dir,
                            long long n, double *partials) {
+    if (gate_off(gate)) return;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) {
         const T r = -res[i];
@@ -122,8 +147,10 @@ __global__ void k_tn_start(const T *__restrict__ res, T *__restrict__ tres, T *_
 
 // phi1 = phi + eps p ; rho1 = phi1^2
 template <typename T, int VEC>
-__global__ void k_tn_shift(const T *__restrict__ phi, const T *__restrict__ p, double eps, T *__restrict__ phi1,
+__global__ void k_tn_shift(const double *gate,
+                           const T *__restrict__ phi, const T *__restrict__ p, double eps, T *__restrict__ phi1,
                            T *__restrict__ rho1, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)phi[i] + eps * (double)p[i]);
         phi1[i] = v;
@@ -133,9 +160,11 @@ __global__ void k_tn_shift(const T *__restrict__ phi, const T *__restrict__ p, d
 
 // Ap = ((V1 - mu) phi1 - res0) / eps ; sum p.Ap   (optimization.py:105-114)
 template <typename T, int VEC>
-__global__ void k_tn_ap(const T *__restrict__ pot1, const T *__restrict__ phi1, const T *__restrict__ res0,
+__global__ void k_tn_ap(const double *gate,
+                        const T *__restrict__ pot1, const T *__restrict__ phi1, const T *__restrict__ res0,
                         const T *__restrict__ p, const double *mu, double eps, T *__restrict__ Ap,
                         long long n, double *partials) {
+    if (gate_off(gate)) return;
     const double m = *mu;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) {
@@ -150,9 +179,11 @@ __global__ void k_tn_ap(const T *__restrict__ pot1, const T *__restrict__ phi1, 
 // from the device (pAp reduced by the previous launch), so the inner CG needs one host round trip per iteration;
 // negative curvature (pAp < 0) leaves dir and tres untouched for the host.
 template <typename T, int VEC>
-__global__ void k_tn_update_dev(T *__restrict__ dir, T *__restrict__ tres, const T *__restrict__ p,
+__global__ void k_tn_update_dev(const double *gate,
+                                T *__restrict__ dir, T *__restrict__ tres, const T *__restrict__ p,
                                 const T *__restrict__ Ap, const double *rzp, const double *pap,
                                 long long n, double *partials) {
+    if (gate_off(gate)) return;
     const double pa = *pap, r0 = *rzp;
     double acc[1] = {0};
     if (!(pa < 0.0)) {
@@ -168,22 +199,29 @@ __global__ void k_tn_update_dev(T *__restrict__ dir, T *__restrict__ tres, const
 }
 
 // y = a x + b y
-template <typename T, int VEC> __global__ void k_axpby(double a, const T *__restrict__ x, double b,
+template <typename T, int VEC> __global__ void k_axpby(const double *gate,
+                                                       double a, const T *__restrict__ x, double b,
                                                        T *__restrict__ y, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) { y[i] = (T)(a * (double)x[i] + b * (double)y[i]); });
 }
 
-template <typename T, int VEC> __global__ void k_copy(const T *__restrict__ x, T *__restrict__ y, long long n) {
+template <typename T, int VEC> __global__ void k_copy(const double *gate,
+                                                      const T *__restrict__ x, T *__restrict__ y, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) { y[i] = x[i]; });
 }
 
-template <typename T, int VEC> __global__ void k_fill(T *__restrict__ y, T v, long long n) {
+template <typename T, int VEC> __global__ void k_fill(const double *gate, T *__restrict__ y, T v, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) { y[i] = v; });
 }
 
 // sums p*phi and phi*phi
-template <typename T, int VEC> __global__ void k_pphi(const T *__restrict__ p, const T *__restrict__ phi,
+template <typename T, int VEC> __global__ void k_pphi(const double *gate,
+                                                      const T *__restrict__ p, const T *__restrict__ phi,
                                                       long long n, double *partials) {
+    if (gate_off(gate)) return;
     double acc[2] = {0, 0};
     pw_loop<VEC>(n, PW_BODY(i) {
         acc[0] += (double)p[i] * (double)phi[i];
@@ -194,7 +232,9 @@ template <typename T, int VEC> __global__ void k_pphi(const T *__restrict__ p, c
 
 // p -= f phi ; sum p^2
 template <typename T, int VEC>
-__global__ void k_orth(T *__restrict__ p, const T *__restrict__ phi, const double *f, long long n, double *partials) {
+__global__ void k_orth(const double *gate,
+                       T *__restrict__ p, const T *__restrict__ phi, const double *f, long long n, double *partials) {
+    if (gate_off(gate)) return;
     const double ff = *f;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) {
@@ -207,8 +247,10 @@ __global__ void k_orth(T *__restrict__ p, const T *__restrict__ phi, const doubl
 
 // sum V phi p
 template <typename T, int VEC>
-__global__ void k_dot3(const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c, long long n,
+__global__ void k_dot3(const double *gate,
+                       const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ c, long long n,
                        double *partials) {
+    if (gate_off(gate)) return;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) { acc[0] += (double)a[i] * (double)b[i] * (double)c[i]; });
     block_sums<1>(acc, partials);
@@ -216,8 +258,13 @@ __global__ void k_dot3(const T *__restrict__ a, const T *__restrict__ b, const T
 
 // newphi = phi cos + p sin ; newrho = newphi^2   (optimization.py:236-238)
 template <typename T, int VEC>
-__global__ void k_rotate(const T *__restrict__ phi, const T *__restrict__ p, double c, double s,
-                         T *__restrict__ nphi, T *__restrict__ nrho, long long n) {
+__global__ void k_rotate(const double *gate, const T *__restrict__ phi, const T *__restrict__ p, double c, double s,
+                         const double *cs, T *__restrict__ nphi, T *__restrict__ nrho, long long n) {
+    if (gate_off(gate)) return;
+    if (cs) {  // (cos, sin) from the device
+        c = cs[0];
+        s = cs[1];
+    }
     pw_loop<VEC>(n, PW_BODY(i) {
         const T v = (T)((double)phi[i] * c + (double)p[i] * s);
         nphi[i] = v;
@@ -227,9 +274,14 @@ __global__ void k_rotate(const T *__restrict__ phi, const T *__restrict__ p, dou
 
 // sum V * newphi * (p cos - phi sin)   (optimization.py:270-276)
 template <typename T, int VEC>
-__global__ void k_theta_grad(const T *__restrict__ pot, const T *__restrict__ nphi, const T *__restrict__ p,
-                             const T *__restrict__ phi, double c, double s, long long n,
-                             double *partials) {
+__global__ void k_theta_grad(const double *gate, const T *__restrict__ pot, const T *__restrict__ nphi,
+                             const T *__restrict__ p, const T *__restrict__ phi, double c, double s, const double *cs,
+                             long long n, double *partials) {
+    if (gate_off(gate)) return;
+    if (cs) {
+        c = cs[0];
+        s = cs[1];
+    }
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) {
         const T p2 = (T)((double)p[i] * c - (double)phi[i] * s);
@@ -239,15 +291,19 @@ __global__ void k_theta_grad(const T *__restrict__ pot, const T *__restrict__ np
 }
 
 // p = -res (steepest descent)
-template <typename T, int VEC> __global__ void k_neg(const T *__restrict__ x, T *__restrict__ y, long long n) {
+template <typename T, int VEC> __global__ void k_neg(const double *gate,
+                                                     const T *__restrict__ x, T *__restrict__ y, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) { y[i] = -x[i]; });
 }
 
 // CG directions (math_utils.py:395-423): sums r1(r1 - r0), d(r1 - r0), r1^2,
 // r0^2, d r0
 template <typename T, int VEC>
-__global__ void k_cg_sums(const T *__restrict__ r1, const T *__restrict__ r0, const T *__restrict__ d, long long n,
+__global__ void k_cg_sums(const double *gate,
+                          const T *__restrict__ r1, const T *__restrict__ r0, const T *__restrict__ d, long long n,
                           double *partials) {
+    if (gate_off(gate)) return;
     double acc[5] = {0, 0, 0, 0, 0};
     pw_loop<VEC>(n, PW_BODY(i) {
         const double a = r1[i], b = r0[i], dd = d[i];
@@ -262,8 +318,10 @@ __global__ void k_cg_sums(const T *__restrict__ r1, const T *__restrict__ r0, co
 
 // CG-CD as math_utils.py:411-412 writes it: beta = -(r1 ** 2) / sum(d r0) is a
 // field (the numerator is not summed), p = -r1 + beta d
-template <typename T, int VEC> __global__ void k_cg_cd(const T *__restrict__ r1, const T *__restrict__ d,
+template <typename T, int VEC> __global__ void k_cg_cd(const double *gate,
+                                                       const T *__restrict__ r1, const T *__restrict__ d,
                                                        double sdr0, T *__restrict__ p, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) {
         const double r = r1[i];
         const double beta = -(r * r) / sdr0;
@@ -274,9 +332,11 @@ template <typename T, int VEC> __global__ void k_cg_cd(const T *__restrict__ r1,
 // L-BFGS (math_utils.py:374-455).  A pair: s = phi - phi_old,
 // y = res - res_old, and the partial sums of y s
 template <typename T, int VEC>
-__global__ void k_lbfgs_pair(const T *__restrict__ phi, const T *__restrict__ phi_old, const T *__restrict__ res,
+__global__ void k_lbfgs_pair(const double *gate,
+                             const T *__restrict__ phi, const T *__restrict__ phi_old, const T *__restrict__ res,
                              const T *__restrict__ res_old, T *__restrict__ s, T *__restrict__ y, long long n,
                              double *partials) {
+    if (gate_off(gate)) return;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) {
         const T a = phi[i] - phi_old[i], b = res[i] - res_old[i];
@@ -288,15 +348,18 @@ __global__ void k_lbfgs_pair(const T *__restrict__ phi, const T *__restrict__ ph
 }
 
 // scal[k] = 1 / scal[k]
-__global__ void k_recip(double *scal, int k) {
+__global__ void k_recip(const double *gate, double *scal, int k) {
+    if (gate_off(gate)) return;
     if (threadIdx.x == 0) scal[k] = 1.0 / scal[k];
 }
 
 // first loop: alpha = rho_i sum(s_i q) (scal[rho] * scal[dot]), q -= alpha y_i;
 // alpha is kept at scal[alpha_out] for the second loop
 template <typename T, int VEC>
-__global__ void k_lbfgs_q(T *__restrict__ q, const T *__restrict__ y, double *scal, int rho, int dot, int alpha_out,
+__global__ void k_lbfgs_q(const double *gate,
+                          T *__restrict__ q, const T *__restrict__ y, double *scal, int rho, int dot, int alpha_out,
                           long long n) {
+    if (gate_off(gate)) return;
     const double alpha = scal[rho] * scal[dot];
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[alpha_out] = alpha;
     pw_loop<VEC>(n, PW_BODY(i) { q[i] = (T)((double)q[i] - alpha * (double)y[i]); });
@@ -305,16 +368,20 @@ __global__ void k_lbfgs_q(T *__restrict__ q, const T *__restrict__ y, double *sc
 // second loop: beta = rho_i sum(y_i d), d += s_i (alpha_i - beta); alpha_in < 0
 // stands for alpha = 0 (the oldest pair, which the first loop does not visit)
 template <typename T, int VEC>
-__global__ void k_lbfgs_d(T *__restrict__ d, const T *__restrict__ s, const double *scal, int rho, int dot,
+__global__ void k_lbfgs_d(const double *gate,
+                          T *__restrict__ d, const T *__restrict__ s, const double *scal, int rho, int dot,
                           int alpha_in, long long n) {
+    if (gate_off(gate)) return;
     const double beta = scal[rho] * scal[dot];
     const double a = (alpha_in < 0 ? 0.0 : scal[alpha_in]) - beta;
     pw_loop<VEC>(n, PW_BODY(i) { d[i] = (T)((double)d[i] + (double)s[i] * a); });
 }
 
 // sums s y and y y (gamma = s.y / y.y of the newest pair)
-template <typename T, int VEC> __global__ void k_sy_yy(const T *__restrict__ s, const T *__restrict__ y, long long n,
+template <typename T, int VEC> __global__ void k_sy_yy(const double *gate,
+                                                       const T *__restrict__ s, const T *__restrict__ y, long long n,
                                                        double *partials) {
+    if (gate_off(gate)) return;
     double acc[2] = {0, 0};
     pw_loop<VEC>(n, PW_BODY(i) {
         const double a = s[i], b = y[i];
@@ -325,27 +392,34 @@ template <typename T, int VEC> __global__ void k_sy_yy(const T *__restrict__ s, 
 }
 
 // x *= scal[a] / scal[b]
-template <typename T, int VEC> __global__ void k_scale_ratio(T *__restrict__ x, const double *scal, int a, int b,
+template <typename T, int VEC> __global__ void k_scale_ratio(const double *gate,
+                                                             T *__restrict__ x, const double *scal, int a, int b,
                                                              long long n) {
+    if (gate_off(gate)) return;
     const double f = scal[a] / scal[b];
     pw_loop<VEC>(n, PW_BODY(i) { x[i] = (T)(f * (double)x[i]); });
 }
 
 // x *= f
-template <typename T, int VEC> __global__ void k_scale(T *__restrict__ x, double f, long long n) {
+template <typename T, int VEC> __global__ void k_scale(const double *gate, T *__restrict__ x, double f, long long n) {
+    if (gate_off(gate)) return;
     pw_loop<VEC>(n, PW_BODY(i) { x[i] = (T)(f * (double)x[i]); });
 }
 
 // p *= sqrt(scal[a] / scal[b])  (normalise p to sum p^2 = scal[a] with scal[b] = sum p^2)
-template <typename T, int VEC> __global__ void k_scale_norm(T *__restrict__ p, const double *scal, int a, int b,
+template <typename T, int VEC> __global__ void k_scale_norm(const double *gate,
+                                                            T *__restrict__ p, const double *scal, int a, int b,
                                                             long long n) {
+    if (gate_off(gate)) return;
     const double f = sqrt(scal[a] / scal[b]);
     pw_loop<VEC>(n, PW_BODY(i) { p[i] = (T)(f * (double)p[i]); });
 }
 
 // sum a.b
-template <typename T, int VEC> __global__ void k_dot2(const T *__restrict__ a, const T *__restrict__ b, long long n,
+template <typename T, int VEC> __global__ void k_dot2(const double *gate,
+                                                      const T *__restrict__ a, const T *__restrict__ b, long long n,
                                                       double *partials) {
+    if (gate_off(gate)) return;
     double acc[1] = {0};
     pw_loop<VEC>(n, PW_BODY(i) { acc[0] += (double)a[i] * (double)b[i]; });
     block_sums<1>(acc, partials);
@@ -364,7 +438,9 @@ struct PrecondParams {
 // M^-1(|G|), linearly interpolated in the radial table, and the 1/N of the
 // inverse transform
 template <typename T>
-__global__ void k_precond(cpx<T> *spec, long long pitch, int nzc, int nx, int ny, PrecondParams pp) {
+__global__ void k_precond(const double *gate, cpx<T> *spec, long long pitch, int nzc, int nx, int ny,
+                          PrecondParams pp) {
+    if (gate_off(gate)) return;
     const int row = blockIdx.x;
     const int ix = row / ny, iy = row - ix * ny;
     const int fx = ix <= (nx - 1) / 2 ? ix : ix - nx;
@@ -390,6 +466,288 @@ __global__ void k_precond(cpx<T> *spec, long long pitch, int nzc, int nx, int ny
     }
 }
 
+
+// ------------------------------------------------------------------ device-driven TN
+// The TN iteration of Opt::run with every scalar decision taken on the
+// device: the inner-CG stopping tests (optimization.py:113-145), the
+// orthonormalisation's uphill restart, the line search (dcsrch_core.h, the
+// same arithmetic as the host loop) and the convergence test
+// (optimization.py:480-508).  The host queues a fixed superstep of kernels
+// (CG start, G inner-CG iterations, orthonormalisation, L line-search
+// evaluations, accept), each section behind a gate that the control kernels
+// open and close, so one superstep is one hipGraph launch and the host reads
+// back a few scalars once per superstep instead of once per step.
+//
+// State: doubles at P.scal + DT.  Gates hold 0 (skip) or 1 (run).
+enum DtSlot : int {
+    DT = 256,
+    G_CGSTART = DT, G_CG, G_ORTH, G_UPHILL, G_LS, G_ACCEPT, G_END,
+    // the update after inner-CG iteration k of a superstep runs when G_AFTER
+    // holds k + 1; T_AFTER holds its AfterOp bits
+    G_AFTER = DT + 8, T_AFTER,
+    // inner CG
+    T_CGIT = DT + 16, T_R0, T_RZ, T_RCONV, T_RMIN, T_BESTZERO, T_RZCUR, T_NEG0_A, T_NEG0_B, T_PUPD_A, T_PUPD_B,
+    // outer iteration (T_IT .. T_NEV are read back after every superstep)
+    T_IT = DT + 32, T_STATUS, T_ENERGY, T_NV, T_NEV, T_THETA, T_STP, T_C, T_S, T_F, T_LSTASK, T_LSCALLS,
+    T_LSSINGLE, T_UPDONE,
+    // options
+    T_ECONV = DT + 56, T_C1, T_C2, T_XTOL, T_DV, T_MAXITER, T_MAXFUN, T_MAXLS, T_NCHECK, T_FP32, T_PRE,
+    // line-search state (15 doubles)
+    T_LS = DT + 80,
+    // energies of the accepted iterations (ring of DT_HCAP)
+    T_HIST = DT + 128
+};
+constexpr int DT_HCAP = kScalSlots - T_HIST;
+constexpr int DT_READ = T_NEV - T_IT + 1;
+
+struct DtOpts {
+    double econv, c1, c2, xtol, dv;
+    int maxiter, maxfun, maxls, ncheck, fp32, pre;
+};
+
+enum DtStage { DS_CGSTART, DS_CGITER, DS_ORTH, DS_LS, DS_ACCEPT };
+
+// the updates that end an inner-CG iteration (optimization.py:114-145), in
+// this order: dir = (r.z / pAp) p at a negative curvature on the first
+// iteration; best = dir; dir = 0 or dir = best on a stall; p = z + beta p
+enum AfterOp { AF_NEG0 = 1, AF_BEST = 2, AF_FBZERO = 4, AF_FBBEST = 8, AF_PUPD = 16 };
+
+__device__ static void dt_finish(double *S, int status) {
+    S[T_STATUS] = status;
+    for (int g = G_CGSTART; g < G_END; ++g) S[g] = 0.0;
+}
+
+__device__ static void dt_trial(double *S) {
+    S[T_C] = cos(S[T_STP]);
+    S[T_S] = sin(S[T_STP]);
+}
+
+// after the starting evaluation and residual (run eagerly, not captured)
+__global__ void k_tn_init(double *S, DtOpts o) {
+    if (threadIdx.x != 0) return;
+    S[T_ECONV] = o.econv;
+    S[T_C1] = o.c1;
+    S[T_C2] = o.c2;
+    S[T_XTOL] = o.xtol;
+    S[T_DV] = o.dv;
+    S[T_MAXITER] = o.maxiter;
+    S[T_MAXFUN] = o.maxfun;
+    S[T_MAXLS] = o.maxls;
+    S[T_NCHECK] = o.ncheck;
+    S[T_FP32] = o.fp32;
+    S[T_PRE] = o.pre;
+    S[T_IT] = 0.0;
+    S[T_STATUS] = -1.0;
+    S[T_ENERGY] = S[32 + OFDFT_E_SUM];
+    S[T_HIST] = S[T_ENERGY];
+    S[T_THETA] = 0.1;
+    S[T_NV] = 0.0;
+    S[T_NEV] = 0.0;
+    for (int g = G_CGSTART; g < G_END; ++g) S[g] = 0.0;
+    S[G_AFTER] = 0.0;
+    if (o.maxiter > 1)
+        S[G_CGSTART] = 1.0;
+    else
+        S[T_STATUS] = 2.0;
+}
+
+// one control step; k: the inner-CG iteration's index within the superstep
+__device__ static void dt_ctrl(double *S, int stage, int k) {
+    const bool pre = S[T_PRE] != 0.0;
+    switch (stage) {
+    case DS_CGSTART: {  // |r|^2 at scal[25], r.z at scal[26]
+        const double r0 = S[25];
+        S[T_CGIT] = 0.0;
+        S[T_R0] = r0;
+        S[T_RZ] = S[T_RZCUR] = pre ? S[26] : r0;
+        S[T_RCONV] = r0 * 0.1;
+        S[T_RMIN] = 1e10;
+        S[T_BESTZERO] = 1.0;
+        S[G_CGSTART] = 0.0;
+        S[G_AFTER] = 0.0;
+        if (S[T_MAXFUN] > 0) {
+            S[G_CG] = 1.0;
+        } else {
+            S[G_ORTH] = 1.0;
+            S[T_UPDONE] = 0.0;
+        }
+        break;
+    }
+    case DS_CGITER: {  // pAp at scal[16], r1 at scal[17], r1.z1 at scal[18]
+        const int it = (int)S[T_CGIT];
+        S[T_NV] += 1.0;
+        const double pAp = S[16], r1 = S[17], rz1 = pre ? S[18] : S[17];
+        bool done = false;
+        int ops = 0;
+        if (pAp < 0.0) {
+            if (it == 0) {
+                S[T_NEG0_A] = S[T_RZ] / pAp;
+                S[T_NEG0_B] = 0.0;
+                ops |= AF_NEG0;
+            }
+            done = true;
+        } else {
+            const double rmin = S[T_RMIN];  // min over the residuals so far
+            if (r1 < rmin) {
+                ops |= AF_BEST;
+                S[T_BESTZERO] = 0.0;
+                S[T_RMIN] = r1;
+            }
+            const double r0 = S[T_R0];
+            if (r1 < S[T_RCONV]) {
+                done = true;
+            } else if (r1 > 1000.0 * rmin || (it > 10 && fabs(r0 - r1) < 0.1 * r0)) {
+                ops |= S[T_BESTZERO] != 0.0 ? AF_FBZERO : AF_FBBEST;
+                done = true;
+            } else {
+                S[T_PUPD_A] = 1.0;
+                S[T_PUPD_B] = rz1 / S[T_RZ];
+                S[T_R0] = r1;
+                S[T_RZ] = S[T_RZCUR] = rz1;
+                ops |= AF_PUPD;
+            }
+        }
+        S[T_AFTER] = ops;
+        S[G_AFTER] = ops ? k + 1 : 0;
+        S[T_CGIT] = it + 1;
+        if (!done && it + 1 >= (int)S[T_MAXFUN]) done = true;
+        if (done) {
+            S[G_CG] = 0.0;
+            S[G_ORTH] = 1.0;
+            S[T_UPDONE] = 0.0;
+        }
+        break;
+    }
+    case DS_ORTH: {  // sum phi^2 at scal[22], sum p^2 at scal[23], sum V phi p at scal[24]
+        const double theta0 = sqrt(S[23] / S[22]), vphip = S[24];
+        if (vphip > 0.0 && S[T_UPDONE] == 0.0) {  // uphill: once more from -res
+            S[G_UPHILL] = 1.0;
+            S[T_UPDONE] = 1.0;
+            break;
+        }
+        S[G_UPHILL] = 0.0;
+        S[G_ORTH] = 0.0;
+        S[G_AFTER] = 0.0;
+        const double theta = ls::dmin(theta0, S[T_THETA]);
+        S[T_THETA] = theta;
+        const double f = S[T_ENERGY], g = 2.0 * vphip * S[T_DV];
+        S[T_STP] = theta;
+        S[T_LSCALLS] = 0.0;
+        if (S[T_FP32] != 0.0 && fabs(g) * theta < 0.1 * S[T_ECONV]) {
+            S[T_LSSINGLE] = 1.0;  // one evaluation at the trial step (the host loop's FP32 rule)
+            dt_trial(S);
+            S[G_LS] = 1.0;
+            break;
+        }
+        S[T_LSSINGLE] = 0.0;
+        if (S[T_MAXLS] <= 0) {
+            dt_finish(S, 1);
+            break;
+        }
+        double stp = theta;
+        const LsTask t = ls::dcsrch_step(stp, f, g, S[T_C1], S[T_C2], S[T_XTOL], 0.0, M_PI, LS_START, S + T_LS);
+        S[T_LSCALLS] = 1.0;
+        S[T_STP] = stp;
+        S[T_LSTASK] = (double)t;
+        if (t == LS_FG) {
+            dt_trial(S);
+            S[G_LS] = 1.0;
+        } else {
+            dt_finish(S, 1);
+        }
+        break;
+    }
+    case DS_LS: {  // energies at scal[32..], sum V nphi p' at scal[32 + E_COUNT]
+        S[T_NEV] += 1.0;
+        const double f = S[32 + OFDFT_E_SUM], g = 2.0 * S[32 + OFDFT_E_COUNT] * S[T_DV];
+        if (S[T_LSSINGLE] != 0.0) {
+            S[T_F] = f;
+            S[G_LS] = 0.0;
+            S[G_ACCEPT] = 1.0;
+            break;
+        }
+        if (S[T_LSCALLS] >= S[T_MAXLS]) {  // maxls calls, the last one asked for this evaluation
+            dt_finish(S, 1);
+            break;
+        }
+        double stp = S[T_STP];
+        const LsTask t = ls::dcsrch_step(stp, f, g, S[T_C1], S[T_C2], S[T_XTOL], 0.0, M_PI, (LsTask)(int)S[T_LSTASK],
+                                         S + T_LS);
+        S[T_LSCALLS] += 1.0;
+        S[T_STP] = stp;
+        S[T_LSTASK] = (double)t;
+        if (t == LS_FG) {
+            dt_trial(S);
+        } else if (t == LS_CONVERGENCE) {
+            S[T_F] = f;
+            S[G_LS] = 0.0;
+            S[G_ACCEPT] = 1.0;
+        } else {
+            dt_finish(S, 1);
+        }
+        break;
+    }
+    case DS_ACCEPT: {
+        S[T_THETA] = S[T_STP];
+        const double e = S[T_F];
+        S[T_ENERGY] = e;
+        const int it = (int)S[T_IT] + 1;
+        S[T_IT] = it;
+        S[T_HIST + it % DT_HCAP] = e;
+        S[G_ACCEPT] = 0.0;
+        const int ncheck = (int)S[T_NCHECK];
+        if (it >= ncheck) {
+            bool conv = true;
+            for (int i = 0; i < ncheck; ++i)
+                if (fabs(S[T_HIST + (it - 1 - i) % DT_HCAP] - e) > S[T_ECONV]) conv = false;
+            if (conv) {
+                dt_finish(S, 0);
+                break;
+            }
+        }
+        if (it + 1 >= (int)S[T_MAXITER]) {
+            dt_finish(S, 2);
+            break;
+        }
+        S[G_CGSTART] = 1.0;
+        break;
+    }
+    default: break;
+    }
+}
+
+// out[t] = sum of the partial column t (k_reduce_final with unit
+// coefficients, the same order of additions), then a control step
+__global__ void __launch_bounds__(kFinalThreads)
+    k_red_ctrl(const double *gate, const double *__restrict__ partials, int nb, int nterms, double *out, double *S,
+               int stage, int k) {
+    if (gate_off(gate)) return;
+    __shared__ double sm[kFinalThreads];
+    sum_columns(partials, nb, nterms, sm);
+    if (threadIdx.x == 0) {
+        for (int t = 0; t < nterms; ++t) out[t] = sm[t] * 1.0;
+        dt_ctrl(S, stage, k);
+    }
+}
+
+// the updates after inner-CG iteration k (AfterOp bits of T_AFTER), each
+// element as the separate kernels of the host-driven loop compute it
+template <typename T, int VEC>
+__global__ void k_cg_after(const double *S, int k, T *__restrict__ dir, T *__restrict__ best, T *__restrict__ tp,
+                           const T *__restrict__ x, long long n) {
+    if (S[G_AFTER] != (double)(k + 1)) return;
+    const int ops = (int)S[T_AFTER];
+    const double a0 = S[T_NEG0_A], b0 = S[T_NEG0_B], a1 = S[T_PUPD_A], b1 = S[T_PUPD_B];
+    pw_loop<VEC>(n, [&](long long i) {
+        T d = dir[i];
+        if (ops & AF_NEG0) d = (T)(a0 * (double)tp[i] + b0 * (double)d);
+        if (ops & AF_BEST) best[i] = d;
+        if (ops & AF_FBZERO) d = (T)0;
+        if (ops & AF_FBBEST) d = best[i];
+        if (ops & (AF_NEG0 | AF_FBZERO | AF_FBBEST)) dir[i] = d;
+        if (ops & AF_PUPD) tp[i] = (T)(a1 * (double)x[i] + b1 * (double)tp[i]);
+    });
+}
 
 // ------------------------------------------------------------------ host helpers
 struct Ctx {
@@ -418,7 +776,7 @@ struct Ctx {
     void reduce_final_cols(int K, int slot) {
         // partials laid out [block][K]
         std::vector<double> ones(K, 1.0);
-        reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s);
+        reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s, P.gate);
     }
     // copy scal[slot..slot+K) to the host
     const double *to_host(int K, int slot) {
@@ -434,9 +792,9 @@ struct Ctx {
 #define PW_LAUNCH_(K, NB, ...)                                                                \
     do {                                                                                      \
         if (C.vec == 4)                                                                       \
-            hipLaunchKernelGGL((K<T, 4>), dim3(NB), dim3(256), 0, C.s, __VA_ARGS__);          \
+            hipLaunchKernelGGL((K<T, 4>), dim3(NB), dim3(256), 0, C.s, P.gate, __VA_ARGS__);  \
         else                                                                                  \
-            hipLaunchKernelGGL((K<T, 1>), dim3(NB), dim3(256), 0, C.s, __VA_ARGS__);          \
+            hipLaunchKernelGGL((K<T, 1>), dim3(NB), dim3(256), 0, C.s, P.gate, __VA_ARGS__);  \
     } while (0)
 #define PW_RED(K, ...) PW_LAUNCH_(K, C.nb, __VA_ARGS__)
 #define PW_STREAM(K, ...) PW_LAUNCH_(K, C.nbw, __VA_ARGS__)
@@ -468,7 +826,7 @@ template <typename T> struct Opt {
     void mu_and_residual(const T *pot, const T *rho, const T *phi, T *res) {
         PW_RED(k_mu_sums, pot, rho, n, P.red);
         C.reduce_final_cols(2, 18);
-        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 2, 18, 19);
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.gate, P.scal, 2, 18, 19);
         PW_RED(k_residual, pot, phi, res, P.scal + 2, n, P.red);
         C.reduce_final_cols(1, 20);
     }
@@ -500,7 +858,7 @@ template <typename T> struct Opt {
             gp.tab_inv_dq = 1.0 / P.tn_pre_dq;
             fft_r2c<T>(P, in, spec, 1, 0, ncp, 1.0, nullptr, nullptr, C.s, pitch, true);
             OFDFT_REQUIRE(xfused_fixed<T>(P.nr[0], P.var_cols[0], spec, 1, ncp, (long long)P.nr[1] * pitch, pitch,
-                                          P.nzc, P.tw[0], gp, C.s),
+                                          P.nzc, P.tw[0], gp, C.s, P.gate),
                           "fused x pass unavailable");
             OFDFT_HIP(hipGetLastError());
             fft_c2r<T>(P, spec, spec, out, 1, ncp, ncp, P.N, 1.0, C.s, pitch, true);
@@ -513,8 +871,8 @@ template <typename T> struct Opt {
         pp.n = P.tn_pre_n;
         pp.inv_dq = 1.0 / P.tn_pre_dq;
         pp.scale = 1.0 / (double)P.N;
-        hipLaunchKernelGGL((k_precond<T>), dim3((unsigned)(P.nr[0] * P.nr[1])), dim3(128), 0, C.s, spec, pitch, P.nzc,
-                           P.nr[0], P.nr[1], pp);
+        hipLaunchKernelGGL((k_precond<T>), dim3((unsigned)(P.nr[0] * P.nr[1])), dim3(128), 0, C.s, P.gate, spec, pitch,
+                           P.nzc, P.nr[0], P.nr[1], pp);
         OFDFT_HIP(hipGetLastError());
         fft_c2r<T>(P, spec, spec, out, 1, ncp, ncp, P.N, 1.0, C.s, pitch);
     }
@@ -627,7 +985,7 @@ template <typename T> struct Opt {
         T *p = F(F_P), *phi = F(F_PHI);
         PW_RED(k_pphi, p, phi, n, P.red);
         C.reduce_final_cols(2, 21);  // scal[21] = sum p phi, scal[22] = sum phi^2
-        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.scal, 3, 21, 22);
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.gate, P.scal, 3, 21, 22);
         PW_RED(k_orth, p, phi, P.scal + 3, n, P.red);
         C.reduce_final_cols(1, 23);  // scal[23] = sum p^2
         PW_STREAM(k_scale_norm, p, P.scal, 22, 23, n);
@@ -644,11 +1002,10 @@ template <typename T> struct Opt {
     // the energies and the gradient sum come back in one copy
     void value_deriv(double th, double &e, double &g) {
         const double c = std::cos(th), s = std::sin(th);
-        PW_STREAM(k_rotate, F(F_PHI), F(F_P), c, s, F(F_NPHI), F(F_NRHO), n);
+        PW_STREAM(k_rotate, F(F_PHI), F(F_P), c, s, (const double *)nullptr, F(F_NPHI), F(F_NRHO), n);
         eval<T>(P, F(F_NRHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_NPOT), energies_d, C.s);
         ++n_ev;
-        PW_RED(k_theta_grad, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s,
-                           n, P.red);
+        PW_RED(k_theta_grad, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s, (const double *)nullptr, n, P.red);
         C.reduce_final_cols(1, 32 + OFDFT_E_COUNT);
         const double *h = C.to_host(OFDFT_E_COUNT + 1, 32);
         e = h[OFDFT_E_SUM];
@@ -683,7 +1040,7 @@ template <typename T> struct Opt {
         PW_RED(k_lbfgs_pair, F(F_PHI), F(F_NPHI), F(F_RES),
                            F(F_RES_PREV), lb_s(L, k), lb_y(L, k), n, P.red);
         C.reduce_final_cols(1, LB_RHO + k);
-        hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, C.s, P.scal, LB_RHO + k);
+        hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, C.s, P.gate, P.scal, LB_RHO + k);
         L.order.push_back(k);
     }
 
@@ -717,6 +1074,252 @@ template <typename T> struct Opt {
             PW_STREAM(k_lbfgs_d, q, lb_s(L, k), P.scal, LB_RHO + k,
                                LB_DOT, i == 0 ? -1 : LB_ALPHA + i, n);
         }
+    }
+
+    // ---------------------------------------------------------------- device-driven TN
+    // the following launches run behind state slot g (g < 0: ungated)
+    void gate(int g) { P.gate = g >= 0 ? P.scal + g : nullptr; }
+    // reduce K partial columns into scal[slot..slot+K) and take control step
+    // `stage`, in one kernel behind gate g
+    void red_ctrl(int K, int slot, int stage, int g, int k = 0) {
+        hipLaunchKernelGGL(k_red_ctrl, dim3(1), dim3(kFinalThreads), 0, C.s, P.scal + g, P.red, C.nb, K, P.scal + slot,
+                           P.scal, stage, k);
+        OFDFT_HIP(hipGetLastError());
+    }
+
+    // One superstep: CG start, G inner-CG iterations, orthonormalisation, L
+    // line-search evaluations and the accept, each behind its gate (DtSlot).
+    // The kernels and their order are those of direction_tn, orthonormalize,
+    // value_deriv and the accept of run(); the direction buffer is p itself
+    // and the accept copies instead of swapping, so every pointer is fixed.
+    void enqueue_superstep(int hvp_mode, int G, int L) {
+        const bool pre = P.tn_pre_n > 0;
+        T *dir = F(F_DIR), *best = F(F_BEST), *tres = F(F_TRES), *tp = F(F_TP), *z = F(F_Z);
+        T *phi1 = F(F_PHI1), *rho1 = F(F_RHO1), *pot1 = F(F_POT1), *Ap = F(F_AP);
+        const double eps = 1.0e-9;
+        gate(G_CGSTART);
+        if (hvp_mode) {
+            int sl[3];
+            work_slots(P, sl);
+            const T *work = reinterpret_cast<const T *>(P.work);
+            if (sl[0] >= 0) PW_STREAM(k_copy, work + sl[0] * n, F(F_BASE_A), n);
+            if (sl[1] >= 0) PW_STREAM(k_copy, work + sl[1] * n, F(F_BASE_C), n);
+        }
+        PW_RED(k_tn_start, F(F_RES), tres, tp, dir, n, P.red);
+        if (pre) {
+            C.reduce_final_cols(1, 25);
+            apply_precond(tres, z);
+            PW_RED(k_dot2, tres, z, n, P.red);
+            PW_STREAM(k_copy, z, tp, n);  // p0 = z0
+            red_ctrl(1, 26, DS_CGSTART, G_CGSTART);
+        } else {
+            red_ctrl(1, 25, DS_CGSTART, G_CGSTART);
+        }
+        for (int k = 0; k < G; ++k) {
+            gate(G_CG);
+            if (hvp_mode) {
+                hvp<T>(P, F(F_RHO), F(F_PHI), tp, F(F_POT), F(F_BASE_A), F(F_BASE_C), P.scal + 2, Ap, C.nb, C.s);
+            } else {
+                PW_STREAM(k_tn_shift, F(F_PHI), tp, eps, phi1, rho1, n);
+                eval<T>(P, rho1, OFDFT_CALC_V, pot1, nullptr, C.s);
+                PW_RED(k_tn_ap, pot1, phi1, F(F_RES), tp, P.scal + 2, eps, Ap, n, P.red);
+            }
+            C.reduce_final_cols(1, 16);
+            PW_RED(k_tn_update_dev, dir, tres, tp, Ap, P.scal + T_RZCUR, P.scal + 16, n, P.red);
+            if (pre) {
+                C.reduce_final_cols(1, 17);
+                apply_precond(tres, z);
+                PW_RED(k_dot2, tres, z, n, P.red);
+                red_ctrl(1, 18, DS_CGITER, G_CG, k);
+            } else {
+                red_ctrl(1, 17, DS_CGITER, G_CG, k);
+            }
+            const T *x = pre ? z : tres;
+            if (C.vec == 4)
+                hipLaunchKernelGGL((k_cg_after<T, 4>), dim3(C.nbw), dim3(256), 0, C.s, P.scal, k, dir, best, tp, x, n);
+            else
+                hipLaunchKernelGGL((k_cg_after<T, 1>), dim3(C.nbw), dim3(256), 0, C.s, P.scal, k, dir, best, tp, x, n);
+            OFDFT_HIP(hipGetLastError());
+        }
+        gate(G_UPHILL);
+        PW_STREAM(k_neg, F(F_RES), dir, n);
+        gate(G_ORTH);
+        PW_RED(k_pphi, dir, F(F_PHI), n, P.red);
+        C.reduce_final_cols(2, 21);
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.gate, P.scal, 3, 21, 22);
+        PW_RED(k_orth, dir, F(F_PHI), P.scal + 3, n, P.red);
+        C.reduce_final_cols(1, 23);
+        PW_STREAM(k_scale_norm, dir, P.scal, 22, 23, n);
+        PW_RED(k_dot3, F(F_POT), F(F_PHI), dir, n, P.red);
+        red_ctrl(1, 24, DS_ORTH, G_ORTH);
+        for (int k = 0; k < L; ++k) {
+            gate(G_LS);
+            PW_STREAM(k_rotate, F(F_PHI), dir, 0.0, 0.0, P.scal + T_C, F(F_NPHI), F(F_NRHO), n);
+            eval<T>(P, F(F_NRHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_NPOT), energies_d, C.s);
+            PW_RED(k_theta_grad, F(F_NPOT), F(F_NPHI), dir, F(F_PHI), 0.0, 0.0, P.scal + T_C, n, P.red);
+            red_ctrl(1, 32 + OFDFT_E_COUNT, DS_LS, G_LS);
+        }
+        gate(G_ACCEPT);
+        PW_STREAM(k_copy, F(F_NPHI), F(F_PHI), n);
+        PW_STREAM(k_copy, F(F_NRHO), F(F_RHO), n);
+        PW_STREAM(k_copy, F(F_NPOT), F(F_POT), n);
+        // mu_and_residual with the accept's control step in its last reduction
+        PW_RED(k_mu_sums, F(F_POT), F(F_RHO), n, P.red);
+        C.reduce_final_cols(2, 18);
+        hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, C.s, P.gate, P.scal, 2, 18, 19);
+        PW_RED(k_residual, F(F_POT), F(F_PHI), F(F_RES), P.scal + 2, n, P.red);
+        red_ctrl(1, 20, DS_ACCEPT, G_ACCEPT);
+        gate(-1);
+    }
+
+    // everything a captured superstep bakes into its kernel arguments
+    std::vector<char> graph_key(int hvp_mode, int G, int L) const {
+        std::vector<char> k;
+        auto put = [&](const void *p, size_t b) {
+            const char *c = static_cast<const char *>(p);
+            k.insert(k.end(), c, c + b);
+        };
+        const int ints[] = {hvp_mode, G, L, (int)sizeof(T), C.vec, C.nb, C.nbw, P.nr[0], P.nr[1], P.nr[2], P.nzc,
+                            P.full, (int)P.use_fixed, P.var_r2c, P.var_c2r, P.var_cols[0], P.var_cols[1],
+                            P.var_cols[2], (int)P.terms, P.tn_pre_n};
+        put(ints, sizeof(ints));
+        const double dbl[] = {P.x, P.y, P.alpha, P.beta, P.rho0, P.sigma, P.eion, P.dV, P.tn_pre_dq};
+        put(dbl, sizeof(dbl));
+        put(P.B, sizeof(P.B));
+        const void *ptrs[] = {P.spec, P.work, P.red, P.scal, P.aux, P.vloc, P.core, P.tn_pre, P.tw[0], P.tw[1],
+                              P.tw[2], P.lmgp};
+        put(ptrs, sizeof(ptrs));
+        return k;
+    }
+
+    // the superstep's executable graph: captured on the plan's own capture
+    // stream (non-blocking, default priority) and cached on the plan; a new
+    // capture with other kernel arguments updates the cached executable in
+    // place when the topology allows
+    hipGraphExec_t superstep_graph(int hvp_mode, int G, int L) {
+        std::vector<char> key = graph_key(hvp_mode, G, L);
+        if (!P.tn_graphs) P.tn_graphs = new TnGraphs();
+        TnGraphs &tg = *P.tn_graphs;
+        if (tg.exec && tg.key == key) return tg.exec;
+        if (!tg.cap) OFDFT_HIP(hipStreamCreateWithFlags(&tg.cap, hipStreamNonBlocking));
+        OFDFT_LEGACY_LOCK;
+        const hipStream_t user = C.s;
+        C.s = tg.cap;
+        hipGraph_t g = nullptr;
+        OFDFT_HIP(hipStreamBeginCapture(C.s, hipStreamCaptureModeRelaxed));
+        try {
+            enqueue_superstep(hvp_mode, G, L);
+        } catch (...) {
+            (void)hipStreamEndCapture(C.s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            C.s = user;
+            P.gate = nullptr;
+            throw;
+        }
+        C.s = user;
+        OFDFT_HIP(hipStreamEndCapture(tg.cap, &g));
+        bool updated = false;
+        if (tg.exec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res;
+            updated = hipGraphExecUpdate(tg.exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            (void)hipGetLastError();
+            if (updated) ++tg.updates;
+            if (!updated) {
+                (void)hipGraphExecDestroy(tg.exec);
+                tg.exec = nullptr;
+            }
+        }
+        if (!updated) {
+            const hipError_t e = hipGraphInstantiate(&tg.exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                tg.exec = nullptr;
+                (void)hipGraphDestroy(g);
+                OFDFT_HIP(e);
+            }
+            ++tg.captures;
+        }
+        (void)hipGraphDestroy(g);
+        tg.key = key;
+        return tg.exec;
+    }
+
+    // Opt::run's TN with the device-driven iteration: one graph launch and
+    // one read-back of DT_READ scalars per superstep
+    int run_graph(T *rho_io, T *pot_out, int hvp_mode, const double *opts, int G, int L, double *stats,
+                  double *history, int hcap) {
+        OFDFT_REQUIRE(G >= 1 && G <= 64 && L >= 1 && L <= 64, "optimize_tn_graph: steps per superstep out of range");
+        OFDFT_REQUIRE(!(P.terms & OFDFT_TERM_LMGP), "optimize_tn_graph: LMGP is not supported");
+        for (int d = 0; d < 3; ++d)
+            OFDFT_REQUIRE(P.use_fixed && (P.nr[d] == 1 || fixed_size(P.nr[d])),
+                          "optimize_tn_graph: every axis needs a size-specialised FFT pass (2/3/5-smooth sizes of "
+                          "the fixed table)");
+        const double econv = opts[0];
+        const int maxiter = (int)opts[1], maxfun = (int)opts[2], maxls = (int)opts[3];
+        const auto t0 = std::chrono::steady_clock::now();
+        auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+        struct GateReset {
+            Plan &P;
+            ~GateReset() { P.gate = nullptr; }
+        } reset{P};
+        // start (optimization.py:305-316), as in run()
+        PW_RED(k_init_phi, rho_io, F(F_PHI), n, P.red);
+        C.reduce_final_cols(2, 16);
+        PW_STREAM(k_scale_square, F(F_PHI), F(F_RHO), P.scal + 16, n);
+        eval<T>(P, F(F_RHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_POT), energies_d, C.s);
+        mu_and_residual(F(F_POT), F(F_RHO), F(F_PHI), F(F_RES));
+        DtOpts o{econv, opts[4], opts[5], opts[6], P.dV, maxiter, maxfun, maxls, (int)opts[7],
+                 P.prec == OFDFT_FP32 ? 1 : 0, P.tn_pre_n > 0 ? 1 : 0};
+        hipLaunchKernelGGL(k_tn_init, dim3(1), dim3(64), 0, C.s, P.scal, o);
+        OFDFT_HIP(hipGetLastError());
+        std::vector<double> times{secs()};
+        int steps = 0, status = maxiter > 1 ? -1 : 2, iters = 0;
+        double nv = 0.0, nev = 0.0, energy = 0.0;
+        if (status < 0) {
+            const hipGraphExec_t ge = superstep_graph(hvp_mode, G, L);
+            // every superstep moves one section on (DtSlot), so this bounds a run
+            const long long cap =
+                (long long)maxiter * ((maxfun + G - 1) / G + (maxls + L - 1) / L + 4) + 4;
+            while (status < 0) {
+                OFDFT_REQUIRE(++steps <= cap, "optimize_tn_graph: no progress (internal error)");
+                OFDFT_HIP(hipGraphLaunch(ge, C.s));
+                const double *h = C.to_host(DT_READ, T_IT);
+                const int it = (int)h[0];
+                while ((int)times.size() <= it) times.push_back(secs());
+                status = (int)h[T_STATUS - T_IT];
+            }
+        }
+        PW_STREAM(k_copy, F(F_RHO), rho_io, n);
+        if (pot_out) PW_STREAM(k_copy, F(F_POT), pot_out, n);
+        const double *h = C.to_host(DT_READ, T_IT);
+        iters = (int)h[0];
+        energy = h[T_ENERGY - T_IT];
+        nv = h[T_NV - T_IT];
+        nev = h[T_NEV - T_IT];
+        const double mu = C.to_host(1, 2)[0];
+        std::vector<double> hist(std::min(iters + 1, DT_HCAP));
+        OFDFT_HIP(hipMemcpyAsync(hist.data(), P.scal + T_HIST, hist.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 C.s));
+        C.wait();
+        if (stats) {
+            for (int i = 0; i < 16; ++i) stats[i] = 0.0;
+            stats[0] = status;
+            stats[1] = iters;
+            stats[2] = energy;
+            stats[3] = mu;
+            stats[4] = nv;
+            stats[5] = nev + 1.0;  // and the starting evaluation
+            stats[6] = secs();
+            stats[7] = steps;
+            stats[8] = P.tn_graphs ? P.tn_graphs->captures : 0;
+            stats[9] = P.tn_graphs ? P.tn_graphs->updates : 0;
+        }
+        if (history)
+            for (int i = 0; i < hcap && i <= iters && i < (int)hist.size(); ++i) {
+                history[i] = hist[i];
+                history[hcap + i] = i < (int)times.size() ? times[i] : times.back();
+            }
+        return status;
     }
 
     int run(T *rho_io, T *pot_out, int method, int hvp, const double *opts, double *stats, double *history,
@@ -889,6 +1492,27 @@ template <typename T> struct Opt {
 }  // namespace ofdft
 
 using namespace ofdft;
+
+extern "C" int ofdft_optimize_tn_graph(ofdft_plan *plan, void *rho, void *pot, int hvp, const double *opts,
+                                      int cg_steps, int ls_steps, double *stats, double *history, int hcap,
+                                      void *stream) {
+    Plan *P = reinterpret_cast<Plan *>(plan);
+    try {
+        OFDFT_REQUIRE(P && rho && opts, "optimize_tn_graph: bad arguments");
+        hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (P->prec == OFDFT_FP64) {
+            Opt<double> o(*P, s);
+            o.run_graph((double *)rho, (double *)pot, hvp, opts, cg_steps, ls_steps, stats, history, hcap);
+        } else {
+            Opt<float> o(*P, s);
+            o.run_graph((float *)rho, (float *)pot, hvp, opts, cg_steps, ls_steps, stats, history, hcap);
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}
 
 extern "C" int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method, int hvp, const double *opts,
                               double *stats, double *history, int hcap, void *stream) {

@@ -358,6 +358,7 @@ static char *stage_host(Plan &P, size_t bytes) {
         OFDFT_HIP(e);
     }
     if (bytes > P.stg_cap) {
+        OFDFT_LEGACY_LOCK;
         // kernels of an earlier call may still read the device copy
         if (P.stg_d) OFDFT_HIP(hipDeviceSynchronize());
         if (P.stg_h) OFDFT_HIP(hipHostFree(P.stg_h));
@@ -390,6 +391,7 @@ struct StageLayout {
 };
 
 void plan_setup_free(Plan &P) {
+    OFDFT_LEGACY_LOCK;
     if (P.stg_evt) (void)hipEventSynchronize(P.stg_evt);
     if (P.stg_evt) (void)hipEventDestroy(P.stg_evt);
     if (P.stg_h) (void)hipHostFree(P.stg_h);
@@ -407,6 +409,7 @@ void plan_setup_free(Plan &P) {
 // B-spline moduli of the plan's grid for `order`, computed once per plan
 static BTab pme_tab(Plan &P, int order) {
     if (P.pme_b == nullptr || P.pme_order != order) {
+        OFDFT_LEGACY_LOCK;
         if (P.pme_b) OFDFT_HIP(hipFree(P.pme_b));
         P.pme_b = nullptr;
         auto x = bspline_bm(P.nr[0], P.nr[0], order);
@@ -894,6 +897,7 @@ __global__ void k_ewald_real_fs(int nat, const double *__restrict__ pos, const d
 // ------------------------------------------------------------------ host
 static double *fs_workspace(Plan &P, size_t bytes) {
     if (bytes > P.fs_cap) {
+        OFDFT_LEGACY_LOCK;
         if (P.fs_ws) OFDFT_HIP(hipFree(P.fs_ws));
         P.fs_ws = nullptr;
         P.fs_cap = 0;

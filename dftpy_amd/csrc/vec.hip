@@ -39,7 +39,9 @@ struct Coefs {
 // out[t] = coef[t] * sum_b partials[b * nterms + t]; optionally out[nterms] =
 // sum_t out[t] + add.
 __global__ void __launch_bounds__(kFinalThreads)
-    k_reduce_final(const double *__restrict__ partials, int nblocks, int nterms, double *out, Coefs co) {
+    k_reduce_final(const double *__restrict__ partials, int nblocks, int nterms, double *out, Coefs co,
+                   const double *gate) {
+    if (gate_off(gate)) return;
     __shared__ double sm[kFinalThreads];
     sum_columns(partials, nblocks, nterms, sm);
     if (threadIdx.x == 0) {
@@ -53,11 +55,11 @@ __global__ void __launch_bounds__(kFinalThreads)
 }
 
 void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
-                  hipStream_t s) {
+                  hipStream_t s, const double *gate) {
     Coefs co{};
     for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
     co.sum_out = 0;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co, gate);
     OFDFT_HIP(hipGetLastError());
 }
 
@@ -67,7 +69,8 @@ void reduce_final_sum(const double *partials, int nblocks, int nterms, double *o
     for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
     co.sum_out = 1;
     co.add = add;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co,
+                       (const double *)nullptr);
     OFDFT_HIP(hipGetLastError());
 }
 

@@ -52,7 +52,8 @@ class Optimization:
         self.optimization_options = dict(optimization_options or {})
         defaults = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200, "maxls": 30,
                     "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
-                    "ncheck": 2, "hvp": "auto", "precond": False}
+                    "ncheck": 2, "hvp": "auto", "precond": False, "graph": False, "graph_cg_steps": 0,
+                    "graph_ls_steps": 0}
         for k, v in defaults.items():
             self.optimization_options.setdefault(k, v)
         if EnergyEvaluator is None:
@@ -130,15 +131,30 @@ class Optimization:
         hist = np.zeros(2 * hcap)
         rho = rho_in.data.clone().contiguous()
         pot = torch.empty_like(rho)
-        _lib.check(_lib.lib.ofdft_optimize(plan.handle, rho.data_ptr(), pot.data_ptr(),
-                                           _method_code(self.optimization_method), hvp, _lib.dptr(opts), _lib.dptr(stats),
-                                           _lib.dptr(hist), hcap, _lib.stream()))
+        graph = bool(o["graph"])
+        if graph and self.optimization_method != "TN":
+            raise NotImplementedError("the device-driven (graph) iteration covers TN only")
+        if graph:
+            # device-driven TN (not in the reference): one hipGraph launch per
+            # superstep of cg_steps inner-CG iterations and ls_steps line-search
+            # evaluations; by default sized to the usual counts of the form
+            cg_steps = int(o["graph_cg_steps"]) or (2 if (o["precond"] or hvp == 1) else 8)
+            ls_steps = int(o["graph_ls_steps"]) or (1 if grid.dtype == torch.float32 else 2)
+            _lib.check(_lib.lib.ofdft_optimize_tn_graph(plan.handle, rho.data_ptr(), pot.data_ptr(), hvp,
+                                                        _lib.dptr(opts), cg_steps, ls_steps, _lib.dptr(stats),
+                                                        _lib.dptr(hist), hcap, _lib.stream()))
+        else:
+            _lib.check(_lib.lib.ofdft_optimize(plan.handle, rho.data_ptr(), pot.data_ptr(),
+                                               _method_code(self.optimization_method), hvp, _lib.dptr(opts),
+                                               _lib.dptr(stats), _lib.dptr(hist), hcap, _lib.stream()))
         del vloc
         n = int(stats[1]) + 1
         self.history = hist[:n].copy()
         self.history_seconds = hist[hcap:hcap + n].copy()
         self.stats = dict(converged=int(stats[0]), niter=int(stats[1]), energy=stats[2], mu=stats[3],
                           n_eval_v=int(stats[4]), n_eval_ev=int(stats[5]), seconds=stats[6])
+        if graph:
+            self.stats.update(supersteps=int(stats[7]), graph_captures=int(stats[8]), graph_updates=int(stats[9]))
         self.converged = int(stats[0])
         self.mu = stats[3]
         self.rho = DirectField(grid, data=rho)

@@ -325,6 +325,25 @@ enum {
  * the call started at each step in [hcap, 2*hcap). */
 int ofdft_optimize(ofdft_plan *plan, void *rho, void *pot, int method, int hvp, const double *opts,
                    double *stats, double *history, int hcap, void *stream);
+/* The TN form of ofdft_optimize (same arguments and results; method TN) with
+ * the iteration driven from the device: the inner-CG stopping tests
+ * (optimization.py:113-145), the uphill restart, the Moré-Thuente line search
+ * (the same arithmetic as ofdft_dcsrch) and the convergence test
+ * (optimization.py:480-508) run in single-thread control kernels, and the
+ * kernels of one superstep -- CG start, cg_steps inner-CG iterations, the
+ * orthonormalisation, ls_steps line-search evaluations and the accept, each
+ * skipped when its control flag is off -- are captured once as a hipGraph
+ * (cached on the plan, updated in place when the lattice or fields change)
+ * and launched with one read-back of a few scalars per superstep.  The
+ * results equal ofdft_optimize's up to the last bits of cos/sin of the line
+ * search step (device vs host libm).  stats[7] = supersteps launched,
+ * stats[8], stats[9] = graph captures / in-place updates on this plan so far.
+ * Needs size-specialised FFT passes on every axis and no LMGP term.  The
+ * first call on a plan (and any call after its settings change) captures;
+ * the capture stream is created once per plan (default priority).
+ * No counterpart in the reference beyond get_direction_TN / optimize_rho. */
+int ofdft_optimize_tn_graph(ofdft_plan *plan, void *rho, void *pot, int hvp, const double *opts, int cg_steps,
+                            int ls_steps, double *stats, double *history, int hcap, void *stream);
 /* Setup call for OFDFT_OPT_LBFGS (no separate counterpart: the reference's
  * LBFGS class, math_utils.py:374-392, keeps its pairs in Python lists): makes
  * the plan hold memory for maxcor + 1 (s, y) pairs, 2 (maxcor + 1) fields of

@@ -1,5 +1,6 @@
 """Concurrent multi-cell optimisation (dftpy_amd/batch.py): cells of uneven
-grid sizes on several streams give the same results as one after another,
+grid sizes on several streams give the same results as one after another
+(and with the device-driven, graph-launched TN as with the host-driven one),
 every cell reaches the reference stop rule, and each FP32 result's FP64
 energy lies within 1e-6 Ha/atom of the cell's FP64 ground state."""
 import pytest
@@ -16,6 +17,21 @@ def test_batch_matches_sequential():
     print(wall, wall1, [(r["nr"], r["niter"], r["evals"]) for r in res])
     for a, b in zip(res, res1):
         assert a["status"] == 0 and b["status"] == 0
+        assert abs(a["energy"] - b["energy"]) < 1e-9 * a["nat"]
+
+
+def test_batch_graph_tn_matches_host_tn():
+    """The batch with the device-driven TN (one captured hipGraph per plan,
+    updated in place from cell to cell, captures in several threads) gives
+    the host-driven TN's energies and iteration counts cell by cell."""
+    from dftpy_amd.batch import make_cells, run_batch
+    cells = make_cells(6, seed=4)
+    res, _ = run_batch(cells, workers=3)
+    resg, _ = run_batch(cells, workers=3, graph=True)
+    print([(r["nr"], r["niter"], r["evals"]) for r in resg])
+    for a, b in zip(res, resg):
+        assert a["status"] == b["status"] == 0
+        assert a["niter"] == b["niter"] and a["evals"] == b["evals"]
         assert abs(a["energy"] - b["energy"]) < 1e-9 * a["nat"]
 
 
