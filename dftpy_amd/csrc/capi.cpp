@@ -178,7 +178,9 @@ int ofdft_plan_autotune(ofdft_plan *plan, int *variants, float *times, void *str
 int ofdft_plan_set_variants(ofdft_plan *plan, const int *variants) {
     return guard([&] {
         OFDFT_REQUIRE(plan != nullptr && variants != nullptr, "plan_set_variants: null pointer");
-        for (int i = 0; i < 4; ++i) OFDFT_REQUIRE(variants[i] >= 0 && variants[i] <= 2, "variant out of range");
+        for (int i = 0; i < 4; ++i)
+            OFDFT_REQUIRE(variants[i] >= 0 && variants[i] < (i < 2 ? kTileVariants : kColVariants),
+                          "variant out of range");
         plan->var_r2c = variants[0];
         plan->var_c2r = variants[1];
         plan->var_cols[0] = variants[2];

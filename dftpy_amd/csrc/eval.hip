@@ -85,15 +85,19 @@ __global__ void k_gspace(cpx<T> *__restrict__ spec, long long nc, GParams p, con
     }
 }
 
-// rho0 = sum / N with the kernel-reuse threshold of wt.py:64
+// rho0 = sum / N with the kernel-reuse threshold of wt.py:64, and the WT
+// constants (tkf, wtfac) of the cached rho0 at rho0_cache[kWtSlot], [kWtSlot + 1]
+constexpr int kWtSlot = 120;  // offset from the rho0 cache (P.scal[0]) in the plan's device scalars
 __global__ void __launch_bounds__(kFinalThreads)
-    k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache, const double *gate) {
+    k_rho0_update(const double *partials, int nb, double inv_n, double *rho0_cache, double alpha, double beta,
+                  const double *gate) {
     if (gate_off(gate)) return;
     __shared__ double sm[kFinalThreads];
     sum_columns(partials, nb, 1, sm);
     if (threadIdx.x == 0) {
         const double r = sm[0] * inv_n;
         if (fabs(*rho0_cache - r) > 1e-6) *rho0_cache = r;
+        wt_consts_at(*rho0_cache, alpha, beta, rho0_cache[kWtSlot], rho0_cache[kWtSlot + 1]);
     }
 }
 
@@ -370,7 +374,7 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     if (auto_rho0) {
         const int nb0 = r2c_num_partials(P, ns);
         hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb0, 1.0 / (double)P.N, rho0_cache,
-                           P.gate);
+                           P.alpha, P.beta, P.gate);
         OFDFT_HIP(hipGetLastError());
     }
     GParams gp;
@@ -394,6 +398,7 @@ static void eval_energy(Plan &P, const T *rho, int ns, int s_vw, int s_wt, int s
     gp.beta = P.beta;
     gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
     gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
+    gp.wt_dev = auto_rho0 ? rho0_cache + kWtSlot : nullptr;
     gp.bs = ncp;
     gp.pitch = (int)pitch;
     PostParams pp;
@@ -442,6 +447,7 @@ static GParams pipeline_gparams(const Plan &P, int s_vw, int s_wt, int s_h, bool
     gp.beta = P.beta;
     gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
     gp.rho0_dev = auto_rho0 ? rho0_cache : nullptr;
+    gp.wt_dev = auto_rho0 ? rho0_cache + kWtSlot : nullptr;
     gp.bs = ncp;
     gp.pitch = (int)pitch;
     return gp;
@@ -501,7 +507,7 @@ void eval(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, hipStr
         if (auto_rho0) {
             const int nb = r2c_num_partials(P, ns);
             hipLaunchKernelGGL(k_rho0_update, dim3(1), dim3(kFinalThreads), 0, s, P.red, nb, 1.0 / (double)P.N,
-                               rho0_cache, P.gate);
+                               rho0_cache, P.alpha, P.beta, P.gate);
             OFDFT_HIP(hipGetLastError());
         }
         int nsa = ns;
@@ -633,6 +639,17 @@ template void eval_fields<double>(Plan &, const double *, unsigned, double *, do
 // spectra and work fields hold whatever the last evaluation left): stage 0 =
 // the x pass with the G-space multiply of every spectrum, in place; stage 1 =
 // the point-wise post pass (potential into pot, energy partials).
+template <typename T> bool bench_xfused(Plan &P, hipStream_t s) {
+    if (!x_fusable(P)) return false;
+    const long long pitch = pipe_pitch(P);
+    const long long ncp = (long long)P.nr[0] * P.nr[1] * pitch;
+    GParams gp = pipeline_gparams(P, 0, 1, 2, false, nullptr, ncp, pitch);
+    gspace_apply<T>(P, reinterpret_cast<cpx<T> *>(P.spec), 3, ncp, pitch, gp, true, s);
+    return true;
+}
+template bool bench_xfused<float>(Plan &, hipStream_t);
+template bool bench_xfused<double>(Plan &, hipStream_t);
+
 template <typename T> void bench_eval_stage(Plan &P, int stage, const T *rho, T *pot, hipStream_t s) {
     const unsigned terms = P.terms;
     int ns = 0;
@@ -700,6 +717,7 @@ void eval_linear(Plan &P, const T *rho, const T *phi, const T *p, hipStream_t s)
     gp.beta = P.beta;
     gp.rho0 = P.rho0 > 0 ? P.rho0 : 1.0;
     gp.rho0_dev = (s_wt >= 0 && !(P.rho0 > 0)) ? P.scal + 0 : nullptr;
+    gp.wt_dev = gp.rho0_dev ? P.scal + kWtSlot : nullptr;
     gp.bs = ncp;
     gp.pitch = (int)pitch;
     gspace_apply<T>(P, spec, ns, ncp, pitch, gp, fused, s);

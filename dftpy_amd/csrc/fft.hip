@@ -622,13 +622,16 @@ template <typename T> static void autotune_t(Plan &P, hipStream_t s, float *time
     cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);
     T *work = reinterpret_cast<T *>(P.work);
     const int iters = 4;
-    auto pick = [&](int *var, int slot, auto &&run) {
+    // times: kColVariants per slot; the row passes have kTileVariants
+    auto pick = [&](int *var, int slot, int nvar, auto &&run) {
         float best = 1e30f;
         int bv = 0;
-        for (int v = 0; v < 3; ++v) {
+        for (int v = 0; v < kColVariants; ++v)
+            if (times) times[slot * kColVariants + v] = -1.0f;
+        for (int v = 0; v < nvar; ++v) {
             *var = v;
             const float t = time_launches(s, iters, run);
-            if (times) times[slot * 3 + v] = t;
+            if (times) times[slot * kColVariants + v] = t;
             if (t < best) {
                 best = t;
                 bv = v;
@@ -642,12 +645,17 @@ template <typename T> static void autotune_t(Plan &P, hipStream_t s, float *time
         pro.code[1] = OP_POW_BETA;
         pro.beta = P.beta;
         pro.b56 = 1;
-        pick(&P.var_r2c, 0, [&] { bench_rows_r2c<T>(P, work, spec, 3, &pro, s); });
-        pick(&P.var_c2r, 1, [&] { bench_rows_c2r<T>(P, spec, work, 3, s); });
+        pick(&P.var_r2c, 0, kTileVariants, [&] { bench_rows_r2c<T>(P, work, spec, 3, &pro, s); });
+        pick(&P.var_c2r, 1, kTileVariants, [&] { bench_rows_c2r<T>(P, spec, work, 3, s); });
     }
+    // the x variant is timed on the pass the evaluation runs: the fused
+    // forward x, G-space multiply and inverse x where the axis has one
     for (int axis = 0; axis < 2; ++axis)
         if (fixed_size(P.nr[axis]))
-            pick(&P.var_cols[axis], 2 + axis, [&] { fft_cols<T>(P, axis, false, false, spec, spec, 3, ncp, 1.0, s, pitch); });
+            pick(&P.var_cols[axis], 2 + axis, kColVariants, [&] {
+                if (axis != 0 || !bench_xfused<T>(P, s))
+                    fft_cols<T>(P, axis, false, false, spec, spec, 3, ncp, 1.0, s, pitch);
+            });
 }
 
 void plan_autotune(Plan &P, hipStream_t s, float *times) {

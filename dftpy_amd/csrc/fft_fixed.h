@@ -68,8 +68,21 @@ constexpr StageList factorize(int n) {
     return s;
 }
 
-template <int N> struct SL {
-    static constexpr StageList s = factorize(N);
+// radix order ORD: 0 as factorize lists it (8s first); 1 reversed (the odd
+// radices first), a variant of the column passes the autotuner times
+constexpr StageList factorize_ord(int n, int ord) {
+    StageList s = factorize(n);
+    if (ord == 1)
+        for (int i = 0, k = s.nst - 1; i < k; ++i, --k) {
+            const int t = s.rad[i];
+            s.rad[i] = s.rad[k];
+            s.rad[k] = t;
+        }
+    return s;
+}
+
+template <int N, int ORD = 0> struct SL {
+    static constexpr StageList s = factorize_ord(N, ORD);
     static constexpr int nst = s.nst;
     static constexpr int rmin = s.rmin;
     static constexpr int tpp = N / s.rmin;  // threads per pencil
@@ -124,17 +137,17 @@ __device__ __forceinline__ void cstage(C *buf, const C *tw, const C *__restrict_
     if (!LAST) __syncthreads();
 }
 
-template <typename C, int N, int TILE, bool INV, int I, int NS>
+template <typename C, int N, int TILE, bool INV, int I, int NS, int ORD>
 __device__ __forceinline__ void cstages(C *buf, const C *tw, const C *src, C *dst, long long S, bool valid, int pen,
                                         int j, typename Real<C>::type scale) {
-    if constexpr (I < SL<N>::nst) {
-        constexpr int R = SL<N>::template rad<I>();
-        cstage<C, N, TILE, R, NS, INV, I == 0, I == SL<N>::nst - 1>(buf, tw, src, dst, S, valid, pen, j, scale);
-        cstages<C, N, TILE, INV, I + 1, NS * R>(buf, tw, src, dst, S, valid, pen, j, scale);
+    if constexpr (I < SL<N, ORD>::nst) {
+        constexpr int R = SL<N, ORD>::template rad<I>();
+        cstage<C, N, TILE, R, NS, INV, I == 0, I == SL<N, ORD>::nst - 1>(buf, tw, src, dst, S, valid, pen, j, scale);
+        cstages<C, N, TILE, INV, I + 1, NS * R, ORD>(buf, tw, src, dst, S, valid, pen, j, scale);
     }
 }
 
-template <typename T, int N, int TILE, bool INV>
+template <typename T, int N, int TILE, bool INV, int ORD>
 __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     k_cols_f(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, long long bs, long long ncols, long long P,
              long long Q, long long S, long long vp, const cpx<T> *__restrict__ twg, T scale, const double *gate) {
@@ -150,20 +163,20 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     const bool valid = c < ncols && cp < vp;  // pad columns of a pitched layout are skipped
     const long long base = valid ? (c / P) * Q + cp : 0;
     const long long off = (long long)blockIdx.y * bs + base;
-    cstages<C, N, TILE, INV, 0, 1>(buf, tw, in + off, out + off, S, valid, pen, j, scale);
+    cstages<C, N, TILE, INV, 0, 1, ORD>(buf, tw, in + off, out + off, S, valid, pen, j, scale);
 }
 
 // Stages with a choice of source and destination: GIN reads the first stage
 // from HBM (else from LDS), GOUT writes the last stage to HBM (else to LDS in
 // natural order).
-template <typename C, int N, int TILE, bool INV, int I, int NS, bool GIN, bool GOUT>
+template <typename C, int N, int TILE, bool INV, int I, int NS, bool GIN, bool GOUT, int ORD>
 __device__ __forceinline__ void cstages_io(C *buf, const C *tw, const C *src, C *dst, long long S, bool valid,
                                            int pen, int j, typename Real<C>::type scale) {
-    if constexpr (I < SL<N>::nst) {
-        constexpr int R = SL<N>::template rad<I>();
-        cstage<C, N, TILE, R, NS, INV, GIN && I == 0, GOUT && I == SL<N>::nst - 1>(buf, tw, src, dst, S, valid, pen,
-                                                                                    j, scale);
-        cstages_io<C, N, TILE, INV, I + 1, NS * R, GIN, GOUT>(buf, tw, src, dst, S, valid, pen, j, scale);
+    if constexpr (I < SL<N, ORD>::nst) {
+        constexpr int R = SL<N, ORD>::template rad<I>();
+        cstage<C, N, TILE, R, NS, INV, GIN && I == 0, GOUT && I == SL<N, ORD>::nst - 1>(buf, tw, src, dst, S, valid,
+                                                                                         pen, j, scale);
+        cstages_io<C, N, TILE, INV, I + 1, NS * R, GIN, GOUT, ORD>(buf, tw, src, dst, S, valid, pen, j, scale);
     }
 }
 
@@ -172,7 +185,7 @@ __device__ __forceinline__ void cstages_io(C *buf, const C *tw, const C *src, C 
 // its kernel value (gfactor: vW, WT or Hartree by spectrum slot), and the
 // inverse x transform starts from LDS and writes back in place.  Saves one
 // HBM round trip of every spectrum on each side of the multiply.
-template <typename T, int N, int TILE>
+template <typename T, int N, int TILE, int ORD>
 __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     k_xfused_f(cpx<T> *__restrict__ data, long long bs, long long ncols, long long pitch, long long vp,
                const cpx<T> *__restrict__ twg, GParams gp, const double *gate) {
@@ -191,7 +204,7 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
     double tkf = 1.0, wtfac = 1.0;
     const int kind = gp.kind[blockIdx.y];
     if (kind == 1) wt_consts(gp, tkf, wtfac);
-    cstages_io<C, N, TILE, false, 0, 1, true, false>(buf, tw, col, nullptr, S, valid, pen, j, (T)1);
+    cstages_io<C, N, TILE, false, 0, 1, true, false, ORD>(buf, tw, col, nullptr, S, valid, pen, j, (T)1);
     // multiply: this thread's pencil (pen) is one (iy, iz) column, its
     // elements are e = j, j + tpp, ...; |G|^2 is built from the column's
     // fixed part plus fx * b1 in the plan precision
@@ -244,7 +257,7 @@ __global__ void __launch_bounds__(TILE *SL<N>::tpp)
         }
     }
     __syncthreads();
-    cstages_io<C, N, TILE, true, 0, 1, false, true>(buf, tw, nullptr, col, S, valid, pen, j, (T)1);
+    cstages_io<C, N, TILE, true, 0, 1, false, true, ORD>(buf, tw, nullptr, col, S, valid, pen, j, (T)1);
 }
 
 // ---------------------------------------------------------------- rows passes
@@ -544,31 +557,38 @@ template <typename T, int N, int NF, int V> constexpr int r2c_pens_v() {
     return p < 1 ? 1 : p;
 }
 
-constexpr int kTileVariants = 3;  // variants 0, 1, 2 of every tiled pass
-
 #define OFDFT_VARIANT(var, CALL) \
     switch (var) {               \
         case 1: CALL(1); break;  \
         case 2: CALL(2); break;  \
         default: CALL(0); break; \
     }
+#define OFDFT_COL_VARIANT(var, CALL) \
+    switch (var) {                   \
+        case 1: CALL(1, 0); break;   \
+        case 2: CALL(2, 0); break;   \
+        case 3: CALL(0, 1); break;   \
+        case 4: CALL(1, 1); break;   \
+        case 5: CALL(2, 1); break;   \
+        default: CALL(0, 0); break;  \
+    }
 
 template <typename T, int N>
 static void launch_cols(int var, bool inv, const cpx<T> *in, cpx<T> *out, int batch, long long bs, long long ncols,
                         long long P, long long Q, long long S, long long vp, const cpx<T> *tw, T scale, hipStream_t s,
                         const double *gate) {
-#define CALL(V)                                                                                                      \
-    {                                                                                                                \
-        constexpr int TILE = col_tile_v<T, N, V>();                                                                  \
-        dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);                                           \
-        if (inv)                                                                                                     \
-            hipLaunchKernelGGL((k_cols_f<T, N, TILE, true>), grid, dim3(TILE * SL<N>::tpp), 0, s, in, out, bs, ncols, \
-                               P, Q, S, vp, tw, scale, gate);                                                        \
-        else                                                                                                         \
-            hipLaunchKernelGGL((k_cols_f<T, N, TILE, false>), grid, dim3(TILE * SL<N>::tpp), 0, s, in, out, bs,       \
-                               ncols, P, Q, S, vp, tw, scale, gate);                                                 \
+#define CALL(V, O)                                                                                                 \
+    {                                                                                                              \
+        constexpr int TILE = col_tile_v<T, N, V>();                                                                \
+        dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);                                         \
+        if (inv)                                                                                                   \
+            hipLaunchKernelGGL((k_cols_f<T, N, TILE, true, O>), grid, dim3(TILE * SL<N>::tpp), 0, s, in, out, bs,   \
+                               ncols, P, Q, S, vp, tw, scale, gate);                                               \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_cols_f<T, N, TILE, false, O>), grid, dim3(TILE * SL<N>::tpp), 0, s, in, out, bs,  \
+                               ncols, P, Q, S, vp, tw, scale, gate);                                               \
     }
-    OFDFT_VARIANT(var, CALL)
+    OFDFT_COL_VARIANT(var, CALL)
 #undef CALL
 }
 
@@ -624,14 +644,14 @@ static void launch_c2r(int var, const cpx<T> *in, T *out, int batch, long long i
 template <typename T, int N>
 static void launch_xfused(int var, cpx<T> *data, int batch, long long bs, long long ncols, long long pitch,
                           long long vp, const cpx<T> *tw, const GParams &gp, hipStream_t s, const double *gate) {
-#define CALL(V)                                                                                                      \
-    {                                                                                                                \
-        constexpr int TILE = col_tile_v<T, N, V>();                                                                  \
-        dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);                                           \
-        hipLaunchKernelGGL((k_xfused_f<T, N, TILE>), grid, dim3(TILE * SL<N>::tpp), 0, s, data, bs, ncols, pitch, vp, \
-                           tw, gp, gate);                                                                            \
+#define CALL(V, O)                                                                                                 \
+    {                                                                                                              \
+        constexpr int TILE = col_tile_v<T, N, V>();                                                                \
+        dim3 grid((unsigned)((ncols + TILE - 1) / TILE), (unsigned)batch);                                         \
+        hipLaunchKernelGGL((k_xfused_f<T, N, TILE, O>), grid, dim3(TILE * SL<N>::tpp), 0, s, data, bs, ncols, pitch, \
+                           vp, tw, gp, gate);                                                                      \
     }
-    OFDFT_VARIANT(var, CALL)
+    OFDFT_COL_VARIANT(var, CALL)
 #undef CALL
 }
 

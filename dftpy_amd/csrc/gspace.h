@@ -57,6 +57,9 @@ struct GParams {
     double lbda, alpha, beta;
     double rho0;             // used when rho0_dev == nullptr
     const double *rho0_dev;  // cached rho0 on device
+    // with rho0_dev: the WT constants (tkf, wtfac) of the cached rho0, which
+    // the kernel that updates the cache computes once (null: per thread)
+    const double *wt_dev = nullptr;
     long long bs;
     // kind 3: a radial table of |G| (linear interpolation, constant past the
     // end), e.g. the optimizer's preconditioner
@@ -133,11 +136,20 @@ __device__ __forceinline__ double gfactor(const GParams &p, int kind, int ix, in
     return f * p.scale;
 }
 
-__device__ __forceinline__ void wt_consts(const GParams &p, double &tkf, double &wtfac) {
-    const double rho0 = p.rho0_dev ? *p.rho0_dev : p.rho0;
+// 2 k_F of rho0 and the WT kernel prefactor (wt.py:66, kernel.py:325-334)
+__device__ __forceinline__ void wt_consts_at(double rho0, double alpha, double beta, double &tkf, double &wtfac) {
     tkf = 2.0 * cbrt(3.0 * rho0 * M_PI * M_PI);
     const double ctf = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
-    wtfac = 5.0 / (9.0 * p.alpha * p.beta * pow(rho0, p.alpha + p.beta - 5.0 / 3.0)) * ctf;
+    wtfac = 5.0 / (9.0 * alpha * beta * pow(rho0, alpha + beta - 5.0 / 3.0)) * ctf;
+}
+
+__device__ __forceinline__ void wt_consts(const GParams &p, double &tkf, double &wtfac) {
+    if (p.wt_dev) {
+        tkf = p.wt_dev[0];
+        wtfac = p.wt_dev[1];
+        return;
+    }
+    wt_consts_at(p.rho0_dev ? *p.rho0_dev : p.rho0, p.alpha, p.beta, tkf, wtfac);
 }
 
 }  // namespace ofdft

@@ -89,6 +89,11 @@ struct Prologue {
 
 struct LmgpState;  // LMGP nonlocal KEDF tables and workspace (lmgp.hip)
 
+// variants of the size-specialised FFT passes the autotuner times: the row
+// passes have 3 tiles; the column passes 6, tile var % 3 and radix order var / 3
+constexpr int kTileVariants = 3;
+constexpr int kColVariants = 6;
+
 // device scalars of a plan (FP64): [0, 256) the solvers' reduction results
 // and step scalars, [256, 1024) the state of the device-driven TN iteration
 constexpr int kScalSlots = 1024;
@@ -196,6 +201,9 @@ template <typename T>
 void bench_rows_r2c(Plan &P, const T *in, cpx<T> *out, int batch, const Prologue *pro, hipStream_t s);
 template <typename T>
 void bench_rows_c2r(Plan &P, const cpx<T> *in, T *out, int batch, hipStream_t s);
+// the fused x pass of the evaluation (vW, WT and Hartree spectra) on the
+// plan's spectrum buffer; false where the x axis has no fused kernel (eval.hip)
+template <typename T> bool bench_xfused(Plan &P, hipStream_t s);
 // an evaluation with the energy-density / second-derivative fields (eval.hip)
 template <typename T>
 void eval_fields(Plan &P, const T *rho, unsigned calc, T *pot, double *energies, T *edens, T *v2, hipStream_t s);

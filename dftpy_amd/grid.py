@@ -32,13 +32,14 @@ class Plan:
             self.autotune()
 
     def autotune(self):
-        """Pick the fastest tile variant of each FFT pass (ofdft_plan_autotune)."""
+        """Pick the fastest variant of each FFT pass (ofdft_plan_autotune): tiles
+        of the row passes; tiles and radix orders of the column passes."""
         v = np.zeros(4, dtype=np.int32)
-        t = np.zeros(12, dtype=np.float32)
+        t = np.zeros(24, dtype=np.float32)
         _lib.check(_lib.lib.ofdft_plan_autotune(self.handle, _lib.iptr(v), t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                                 _lib.stream()))
         self.variants = v.tolist()
-        self.variant_ms = t.reshape(4, 3)
+        self.variant_ms = t.reshape(4, 6)
         return self.variants
 
     def set_variants(self, variants):

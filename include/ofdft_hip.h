@@ -76,11 +76,14 @@ int ofdft_plan_set_lattice(ofdft_plan *plan, const double lattice[9]);
  * default) uses the size-specialised kernels for the lengths that have them,
  * 0 forces the generic mixed-radix kernels everywhere. */
 int ofdft_plan_set_fft_kernels(ofdft_plan *plan, int fixed);
-/* FFT autotuner (no counterpart in the reference): times the tile variants
- * (0..2: default tile, halved, quartered) of each size-specialised pass on the
- * plan's shapes and keeps the fastest.  variants (optional, 4 ints): chosen
- * variant of the z R2C, z C2R, x and y passes; times (optional, 12 floats):
- * mean ms of the three variants of each.  Synchronises stream. */
+/* FFT autotuner (no counterpart in the reference): times the variants of each
+ * size-specialised pass on the plan's shapes and keeps the fastest.  The z
+ * R2C and C2R row passes have 3 (0..2: default tile, halved, quartered); the
+ * x and y column passes 6: tile v % 3 with the radix order v / 3 (0: radix
+ * 8s first, as the length factorises; 1: reversed, odd radices first).
+ * variants (optional, 4 ints): chosen variant of the z R2C, z C2R, x and y
+ * passes; times (optional, 24 floats): mean ms of each variant, 6 per pass
+ * in that order (-1 where a pass has fewer).  Synchronises stream. */
 int ofdft_plan_autotune(ofdft_plan *plan, int *variants, float *times, void *stream);
 /* Set the tile variants explicitly (same order as above). */
 int ofdft_plan_set_variants(ofdft_plan *plan, const int *variants);
