@@ -97,8 +97,9 @@ __global__ void __launch_bounds__(256) k_cnn_in_mfma(Geo g, const float *__restr
         for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int k = kb * 32 + kq * 8 + e;
-                wa[kb][nt][e] = k < 2 * NTAP ? (_Float16)w[(k >> 1) * 64 + (k & 1) * 32 + nt * 16 + m] : (_Float16)0.0f;
+                const int k = kb * 32 + kq * 8 + e, kc = min(k, 2 * NTAP - 1);
+                const float v = w[(kc >> 1) * 64 + (kc & 1) * 32 + nt * 16 + m];  // unconditional load
+                wa[kb][nt][e] = k < 2 * NTAP ? (_Float16)v : (_Float16)0.0f;
             }
     float bl[2][4];
 #pragma unroll
@@ -146,6 +147,8 @@ __global__ void __launch_bounds__(256) k_cnn_in_mfma(Geo g, const float *__restr
 constexpr int IN_ROWS_THREADS = 256;
 constexpr int IN_ROWS_MAX_NZ = 512;
 
+// CHUNKS: 16-byte row chunks per lane (nz <= 256 * CHUNKS)
+template <int CHUNKS>
 __global__ void __launch_bounds__(IN_ROWS_THREADS)
     k_cnn_in_rows(Geo g, const float *__restrict__ x0, const float *__restrict__ w, const float *__restrict__ b,
                   __bf16 *__restrict__ h) {
@@ -156,20 +159,42 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
     const int y0 = (gy / d) * 4 * d + gy % d;
     const long long nyz = (long long)g.ny * nz;
     // stage: row q = (i * 6 + r) * 2 + c holds xs[q * nz + z]; wave w takes
-    // rows w, w + 4, ..., 16 bytes per lane per load (nz % 16 == 0)
+    // rows w, w + 4, ..., 16 bytes per lane per load (nz % 16 == 0).  Every
+    // load (and the weights' below) is issued before any is used, from a
+    // clamped address, so a workgroup waits for one memory round trip
+    // instead of one per row
     constexpr int NW = IN_ROWS_THREADS / 64;
+    constexpr int RPW = 36 / NW;  // staged rows per wave
     const int nz4 = nz >> 2;
+    float4 stv[RPW][CHUNKS];
 #pragma unroll
-    for (int u = 0; u < 36 / NW; ++u) {
+    for (int u = 0; u < RPW; ++u) {
         const int q = wave + NW * u;
         const int c = q & 1, r = (q >> 1) % 6, i = q / 12;
         const int xp = wrapi(x + (i - 1) * d, g.nx), yr = wrapi(y0 + (r - 1) * d, g.ny);
         const float4 *src = reinterpret_cast<const float4 *>(x0 + (long long)c * g.n + (long long)xp * nyz +
                                                              (long long)yr * nz);
-        f16x4 *dst = reinterpret_cast<f16x4 *>(xs + q * nz);
-        for (int z4 = lane; z4 < nz4; z4 += 64) {
-            const float4 v = src[z4];
-            dst[z4] = f16x4{(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
+#pragma unroll
+        for (int ch = 0; ch < CHUNKS; ++ch) stv[u][ch] = src[min(lane + 64 * ch, nz4 - 1)];
+    }
+    float wl[2][2][8];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = min(kb * 32 + kq * 8 + e, 2 * NTAP - 1);
+                wl[kb][nt][e] = w[(k >> 1) * 64 + (k & 1) * 32 + nt * 16 + m];
+            }
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+        f16x4 *dst = reinterpret_cast<f16x4 *>(xs + (wave + NW * u) * nz);
+#pragma unroll
+        for (int ch = 0; ch < CHUNKS; ++ch) {
+            const int z4 = lane + 64 * ch;
+            const float4 v = stv[u][ch];
+            if (z4 < nz4) dst[z4] = f16x4{(_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
         }
     }
     f16x8 wa[2][2];
@@ -178,10 +203,8 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = kb * 32 + kq * 8 + e;
-                wa[kb][nt][e] = k < 2 * NTAP ? (_Float16)w[(k >> 1) * 64 + (k & 1) * 32 + nt * 16 + m] : (_Float16)0.0f;
-            }
+            for (int e = 0; e < 8; ++e)
+                wa[kb][nt][e] = kb * 32 + kq * 8 + e < 2 * NTAP ? (_Float16)wl[kb][nt][e] : (_Float16)0.0f;
     float bl[2][4];
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt)
@@ -200,7 +223,9 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
             lsel[kb][e] = t % 3;
         }
     // per-wave staging of one output tile (16 points x 32 channels, bf16)
-    // for coalesced 16-byte stores
+    // for coalesced 16-byte stores; the wave-scope fences around the stage
+    // order its LDS accesses only ("local"), so a tile's global stores stay
+    // in flight while the next tile computes
     bf16x4 *ot = reinterpret_cast<bf16x4 *>(xs + 36 * nz) + wave * 128;
     __syncthreads();
     const int tiles_per_row = nz / 16, ntile = 4 * tiles_per_row;
@@ -230,16 +255,16 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
             for (int j = 0; j < 4; ++j) o[j] = (__bf16)fmaxf(acc[nt][j] + bl[nt][j], 0.0f);
             ot[m * 8 + nt * 4 + kq] = o;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
         // lane l stores 16 bytes: point l / 4, channels (l % 4) * 8 .. +7
         const long long p0 = (long long)x * nyz + (long long)(y0 + k * d) * nz + zt * 16;
         const bf16x8 v = reinterpret_cast<const bf16x8 *>(ot)[lane];
         reinterpret_cast<bf16x8 *>(h)[(p0 + (lane >> 2)) * 4 + (lane & 3)] = v;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
     }
 }
 
@@ -804,7 +829,8 @@ __global__ void __launch_bounds__(PROJ_THREADS) k_cnn_out_proj16(Geo g, const __
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int t = nt * 16 + m;
-            wb[nt][e] = t < NTAP ? (_Float16)w[t * CH + kq * 8 + e] : (_Float16)0.0f;
+            const float v = w[min(t, NTAP - 1) * CH + kq * 8 + e];  // unconditional: all in flight together
+            wb[nt][e] = t < NTAP ? (_Float16)v : (_Float16)0.0f;
         }
     const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h);
     _Float16 *st = reinterpret_cast<_Float16 *>(stage[wave]);
@@ -828,9 +854,9 @@ __global__ void __launch_bounds__(PROJ_THREADS) k_cnn_out_proj16(Geo g, const __
                         f16x4{(_Float16)d[0], (_Float16)d[1], (_Float16)d[2], (_Float16)d[3]};
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
         // chunk c = tap * 8 + k8 holds points k8*8 .. +7 of that tap
         for (int c = lane; c < NTAP * 8; c += 64) {
             const int t = c >> 3, k8 = c & 7;
@@ -844,9 +870,9 @@ __global__ void __launch_bounds__(PROJ_THREADS) k_cnn_out_proj16(Geo g, const __
                     if (q + j < g.n) dst[j] = v[j];
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
     }
 }
 
@@ -1263,8 +1289,12 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
             (reinterpret_cast<uintptr_t>(x0) & 15) == 0) {
             // staged features (16-byte row loads) + one 1 KB output tile per wave
             const size_t lds = (size_t)36 * g.nz * sizeof(_Float16) + (size_t)(IN_ROWS_THREADS / 64) * 1024;
-            hipLaunchKernelGGL(k_cnn_in_rows, dim3((unsigned)(g.ny / 4), (unsigned)g.nx), dim3(IN_ROWS_THREADS), lds,
-                               St(stream), g, x0, w, b, reinterpret_cast<__bf16 *>(h));
+            if (g.nz <= 256)
+                hipLaunchKernelGGL(k_cnn_in_rows<1>, dim3((unsigned)(g.ny / 4), (unsigned)g.nx), dim3(IN_ROWS_THREADS),
+                                   lds, St(stream), g, x0, w, b, reinterpret_cast<__bf16 *>(h));
+            else
+                hipLaunchKernelGGL(k_cnn_in_rows<2>, dim3((unsigned)(g.ny / 4), (unsigned)g.nx), dim3(IN_ROWS_THREADS),
+                                   lds, St(stream), g, x0, w, b, reinterpret_cast<__bf16 *>(h));
         } else {
             const long long ntiles = (g.n + 15) / 16;
             // 8 tiles per wave amortise the weight loads
