@@ -11,8 +11,11 @@
 // Normalisation is left to the caller through `scale`, applied on the final
 // store of the last pass.  The DFTpy conventions (forward * dV, inverse
 // / (N dV), field.py:373-390,739-764) are applied by the C ABI layer.
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "dft.h"
 #include "internal.h"
@@ -658,12 +661,34 @@ template <typename T> static void autotune_t(Plan &P, hipStream_t s, float *time
             });
 }
 
+// The variants chosen for a grid shape and precision are kept for the rest of
+// the process: every later plan of that shape takes them without timing, so
+// all plans of a shape transform identically (the radix order changes the
+// rounding) whichever thread or stream created them.
 void plan_autotune(Plan &P, hipStream_t s, float *times) {
     if (!P.use_fixed) return;
+    using Key = std::array<int, 5>;
+    static std::mutex mu;
+    static std::map<Key, std::array<int, 5>> chosen;
+    const Key key{P.nr[0], P.nr[1], P.nr[2], P.prec, P.full};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = chosen.find(key);
+        if (it != chosen.end()) {
+            P.var_r2c = it->second[0];
+            P.var_c2r = it->second[1];
+            for (int d = 0; d < 3; ++d) P.var_cols[d] = it->second[2 + d];
+            if (times)
+                for (int i = 0; i < 4 * kColVariants; ++i) times[i] = -1.0f;
+            return;
+        }
+    }
     if (P.prec == OFDFT_FP64)
         autotune_t<double>(P, s, times);
     else
         autotune_t<float>(P, s, times);
+    std::lock_guard<std::mutex> lk(mu);
+    chosen.emplace(key, std::array<int, 5>{P.var_r2c, P.var_c2r, P.var_cols[0], P.var_cols[1], P.var_cols[2]});
 }
 
 template void bench_rows_r2c<float>(Plan &, const float *, float2 *, int, const Prologue *, hipStream_t);

@@ -569,7 +569,6 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
         OFDFT_HIP(hipGetLastError());
         const double c = 4.0 * M_PI / P.volume;
         reduce_final(P.red, nbg, 1, res + 1, &c, s);
-        OFDFT_HIP(hipStreamSynchronize(s));
     } else {
         BTab none{nullptr, nullptr, nullptr};
         hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, nullptr, P.Nc, g, eta, none, nat,
@@ -578,10 +577,10 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
         const double c = 4.0 * M_PI / P.volume;
         reduce_final(P.red, nbg, 1, res + 1, &c, s);
     }
-    double h[2];
-    OFDFT_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
-    OFDFT_HIP(hipStreamSynchronize(s));
-    const double e_real = h[0] / 2.0, e_rec = h[1];
+    // both sums in one copy to the plan's pinned buffer, waited for by polling
+    OFDFT_HIP(hipMemcpyAsync(P.pinned, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    stream_wait(s);
+    const double e_real = P.pinned[0] / 2.0, e_rec = P.pinned[1];
     out[0] = e_corr + e_real + e_rec;
     out[1] = e_corr;
     out[2] = e_real;

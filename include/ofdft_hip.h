@@ -83,7 +83,10 @@ int ofdft_plan_set_fft_kernels(ofdft_plan *plan, int fixed);
  * 8s first, as the length factorises; 1: reversed, odd radices first).
  * variants (optional, 4 ints): chosen variant of the z R2C, z C2R, x and y
  * passes; times (optional, 24 floats): mean ms of each variant, 6 per pass
- * in that order (-1 where a pass has fewer).  Synchronises stream. */
+ * in that order (-1 where a pass has fewer).  The choice for a grid shape and
+ * precision is kept for the process: later plans of that shape take it
+ * without timing (times all -1), so every plan of a shape gives identical
+ * transforms.  Synchronises stream. */
 int ofdft_plan_autotune(ofdft_plan *plan, int *variants, float *times, void *stream);
 /* Set the tile variants explicitly (same order as above). */
 int ofdft_plan_set_variants(ofdft_plan *plan, const int *variants);

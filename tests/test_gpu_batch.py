@@ -23,7 +23,9 @@ def test_batch_matches_sequential():
 def test_batch_graph_tn_matches_host_tn():
     """The batch with the device-driven TN (one captured hipGraph per plan,
     updated in place from cell to cell, captures in several threads) gives
-    the host-driven TN's energies and iteration counts cell by cell."""
+    the host-driven TN's iteration counts cell by cell and its energies to
+    1e-8 Ha/atom (the two differ in the last bits of cos/sin of the line
+    search step, which FP32 fields can carry into the trajectory)."""
     from dftpy_amd.batch import make_cells, run_batch
     cells = make_cells(6, seed=4)
     res, _ = run_batch(cells, workers=3)
@@ -32,7 +34,7 @@ def test_batch_graph_tn_matches_host_tn():
     for a, b in zip(res, resg):
         assert a["status"] == b["status"] == 0
         assert a["niter"] == b["niter"] and a["evals"] == b["evals"]
-        assert abs(a["energy"] - b["energy"]) < 1e-9 * a["nat"]
+        assert abs(a["energy"] - b["energy"]) < 1e-8 * a["nat"]
 
 
 def test_batch_fp32_energies_vs_fp64():
