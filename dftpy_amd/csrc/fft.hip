@@ -671,23 +671,22 @@ void plan_autotune(Plan &P, hipStream_t s, float *times) {
     static std::mutex mu;
     static std::map<Key, std::array<int, 5>> chosen;
     const Key key{P.nr[0], P.nr[1], P.nr[2], P.prec, P.full};
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = chosen.find(key);
-        if (it != chosen.end()) {
-            P.var_r2c = it->second[0];
-            P.var_c2r = it->second[1];
-            for (int d = 0; d < 3; ++d) P.var_cols[d] = it->second[2 + d];
-            if (times)
-                for (int i = 0; i < 4 * kColVariants; ++i) times[i] = -1.0f;
-            return;
-        }
+    // held through the timing, so plans of one shape created concurrently
+    // wait for the first one's choice
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = chosen.find(key);
+    if (it != chosen.end()) {
+        P.var_r2c = it->second[0];
+        P.var_c2r = it->second[1];
+        for (int d = 0; d < 3; ++d) P.var_cols[d] = it->second[2 + d];
+        if (times)
+            for (int i = 0; i < 4 * kColVariants; ++i) times[i] = -1.0f;
+        return;
     }
     if (P.prec == OFDFT_FP64)
         autotune_t<double>(P, s, times);
     else
         autotune_t<float>(P, s, times);
-    std::lock_guard<std::mutex> lk(mu);
     chosen.emplace(key, std::array<int, 5>{P.var_r2c, P.var_c2r, P.var_cols[0], P.var_cols[1], P.var_cols[2]});
 }
 
