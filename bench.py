@@ -214,8 +214,7 @@ def cnn_layer_timings(net, nr, iters=20):
                                                   net.param("b%d" % l).data_ptr(), None, None, h[1].data_ptr(), None,
                                                   s))
     calls["conv_out"] = lambda: L.ofdft_cnn_conv_out(nrp, net.dil[5], h[0].data_ptr(), net.param("W6").data_ptr(),
-                                                      net.param("b6").data_ptr(), b["out"].data_ptr(),
-                                                      b["proj"].data_ptr(), s)
+                                                      net.param("b6").data_ptr(), b["out"].data_ptr(), None, s)
     out = {}
     for name, fn in calls.items():
         _lib.check(fn())

@@ -12,8 +12,8 @@
 //   h1 = relu(conv(x0; W1) + b1)                          FP16 MFMA (K = 54)
 //   h_l = h_{l-1} + relu(conv(h_{l-1}; W_l) + b_l), l=2..5  BF16 MFMA, FP32 acc
 //   out = conv(h5; W6) + b6                               FP16 MFMA projection
-//                                                         onto 27 tap planes
-//                                                         (FP16) + gather
+//                                                         onto the 27 taps,
+//                                                         summed per point
 //
 // The 32 -> 32 layers are an implicit GEMM per tap: D[p][co] += X[p+off][ci]
 // W[co][ci] with v_mfma_f32_16x16x32_bf16 (16 points x 16 channels x 32 input
@@ -22,6 +22,7 @@
 // points (K = points) through LDS-transposed tiles.
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 
 #include "internal.h"
 #include "reduce.h"
@@ -782,32 +783,6 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
 }
 
 // ------------------------------------------------------------ layer 6 (32 -> 1)
-__global__ void __launch_bounds__(256) k_cnn_out(Geo g, const __bf16 *__restrict__ h, const float *__restrict__ w,
-                                                 const float *__restrict__ b, float *__restrict__ out) {
-    __shared__ float ws[KOUT];
-    for (int i = threadIdx.x; i < KOUT; i += blockDim.x) ws[i] = w[i];
-    __syncthreads();
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.n) return;
-    int x, y, z;
-    decompose(g, p, x, y, z);
-    Nbr nb;
-    nb.init(g, x, y, z);
-    float acc = b[0];
-    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h);
-#pragma unroll 3
-    for (int t = 0; t < NTAP; ++t) {
-        const long long q = nb.at(t);
-#pragma unroll
-        for (int c4 = 0; c4 < 4; ++c4) {
-            const bf16x8 v = hv[q * 4 + c4];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) acc += (float)v[e] * ws[t * CH + c4 * 8 + e];
-        }
-    }
-    out[p] = acc;
-}
-
 // MFMA projection: u[t][q] = sum_c W6[t][c] h[q][c] as 27 FP16 planes (the
 // hidden state converts to FP16 exactly in its range, W6 is rounded to FP16).
 // The hidden points are the A operand (rows = 16 consecutive points), W6 the
@@ -893,6 +868,137 @@ __global__ void __launch_bounds__(256) k_cnn_out_gather16(Geo g, const _Float16 
         for (int t = 0; t < NTAP; ++t)
             acc += (float)u[(long long)t * g.n + bx[t / 9] + by[(t / 3) % 3] + bz[t % 3]];
         out[bx[1] + by[1] + z] = acc;
+    }
+}
+
+// The projection form in one pass, without the tap planes.  A workgroup owns
+// TY output rows (y) x zt output points (z) and walks x along x_k = c + k d
+// (mod nx), so the planes x - d, x, x + d of an output plane are consecutive
+// steps.  A step projects one input plane's rows y0 - d .. y0 + TY - 1 + d and
+// points z0 - d .. z0 + zt - 1 + d onto the 27 taps (FP16 MFMA with the
+// operands of k_cnn_out_proj16) into LDS, then adds each output point's 9
+// taps of that plane to one of three accumulators: taps 0-8 start plane
+// x_{k+1} (from the bias), taps 9-17 go to x_k, taps 18-26 complete x_{k-1},
+// which is written.  Every output is summed in the tap order of
+// k_cnn_out_gather16, so the two forms agree bit for bit.  The next plane's
+// loads are issued before the tap sums of the current one.
+constexpr int OUTW_THREADS = 512;
+constexpr int OUTW_NW = OUTW_THREADS / 64;
+constexpr int OUTW_MAXT = 8;                        // 16-point MFMA tiles per wave and step
+constexpr int OUTW_NP = OUTW_NW * OUTW_MAXT * 16;  // input points per step
+constexpr int OUTW_TS = OUTW_NP + 4;               // LDS tap stride: 16 taps' stores hit distinct banks
+
+struct OutWalk {
+    int zt, cols, npts, ntiles;  // output points per row of a block; input points per row, per step; MFMA tiles
+    int nyt;                     // blocks along y
+    int cyc, len, xc;            // gcd(nx, d) walk cycles of nx / cyc planes; xc output planes per block
+};
+
+template <int TY>
+__global__ void __launch_bounds__(OUTW_THREADS) k_cnn_out_walk(Geo g, OutWalk wk, const __bf16 *__restrict__ h,
+                                                               const float *__restrict__ w,
+                                                               const float *__restrict__ b, float *__restrict__ out) {
+    __shared__ _Float16 P[NTAP * OUTW_TS];  // [tap][input point of the step]
+    constexpr int OPT = 2;                  // TY * zt < OUTW_NP: at most 2 output points per thread
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
+    const int yb = blockIdx.x % wk.nyt, zb = blockIdx.x / wk.nyt;
+    const int y0 = yb * TY, z0 = zb * wk.zt;
+    const int cyc = blockIdx.y % wk.cyc;
+    const int ka = (blockIdx.y / wk.cyc) * wk.xc, kb = min(wk.len, ka + wk.xc);
+    const int nsteps = kb - ka + 2;
+    const int nyz = g.ny * g.nz, d = g.d, cols = wk.cols;
+    f16x8 wb[2];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = nt * 16 + m;
+            const float v = w[min(t, NTAP - 1) * CH + kq * 8 + e];
+            wb[nt][e] = t < NTAP ? (_Float16)v : (_Float16)0.0f;
+        }
+    const float b0 = b[0];
+    // in-plane offsets of this lane's input points (clamped into the step's
+    // points), kept in LDS: each thread reads back only its own entries
+    __shared__ int offs[OUTW_MAXT * OUTW_THREADS];
+#pragma unroll
+    for (int k = 0; k < OUTW_MAXT; ++k) {
+        const int p = min((wave + k * OUTW_NW) * 16 + m, wk.npts - 1);
+        const int r = p / cols, c = p - r * cols;
+        const int y = ((y0 - d + r) % g.ny + g.ny) % g.ny, z = ((z0 - d + c) % g.nz + g.nz) % g.nz;
+        offs[k * OUTW_THREADS + threadIdx.x] = y * g.nz + z;
+    }
+    // this thread's output points: LDS index of the centre input point, in-plane output offset
+    int oin[OPT], oout[OPT];
+    bool ov[OPT];
+#pragma unroll
+    for (int i = 0; i < OPT; ++i) {
+        const int o = min((int)threadIdx.x + i * OUTW_THREADS, TY * wk.zt - 1);
+        const int r = o / wk.zt, c = o - r * wk.zt;
+        ov[i] = (int)threadIdx.x + i * OUTW_THREADS < TY * wk.zt && y0 + r < g.ny && z0 + c < g.nz;
+        oin[i] = (r + d) * cols + c + d;
+        oout[i] = (y0 + r) * g.nz + z0 + c;
+    }
+    const int dr = d * cols;
+    auto plane = [&](int k) -> long long {  // x_k (k taken modulo the cycle length)
+        const int kk = (k % wk.len + wk.len) % wk.len;
+        return ((long long)cyc + (long long)kk * d) % g.nx;
+    };
+    const bf16x8 *hv = reinterpret_cast<const bf16x8 *>(h);
+    bf16x8 cur[OUTW_MAXT];
+    auto load = [&](int k) {
+        const long long base = plane(k) * nyz;
+#pragma unroll
+        for (int j = 0; j < OUTW_MAXT; ++j) cur[j] = hv[(base + offs[j * OUTW_THREADS + threadIdx.x]) * 4 + kq];
+    };
+    float s0[OPT], s1[OPT], s2[OPT];
+#pragma unroll
+    for (int i = 0; i < OPT; ++i) s0[i] = s1[i] = s2[i] = 0.0f;
+    load(ka - 1);
+    auto step = [&](int s, float(&ai)[OPT], float(&am)[OPT], float(&af)[OPT]) {
+#pragma unroll
+        for (int k = 0; k < OUTW_MAXT; ++k) {
+            const int tile = wave + k * OUTW_NW;
+            if (tile < wk.ntiles) {
+                f16x8 a;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) a[e] = (_Float16)(float)cur[k][e];
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) {
+                    const floatx4 dd =
+                        __builtin_amdgcn_mfma_f32_16x16x32_f16(a, wb[nt], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    const int t = nt * 16 + m;  // lane holds points tile*16 + kq*4 + j of tap t
+                    if (t < NTAP)
+                        *reinterpret_cast<f16x4 *>(P + t * OUTW_TS + tile * 16 + kq * 4) =
+                            f16x4{(_Float16)dd[0], (_Float16)dd[1], (_Float16)dd[2], (_Float16)dd[3]};
+                }
+            }
+        }
+        if (s + 1 < nsteps) load(ka + s);
+        __syncthreads();
+        const long long xo = plane(ka - 2 + s) * nyz;  // plane completed by this step
+#pragma unroll
+        for (int i = 0; i < OPT; ++i) {
+            const _Float16 *q = P + oin[i];
+            float a0 = b0;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) a0 += (float)q[t * OUTW_TS + (t / 3 - 1) * dr + (t % 3 - 1) * d];
+            float a1 = am[i];
+#pragma unroll
+            for (int t = 9; t < 18; ++t) a1 += (float)q[t * OUTW_TS + ((t / 3) % 3 - 1) * dr + (t % 3 - 1) * d];
+            float a2 = af[i];
+#pragma unroll
+            for (int t = 18; t < 27; ++t) a2 += (float)q[t * OUTW_TS + ((t / 3) % 3 - 1) * dr + (t % 3 - 1) * d];
+            ai[i] = a0;
+            am[i] = a1;
+            if (s >= 2 && ov[i]) out[xo + oout[i]] = a2;
+        }
+        __syncthreads();
+    };
+#pragma unroll 1
+    for (int s = 0; s < nsteps; s += 3) {
+        step(s, s0, s2, s1);
+        if (s + 1 < nsteps) step(s + 1, s1, s0, s2);
+        if (s + 2 < nsteps) step(s + 2, s2, s1, s0);
     }
 }
 
@@ -1384,6 +1490,31 @@ int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const 
     });
 }
 
+// one-pass output layer: blocks of TY rows x zt points, about 512 workgroups
+// (two per CU) walking at least one output plane each
+static void launch_out_walk(const Geo &g, const void *h, const float *w, const float *b, float *out, hipStream_t st) {
+    constexpr int TY = 8;
+    OutWalk wk;
+    const int rows = TY + 2 * g.d;
+    const int ztmax = OUTW_NP / rows - 2 * g.d;
+    OFDFT_REQUIRE(ztmax >= 1, "dilation too large for the one-pass output layer");
+    const int nzt = (g.nz + ztmax - 1) / ztmax;
+    wk.zt = (g.nz + nzt - 1) / nzt;
+    wk.cols = wk.zt + 2 * g.d;
+    wk.npts = rows * wk.cols;  // <= OUTW_NP, and TY * zt < OUTW_NP
+    wk.ntiles = (wk.npts + 15) / 16;
+    wk.nyt = (g.ny + TY - 1) / TY;
+    wk.cyc = std::gcd(g.nx, g.d);
+    wk.len = g.nx / wk.cyc;
+    const long long byz = (long long)wk.nyt * nzt;
+    int nch = (int)std::min<long long>(wk.len, std::max<long long>(1, (512 + byz * wk.cyc - 1) / (byz * wk.cyc)));
+    wk.xc = (wk.len + nch - 1) / nch;
+    nch = (wk.len + wk.xc - 1) / wk.xc;
+    OFDFT_REQUIRE(byz < (1LL << 31) && (long long)wk.cyc * nch <= 65535, "grid too large for the output layer");
+    hipLaunchKernelGGL(k_cnn_out_walk<TY>, dim3((unsigned)byz, (unsigned)(wk.cyc * nch)), dim3(OUTW_THREADS), 0, st,
+                       g, wk, reinterpret_cast<const __bf16 *>(h), w, b, out);
+}
+
 int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, const float *b, float *out,
                        float *work, void *stream) {
     return cnn_guard([&] {
@@ -1401,8 +1532,7 @@ int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, 
             hipLaunchKernelGGL(k_cnn_out_gather16, dim3((unsigned)g.ny, (unsigned)g.nx), dim3(threads), 0, St(stream),
                                g, u, b, out);
         } else {
-            hipLaunchKernelGGL(k_cnn_out, dim3(blocks_for(g.n, 256)), dim3(256), 0, St(stream), g,
-                               reinterpret_cast<const __bf16 *>(h), w, b, out);
+            launch_out_walk(g, h, w, b, out, St(stream));
         }
         OFDFT_HIP(hipGetLastError());
     });

@@ -216,8 +216,7 @@ class WarmStartNet:
         n = int(np.prod(nr))
         dev = self.device
         bf = lambda: torch.empty((n, C), dtype=torch.bfloat16, device=dev)  # noqa: E731
-        b = {"out": torch.empty(n, dtype=torch.float32, device=dev),
-             "proj": torch.empty(NTAP * n, dtype=torch.float32, device=dev)}
+        b = {"out": torch.empty(n, dtype=torch.float32, device=dev)}
         if train:
             b["h"] = [bf() for _ in range(5)]  # h1..h5
             b["r"] = [bf() for _ in range(4)]  # r2..r5
@@ -255,7 +254,7 @@ class WarmStartNet:
                                             self.param("b%d" % l).data_ptr(), None, None, hout.data_ptr(), r, s))
         hlast = h[4] if train else h[0]
         _lib.check(L.ofdft_cnn_conv_out(nrp, self.dil[5], hlast.data_ptr(), self.param("W6").data_ptr(),
-                                        self.param("b6").data_ptr(), b["out"].data_ptr(), b["proj"].data_ptr(), s))
+                                        self.param("b6").data_ptr(), b["out"].data_ptr(), None, s))
         return b["out"]
 
     def backward(self, x0, nr, dout):

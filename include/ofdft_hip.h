@@ -405,9 +405,11 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
 int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const void *w, const float *bias,
                        const float *resid, const void *mask, void *out0, void *out1, void *stream);
 /* out = conv(h; w) + b: h [N][32] bf16, w [27][32] FP32, b [1], out [N] FP32.
- * work (27 N floats of capacity) selects the projection + gather form: the 27
- * projected tap planes are kept there in FP16 (h and w rounded to FP16 on MFMA);
- * NULL selects the direct FP32 form. */
+ * Each point's 27 tap projections are computed on FP16 MFMA (h and w rounded to
+ * FP16), rounded to FP16 and summed in FP32 in tap order.  work NULL: one pass,
+ * the projections of a plane kept in LDS while the layer walks the grid.  work
+ * (27 N floats of capacity): two passes through 27 FP16 tap planes kept there.
+ * Both forms give the same bits. */
 int ofdft_cnn_conv_out(const int nr[3], int dil, const void *h, const float *w, const float *b, float *out,
                        float *work, void *stream);
 /* dh [N][32] FP32 = d(sum dout*out)/dh; gout (bf16) = mask > 0 ? dh : 0 */

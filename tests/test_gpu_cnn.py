@@ -135,6 +135,45 @@ def test_ring_forward_matches_row_kernel_at_160():
     assert float((a - b).abs().max()) < 1e-2 * float(b.abs().max())
 
 
+@pytest.mark.parametrize("shape,dil", [((12, 10, 15), 1), ((4, 8, 160), 1), ((9, 6, 40), 2), ((8, 7, 300), 3),
+                                       ((160, 160, 160), 1)])
+def test_output_layer_one_pass_equals_tap_planes(shape, dil):
+    """The output layer's one-pass form (work NULL: projections of a plane
+    in LDS while the layer walks x) against the two-pass form through 27 FP16
+    tap planes: the same FP16 projections summed in the same tap order, so the
+    outputs are identical.  Shapes: walk cycles gcd(nx, d) > 1 (9, 6, 40; d 2
+    is coprime to 9 and ny 6 = 3 y-blocks of 2 rows wrap), nz above one LDS
+    block (300), and the 160^3 grid of the 500-atom cell."""
+    from dftpy_amd import _lib
+    n = int(np.prod(shape))
+    g = torch.Generator(device="cuda").manual_seed(11)
+    h = (torch.randn((n, 32), device="cuda", generator=g) * 2).to(torch.bfloat16)
+    w = torch.randn((27, 32), device="cuda", generator=g) * 0.1
+    b = torch.tensor([0.25], device="cuda")
+    nr = np.ascontiguousarray(shape, dtype=np.int32)
+    one = torch.full((n,), float("nan"), device="cuda")
+    two = torch.full((n,), float("nan"), device="cuda")
+    work = torch.empty(27 * n, device="cuda")
+    s = _lib.stream()
+    L = _lib.lib
+    _lib.check(L.ofdft_cnn_conv_out(_lib.iptr(nr), dil, h.data_ptr(), w.data_ptr(), b.data_ptr(), one.data_ptr(),
+                                    None, s))
+    _lib.check(L.ofdft_cnn_conv_out(_lib.iptr(nr), dil, h.data_ptr(), w.data_ptr(), b.data_ptr(), two.data_ptr(),
+                                    work.data_ptr(), s))
+    assert bool(torch.isfinite(one).all())
+    assert torch.equal(one, two)
+    if n <= 20000:  # and both against the FP64 convolution of the FP16-rounded operands
+        hh = h.float().cpu().numpy().astype(np.float64).reshape(shape + (32,))
+        ww = w.half().double().cpu().numpy()
+        ref = np.full(shape, 0.25)
+        for t in range(27):
+            dx, dy, dz = t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1
+            ref += np.roll(hh @ ww[t], shift=(-dx * dil, -dy * dil, -dz * dil), axis=(0, 1, 2))
+        err = np.abs(one.cpu().numpy().reshape(shape) - ref).max()
+        print("one-pass vs FP64: max %.2e (|ref| max %.2e)" % (err, np.abs(ref).max()))
+        assert err < 2e-3 * np.abs(ref).max()
+
+
 def _oracle_adam(p, x0, tgt, lr, steps):
     from oracle import cnn
     q = {k: v.copy() for k, v in p.items()}
