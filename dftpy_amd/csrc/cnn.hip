@@ -153,8 +153,9 @@ template <int CHUNKS>
 __global__ void __launch_bounds__(IN_ROWS_THREADS)
     k_cnn_in_rows(Geo g, const float *__restrict__ x0, const float *__restrict__ w, const float *__restrict__ b,
                   __bf16 *__restrict__ h) {
-    extern __shared__ _Float16 xs[];  // [3 planes][6 rows][2 channels][nz]
+    extern __shared__ _Float16 xs[];  // [3 planes][6 rows][2 channels][rs]: z = -d .. nz + d - 1 at lp - d ..
     const int nz = g.nz, d = g.d;
+    const int lp = (d + 3) & ~3, rs = nz + 2 * lp;  // 8-byte aligned rows with a periodic halo of d each side
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, m = lane & 15, kq = lane >> 4;
     const int x = blockIdx.y, gy = blockIdx.x;
     const int y0 = (gy / d) * 4 * d + gy % d;
@@ -190,7 +191,7 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
             }
 #pragma unroll
     for (int u = 0; u < RPW; ++u) {
-        f16x4 *dst = reinterpret_cast<f16x4 *>(xs + (wave + NW * u) * nz);
+        f16x4 *dst = reinterpret_cast<f16x4 *>(xs + (wave + NW * u) * rs + lp);
 #pragma unroll
         for (int ch = 0; ch < CHUNKS; ++ch) {
             const int z4 = lane + 64 * ch;
@@ -212,36 +213,42 @@ __global__ void __launch_bounds__(IN_ROWS_THREADS)
 #pragma unroll
         for (int j = 0; j < 4; ++j) bl[nt][j] = b[nt * 16 + kq * 4 + j];
     // this lane's 16 operand slots k = 2 t + c: LDS offset of (plane i, row
-    // j, channel c) relative to output row 0, and which z shift l they take
-    int off[2][8], lsel[2][8];
+    // j, channel c, z shift) relative to output row 0 and point z = 0; the
+    // 10 padding slots (zero weights) read the centre tap
+    int off[2][8];
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int kk = kb * 32 + kq * 8 + e;
-            const int t = kk >> 1, c = kk & 1;
-            off[kb][e] = kk < 2 * NTAP ? (((t / 9) * 6 + (t / 3) % 3) * 2 + c) * nz : -1;
-            lsel[kb][e] = t % 3;
+            const int t = kk < 2 * NTAP ? kk >> 1 : 13, c = kk < 2 * NTAP ? kk & 1 : 0;
+            off[kb][e] = (((t / 9) * 6 + (t / 3) % 3) * 2 + c) * rs + lp + (t % 3 - 1) * d;
         }
     // per-wave staging of one output tile (16 points x 32 channels, bf16)
     // for coalesced 16-byte stores; the wave-scope fences around the stage
     // order its LDS accesses only ("local"), so a tile's global stores stay
     // in flight while the next tile computes
-    bf16x4 *ot = reinterpret_cast<bf16x4 *>(xs + 36 * nz) + wave * 128;
+    bf16x4 *ot = reinterpret_cast<bf16x4 *>(xs + 36 * rs) + wave * 128;
+    __syncthreads();
+    // periodic halo: z = -d .. -1 and nz .. nz + d - 1 of every staged row
+    for (int q = threadIdx.x; q < 36 * 2 * d; q += IN_ROWS_THREADS) {
+        const int row = q / (2 * d), e = q - row * 2 * d;
+        _Float16 *r = xs + row * rs + lp;
+        if (e < d)
+            r[e - d] = r[nz - d + e];
+        else
+            r[nz + e - d] = r[e - d];
+    }
     __syncthreads();
     const int tiles_per_row = nz / 16, ntile = 4 * tiles_per_row;
     for (int tile = wave; tile < ntile; tile += IN_ROWS_THREADS / 64) {
         const int k = tile / tiles_per_row, zt = tile - k * tiles_per_row;
-        const int zc = zt * 16 + m;
-        const int zm = wrapi(zc - d, nz), zp = wrapi(zc + d, nz);
+        const _Float16 *xk = xs + k * 2 * rs + zt * 16 + m;
         f16x8 xb[2];
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int zz = lsel[kb][e] == 0 ? zm : (lsel[kb][e] == 1 ? zc : zp);
-                xb[kb][e] = off[kb][e] >= 0 ? xs[off[kb][e] + k * 2 * nz + zz] : (_Float16)0.0f;
-            }
+            for (int e = 0; e < 8; ++e) xb[kb][e] = xk[off[kb][e]];
         floatx4 acc[2] = {floatx4{0.f, 0.f, 0.f, 0.f}, floatx4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
@@ -1391,10 +1398,11 @@ int ofdft_cnn_conv_in(const int nr[3], int dil, const float *x0, const float *w,
     return cnn_guard([&] {
         const Geo g = make_geo(nr, dil);
         OFDFT_REQUIRE(x0 && w && b && h, "null pointer");
-        if (g.nz % 16 == 0 && g.nz <= IN_ROWS_MAX_NZ && g.ny % (4 * g.d) == 0 &&
+        if (g.nz % 16 == 0 && g.nz <= IN_ROWS_MAX_NZ && g.ny % (4 * g.d) == 0 && g.d <= 64 &&
             (reinterpret_cast<uintptr_t>(x0) & 15) == 0) {
             // staged features (16-byte row loads) + one 1 KB output tile per wave
-            const size_t lds = (size_t)36 * g.nz * sizeof(_Float16) + (size_t)(IN_ROWS_THREADS / 64) * 1024;
+            const size_t lds = (size_t)36 * (g.nz + 2 * ((g.d + 3) & ~3)) * sizeof(_Float16) +
+                               (size_t)(IN_ROWS_THREADS / 64) * 1024;
             if (g.nz <= 256)
                 hipLaunchKernelGGL(k_cnn_in_rows<1>, dim3((unsigned)(g.ny / 4), (unsigned)g.nx), dim3(IN_ROWS_THREADS),
                                    lds, St(stream), g, x0, w, b, reinterpret_cast<__bf16 *>(h));

@@ -55,6 +55,22 @@ def test_forward_matches_oracle(train, shape):
     assert np.abs(out - ref).max() < 2e-2 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("dil", [(2, 1, 1, 1, 1, 1), (3, 1, 1, 1, 1, 2)])
+def test_forward_first_layer_dilations(dil):
+    """The row-staged first layer with d > 1 (its periodic z halo in LDS is d
+    points each side) and the one-pass output layer with d = 2."""
+    from oracle import cnn
+    shape = (5, 24, 48)  # ny % 4d == 0 for d = 2, 3
+    p, x0, _ = _setup(4, shape=shape)
+    ref, _ = cnn.forward(p, x0, dil, bf16=True)
+    net = _net(p, dil)
+    x = torch.as_tensor(x0.reshape(2, -1), dtype=torch.float32, device="cuda")
+    out = net.forward(x, shape).cpu().numpy().astype(np.float64).reshape(shape)
+    print("forward rel L2 %.2e" % _rel(out, ref))
+    assert _rel(out, ref) < 2e-3
+    assert np.abs(out - ref).max() < 2e-2 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_backward_matches_oracle(shape):
     from oracle import cnn
