@@ -48,21 +48,28 @@ class Ions:
         return self.nat
 
     def get_scaled_positions(self):
-        """Unwrapped fractional coordinates (the reference's Ions carry no pbc)."""
-        return np.linalg.solve(self.cell.T, self.positions.T).T
+        """Unwrapped fractional coordinates (the reference's Ions carry no pbc):
+        f with f @ cell = positions."""
+        return self.positions @ np.linalg.inv(self.cell)
 
     def get_positions(self):
         return self.positions.copy()
 
     def species_index(self, keys):
-        """int32 index of each atom's symbol in the sorted list ``keys``."""
-        keys = np.asarray(keys)
-        sym = np.asarray(self.symbols)
-        idx = np.searchsorted(keys, sym)
-        if idx.size and (idx.max() >= keys.size or not np.array_equal(keys[idx], sym)):
-            missing = sorted(set(self.symbols) - set(keys.tolist()))
+        """int32 index of each atom's symbol in the sorted list ``keys`` (kept
+        for the current symbols, which are compared on every call)."""
+        keys = tuple(keys)
+        hit = self.__dict__.get("_species_index")
+        if hit is not None and hit[0] == keys and hit[1] == self.symbols:
+            return hit[2]
+        pos = {k: i for i, k in enumerate(keys)}
+        missing = sorted(set(self.symbols) - set(keys))
+        if missing:
             raise ValueError("no table for species %s" % missing)
-        return idx.astype(np.int32)
+        idx = np.fromiter((pos[s] for s in self.symbols), dtype=np.int32, count=len(self.symbols))
+        idx.flags.writeable = False
+        self._species_index = (keys, list(self.symbols), idx)
+        return idx
 
     @property
     def symbols_uniq(self):

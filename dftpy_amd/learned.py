@@ -117,19 +117,21 @@ def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
     if ck not in cache:
         tabs = [response_table(readpp.gp[k], readpp.vp[k], readpp.zval[k], rho0) for k in keys]
         coefs = [np.ascontiguousarray(t.coef.reshape(-1)) for t in tabs]
-        cache[ck] = (np.array([t.q0 for t in tabs]), np.array([t.dq for t in tabs]),
-                     np.array([t.coef.shape[0] for t in tabs], dtype=np.int32), np.array([t.qmax for t in tabs]),
-                     np.array([v for t in tabs for v in t.vsmall]), np.array([t.dqs for t in tabs]), coefs,
-                     (_lib._DP * ns)(*[_lib.dptr(c) for c in coefs]))
-    q0, dq, nco, qmax, vsmall, dqs, coefs, cptr = cache[ck]
+        arrs = (np.array([t.q0 for t in tabs]), np.array([t.dq for t in tabs]),
+                np.array([t.coef.shape[0] for t in tabs], dtype=np.int32), np.array([t.qmax for t in tabs]),
+                np.array([v for t in tabs for v in t.vsmall]), np.array([t.dqs for t in tabs]))
+        # the arrays stay referenced by the cache entry, so their pointers stay valid
+        cptr = (_lib._DP * ns)(*[_lib.dptr(c) for c in coefs])
+        ptrs = (_lib.dptr(arrs[0]), _lib.dptr(arrs[1]), _lib.iptr(arrs[2]), cptr, _lib.dptr(arrs[3]),
+                _lib.dptr(arrs[4]), _lib.dptr(arrs[5]))
+        cache[ck] = (arrs, coefs, ptrs)
+    ptrs = cache[ck][2]
     pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
     frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
     spec = ions.species_index(keys)
     out = torch.empty(tuple(int(n) for n in grid.nrR), dtype=torch.float64, device=grid.device)
-    _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, _lib.dptr(q0), _lib.dptr(dq), _lib.iptr(nco), cptr,
-                                       _lib.dptr(qmax), _lib.dptr(vsmall), _lib.dptr(dqs), ions.nat, _lib.dptr(pos),
-                                       _lib.dptr(frac), _lib.iptr(spec), int(bool(pme)), int(order), out.data_ptr(),
-                                       _lib.stream()))
+    _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, *ptrs, ions.nat, _lib.dptr(pos), _lib.dptr(frac),
+                                       _lib.iptr(spec), int(bool(pme)), int(order), out.data_ptr(), _lib.stream()))
     return out
 
 
