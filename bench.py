@@ -135,17 +135,17 @@ def run_fp32_form(ev, start_fn, g64, e_ref, tol, nat, precond=False, options=Non
 def learned_start_fn(ions, pp, g64, g32, net, use_net=True):
     """The learned warm start: superposition of screened-ion densities,
     network inputs, the CNN forward at the full grid and the normalised start
-    density (all on the GPU)."""
+    density (all on the GPU, in FP32)."""
     import torch
     from dftpy_amd.learned import superposition_density
     ne = float(ions.get_ncharges())
     rho0 = ne / g64.volume
-    vloc = pp.vreal_for(g64).data
-    plan64 = g64.plan(torch.float64)
+    vloc = pp.vreal_for(g32).data
+    plan32 = g32.plan(torch.float32)
 
     def fn():
-        rho_s = superposition_density(g64, ions, pp.readpp, rho0)
-        x0 = net.features(plan64, rho_s, vloc, rho0)
+        rho_s = superposition_density(g32, ions, pp.readpp, rho0)
+        x0 = net.features(plan32, rho_s, vloc, rho0)
         return net.start_density(g32, x0, rho0, ne, use_net=use_net)
     return fn
 

@@ -80,8 +80,8 @@ class _Worker:
                                  H=Functional(type="HARTREE"), PSEUDO=pp)
             ne = float(ions.get_ncharges())
             rho0 = ne / g64.volume
-            rho_s = superposition_density(g64, ions, self.readpp, rho0)
-            x0 = self.net.features(g64.plan(torch.float64), rho_s, pp.vreal_for(g64).data, rho0)
+            rho_s = superposition_density(g32, ions, self.readpp, rho0)
+            x0 = self.net.features(g32.plan(torch.float32), rho_s, pp.vreal_for(g32).data, rho0)
             start = self.net.start_density(g32, x0, rho0, ne)
             opt = Optimization("TN", optimization_options={"econv": 1e-6 * ions.nat, "maxiter": 60, "precond": True,
                                                            "graph": self.graph},

@@ -24,7 +24,7 @@ std::recursive_mutex &legacy_lock() {
 
 void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
-              const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, double *vreal,
+              const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, void *vreal,
               hipStream_t s);
 void pp_forces(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
                const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
@@ -457,7 +457,7 @@ int ofdft_local_pp(ofdft_plan *P, int nspecies, const double *q0, const double *
         for (int a = 0; a < nat; ++a)
             OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "local_pp: species index out of range");
         local_pp(*P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species, pme, order,
-                 (double *)vreal, S(stream));
+                 vreal, S(stream));
     });
 }
 

@@ -81,8 +81,10 @@ __device__ __forceinline__ bool freqs_of(const GGeom &g, int ix, int iy, int iz,
     return true;
 }
 
-// v(G) = sum_a vloc_s(a)(|G|) exp(-i G.R_a) on the half grid, scaled.
-__global__ void k_vloc_exact(double2 *__restrict__ out, long long nc, GGeom g, VlocTable t, int nat,
+// v(G) = sum_a vloc_s(a)(|G|) exp(-i G.R_a) on the half grid, scaled (FP64
+// per point, stored in the plan's precision T).
+template <typename T>
+__global__ void k_vloc_exact(cpx<T> *__restrict__ out, long long nc, GGeom g, VlocTable t, int nat,
                              const double *__restrict__ pos, const int *__restrict__ spec, double scale) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
          i += (long long)gridDim.x * blockDim.x) {
@@ -119,7 +121,7 @@ __global__ void k_vloc_exact(double2 *__restrict__ out, long long nc, GGeom g, V
                 im -= 0.5 * vs2[spec[a]] * sn;
             }
         }
-        out[i] = double2{re * scale, im * scale};
+        out[i] = cpx<T>{(T)(re * scale), (T)(im * scale)};
     }
 }
 
@@ -136,7 +138,8 @@ __device__ __forceinline__ void calc_mn(double x, int order, double *m /* order+
 }
 
 // Q[floor(u)+1-i] += w M(frac(u)+i) per axis, one workgroup per atom.
-__global__ void k_pme_spread(double *__restrict__ Q, int nx, int ny, int nz, int order, int nat,
+template <typename T>
+__global__ void k_pme_spread(T *__restrict__ Q, int nx, int ny, int nz, int order, int nat,
                              const double *__restrict__ frac, const double *__restrict__ w, const int *__restrict__ spec,
                              int want_species) {
     const int a = blockIdx.x;
@@ -165,7 +168,7 @@ __global__ void k_pme_spread(double *__restrict__ Q, int nx, int ny, int nz, int
         if (ly < 0) ly += ny;
         int lz = (1 + fl[2] - k) % nz;
         if (lz < 0) lz += nz;
-        atomicAdd(&Q[((long long)lx * ny + ly) * nz + lz], wa * mx[i + 1] * my[j + 1] * mz[k + 1]);
+        atomicAdd(&Q[((long long)lx * ny + ly) * nz + lz], (T)(wa * mx[i + 1] * my[j + 1] * mz[k + 1]));
     }
 }
 
@@ -174,7 +177,8 @@ struct BTab {
 };
 
 // acc(G) += vloc_s(|G|)_eff * S_s(G) * B(G)  (S_s = rfftn of the species Q array)
-__global__ void k_vloc_pme_acc(double2 *__restrict__ acc, const double2 *__restrict__ sq, long long nc, GGeom g,
+template <typename T>
+__global__ void k_vloc_pme_acc(cpx<T> *__restrict__ acc, const cpx<T> *__restrict__ sq, long long nc, GGeom g,
                                VlocTable t, int s, BTab b, int first) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
          i += (long long)gridDim.x * blockDim.x) {
@@ -193,12 +197,12 @@ __global__ void k_vloc_pme_acc(double2 *__restrict__ acc, const double2 *__restr
         const double2 bx = b.bx[ix], by = b.by[iy], bz = b.bz[iz];
         double2 bb{bx.x * by.x - bx.y * by.y, bx.x * by.y + bx.y * by.x};
         bb = double2{bb.x * bz.x - bb.y * bz.y, bb.x * bz.y + bb.y * bz.x};
-        const double2 q = sq[i];
-        const double2 val{v * (q.x * bb.x - q.y * bb.y), v * (q.x * bb.y + q.y * bb.x)};
+        const cpx<T> q = sq[i];
+        const double2 val{v * ((double)q.x * bb.x - (double)q.y * bb.y), v * ((double)q.x * bb.y + (double)q.y * bb.x)};
         if (first)
-            acc[i] = val;
+            acc[i] = cpx<T>{(T)val.x, (T)val.y};
         else
-            acc[i] = double2{acc[i].x + val.x, acc[i].y + val.y};
+            acc[i] = cpx<T>{(T)((double)acc[i].x + val.x), (T)((double)acc[i].y + val.y)};
     }
 }
 
@@ -431,11 +435,41 @@ static int nblk(long long n) {
     return (int)std::max(1LL, std::min(b, 4096LL));
 }
 
+// the local PP on the grid: the real-space potential, or any radial recipe
+// summed over atoms; per-point math in FP64, fields and transforms in the
+// plan's precision T
+template <typename T>
+static void local_pp_t(Plan &P, int nspecies, const VlocTable &t, int nat, const double *dpos, const double *dfrac,
+                       const int *dspec, int pme, int order, T *vreal, hipStream_t s) {
+    const GGeom g = geom_of(P);
+    cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);  // slot 0: accumulated v(G), slot 1: species S(G)
+    T *Q = reinterpret_cast<T *>(P.work);
+    if (!pme) {
+        // vreal = irfftn(v) / dV
+        hipLaunchKernelGGL(k_vloc_exact<T>, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, nat, dpos, dspec,
+                           1.0);
+        OFDFT_HIP(hipGetLastError());
+    } else {
+        const BTab bt = pme_tab(P, order);
+        for (int sp = 0; sp < nspecies; ++sp) {
+            OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(T) * P.N, s));
+            hipLaunchKernelGGL(k_pme_spread<T>, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
+                               dfrac, nullptr, dspec, sp);
+            OFDFT_HIP(hipGetLastError());
+            fft_r2c<T>(P, Q, spec + P.Nc, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
+            // v = sum_s vloc_s * rfftn(Q_s) * dV * B * N / V = sum_s vloc_s rfftn(Q_s) B
+            hipLaunchKernelGGL(k_vloc_pme_acc<T>, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc, g, t,
+                               sp, bt, sp == 0 ? 1 : 0);
+            OFDFT_HIP(hipGetLastError());
+        }
+    }
+    fft_c2r<T>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
+}
+
 void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
-              const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, double *vreal,
+              const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, void *vreal,
               hipStream_t s) {
-    OFDFT_REQUIRE(P.prec == OFDFT_FP64, "local_pp needs an FP64 plan");
     OFDFT_REQUIRE(nspecies >= 1 && nspecies <= kMaxSpecies, "1..8 species supported");
     OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
     VlocTable t;
@@ -465,28 +499,10 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
     const double *dpos = reinterpret_cast<const double *>(d + opos);
     const double *dfrac = reinterpret_cast<const double *>(d + ofrac);
     const int *dspec = reinterpret_cast<const int *>(d + ospec);
-    const GGeom g = geom_of(P);
-    double2 *spec = reinterpret_cast<double2 *>(P.spec);  // slot 0: accumulated v(G), slot 1: species S(G)
-    double *Q = reinterpret_cast<double *>(P.work);
-    if (!pme) {
-        // vreal = irfftn(v) / dV
-        hipLaunchKernelGGL(k_vloc_exact, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, nat, dpos, dspec, 1.0);
-        OFDFT_HIP(hipGetLastError());
-    } else {
-        const BTab bt = pme_tab(P, order);
-        for (int sp = 0; sp < nspecies; ++sp) {
-            OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
-            hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
-                               dfrac, nullptr, dspec, sp);
-            OFDFT_HIP(hipGetLastError());
-            fft_r2c<double>(P, Q, spec + P.Nc, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
-            // v = sum_s vloc_s * rfftn(Q_s) * dV * B * N / V = sum_s vloc_s rfftn(Q_s) B
-            hipLaunchKernelGGL(k_vloc_pme_acc, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc, g, t, sp,
-                               bt, sp == 0 ? 1 : 0);
-            OFDFT_HIP(hipGetLastError());
-        }
-    }
-    fft_c2r<double>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
+    if (P.prec == OFDFT_FP64)
+        local_pp_t<double>(P, nspecies, t, nat, dpos, dfrac, dspec, pme, order, (double *)vreal, s);
+    else
+        local_pp_t<float>(P, nspecies, t, nat, dpos, dfrac, dspec, pme, order, (float *)vreal, s);
 }
 
 static double erfcinv_host(double y) {
@@ -560,8 +576,8 @@ void ewald(Plan &P, int nat, const double *pos_cart, const double *pos_frac, con
         const BTab bt = pme_tab(P, order);
         double *Q = reinterpret_cast<double *>(P.work);
         OFDFT_HIP(hipMemsetAsync(Q, 0, sizeof(double) * P.N, s));
-        hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
-                           dfrac, dq, nullptr, -1);
+        hipLaunchKernelGGL(k_pme_spread<double>, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order,
+                           nat, dfrac, dq, nullptr, -1);
         OFDFT_HIP(hipGetLastError());
         fft_r2c<double>(P, Q, spec, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
         hipLaunchKernelGGL(k_ewald_rec, dim3(nbg), dim3(256), 0, s, spec, P.Nc, g, eta, bt, nat, nullptr,
@@ -1358,8 +1374,8 @@ void pp_stress_pme(Plan &P, const double *rho, int nspecies, const double *q0, c
     fft_r2c<double>(P, rho, rhoG, 1, P.N, nc, P.dV, nullptr, nullptr, s);
     for (int sp = 0; sp < nspecies; ++sp) {
         hipLaunchKernelGGL(k_fill_zero, dim3(nblk(P.N)), dim3(256), 0, s, Q, P.N);
-        hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat, in.frac,
-                           (const double *)nullptr, in.spec, sp);
+        hipLaunchKernelGGL(k_pme_spread<double>, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order,
+                           nat, in.frac, (const double *)nullptr, in.spec, sp);
         OFDFT_HIP(hipGetLastError());
         fft_r2c<double>(P, Q, Ss + (size_t)sp * nc, 1, P.N, nc, 1.0, nullptr, nullptr, s);
         hipLaunchKernelGGL(k_bmul, dim3(nblk(nc)), dim3(256), 0, s, Ss + (size_t)sp * nc, nc, g, bt);
@@ -1405,8 +1421,8 @@ void ewald_fs_pme(Plan &P, int nat, const double *pos_cart, const double *pos_fr
     const BTab bt = pme_tab(P, order);
     const GGeom g = geom_of(P);
     hipLaunchKernelGGL(k_fill_zero, dim3(nblk(P.N)), dim3(256), 0, s, Q, P.N);
-    hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat, in.frac,
-                       in.w, (const int *)nullptr, -1);
+    hipLaunchKernelGGL(k_pme_spread<double>, dim3(nat), dim3(256), 0, s, Q, P.nr[0], P.nr[1], P.nr[2], order, nat,
+                       in.frac, in.w, (const int *)nullptr, -1);
     OFDFT_HIP(hipGetLastError());
     fft_r2c<double>(P, Q, SZ, 1, P.N, nc, 1.0, nullptr, nullptr, s);  // rfftn(Q_Z)
     // real space, as ewald_fs

@@ -105,9 +105,10 @@ def metric_weights(grid, rho0):
 
 
 def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
-    """rho_s on ``grid`` (FP64 device tensor): sum over atoms of the screening
-    densities (the G = 0 term gives the electron count)."""
-    plan = grid.plan(torch.float64)
+    """rho_s on ``grid`` (device tensor in the grid's precision; the network's
+    inputs take FP32): sum over atoms of the screening densities (the G = 0
+    term gives the electron count)."""
+    plan = grid.plan(grid.dtype)
     keys = sorted(readpp.tables)
     ns = len(keys)
     # the radial tables depend on the species and rho0 only: kept on the
@@ -129,7 +130,7 @@ def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
     pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
     frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
     spec = ions.species_index(keys)
-    out = torch.empty(tuple(int(n) for n in grid.nrR), dtype=torch.float64, device=grid.device)
+    out = torch.empty(tuple(int(n) for n in grid.nrR), dtype=grid.dtype, device=grid.device)
     _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, *ptrs, ions.nat, _lib.dptr(pos), _lib.dptr(frac),
                                        _lib.iptr(spec), int(bool(pme)), int(order), out.data_ptr(), _lib.stream()))
     return out
@@ -336,10 +337,15 @@ def learned_start(grid, ions, pseudo, net=None, use_net=True):
     net = net or WarmStartNet.load()
     ne = float(ions.get_ncharges())
     rho0 = ne / grid.volume
-    g64 = pseudo.grid if pseudo.grid.dtype == torch.float64 else pseudo.grid.with_dtype(torch.float64)
-    rho_s = superposition_density(g64, ions, pseudo.readpp, rho0)
-    vloc = pseudo.vreal_for(g64).data
-    x0 = net.features(g64.plan(torch.float64), rho_s, vloc, rho0)
+    # the features in FP32 on an FP32 target grid (the network's input
+    # precision), else in FP64 on the pseudopotential's grid
+    if grid.dtype == torch.float32:
+        gs = grid
+    else:
+        gs = pseudo.grid if pseudo.grid.dtype == torch.float64 else pseudo.grid.with_dtype(torch.float64)
+    rho_s = superposition_density(gs, ions, pseudo.readpp, rho0)
+    vloc = pseudo.vreal_for(gs).data
+    x0 = net.features(gs.plan(gs.dtype), rho_s, vloc, rho0)
     return net.start_density(grid, x0, rho0, ne, use_net=use_net)
 
 

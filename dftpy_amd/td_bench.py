@@ -50,8 +50,8 @@ def ground_state(ions, nr, pp_path):
     net = WarmStartNet.load()
     ne = float(ions.get_ncharges())
     rho0 = ne / g64.volume
-    rho_s = superposition_density(g64, ions, pp.readpp, rho0)
-    x0 = net.features(g64.plan(torch.float64), rho_s, pp.vreal_for(g64).data, rho0)
+    rho_s = superposition_density(g32, ions, pp.readpp, rho0)
+    x0 = net.features(g32.plan(torch.float32), rho_s, pp.vreal_for(g32).data, rho0)
     start = net.start_density(g32, x0, rho0, ne)
     nat = ions.nat
     r32 = Optimization("TN", optimization_options={"econv": 1e-8 * nat, "maxiter": 60},

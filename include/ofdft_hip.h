@@ -181,7 +181,8 @@ int ofdft_bench_pass(ofdft_plan *plan, int which, int batch, int iters, const vo
  * patch below qsmall: vsmall[3] = (v0, v1, v2), dqs.  Host arrays.
  * Atoms: pos_cart[3*nat] (bohr), pos_frac[3*nat], species[nat].
  * pme != 0 uses order-`order` cardinal B-splines; else the exact
- * structure factor.  Writes vreal (device, N reals, plan precision). */
+ * structure factor.  Writes vreal (device, N reals, plan precision: per-point
+ * math in FP64, the B-spline grid and the transforms in the plan's precision). */
 int ofdft_local_pp(ofdft_plan *plan, int nspecies, const double *q0, const double *dq,
                    const int *ncoef, const double *const *coef, const double *qmax,
                    const double *vsmall, const double *dqs, int nat, const double *pos_cart,

@@ -176,8 +176,8 @@ def test_c2_fp32_bulk_errors_against_fp64(c2, start):
         pp = c2.ev[torch.float64].PSEUDO
         net = WarmStartNet.load()
         rho0 = c2.ne / g64.volume
-        rho_s = superposition_density(g64, c2.ions, pp.readpp, rho0)
-        x0 = net.features(g64.plan(torch.float64), rho_s, pp.vreal_for(g64).data, rho0)
+        rho_s = superposition_density(g32, c2.ions, pp.readpp, rho0)
+        x0 = net.features(g32.plan(torch.float32), rho_s, pp.vreal_for(g32).data, rho0)
         rho = net.start_density(g32, x0, rho0, c2.ne, use_net=start == "learned")
     opt = Optimization(optimization_method="TN", EnergyEvaluator=c2.ev[torch.float32],
                        optimization_options={"econv": 1e-6 * c2.nat, "maxiter": 60, "hvp": "analytic",

@@ -198,6 +198,39 @@ def test_local_pp_golden_and_vreal(al_pp, pme, full):
 
 
 @pytest.mark.parametrize("pme", [True, False])
+@pytest.mark.parametrize("full", [True, False])
+def test_local_pp_setup_on_fp32_plan(al_pp, pme, full):
+    """The local-PP setup (ofdft_local_pp) on an FP32 plan (per-point math
+    in FP64, the B-spline grid and the transforms in FP32) against the
+    oracle's FP64 vreal, and the learned start's screening superposition
+    (the same entry point with other radial tables) on FP32 against FP64."""
+    from dftpy_amd import Ions, _lib
+    from dftpy_amd.functional.pseudo import LocalPseudo
+    from dftpy_amd.learned import superposition_density
+    atoms, rho, lat = al_pp
+    ions = Ions(symbols=atoms.symbols, positions=atoms.positions, cell=lat)
+    g32 = _dg(lat, rho.shape, full, torch.float32)
+    pp = LocalPseudo(grid=_dg(lat, rho.shape, full), ions=ions, PP_list={"Al": data("al.lda.recpot")}, PME=pme)
+    keep, args = pp._table_args()
+    v32 = torch.empty(tuple(rho.shape), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib.ofdft_local_pp(g32.plan(torch.float32).handle, *args, int(pme), 10, v32.data_ptr(),
+                                       _lib.stream()))
+    og = OG.Grid(lat, rho.shape, full=full)
+    gp, vp, _ = OIO.read_recpot(data("al.lda.recpot"))
+    v_ref = OP.local_pp(og, atoms, {"Al": (gp, vp)}, pme=pme)
+    err = np.max(np.abs(v32.double().cpu().numpy() - v_ref)) / np.max(np.abs(v_ref))
+    print("FP32 vreal vs oracle: %.2e" % err)
+    assert err < 3e-6
+    del keep
+    rho0 = float(ions.get_ncharges()) / g32.volume
+    s32 = superposition_density(g32, ions, pp.readpp, rho0, pme=pme).double().cpu().numpy()
+    s64 = superposition_density(_dg(lat, rho.shape, full), ions, pp.readpp, rho0, pme=pme).cpu().numpy()
+    err = np.max(np.abs(s32 - s64)) / np.max(np.abs(s64))
+    print("superposition FP32 vs FP64: %.2e" % err)
+    assert err < 3e-6
+
+
+@pytest.mark.parametrize("pme", [True, False])
 def test_ewald_golden(al_pp, pme):
     # examples/test/test_ewald.py:33-35
     from dftpy_amd import Ions
