@@ -121,7 +121,7 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             std::memset(P->tw, 0, sizeof(P->tw));
             P->spec = P->work = P->aux = nullptr;
             P->red = P->scal = nullptr;
-            P->pinned = nullptr;
+            P->pinned = P->pinned_dev = nullptr;
             for (int d = 0; d < 3; ++d) P->nr[d] = nr[d];
             P->nzc = nr[2] / 2 + 1;
             {
@@ -149,7 +149,8 @@ int ofdft_plan_create(const int nr[3], const double lattice[9], int precision, i
             OFDFT_HIP(hipMalloc(&P->red, sizeof(double) * 4096 * 16));
             OFDFT_HIP(hipMalloc(&P->scal, sizeof(double) * kScalSlots));
             OFDFT_HIP(hipMemset(P->scal, 0, sizeof(double) * kScalSlots));
-            OFDFT_HIP(hipHostMalloc((void **)&P->pinned, 128 * sizeof(double), hipHostMallocDefault));
+            OFDFT_HIP(hipHostMalloc((void **)&P->pinned, 128 * sizeof(double), hipHostMallocMapped));
+            OFDFT_HIP(hipHostGetDevicePointer((void **)&P->pinned_dev, P->pinned, 0));
             P->ws_bytes = spec_bytes + work_bytes + sizeof(double) * (4096 * 16 + kScalSlots);
             plan_aux_setup(*P);
             P->ws_bytes += P->aux_bytes;

@@ -125,6 +125,7 @@ struct Plan {
     size_t aux_bytes;
     size_t ws_bytes;
     double *pinned;  // 128 doubles of pinned host memory for scalar read-backs (optimizer, propagator)
+    double *pinned_dev;  // its device view (hipHostGetDevicePointer), written by publishing reductions
 
     // functional settings
     double x = 1.0, y = 1.0, alpha = 5.0 / 6.0, beta = 5.0 / 6.0, rho0 = -1.0, sigma = -1.0;
@@ -219,8 +220,12 @@ void plan_autotune(Plan &P, hipStream_t s, float *times);
 int reduce_blocks(long long n);
 template <typename T>
 void dot(Plan &P, const T *a, const T *b, const T *c, long long n, double *out, double scale, hipStream_t s);
+// With pub_n > 0 the same launch then copies pub_src[0..pub_n) (which may
+// include the outputs) to pub_dst, the device view of pinned host memory: a
+// host round trip reads it after the stream completes, without a copy call.
 void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
-                  hipStream_t s, const double *gate = nullptr);
+                  hipStream_t s, const double *gate = nullptr, const double *pub_src = nullptr,
+                  double *pub_dst = nullptr, int pub_n = 0);
 
 // eval.hip
 template <typename T>

@@ -773,10 +773,18 @@ struct Ctx {
         wait();
         return host;
     }
-    void reduce_final_cols(int K, int slot) {
+    // pub_n > 0: the same launch then publishes scal[pub_slot..pub_slot+pub_n)
+    // to the pinned host buffer, read by published()
+    void reduce_final_cols(int K, int slot, int pub_n = 0, int pub_slot = 0) {
         // partials laid out [block][K]
         std::vector<double> ones(K, 1.0);
-        reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s, P.gate);
+        reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s, P.gate, pub_n ? P.scal + pub_slot : nullptr,
+                     pub_n ? P.pinned_dev : nullptr, pub_n);
+    }
+    // the scalars the last publishing reduction sent, once the stream is done
+    const double *published() const {
+        wait();
+        return host;
     }
     // copy scal[slot..slot+K) to the host
     const double *to_host(int K, int slot) {
@@ -930,12 +938,13 @@ template <typename T> struct Opt {
             const int rz_slot = it == 0 ? (pre ? 26 : 25) : (pre ? 18 : 17);
             PW_RED(k_tn_update_dev, dir, tres, tp, Ap, P.scal + rz_slot,
                                P.scal + 16, n, P.red);
-            C.reduce_final_cols(1, 17);
+            const int nh = it == 0 ? 11 : (pre ? 3 : 2);  // scal[16..16+nh) to the host
+            C.reduce_final_cols(1, 17, pre ? 0 : nh, 16);
             if (pre) {
                 precond_dot();
-                C.reduce_final_cols(1, 18);
+                C.reduce_final_cols(1, 18, nh, 16);
             }
-            const double *hh = C.to_host(it == 0 ? 11 : (pre ? 3 : 2), 16);
+            const double *hh = C.published();
             if (it == 0) {
                 r0 = hh[25 - 16];
                 rz = pre ? hh[26 - 16] : r0;
@@ -990,8 +999,9 @@ template <typename T> struct Opt {
         C.reduce_final_cols(1, 23);  // scal[23] = sum p^2
         PW_STREAM(k_scale_norm, p, P.scal, 22, 23, n);
         PW_RED(k_dot3, F(F_POT), phi, p, n, P.red);
-        C.reduce_final_cols(1, 24);  // scal[24] = sum V phi p
-        const double *h = C.to_host(energies ? 32 + OFDFT_E_COUNT - 21 : 4, 21);
+        // scal[24] = sum V phi p; scal[21..] to the host
+        C.reduce_final_cols(1, 24, energies ? 32 + OFDFT_E_COUNT - 21 : 4, 21);
+        const double *h = C.published();
         theta0 = std::sqrt(h[2] / h[1]);
         vphip = h[3];
         if (energies)
@@ -1006,8 +1016,8 @@ template <typename T> struct Opt {
         eval<T>(P, F(F_NRHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_NPOT), energies_d, C.s);
         ++n_ev;
         PW_RED(k_theta_grad, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), c, s, (const double *)nullptr, n, P.red);
-        C.reduce_final_cols(1, 32 + OFDFT_E_COUNT);
-        const double *h = C.to_host(OFDFT_E_COUNT + 1, 32);
+        C.reduce_final_cols(1, 32 + OFDFT_E_COUNT, OFDFT_E_COUNT + 1, 32);
+        const double *h = C.published();
         e = h[OFDFT_E_SUM];
         g = 2.0 * h[OFDFT_E_COUNT] * P.dV;
     }

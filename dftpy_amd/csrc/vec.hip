@@ -40,7 +40,7 @@ struct Coefs {
 // sum_t out[t] + add.
 __global__ void __launch_bounds__(kFinalThreads)
     k_reduce_final(const double *__restrict__ partials, int nblocks, int nterms, double *out, Coefs co,
-                   const double *gate) {
+                   const double *gate, const double *pub_src, double *pub_dst, int pub_n) {
     if (gate_off(gate)) return;
     __shared__ double sm[kFinalThreads];
     sum_columns(partials, nblocks, nterms, sm);
@@ -52,14 +52,21 @@ __global__ void __launch_bounds__(kFinalThreads)
         }
         if (co.sum_out) out[nterms] = tot + co.add;
     }
+    if (pub_n > 0) {
+        __syncthreads();  // thread 0's outputs are visible to the workgroup
+        if ((int)threadIdx.x < pub_n) pub_dst[threadIdx.x] = pub_src[threadIdx.x];
+    }
 }
 
 void reduce_final(const double *partials, int nblocks, int nterms, double *out, const double *coef,
-                  hipStream_t s, const double *gate) {
+                  hipStream_t s, const double *gate, const double *pub_src, double *pub_dst, int pub_n) {
+    OFDFT_REQUIRE(pub_n >= 0 && pub_n <= kFinalThreads && (pub_n == 0 || (pub_src && pub_dst)),
+                  "reduce_final: bad publish range");
     Coefs co{};
     for (int t = 0; t < nterms && t < kMaxTerms; ++t) co.c[t] = coef ? coef[t] : 1.0;
     co.sum_out = 0;
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co, gate);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co, gate,
+                       pub_src, pub_dst, pub_n);
     OFDFT_HIP(hipGetLastError());
 }
 
@@ -70,7 +77,7 @@ void reduce_final_sum(const double *partials, int nblocks, int nterms, double *o
     co.sum_out = 1;
     co.add = add;
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kFinalThreads), 0, s, partials, nblocks, nterms, out, co,
-                       (const double *)nullptr);
+                       (const double *)nullptr, (const double *)nullptr, (double *)nullptr, 0);
     OFDFT_HIP(hipGetLastError());
 }
 
