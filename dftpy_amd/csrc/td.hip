@@ -155,10 +155,10 @@ template <typename T> struct CN {
         kin = reinterpret_cast<cpx<T> *>(P.spec);
     }
 
+    // the K sums, published by the reduction itself to the pinned buffer
     const double *fetch(int K) {
         std::vector<double> ones(K, 1.0);
-        reduce_final(P.red, nb, K, P.scal + 64, ones.data(), s);
-        OFDFT_HIP(hipMemcpyAsync(host, P.scal + 64, K * sizeof(double), hipMemcpyDeviceToHost, s));
+        reduce_final(P.red, nb, K, P.scal + 64, ones.data(), s, nullptr, P.scal + 64, P.pinned_dev, K);
         stream_wait(s);
         return host;
     }
@@ -641,13 +641,18 @@ template <typename T> __global__ void k_tdb_pupd(const cpx<T> *res, cpx<T> *p, R
 }
 
 // out[run * K + t] = sum_b partials[(run * nblk + b) * K + t]
-__global__ void k_tdb_reduce(const double *partials, int nblk, int K, double *out) {
+// one workgroup per (run, term); pub (the device view of pinned host memory,
+// or null) receives a copy for the host
+__global__ void k_tdb_reduce(const double *partials, int nblk, int K, double *out, double *pub) {
     __shared__ double sm[16];
     const int run = blockIdx.x / K, t = blockIdx.x % K;
     double acc = 0.0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[((long long)run * nblk + b) * K + t];
     acc = block_sum(acc, sm);
-    if (threadIdx.x == 0) out[blockIdx.x] = acc;
+    if (threadIdx.x == 0) {
+        out[blockIdx.x] = acc;
+        if (pub) pub[blockIdx.x] = acc;
+    }
 }
 
 #undef RUN_LOOP
@@ -674,9 +679,9 @@ template <typename T> struct CNBatch {
     }
 
     const double *fetch(int K) {
-        hipLaunchKernelGGL(k_tdb_reduce, dim3(nb * K), dim3(256), 0, s, P.red, nblk, K, P.scal + 128);
+        OFDFT_REQUIRE(nb * K <= 128, "cn_step_batch: read-back larger than the pinned buffer");
+        hipLaunchKernelGGL(k_tdb_reduce, dim3(nb * K), dim3(256), 0, s, P.red, nblk, K, P.scal + 128, P.pinned_dev);
         OFDFT_HIP(hipGetLastError());
-        OFDFT_HIP(hipMemcpyAsync(host, P.scal + 128, (size_t)nb * K * sizeof(double), hipMemcpyDeviceToHost, s));
         stream_wait(s);
         return host;
     }
