@@ -64,6 +64,10 @@ SIGNATURES = {
     "ofdft_optimize": (_I, [_P, _P, _P, _I, _I, _DP, _DP, _DP, _I, _P]),
     "ofdft_optimize_tn_graph": (_I, [_P, _P, _P, _I, _DP, _I, _I, _DP, _DP, _I, _P]),
     "ofdft_set_tn_precond": (_I, [_P, _I, _D, _DP]),
+    "ofdft_set_tn_precond_uniform": (_I, [_P, _I, _D, _D, _DP, _P]),
+    "ofdft_get_tn_precond": (_I, [_P, _P, _I, _P]),
+    "ofdft_screened_superposition": (_I, [_P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _DP, _DP, _DP,
+                                          _DP, _I, _DP, _DP, _IP, _I, _I, _P, _P]),
     "ofdft_plan_reserve_lbfgs": (_I, [_P, _I]),
     "ofdft_local_pp_forces": (_I, [_P, _P, _I, _DP, _DP, _IP, ctypes.POINTER(_DP), _DP, _DP, _DP, _I, _DP, _DP, _IP,
                                    _DP, _P]),

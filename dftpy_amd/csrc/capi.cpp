@@ -25,7 +25,9 @@ std::recursive_mutex &legacy_lock() {
 void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
               const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, void *vreal,
-              hipStream_t s);
+              hipStream_t s, const double *screen_z = nullptr, const double *screen_r1 = nullptr,
+              const double *screen_r2 = nullptr, const double *resp = nullptr);
+void tn_precond_fill(Plan &P, int n, double dq, double rho0, const double *resp, hipStream_t s);
 void pp_forces(Plan &P, const double *rho, int nspecies, const double *q0, const double *dq, const int *ncoef,
                const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
                const double *pos_cart, const double *pos_frac, const int *species, double *F, hipStream_t s);
@@ -459,6 +461,46 @@ int ofdft_local_pp(ofdft_plan *P, int nspecies, const double *q0, const double *
             OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "local_pp: species index out of range");
         local_pp(*P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species, pme, order,
                  vreal, S(stream));
+    });
+}
+
+int ofdft_screened_superposition(ofdft_plan *P, int nspecies, const double *q0, const double *dq, const int *ncoef,
+                                 const double *const *coef, const double *qmax, const double *vsmall,
+                                 const double *dqs, const double *zval, const double *r1, const double *r2,
+                                 const double *resp, int nat, const double *pos_cart, const double *pos_frac,
+                                 const int *species, int pme, int order, void *out, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P && out && nat >= 1 && zval && r1 && r2 && resp, "screened_superposition: bad arguments");
+        for (int a = 0; a < nat; ++a)
+            OFDFT_REQUIRE(species[a] >= 0 && species[a] < nspecies, "screened_superposition: species out of range");
+        local_pp(*P, nspecies, q0, dq, ncoef, coef, qmax, vsmall, dqs, nat, pos_cart, pos_frac, species, pme, order,
+                 out, S(stream), zval, r1, r2, resp);
+    });
+}
+
+int ofdft_set_tn_precond_uniform(ofdft_plan *P, int n, double dq, double rho0, const double *resp, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P != nullptr && n >= 2 && dq > 0.0 && rho0 > 0.0 && resp && resp[3] > 0.0,
+                      "set_tn_precond_uniform: bad arguments");
+        if (n > P->tn_pre_cap) {
+            OFDFT_LEGACY_LOCK;
+            if (P->tn_pre) OFDFT_HIP(hipFree(P->tn_pre));
+            P->tn_pre = nullptr;
+            P->tn_pre_cap = 0;
+            OFDFT_HIP(hipMalloc(&P->tn_pre, sizeof(double) * n));
+            P->tn_pre_cap = n;
+        }
+        tn_precond_fill(*P, n, dq, rho0, resp, S(stream));
+        P->tn_pre_n = n;
+        P->tn_pre_dq = dq;
+    });
+}
+
+int ofdft_get_tn_precond(ofdft_plan *P, double *out, int n, void *stream) {
+    return guard([&] {
+        OFDFT_REQUIRE(P != nullptr && out != nullptr && n >= 1 && n <= P->tn_pre_n,
+                      "get_tn_precond: no table of that length");
+        OFDFT_HIP(hipMemcpyAsync(out, P->tn_pre, sizeof(double) * n, hipMemcpyDeviceToDevice, S(stream)));
     });
 }
 

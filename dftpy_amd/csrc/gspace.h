@@ -44,6 +44,22 @@ template <typename T> __device__ __forceinline__ T lindhard_t(T eta, T lbda, T m
 
 __device__ __forceinline__ double lindhard(double eta, double lbda, double mu) { return lindhard_t<double>(eta, lbda, mu); }
 
+// K(q) = d^2E/drho(q)^2 of the functional's terms at a uniform density:
+// kc (TF and LDA) + vw q^2 + wt G(q / tkf) + 4 pi / q^2 (Hartree; infinite
+// at q = 0), the host giving the rho0-dependent coefficients
+struct RespK {
+    double kc, vw, wt, tkf, lbda;
+    int hartree;
+};
+
+// in CT (double, or float with the hardware math for FP32 fields)
+template <typename CT> __device__ __forceinline__ CT resp_k(const RespK &r, CT q) {
+    CT k = (CT)r.kc + (CT)r.vw * q * q;
+    if (r.wt != 0.0) k += (CT)r.wt * lindhard_t<CT>(q * (CT)(1.0 / r.tkf), (CT)r.lbda, (CT)1);
+    if (r.hartree) k += q > (CT)0 ? fm::div((CT)(4.0 * M_PI), q * q) : (CT)INFINITY;
+    return k;
+}
+
 struct GParams {
     double B[9];
     int kind[4];  // per spectrum slot: 0 vW, 1 WT, 2 Hartree, 3 radial table, -1 none

@@ -1499,6 +1499,23 @@ template <typename T> struct Opt {
 #undef PW_RED
 #undef PW_LAUNCH_
 
+// The TN preconditioner table M^-1(q_i), M = 2 rho0 K(q_i) at q_i = i dq, K
+// the uniform-gas response of the functional's terms; 0 where M is infinite
+// (Hartree at q = 0)
+__global__ void k_tn_precond_fill(double *tab, int n, double dq, double rho0, RespK r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double M = 2.0 * rho0 * resp_k<double>(r, i * dq);
+    tab[i] = isinf(M) ? 0.0 : 1.0 / fmax(M, 1e-8);
+}
+
+void tn_precond_fill(Plan &P, int n, double dq, double rho0, const double *resp, hipStream_t s) {
+    const RespK r{resp[0], resp[1], resp[2], resp[3], resp[4], resp[5] != 0.0 ? 1 : 0};
+    hipLaunchKernelGGL(k_tn_precond_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.tn_pre, n, dq, rho0,
+                       r);
+    OFDFT_HIP(hipGetLastError());
+}
+
 }  // namespace ofdft
 
 using namespace ofdft;

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <vector>
 
+#include "gspace.h"
 #include "internal.h"
 #include "pointwise.h"
 #include "reduce.h"
@@ -48,6 +49,34 @@ __device__ __forceinline__ double vloc_eval(const VlocTable &t, int s, double q)
     const double x = q - (t.q0[s] + i * t.dq[s]);
     const double *c = t.coef[s] + 4 * (long long)i;
     return c[0] + x * (c[1] + x * (c[2] + x * c[3]));
+}
+
+// Screening densities of the ions, rho_a(q) = -v_a(q) / K(q) (the linear
+// response of the uniform gas to each local pseudopotential, rho_a(0) = Z):
+// below the first table step the quartic through (r2, r1, Z, r1, r2), the
+// screened values at the first two table points; above it the PP table
+// divided by K on the fly
+struct ScreenParams {
+    RespK k;
+    double z[kMaxSpecies], r1[kMaxSpecies], r2[kMaxSpecies];
+};
+
+// the response in T (FP32 fields: FP32 hardware math)
+template <bool SCREEN, typename T = double>
+__device__ __forceinline__ double vloc_or_screened(const VlocTable &t, const ScreenParams &sp, int s, double q) {
+    if constexpr (!SCREEN) {
+        return vloc_eval(t, s, q);
+    } else {
+        if (q < t.dqs[s]) {
+            const double f0 = sp.r2[s], f1 = sp.r1[s], f2 = sp.z[s];
+            const double t2 = -f0 + 16.0 * f1 - 30.0 * f2 + 16.0 * f1 - f0;
+            const double t4 = f0 - 4.0 * f1 + 6.0 * f2 - 4.0 * f1 + f0;
+            const double dx = q / t.dqs[s];
+            return f2 + dx * (dx * (t2 + dx * (dx * t4))) / 24.0;
+        }
+        const double v = vloc_eval(t, s, q);
+        return v == 0.0 ? 0.0 : -v / (double)resp_k<T>(sp.k, (T)q);
+    }
 }
 
 __device__ __forceinline__ int ffreq_(int i, int n) { return i <= (n - 1) / 2 ? i : i - n; }
@@ -83,9 +112,9 @@ __device__ __forceinline__ bool freqs_of(const GGeom &g, int ix, int iy, int iz,
 
 // v(G) = sum_a vloc_s(a)(|G|) exp(-i G.R_a) on the half grid, scaled (FP64
 // per point, stored in the plan's precision T).
-template <typename T>
-__global__ void k_vloc_exact(cpx<T> *__restrict__ out, long long nc, GGeom g, VlocTable t, int nat,
-                             const double *__restrict__ pos, const int *__restrict__ spec, double scale) {
+template <typename T, bool SCREEN>
+__global__ void k_vloc_exact(cpx<T> *__restrict__ out, long long nc, GGeom g, VlocTable t, ScreenParams sp,
+                             int nat, const double *__restrict__ pos, const int *__restrict__ spec, double scale) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
          i += (long long)gridDim.x * blockDim.x) {
         const int iz = (int)(i % g.nzc);
@@ -97,7 +126,8 @@ __global__ void k_vloc_exact(cpx<T> *__restrict__ out, long long nc, GGeom g, Vl
         gvec(g, f[0], f[1], f[2], G1);
         double vs1[kMaxSpecies], vs2[kMaxSpecies];
         const double q1 = sqrt(G1[0] * G1[0] + G1[1] * G1[1] + G1[2] * G1[2]);
-        for (int s = 0; s < kMaxSpecies; ++s) vs1[s] = t.coef[s] ? vloc_eval(t, s, q1) : 0.0;
+        for (int s = 0; s < kMaxSpecies; ++s)
+            vs1[s] = t.coef[s] ? vloc_or_screened<SCREEN, T>(t, sp, s, q1) : 0.0;
         if (two) {
             // partner enters as f(-G_partner)
             gvec(g, fp[0], fp[1], fp[2], G2);
@@ -105,7 +135,8 @@ __global__ void k_vloc_exact(cpx<T> *__restrict__ out, long long nc, GGeom g, Vl
             G2[1] = -G2[1];
             G2[2] = -G2[2];
             const double q2 = sqrt(G2[0] * G2[0] + G2[1] * G2[1] + G2[2] * G2[2]);
-            for (int s = 0; s < kMaxSpecies; ++s) vs2[s] = t.coef[s] ? vloc_eval(t, s, q2) : 0.0;
+            for (int s = 0; s < kMaxSpecies; ++s)
+                vs2[s] = t.coef[s] ? vloc_or_screened<SCREEN, T>(t, sp, s, q2) : 0.0;
         }
         double re = 0.0, im = 0.0;
         for (int a = 0; a < nat; ++a) {
@@ -177,9 +208,9 @@ struct BTab {
 };
 
 // acc(G) += vloc_s(|G|)_eff * S_s(G) * B(G)  (S_s = rfftn of the species Q array)
-template <typename T>
+template <typename T, bool SCREEN>
 __global__ void k_vloc_pme_acc(cpx<T> *__restrict__ acc, const cpx<T> *__restrict__ sq, long long nc, GGeom g,
-                               VlocTable t, int s, BTab b, int first) {
+                               VlocTable t, ScreenParams sp, int s, BTab b, int first) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
          i += (long long)gridDim.x * blockDim.x) {
         const int iz = (int)(i % g.nzc);
@@ -189,10 +220,10 @@ __global__ void k_vloc_pme_acc(cpx<T> *__restrict__ acc, const cpx<T> *__restric
         const bool two = freqs_of(g, ix, iy, iz, f, fp);
         double G[3];
         gvec(g, f[0], f[1], f[2], G);
-        double v = vloc_eval(t, s, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]));
+        double v = vloc_or_screened<SCREEN, T>(t, sp, s, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]));
         if (two) {
             gvec(g, fp[0], fp[1], fp[2], G);
-            v = 0.5 * (v + vloc_eval(t, s, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2])));
+            v = 0.5 * (v + vloc_or_screened<SCREEN, T>(t, sp, s, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2])));
         }
         const double2 bx = b.bx[ix], by = b.by[iy], bz = b.bz[iz];
         double2 bb{bx.x * by.x - bx.y * by.y, bx.x * by.y + bx.y * by.x};
@@ -438,16 +469,16 @@ static int nblk(long long n) {
 // the local PP on the grid: the real-space potential, or any radial recipe
 // summed over atoms; per-point math in FP64, fields and transforms in the
 // plan's precision T
-template <typename T>
-static void local_pp_t(Plan &P, int nspecies, const VlocTable &t, int nat, const double *dpos, const double *dfrac,
-                       const int *dspec, int pme, int order, T *vreal, hipStream_t s) {
+template <typename T, bool SCREEN>
+static void local_pp_t(Plan &P, int nspecies, const VlocTable &t, const ScreenParams &scr, int nat, const double *dpos,
+                       const double *dfrac, const int *dspec, int pme, int order, T *vreal, hipStream_t s) {
     const GGeom g = geom_of(P);
     cpx<T> *spec = reinterpret_cast<cpx<T> *>(P.spec);  // slot 0: accumulated v(G), slot 1: species S(G)
     T *Q = reinterpret_cast<T *>(P.work);
     if (!pme) {
         // vreal = irfftn(v) / dV
-        hipLaunchKernelGGL(k_vloc_exact<T>, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, nat, dpos, dspec,
-                           1.0);
+        hipLaunchKernelGGL((k_vloc_exact<T, SCREEN>), dim3(nblk(P.Nc)), dim3(256), 0, s, spec, P.Nc, g, t, scr, nat,
+                           dpos, dspec, 1.0);
         OFDFT_HIP(hipGetLastError());
     } else {
         const BTab bt = pme_tab(P, order);
@@ -458,18 +489,22 @@ static void local_pp_t(Plan &P, int nspecies, const VlocTable &t, int nat, const
             OFDFT_HIP(hipGetLastError());
             fft_r2c<T>(P, Q, spec + P.Nc, 1, 0, P.Nc, 1.0, nullptr, nullptr, s);
             // v = sum_s vloc_s * rfftn(Q_s) * dV * B * N / V = sum_s vloc_s rfftn(Q_s) B
-            hipLaunchKernelGGL(k_vloc_pme_acc<T>, dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc, g, t,
-                               sp, bt, sp == 0 ? 1 : 0);
+            hipLaunchKernelGGL((k_vloc_pme_acc<T, SCREEN>), dim3(nblk(P.Nc)), dim3(256), 0, s, spec, spec + P.Nc, P.Nc,
+                               g, t, scr, sp, bt, sp == 0 ? 1 : 0);
             OFDFT_HIP(hipGetLastError());
         }
     }
     fft_c2r<T>(P, spec, spec, vreal, 1, P.Nc, P.Nc, P.N, 1.0 / ((double)P.N * P.dV), s);
 }
 
+// screen (null: the potential itself): per species Z, the screened values at
+// the first two table points, and K(q) = resp[0] + resp[1] q^2 + resp[2]
+// G(q / resp[3]; lambda = resp[4]) + (resp[5] != 0) 4 pi / q^2
 void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const int *ncoef,
               const double *const *coef, const double *qmax, const double *vsmall, const double *dqs, int nat,
               const double *pos_cart, const double *pos_frac, const int *species, int pme, int order, void *vreal,
-              hipStream_t s) {
+              hipStream_t s, const double *screen_z, const double *screen_r1, const double *screen_r2,
+              const double *resp) {
     OFDFT_REQUIRE(nspecies >= 1 && nspecies <= kMaxSpecies, "1..8 species supported");
     OFDFT_REQUIRE(order >= 3 && order <= kMaxOrder, "B-spline order must be 3..16");
     VlocTable t;
@@ -499,10 +534,29 @@ void local_pp(Plan &P, int nspecies, const double *q0, const double *dq, const i
     const double *dpos = reinterpret_cast<const double *>(d + opos);
     const double *dfrac = reinterpret_cast<const double *>(d + ofrac);
     const int *dspec = reinterpret_cast<const int *>(d + ospec);
-    if (P.prec == OFDFT_FP64)
-        local_pp_t<double>(P, nspecies, t, nat, dpos, dfrac, dspec, pme, order, (double *)vreal, s);
-    else
-        local_pp_t<float>(P, nspecies, t, nat, dpos, dfrac, dspec, pme, order, (float *)vreal, s);
+    ScreenParams scr;
+    std::memset(&scr, 0, sizeof(scr));
+    const bool screen = screen_z != nullptr;
+    if (screen) {
+        OFDFT_REQUIRE(screen_r1 && screen_r2 && resp && resp[3] > 0.0, "screened superposition: bad response");
+        scr.k = RespK{resp[0], resp[1], resp[2], resp[3], resp[4], resp[5] != 0.0 ? 1 : 0};
+        for (int sp = 0; sp < nspecies; ++sp) {
+            scr.z[sp] = screen_z[sp];
+            scr.r1[sp] = screen_r1[sp];
+            scr.r2[sp] = screen_r2[sp];
+        }
+    }
+    if (P.prec == OFDFT_FP64) {
+        if (screen)
+            local_pp_t<double, true>(P, nspecies, t, scr, nat, dpos, dfrac, dspec, pme, order, (double *)vreal, s);
+        else
+            local_pp_t<double, false>(P, nspecies, t, scr, nat, dpos, dfrac, dspec, pme, order, (double *)vreal, s);
+    } else {
+        if (screen)
+            local_pp_t<float, true>(P, nspecies, t, scr, nat, dpos, dfrac, dspec, pme, order, (float *)vreal, s);
+        else
+            local_pp_t<float, false>(P, nspecies, t, scr, nat, dpos, dfrac, dspec, pme, order, (float *)vreal, s);
+    }
 }
 
 static double erfcinv_host(double y) {

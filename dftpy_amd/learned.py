@@ -28,7 +28,7 @@ import torch
 from . import _lib
 from .field import DirectField
 from .functional.pseudo import RadialTable
-from .math_utils import lindhard
+from .math_utils import lindhard, uniform_response_coefs
 
 C = 32
 NTAP = 27
@@ -106,33 +106,49 @@ def metric_weights(grid, rho0):
 
 def superposition_density(grid, ions, readpp, rho0, pme=True, order=10):
     """rho_s on ``grid`` (device tensor in the grid's precision; the network's
-    inputs take FP32): sum over atoms of the screening densities (the G = 0
-    term gives the electron count)."""
+    inputs take FP32): sum over atoms of the screening densities
+    rho_a(q) = -v_a(q) / K(q) (the G = 0 term gives the electron count),
+    evaluated on the device from each species' pseudopotential table and the
+    closed-form uniform-gas response K at rho0 (response_table gives the same
+    function as a radial table)."""
     plan = grid.plan(grid.dtype)
     keys = sorted(readpp.tables)
     ns = len(keys)
-    # the radial tables depend on the species and rho0 only: kept on the
-    # pseudopotential reader like its own tables
-    cache = readpp.__dict__.setdefault("_screening_tables", {})
-    ck = (tuple(keys), float(rho0))
-    if ck not in cache:
-        tabs = [response_table(readpp.gp[k], readpp.vp[k], readpp.zval[k], rho0) for k in keys]
+    # the pseudopotential tables' arguments, kept on the reader with their arrays
+    cache = readpp.__dict__.get("_superposition_args")
+    if cache is None or cache[0] != tuple(keys):
+        tabs = [readpp.tables[k] for k in keys]
         coefs = [np.ascontiguousarray(t.coef.reshape(-1)) for t in tabs]
         arrs = (np.array([t.q0 for t in tabs]), np.array([t.dq for t in tabs]),
                 np.array([t.coef.shape[0] for t in tabs], dtype=np.int32), np.array([t.qmax for t in tabs]),
-                np.array([v for t in tabs for v in t.vsmall]), np.array([t.dqs for t in tabs]))
-        # the arrays stay referenced by the cache entry, so their pointers stay valid
+                np.array([v for t in tabs for v in t.vsmall]), np.array([t.dqs for t in tabs]),
+                np.array([float(readpp.zval[k]) for k in keys]))
         cptr = (_lib._DP * ns)(*[_lib.dptr(c) for c in coefs])
         ptrs = (_lib.dptr(arrs[0]), _lib.dptr(arrs[1]), _lib.iptr(arrs[2]), cptr, _lib.dptr(arrs[3]),
-                _lib.dptr(arrs[4]), _lib.dptr(arrs[5]))
-        cache[ck] = (arrs, coefs, ptrs)
-    ptrs = cache[ck][2]
+                _lib.dptr(arrs[4]), _lib.dptr(arrs[5]), _lib.dptr(arrs[6]))
+        q12 = np.array([[readpp.gp[k][1], readpp.gp[k][2]] for k in keys])
+        v12 = np.array([[readpp.vp[k][1], readpp.vp[k][2]] for k in keys])
+        cache = (tuple(keys), arrs, coefs, ptrs, q12, v12)
+        readpp._superposition_args = cache
+    ptrs, q12, v12 = cache[3], cache[4], cache[5]
+    # the response coefficients at rho0 and the screened values at the first
+    # two table points (the small-q patch), kept for the last rho0
+    rc = readpp.__dict__.get("_superposition_resp")
+    if rc is None or rc[0] != float(rho0):
+        K = response_kernel(q12.reshape(-1), rho0).reshape(q12.shape)
+        r1 = np.ascontiguousarray(-v12[:, 0] / K[:, 0])
+        r2 = np.ascontiguousarray(-v12[:, 1] / K[:, 1])
+        resp = uniform_response_coefs(rho0)
+        rc = (float(rho0), r1, r2, resp, _lib.dptr(r1), _lib.dptr(r2), _lib.dptr(resp))
+        readpp._superposition_resp = rc
     pos = np.ascontiguousarray(ions.positions, dtype=np.float64)
     frac = np.ascontiguousarray(ions.get_scaled_positions(), dtype=np.float64)
     spec = ions.species_index(keys)
     out = torch.empty(tuple(int(n) for n in grid.nrR), dtype=grid.dtype, device=grid.device)
-    _lib.check(_lib.lib.ofdft_local_pp(plan.handle, ns, *ptrs, ions.nat, _lib.dptr(pos), _lib.dptr(frac),
-                                       _lib.iptr(spec), int(bool(pme)), int(order), out.data_ptr(), _lib.stream()))
+    _lib.check(_lib.lib.ofdft_screened_superposition(plan.handle, ns, *ptrs, rc[4], rc[5], rc[6], ions.nat,
+                                                     _lib.dptr(pos), _lib.dptr(frac),
+                                                     _lib.iptr(spec), int(bool(pme)), int(order), out.data_ptr(),
+                                                     _lib.stream()))
     return out
 
 

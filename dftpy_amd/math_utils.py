@@ -129,6 +129,24 @@ def lda_dv(rho0):
     return fx + fc / rho0
 
 
+def uniform_response_coefs(rho0, tf=True, vw=True, wt=True, xc=True, hartree=True, x=1.0, y=1.0, alpha=5.0 / 6.0,
+                           beta=5.0 / 6.0):
+    """uniform_response_kernel as the coefficients the device evaluates:
+    K(q) = c[0] + c[1] q^2 + c[2] G(q / c[3]; lambda = c[4]) + c[5] 4 pi / q^2
+    (G the inverse Lindhard function)."""
+    kc = 0.0
+    if tf:
+        kc += x * (10.0 / 9.0) * CTF * rho0 ** (-1.0 / 3.0)
+    if xc:
+        kc += lda_dv(rho0)
+    tkf = 2.0 * (3.0 * rho0 * np.pi ** 2) ** (1.0 / 3.0)
+    wtc = 0.0
+    if wt:
+        wtfac = 5.0 / (9.0 * alpha * beta * rho0 ** (alpha + beta - 5.0 / 3.0)) * CTF
+        wtc = 2.0 * alpha * beta * rho0 ** (alpha + beta - 2) * wtfac
+    return np.array([kc, y / (4.0 * rho0) if vw else 0.0, wtc, tkf, float(x), 1.0 if hartree else 0.0])
+
+
 def uniform_response_kernel(q, rho0, tf=True, vw=True, wt=True, xc=True, hartree=True, x=1.0, y=1.0,
                             alpha=5.0 / 6.0, beta=5.0 / 6.0):
     """K(q) = d^2 E / d rho(q)^2 at the uniform density rho0 for the chosen

@@ -188,6 +188,19 @@ int ofdft_local_pp(ofdft_plan *plan, int nspecies, const double *q0, const doubl
                    const double *vsmall, const double *dqs, int nat, const double *pos_cart,
                    const double *pos_frac, const int *species, int pme, int order, void *vreal,
                    void *stream);
+/* The learned warm start's superposition of screened ions (no counterpart in
+ * the reference): as ofdft_local_pp with each species' radial table v_s(q)
+ * replaced by rho_s(q) = -v_s(q) / K(q), K the uniform-gas response (resp as
+ * in ofdft_set_tn_precond_uniform), evaluated per G point; below the first
+ * table step dqs the quartic through (r2, r1, zval, r1, r2) (zval, r1, r2:
+ * host arrays of nspecies, the screened values at q = 0, q1, q2).  Writes out
+ * (device, N reals, plan precision). */
+int ofdft_screened_superposition(ofdft_plan *plan, int nspecies, const double *q0, const double *dq,
+                                 const int *ncoef, const double *const *coef, const double *qmax,
+                                 const double *vsmall, const double *dqs, const double *zval,
+                                 const double *r1, const double *r2, const double *resp, int nat,
+                                 const double *pos_cart, const double *pos_frac, const int *species, int pme,
+                                 int order, void *out, void *stream);
 
 /* Ewald energy (ewald.py:197-506): real-space erfc sum over images
  * (Energy_real, 280-320), reciprocal sum over the plan's G grid, exact
@@ -297,6 +310,16 @@ int ofdft_atomic_density_noalloc(ofdft_plan *plan, int kind, int nspecies, const
  * (preconditioned CG; the stopping tests stay on the unpreconditioned
  * residual).  n <= 0 removes it.  Setup call (copies the table). */
 int ofdft_set_tn_precond(ofdft_plan *plan, int n, double dq, const double *table);
+/* The same preconditioner with its table built on the device, in stream
+ * order (no copy, no host table): M(q) = 2 rho0 K(q) at q = i dq, i < n, with
+ * the uniform-gas response K(q) = resp[0] + resp[1] q^2 + resp[2] G(q /
+ * resp[3]) + resp[5] 4 pi / q^2, G the inverse Lindhard function with
+ * lambda = resp[4]; M^-1 = 0 where M is infinite. */
+int ofdft_set_tn_precond_uniform(ofdft_plan *plan, int n, double dq, double rho0, const double *resp,
+                                 void *stream);
+/* Copy the first n values of the plan's preconditioner table to out (device,
+ * stream-ordered); n at most the table's length. */
+int ofdft_get_tn_precond(ofdft_plan *plan, double *out, int n, void *stream);
 enum {
     OFDFT_OPT_TN = 0,
     OFDFT_OPT_CG_HS,

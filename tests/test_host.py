@@ -211,3 +211,19 @@ def test_ions_scaled_positions_and_species_index():
     assert ions.species_index(keys).tolist() == [2, 1, 2, 2, 1, 2, 0]
     with pytest.raises(ValueError, match="Al"):
         ions.species_index(["As", "Ga"])
+
+
+@pytest.mark.parametrize("terms", [(True, True, True, True, True), (True, False, True, True, False),
+                                   (False, True, False, False, True)])
+def test_uniform_response_coefs_reproduce_the_kernel(terms):
+    """The coefficient form the device evaluates equals the host response
+    kernel on a q grid (all terms, and subsets)."""
+    from dftpy_amd.math_utils import lindhard, uniform_response_coefs, uniform_response_kernel
+    tf, vw, wt, xc, h = terms
+    rho0 = 0.0231
+    q = np.linspace(0.0, 9.0, 777)[1:]
+    kw = dict(tf=tf, vw=vw, wt=wt, xc=xc, hartree=h, x=0.9, y=0.8, alpha=5.0 / 6.0, beta=5.0 / 6.0)
+    ref = uniform_response_kernel(q, rho0, **kw)
+    c = uniform_response_coefs(rho0, **kw)
+    got = c[0] + c[1] * q * q + c[2] * lindhard(q / c[3], c[4], 1.0) + c[5] * 4.0 * np.pi / (q * q)
+    assert np.allclose(got, ref, rtol=1e-13, atol=0)
