@@ -571,9 +571,12 @@ __global__ void __launch_bounds__(ROWS_MAX_THREADS)
     if (MODE == 0 && nq > 0) copy_out(nq - 1);
 }
 
-// Ring-staged forward (inference: no ReLU output kept) for nz % 32 == 0,
-// ny % (4 d) == 0, nx % d == 0 and d <= 4.  A step computes 4 output rows
-// y0 + k d (k = 0..3) of one x plane for one 32-point z block.  The steps of
+// Ring-staged forward (inference: no ReLU output kept) for nz >= 36,
+// ny % d == 0, ny >= 4 d, nx % d == 0 and d <= 4.  A step computes 4 output
+// rows y0 + k d (k = 0..3) of one x plane for one 32-point z block; the rows
+// past ny of a residue class's last group and the points past nz of the last
+// z block are computed from wrapped inputs and not written (the staged rows
+// and points wrap once: ny >= 4 d, nz >= 36).  The steps of
 // a walk visit x = c, c + d, c + 2d, ... (one residue class mod d, nx / d
 // planes), so the input planes x - d, x, x + d of step j are planes j - 1,
 // j, j + 1 of the walk: a ring of 4 LDS plane slots (6 staged rows y0 +
@@ -622,7 +625,8 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
     // the bias lives in LDS after the output stages (read once per step)
     float *bias_s = reinterpret_cast<float *>(ostage + 2 * RING_OUT);
     if (tid < CH) bias_s[tid] = bias[tid];
-    const int ngroups = ny / 4, nzb = nz / RING_Z, walk = nx / d;
+    // y groups: per residue class mod d, ceil((ny / d) / 4) groups of 4 rows
+    const int ngroups = d * ((ny / d + 3) / 4), nzb = (nz + RING_Z - 1) / RING_Z, walk = nx / d;
     const long long total = (long long)ngroups * nzb * d * walk;
     const int nwg = gridDim.x, per_xcd = nwg / ROWS_XCDS;
     const int xcd = blockIdx.x % ROWS_XCDS, slotw = blockIdx.x / ROWS_XCDS;
@@ -692,21 +696,25 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
         }
     };
     // output stage element e of this thread: row k, 16-byte chunk rem; its
-    // offset from the step's first output point
-    int oc[RING_OUT / RING_THREADS];
+    // offset from the step's first output point, and its row and z offsets
+    int oc[RING_OUT / RING_THREADS], ok[RING_OUT / RING_THREADS], oz[RING_OUT / RING_THREADS];
 #pragma unroll
     for (int q = 0; q < RING_OUT / RING_THREADS; ++q) {
         const int e = tid + q * RING_THREADS;
         const int k = e / (RING_Z * 4), rem = e - k * (RING_Z * 4);
         oc[q] = k * d * nz * 4 + rem;
+        ok[q] = k * d;
+        oz[q] = rem >> 2;
     }
     auto copy_out = [&](const Step &st, int buf) {
         const int x = st.c + d * st.j;
         const bf16x8 *src = ostage + buf * RING_OUT;
         bf16x8 *ov =
             reinterpret_cast<bf16x8 *>(out) + ((long long)x * nyz + (long long)st.y0 * nz + st.zb * RING_Z) * 4;
+        const int ylim = ny - st.y0, zlim = nz - st.zb * RING_Z;  // rows and points of the step inside the grid
 #pragma unroll
-        for (int q = 0; q < RING_OUT / RING_THREADS; ++q) ov[oc[q]] = src[tid + q * RING_THREADS];
+        for (int q = 0; q < RING_OUT / RING_THREADS; ++q)
+            if (ok[q] < ylim && oz[q] < zlim) ov[oc[q]] = src[tid + q * RING_THREADS];
     };
     int head = 0;  // slot of walk plane j - 1; j at head + 1, j + 1 at head + 2, j + 2 lands at head + 3
     bf16x8 sa[RING_PER], sb[RING_PER];
@@ -1446,8 +1454,8 @@ int ofdft_cnn_conv_mid(const int nr[3], int dil, int mode, const void *x, const 
         OFDFT_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (forward) or 1 (backward-data)");
         OFDFT_REQUIRE(mode == 1 || bias, "forward mode needs the bias");
         OFDFT_REQUIRE(mode == 0 || (resid && mask), "backward mode needs resid and mask");
-        if (mode == 0 && out1 == nullptr && g.nz % RING_Z == 0 && g.ny % (4 * g.d) == 0 && g.nx % g.d == 0 &&
-            g.d <= RING_DMAX) {
+        if (mode == 0 && out1 == nullptr && g.nz >= RING_Z + RING_DMAX && g.ny % g.d == 0 && g.ny >= 4 * g.d &&
+            g.nx % g.d == 0 && g.d <= RING_DMAX) {
             rows_lds_limit((const void *)&k_cnn_mid_ring, RING_LDS);
             hipLaunchKernelGGL(k_cnn_mid_ring, dim3((unsigned)(2 * rows_workgroups())), dim3(RING_THREADS), RING_LDS,
                                St(stream), g, xb, wb, bias, reinterpret_cast<__bf16 *>(out0));

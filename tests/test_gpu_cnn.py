@@ -13,8 +13,11 @@ DIL = (1, 2, 1, 3, 2, 1)
 # (12, 10, 15): every layer on the point-tiled kernel; (6, 8, 32),
 # (4, 16, 64), (4, 8, 96) and (4, 8, 160): layers with ny % 4d == 0 on the
 # row-tiled kernel (one instance per nz / 32 = 1, 2, 3, 5: 96 is the
-# training grid, 160 the 500-atom cell's), the rest on the point-tiled one
-SHAPES = [(12, 10, 15), (6, 8, 32), (4, 16, 64), (4, 8, 96), (4, 8, 160)]
+# training grid, 160 the 500-atom cell's), the rest on the point-tiled one;
+# (6, 12, 40): the inference forward's ring-staged kernel with a partial z
+# block (40 = 32 + 8) and partial y groups (12 / d rows per residue class,
+# d = 1, 2, 3)
+SHAPES = [(12, 10, 15), (6, 8, 32), (4, 16, 64), (4, 8, 96), (4, 8, 160), (6, 12, 40)]
 
 
 def _setup(seed=0, scale=1.5, shape=SHAPE):
@@ -92,13 +95,16 @@ def test_backward_matches_oracle(shape):
         assert _rel(got, g) < 1e-2, k
 
 
-@pytest.mark.parametrize("shape", [(8, 16, 96), (8, 16, 160)])
+@pytest.mark.parametrize("shape", [(8, 16, 96), (8, 16, 160), (8, 20, 72)])
 def test_shipped_dilations_match_oracle(shape):
-    """The shipped network's dilations (1, 2, 4, 4, 2, 1) with ny a multiple
-    of 4d and nx of d for every layer: the inference forward runs all four
+    """The shipped network's dilations (1, 2, 4, 4, 2, 1) with nx and ny
+    multiples of d for every layer: the inference forward runs all four
     middle layers on the ring-staged kernel, the training forward and the
     backward on the row-tiled instances of the 96^3 training grid
-    (nz / 32 = 3) and of the 160^3 cell (nz / 32 = 5)."""
+    (nz / 32 = 3) and of the 160^3 cell (nz / 32 = 5); (8, 20, 72) is a
+    batch-cell z size (72 = 2 x 32 + 8) with y groups that end short for
+    d = 4 (20 / 4 = 5 rows per residue class), its training forward and
+    backward on the point-tiled kernel."""
     from oracle import cnn
     from dftpy_amd import DirectGrid
     from dftpy_amd.learned import WarmStartNet
