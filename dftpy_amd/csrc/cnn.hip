@@ -687,16 +687,21 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
 #pragma unroll
         for (int q = 0; q < RING_PER; ++q) stg[q] = src[in_off[q]];
     };
+    // LDS layout of a staged point (z index zs): its four 16-byte channel
+    // chunks rotated by zs / 4, so that the 16 points of an MFMA operand read
+    // (64-byte stride) fall on distinct banks
     auto land = [&](int slot, const bf16x8 *stg) {
         bf16x8 *dst = lds + slot * RING_PLANE;
 #pragma unroll
         for (int q = 0; q < RING_PER; ++q) {
             const int e = tid + q * RING_THREADS;
-            if (e < RING_PLANE) dst[e] = stg[q];
+            const int pnt = e >> 2, zs = pnt % RING_ZS;
+            if (e < RING_PLANE) dst[(pnt << 2) | (((e & 3) + (zs >> 2)) & 3)] = stg[q];
         }
     };
     // output stage element e of this thread: row k, 16-byte chunk rem; its
     // offset from the step's first output point, and its row and z offsets
+    // (the output stage rotates the chunks of a point by z / 4 like the planes)
     int oc[RING_OUT / RING_THREADS], ok[RING_OUT / RING_THREADS], oz[RING_OUT / RING_THREADS];
 #pragma unroll
     for (int q = 0; q < RING_OUT / RING_THREADS; ++q) {
@@ -714,7 +719,10 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
         const int ylim = ny - st.y0, zlim = nz - st.zb * RING_Z;  // rows and points of the step inside the grid
 #pragma unroll
         for (int q = 0; q < RING_OUT / RING_THREADS; ++q)
-            if (ok[q] < ylim && oz[q] < zlim) ov[oc[q]] = src[tid + q * RING_THREADS];
+            if (ok[q] < ylim && oz[q] < zlim) {
+                const int e = tid + q * RING_THREADS;
+                ov[oc[q]] = src[(e & ~3) | (((e & 3) + (oz[q] >> 2)) & 3)];
+            }
     };
     int head = 0;  // slot of walk plane j - 1; j at head + 1, j + 1 at head + 2, j + 2 lands at head + 3
     bf16x8 sa[RING_PER], sb[RING_PER];
@@ -742,8 +750,9 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
             const int dx = t / 3, dz = t % 3;
             const bf16x8 *base = lds + ((head + dx) & 3) * RING_PLANE;
             const int zi = RING_DMAX + zt * 16 + m + (dz - 1) * d;
+            const int ch = (kq + (zi >> 2)) & 3;  // the rotated chunk
 #pragma unroll
-            for (int r = 0; r < 6; ++r) av[set][r] = base[(r * RING_ZS + zi) * 4 + kq];
+            for (int r = 0; r < 6; ++r) av[set][r] = base[(r * RING_ZS + zi) * 4 + ch];
         };
         load_a(0, 0);
 #pragma unroll
@@ -764,17 +773,20 @@ __global__ void __launch_bounds__(RING_THREADS, 2)
             __bf16 *os = reinterpret_cast<__bf16 *>(ostage + (int)(s & 1) * RING_OUT);
             const int z = zt * 16 + m;
             const float4 bco = *reinterpret_cast<const float4 *>(bias_s + co0);
+            // channels co0 .. co0 + 3 lie in chunk co0 / 8 of a point, rotated by the point's z / 4
+            const int zs = RING_DMAX + z;
+            const int rin = (((co0 >> 3) + (zs >> 2)) & 3) * 8 + (co0 & 7);
+            const int rout = (((co0 >> 3) + (z >> 2)) & 3) * 8 + (co0 & 7);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 // 4 channels of one point: one 8-byte LDS read and write
-                const bf16x4 x4 =
-                    *reinterpret_cast<const bf16x4 *>(rs + ((k + 1) * RING_ZS + RING_DMAX + z) * CH + co0);
+                const bf16x4 x4 = *reinterpret_cast<const bf16x4 *>(rs + ((k + 1) * RING_ZS + zs) * CH + rin);
                 bf16x4 o4;
                 o4[0] = (__bf16)((float)x4[0] + fmaxf(acc[k][0] + bco.x, 0.0f));
                 o4[1] = (__bf16)((float)x4[1] + fmaxf(acc[k][1] + bco.y, 0.0f));
                 o4[2] = (__bf16)((float)x4[2] + fmaxf(acc[k][2] + bco.z, 0.0f));
                 o4[3] = (__bf16)((float)x4[3] + fmaxf(acc[k][3] + bco.w, 0.0f));
-                *reinterpret_cast<bf16x4 *>(os + (k * RING_Z + z) * CH + co0) = o4;
+                *reinterpret_cast<bf16x4 *>(os + (k * RING_Z + z) * CH + rout) = o4;
             }
         }
         land((head + 3) & 3, landing);
