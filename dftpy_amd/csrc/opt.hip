@@ -522,6 +522,18 @@ __device__ static void dt_trial(double *S) {
     S[T_S] = sin(S[T_STP]);
 }
 
+// the line search's first trial of the host-driven FP32 TN, queued before
+// the orthonormalisation's results are read: theta = min(theta0, previous
+// step) with theta0 = sqrt(scal[23] / scal[22]) as the host computes it, and
+// its cosine and sine for the rotation
+__global__ void k_trial_theta(const double *gate, double *S, double theta_prev) {
+    if (gate_off(gate)) return;
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double theta0 = sqrt(S[23] / S[22]);
+    S[T_STP] = theta0 < theta_prev ? theta0 : theta_prev;
+    dt_trial(S);
+}
+
 // after the starting evaluation and residual (run eagerly, not captured)
 __global__ void k_tn_init(double *S, DtOpts o) {
     if (threadIdx.x != 0) return;
@@ -775,11 +787,11 @@ struct Ctx {
     }
     // pub_n > 0: the same launch then publishes scal[pub_slot..pub_slot+pub_n)
     // to the pinned host buffer, read by published()
-    void reduce_final_cols(int K, int slot, int pub_n = 0, int pub_slot = 0) {
+    void reduce_final_cols(int K, int slot, int pub_n = 0, int pub_slot = 0, int pub_off = 0) {
         // partials laid out [block][K]
         std::vector<double> ones(K, 1.0);
         reduce_final(P.red, nb, K, P.scal + slot, ones.data(), s, P.gate, pub_n ? P.scal + pub_slot : nullptr,
-                     pub_n ? P.pinned_dev : nullptr, pub_n);
+                     pub_n ? P.pinned_dev + pub_off : nullptr, pub_n);
     }
     // the scalars the last publishing reduction sent, once the stream is done
     const double *published() const {
@@ -991,6 +1003,10 @@ template <typename T> struct Opt {
     // The same copy also brings back the energies of the last E+V
     // evaluation (scal[32..]), which the first line search needs.
     void orthonormalize(double &theta0, double &vphip, double *energies = nullptr) {
+        orth_queue(energies != nullptr);
+        orth_parse(C.published(), theta0, vphip, energies);
+    }
+    void orth_queue(bool energies) {
         T *p = F(F_P), *phi = F(F_PHI);
         PW_RED(k_pphi, p, phi, n, P.red);
         C.reduce_final_cols(2, 21);  // scal[21] = sum p phi, scal[22] = sum phi^2
@@ -1001,11 +1017,26 @@ template <typename T> struct Opt {
         PW_RED(k_dot3, F(F_POT), phi, p, n, P.red);
         // scal[24] = sum V phi p; scal[21..] to the host
         C.reduce_final_cols(1, 24, energies ? 32 + OFDFT_E_COUNT - 21 : 4, 21);
-        const double *h = C.published();
+    }
+    static void orth_parse(const double *h, double &theta0, double &vphip, double *energies) {
         theta0 = std::sqrt(h[2] / h[1]);
         vphip = h[3];
         if (energies)
             for (int i = 0; i < OFDFT_E_COUNT; ++i) energies[i] = h[32 - 21 + i];
+    }
+
+    // the first trial of the line search at theta = min(theta0, theta_prev),
+    // queued behind the orthonormalisation (cosine and sine on the device);
+    // its energies and gradient sum are published at pinned[kTrialPub..]
+    static constexpr int kTrialPub = 64;
+    void trial_queue(double theta_prev) {
+        hipLaunchKernelGGL(k_trial_theta, dim3(1), dim3(64), 0, C.s, P.gate, P.scal, theta_prev);
+        OFDFT_HIP(hipGetLastError());
+        PW_STREAM(k_rotate, F(F_PHI), F(F_P), 0.0, 0.0, P.scal + T_C, F(F_NPHI), F(F_NRHO), n);
+        eval<T>(P, F(F_NRHO), OFDFT_CALC_E | OFDFT_CALC_V, F(F_NPOT), energies_d, C.s);
+        ++n_ev;
+        PW_RED(k_theta_grad, F(F_NPOT), F(F_NPHI), F(F_P), F(F_PHI), 0.0, 0.0, P.scal + T_C, n, P.red);
+        C.reduce_final_cols(1, 32 + OFDFT_E_COUNT, OFDFT_E_COUNT + 1, 32, kTrialPub);
     }
 
     // value and derivative at theta (ValueAndDerivative, optimization.py:227-279);
@@ -1401,12 +1432,28 @@ template <typename T> struct Opt {
                 }
             }
             double theta0, vphip;
-            orthonormalize(theta0, vphip, energy_pending ? energies_h : nullptr);
+            // FP32 TN: the line search's first trial (its step follows from
+            // the orthonormalisation's sums) is queued behind them, and both
+            // come back in one round trip
+            bool have_trial = false;
+            double trial_f = 0.0, trial_g = 0.0;
+            if (P.prec == OFDFT_FP32 && method == OFDFT_OPT_TN) {
+                orth_queue(energy_pending);
+                trial_queue(theta);
+                const double *h = C.published();
+                orth_parse(h, theta0, vphip, energy_pending ? energies_h : nullptr);
+                trial_f = h[kTrialPub + OFDFT_E_SUM];
+                trial_g = 2.0 * h[kTrialPub + OFDFT_E_COUNT] * P.dV;
+                have_trial = true;
+            } else {
+                orthonormalize(theta0, vphip, energy_pending ? energies_h : nullptr);
+            }
             if (energy_pending) {
                 energy = hist[0] = energies_h[OFDFT_E_SUM];
                 energy_pending = false;
             }
             if (vphip > 0) {  // uphill: restart from the steepest descent direction
+                have_trial = false;
                 PW_STREAM(k_neg, F(F_RES), F(F_P), n);
                 orthonormalize(theta0, vphip);
             }
@@ -1424,13 +1471,24 @@ template <typename T> struct Opt {
                 // would compare energy differences at single-precision
                 // resolution and could only exhaust maxls; the trial (for TN
                 // the Newton) step is taken as it is, with one evaluation
-                value_deriv(stp, f, g);
+                if (have_trial) {
+                    f = trial_f;
+                    g = trial_g;
+                } else {
+                    value_deriv(stp, f, g);
+                }
                 ok = evaluated = true;
             } else {
                 for (int i = 0; i < maxls; ++i) {
                     task = dcsrch(stp, f, g, c1, c2, xtol, 0.0, M_PI, task, st);
                     if (task == LS_FG) {
-                        value_deriv(stp, f, g);
+                        if (have_trial && stp == theta) {  // the queued trial is dcsrch's first
+                            f = trial_f;
+                            g = trial_g;
+                        } else {
+                            value_deriv(stp, f, g);
+                        }
+                        have_trial = false;
                         evaluated = true;
                     } else {
                         ok = (task == LS_CONVERGENCE);
